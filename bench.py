@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Ray-cast benchmark on one MI355X.
+
+Workload: closest hit for 6,073,600 rays (1920x1080 coherent pinhole rays +
+4,000,000 incoherent rays) against a 1,000,000-triangle seeded soup
+(centroids uniform in the unit cube, Gaussian edges sigma 0.003).  A step is
+one closest-hit pass over the whole ray batch, rays and BVH already resident
+in HBM.  ``value`` is Mrays/s over the timed steps.
+
+The same JSON line carries:
+* roofline: algorithmic bytes per ray B_ray = 32 + 80 * nodes/ray + 48 *
+  triangles/ray (24 B ray in, 8 B result out; 80 B wide node; 48 B triangle),
+  counts from the counted kernel variant on the same rays, times rays per
+  launch, divided by the average launch time measured with HIP events on the
+  launch stream; HBM traffic per launch from rocprofv3 counters when a
+  measured summary is committed (profiles/closest_hit_traffic.json).
+* cpu_baseline: the reference's own BVH4 mesh path (oracle/_ref, built with
+  the reference's Linux flags) on this host's cores over an evenly spaced
+  sample of the same rays, with its ids checked against the GPU's.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s + render requests/s with p50/p99 latency at ≤1% rel-RMSE, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 80
+TRI_BYTES = 48
+RAY_IO_BYTES = 32
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def copy_calibration(steps: int) -> None:
+    """A plain device copy of 1 GiB, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    import torch
+
+    n = 1 << 28  # float32 elements = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {steps} x {n * 4} bytes")
+
+
+def load_traffic():
+    p = ROOT / "profiles" / "closest_hit_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true", help="run only the 1 GiB copy kernel")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU only: this build has no multi-GPU path "
+            "(the film/request sharding across GPUs is described in DESIGN.md).",
+            file=sys.stderr,
+        )
+        return 2
+
+    import numpy as np
+    import torch
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        return 1
+    if args.calibrate_copy:
+        copy_calibration(max(1, args.steps))
+        return 0
+
+    from rise_amd import raycast, scenes
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    tris = scenes.soup_triangles()
+    rays_np = scenes.cast_rays()
+    n = rays_np.shape[1]
+    log(f"workload generated: {tris.shape[0]} tris, {n} rays ({time.time() - t0:.1f}s)")
+
+    om = raycast.ObjectManager()
+    om.add_mesh(tris)
+    om.prepare_for_rendering()
+    st = om.stats()
+    log(f"BVH: {st['n_nodes']} nodes, depth {st['depth']}, build {st['build_ms']:.0f} ms")
+
+    rays = torch.from_numpy(rays_np).to(dev)
+    pid = torch.empty(n, dtype=torch.int32, device=dev)
+    tt = torch.empty(n, dtype=torch.float32, device=dev)
+    counters = torch.zeros(2, dtype=torch.int64, device=dev)
+    om.intersect_ray(rays, pid, tt, counters=counters)
+    torch.cuda.synchronize()
+    cnt = counters.cpu().numpy().astype(np.float64)
+    nodes_per_ray = cnt[0] / n
+    tris_per_ray = cnt[1] / n
+    b_ray = RAY_IO_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
+    log(f"per ray: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} triangles -> {b_ray:.1f} algorithmic bytes")
+
+    for _ in range(args.warmup):
+        om.intersect_ray(rays, pid, tt)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        om.intersect_ray(rays, pid, tt)
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = start.elapsed_time(end)
+    ms_per_step = ms / args.steps
+    mrays = n * args.steps / (ms / 1e3) / 1e6
+    achieved = b_ray * n / (ms_per_step / 1e3) / 1e9
+    log(f"{args.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.1f} ms), {mrays:.1f} Mrays/s, {achieved:.0f} GB/s algorithmic")
+    gpu_ids = pid.cpu().numpy()
+    hit_frac = float((gpu_ids >= 0).mean())
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        import oracle
+
+        if oracle.ref_available("fast"):
+            th = cpu_threads()
+            tb = time.time()
+            ref = oracle.RefMesh(tris, "fast")
+            build_s = time.time() - tb
+            probe = np.arange(0, n, max(1, n // 20000))
+            tp = time.perf_counter()
+            ref.closest(rays_np[:, probe], threads=th)
+            rate = len(probe) / max(1e-6, time.perf_counter() - tp)
+            m = int(min(n, max(len(probe), rate * args.cpu_seconds)))
+            idx = np.linspace(0, n - 1, m).astype(np.int64)
+            sample = np.ascontiguousarray(rays_np[:, idx])
+            tc = time.perf_counter()
+            rp, _, _ = ref.closest(sample, threads=th)
+            sec = time.perf_counter() - tc
+            cpu = {
+                "value": round(m / sec / 1e6, 4),
+                "unit": "Mrays/s",
+                "cores": th,
+                "kind": "reference",
+                "sample": f"{m} of the {n} rays (evenly spaced), closest hit, RISE BVH4 mesh path "
+                f"built with its Linux flags; {sec:.1f} s of work; BVH build {build_s:.1f} s not counted",
+            }
+            mism = int((rp != gpu_ids[idx]).sum())
+            parity = {"rays_checked": m, "id_mismatches": mism, "against": "reference BVH4 (oracle/_ref)"}
+            log(f"CPU reference: {cpu['value']} Mrays/s on {th} threads; id mismatches {mism}/{m}")
+            ref.close()
+        else:
+            log("oracle/_ref not built: no CPU baseline")
+
+    traffic = load_traffic()
+    line = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 traversal / f64 certify",
+        "data": "synthetic",
+        "config": {
+            "workload": "ray-cast closest hit: 1,000,000-triangle seeded soup (centroids U[0,1]^3, edges N(0,0.003^2)), "
+            "2,073,600 coherent pinhole rays (1920x1080, 40 deg) + 4,000,000 incoherent rays",
+            "n_tris": int(tris.shape[0]),
+            "rays_per_step": int(n),
+            "hit_fraction": round(hit_frac, 4),
+            "bvh": {"nodes": st["n_nodes"], "depth": st["depth"], "build_ms": round(st["build_ms"], 1)},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "closest_kernel",
+            "bytes_per_ray": round(b_ray, 2),
+            "nodes_per_ray": round(nodes_per_ray, 3),
+            "tris_per_ray": round(tris_per_ray, 3),
+        },
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    om.release()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

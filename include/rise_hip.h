@@ -1,0 +1,134 @@
+/*
+ * rise_hip.h -- C ABI of the MI355X (gfx950) implementation of RISE's
+ * ray-cast and spectral path-tracing hot path.
+ *
+ * Conventions
+ *   - Every entry point returns rise_hip_status; RISE_HIP_OK is 0.  On
+ *     failure, rise_hip_last_error() returns a message for the calling
+ *     thread.  No C++ exception crosses this boundary.  (RISE reports
+ *     failure through bool returns plus GlobalLog(); see
+ *     src/Library/RISE_API.h factory functions.)
+ *   - Objects are opaque handles created by *_create / *_upload and freed by
+ *     the matching *_release.  The library owns every device buffer it
+ *     allocates until release.
+ *   - Buffers named "device" are device (HBM) pointers; "host" are host
+ *     pointers.  Work is enqueued on the caller's stream and is asynchronous
+ *     unless stated otherwise.  Calls on one scene from several threads are
+ *     safe; a scene must not be released while work using it is in flight.
+ *
+ * Reference interfaces replaced (paths relative to src/Library/ of RISE):
+ *   - Interfaces/IObjectManager.h:38-68 (IntersectRay, IntersectShadowRay,
+ *     PrepareForRendering): rise_hip_scene_upload + rise_hip_intersect_*.
+ *   - Interfaces/IRasterizer.h:70 (RasterizeScene) with the per-pixel hook
+ *     PixelBasedRasterizerHelper::IntegratePixel
+ *     (Rendering/PathTracingSpectralRasterizer.h:48-58): rise_hip_render.
+ */
+#ifndef RISE_HIP_H
+#define RISE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rise_hip_status {
+    RISE_HIP_OK = 0,
+    RISE_HIP_ERR_INVALID_ARGUMENT = 1,
+    RISE_HIP_ERR_OUT_OF_MEMORY = 2,
+    RISE_HIP_ERR_DEVICE = 3,
+    RISE_HIP_ERR_BUILD = 4,
+    RISE_HIP_ERR_NOT_SUPPORTED = 5
+} rise_hip_status;
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char* rise_hip_last_error(void);
+
+/* Library version, "major.minor.patch". */
+const char* rise_hip_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Scenes (geometry resident in HBM)                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rise_hip_scene rise_hip_scene;
+
+/* One triangle mesh in world space (identity object transform).
+ * vertices: host, n_tris * 9 floats (v0.xyz, v1.xyz, v2.xyz per triangle).
+ * tri_ids:  host, optional n_tris ids reported for hits (NULL: the running
+ *           index over all meshes of the scene, starting at 0). */
+typedef struct rise_hip_mesh_desc {
+    const float* vertices;
+    int64_t n_tris;
+    const int32_t* tri_ids;
+} rise_hip_mesh_desc;
+
+typedef struct rise_hip_scene_stats {
+    int64_t n_tris;
+    int64_t n_nodes;        /* 8-wide nodes */
+    int64_t n_bvh2_nodes;   /* nodes of the binary build before collapse */
+    int32_t depth;          /* wide-node levels */
+    int32_t node_bytes;     /* 80 */
+    int32_t tri_bytes;      /* 48 */
+    int32_t reserved;
+    double build_ms;        /* host build time */
+    double sah_cost;        /* SAH cost of the binary build / root area */
+    int64_t device_bytes;   /* HBM held by the scene */
+} rise_hip_scene_stats;
+
+/* Builds the acceleration structure on the host and uploads it to the
+ * current device.  Replaces ObjectManager::PrepareForRendering /
+ * TriangleMeshGeometryIndexed::DoneIndexedTriangles
+ * (Managers/ObjectManager.cpp:404-423, Geometry/TriangleMeshGeometryIndexed.cpp:444-540).
+ * Synchronous. */
+rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n_meshes,
+                                      rise_hip_scene** out_scene);
+
+rise_hip_status rise_hip_scene_release(rise_hip_scene* scene);
+
+rise_hip_status rise_hip_scene_get_stats(const rise_hip_scene* scene, rise_hip_scene_stats* out_stats);
+
+/* ------------------------------------------------------------------------ */
+/* Batch ray queries                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Closest hit for n rays.  Replaces IObjectManager::IntersectRay
+ * (Interfaces/IObjectManager.h:38; Managers/ObjectManager.cpp:270-318) for
+ * front- and back-face hits.
+ * rays_soa: device, 6*n floats: ox[n], oy[n], oz[n], dx[n], dy[n], dz[n]
+ *           (world space; d need not be unit length).
+ * prim_id:  device, n int32: closest triangle id, -1 on a miss; exact ties
+ *           in the double-precision hit distance go to the lowest id.
+ * t:        device, n floats: world distance to the hit (Object.cpp:641-643),
+ *           +inf on a miss. */
+rise_hip_status rise_hip_intersect_closest(rise_hip_scene* scene, const float* rays_soa, int64_t n,
+                                           int32_t* prim_id, float* t, hipStream_t stream);
+
+/* Any hit within tmax (world distance, the reference's dHowFar) for n rays.
+ * Replaces IObjectManager::IntersectShadowRay (IObjectManager.h:46;
+ * ObjectManager.cpp:320-373).  tmax: device, n floats, or NULL for
+ * RISE_INFINITY.  occluded: device, n bytes, 1 if occluded else 0. */
+rise_hip_status rise_hip_intersect_any(rise_hip_scene* scene, const float* rays_soa, const float* tmax,
+                                       int64_t n, uint8_t* occluded, hipStream_t stream);
+
+/* As rise_hip_intersect_closest / _any, and also accumulates into
+ * counters[0] the 8-wide nodes visited and into counters[1] the triangles
+ * certified (counters: device, 2 uint64, not cleared by the call).  Used to
+ * state the bytes a ray fetches; slower than the plain entry points.
+ * No counterpart in the reference (its profiling counters are compile-gated,
+ * Utilities/Profiling.h:163). */
+rise_hip_status rise_hip_intersect_closest_counted(rise_hip_scene* scene, const float* rays_soa, int64_t n,
+                                                   int32_t* prim_id, float* t, uint64_t* counters,
+                                                   hipStream_t stream);
+rise_hip_status rise_hip_intersect_any_counted(rise_hip_scene* scene, const float* rays_soa, const float* tmax,
+                                               int64_t n, uint8_t* occluded, uint64_t* counters,
+                                               hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RISE_HIP_H */

@@ -1,0 +1,33 @@
+/*
+ * rise_hip_debug.h -- validation entry points that need no GPU.
+ *
+ * These have no counterpart in the reference.  They exist for callers that
+ * check the library on a machine without a GPU (the CPU test suite): they
+ * build the same 8-wide BVH the GPU path uploads and run the same traversal
+ * and certifier code (trace_core.h is compiled for host and device) on host
+ * threads.  They are validation tools, not a fallback: the GPU entry points
+ * never call them.
+ */
+#ifndef RISE_HIP_DEBUG_H
+#define RISE_HIP_DEBUG_H
+
+#include "rise_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* mode 0: closest hit (fills prim_id, t); mode 1: any hit (fills occluded;
+ * tmax may be NULL).  All buffers are host memory; rays_soa as in
+ * rise_hip_intersect_closest.  counters (optional, 2 uint64) receive nodes
+ * visited and triangles certified; stats (optional) the build statistics. */
+rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t n_tris, const float* rays_soa,
+                                          const float* tmax, int64_t n, int32_t mode, int32_t* prim_id,
+                                          float* t, uint8_t* occluded, uint64_t* counters,
+                                          rise_hip_scene_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RISE_HIP_DEBUG_H */

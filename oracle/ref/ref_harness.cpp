@@ -1,0 +1,218 @@
+// ref_harness.cpp -- drives RISE's own mesh intersection path, compiled from
+// the reference sources where they lie (see oracle/ref/Makefile), so the
+// oracle and the HIP path can be checked against the genuine code.
+// TEST INFRASTRUCTURE ONLY (checker and CPU baseline); never shipped.
+//
+// Path exercised (reference paths relative to src/Library/):
+//   TriangleMeshGeometryIndexed::DoneIndexedTriangles (Geometry/TriangleMeshGeometryIndexed.cpp:444-540)
+//     -> BVH<const PointerTriangle*> SAH BVH2 build, float filter, BVH4 collapse (Acceleration/BVH.h)
+//   TriangleMeshGeometryIndexed::IntersectRay / _IntersectionOnly (TriangleMeshGeometryIndexed.cpp:151-205)
+//     -> BVH::IntersectRay4 -> MollerTrumboreFloat -> RayElementIntersection -> RayTriangleIntersection
+//
+// The Object layer (Objects/Object.cpp) cannot be built from a few sources
+// (it links the whole material library), so for an identity transform this
+// harness performs Object::IntersectRay's two steps around the mesh call with
+// the reference's own Math3D operations: normalize the direction
+// (Object.cpp:414-425) and recompute the world range after the 1e-12
+// back-off (Object.cpp:641-643); for the shadow ray, the dHowFar scaling of
+// Object.cpp:472-493.
+//
+// Triangle ids: triangle i gets its own texture coordinate (i, 0) on all three
+// corners, so the reference's interpolated ri.ptCoord.x returns i exactly.
+
+#include "Geometry/TriangleMeshGeometryIndexed.h"
+#include "Intersection/RayIntersectionGeometric.h"
+
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <cmath>
+#include <cstdint>
+#include <thread>
+#include <vector>
+
+using namespace RISE;
+
+namespace {
+
+struct RefMesh {
+    Implementation::TriangleMeshGeometryIndexed* mesh;
+};
+
+// Persistent worker pool: RISE renders with long-lived pool threads
+// (Utilities/ThreadPool.h), so per-thread state (the BVH's thread_local
+// traversal stack, the mesh's thread_local mailbox) is allocated once, not
+// once per batch.
+class Pool {
+public:
+    static Pool& get()
+    {
+        static Pool p;
+        return p;
+    }
+
+    void run(int64_t n, int n_threads, const std::function<void(int64_t)>& fn)
+    {
+        if (n_threads <= 0) {
+            n_threads = (int)std::thread::hardware_concurrency();
+            if (n_threads <= 0) n_threads = 1;
+        }
+        std::unique_lock<std::mutex> lk(m_);
+        while ((int)th_.size() < n_threads - 1) {
+            const int idx = (int)th_.size();
+            th_.emplace_back([this, idx] { loop(idx); });
+        }
+        fn_ = &fn;
+        n_ = n;
+        next_ = 0;
+        want_ = n_threads - 1;
+        done_ = 0;
+        ++gen_;
+        cv_.notify_all();
+        lk.unlock();
+        work();
+        lk.lock();
+        done_cv_.wait(lk, [this] { return done_ == want_; });
+        fn_ = nullptr;
+    }
+
+    ~Pool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+
+private:
+    void work()
+    {
+        for (;;) {
+            const int64_t s = next_.fetch_add(256);
+            if (s >= n_) break;
+            const int64_t e = s + 256 < n_ ? s + 256 : n_;
+            for (int64_t i = s; i < e; ++i) (*fn_)(i);
+        }
+    }
+
+    void loop(int idx)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> lk(m_);
+            cv_.wait(lk, [&] { return gen_ != seen; });
+            seen = gen_;
+            if (stop_) return;
+            if (idx >= want_) continue;
+            lk.unlock();
+            work();
+            lk.lock();
+            ++done_;
+            done_cv_.notify_all();
+        }
+    }
+
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(int64_t)>* fn_ = nullptr;
+    int64_t n_ = 0;
+    std::atomic<int64_t> next_{0};
+    int want_ = 0;
+    int done_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+template <class F>
+void parallel_rays(int64_t n, int n_threads, F&& fn)
+{
+    const std::function<void(int64_t)> f(fn);
+    Pool::get().run(n, n_threads, f);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Builds the reference mesh (and its BVH) over n_tris triangles given as
+// 9 floats each (v0, v1, v2).  Returns an opaque handle.
+void* rise_ref_mesh_create(const float* tris, int64_t n_tris)
+{
+    RefMesh* h = new RefMesh;
+    h->mesh = new Implementation::TriangleMeshGeometryIndexed(false, true);
+    h->mesh->BeginIndexedTriangles();
+    for (int64_t i = 0; i < n_tris; ++i) {
+        const float* f = tris + 9 * i;
+        for (int k = 0; k < 3; ++k)
+            h->mesh->AddVertex(Vertex((double)f[3 * k + 0], (double)f[3 * k + 1], (double)f[3 * k + 2]));
+        h->mesh->AddTexCoord(TexCoord((double)i, 0.0));
+        IndexedTriangle t;
+        for (int k = 0; k < 3; ++k) {
+            t.iVertices[k] = (unsigned int)(3 * i + k);
+            t.iNormals[k] = 0;
+            t.iCoords[k] = (unsigned int)i;
+        }
+        h->mesh->AddIndexedTriangle(t);
+    }
+    h->mesh->DoneIndexedTriangles();
+    return h;
+}
+
+void rise_ref_mesh_release(void* handle)
+{
+    RefMesh* h = (RefMesh*)handle;
+    if (!h) return;
+    h->mesh->release();
+    delete h;
+}
+
+// Closest hit; rays SoA (ox, oy, oz, dx, dy, dz), each n floats.
+void rise_ref_closest(void* handle, const float* rays, int64_t n, int32_t* prim_id, float* t,
+                      double* t64, int n_threads)
+{
+    RefMesh* h = (RefMesh*)handle;
+    parallel_rays(n, n_threads, [&](int64_t i) {
+        const Point3 o(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        const Vector3 d(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        const Vector3 dn = Vector3Ops::Normalize(d);
+        const RasterizerState rs = {0, 0};
+        RayIntersectionGeometric ri(Ray(o, dn), rs);
+        h->mesh->IntersectRay(ri, true, true, false);
+        if (ri.bHit) {
+            const Point3 p = ri.ray.PointAtLength(ri.range - 1e-12);
+            const Scalar range = Vector3Ops::Magnitude(Vector3Ops::mkVector3(p, o));
+            prim_id[i] = (int32_t)ri.ptCoord.x;
+            t[i] = (float)range;
+            if (t64) t64[i] = range;
+        } else {
+            prim_id[i] = -1;
+            t[i] = INFINITY;
+            if (t64) t64[i] = INFINITY;
+        }
+    });
+}
+
+// Any hit within tmax[i] (world distance; NULL = RISE_INFINITY).
+void rise_ref_any(void* handle, const float* rays, const float* tmax, int64_t n, uint8_t* occluded,
+                  int n_threads)
+{
+    RefMesh* h = (RefMesh*)handle;
+    parallel_rays(n, n_threads, [&](int64_t i) {
+        const Point3 o(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        const Vector3 d(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        const Scalar mag = Vector3Ops::Magnitude(d);
+        const Vector3 dn = Vector3Ops::Normalize(d);
+        const Scalar factor = (mag > NEARZERO) ? mag : Scalar(1.0);
+        const Scalar how_far = tmax ? (Scalar)tmax[i] : RISE_INFINITY;
+        Scalar how_far2 = how_far;
+        if ((how_far != RISE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
+        occluded[i] = h->mesh->IntersectRay_IntersectionOnly(Ray(o, dn), how_far2, true, true) ? 1 : 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,518 @@
+// bvh_build.cpp -- binned-SAH BVH2 build, collapse to 8-wide nodes, quantization.
+// See bvh_build.h for the reference lines this follows.
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <thread>
+
+namespace rise_hip {
+
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+    void reset()
+    {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::numeric_limits<float>::infinity();
+            hi[k] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const Box& b)
+    {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    void grow(const float p[3])
+    {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    bool empty() const { return lo[0] > hi[0]; }
+    // Surface area in double (BVH.h:1083-1088 SurfaceArea: 0 for an inverted box).
+    double area() const
+    {
+        const double ex = (double)hi[0] - lo[0], ey = (double)hi[1] - lo[1], ez = (double)hi[2] - lo[2];
+        if (ex < 0 || ey < 0 || ez < 0) return 0.0;
+        return 2.0 * (ex * ey + ey * ez + ez * ex);
+    }
+};
+
+struct Node2 {
+    Box box;
+    int32_t left = -1;   // internal: children left, left + 1
+    uint32_t first = 0;  // leaf: first index into idx
+    uint32_t count = 0;  // leaf: > 0
+};
+
+struct Bvh2Builder {
+    const BuildParams& P;
+    std::vector<Box> pbox;
+    std::vector<float> cent;  // 3 per prim
+    std::vector<uint32_t> idx;
+    std::vector<Node2> nodes;
+    std::atomic<int64_t> next_node{1};
+    std::atomic<int> spawn_budget{0};
+
+    explicit Bvh2Builder(const BuildParams& p) : P(p) {}
+
+    int64_t alloc_pair() { return next_node.fetch_add(2); }
+
+    void make_leaf(int64_t ni, uint32_t first, uint32_t count)
+    {
+        nodes[ni].left = -1;
+        nodes[ni].first = first;
+        nodes[ni].count = count;
+    }
+
+    void build(int64_t ni, uint32_t first, uint32_t last)
+    {
+        const uint32_t count = last - first;
+        Box box, cbox;
+        box.reset();
+        cbox.reset();
+        for (uint32_t i = first; i < last; ++i) {
+            box.grow(pbox[idx[i]]);
+            cbox.grow(&cent[3 * (size_t)idx[i]]);
+        }
+        nodes[ni].box = box;
+        if (count <= 1) {
+            make_leaf(ni, first, count);
+            return;
+        }
+
+        // Binned SAH over all three axes.
+        const int B = P.bins;
+        struct Bin {
+            Box b;
+            uint32_t n;
+        };
+        int best_axis = -1, best_split = -1;
+        double best_cost = std::numeric_limits<double>::infinity();
+        const double parent_area = box.area();
+        std::vector<Bin> bins((size_t)B * 3);
+        std::vector<Box> rbox(B);
+        std::vector<uint32_t> rcnt(B);
+        float cmin[3], inv[3];
+        bool ok_axis[3];
+        for (int a = 0; a < 3; ++a) {
+            cmin[a] = cbox.lo[a];
+            const float ext = cbox.hi[a] - cbox.lo[a];
+            ok_axis[a] = ext > 0.0f;
+            inv[a] = ok_axis[a] ? (float)B / ext : 0.0f;
+            for (int b = 0; b < B; ++b) {
+                bins[(size_t)a * B + b].b.reset();
+                bins[(size_t)a * B + b].n = 0;
+            }
+        }
+        for (uint32_t i = first; i < last; ++i) {
+            const uint32_t p = idx[i];
+            for (int a = 0; a < 3; ++a) {
+                if (!ok_axis[a]) continue;
+                int b = (int)((cent[3 * (size_t)p + a] - cmin[a]) * inv[a]);
+                b = std::min(std::max(b, 0), B - 1);
+                Bin& bin = bins[(size_t)a * B + b];
+                bin.b.grow(pbox[p]);
+                bin.n++;
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (!ok_axis[a]) continue;
+            const Bin* ab = &bins[(size_t)a * B];
+            Box acc;
+            acc.reset();
+            uint32_t n = 0;
+            for (int b = B - 1; b >= 1; --b) {
+                if (ab[b].n) acc.grow(ab[b].b);
+                n += ab[b].n;
+                rbox[b] = acc;
+                rcnt[b] = n;
+            }
+            acc.reset();
+            n = 0;
+            for (int b = 0; b + 1 < B; ++b) {
+                if (ab[b].n) acc.grow(ab[b].b);
+                n += ab[b].n;
+                const uint32_t nr = rcnt[b + 1];
+                if (n == 0 || nr == 0) continue;
+                const double cost = P.sah_traversal +
+                                    P.sah_intersect * (acc.area() * n + rbox[b + 1].area() * nr) /
+                                        (parent_area > 0 ? parent_area : 1.0);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = a;
+                    best_split = b;
+                }
+            }
+        }
+
+        const double leaf_cost = P.sah_intersect * (double)count;
+        if (count <= (uint32_t)P.max_leaf && (best_axis < 0 || best_cost >= leaf_cost)) {
+            make_leaf(ni, first, count);
+            return;
+        }
+
+        uint32_t mid = first;
+        if (best_axis >= 0) {
+            const int a = best_axis;
+            for (uint32_t i = first; i < last; ++i) {
+                const uint32_t p = idx[i];
+                int b = (int)((cent[3 * (size_t)p + a] - cmin[a]) * inv[a]);
+                b = std::min(std::max(b, 0), B - 1);
+                if (b <= best_split) std::swap(idx[i], idx[mid++]);
+            }
+        }
+        if (mid == first || mid == last) {
+            // Median fallback (BVH.h:1002-1012): all centroids coincide or the
+            // bins collapsed; split by centroid order on the widest axis, ties
+            // by primitive index so the result is deterministic.
+            int a = 0;
+            const float e0 = cbox.hi[0] - cbox.lo[0], e1 = cbox.hi[1] - cbox.lo[1], e2 = cbox.hi[2] - cbox.lo[2];
+            if (e1 > e0) a = 1;
+            if (e2 > std::max(e0, e1)) a = 2;
+            mid = first + count / 2;
+            std::nth_element(idx.begin() + first, idx.begin() + mid, idx.begin() + last,
+                             [&](uint32_t x, uint32_t y) {
+                                 const float cx = cent[3 * (size_t)x + a], cy = cent[3 * (size_t)y + a];
+                                 return cx < cy || (cx == cy && x < y);
+                             });
+        }
+
+        const int64_t l = alloc_pair();
+        nodes[ni].left = (int32_t)l;
+        nodes[ni].count = 0;
+        const bool spawn = count > 65536 && spawn_budget.fetch_sub(1) > 0;
+        if (spawn) {
+            auto fut = std::async(std::launch::async, [this, l, first, mid] { build(l, first, mid); });
+            build(l + 1, mid, last);
+            fut.get();
+        } else {
+            build(l, first, mid);
+            build(l + 1, mid, last);
+        }
+    }
+};
+
+float exp2f_int(int e) { return std::ldexp(1.0f, e); }
+
+// Quantize one axis of all children of a node; returns false if the chosen
+// exponent cannot represent the range (caller retries with e + 1).
+bool quantize_axis(float p, int e, const float* clo, const float* chi, int nchild, uint8_t* qlo, uint8_t* qhi)
+{
+    const float s = exp2f_int(e);
+    for (int c = 0; c < nchild; ++c) {
+        double fl = std::floor(((double)clo[c] - p) / s);
+        double fh = std::ceil(((double)chi[c] - p) / s);
+        int lo = (int)std::max(0.0, std::min(255.0, fl));
+        int hi = (int)std::max(0.0, std::min(256.0, fh));
+        while (lo > 0 && std::fmaf((float)lo, s, p) > clo[c]) --lo;
+        if (std::fmaf((float)lo, s, p) > clo[c]) return false;
+        while (hi <= 255 && std::fmaf((float)hi, s, p) < chi[c]) ++hi;
+        if (hi > 255) return false;
+        if (std::fmaf((float)hi, s, p) < chi[c]) return false;
+        qlo[c] = (uint8_t)lo;
+        qhi[c] = (uint8_t)hi;
+    }
+    return true;
+}
+
+}  // namespace
+
+void decode_child_box(const Node8& n, int s, float lo[3], float hi[3])
+{
+    const float sx = exp2f_int(n.ex), sy = exp2f_int(n.ey), sz = exp2f_int(n.ez);
+    lo[0] = std::fmaf((float)n.qlox[s], sx, n.px);
+    lo[1] = std::fmaf((float)n.qloy[s], sy, n.py);
+    lo[2] = std::fmaf((float)n.qloz[s], sz, n.pz);
+    hi[0] = std::fmaf((float)n.qhix[s], sx, n.px);
+    hi[1] = std::fmaf((float)n.qhiy[s], sy, n.py);
+    hi[2] = std::fmaf((float)n.qhiz[s], sz, n.pz);
+}
+
+bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const BuildParams& params,
+                    WideBVH& out, std::string& err)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    out = WideBVH();
+    if (n < 0 || n > (int64_t)0x7fffffff) {
+        err = "triangle count out of range";
+        return false;
+    }
+    if (params.max_leaf < 1 || params.max_leaf > kMaxLeafTris || params.bins < 2) {
+        err = "invalid build parameters";
+        return false;
+    }
+    if (n == 0) {
+        // Empty scene: a single node with no children; every ray misses.
+        Node8 root;
+        std::memset(&root, 0, sizeof(root));
+        out.nodes.push_back(root);
+        out.stats.nodes = 1;
+        out.stats.depth = 1;
+        return true;
+    }
+
+    Bvh2Builder b(params);
+    b.pbox.resize((size_t)n);
+    b.cent.resize((size_t)n * 3);
+    b.idx.resize((size_t)n);
+    Box scene;
+    scene.reset();
+    for (int64_t i = 0; i < n; ++i) {
+        const float* f = tris + 9 * i;
+        for (int k = 0; k < 9; ++k) {
+            if (!std::isfinite(f[k])) {
+                err = "non-finite vertex coordinate in triangle " + std::to_string(i);
+                return false;
+            }
+        }
+        Box bb;
+        bb.reset();
+        bb.grow(f);
+        bb.grow(f + 3);
+        bb.grow(f + 6);
+        scene.grow(bb);
+        b.pbox[i] = bb;
+        b.idx[i] = (uint32_t)i;
+    }
+    // Pad every primitive box outward.  The double-precision certifier accepts
+    // barycentrics up to 1e-12 outside [0,1] (RayTriangleIntersection.cpp:46,54),
+    // i.e. hit points a hair outside the triangle; the pad (1e-9 of the scene
+    // extent plus one float step) keeps those inside the box.
+    const float ext = std::max(std::max(scene.hi[0] - scene.lo[0], scene.hi[1] - scene.lo[1]),
+                               std::max(scene.hi[2] - scene.lo[2], 1e-30f));
+    const float pad = ext * 1e-9f;
+    for (int64_t i = 0; i < n; ++i) {
+        Box& bb = b.pbox[i];
+        for (int k = 0; k < 3; ++k) {
+            bb.lo[k] = std::nextafter(bb.lo[k] - pad, -std::numeric_limits<float>::infinity());
+            bb.hi[k] = std::nextafter(bb.hi[k] + pad, std::numeric_limits<float>::infinity());
+            b.cent[3 * i + k] = 0.5f * bb.lo[k] + 0.5f * bb.hi[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        out.scene_lo[k] = scene.lo[k];
+        out.scene_hi[k] = scene.hi[k];
+    }
+
+    b.nodes.resize((size_t)(2 * n + 1));
+    int threads = params.threads > 0 ? params.threads : (int)std::thread::hardware_concurrency();
+    b.spawn_budget = std::max(0, 4 * threads);
+    b.build(0, 0, (uint32_t)n);
+    const int64_t n2 = b.next_node.load();
+    b.nodes.resize((size_t)n2);
+    out.stats.bvh2_nodes = n2;
+    {
+        const double ra = b.nodes[0].box.area();
+        double c = 0;
+        for (const Node2& nd : b.nodes)
+            c += (nd.left < 0 ? params.sah_intersect * nd.count : params.sah_traversal) * nd.box.area();
+        out.stats.sah_cost = ra > 0 ? c / ra : 0;
+    }
+
+    // Collapse to 8-wide nodes in breadth-first order.
+    struct Pending {
+        int32_t n2;
+        uint32_t n8;
+        int level;
+    };
+    std::vector<Pending> queue;
+    queue.reserve((size_t)n2 / 4 + 2);
+    out.nodes.reserve((size_t)n2 / 4 + 2);
+    out.tris.reserve((size_t)n);
+    out.nodes.emplace_back();
+    queue.push_back({0, 0, 1});
+    int max_level = 1;
+    for (size_t qi = 0; qi < queue.size(); ++qi) {
+        const Pending pd = queue[qi];
+        max_level = std::max(max_level, pd.level);
+        int32_t kids[8];
+        int nk = 0;
+        const Node2& root2 = b.nodes[pd.n2];
+        if (root2.left < 0) {
+            kids[nk++] = pd.n2;  // a leaf root: one leaf child
+        } else {
+            kids[nk++] = root2.left;
+            kids[nk++] = root2.left + 1;
+            while (nk < 8) {
+                int pick = -1;
+                double pa = -1;
+                for (int c = 0; c < nk; ++c) {
+                    const Node2& cn = b.nodes[kids[c]];
+                    if (cn.left >= 0 && cn.box.area() > pa) {
+                        pa = cn.box.area();
+                        pick = c;
+                    }
+                }
+                if (pick < 0) break;
+                const int32_t l = b.nodes[kids[pick]].left;
+                kids[pick] = l;
+                kids[nk++] = l + 1;
+            }
+        }
+
+        // Octant slot assignment: greedily give each slot the child whose
+        // centroid lies furthest toward that slot's octant direction.
+        const Box& nb = root2.box;
+        float ctr[3];
+        for (int k = 0; k < 3; ++k) ctr[k] = 0.5f * nb.lo[k] + 0.5f * nb.hi[k];
+        int slot_of[8];
+        int child_in[8];
+        for (int s = 0; s < 8; ++s) child_in[s] = -1;
+        for (int c = 0; c < nk; ++c) slot_of[c] = -1;
+        for (int round = 0; round < nk; ++round) {
+            double best = -std::numeric_limits<double>::infinity();
+            int bc = -1, bs = -1;
+            for (int c = 0; c < nk; ++c) {
+                if (slot_of[c] >= 0) continue;
+                const Box& cb = b.nodes[kids[c]].box;
+                double d[3];
+                for (int k = 0; k < 3; ++k) d[k] = (0.5 * cb.lo[k] + 0.5 * cb.hi[k]) - ctr[k];
+                for (int s = 0; s < 8; ++s) {
+                    if (child_in[s] >= 0) continue;
+                    const double v = ((s & 1) ? -d[0] : d[0]) + ((s & 2) ? -d[1] : d[1]) + ((s & 4) ? -d[2] : d[2]);
+                    if (v > best) {
+                        best = v;
+                        bc = c;
+                        bs = s;
+                    }
+                }
+            }
+            slot_of[bc] = bs;
+            child_in[bs] = bc;
+        }
+
+        Node8 node;
+        std::memset(&node, 0, sizeof(node));
+        // Internal children get contiguous node indices in slot order; leaf
+        // triangles are appended contiguously in slot order.
+        uint32_t child_base = (uint32_t)out.nodes.size();
+        uint32_t tri_base = (uint32_t)out.tris.size();
+        int n_internal = 0;
+        uint32_t tri_off = 0;
+        float clo[3][8], chi[3][8];
+        int slot_list[8];
+        int ns = 0;
+        for (int s = 0; s < 8; ++s) {
+            const int c = child_in[s];
+            if (c < 0) continue;
+            const Node2& cn = b.nodes[kids[c]];
+            for (int k = 0; k < 3; ++k) {
+                clo[k][ns] = cn.box.lo[k];
+                chi[k][ns] = cn.box.hi[k];
+            }
+            slot_list[ns++] = s;
+            if (cn.left >= 0) {
+                node.imask |= (uint8_t)(1u << s);
+                node.meta[s] = 1;
+                ++n_internal;
+            } else {
+                if (cn.count < 1 || cn.count > (uint32_t)kMaxLeafTris || tri_off + cn.count > 32) {
+                    err = "internal error: leaf layout exceeds node limits";
+                    return false;
+                }
+                node.meta[s] = (uint8_t)((cn.count << 5) | tri_off);
+                for (uint32_t t = 0; t < cn.count; ++t) {
+                    const uint32_t p = b.idx[cn.first + t];
+                    const float* f = tris + 9 * (size_t)p;
+                    Tri48 tr;
+                    for (int k = 0; k < 3; ++k) {
+                        tr.v0[k] = f[k];
+                        tr.v1[k] = f[3 + k];
+                        tr.v2[k] = f[6 + k];
+                    }
+                    const int32_t id = ids ? ids[p] : (int32_t)p;
+                    std::memcpy(&tr.v0[3], &id, 4);
+                    tr.v1[3] = 0.0f;
+                    tr.v2[3] = 0.0f;
+                    out.tris.push_back(tr);
+                }
+                tri_off += cn.count;
+            }
+        }
+        node.child_base = child_base;
+        node.tri_base = tri_base;
+        // Internal children: allocate in slot order.
+        for (int s = 0; s < 8; ++s) {
+            const int c = child_in[s];
+            if (c < 0 || !(node.imask & (1u << s))) continue;
+            const uint32_t idx8 = (uint32_t)out.nodes.size();
+            out.nodes.emplace_back();
+            queue.push_back({kids[c], idx8, pd.level + 1});
+        }
+        (void)n_internal;
+
+        // Quantize child boxes relative to the node box.
+        Box qb;
+        qb.reset();
+        for (int i = 0; i < ns; ++i)
+            for (int k = 0; k < 3; ++k) {
+                qb.lo[k] = std::min(qb.lo[k], clo[k][i]);
+                qb.hi[k] = std::max(qb.hi[k], chi[k][i]);
+            }
+        float pp[3];
+        int8_t ee[3];
+        uint8_t ql[3][8], qh[3][8];
+        for (int k = 0; k < 3; ++k) {
+            pp[k] = qb.lo[k];
+            const double span = (double)qb.hi[k] - qb.lo[k];
+            int e = -126;
+            if (span > 0) {
+                int ex;
+                std::frexp(span / 255.0, &ex);  // span/255 < 2^ex
+                e = std::max(-126, ex);
+            }
+            for (;;) {
+                if (e > 127) {
+                    err = "internal error: child box quantization overflow";
+                    return false;
+                }
+                if (quantize_axis(pp[k], e, clo[k], chi[k], ns, ql[k], qh[k])) break;
+                ++e;
+            }
+            ee[k] = (int8_t)e;
+        }
+        node.px = pp[0];
+        node.py = pp[1];
+        node.pz = pp[2];
+        node.ex = ee[0];
+        node.ey = ee[1];
+        node.ez = ee[2];
+        for (int s = 0; s < 8; ++s) {
+            node.qlox[s] = node.qloy[s] = node.qloz[s] = 255;
+            node.qhix[s] = node.qhiy[s] = node.qhiz[s] = 0;
+        }
+        for (int i = 0; i < ns; ++i) {
+            const int s = slot_list[i];
+            node.qlox[s] = ql[0][i];
+            node.qloy[s] = ql[1][i];
+            node.qloz[s] = ql[2][i];
+            node.qhix[s] = qh[0][i];
+            node.qhiy[s] = qh[1][i];
+            node.qhiz[s] = qh[2][i];
+        }
+        out.nodes[pd.n8] = node;
+    }
+    if (max_level > kMaxStack) {
+        err = "BVH too deep for the traversal stack (" + std::to_string(max_level) + " levels)";
+        return false;
+    }
+    out.stats.nodes = (int64_t)out.nodes.size();
+    out.stats.tris = (int64_t)out.tris.size();
+    out.stats.depth = max_level;
+    out.stats.build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return true;
+}
+
+}  // namespace rise_hip

@@ -1,0 +1,52 @@
+// bvh_build.h -- host-side build of the 8-wide quantized BVH.
+//
+// Stage 1 is a top-down binned-SAH BVH2 in the spirit of the reference's
+// builder (Acceleration/BVH.h:772-1034: binned SAH, unit traversal and
+// intersection costs, median fallback), evaluated on all three axes and run
+// in parallel over subtrees.  Stage 2 collapses it into 8-wide nodes (the
+// reference collapses to 4-wide, BVH.h:477-642), assigns child slots by
+// octant, and quantizes child boxes outward to 8 bits per bound.
+#pragma once
+
+#include "bvh8.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace rise_hip {
+
+struct BuildParams {
+    int bins = 32;              // SAH bins per axis (reference: 32)
+    int max_leaf = kMaxLeafTris;
+    float sah_traversal = 1.0f;  // reference mesh costs (TriangleMeshGeometryIndexed.cpp:529-534)
+    float sah_intersect = 1.0f;
+    int threads = 0;            // 0 = hardware concurrency
+};
+
+struct BuildStats {
+    int64_t bvh2_nodes = 0;
+    int64_t nodes = 0;
+    int64_t tris = 0;
+    int depth = 0;             // wide-node levels (root = 1)
+    double build_ms = 0.0;
+    double sah_cost = 0.0;     // SAH cost of the BVH2, normalized by root area
+};
+
+struct WideBVH {
+    std::vector<Node8> nodes;
+    std::vector<Tri48> tris;   // leaf order; v0.w carries the caller's id
+    float scene_lo[3] = {0, 0, 0};
+    float scene_hi[3] = {0, 0, 0};
+    BuildStats stats;
+};
+
+// tris: n * 9 floats (v0, v1, v2).  ids: optional caller ids (NULL = index).
+// Returns false and fills err on failure (too deep, bad input).
+bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const BuildParams& params,
+                    WideBVH& out, std::string& err);
+
+// Decoded child box of slot s (exactly the arithmetic the kernels use).
+void decode_child_box(const Node8& n, int s, float lo[3], float hi[3]);
+
+}  // namespace rise_hip

@@ -1,0 +1,109 @@
+// host_check.hip -- runs the kernels' traversal code (trace_core.h) on host
+// threads, for validation on machines without a GPU.
+#include "rise_hip_debug.h"
+
+#include "api_common.h"
+#include "bvh_build.h"
+#include "trace_core.h"
+
+#include <atomic>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace rise_hip;
+
+namespace {
+
+struct HostStack {
+    uint2 e[kMaxStack];
+    int sp = 0;
+    void push(uint2 v)
+    {
+        if (sp >= kMaxStack) throw std::runtime_error("host traversal stack overflow");
+        e[sp++] = v;
+    }
+    uint2 pop() { return e[--sp]; }
+    bool empty() const { return sp == 0; }
+};
+
+}  // namespace
+
+extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t n_tris, const float* rays,
+                                                     const float* tmax, int64_t n, int32_t mode, int32_t* prim_id,
+                                                     float* t, uint8_t* occluded, uint64_t* counters,
+                                                     rise_hip_scene_stats* stats)
+{
+    clear_error();
+    if (n_tris < 0 || n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative count");
+    if ((n_tris > 0 && !tris) || (n > 0 && !rays)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (mode == 0 && n > 0 && (!prim_id || !t)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if (mode == 1 && n > 0 && !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if (mode != 0 && mode != 1) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
+    WideBVH bvh;
+    std::string err;
+    try {
+        if (!build_wide_bvh(tris, n_tris, nullptr, BuildParams(), bvh, err))
+            return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    if (stats) {
+        *stats = rise_hip_scene_stats{};
+        stats->n_tris = n_tris;
+        stats->n_nodes = (int64_t)bvh.nodes.size();
+        stats->n_bvh2_nodes = bvh.stats.bvh2_nodes;
+        stats->depth = bvh.stats.depth;
+        stats->node_bytes = (int32_t)sizeof(Node8);
+        stats->tri_bytes = (int32_t)sizeof(Tri48);
+        stats->build_ms = bvh.stats.build_ms;
+        stats->sah_cost = bvh.stats.sah_cost;
+    }
+    const Node8* nodes = bvh.nodes.data();
+    const Tri48* trs = bvh.tris.data();
+    std::atomic<int64_t> next(0);
+    std::atomic<unsigned long long> cn(0), ct(0);
+    std::atomic<bool> overflow(false);
+    auto worker = [&]() {
+        unsigned long long ln = 0, lt = 0;
+        for (;;) {
+            const int64_t s = next.fetch_add(64);
+            if (s >= n) break;
+            const int64_t e = std::min<int64_t>(s + 64, n);
+            for (int64_t i = s; i < e; ++i) {
+                RayState r;
+                init_ray(r, rays[i], rays[n + i], rays[2 * n + i], rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+                HostStack st;
+                unsigned nn = 0, nt = 0;
+                try {
+                    if (mode == 0) {
+                        double best;
+                        const int id = trace_closest<true>(r, nodes, trs, st, best, nn, nt);
+                        prim_id[i] = id;
+                        t[i] = id >= 0 ? (float)world_range(r, best) : INFINITY;
+                    } else {
+                        const double hf = tmax ? (double)tmax[i] : kRiseInfinity;
+                        occluded[i] = trace_any<true>(r, nodes, trs, st, scale_how_far(r, hf), nn, nt) ? 1 : 0;
+                    }
+                } catch (const std::exception&) {
+                    overflow = true;
+                }
+                ln += nn;
+                lt += nt;
+            }
+        }
+        cn += ln;
+        ct += lt;
+    };
+    const int nth = std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::thread> th;
+    for (int k = 1; k < nth; ++k) th.emplace_back(worker);
+    worker();
+    for (auto& x : th) x.join();
+    if (overflow) return fail(RISE_HIP_ERR_BUILD, "traversal stack overflow");
+    if (counters) {
+        counters[0] = cn.load();
+        counters[1] = ct.load();
+    }
+    return RISE_HIP_OK;
+}

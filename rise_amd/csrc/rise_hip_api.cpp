@@ -1,0 +1,200 @@
+// rise_hip_api.cpp -- C ABI: scene upload and batch ray queries.
+#include "rise_hip.h"
+
+#include "api_common.h"
+#include "bvh_build.h"
+#include "trace_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace rise_hip;
+
+struct rise_hip_scene {
+    Node8* d_nodes = nullptr;
+    Tri48* d_tris = nullptr;
+    int device = 0;
+    rise_hip_scene_stats stats;
+    TraceScene view() const { return TraceScene{d_nodes, d_tris, stats.depth}; }
+};
+
+namespace rise_hip {
+
+static thread_local std::string g_last_error;
+
+rise_hip_status fail(rise_hip_status s, const std::string& msg)
+{
+    g_last_error = msg;
+    return s;
+}
+
+rise_hip_status fail_hip(hipError_t e, const char* what)
+{
+    return fail(e == hipErrorOutOfMemory ? RISE_HIP_ERR_OUT_OF_MEMORY : RISE_HIP_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void clear_error() { g_last_error.clear(); }
+
+}  // namespace rise_hip
+
+extern "C" {
+
+const char* rise_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char* rise_hip_version(void) { return "0.1.0"; }
+
+rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n_meshes, rise_hip_scene** out_scene)
+{
+    clear_error();
+    if (!out_scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "out_scene is NULL");
+    *out_scene = nullptr;
+    if (n_meshes > 0 && !meshes) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "meshes is NULL");
+    int64_t total = 0;
+    for (size_t m = 0; m < n_meshes; ++m) {
+        if (meshes[m].n_tris < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative triangle count");
+        if (meshes[m].n_tris > 0 && !meshes[m].vertices)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh vertices is NULL");
+        total += meshes[m].n_tris;
+    }
+    if (total > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "more than 2^31-1 triangles");
+
+    // Concatenate meshes (identity transforms: world space == object space).
+    std::vector<float> verts;
+    std::vector<int32_t> ids;
+    try {
+        verts.resize((size_t)total * 9);
+        ids.resize((size_t)total);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    int64_t off = 0;
+    for (size_t m = 0; m < n_meshes; ++m) {
+        const int64_t k = meshes[m].n_tris;
+        if (k == 0) continue;
+        std::memcpy(verts.data() + off * 9, meshes[m].vertices, (size_t)k * 9 * sizeof(float));
+        for (int64_t i = 0; i < k; ++i) ids[off + i] = meshes[m].tri_ids ? meshes[m].tri_ids[i] : (int32_t)(off + i);
+        off += k;
+    }
+
+    WideBVH bvh;
+    std::string err;
+    BuildParams params;
+    try {
+        if (!build_wide_bvh(verts.data(), total, ids.data(), params, bvh, err))
+            return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+    }
+
+    rise_hip_scene* s = new (std::nothrow) rise_hip_scene;
+    if (!s) return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    hipError_t e = hipGetDevice(&s->device);
+    if (e != hipSuccess) {
+        delete s;
+        return fail_hip(e, "hipGetDevice");
+    }
+    const size_t nb = bvh.nodes.size() * sizeof(Node8);
+    const size_t tb = std::max<size_t>(bvh.tris.size(), 1) * sizeof(Tri48);
+    e = hipMalloc((void**)&s->d_nodes, nb);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_tris, tb);
+    if (e == hipSuccess) e = hipMemcpy(s->d_nodes, bvh.nodes.data(), nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !bvh.tris.empty())
+        e = hipMemcpy(s->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(Tri48), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(s->d_nodes);
+        hipFree(s->d_tris);
+        delete s;
+        return fail_hip(e, "scene upload");
+    }
+    std::memset(&s->stats, 0, sizeof(s->stats));
+    s->stats.n_tris = total;
+    s->stats.n_nodes = (int64_t)bvh.nodes.size();
+    s->stats.n_bvh2_nodes = bvh.stats.bvh2_nodes;
+    s->stats.depth = bvh.stats.depth;
+    s->stats.node_bytes = (int32_t)sizeof(Node8);
+    s->stats.tri_bytes = (int32_t)sizeof(Tri48);
+    s->stats.build_ms = bvh.stats.build_ms;
+    s->stats.sah_cost = bvh.stats.sah_cost;
+    s->stats.device_bytes = (int64_t)(nb + tb);
+    *out_scene = s;
+    return RISE_HIP_OK;
+}
+
+rise_hip_status rise_hip_scene_release(rise_hip_scene* scene)
+{
+    clear_error();
+    if (!scene) return RISE_HIP_OK;
+    hipFree(scene->d_nodes);
+    hipFree(scene->d_tris);
+    delete scene;
+    return RISE_HIP_OK;
+}
+
+rise_hip_status rise_hip_scene_get_stats(const rise_hip_scene* scene, rise_hip_scene_stats* out_stats)
+{
+    clear_error();
+    if (!scene || !out_stats) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out_stats = scene->stats;
+    return RISE_HIP_OK;
+}
+
+static rise_hip_status closest_impl(rise_hip_scene* scene, const float* rays, int64_t n, int32_t* prim_id, float* t,
+                                    uint64_t* counters, hipStream_t stream)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n == 0) return RISE_HIP_OK;
+    if (!rays || !prim_id || !t) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    const hipError_t e = launch_closest(scene->view(), rays, n, prim_id, t,
+                                        reinterpret_cast<unsigned long long*>(counters), stream);
+    if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
+    return RISE_HIP_OK;
+}
+
+static rise_hip_status any_impl(rise_hip_scene* scene, const float* rays, const float* tmax, int64_t n,
+                                uint8_t* occluded, uint64_t* counters, hipStream_t stream)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n == 0) return RISE_HIP_OK;
+    if (!rays || !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    const hipError_t e = launch_any(scene->view(), rays, tmax, n, occluded,
+                                    reinterpret_cast<unsigned long long*>(counters), stream);
+    if (e != hipSuccess) return fail_hip(e, "any-hit launch");
+    return RISE_HIP_OK;
+}
+
+rise_hip_status rise_hip_intersect_closest(rise_hip_scene* scene, const float* rays_soa, int64_t n, int32_t* prim_id,
+                                           float* t, hipStream_t stream)
+{
+    return closest_impl(scene, rays_soa, n, prim_id, t, nullptr, stream);
+}
+
+rise_hip_status rise_hip_intersect_any(rise_hip_scene* scene, const float* rays_soa, const float* tmax, int64_t n,
+                                       uint8_t* occluded, hipStream_t stream)
+{
+    return any_impl(scene, rays_soa, tmax, n, occluded, nullptr, stream);
+}
+
+rise_hip_status rise_hip_intersect_closest_counted(rise_hip_scene* scene, const float* rays_soa, int64_t n,
+                                                   int32_t* prim_id, float* t, uint64_t* counters, hipStream_t stream)
+{
+    if (!counters) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "counters is NULL");
+    return closest_impl(scene, rays_soa, n, prim_id, t, counters, stream);
+}
+
+rise_hip_status rise_hip_intersect_any_counted(rise_hip_scene* scene, const float* rays_soa, const float* tmax,
+                                               int64_t n, uint8_t* occluded, uint64_t* counters, hipStream_t stream)
+{
+    if (!counters) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "counters is NULL");
+    return any_impl(scene, rays_soa, tmax, n, occluded, counters, stream);
+}
+
+}  // extern "C"

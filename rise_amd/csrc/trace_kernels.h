@@ -1,0 +1,27 @@
+// trace_kernels.h -- host-side launchers for the traversal kernels.
+#pragma once
+
+#include "bvh8.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace rise_hip {
+
+struct TraceScene {
+    const Node8* nodes;
+    const Tri48* tris;
+    int depth;  // wide-node levels; selects the stack variant
+};
+
+// Closest hit: rays SoA (ox, oy, oz, dx, dy, dz), n each.  counters: NULL, or
+// two uint64 (nodes visited, triangles tested) accumulated over all rays.
+hipError_t launch_closest(const TraceScene& s, const float* rays, int64_t n, int32_t* prim_id, float* t,
+                          unsigned long long* counters, hipStream_t stream);
+
+// Any hit within tmax (world distance per ray; NULL = unbounded).
+hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax, int64_t n, uint8_t* occluded,
+                      unsigned long long* counters, hipStream_t stream);
+
+}  // namespace rise_hip

@@ -1,0 +1,157 @@
+"""Python mirror of RISE's object-manager ray queries over the C ABI.
+
+``ObjectManager`` follows ``IObjectManager`` (Interfaces/IObjectManager.h:38-68
+in the reference): meshes are added, ``prepare_for_rendering`` builds and
+uploads the acceleration structure (ObjectManager::PrepareForRendering,
+Managers/ObjectManager.cpp:404-423), and ``intersect_ray`` /
+``intersect_shadow_ray`` answer closest-hit and any-hit queries, here for a
+whole batch of rays resident in device memory.  Errors raise
+:class:`rise_amd._lib.RiseHipError` (the reference returns bool and logs).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        import torch
+
+        return torch.cuda.current_stream().cuda_stream
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+class ObjectManager:
+    """Resident triangle scene answering batched ray queries on the GPU."""
+
+    def __init__(self) -> None:
+        self._meshes: list[tuple[np.ndarray, np.ndarray | None]] = []
+        self._scene = ctypes.c_void_p(None)
+
+    def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None) -> None:
+        """tris: (n, 9) float32 world-space triangles; ids: optional (n,) int32."""
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
+        if i is not None and i.shape[0] != t.shape[0]:
+            raise ValueError("ids must have one entry per triangle")
+        self._meshes.append((t, i))
+
+    def prepare_for_rendering(self) -> None:
+        L = _lib.lib()
+        self.release()
+        descs = (_lib.MeshDesc * max(1, len(self._meshes)))()
+        for k, (t, i) in enumerate(self._meshes):
+            descs[k].vertices = t.ctypes.data
+            descs[k].n_tris = t.shape[0]
+            descs[k].tri_ids = None if i is None else i.ctypes.data
+        out = ctypes.c_void_p(None)
+        _lib.check(L.rise_hip_scene_upload(descs, len(self._meshes), ctypes.byref(out)))
+        self._scene = out
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if not self._scene.value:
+            raise RuntimeError("prepare_for_rendering() has not been called")
+        return self._scene
+
+    def stats(self) -> dict:
+        s = _lib.SceneStats()
+        _lib.check(_lib.lib().rise_hip_scene_get_stats(self.handle, ctypes.byref(s)))
+        return s.as_dict()
+
+    def intersect_ray(self, rays, prim_id=None, t=None, stream=None, counters=None):
+        """rays: (6, n) float32 CUDA tensor.  Returns (prim_id int32, t float32)."""
+        import torch
+
+        n = rays.shape[1]
+        _check_rays(rays)
+        if prim_id is None:
+            prim_id = torch.empty(n, dtype=torch.int32, device=rays.device)
+        if t is None:
+            t = torch.empty(n, dtype=torch.float32, device=rays.device)
+        L = _lib.lib()
+        if counters is None:
+            st = L.rise_hip_intersect_closest(
+                self.handle, rays.data_ptr(), n, prim_id.data_ptr(), t.data_ptr(), _stream_handle(stream)
+            )
+        else:
+            st = L.rise_hip_intersect_closest_counted(
+                self.handle, rays.data_ptr(), n, prim_id.data_ptr(), t.data_ptr(), counters.data_ptr(),
+                _stream_handle(stream),
+            )
+        _lib.check(st)
+        return prim_id, t
+
+    def intersect_shadow_ray(self, rays, how_far=None, occluded=None, stream=None, counters=None):
+        """Any-hit within how_far (world distance, (n,) float32 tensor or None)."""
+        import torch
+
+        n = rays.shape[1]
+        _check_rays(rays)
+        if occluded is None:
+            occluded = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        hf = 0 if how_far is None else how_far.data_ptr()
+        L = _lib.lib()
+        if counters is None:
+            st = L.rise_hip_intersect_any(self.handle, rays.data_ptr(), hf, n, occluded.data_ptr(), _stream_handle(stream))
+        else:
+            st = L.rise_hip_intersect_any_counted(
+                self.handle, rays.data_ptr(), hf, n, occluded.data_ptr(), counters.data_ptr(), _stream_handle(stream)
+            )
+        _lib.check(st)
+        return occluded
+
+    def release(self) -> None:
+        if self._scene.value:
+            _lib.lib().rise_hip_scene_release(self._scene)
+            self._scene = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def _check_rays(rays) -> None:
+    import torch
+
+    if rays.dtype != torch.float32 or rays.dim() != 2 or rays.shape[0] != 6 or not rays.is_contiguous():
+        raise ValueError("rays must be a contiguous (6, n) float32 tensor")
+    if not rays.is_cuda:
+        raise ValueError("rays must be in device memory")
+
+
+def host_trace(tris: np.ndarray, rays: np.ndarray, mode: int = 0, tmax: np.ndarray | None = None):
+    """Runs the kernels' traversal code on host threads (validation only).
+    Returns (prim_id, t, counters, stats) for mode 0, (occluded, counters, stats)
+    for mode 1."""
+    L = _lib.lib()
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    counters = np.zeros(2, dtype=np.uint64)
+    stats = _lib.SceneStats()
+    tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
+    if mode == 0:
+        pid = np.empty(n, dtype=np.int32)
+        t = np.empty(n, dtype=np.float32)
+        _lib.check(
+            L.rise_hip_debug_trace_host(
+                tris.ctypes.data, tris.shape[0], rays.ctypes.data, None, n, 0, pid.ctypes.data, t.ctypes.data,
+                None, counters.ctypes.data, ctypes.byref(stats),
+            )
+        )
+        return pid, t, counters, stats.as_dict()
+    occ = np.empty(n, dtype=np.uint8)
+    _lib.check(
+        L.rise_hip_debug_trace_host(
+            tris.ctypes.data, tris.shape[0], rays.ctypes.data, None if tm is None else tm.ctypes.data, n, 1,
+            None, None, occ.ctypes.data, counters.ctypes.data, ctypes.byref(stats),
+        )
+    )
+    return occ, counters, stats.as_dict()

@@ -1,0 +1,96 @@
+"""Seeded synthetic workloads (numpy).
+
+``soup_triangles`` and ``cast_rays`` build the ray-cast workload: one million
+triangles with centroids uniform in the unit cube and Gaussian edge vectors,
+1920x1080 coherent pinhole rays plus four million incoherent rays.  All
+arrays are float32, triangles as (n, 9) rows (v0, v1, v2) and rays as a
+(6, n) SoA (ox, oy, oz, dx, dy, dz), the layout the C ABI takes.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SOUP_TRIS = 1_000_000
+SOUP_SIGMA = 0.003
+SOUP_SEED = 2
+CAST_WIDTH = 1920
+CAST_HEIGHT = 1080
+CAST_INCOHERENT = 4_000_000
+CAST_SEED = 22
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def soup_triangles(n: int = SOUP_TRIS, seed: int = SOUP_SEED, sigma: float = SOUP_SIGMA) -> np.ndarray:
+    """Random triangle soup: centroid c ~ U[0,1)^3, edges e1, e2 ~ N(0, sigma^2)^3,
+    v0 = c - (e1 + e2) / 3, v1 = v0 + e1, v2 = v0 + e2.  Returns (n, 9) float32."""
+    g = _rng(seed)
+    c = g.random((n, 3))
+    e1 = g.normal(0.0, sigma, (n, 3))
+    e2 = g.normal(0.0, sigma, (n, 3))
+    v0 = c - (e1 + e2) / 3.0
+    out = np.empty((n, 9), dtype=np.float32)
+    out[:, 0:3] = v0
+    out[:, 3:6] = v0 + e1
+    out[:, 6:9] = v0 + e2
+    return out
+
+
+def pinhole_rays(
+    width: int,
+    height: int,
+    eye=(0.5, 0.5, -1.0),
+    fov_deg: float = 40.0,
+    jitter: np.ndarray | None = None,
+) -> np.ndarray:
+    """Pixel-centre pinhole rays looking down +z, vertical field of view
+    fov_deg, square pixels, row 0 at the top.  The screen mapping follows
+    PinholeCamera::ComputeScaleFromFOV (Cameras/PinholeCamera.cpp:207-212):
+    x' = (sx - W/2) * h / H, y' = -(sy - H/2) * h / H, h = 2 tan(fov/2).
+    Returns (6, width*height) float32 SoA with unit directions."""
+    h = 2.0 * math.tan(math.radians(fov_deg) * 0.5)
+    ys, xs = np.meshgrid(np.arange(height, dtype=np.float64), np.arange(width, dtype=np.float64), indexing="ij")
+    sx = xs + 0.5
+    sy = ys + 0.5
+    if jitter is not None:
+        sx = sx + jitter[0].reshape(height, width) - 0.5
+        sy = sy + jitter[1].reshape(height, width) - 0.5
+    x = (sx - width * 0.5) * h / height
+    y = -(sy - height * 0.5) * h / height
+    d = np.stack([x.ravel(), y.ravel(), np.ones(width * height)], 0)
+    d /= np.linalg.norm(d, axis=0, keepdims=True)
+    out = np.empty((6, width * height), dtype=np.float32)
+    out[0], out[1], out[2] = eye
+    out[3:6] = d
+    return out
+
+
+def incoherent_rays(n: int, seed: int) -> np.ndarray:
+    """Origins uniform in the unit cube, directions uniform on the sphere."""
+    g = _rng(seed)
+    o = g.random((3, n))
+    d = g.normal(0.0, 1.0, (3, n))
+    d /= np.linalg.norm(d, axis=0, keepdims=True)
+    return np.concatenate([o, d], 0).astype(np.float32)
+
+
+def cast_rays(
+    width: int = CAST_WIDTH,
+    height: int = CAST_HEIGHT,
+    n_incoherent: int = CAST_INCOHERENT,
+    seed: int = CAST_SEED,
+) -> np.ndarray:
+    """The ray-cast ray set: width*height coherent pinhole rays followed by
+    n_incoherent incoherent rays, (6, n) float32."""
+    a = pinhole_rays(width, height)
+    b = incoherent_rays(n_incoherent, seed)
+    return np.ascontiguousarray(np.concatenate([a, b], 1))
+
+
+def rays_from_rows(o: np.ndarray, d: np.ndarray) -> np.ndarray:
+    """(n,3) origins and directions to a contiguous (6, n) float32 SoA."""
+    return np.ascontiguousarray(np.concatenate([np.asarray(o, np.float32).T, np.asarray(d, np.float32).T], 0))

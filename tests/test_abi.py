@@ -1,0 +1,35 @@
+"""The C ABI library loads on a machine without a GPU, exports every function
+declared in include/*.h, and rejects bad arguments with status codes (no
+compute calls here)."""
+import ctypes
+
+from rise_amd import _lib
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = _lib.declared_symbols()
+    assert "rise_hip_intersect_closest" in declared
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_version_and_error_strings():
+    L = _lib.lib()
+    assert L.rise_hip_version().decode().count(".") == 2
+    assert isinstance(L.rise_hip_last_error(), bytes)
+
+
+def test_null_arguments_rejected():
+    L = _lib.lib()
+    assert L.rise_hip_intersect_closest(None, None, 10, None, None, None) == 1
+    assert b"scene" in L.rise_hip_last_error()
+    assert L.rise_hip_intersect_any(None, None, None, 10, None, None) == 1
+    assert L.rise_hip_scene_upload(None, 0, None) == 1
+    assert L.rise_hip_scene_get_stats(None, None) == 1
+    assert L.rise_hip_scene_release(None) == 0
+    out = ctypes.c_void_p()
+    desc = _lib.MeshDesc(None, 5, None)
+    assert L.rise_hip_scene_upload(ctypes.byref(desc), 1, ctypes.byref(out)) == 1
+    desc = _lib.MeshDesc(None, -1, None)
+    assert L.rise_hip_scene_upload(ctypes.byref(desc), 1, ctypes.byref(out)) == 1

@@ -1,0 +1,151 @@
+"""Pins the ray-cast oracle (oracle/rt_oracle.c) before anything is checked
+against it.
+
+* Known-answer tests restated from the reference's own
+  tests/RayTriangleIntersectionTest.cpp:16-119.
+* Closed forms: axis-aligned triangles hit along an axis.
+* The reference's BVH4 path compiled from its sources (oracle/_ref), on
+  random soups, and the golden fixture that path produced
+  (tests/golden/raycast_small.npz, tests/golden/make_raycast_golden.py).
+* The tie rule (exact-t ties -> lowest id) on duplicated triangles.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from rise_amd import scenes
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "raycast_small.npz"
+
+V0 = (0.0, 0.0, 0.0)
+EA = (1.0, 0.0, 0.0)
+EB = (0.0, 1.0, 0.0)
+
+
+def close(a, b, eps=1e-6):
+    return abs(a - b) <= eps
+
+
+# --- reference KATs: tests/RayTriangleIntersectionTest.cpp -----------------
+
+
+def test_kat_interior_hit():
+    hit, t, a, b = oracle.ray_triangle((0.25, 0.25, 1.0), (0, 0, -1), V0, EA, EB)
+    assert hit and close(t, 1.0) and close(a, 0.25) and close(b, 0.25)
+
+
+def test_kat_edge_and_vertex_hits():
+    hit, t, a, b = oracle.ray_triangle((0.5, 0.5, 2.0), (0, 0, -2), V0, EA, EB)
+    assert hit and close(t, 1.0) and close(a, 0.5) and close(b, 0.5)
+    hit, t, a, b = oracle.ray_triangle((1.0, 0.0, 3.0), (0, 0, -3), V0, EA, EB)
+    assert hit and close(t, 1.0) and close(a, 1.0) and close(b, 0.0)
+
+
+def test_kat_misses():
+    assert not oracle.ray_triangle((0.75, 0.75, 1.0), (0, 0, -1), V0, EA, EB)[0]
+    assert not oracle.ray_triangle((0.25, 0.25, -1.0), (0, 0, -1), V0, EA, EB)[0]
+    assert not oracle.ray_triangle((0.25, 0.25, 1.0), (1, 0, 0), V0, EA, EB)[0]
+
+
+def test_kat_winding_independence():
+    f = oracle.ray_triangle((0.2, 0.3, 1.0), (0, 0, -1), V0, (1, 0, 0), (0, 1, 0))
+    r = oracle.ray_triangle((0.2, 0.3, 1.0), (0, 0, -1), V0, (0, 1, 0), (1, 0, 0))
+    assert f[0] and r[0] and close(f[1], r[1])
+
+
+def test_nearzero_tolerances():
+    # alpha may undershoot 0 by up to 1e-12 (RayTriangleIntersection.cpp:46).
+    assert oracle.ray_triangle((-5e-13, 0.25, 1.0), (0, 0, -1), V0, EA, EB)[0]
+    assert not oracle.ray_triangle((-5e-12, 0.25, 1.0), (0, 0, -1), V0, EA, EB)[0]
+    # A hit at t below NEARZERO is rejected.
+    assert not oracle.ray_triangle((0.25, 0.25, 1e-13), (0, 0, -1), V0, EA, EB)[0]
+
+
+# --- closed forms ----------------------------------------------------------
+
+
+def test_axis_aligned_closed_form():
+    g = np.random.Generator(np.random.PCG64(7))
+    n = 64
+    z = g.uniform(-1, 1, n)
+    tris = np.zeros((n, 9), np.float32)
+    for i in range(n):
+        # Large triangle in plane z = z[i], offset in x so each ray sees one.
+        x0 = 10.0 * i
+        tris[i] = [x0, 0, z[i], x0 + 4, 0, z[i], x0, 4, z[i]]
+    z32 = tris[:, 2].astype(np.float64)
+    o = np.stack([10.0 * np.arange(n) + 1.0, np.full(n, 1.0), np.full(n, 5.0)], 1)
+    d = np.tile([0.0, 0.0, -1.0], (n, 1))
+    rays = scenes.rays_from_rows(o, d)
+    pid, t, t64 = oracle.closest(tris, rays)
+    assert np.array_equal(pid, np.arange(n))
+    expect = 5.0 - z32  # exact distance
+    assert np.all(np.abs(t64 - expect) <= 4 * np.spacing(expect) + 1e-12 + 1e-15)
+
+
+def test_world_range_normalizes_direction():
+    # The same ray with direction scaled by 7 reports the same world distance
+    # (Object.cpp normalizes the direction before the mesh test).
+    tris = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    r1 = scenes.rays_from_rows([[0.25, 0.25, 2.0]], [[0, 0, -1]])
+    r7 = scenes.rays_from_rows([[0.25, 0.25, 2.0]], [[0, 0, -7]])
+    a = oracle.closest(tris, r1)
+    b = oracle.closest(tris, r7)
+    assert a[0][0] == b[0][0] == 0
+    assert abs(a[2][0] - 2.0) < 1e-11 and abs(b[2][0] - 2.0) < 1e-11
+
+
+# --- tie rule --------------------------------------------------------------
+
+
+def test_ties_go_to_lowest_id():
+    base = np.array([0.1, 0.1, 0.5, 0.9, 0.1, 0.5, 0.1, 0.9, 0.5], np.float32)
+    tris = np.stack([base + np.float32(0.0)] * 5)
+    tris[0, 2::3] += 0.25  # id 0 lies farther away than the four duplicates
+    rays = scenes.rays_from_rows([[0.3, 0.3, 0.0]], [[0, 0, 1]])
+    pid, _, t64 = oracle.closest(tris, rays)
+    assert pid[0] == 1 and abs(t64[0] - 0.5) < 1e-12
+
+
+# --- the reference's own code ----------------------------------------------
+
+
+def test_golden_fixture_closest_and_any():
+    z = np.load(GOLDEN)
+    pid, t, t64 = oracle.closest(z["tris"], z["rays"])
+    assert np.array_equal(pid, z["prim_id"])
+    hit = pid >= 0
+    assert np.array_equal(t64[hit], z["t64"][hit])
+    occ = oracle.any_hit(z["tris"], z["rays"], z["tmax"])
+    assert np.array_equal(occ, z["occluded"])
+    assert 0.3 < hit.mean() < 0.9 and 0.1 < occ.mean() < 0.9
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("seed,n,sigma", [(11, 20000, 0.003), (12, 8000, 0.03), (13, 2000, 0.1)])
+def test_oracle_matches_reference_bvh(seed, n, sigma):
+    tris = scenes.soup_triangles(n, seed=seed, sigma=sigma)
+    rays = np.concatenate([scenes.pinhole_rays(96, 64), scenes.incoherent_rays(6000, seed + 100)], 1)
+    pid, t, t64 = oracle.closest(tris, rays)
+    ref = oracle.RefMesh(tris, "ieee")
+    rp, rt, rt64 = ref.closest(rays)
+    assert np.array_equal(pid, rp)
+    hit = pid >= 0
+    assert np.array_equal(t64[hit], rt64[hit])
+    tmax = np.where(hit, np.where(np.arange(len(pid)) % 2 == 0, t * 0.999, t * 1.001), 0.3).astype(np.float32)
+    assert np.array_equal(oracle.any_hit(tris, rays, tmax), ref.any_hit(rays, tmax))
+
+
+@pytest.mark.skipif(not oracle.ref_available("fast"), reason="oracle/_ref not built (no reference tree)")
+def test_reference_linux_flags_agree_on_ids():
+    # The reference built with its own Linux flags (FMA contraction allowed)
+    # commits the same triangles; distances agree far inside 1e-5 relative.
+    tris = scenes.soup_triangles(20000, seed=21, sigma=0.01)
+    rays = np.concatenate([scenes.pinhole_rays(96, 64), scenes.incoherent_rays(6000, 121)], 1)
+    pid, t, t64 = oracle.closest(tris, rays)
+    rp, rt, rt64 = oracle.RefMesh(tris, "fast").closest(rays)
+    assert np.array_equal(pid, rp)
+    hit = pid >= 0
+    assert np.max(np.abs(rt64[hit] - t64[hit]) / t64[hit]) < 1e-12
