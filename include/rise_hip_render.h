@@ -1,0 +1,118 @@
+/*
+ * rise_hip_render.h -- C ABI of the spectral (hero-wavelength) path tracer.
+ *
+ * Replaces the pass loop and per-pixel integration of the reference
+ * (paths relative to src/Library/ of RISE):
+ *   - IRasterizer::RasterizeScene (Interfaces/IRasterizer.h:70) driven by
+ *     PixelBasedRasterizerHelper::RasterizeScene / RasterizeScenePass
+ *     (Rendering/PixelBasedRasterizerHelper.cpp:876-1300), and
+ *   - PathTracingSpectralRasterizer::IntegratePixel / IntegratePixelSpectral
+ *     (Rendering/PathTracingSpectralRasterizer.cpp:121-479) with
+ *     PathTracingIntegrator::IntegrateRayHWSS / IntegrateFromHitHWSS
+ *     (Shaders/PathTracingIntegrator.cpp:3939-5126),
+ * configured as RISE_API_CreatePathTracingSpectralRasterizer does
+ * (RISE_API.h:3112-3141): HWSS with 4 wavelengths, box pixel filter, no
+ * denoiser, no adaptive sampling.
+ *
+ * Conventions as in rise_hip.h.  All *_desc inputs are host memory and are
+ * copied; film outputs are device memory.
+ */
+#ifndef RISE_HIP_RENDER_H
+#define RISE_HIP_RENDER_H
+
+#include "rise_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A step spectrum: n equal bins over [lambda_begin, lambda_end]
+ * (SpectralPacket, Utilities/Color/SpectralPacket.h:109-124, 285-297). */
+typedef struct rise_hip_spd_desc {
+    float lambda_begin;
+    float lambda_end;
+    int32_t n;
+    const float* amplitudes; /* n values */
+} rise_hip_spd_desc;
+
+typedef enum rise_hip_material_type {
+    RISE_HIP_MATERIAL_LAMBERTIAN = 0 /* LambertianMaterial (+ LambertianLuminaireMaterial emitter) */
+} rise_hip_material_type;
+
+typedef struct rise_hip_material_desc {
+    int32_t type;            /* rise_hip_material_type */
+    int32_t reflectance_spd; /* index into the SPD list */
+    int32_t emission_spd;    /* -1: not an emitter (LambertianEmitter otherwise) */
+    float emission_scale;    /* LambertianEmitter scale: L = spd / pi * scale */
+    float reserved[4];
+} rise_hip_material_desc;
+
+/* A world-space triangle mesh with one material (an Object with identity
+ * transform over a TriangleMeshGeometryIndexed with face normals). */
+typedef struct rise_hip_object_desc {
+    const float* vertices; /* n_tris * 9 floats */
+    int64_t n_tris;
+    int32_t material;
+    int32_t double_sided;  /* TriangleMeshGeometryIndexed bDoubleSided */
+} rise_hip_object_desc;
+
+typedef struct rise_hip_pt_scene rise_hip_pt_scene;
+
+/* Builds the BVH, material, spectrum and luminary tables and uploads them.
+ * Objects whose material emits become area luminaries (LuminaryManager,
+ * LightSampler::Prepare, Lights/LightSampler.cpp:635-760).  Synchronous. */
+rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* objects, size_t n_objects,
+                                         const rise_hip_material_desc* materials, size_t n_materials,
+                                         const rise_hip_spd_desc* spds, size_t n_spds,
+                                         rise_hip_pt_scene** out_scene);
+
+rise_hip_status rise_hip_pt_scene_release(rise_hip_pt_scene* scene);
+
+/* Pinhole camera: vertical field of view, square pixels, pixel row 0 at the top. */
+typedef struct rise_hip_camera_desc {
+    float position[3];
+    float look_at[3];
+    float up[3];
+    float fov_deg;
+} rise_hip_camera_desc;
+
+typedef struct rise_hip_render_desc {
+    int32_t width;
+    int32_t height;
+    int32_t spp;          /* samples rendered by this call */
+    int32_t spp_offset;   /* index of the first sample (progressive passes) */
+    int32_t spp_total;    /* samples per pixel of the whole render (random-number layout) */
+    float lambda_begin;   /* HWSS range, nm */
+    float lambda_end;
+    int32_t max_diffuse_bounce;      /* StabilityConfig bounce caps (Utilities/StabilityConfig.h:103-112) */
+    int32_t max_glossy_bounce;
+    int32_t max_transmission_bounce;
+    int32_t rr_min_depth;            /* Russian roulette (PathTransportUtilities.h:86-110) */
+    float rr_threshold;
+    uint64_t seed;
+    rise_hip_camera_desc camera;
+} rise_hip_render_desc;
+
+typedef struct rise_hip_render_stats {
+    int64_t extension_rays;  /* closest-hit queries */
+    int64_t shadow_rays;     /* any-hit queries (next-event estimation) */
+    int64_t paths;           /* camera samples */
+    int64_t bounces;         /* extension passes launched */
+} rise_hip_render_stats;
+
+/* Renders samples [spp_offset, spp_offset + spp) of every pixel.
+ * film_xyz: device, width*height*3 floats: per-pixel mean XYZ of those
+ *   samples (the ProgressiveFilm colorSum / alphaSum resolve,
+ *   Rendering/ProgressiveFilm.h:36-60, before the XYZ -> RGB conversion).
+ * film_xyz_sq: device, optional (NULL): per-pixel mean of squared XYZ.
+ * stats: host, optional (NULL); when given the call waits for the stream.
+ * The scene's work buffers are reused between calls: calls on one scene
+ * must not overlap. */
+rise_hip_status rise_hip_render(rise_hip_pt_scene* scene, const rise_hip_render_desc* desc, float* film_xyz,
+                                float* film_xyz_sq, rise_hip_render_stats* stats, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RISE_HIP_RENDER_H */

@@ -1,0 +1,647 @@
+/*
+ * pt_oracle.c -- FP64 restatement of RISE's HWSS spectral path tracer for
+ * Lambertian surfaces and Lambertian area luminaires.  TEST INFRASTRUCTURE
+ * ONLY.  See pt_oracle.h for the reference lines followed and the random
+ * number contract.  Intersections are brute force over all triangles with
+ * the ray-cast oracle's semantics (rt_oracle.c), ties to the lowest id.
+ */
+#include "pt_oracle.h"
+#include "rt_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#define NZ 1e-12
+#define INV_PI 0.31830988618379067154
+#define TWO_PI 6.28318530717958647692
+#define HWSS_N 4
+#define MAX_PATH_DEPTH 128 /* PathTracingIntegrator mMaxPathDepth */
+
+/* ---- random numbers ------------------------------------------------------ */
+
+double rise_oracle_rng(uint64_t seed, uint32_t path, uint32_t dim)
+{
+    uint64_t x = ((((uint64_t)path) << 32) | dim) * 0x9E3779B97F4A7C15ull + seed;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (double)(x >> 40) * (1.0 / 16777216.0);
+}
+
+/* ---- CIE 1931 2-degree colour matching functions, 380-780 nm, 5 nm ------- */
+/* (ColorUtils.cpp:81; the z table is zero past 630 nm there, as here.) */
+static const double CIE_X[81] = {
+    0.0014, 0.0022, 0.0042, 0.0077, 0.0143, 0.0232, 0.0435, 0.0776, 0.1344, 0.2148, 0.2839, 0.3285, 0.3483, 0.3481,
+    0.3362, 0.3187, 0.2908, 0.2511, 0.1954, 0.1421, 0.0956, 0.0580, 0.0320, 0.0147, 0.0049, 0.0024, 0.0093, 0.0291,
+    0.0633, 0.1096, 0.1655, 0.2257, 0.2904, 0.3597, 0.4334, 0.5121, 0.5945, 0.6784, 0.7621, 0.8425, 0.9163, 0.9786,
+    1.0263, 1.0567, 1.0622, 1.0456, 1.0026, 0.9384, 0.8544, 0.7514, 0.6424, 0.5419, 0.4479, 0.3608, 0.2835, 0.2187,
+    0.1649, 0.1212, 0.0874, 0.0636, 0.0468, 0.0329, 0.0227, 0.0158, 0.0114, 0.0081, 0.0058, 0.0041, 0.0029, 0.0020,
+    0.0014, 0.0010, 0.0007, 0.0005, 0.0003, 0.0002, 0.0002, 0.0001, 0.0001, 0.0001, 0.0000};
+static const double CIE_Y[81] = {
+    0, 0.0001, 0.0001, 0.0002, 0.0004, 0.0006, 0.0012, 0.0022, 0.004, 0.0073, 0.0116, 0.0168, 0.023, 0.0298,
+    0.038, 0.048, 0.06, 0.0739, 0.091, 0.1126, 0.139, 0.1693, 0.208, 0.2586, 0.323, 0.4073, 0.503, 0.6082,
+    0.71, 0.7932, 0.862, 0.9149, 0.954, 0.9803, 0.995, 1, 0.995, 0.9786, 0.952, 0.9154, 0.87, 0.8163,
+    0.757, 0.6949, 0.631, 0.5668, 0.503, 0.4412, 0.381, 0.321, 0.265, 0.217, 0.175, 0.1382, 0.107, 0.0816,
+    0.061, 0.0446, 0.032, 0.0232, 0.017, 0.0119, 0.0082, 0.0057, 0.0041, 0.0029, 0.0021, 0.0015, 0.001, 0.0007,
+    0.0005, 0.0004, 0.0002, 0.0002, 0.0001, 0.0001, 0.0001, 0, 0, 0, 0};
+static const double CIE_Z[81] = {
+    0.0065, 0.0105, 0.0201, 0.0362, 0.0679, 0.1102, 0.2074, 0.3713, 0.6456, 1.0391, 1.3856, 1.623, 1.7471, 1.7826,
+    1.7721, 1.7441, 1.6692, 1.5281, 1.2876, 1.0419, 0.813, 0.6162, 0.4652, 0.3533, 0.272, 0.2123, 0.1582, 0.1117,
+    0.0782, 0.0573, 0.0422, 0.0298, 0.0203, 0.0134, 0.0087, 0.0057, 0.0039, 0.0027, 0.0021, 0.0018, 0.0017, 0.0014,
+    0.0011, 0.001, 0.0008, 0.0006, 0.0003, 0.0002, 0.0002, 0.0001, 0};
+
+/* InterpSPDIndices + ApplySPDFunction (ColorUtils.cpp:270-298, ColorUtils.h:52-62). */
+int rise_oracle_xyz_from_nm(double nm, double xyz[3])
+{
+    if (nm < 380.0 || nm > 780.0) return 0;
+    double x = (nm - 380.0) / 5.0;
+    unsigned lo = (unsigned)floor(x);
+    unsigned hi = lo + 1;
+    if (hi > 80) hi = 80;
+    const double w = x - lo;
+    xyz[0] = CIE_X[lo] + (CIE_X[hi] - CIE_X[lo]) * w;
+    xyz[1] = CIE_Y[lo] + (CIE_Y[hi] - CIE_Y[lo]) * w;
+    xyz[2] = CIE_Z[lo] + (CIE_Z[hi] - CIE_Z[lo]) * w;
+    return 1;
+}
+
+double rise_oracle_cie_y_integral(double a, double b)
+{
+    const double lo = a < 380.0 ? 380.0 : a;
+    const double hi = b > 780.0 ? 780.0 : b;
+    if (hi <= lo) return 0.0;
+    const int first = (int)((lo - 380.0) / 5.0);
+    const int last = (int)((hi - 380.0) / 5.0);
+    double s = 0.0;
+    for (int i = first; i < last; ++i) s += 0.5 * (CIE_Y[i] + CIE_Y[i + 1]) * 5.0;
+    return s;
+}
+
+/* SpectralPacket::ValueAtNM (SpectralPacket.h:285-297): step bins. */
+static double spd_value(const ro_spd* s, double nm)
+{
+    if (nm < s->lambda_begin || nm > s->lambda_end) return 0.0;
+    const double delta = (s->lambda_end - s->lambda_begin) / (double)s->n;
+    int idx = (int)((nm - s->lambda_begin) / delta);
+    if (idx >= s->n) idx = s->n - 1; /* nm == lambda_end */
+    return s->amp[idx];
+}
+
+/* ---- small vector helpers ------------------------------------------------ */
+
+typedef struct { double x, y, z; } v3;
+static v3 mk(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 mul(v3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+static v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static double mag(v3 a) { return sqrt(dot(a, a)); }
+static v3 normalize(v3 a)
+{
+    const double m = mag(a);
+    if (m > 0.0) {
+        const double o = 1.0 / m;
+        return mk(a.x * o, a.y * o, a.z * o);
+    }
+    return a;
+}
+
+typedef struct { v3 u, v, w; } onb;
+
+/* OrthonormalBasis3D::CreateFromW (OrthonormalBasis3D.cpp:37-56). */
+static onb onb_from_w(v3 n)
+{
+    onb b;
+    b.w = normalize(n);
+    const v3 cu = mk(1, 0, 0), cv = mk(0, 1, 0), cw = mk(0, 0, 1);
+    if (fabs(dot(b.w, cv) - 1) < NZ) {
+        b.u = cw;
+        b.v = cu;
+    } else if (fabs(dot(b.w, cv) + 1) < NZ) {
+        b.u = cu;
+        b.v = cw;
+    } else {
+        b.u = normalize(cross(b.w, cv));
+        b.v = normalize(cross(b.w, b.u));
+    }
+    return b;
+}
+
+static v3 onb_xform(const onb* b, v3 a)
+{
+    return mk(b->u.x * a.x + b->v.x * a.y + b->w.x * a.z, b->u.y * a.x + b->v.y * a.y + b->w.y * a.z,
+              b->u.z * a.x + b->v.z * a.y + b->w.z * a.z);
+}
+
+static double power_heuristic(double pa, double pb)
+{
+    const double a2 = pa * pa;
+    return a2 / (a2 + pb * pb);
+}
+
+/* ---- scene preparation --------------------------------------------------- */
+
+typedef struct {
+    int32_t object;
+    double area;
+    double weight;
+    int64_t* tris;   /* triangle indices of this luminary */
+    double* cdf;     /* area CDF over those triangles */
+    int64_t n;
+} luminary;
+
+typedef struct {
+    const ro_scene* s;
+    double* v;       /* n_tris * 9 doubles */
+    luminary* lums;
+    int32_t n_lums;
+    double lum_total;
+    int32_t* lum_of_object; /* -1 if not a luminary */
+    double y_norm;
+    /* work distribution */
+    int32_t s0, s1;
+    double* xyz_sum;
+    double* xyz_sq;
+    int64_t next_pixel;
+    pthread_mutex_t lock;
+    uint64_t n_ext, n_shadow, n_paths;
+} prep;
+
+static void tri_verts(const prep* P, int64_t t, v3* a, v3* b, v3* c)
+{
+    const double* f = P->v + 9 * t;
+    *a = mk(f[0], f[1], f[2]);
+    *b = mk(f[3], f[4], f[5]);
+    *c = mk(f[6], f[7], f[8]);
+}
+
+static double tri_area(const prep* P, int64_t t)
+{
+    v3 a, b, c;
+    tri_verts(P, t, &a, &b, &c);
+    return mag(cross(sub(b, a), sub(c, a))) * 0.5;
+}
+
+static int prepare(prep* P, const ro_scene* s)
+{
+    memset(P, 0, sizeof(*P));
+    P->s = s;
+    P->v = (double*)malloc(sizeof(double) * 9 * (size_t)(s->n_tris > 0 ? s->n_tris : 1));
+    for (int64_t i = 0; i < 9 * s->n_tris; ++i) P->v[i] = s->tris[i];
+    P->lum_of_object = (int32_t*)malloc(sizeof(int32_t) * (size_t)(s->n_objects > 0 ? s->n_objects : 1));
+    P->lums = (luminary*)calloc((size_t)(s->n_objects > 0 ? s->n_objects : 1), sizeof(luminary));
+    for (int o = 0; o < s->n_objects; ++o) {
+        P->lum_of_object[o] = -1;
+        const ro_material* m = &s->materials[s->object_material[o]];
+        if (m->emit_spd < 0) continue;
+        luminary L;
+        memset(&L, 0, sizeof(L));
+        L.object = o;
+        int64_t cnt = 0;
+        for (int64_t t = 0; t < s->n_tris; ++t) cnt += s->tri_object[t] == o;
+        if (cnt == 0) continue;
+        L.tris = (int64_t*)malloc(sizeof(int64_t) * (size_t)cnt);
+        L.cdf = (double*)malloc(sizeof(double) * (size_t)cnt);
+        /* ComputeAreas (TriangleMeshGeometryIndexed.cpp:404-442). */
+        double total = 0;
+        for (int64_t t = 0; t < s->n_tris; ++t)
+            if (s->tri_object[t] == o) {
+                L.tris[L.n] = t;
+                const double a = tri_area(P, t);
+                L.cdf[L.n] = a;
+                total += a;
+                L.n++;
+            }
+        if (!(total > 0)) continue;
+        const double inv = 1.0 / total;
+        double sum = 0;
+        for (int64_t k = 0; k < L.n; ++k) {
+            sum += L.cdf[k] * inv;
+            L.cdf[k] = sum;
+        }
+        L.area = total;
+        /* Selection weight: mean exitance of the SPD times scale times area
+         * (LightSampler::Prepare, LightSampler.cpp:666-686, weights by
+         * exitance * area). */
+        const ro_spd* e = &s->spds[m->emit_spd];
+        double mean = 0;
+        for (int k = 0; k < e->n; ++k) mean += e->amp[k];
+        mean /= (double)e->n;
+        L.weight = mean * m->emit_scale * total;
+        if (!(L.weight > 0)) continue;
+        P->lum_of_object[o] = P->n_lums;
+        P->lums[P->n_lums++] = L;
+        P->lum_total += L.weight;
+    }
+    const double ky = rise_oracle_cie_y_integral(s->lambda_begin, s->lambda_end);
+    P->y_norm = ky > NZ ? (s->lambda_end - s->lambda_begin) / ky : 1.0;
+    return 0;
+}
+
+static void release(prep* P)
+{
+    for (int i = 0; i < P->n_lums; ++i) {
+        free(P->lums[i].tris);
+        free(P->lums[i].cdf);
+    }
+    free(P->lums);
+    free(P->lum_of_object);
+    free(P->v);
+}
+
+/* ---- ray queries (rt_oracle semantics, inlined for speed) ---------------- */
+
+/* Closest hit for a unit-direction ray; returns tri or -1 and the MT t. */
+static int64_t closest(const prep* P, v3 o, v3 dn, double* t_out)
+{
+    double best = RISE_ORACLE_INFINITY;
+    int64_t id = -1;
+    const double oo[3] = {o.x, o.y, o.z}, dd[3] = {dn.x, dn.y, dn.z};
+    for (int64_t k = 0; k < P->s->n_tris; ++k) {
+        const double* f = P->v + 9 * k;
+        const double eA[3] = {f[3] - f[0], f[4] - f[1], f[5] - f[2]};
+        const double eB[3] = {f[6] - f[0], f[7] - f[1], f[8] - f[2]};
+        double t;
+        if (rise_oracle_ray_triangle(oo, dd, f, eA, eB, &t, NULL, NULL) && t < best) {
+            best = t;
+            id = k;
+        }
+    }
+    *t_out = best;
+    return id;
+}
+
+/* Any hit in (NEARZERO, dHowFar) after the Object layer's normalization
+ * (Object.cpp:652-712). */
+static int occluded(const prep* P, v3 o, v3 d, double how_far)
+{
+    const v3 dn = normalize(d);
+    const double m = mag(d);
+    const double factor = (m > NZ) ? m : 1.0;
+    double how_far2 = how_far;
+    if ((how_far != RISE_ORACLE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
+    const double oo[3] = {o.x, o.y, o.z}, dd[3] = {dn.x, dn.y, dn.z};
+    for (int64_t k = 0; k < P->s->n_tris; ++k) {
+        const double* f = P->v + 9 * k;
+        const double eA[3] = {f[3] - f[0], f[4] - f[1], f[5] - f[2]};
+        const double eB[3] = {f[6] - f[0], f[7] - f[1], f[8] - f[2]};
+        double t;
+        if (rise_oracle_ray_triangle(oo, dd, f, eA, eB, &t, NULL, NULL) && t > NZ && t < how_far2) return 1;
+    }
+    return 0;
+}
+
+/* ---- shading ------------------------------------------------------------- */
+
+typedef struct {
+    v3 p;        /* ptIntersection */
+    v3 dir;      /* incoming ray direction (ri.ray.Dir()) */
+    v3 origin;   /* incoming ray origin */
+    v3 n;        /* vNormal (shading) */
+    v3 ng;       /* vGeomNormal */
+    onb frame;
+    int32_t object;
+    const ro_material* mat;
+} hit;
+
+static void make_hit(const prep* P, v3 o, v3 dn, int64_t tri, double t, hit* h)
+{
+    const ro_scene* s = P->s;
+    v3 a, b, c;
+    tri_verts(P, tri, &a, &b, &c);
+    const v3 fn = cross(sub(b, a), sub(c, a));
+    h->object = s->tri_object[tri];
+    h->mat = &s->materials[s->object_material[h->object]];
+    h->dir = dn;
+    h->origin = o;
+    /* Object.cpp:641-643 */
+    h->p = add(o, mul(dn, t - 1e-12));
+    /* Face normal as both shading and geometric normal
+     * (...Specializations.h:194-213), normalized by Object.cpp:543-549. */
+    h->n = normalize(fn);
+    h->ng = normalize(fn);
+    if (s->object_double_sided[h->object]) {
+        /* TriangleMeshGeometryIndexed.cpp:171-188 */
+        if (dot(h->n, dn) > 0) h->n = neg(h->n);
+        if (dot(h->ng, dn) > 0) h->ng = neg(h->ng);
+    }
+    h->frame = onb_from_w(h->n);
+}
+
+/* Geometric normal oriented against the incoming ray (LambertianBRDF.cpp:51-53). */
+static v3 geom_against_ray(const hit* h, v3 fallback)
+{
+    const v3 raw = (dot(h->ng, h->ng) > 1e-12) ? h->ng : fallback;
+    return (dot(raw, h->dir) < 0) ? raw : neg(raw);
+}
+
+/* ShouldReflect (LambertianBRDF.cpp:44-64). */
+static int should_reflect(const hit* h, v3 vin)
+{
+    const v3 n = h->frame.w;
+    const v3 v = normalize(vin);
+    const v3 r = normalize(neg(h->dir));
+    const double nr = dot(n, r), nv = dot(n, v);
+    const v3 g = geom_against_ray(h, n);
+    if ((nr <= -NZ && nv <= -NZ) || (nr >= NZ && nv >= NZ)) return dot(v, g) > 0 && dot(r, g) > 0;
+    return 0;
+}
+
+static double lambert_value(const prep* P, const hit* h, v3 vin, double nm)
+{
+    if (!should_reflect(h, vin)) return 0.0;
+    return spd_value(&P->s->spds[h->mat->reflect_spd], nm) * INV_PI;
+}
+
+/* LambertianSPF::Pdf (LambertianSPF.cpp:147-163). */
+static double lambert_pdf(const hit* h, v3 wo)
+{
+    const int front = dot(h->dir, h->frame.w) <= NZ;
+    const double c = front ? dot(wo, h->frame.w) : -dot(wo, h->frame.w);
+    const v3 neff = front ? h->frame.w : neg(h->frame.w);
+    const v3 g = geom_against_ray(h, neff);
+    return (c > 0 && dot(wo, g) > 0) ? c * INV_PI : 0.0;
+}
+
+/* LambertianEmitter::emittedRadianceNM (LambertianEmitter.cpp:82-89). */
+static double emitted(const prep* P, const ro_material* m, v3 out, v3 N, double nm)
+{
+    if (dot(out, N) <= 0) return 0.0;
+    return spd_value(&P->s->spds[m->emit_spd], nm) * INV_PI * m->emit_scale;
+}
+
+/* Luminary selection by CDF inversion over the weights; pdf = w / total. */
+static int select_luminary(const prep* P, double u, double* pdf)
+{
+    double acc = 0;
+    for (int i = 0; i < P->n_lums; ++i) {
+        acc += P->lums[i].weight;
+        if (u * P->lum_total < acc || i == P->n_lums - 1) {
+            *pdf = P->lums[i].weight / P->lum_total;
+            return i;
+        }
+    }
+    *pdf = 0;
+    return -1;
+}
+
+static double lum_pdf_select(const prep* P, int lum)
+{
+    return P->lums[lum].weight / P->lum_total;
+}
+
+/* EvaluateDirectLightingNM for area luminaries (LightSampler.cpp:1987-2269). */
+static double direct_nm(prep* P, const hit* h, double nm, uint64_t seed, uint32_t path, uint32_t dim0,
+                        uint64_t* n_shadow)
+{
+    if (P->n_lums == 0) return 0.0;
+    const int self = P->lum_of_object[h->object];
+    double pdf_alias;
+    const int li = select_luminary(P, rise_oracle_rng(seed, path, dim0), &pdf_alias);
+    if (li < 0 || li == self) return 0.0;
+    const luminary* L = &P->lums[li];
+    const double ux = rise_oracle_rng(seed, path, dim0 + 1);
+    const double uy = rise_oracle_rng(seed, path, dim0 + 2);
+    const double uz = rise_oracle_rng(seed, path, dim0 + 3);
+    /* UniformRandomPoint (TriangleMeshGeometryIndexed.cpp:207-218): first
+     * CDF entry >= uz, then PointOnTriangle (GeometricUtilities.cpp:543-560). */
+    int64_t k = 0;
+    while (k < L->n - 1 && L->cdf[k] < uz) ++k;
+    v3 a, b, c;
+    tri_verts(P, L->tris[k], &a, &b, &c);
+    const double sa = sqrt(1.0 - ux);
+    const double alpha = 1.0 - sa, beta = sa * uy, gamma = 1.0 - alpha - beta;
+    const v3 pl = mk(alpha * b.x + beta * c.x + gamma * a.x, alpha * b.y + beta * c.y + gamma * a.y,
+                     alpha * b.z + beta * c.z + gamma * a.z);
+    const v3 ln = normalize(cross(sub(b, a), sub(c, a)));
+    v3 to = sub(pl, h->p);
+    const double dist = mag(to);
+    if (dist > 0.0) to = mul(to, 1.0 / dist);
+    const double cos_s = dot(to, h->n);
+    const double cos_l = dot(neg(to), ln);
+    if (cos_l <= 0 || cos_s <= 0) return 0.0;
+    ++*n_shadow;
+    if (occluded(P, h->p, to, dist - 0.001)) return 0.0;
+    const ro_material* lm = &P->s->materials[P->s->object_material[L->object]];
+    const double le = emitted(P, lm, neg(to), ln, nm);
+    const double geom = L->area * cos_l / (dist * dist);
+    double contrib = le * cos_s * geom * lambert_value(P, h, to, nm);
+    if (L->area > 0 && cos_l > 0) {
+        const double p_light = pdf_alias * (dist * dist) / (L->area * cos_l);
+        const double p_bsdf = lambert_pdf(h, to);
+        if (p_bsdf > 0) contrib *= power_heuristic(p_light, p_bsdf);
+    }
+    return contrib / pdf_alias;
+}
+
+/* One camera sample: IntegratePixelSpectral + IntegrateRayHWSS +
+ * IntegrateFromHitHWSS.  Returns the sample XYZ. */
+static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_out[3])
+{
+    const ro_scene* s = P->s;
+    const uint64_t seed = s->seed;
+    const uint32_t path = (uint32_t)(((uint64_t)py * s->width + px) * (uint64_t)s->spp_total + (uint64_t)si);
+    /* Camera (stream 0): jitter, then hero wavelength. */
+    const double jx = rise_oracle_rng(seed, path, 0), jy = rise_oracle_rng(seed, path, 1);
+    const double ul = rise_oracle_rng(seed, path, 2);
+    const v3 cp = mk(s->cam_pos[0], s->cam_pos[1], s->cam_pos[2]);
+    const v3 f = normalize(sub(mk(s->cam_look[0], s->cam_look[1], s->cam_look[2]), cp));
+    const v3 r = normalize(cross(f, mk(s->cam_up[0], s->cam_up[1], s->cam_up[2])));
+    const v3 u = cross(r, f);
+    const double hh = 2.0 * tan(s->cam_fov_deg * (M_PI / 180.0) * 0.5);
+    const double sx = px + jx, sy = py + jy;
+    const double cx = (sx - 0.5 * s->width) * hh / s->height;
+    const double cy = -(sy - 0.5 * s->height) * hh / s->height;
+    v3 dir = normalize(add(add(f, mul(r, cx)), mul(u, cy)));
+    v3 org = cp;
+
+    /* SampledWavelengths::SampleEquidistant (SampledWavelengths.h:79-106). */
+    const double a = s->lambda_begin, b = s->lambda_end, range = b - a, delta = range / HWSS_N;
+    double lam[HWSS_N];
+    lam[0] = a + ul * range;
+    for (int i = 1; i < HWSS_N; ++i) lam[i] = a + fmod(lam[0] - a + i * delta, range);
+
+    double result[HWSS_N] = {0, 0, 0, 0};
+    double thr[HWSS_N] = {1, 1, 1, 1};
+    double bsdf_pdf = 0.0;
+    int diffuse = 0;
+    uint64_t n_ext = 0, n_shadow = 0;
+
+    for (int depth = 0; depth < MAX_PATH_DEPTH; ++depth) {
+        /* Runaway guard (PathTracingIntegrator.cpp:4096-4110). */
+        double mx = 0;
+        int bad = 0;
+        for (int w = 0; w < HWSS_N; ++w) {
+            if (!isfinite(thr[w])) bad = 1;
+            if (fabs(thr[w]) > mx) mx = fabs(thr[w]);
+        }
+        if (bad || mx > 1e6) break;
+        const uint32_t base = 32u * (uint32_t)(16 + depth);
+
+        double t;
+        ++n_ext;
+        const int64_t tri = closest(P, org, dir, &t);
+        if (tri < 0) break; /* no environment in these scenes */
+        hit h;
+        make_hit(P, org, dir, tri, t, &h);
+
+        /* Emission with MIS (PathTracingIntegrator.cpp:4422-4515). */
+        if (h.mat->emit_spd >= 0) {
+            for (int w = 0; w < HWSS_N; ++w) {
+                double e = emitted(P, h.mat, neg(h.dir), h.ng, lam[w]);
+                const int li = P->lum_of_object[h.object];
+                if (bsdf_pdf > 0 && li >= 0) {
+                    const double area = P->lums[li].area;
+                    const double cos_l = fabs(dot(h.dir, h.ng));
+                    if (area > 0 && cos_l > 0) {
+                        const double dist = mag(sub(h.p, h.origin));
+                        double ps = lum_pdf_select(P, li);
+                        if (ps <= 0) ps = 1.0;
+                        const double p_nee = ps * (dist * dist) / (area * cos_l);
+                        e *= power_heuristic(bsdf_pdf, p_nee);
+                    }
+                }
+                result[w] += thr[w] * e;
+            }
+        }
+        /* Next-event estimation, one light sample per wavelength
+         * (PathTracingIntegrator.cpp:4521-4552). */
+        for (int w = 0; w < HWSS_N; ++w)
+            result[w] += thr[w] * direct_nm(P, &h, lam[w], seed, path, base + 4u * (uint32_t)w, &n_shadow);
+
+        /* LambertianSPF::ScatterNM at the hero wavelength (LambertianSPF.cpp:90-131). */
+        const double ua = rise_oracle_rng(seed, path, base + 16), ub = rise_oracle_rng(seed, path, base + 17);
+        onb fr = h.frame;
+        const int back = dot(h.dir, h.frame.w) > NZ;
+        if (back) {
+            fr.w = neg(fr.w);
+            fr.u = neg(fr.u);
+        }
+        const double phi = TWO_PI * ua, st = sqrt(ub), ct = sqrt(1.0 - ub);
+        const v3 nd = onb_xform(&fr, mk(cos(phi) * st, sin(phi) * st, ct));
+        const double pdf = fabs(dot(nd, h.frame.w)) * INV_PI;
+        const v3 neff = back ? neg(h.frame.w) : h.frame.w;
+        const v3 g = geom_against_ray(&h, neff);
+        if (!(dot(nd, g) > 0)) break;
+        double sc[HWSS_N];
+        sc[0] = spd_value(&s->spds[h.mat->reflect_spd], lam[0]);
+        for (int w = 1; w < HWSS_N; ++w) {
+            /* Companion: BSDF value * |cos| / pdf (PathTracingIntegrator.cpp:4693-4716). */
+            double cw = lambert_value(P, &h, nd, lam[w]);
+            cw *= fabs(dot(nd, h.n));
+            if (pdf > 0) cw /= pdf;
+            sc[w] = cw > 0 ? cw : 0;
+        }
+        /* Russian roulette over the max across wavelengths (:4721-4768). */
+        double mprev = fabs(thr[0]), mcurr = fabs(thr[0] * sc[0]);
+        for (int w = 1; w < HWSS_N; ++w) {
+            if (fabs(thr[w]) > mprev) mprev = fabs(thr[w]);
+            if (fabs(thr[w] * sc[w]) > mcurr) mcurr = fabs(thr[w] * sc[w]);
+        }
+        if (mcurr <= NZ) break;
+        if (depth >= s->rr_min_depth) {
+            const double thrs = mprev > s->rr_threshold ? mprev : s->rr_threshold;
+            double p = mcurr / thrs;
+            if (p > 1.0) p = 1.0;
+            if (rise_oracle_rng(seed, path, base + 19) >= p) break;
+            if (p > 0 && p < 1.0)
+                for (int w = 0; w < HWSS_N; ++w) sc[w] *= 1.0 / p;
+        }
+        /* Bounce limit (PathTransportUtilities.h:170-210). */
+        if (++diffuse > s->max_diffuse_bounce) break;
+        for (int w = 0; w < HWSS_N; ++w) thr[w] *= sc[w];
+        bsdf_pdf = pdf;
+        /* currentRay = scattered ray from ptIntersection; Advance(1e-8). */
+        org = add(h.p, mul(nd, 1e-8));
+        dir = nd;
+    }
+
+    double acc[3] = {0, 0, 0};
+    for (int w = 0; w < HWSS_N; ++w) {
+        double c[3];
+        if (rise_oracle_xyz_from_nm(lam[w], c)) {
+            acc[0] += c[0] * result[w];
+            acc[1] += c[1] * result[w];
+            acc[2] += c[2] * result[w];
+        }
+    }
+    const double k = P->y_norm / HWSS_N;
+    xyz_out[0] = acc[0] * k;
+    xyz_out[1] = acc[1] * k;
+    xyz_out[2] = acc[2] * k;
+    pthread_mutex_lock(&P->lock);
+    P->n_ext += n_ext;
+    P->n_shadow += n_shadow;
+    P->n_paths += 1;
+    pthread_mutex_unlock(&P->lock);
+}
+
+static void* render_worker(void* arg)
+{
+    prep* P = (prep*)arg;
+    const ro_scene* s = P->s;
+    const int64_t npx = (int64_t)s->width * s->height;
+    for (;;) {
+        pthread_mutex_lock(&P->lock);
+        const int64_t i = P->next_pixel++;
+        pthread_mutex_unlock(&P->lock);
+        if (i >= npx) break;
+        const int32_t px = (int32_t)(i % s->width), py = (int32_t)(i / s->width);
+        double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+        for (int32_t si = P->s0; si < P->s1; ++si) {
+            double x[3];
+            sample_xyz(P, px, py, si, x);
+            for (int c = 0; c < 3; ++c) {
+                sum[c] += x[c];
+                sq[c] += x[c] * x[c];
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            P->xyz_sum[3 * i + c] = sum[c];
+            if (P->xyz_sq) P->xyz_sq[3 * i + c] = sq[c];
+        }
+    }
+    return NULL;
+}
+
+int rise_oracle_render(const ro_scene* s, int32_t s0, int32_t s1, double* xyz_sum, double* xyz_sq,
+                       uint64_t* counters, int n_threads)
+{
+    if (!s || !xyz_sum || s->width <= 0 || s->height <= 0 || s0 < 0 || s1 < s0 || s1 > s->spp_total) return -1;
+    for (int o = 0; o < s->n_objects; ++o) {
+        const int32_t mi = s->object_material[o];
+        if (mi < 0 || mi >= s->n_materials) return -1;
+        const ro_material* m = &s->materials[mi];
+        if (m->type != 0 || m->reflect_spd < 0 || m->reflect_spd >= s->n_spds || m->emit_spd >= s->n_spds) return -1;
+    }
+    prep P;
+    prepare(&P, s);
+    P.s0 = s0;
+    P.s1 = s1;
+    P.xyz_sum = xyz_sum;
+    P.xyz_sq = xyz_sq;
+    pthread_mutex_init(&P.lock, NULL);
+    if (n_threads <= 0) {
+        const long c = sysconf(_SC_NPROCESSORS_ONLN);
+        n_threads = c > 0 ? (int)c : 1;
+    }
+    if (n_threads > 256) n_threads = 256;
+    pthread_t th[256];
+    for (int i = 0; i < n_threads; ++i) pthread_create(&th[i], NULL, render_worker, &P);
+    for (int i = 0; i < n_threads; ++i) pthread_join(th[i], NULL);
+    if (counters) {
+        counters[0] = P.n_ext;
+        counters[1] = P.n_shadow;
+        counters[2] = P.n_paths;
+    }
+    pthread_mutex_destroy(&P.lock);
+    release(&P);
+    return 0;
+}

@@ -1,0 +1,93 @@
+/*
+ * pt_oracle.h -- FP64 CPU oracle for RISE's hero-wavelength spectral path
+ * tracer (TEST INFRASTRUCTURE ONLY: tests/, smoke() and bench.py's CPU
+ * baseline leg use it as the checker or as the timed CPU port; the product
+ * path never links it).
+ *
+ * Restates, for Lambertian surfaces and Lambertian area luminaires (paths
+ * relative to the reference's src/Library/):
+ *   Rendering/PathTracingSpectralRasterizer.cpp:121-198, 206-479  pixel/sample loop, XYZ film
+ *   Shaders/PathTracingIntegrator.cpp:3939-4822, 4855-5126          IntegrateRayHWSS / IntegrateFromHitHWSS
+ *   Lights/LightSampler.cpp:1942-2362                               EvaluateDirectLightingNM (area luminaries)
+ *   Materials/LambertianBRDF.cpp:44-93, LambertianSPF.cpp:90-172    BRDF, sampling, pdf
+ *   Materials/LambertianEmitter.cpp:82-89                           one-sided emission
+ *   Utilities/PathTransportUtilities.h:86-210, 251-258              RR, bounce limits, power heuristic
+ *   Utilities/Color/SampledWavelengths.h:79-106                     equidistant hero wavelengths
+ *   Utilities/Color/ColorUtils.cpp:270-360                          XYZFromNM, CIE_Y_Integral
+ *   Utilities/Color/SpectralPacket.h:285-297                        step-SPD lookup
+ *   Utilities/GeometricUtilities.cpp:85-98, 543-568                 cosine sampling, point on triangle
+ *   Utilities/OrthonormalBasis3D.cpp:37-56, 138-144                 ONB from W
+ *   Objects/Object.cpp:641-643                                      hit point back-off
+ * Random numbers: the reference draws per-dimension samples from an
+ * Owen-scrambled Sobol sequence (stream 16 + depth, 32 dimensions per
+ * stream) and hero wavelength / pixel jitter from a Mersenne Twister; it is
+ * not reproducible run to run.  The oracle instead draws every number from
+ * a counter-based hash of (seed, path, dimension) with the same stream
+ * layout, the same function the HIP renderer uses, so both follow the same
+ * random decisions and their per-sample results can be compared directly.
+ */
+#ifndef RISE_PT_ORACLE_H
+#define RISE_PT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ro_spd {
+    double lambda_begin, lambda_end; /* SpectralPacket range */
+    int32_t n;                       /* number of step bins */
+    const double* amp;               /* n amplitudes */
+} ro_spd;
+
+typedef struct ro_material {
+    int32_t type;          /* 0 = Lambertian */
+    int32_t reflect_spd;   /* reflectance SPD index */
+    int32_t emit_spd;      /* -1: not an emitter */
+    double emit_scale;     /* LambertianEmitter scale */
+} ro_material;
+
+typedef struct ro_scene {
+    const float* tris;          /* n_tris * 9 */
+    int64_t n_tris;
+    const int32_t* tri_object;  /* object index per triangle */
+    int32_t n_objects;
+    const int32_t* object_material;
+    const uint8_t* object_double_sided;
+    const ro_material* materials;
+    int32_t n_materials;
+    const ro_spd* spds;
+    int32_t n_spds;
+    double cam_pos[3], cam_look[3], cam_up[3], cam_fov_deg;
+    int32_t width, height;
+    int32_t spp_total;          /* samples per pixel of the whole render (path id layout) */
+    double lambda_begin, lambda_end;
+    int32_t max_diffuse_bounce, max_glossy_bounce, max_transmission_bounce;
+    int32_t rr_min_depth;
+    double rr_threshold;
+    uint64_t seed;
+} ro_scene;
+
+/* Counter-based uniform in [0,1) with 24 random bits (exactly representable
+ * in float): splitmix64 finalizer of key * golden + seed, key = path << 32 | dim. */
+double rise_oracle_rng(uint64_t seed, uint32_t path, uint32_t dim);
+
+/* Renders samples [s0, s1) of every pixel.  xyz_sum (W*H*3) receives the
+ * sum of per-sample XYZ, xyz_sq (optional) the sum of squares; counters
+ * (optional, 3 uint64) extension rays, shadow rays, paths.  n_threads <= 0:
+ * all online CPUs.  Returns 0 on success, -1 on invalid input. */
+int rise_oracle_render(const ro_scene* scene, int32_t s0, int32_t s1, double* xyz_sum, double* xyz_sq,
+                       uint64_t* counters, int n_threads);
+
+/* XYZFromNM (ColorUtils.cpp:313-325); returns 0 outside 380-780 nm. */
+int rise_oracle_xyz_from_nm(double nm, double xyz[3]);
+
+/* CIE_Y_Integral (ColorUtils.cpp:327-350). */
+double rise_oracle_cie_y_integral(double a, double b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -78,7 +78,8 @@ extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t 
                 try {
                     if (mode == 0) {
                         double best;
-                        const int id = trace_closest<true>(r, nodes, trs, st, best, nn, nt);
+                        unsigned leaf;
+                        const int id = trace_closest<true>(r, nodes, trs, st, best, leaf, nn, nt);
                         prim_id[i] = id;
                         t[i] = id >= 0 ? (float)world_range(r, best) : INFINITY;
                     } else {

@@ -211,15 +211,17 @@ struct PrivStack {
     RISE_HD inline bool empty() const { return sp == 0; }
 };
 
-// Closest-hit traversal.  Returns the caller id (-1 on miss) and the best
-// parametric distance along dn in best_t.
+// Closest-hit traversal.  Returns the caller id (-1 on miss), the best
+// parametric distance along dn in best_t and the leaf-order triangle index
+// of the hit in best_leaf.
 template <bool COUNT, class Stack>
 RISE_HD inline int trace_closest(const RayState& r, const Node8* __restrict__ nodes,
                                     const Tri48* __restrict__ tris, Stack& st, double& best_t,
-                                    unsigned& n_nodes, unsigned& n_tris)
+                                    unsigned& best_leaf, unsigned& n_nodes, unsigned& n_tris)
 {
     best_t = kRiseInfinity;
     int best_id = -1;
+    best_leaf = 0xffffffffu;
     float tlim = INFINITY;
     unsigned gbase = 0, ghits = 1u << (0u ^ r.oinv), gimask = 1u;
     for (;;) {
@@ -248,6 +250,7 @@ RISE_HD inline int trace_closest(const RayState& r, const Node8* __restrict__ no
                 if (tt < best_t || (tt == best_t && id < best_id)) {
                     best_t = tt;
                     best_id = id;
+                    best_leaf = tbase + j;
                     tlim = box_limit(r, best_t);
                 }
             }

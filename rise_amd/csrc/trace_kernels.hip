@@ -51,14 +51,15 @@ __global__ __launch_bounds__(kBlock) void closest_kernel(const Node8* __restrict
     init_ray(r, ox, oy, oz, dx, dy, dz);
     unsigned nn = 0, nt = 0;
     double best;
+    unsigned leaf;
     int id;
     if (LDS) {
         LdsStack<kLdsStack> st{lds + threadIdx.x, kBlock, 0};
-        id = trace_closest<COUNT>(r, nodes, tris, st, best, nn, nt);
+        id = trace_closest<COUNT>(r, nodes, tris, st, best, leaf, nn, nt);
     } else {
         PrivStack st;
         st.sp = 0;
-        id = trace_closest<COUNT>(r, nodes, tris, st, best, nn, nt);
+        id = trace_closest<COUNT>(r, nodes, tris, st, best, leaf, nn, nt);
     }
     prim_id[i] = id;
     t_out[i] = id >= 0 ? (float)world_range(r, best) : INFINITY;

@@ -146,8 +146,19 @@ def test_gpu_full_ray_cast_workload(cuda):
         log_case("full workload vs reference BVH4 (oracle/_ref)")
         rp, rt, rt64 = oracle.RefMesh(tris, "ieee").closest(rays, threads=16)
         bad = np.nonzero(rp != pid)[0]
-        assert bad.size == 0, f"{bad.size} id mismatches vs reference, first {bad[:5]}"
-        rel = np.abs(t[hit].astype(np.float64) - rt64[hit]) / rt64[hit]
+        # The reference's FP32 pre-filter (BVH.h:1111-1167, EPS 1e-5) is not
+        # conservative for sliver triangles: about one ray in 10^6 loses its
+        # closest hit when that hit lies within ~1e-4 of an edge in
+        # barycentrics.  Every such ray must be one where the brute-force
+        # FP64 oracle agrees with the GPU and the reference reported a farther
+        # hit or none.
+        assert bad.size <= 20, f"{bad.size} id mismatches vs reference"
+        if bad.size:
+            bp, _, bt64 = oracle.closest(tris, np.ascontiguousarray(rays[:, bad]))
+            assert np.array_equal(bp, pid[bad])
+            assert np.all((rp[bad] < 0) | (rt64[bad] > bt64))
+        ok = hit & (rp == pid)
+        rel = np.abs(t[ok].astype(np.float64) - rt64[ok]) / rt64[ok]
         assert rel.max() <= 1e-5
     # Size-independent shadow-ray property: nothing lies closer than the
     # closest hit, and the closest hit occludes a limit just beyond it.
