@@ -71,6 +71,82 @@ def load_traffic():
         return None
 
 
+def render_leg(args, cpu_enabled: bool):
+    """Cornell-box spectral path tracing: 256x256, 16 spp, 4 hero wavelengths,
+    diffuse depth 5, Russian roulette from depth 3.  Rays = extension + shadow
+    queries actually traced."""
+    import numpy as np
+    import torch
+
+    from rise_amd import pt_scene
+    from rise_amd.render import PathTracingSpectralRasterizer
+
+    scene, rs = pt_scene.cornell_box()
+    pt = PathTracingSpectralRasterizer(scene)
+    film, _, stats = pt.rasterize_scene(rs)  # warmup + ray counts
+    torch.cuda.synchronize()
+    rays = stats["extension_rays"] + stats["shadow_rays"]
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    lat = []
+    for _ in range(args.render_steps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        pt.rasterize_scene(rs, film=film, stats=False)
+        b.record()
+        torch.cuda.synchronize()
+        lat.append(a.elapsed_time(b))
+    start.record()
+    for _ in range(args.render_steps):
+        pt.rasterize_scene(rs, film=film, stats=False)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.render_steps
+    out = {
+        "workload": "Cornell box (36 tris), 256x256, 16 spp, HWSS 4 wavelengths 380-720 nm, diffuse depth 5, RR from 3",
+        "ms_per_render": round(ms, 3),
+        "mrays_per_s": round(rays / (ms / 1e3) / 1e6, 2),
+        "paths_per_s": round(stats["paths"] / (ms / 1e3), 1),
+        "renders_per_s": round(1e3 / ms, 2),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "rays_per_render": {"extension": stats["extension_rays"], "shadow": stats["shadow_rays"]},
+        "cpu_baseline": None,
+        "parity": None,
+    }
+    log(f"render: {ms:.2f} ms, {out['mrays_per_s']} Mrays/s, {out['paths_per_s']:.3g} paths/s")
+    if cpu_enabled:
+        import oracle
+
+        th = cpu_threads()
+        t0 = time.perf_counter()
+        sm, _, cnt = oracle.render(scene, rs, threads=th)
+        sec = time.perf_counter() - t0
+        cpu_rays = int(cnt[0] + cnt[1])
+        out["cpu_baseline"] = {
+            "value": round(cpu_rays / sec / 1e6, 4),
+            "unit": "Mrays/s",
+            "cores": th,
+            "kind": "port",
+            "sample": f"the whole render (65,536 pixels x 16 spp) with the FP64 C oracle, brute-force "
+            f"intersection over the 36 triangles; {sec:.1f} s",
+        }
+        g = film.cpu().numpy().astype(np.float64)
+        o = sm / rs.spp
+        floor = 0.01 * np.abs(o[..., 1]).mean()
+        rel = np.abs(g - o) / (np.abs(o) + floor)
+        out["parity"] = {
+            "against": "FP64 oracle on the same random streams",
+            "pixels_within_1e-3": round(float(np.mean(np.all(rel <= 1e-3, axis=-1))), 5),
+            "image_mean_Y_rel_diff": float(abs(g[..., 1].mean() / o[..., 1].mean() - 1.0)),
+            "extension_rays_rel_diff": float(abs(stats["extension_rays"] / cnt[0] - 1.0)),
+        }
+        log(f"render CPU port: {out['cpu_baseline']['value']} Mrays/s on {th} threads; parity {out['parity']}")
+    pt.release()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +155,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only the 1 GiB copy kernel")
+    ap.add_argument("--no-render", action="store_true", help="skip the Cornell-box render leg")
+    ap.add_argument("--render-steps", type=int, default=5)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -192,6 +270,8 @@ def main() -> int:
         else:
             log("oracle/_ref not built: no CPU baseline")
 
+    render = None if args.no_render else render_leg(args, cpu_enabled=not args.no_cpu_baseline)
+
     traffic = load_traffic()
     line = {
         "metric": METRIC,
@@ -228,6 +308,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity": parity,
+        "render": render,
     }
     om.release()
     print(json.dumps(line), flush=True)

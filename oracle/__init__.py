@@ -141,3 +141,111 @@ class RefMesh:
 
 def ref_available(variant: str = "ieee") -> bool:
     return (REF_DIR / ("libriseref_ieee.so" if variant == "ieee" else "libriseref.so")).exists()
+
+
+# ---- path-tracing oracle (pt_oracle.c) -------------------------------------
+
+
+class _RoSpd(ctypes.Structure):
+    _fields_ = [("lambda_begin", ctypes.c_double), ("lambda_end", ctypes.c_double), ("n", ctypes.c_int32),
+                ("amp", ctypes.c_void_p)]
+
+
+class _RoMaterial(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("reflect_spd", ctypes.c_int32), ("emit_spd", ctypes.c_int32),
+                ("emit_scale", ctypes.c_double)]
+
+
+class _RoScene(ctypes.Structure):
+    _fields_ = [
+        ("tris", ctypes.c_void_p), ("n_tris", ctypes.c_int64), ("tri_object", ctypes.c_void_p),
+        ("n_objects", ctypes.c_int32), ("object_material", ctypes.c_void_p),
+        ("object_double_sided", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_int32),
+        ("spds", ctypes.c_void_p), ("n_spds", ctypes.c_int32),
+        ("cam_pos", ctypes.c_double * 3), ("cam_look", ctypes.c_double * 3), ("cam_up", ctypes.c_double * 3),
+        ("cam_fov_deg", ctypes.c_double), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+        ("spp_total", ctypes.c_int32), ("lambda_begin", ctypes.c_double), ("lambda_end", ctypes.c_double),
+        ("max_diffuse_bounce", ctypes.c_int32), ("max_glossy_bounce", ctypes.c_int32),
+        ("max_transmission_bounce", ctypes.c_int32), ("rr_min_depth", ctypes.c_int32),
+        ("rr_threshold", ctypes.c_double), ("seed", ctypes.c_uint64),
+    ]
+
+
+def render(scene, settings, s0: int = 0, s1: int | None = None, spp_total: int | None = None, threads: int = 0):
+    """FP64 oracle render of samples [s0, s1).  Returns (sum XYZ (H, W, 3),
+    sum of squared XYZ (H, W, 3), counters [extension rays, shadow rays, paths])."""
+    L = oracle_lib()
+    L.rise_oracle_render.argtypes = [ctypes.POINTER(_RoScene), ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
+                                     ctypes.c_int]
+    L.rise_oracle_render.restype = ctypes.c_int
+    s1 = settings.spp if s1 is None else s1
+    keep = []
+    tris, tobj = scene.flat()
+    keep += [tris, tobj]
+    spds = (_RoSpd * len(scene.spds))()
+    for k, s in enumerate(scene.spds):
+        a = np.ascontiguousarray(s.amp, np.float64)
+        keep.append(a)
+        spds[k] = _RoSpd(float(np.float32(s.begin)), float(np.float32(s.end)), a.size, a.ctypes.data)
+    mats = (_RoMaterial * len(scene.materials))()
+    for k, m in enumerate(scene.materials):
+        mats[k] = _RoMaterial(m.kind, m.reflectance, m.emission, float(np.float32(m.emission_scale)))
+    om = np.array([o.material for o in scene.objects], np.int32)
+    ods = np.array([1 if o.double_sided else 0 for o in scene.objects], np.uint8)
+    keep += [om, ods]
+    rs = _RoScene()
+    rs.tris = tris.ctypes.data
+    rs.n_tris = tris.shape[0]
+    rs.tri_object = tobj.ctypes.data
+    rs.n_objects = len(scene.objects)
+    rs.object_material = om.ctypes.data
+    rs.object_double_sided = ods.ctypes.data
+    rs.materials = ctypes.cast(mats, ctypes.c_void_p)
+    rs.n_materials = len(scene.materials)
+    rs.spds = ctypes.cast(spds, ctypes.c_void_p)
+    rs.n_spds = len(scene.spds)
+    c = scene.camera
+    rs.cam_pos[:] = c.position
+    rs.cam_look[:] = c.look_at
+    rs.cam_up[:] = c.up
+    rs.cam_fov_deg = c.fov_deg
+    rs.width, rs.height = settings.width, settings.height
+    rs.spp_total = settings.spp if spp_total is None else spp_total
+    rs.lambda_begin = float(np.float32(settings.lambda_begin))
+    rs.lambda_end = float(np.float32(settings.lambda_end))
+    rs.max_diffuse_bounce = settings.max_diffuse_bounce
+    rs.max_glossy_bounce = settings.max_glossy_bounce
+    rs.max_transmission_bounce = settings.max_transmission_bounce
+    rs.rr_min_depth = settings.rr_min_depth
+    rs.rr_threshold = float(np.float32(settings.rr_threshold))
+    rs.seed = settings.seed
+    H, W = settings.height, settings.width
+    sm = np.zeros((H, W, 3), np.float64)
+    sq = np.zeros((H, W, 3), np.float64)
+    cnt = np.zeros(3, np.uint64)
+    rc = L.rise_oracle_render(ctypes.byref(rs), s0, s1, sm.ctypes.data, sq.ctypes.data, cnt.ctypes.data, threads)
+    if rc != 0:
+        raise ValueError("oracle render rejected the scene")
+    return sm, sq, cnt
+
+
+def rng(seed: int, path: int, dim: int) -> float:
+    L = oracle_lib()
+    L.rise_oracle_rng.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+    L.rise_oracle_rng.restype = ctypes.c_double
+    return L.rise_oracle_rng(seed, path, dim)
+
+
+def xyz_from_nm(nm: float):
+    L = oracle_lib()
+    L.rise_oracle_xyz_from_nm.argtypes = [ctypes.c_double, _P]
+    out = np.zeros(3)
+    ok = L.rise_oracle_xyz_from_nm(nm, out.ctypes.data)
+    return bool(ok), out
+
+
+def cie_y_integral(a: float, b: float) -> float:
+    L = oracle_lib()
+    L.rise_oracle_cie_y_integral.argtypes = [ctypes.c_double, ctypes.c_double]
+    L.rise_oracle_cie_y_integral.restype = ctypes.c_double
+    return L.rise_oracle_cie_y_integral(a, b)

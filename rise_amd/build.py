@@ -26,6 +26,7 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "librise_hip.so"
+DEMO = PKG / "adapter_demo"
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -84,9 +85,21 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_demo()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DEMO}")
     return LIB
+
+
+def _build_demo() -> None:
+    """The C++ adapter demo (rise_amd/host), linked against the library."""
+    src = PKG / "host" / "adapter_demo.cpp"
+    cmd = [HIPCC, "-x", "c++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           f"-I{INCLUDE}", f"-I{PKG / 'host'}", str(src), "-o", str(DEMO), f"-L{PKG}", "-lrise_hip",
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,229 @@
+// pt_device.h -- device-side scene tables and shading functions of the
+// spectral path tracer (Lambertian surfaces, Lambertian area luminaries,
+// hero-wavelength spectral sampling with 4 wavelengths).
+//
+// Each function cites the reference routine whose behaviour it follows
+// (paths relative to the reference's src/Library/).  Arithmetic is FP32;
+// intersection certification stays FP64 (trace_core.h).
+#pragma once
+
+#include "trace_core.h"
+
+namespace rise_hip {
+
+constexpr int kHwss = 4;             // SampledWavelengths::N (Utilities/Color/SampledWavelengths.h:57)
+constexpr int kMaxPathDepth = 128;   // PathTracingIntegrator mMaxPathDepth
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr float kNearZeroF = 1e-12f;
+
+struct DevSpd {
+    float begin, end, inv_delta;
+    int n;
+    int offset;  // into the amplitude table
+    int pad[3];
+};
+
+struct DevMaterial {
+    int type;
+    int refl_spd;
+    int emit_spd;   // -1: none
+    float emit_scale;
+};
+
+struct DevObject {
+    int material;
+    int double_sided;
+    int luminary;   // -1: not a luminary
+    int pad;
+};
+
+struct DevLuminary {
+    int object;
+    int tri_first;   // into lum_tris / lum_cdf
+    int tri_count;
+    float area;
+    float pdf_select;   // weight / total weight
+    float cdf_select;   // cumulative selection probability up to and including this luminary
+    int material;
+    int pad;
+};
+
+struct PtSceneView {
+    const Node8* nodes;
+    const Tri48* tris;      // leaf order; v0.w = global id, v1.w = object id
+    int depth;
+    int n_lums;
+    const DevObject* objects;
+    const DevMaterial* materials;
+    const DevSpd* spds;
+    const float* spd_amp;
+    const DevLuminary* lums;
+    const unsigned* lum_tris;   // leaf-order triangle indices
+    const float* lum_cdf;       // area CDF per luminary
+};
+
+// ---- random numbers: counter-based hash ------------------------------------
+// u = top 24 bits of splitmix64(key * golden + seed), key = path << 32 | dim.
+// The oracle (oracle/pt_oracle.c) uses the same function, so both follow the
+// same random decisions.  Dimension layout: stream s covers dims 32s..32s+31;
+// the camera uses stream 0, bounce `depth` uses stream 16 + depth, as
+// SobolSampler::StartStream(16 + depth) does (PathTracingIntegrator.cpp:4112).
+RISE_HD inline float rng_u(unsigned long long seed, unsigned path, unsigned dim)
+{
+    unsigned long long x = ((((unsigned long long)path) << 32) | dim) * 0x9E3779B97F4A7C15ull + seed;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (float)(unsigned)(x >> 40) * (1.0f / 16777216.0f);
+}
+
+// ---- small float vector ----------------------------------------------------
+struct f3 {
+    float x, y, z;
+};
+RISE_HD inline f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+RISE_HD inline f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+RISE_HD inline f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+RISE_HD inline f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+RISE_HD inline f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+RISE_HD inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RISE_HD inline f3 cross3(f3 a, f3 b)
+{
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+RISE_HD inline float len3(f3 a) { return sqrtf(dot3(a, a)); }
+RISE_HD inline f3 norm3(f3 a)
+{
+    const float m = len3(a);
+    return m > 0.0f ? a * (1.0f / m) : a;
+}
+
+struct Onb {
+    f3 u, v, w;
+};
+
+// OrthonormalBasis3D::CreateFromW (Utilities/OrthonormalBasis3D.cpp:37-56).
+RISE_HD inline Onb onb_from_w(f3 n)
+{
+    Onb b;
+    b.w = norm3(n);
+    const f3 cu = mk3(1, 0, 0), cv = mk3(0, 1, 0), cw = mk3(0, 0, 1);
+    if (fabsf(dot3(b.w, cv) - 1.0f) < kNearZeroF) {
+        b.u = cw;
+        b.v = cu;
+    } else if (fabsf(dot3(b.w, cv) + 1.0f) < kNearZeroF) {
+        b.u = cu;
+        b.v = cw;
+    } else {
+        b.u = norm3(cross3(b.w, cv));
+        b.v = norm3(cross3(b.w, b.u));
+    }
+    return b;
+}
+
+// OrthonormalBasis3D::Transform (OrthonormalBasis3D.cpp:138-144).
+RISE_HD inline f3 onb_xform(const Onb& b, f3 a) { return b.u * a.x + b.v * a.y + b.w * a.z; }
+
+// PowerHeuristic (Utilities/PathTransportUtilities.h:251-258).
+RISE_HD inline float power_heuristic(float pa, float pb)
+{
+    const float a2 = pa * pa;
+    return a2 / (a2 + pb * pb);
+}
+
+// SpectralPacket::ValueAtNM (Utilities/Color/SpectralPacket.h:285-297).
+RISE_HD inline float spd_value(const PtSceneView& s, int spd, float nm)
+{
+    const DevSpd d = s.spds[spd];
+    if (nm < d.begin || nm > d.end) return 0.0f;
+    int idx = (int)((nm - d.begin) * d.inv_delta);
+    if (idx >= d.n) idx = d.n - 1;
+    if (idx < 0) idx = 0;
+    return s.spd_amp[d.offset + idx];
+}
+
+// Offsets a surface point along the geometric normal so a ray leaving it
+// does not re-hit the surface it starts on: Waechter and Binder, "A Fast and
+// Robust Method for Avoiding Self-Intersection", Ray Tracing Gems (2019),
+// ch. 6 (integer ULP offset, a fixed epsilon near the origin).  The reference
+// instead backs off 1e-12 and advances 1e-8 in FP64 (Objects/Object.cpp:641,
+// PathTracingIntegrator.cpp:4812), which FP32 cannot resolve.
+RISE_HD inline f3 offset_ray(f3 p, f3 n)
+{
+    const float origin = 1.0f / 32.0f, float_scale = 1.0f / 65536.0f, int_scale = 256.0f;
+    const int ox = (int)(int_scale * n.x), oy = (int)(int_scale * n.y), oz = (int)(int_scale * n.z);
+    const float px = bits_to_float((unsigned)(float_to_bits(p.x) + (p.x < 0 ? -ox : ox)));
+    const float py = bits_to_float((unsigned)(float_to_bits(p.y) + (p.y < 0 ? -oy : oy)));
+    const float pz = bits_to_float((unsigned)(float_to_bits(p.z) + (p.z < 0 ? -oz : oz)));
+    return mk3(fabsf(p.x) < origin ? p.x + float_scale * n.x : px, fabsf(p.y) < origin ? p.y + float_scale * n.y : py,
+               fabsf(p.z) < origin ? p.z + float_scale * n.z : pz);
+}
+
+// Surface record at a hit (the fields of RayIntersectionGeometric the
+// Lambertian routines read).
+struct Surf {
+    f3 p;       // hit point
+    f3 dir;     // incoming direction (ri.ray.Dir())
+    f3 n;       // shading normal (face normal; ...Specializations.h:194-200)
+    f3 ng;      // geometric normal (:207-213)
+    Onb frame;  // ri.onb
+    int object;
+    int material;
+};
+
+// Geometric normal facing against the incoming ray (LambertianBRDF.cpp:51-53).
+RISE_HD inline f3 geom_against_ray(const Surf& h, f3 fallback)
+{
+    const f3 raw = dot3(h.ng, h.ng) > 1e-12f ? h.ng : fallback;
+    return dot3(raw, h.dir) < 0.0f ? raw : -raw;
+}
+
+// ShouldReflect (Materials/LambertianBRDF.cpp:44-64).
+RISE_HD inline bool should_reflect(const Surf& h, f3 vin)
+{
+    const f3 n = h.frame.w;
+    const f3 v = norm3(vin);
+    const f3 r = norm3(-h.dir);
+    const float nr = dot3(n, r), nv = dot3(n, v);
+    const f3 g = geom_against_ray(h, n);
+    if ((nr <= -kNearZeroF && nv <= -kNearZeroF) || (nr >= kNearZeroF && nv >= kNearZeroF))
+        return dot3(v, g) > 0.0f && dot3(r, g) > 0.0f;
+    return false;
+}
+
+// LambertianBRDF::valueNM (LambertianBRDF.cpp:76-84).
+RISE_HD inline float lambert_value(const PtSceneView& s, const Surf& h, int refl_spd, f3 vin, float nm)
+{
+    if (!should_reflect(h, vin)) return 0.0f;
+    return spd_value(s, refl_spd, nm) * kInvPi;
+}
+
+// LambertianSPF::Pdf (Materials/LambertianSPF.cpp:147-163).
+RISE_HD inline float lambert_pdf(const Surf& h, f3 wo)
+{
+    const bool front = dot3(h.dir, h.frame.w) <= kNearZeroF;
+    const float c = front ? dot3(wo, h.frame.w) : -dot3(wo, h.frame.w);
+    const f3 neff = front ? h.frame.w : -h.frame.w;
+    const f3 g = geom_against_ray(h, neff);
+    return (c > 0.0f && dot3(wo, g) > 0.0f) ? c * kInvPi : 0.0f;
+}
+
+// LambertianEmitter::emittedRadianceNM (Materials/LambertianEmitter.cpp:82-89).
+RISE_HD inline float emitted(const PtSceneView& s, const DevMaterial& m, f3 out, f3 N, float nm)
+{
+    if (dot3(out, N) <= 0.0f) return 0.0f;
+    return spd_value(s, m.emit_spd, nm) * kInvPi * m.emit_scale;
+}
+
+// SampledWavelengths::SampleEquidistant (SampledWavelengths.h:79-106).
+RISE_HD inline void wavelengths(float lambda0, float a, float b, float lam[kHwss])
+{
+    const float range = b - a, delta = range / (float)kHwss;
+    lam[0] = lambda0;
+    for (int i = 1; i < kHwss; ++i) lam[i] = a + fmodf(lambda0 - a + (float)i * delta, range);
+}
+
+}  // namespace rise_hip

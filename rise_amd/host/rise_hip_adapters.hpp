@@ -1,0 +1,145 @@
+// rise_hip_adapters.hpp -- C++ host classes over the C ABI, shaped like the
+// reference interfaces they stand in for, so RISE-side code can call the GPU
+// path through familiar entry points.
+//
+//   HipObjectManager      ~ IObjectManager (Interfaces/IObjectManager.h:38-68):
+//                           PrepareForRendering, IntersectRay, IntersectShadowRay,
+//                           here over batches of rays in device memory.
+//   HipSpectralRasterizer ~ IRasterizer::RasterizeScene (Interfaces/IRasterizer.h:70)
+//                           for PathTracingSpectralRasterizer
+//                           (Rendering/PathTracingSpectralRasterizer.h:48-58).
+//
+// Error convention as in RISE: methods return bool and log the reason
+// (GlobalLog()->PrintEasyError in the reference; a caller-settable sink here);
+// no exception leaves these classes.  Header-only; link librise_hip.so.
+#pragma once
+
+#include "rise_hip.h"
+#include "rise_hip_render.h"
+
+#include <cstdint>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace rise_hip_host {
+
+using LogSink = std::function<void(const std::string&)>;
+
+inline LogSink& log_sink()
+{
+    static LogSink sink = [](const std::string& m) { std::fprintf(stderr, "rise_hip: %s\n", m.c_str()); };
+    return sink;
+}
+
+inline bool check(rise_hip_status s, const char* what)
+{
+    if (s == RISE_HIP_OK) return true;
+    log_sink()(std::string(what) + " failed: " + rise_hip_last_error());
+    return false;
+}
+
+// Batch ray queries over a resident triangle scene.
+class HipObjectManager {
+public:
+    HipObjectManager() = default;
+    HipObjectManager(const HipObjectManager&) = delete;
+    HipObjectManager& operator=(const HipObjectManager&) = delete;
+    ~HipObjectManager() { rise_hip_scene_release(scene_); }
+
+    // Adds a world-space mesh (n_tris * 9 floats); copied at PrepareForRendering.
+    void AddMesh(const float* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr)
+    {
+        meshes_.push_back({vertices, n_tris, tri_ids});
+    }
+
+    // ObjectManager::PrepareForRendering (Managers/ObjectManager.cpp:404-423):
+    // builds the acceleration structure and uploads it.
+    bool PrepareForRendering()
+    {
+        rise_hip_scene_release(scene_);
+        scene_ = nullptr;
+        return check(rise_hip_scene_upload(meshes_.data(), meshes_.size(), &scene_), "PrepareForRendering");
+    }
+
+    // IntersectRay for a batch: rays_soa (device, 6*n floats), outputs device.
+    bool IntersectRay(const float* rays_soa, int64_t n, int32_t* prim_id, float* t, hipStream_t stream) const
+    {
+        return check(rise_hip_intersect_closest(scene_, rays_soa, n, prim_id, t, stream), "IntersectRay");
+    }
+
+    // IntersectShadowRay for a batch; tmax (device) may be NULL.
+    bool IntersectShadowRay(const float* rays_soa, const float* tmax, int64_t n, uint8_t* occluded,
+                            hipStream_t stream) const
+    {
+        return check(rise_hip_intersect_any(scene_, rays_soa, tmax, n, occluded, stream), "IntersectShadowRay");
+    }
+
+    bool GetStats(rise_hip_scene_stats& out) const
+    {
+        return check(rise_hip_scene_get_stats(scene_, &out), "GetStats");
+    }
+
+private:
+    std::vector<rise_hip_mesh_desc> meshes_;
+    rise_hip_scene* scene_ = nullptr;
+};
+
+// The spectral path-tracing rasterizer for one resident scene.
+class HipSpectralRasterizer {
+public:
+    HipSpectralRasterizer() = default;
+    HipSpectralRasterizer(const HipSpectralRasterizer&) = delete;
+    HipSpectralRasterizer& operator=(const HipSpectralRasterizer&) = delete;
+    ~HipSpectralRasterizer() { rise_hip_pt_scene_release(scene_); }
+
+    int AddSpectrum(float begin, float end, const std::vector<float>& amplitudes)
+    {
+        amps_.push_back(amplitudes);
+        spds_.push_back({begin, end, (int32_t)amplitudes.size(), nullptr});
+        return (int)spds_.size() - 1;
+    }
+
+    int AddLambertianMaterial(int reflectance_spd, int emission_spd = -1, float emission_scale = 1.0f)
+    {
+        rise_hip_material_desc m{};
+        m.type = RISE_HIP_MATERIAL_LAMBERTIAN;
+        m.reflectance_spd = reflectance_spd;
+        m.emission_spd = emission_spd;
+        m.emission_scale = emission_scale;
+        materials_.push_back(m);
+        return (int)materials_.size() - 1;
+    }
+
+    void AddObject(const float* vertices, int64_t n_tris, int material, bool double_sided = false)
+    {
+        objects_.push_back({vertices, n_tris, material, double_sided ? 1 : 0});
+    }
+
+    bool PrepareForRendering()
+    {
+        for (size_t i = 0; i < spds_.size(); ++i) spds_[i].amplitudes = amps_[i].data();
+        rise_hip_pt_scene_release(scene_);
+        scene_ = nullptr;
+        return check(rise_hip_pt_scene_create(objects_.data(), objects_.size(), materials_.data(), materials_.size(),
+                                              spds_.data(), spds_.size(), &scene_),
+                     "PrepareForRendering");
+    }
+
+    // RasterizeScene: renders desc into film_xyz (device, W*H*3 mean XYZ).
+    bool RasterizeScene(const rise_hip_render_desc& desc, float* film_xyz, hipStream_t stream,
+                        rise_hip_render_stats* stats = nullptr) const
+    {
+        return check(rise_hip_render(scene_, &desc, film_xyz, nullptr, stats, stream), "RasterizeScene");
+    }
+
+private:
+    std::vector<std::vector<float>> amps_;
+    std::vector<rise_hip_spd_desc> spds_;
+    std::vector<rise_hip_material_desc> materials_;
+    std::vector<rise_hip_object_desc> objects_;
+    rise_hip_pt_scene* scene_ = nullptr;
+};
+
+}  // namespace rise_hip_host

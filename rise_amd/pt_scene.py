@@ -1,0 +1,174 @@
+"""Spectral path-tracing scene description and the seeded test scenes.
+
+A scene is a list of step spectra (the reference's ``spectral_painter``:
+a SpectralPacket of n equal bins over [begin, end]), Lambertian materials
+(optionally emitting, as ``lambertian_luminaire_material``), world-space
+triangle-mesh objects and a pinhole camera.  ``cornell_box()`` builds the
+Cornell-box render workload (closed box of 5 quads, 2 axis-aligned boxes,
+a 0.5 m^2 area light of radiance 15, 256x256, 16 spp, depth 5, Russian
+roulette from depth 3, 4 hero wavelengths over 380-720 nm).
+
+RGB albedos (white 0.75, red 0.63/0.06/0.05, green 0.14/0.45/0.09) are
+mapped to 3-band step spectra over [380, 720] nm (bands of 113.3 nm: blue,
+green, red), because the reference's RGB-to-spectrum table is not available
+(its RGB painters evaluate to a constant 0.5 without it); scenes for the
+reference therefore have to be authored with spectral painters, and this is
+the mapping used on every side.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Spd:
+    begin: float
+    end: float
+    amp: np.ndarray
+
+    @staticmethod
+    def flat(v: float, begin: float = 380.0, end: float = 720.0) -> "Spd":
+        return Spd(begin, end, np.array([v], np.float32))
+
+    @staticmethod
+    def rgb_bands(r: float, g: float, b: float, begin: float = 380.0, end: float = 720.0) -> "Spd":
+        return Spd(begin, end, np.array([b, g, r], np.float32))
+
+
+@dataclass
+class Material:
+    reflectance: int            # SPD index
+    emission: int = -1          # SPD index or -1
+    emission_scale: float = 1.0
+    kind: int = 0               # 0 = Lambertian
+
+
+@dataclass
+class Object:
+    tris: np.ndarray            # (n, 9) float32
+    material: int
+    double_sided: bool = False
+
+
+@dataclass
+class Camera:
+    position: tuple
+    look_at: tuple
+    up: tuple = (0.0, 1.0, 0.0)
+    fov_deg: float = 40.0
+
+
+@dataclass
+class RenderSettings:
+    width: int
+    height: int
+    spp: int
+    lambda_begin: float = 380.0
+    lambda_end: float = 720.0
+    max_diffuse_bounce: int = 5
+    max_glossy_bounce: int = 5
+    max_transmission_bounce: int = 5
+    rr_min_depth: int = 3
+    rr_threshold: float = 0.05
+    seed: int = 1
+
+
+@dataclass
+class Scene:
+    spds: list = field(default_factory=list)
+    materials: list = field(default_factory=list)
+    objects: list = field(default_factory=list)
+    camera: Camera | None = None
+
+    def flat(self):
+        """All triangles (n, 9) float32 and their object index (n,) int32."""
+        tris = np.concatenate([o.tris for o in self.objects], 0).astype(np.float32)
+        obj = np.concatenate([np.full(len(o.tris), k, np.int32) for k, o in enumerate(self.objects)])
+        return np.ascontiguousarray(tris), obj
+
+    @property
+    def n_tris(self) -> int:
+        return int(sum(len(o.tris) for o in self.objects))
+
+
+def quad(p0, p1, p2, p3) -> np.ndarray:
+    """Two triangles (p0, p1, p2), (p0, p2, p3); face normal = (p1-p0) x (p2-p0)."""
+    p0, p1, p2, p3 = (np.asarray(p, np.float64) for p in (p0, p1, p2, p3))
+    return np.array([np.concatenate([p0, p1, p2]), np.concatenate([p0, p2, p3])], np.float32)
+
+
+def box(lo, hi) -> np.ndarray:
+    """Axis-aligned box, 12 triangles, outward normals."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    faces = [
+        quad((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0)),  # -x
+        quad((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1)),  # +x
+        quad((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1)),  # -y
+        quad((x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0)),  # +y
+        quad((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0)),  # -z
+        quad((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)),  # +z
+    ]
+    return np.concatenate(faces, 0)
+
+
+def cornell_box():
+    """The Cornell-box workload: (Scene, RenderSettings).
+
+    Box x in [-1, 1], y in [0, 2], z in [-1, 1], open toward +z where the
+    camera looks in; inward normals.  10 wall triangles, 24 box triangles,
+    2 light triangles (36 in all)."""
+    s = Scene()
+    s.spds = [
+        Spd.flat(0.75),                      # 0 white
+        Spd.rgb_bands(0.63, 0.06, 0.05),     # 1 red
+        Spd.rgb_bands(0.14, 0.45, 0.09),     # 2 green
+        Spd.flat(15.0),                      # 3 emission: L = 15 / pi * pi = 15
+    ]
+    s.materials = [
+        Material(0),                                         # 0 white
+        Material(1),                                         # 1 red
+        Material(2),                                         # 2 green
+        Material(0, emission=3, emission_scale=math.pi),     # 3 light
+    ]
+    floor = quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))          # +y
+    ceiling = quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1))        # -y
+    back = quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1))         # +z
+    left = quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1))         # +x
+    right = quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1))            # -x
+    h = math.sqrt(0.5) / 2.0
+    light = quad((-h, 1.98, -h), (h, 1.98, -h), (h, 1.98, h), (-h, 1.98, h))  # -y, area 0.5
+    s.objects = [
+        Object(np.concatenate([floor, ceiling, back]), 0),
+        Object(left, 1),
+        Object(right, 2),
+        Object(box((0.05, 0.0, -0.05), (0.65, 0.6, 0.55)), 0),
+        Object(box((-0.65, 0.0, -0.6), (-0.05, 1.2, 0.0)), 0),
+        Object(light, 3),
+    ]
+    s.camera = Camera(position=(0.0, 1.0, 3.4), look_at=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=40.0)
+    r = RenderSettings(width=256, height=256, spp=16)
+    return s, r
+
+
+def furnace_box(albedo: float = 0.5, radiance: float = 1.0, max_bounce: int = 64):
+    """White furnace: a closed box whose six faces (six objects, so each face is
+    its own luminary) all emit `radiance` (flat) and reflect `albedo` (flat).
+    From inside, L = Le / (1 - albedo) up to the albedo^(max_bounce + 1) term
+    (the last vertex keeps only the NEE share of its MIS pair), so with a
+    large cap the render checks MIS, NEE, Russian roulette and the
+    Y normalization (E[Y] = L for a flat spectrum) against a closed form."""
+    s = Scene()
+    s.spds = [Spd.flat(albedo), Spd.flat(radiance)]
+    s.materials = [Material(0, emission=1, emission_scale=math.pi)]
+    tris = box((-1, -1, -1), (1, 1, 1))
+    # inward normals: swap two vertices of every outward triangle
+    tris = np.ascontiguousarray(tris.reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9))
+    s.objects = [Object(tris[2 * f: 2 * f + 2].copy(), 0) for f in range(6)]
+    s.camera = Camera(position=(0.1, 0.05, 0.2), look_at=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov_deg=60.0)
+    r = RenderSettings(width=16, height=16, spp=64, max_diffuse_bounce=max_bounce, max_glossy_bounce=0,
+                       max_transmission_bounce=0)
+    return s, r

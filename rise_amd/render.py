@@ -1,0 +1,193 @@
+"""Python mirror of RISE's spectral path-tracing rasterizer over the C ABI.
+
+``PathTracingSpectralRasterizer`` follows the reference's
+``RISE_API_CreatePathTracingSpectralRasterizer`` + ``RasterizeScene``
+(RISE_API.h:3112-3141, Interfaces/IRasterizer.h:70): it is created for a
+scene, and ``rasterize_scene`` renders the configured samples of every pixel
+into an XYZ film on the GPU (mean XYZ per pixel; the reference resolves the
+same XYZ sums to RGB afterwards).  Errors raise RiseHipError.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .pt_scene import RenderSettings, Scene
+
+
+class SpdDesc(ctypes.Structure):
+    _fields_ = [
+        ("lambda_begin", ctypes.c_float),
+        ("lambda_end", ctypes.c_float),
+        ("n", ctypes.c_int32),
+        ("amplitudes", ctypes.c_void_p),
+    ]
+
+
+class MaterialDesc(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("reflectance_spd", ctypes.c_int32),
+        ("emission_spd", ctypes.c_int32),
+        ("emission_scale", ctypes.c_float),
+        ("reserved", ctypes.c_float * 4),
+    ]
+
+
+class ObjectDesc(ctypes.Structure):
+    _fields_ = [
+        ("vertices", ctypes.c_void_p),
+        ("n_tris", ctypes.c_int64),
+        ("material", ctypes.c_int32),
+        ("double_sided", ctypes.c_int32),
+    ]
+
+
+class CameraDesc(ctypes.Structure):
+    _fields_ = [
+        ("position", ctypes.c_float * 3),
+        ("look_at", ctypes.c_float * 3),
+        ("up", ctypes.c_float * 3),
+        ("fov_deg", ctypes.c_float),
+    ]
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("spp", ctypes.c_int32),
+        ("spp_offset", ctypes.c_int32),
+        ("spp_total", ctypes.c_int32),
+        ("lambda_begin", ctypes.c_float),
+        ("lambda_end", ctypes.c_float),
+        ("max_diffuse_bounce", ctypes.c_int32),
+        ("max_glossy_bounce", ctypes.c_int32),
+        ("max_transmission_bounce", ctypes.c_int32),
+        ("rr_min_depth", ctypes.c_int32),
+        ("rr_threshold", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+        ("camera", CameraDesc),
+    ]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [
+        ("extension_rays", ctypes.c_int64),
+        ("shadow_rays", ctypes.c_int64),
+        ("paths", ctypes.c_int64),
+        ("bounces", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_SIG = {
+    "rise_hip_pt_scene_create": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.POINTER(_P)],
+                                 ctypes.c_int),
+    "rise_hip_pt_scene_release": ([_P], ctypes.c_int),
+    "rise_hip_render": ([_P, ctypes.POINTER(RenderDesc), _P, _P, ctypes.POINTER(RenderStats), _P], ctypes.c_int),
+}
+
+
+def _lib_render():
+    L = _lib.lib()
+    for name, (a, r) in _SIG.items():
+        fn = getattr(L, name)
+        fn.argtypes = a
+        fn.restype = r
+    return L
+
+
+def render_desc(settings: RenderSettings, scene: Scene, spp: int | None = None, spp_offset: int = 0,
+                spp_total: int | None = None) -> RenderDesc:
+    d = RenderDesc()
+    d.width = settings.width
+    d.height = settings.height
+    d.spp = settings.spp if spp is None else spp
+    d.spp_offset = spp_offset
+    d.spp_total = (settings.spp if spp_total is None else spp_total)
+    d.lambda_begin = settings.lambda_begin
+    d.lambda_end = settings.lambda_end
+    d.max_diffuse_bounce = settings.max_diffuse_bounce
+    d.max_glossy_bounce = settings.max_glossy_bounce
+    d.max_transmission_bounce = settings.max_transmission_bounce
+    d.rr_min_depth = settings.rr_min_depth
+    d.rr_threshold = settings.rr_threshold
+    d.seed = settings.seed
+    c = scene.camera
+    d.camera.position[:] = c.position
+    d.camera.look_at[:] = c.look_at
+    d.camera.up[:] = c.up
+    d.camera.fov_deg = c.fov_deg
+    return d
+
+
+class PathTracingSpectralRasterizer:
+    """A scene resident on the GPU plus a render entry point."""
+
+    def __init__(self, scene: Scene):
+        L = _lib_render()
+        self.scene = scene
+        self._keep = []
+        spds = (SpdDesc * max(1, len(scene.spds)))()
+        for k, s in enumerate(scene.spds):
+            a = np.ascontiguousarray(s.amp, np.float32)
+            self._keep.append(a)
+            spds[k].lambda_begin = s.begin
+            spds[k].lambda_end = s.end
+            spds[k].n = a.size
+            spds[k].amplitudes = a.ctypes.data
+        mats = (MaterialDesc * max(1, len(scene.materials)))()
+        for k, m in enumerate(scene.materials):
+            mats[k].type = m.kind
+            mats[k].reflectance_spd = m.reflectance
+            mats[k].emission_spd = m.emission
+            mats[k].emission_scale = m.emission_scale
+        objs = (ObjectDesc * max(1, len(scene.objects)))()
+        for k, o in enumerate(scene.objects):
+            t = np.ascontiguousarray(o.tris, np.float32).reshape(-1, 9)
+            self._keep.append(t)
+            objs[k].vertices = t.ctypes.data
+            objs[k].n_tris = t.shape[0]
+            objs[k].material = o.material
+            objs[k].double_sided = 1 if o.double_sided else 0
+        out = _P(None)
+        _lib.check(L.rise_hip_pt_scene_create(objs, len(scene.objects), mats, len(scene.materials), spds,
+                                              len(scene.spds), ctypes.byref(out)))
+        self._h = out
+        self._keep = []
+
+    def rasterize_scene(self, settings: RenderSettings, spp: int | None = None, spp_offset: int = 0,
+                        spp_total: int | None = None, want_sq: bool = False, stream=None, stats: bool = True,
+                        film=None, film_sq=None):
+        """Renders and returns (mean XYZ (H, W, 3) tensor, mean squared XYZ or
+        None, stats dict or None).  Tensors stay on the GPU."""
+        import torch
+
+        d = render_desc(settings, self.scene, spp, spp_offset, spp_total)
+        if film is None:
+            film = torch.empty((settings.height, settings.width, 3), dtype=torch.float32, device="cuda")
+        if want_sq and film_sq is None:
+            film_sq = torch.empty_like(film)
+        st = RenderStats() if stats else None
+        s = torch.cuda.current_stream().cuda_stream if stream is None else int(getattr(stream, "cuda_stream", stream))
+        _lib.check(_lib_render().rise_hip_render(self._h, ctypes.byref(d), film.data_ptr(),
+                                                 film_sq.data_ptr() if want_sq else None,
+                                                 ctypes.byref(st) if stats else None, s))
+        return film, (film_sq if want_sq else None), (st.as_dict() if stats else None)
+
+    def release(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib_render().rise_hip_pt_scene_release(self._h)
+            self._h = _P(None)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

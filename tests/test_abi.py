@@ -33,3 +33,15 @@ def test_null_arguments_rejected():
     assert L.rise_hip_scene_upload(ctypes.byref(desc), 1, ctypes.byref(out)) == 1
     desc = _lib.MeshDesc(None, -1, None)
     assert L.rise_hip_scene_upload(ctypes.byref(desc), 1, ctypes.byref(out)) == 1
+
+
+def test_cpp_adapter_demo_links():
+    # The C++ host adapters (rise_amd/host) build and load the library; with no
+    # arguments the demo prints its usage without touching the GPU.
+    import subprocess
+
+    from rise_amd import build
+
+    assert build.DEMO.exists()
+    r = subprocess.run([str(build.DEMO)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 64 and "usage" in r.stderr

@@ -1,0 +1,93 @@
+"""Pins the path-tracing oracle (oracle/pt_oracle.c).
+
+* Colour: the CIE Y integral over 380-780 nm is the value the reference
+  states for its table (~106.86, PixelBasedSpectralIntegratingRasterizer.cpp:58-62),
+  XYZFromNM reproduces the table at its nodes and interpolates linearly.
+* Random numbers: uniform in [0, 1) with 24 bits, distinct per path and dim.
+* White furnace (closed form L = Le / (1 - albedo)) with MIS, NEE, Russian
+  roulette and the Y normalization all active.
+* Directly viewed emitter: Y = Le (no MIS at the camera vertex).
+* Reciprocity of the image under a seed change (statistical agreement of
+  two independent renders).
+"""
+import math
+
+import numpy as np
+
+import oracle
+from rise_amd import pt_scene
+
+
+def test_cie_y_integral_and_table():
+    assert abs(oracle.cie_y_integral(380, 780) - 106.86) < 0.01
+    ok, xyz = oracle.xyz_from_nm(555.0)
+    assert ok and abs(xyz[1] - 1.0) < 1e-12
+    ok, a = oracle.xyz_from_nm(550.0)
+    ok, b = oracle.xyz_from_nm(560.0)
+    ok, m = oracle.xyz_from_nm(552.5)
+    assert np.allclose(m, a + (oracle.xyz_from_nm(555.0)[1] - a) * 0.5)
+    assert not oracle.xyz_from_nm(379.0)[0] and not oracle.xyz_from_nm(781.0)[0]
+    # The trapezoid sum equals the integral of the interpolated table.
+    xs = np.linspace(380, 720, 340 * 64 + 1)
+    ys = np.array([oracle.xyz_from_nm(x)[1][1] for x in xs[::64]])
+    assert abs(np.trapezoid(ys, xs[::64]) - oracle.cie_y_integral(380, 720)) < 1e-9
+
+
+def test_rng_uniform_24_bit():
+    u = np.array([oracle.rng(7, p, d) for p in range(200) for d in range(50)])
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert np.all(np.abs(u * 16777216.0 - np.round(u * 16777216.0)) == 0)
+    assert abs(u.mean() - 0.5) < 0.01
+    assert len(np.unique(u)) > 0.99 * u.size
+    assert oracle.rng(1, 5, 3) != oracle.rng(2, 5, 3)
+
+
+def _y_stats(sm, sq, n):
+    y = sm[..., 1].sum() / n
+    var = sq[..., 1].sum() / n - y * y
+    return y, math.sqrt(max(var, 0.0) / n)
+
+
+def test_white_furnace_closed_form():
+    s, r = pt_scene.furnace_box(albedo=0.5, radiance=1.0, max_bounce=64)
+    sm, sq, cnt = oracle.render(s, r)
+    n = r.width * r.height * r.spp
+    y, se = _y_stats(sm, sq, n)
+    assert abs(y - 2.0) < 4 * se + 1e-3, (y, se)
+    # Russian roulette keeps the estimate unbiased when it starts at depth 0.
+    r.rr_min_depth = 0
+    r.seed = 5
+    sm, sq, cnt2 = oracle.render(s, r)
+    y0, se0 = _y_stats(sm, sq, n)
+    assert abs(y0 - 2.0) < 4 * se0 + 1e-3, (y0, se0)
+    assert cnt2[0] < cnt[0]
+
+
+def test_emitter_seen_directly():
+    s, r = pt_scene.cornell_box()
+    # A black emitter (no reflected light on top of Le), seen head-on.
+    s.spds.append(pt_scene.Spd.flat(0.0))
+    s.materials[3].reflectance = len(s.spds) - 1
+    s.camera = pt_scene.Camera(position=(0.0, 1.0, 0.0), look_at=(0.0, 2.0, 0.0), up=(0.0, 0.0, 1.0), fov_deg=10.0)
+    r.width = r.height = 8
+    r.spp = 256
+    sm, sq, _ = oracle.render(s, r)
+    y, se = _y_stats(sm, sq, r.width * r.height * r.spp)
+    assert abs(y - 15.0) < 4 * se + 1e-6, (y, se)
+
+
+def test_two_seeds_agree_statistically():
+    s, r = pt_scene.cornell_box()
+    r.width = r.height = 24
+    r.spp = 64
+    a, aq, _ = oracle.render(s, r)
+    r.seed = 99
+    b, bq, _ = oracle.render(s, r)
+    n = r.spp
+    ma, mb = a / n, b / n
+    va = np.maximum(aq / n - ma * ma, 0) / n
+    vb = np.maximum(bq / n - mb * mb, 0) / n
+    z = (ma - mb) / np.sqrt(va + vb + 1e-30)
+    frac = np.mean(np.abs(z[..., 1]) < 3.0)
+    assert frac >= 0.97, frac
+    assert abs(ma[..., 1].mean() / mb[..., 1].mean() - 1) < 0.02
