@@ -34,7 +34,6 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s + render requests/s with p50/p99 latency at ≤1% rel-RMSE, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 80
 TRI_BYTES = 48
 RAY_IO_BYTES = 32
 
@@ -201,7 +200,8 @@ def main() -> int:
     cnt = counters.cpu().numpy().astype(np.float64)
     nodes_per_ray = cnt[0] / n
     tris_per_ray = cnt[1] / n
-    b_ray = RAY_IO_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
+    node_bytes = int(st["node_bytes"])  # node record stride fetched per visit
+    b_ray = RAY_IO_BYTES + node_bytes * nodes_per_ray + TRI_BYTES * tris_per_ray
     log(f"per ray: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} triangles -> {b_ray:.1f} algorithmic bytes")
 
     for _ in range(args.warmup):
@@ -303,6 +303,7 @@ def main() -> int:
             "traffic": traffic,
             "kernel": "closest_kernel",
             "bytes_per_ray": round(b_ray, 2),
+            "node_record_bytes": node_bytes,
             "nodes_per_ray": round(nodes_per_ray, 3),
             "tris_per_ray": round(tris_per_ray, 3),
         },

@@ -27,6 +27,20 @@ struct alignas(16) Node8 {
 };
 static_assert(sizeof(Node8) == 80, "Node8 must be 80 bytes");
 
+// Nodes are stored RISE_NODE_STRIDE bytes apart.  At 128 every node starts on
+// a 128-byte cache line and is fetched as one line; at 80 half of the nodes
+// straddle two lines.
+#ifndef RISE_NODE_STRIDE
+#define RISE_NODE_STRIDE 128
+#endif
+#if RISE_NODE_STRIDE == 80
+using NodeSlot = Node8;
+#else
+struct alignas(RISE_NODE_STRIDE) NodeSlot : Node8 {
+};
+#endif
+static_assert(sizeof(NodeSlot) == RISE_NODE_STRIDE, "node stride");
+
 // 48 bytes: three float4 loads.  w of v0 carries the caller's triangle id
 // (bit pattern of an int32), w of v1 / v2 is 0.
 struct alignas(16) Tri48 {

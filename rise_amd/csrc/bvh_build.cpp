@@ -254,7 +254,7 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
     }
     if (n == 0) {
         // Empty scene: a single node with no children; every ray misses.
-        Node8 root;
+        NodeSlot root;
         std::memset(&root, 0, sizeof(root));
         out.nodes.push_back(root);
         out.stats.nodes = 1;
@@ -501,7 +501,7 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
             node.qhiy[s] = qh[1][i];
             node.qhiz[s] = qh[2][i];
         }
-        out.nodes[pd.n8] = node;
+        static_cast<Node8&>(out.nodes[pd.n8]) = node;
     }
     if (max_level > kMaxStack) {
         err = "BVH too deep for the traversal stack (" + std::to_string(max_level) + " levels)";

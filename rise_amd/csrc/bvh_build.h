@@ -34,7 +34,7 @@ struct BuildStats {
 };
 
 struct WideBVH {
-    std::vector<Node8> nodes;
+    std::vector<NodeSlot> nodes;
     std::vector<Tri48> tris;   // leaf order; v0.w carries the caller's id
     float scene_lo[3] = {0, 0, 0};
     float scene_hi[3] = {0, 0, 0};

@@ -54,12 +54,12 @@ extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t 
         stats->n_nodes = (int64_t)bvh.nodes.size();
         stats->n_bvh2_nodes = bvh.stats.bvh2_nodes;
         stats->depth = bvh.stats.depth;
-        stats->node_bytes = (int32_t)sizeof(Node8);
+        stats->node_bytes = (int32_t)sizeof(NodeSlot);
         stats->tri_bytes = (int32_t)sizeof(Tri48);
         stats->build_ms = bvh.stats.build_ms;
         stats->sah_cost = bvh.stats.sah_cost;
     }
-    const Node8* nodes = bvh.nodes.data();
+    const NodeSlot* nodes = bvh.nodes.data();
     const Tri48* trs = bvh.tris.data();
     std::atomic<int64_t> next(0);
     std::atomic<unsigned long long> cn(0), ct(0);

@@ -50,7 +50,7 @@ struct DevLuminary {
 };
 
 struct PtSceneView {
-    const Node8* nodes;
+    const NodeSlot* nodes;
     const Tri48* tris;      // leaf order; v0.w = global id, v1.w = object id
     int depth;
     int n_lums;

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <new>
 #include <string>
@@ -15,11 +16,20 @@
 using namespace rise_hip;
 
 struct rise_hip_scene {
-    Node8* d_nodes = nullptr;
+    NodeSlot* d_nodes = nullptr;
     Tri48* d_tris = nullptr;
+    // Dynamic-fetch counters: each query takes the next slot, so queries on
+    // different streams do not share a counter (up to kFetchSlots in flight).
+    static constexpr int kFetchSlots = 64;
+    unsigned long long* d_fetch = nullptr;
+    std::atomic<unsigned> next_slot{0};
     int device = 0;
     rise_hip_scene_stats stats;
-    TraceScene view() const { return TraceScene{d_nodes, d_tris, stats.depth}; }
+    TraceScene view()
+    {
+        const unsigned k = next_slot.fetch_add(1u) % kFetchSlots;
+        return TraceScene{d_nodes, d_tris, stats.depth, d_fetch + k};
+    }
 };
 
 namespace rise_hip {
@@ -98,16 +108,18 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
         delete s;
         return fail_hip(e, "hipGetDevice");
     }
-    const size_t nb = bvh.nodes.size() * sizeof(Node8);
+    const size_t nb = bvh.nodes.size() * sizeof(NodeSlot);
     const size_t tb = std::max<size_t>(bvh.tris.size(), 1) * sizeof(Tri48);
     e = hipMalloc((void**)&s->d_nodes, nb);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_tris, tb);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_fetch, sizeof(unsigned long long) * rise_hip_scene::kFetchSlots);
     if (e == hipSuccess) e = hipMemcpy(s->d_nodes, bvh.nodes.data(), nb, hipMemcpyHostToDevice);
     if (e == hipSuccess && !bvh.tris.empty())
         e = hipMemcpy(s->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(Tri48), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(s->d_nodes);
-        hipFree(s->d_tris);
+        (void)hipFree(s->d_nodes);
+        (void)hipFree(s->d_tris);
+        (void)hipFree(s->d_fetch);
         delete s;
         return fail_hip(e, "scene upload");
     }
@@ -116,7 +128,7 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
     s->stats.n_nodes = (int64_t)bvh.nodes.size();
     s->stats.n_bvh2_nodes = bvh.stats.bvh2_nodes;
     s->stats.depth = bvh.stats.depth;
-    s->stats.node_bytes = (int32_t)sizeof(Node8);
+    s->stats.node_bytes = (int32_t)sizeof(NodeSlot);
     s->stats.tri_bytes = (int32_t)sizeof(Tri48);
     s->stats.build_ms = bvh.stats.build_ms;
     s->stats.sah_cost = bvh.stats.sah_cost;
@@ -129,8 +141,9 @@ rise_hip_status rise_hip_scene_release(rise_hip_scene* scene)
 {
     clear_error();
     if (!scene) return RISE_HIP_OK;
-    hipFree(scene->d_nodes);
-    hipFree(scene->d_tris);
+    (void)hipFree(scene->d_nodes);
+    (void)hipFree(scene->d_tris);
+    (void)hipFree(scene->d_fetch);
     delete scene;
     return RISE_HIP_OK;
 }

@@ -40,14 +40,23 @@ constexpr double kSurfaceBackoff = 1e-12;                // Objects/Object.cpp:3
 // running best by it again) never rejects a box the exact ray enters.
 #define RISE_SLAB_PAD 1.0000019073486328f
 
+// The certifier's direction is d normalized in double exactly as
+// Vector3Ops::Normalize computes it (Object.cpp:414-425): dn_k = d_k * ov with
+// ov = 1/|d| (or 1 when |d| = 0).  Only ov is kept in registers; dn and the
+// double origin are rebuilt from the float inputs when a triangle is
+// certified, bit-identically.
 struct RayState {
     float ox, oy, oz;      // float origin (exact input)
+    float dx, dy, dz;      // direction as given
     float ix, iy, iz;      // 1/d with d == 0 -> FLT_MAX (BVH.h:1212-1225)
     unsigned oinv;         // (~octant) & 7
-    double o[3];           // certifier origin
-    double dn[3];          // certifier direction: d normalized in double (Object.cpp:414-425)
+    double ov;             // Normalize's 1/|d|
     double len_d;          // |d| in double
 };
+
+RISE_HD inline double dn0(const RayState& r) { return (double)r.dx * r.ov; }
+RISE_HD inline double dn1(const RayState& r) { return (double)r.dy * r.ov; }
+RISE_HD inline double dn2(const RayState& r) { return (double)r.dz * r.ov; }
 
 RISE_HD inline float safe_inv(float d)
 {
@@ -83,12 +92,13 @@ RISE_HD inline void init_ray(RayState& r, float ox, float oy, float oz, float dx
     r.iz = safe_inv(dz);
     const unsigned oct = (dx < 0.0f ? 1u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 4u : 0u);
     r.oinv = (~oct) & 7u;
-    r.o[0] = ox;
-    r.o[1] = oy;
-    r.o[2] = oz;
+    r.dx = dx;
+    r.dy = dy;
+    r.dz = dz;
     const double d[3] = {dx, dy, dz};
-    normalize_rise(d, r.dn);
-    r.len_d = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double mag = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    r.ov = mag > 0.0 ? 1.0 / mag : 1.0;
+    r.len_d = mag;
 }
 
 // FP64 Moller-Trumbore, RayTriangleIntersection.cpp:21-68 with the edges of
@@ -98,14 +108,14 @@ RISE_HD inline bool certify(const RayState& r, const float4& a, const float4& b,
     const double v0x = a.x, v0y = a.y, v0z = a.z;
     const double eAx = (double)b.x - v0x, eAy = (double)b.y - v0y, eAz = (double)b.z - v0z;
     const double eBx = (double)c.x - v0x, eBy = (double)c.y - v0y, eBz = (double)c.z - v0z;
-    const double* d = r.dn;
+    const double d[3] = {dn0(r), dn1(r), dn2(r)};
     const double p0 = d[1] * eBz - d[2] * eBy;
     const double p1 = d[2] * eBx - d[0] * eBz;
     const double p2 = d[0] * eBy - d[1] * eBx;
     const double det = eAx * p0 + eAy * p1 + eAz * p2;
     if (det > -kNearZero && det < kNearZero) return false;
     const double oodet = 1.0 / det;
-    const double t0 = r.o[0] - v0x, t1 = r.o[1] - v0y, t2 = r.o[2] - v0z;
+    const double t0 = (double)r.ox - v0x, t1 = (double)r.oy - v0y, t2 = (double)r.oz - v0z;
     const double al = (t0 * p0 + t1 * p1 + t2 * p2) * oodet;
     if ((al < 0.0 - kNearZero) || (al > 1.0 + kNearZero)) return false;
     const double q0 = t1 * eAz - t2 * eAy;
@@ -126,8 +136,9 @@ RISE_HD inline bool certify(const RayState& r, const float4& a, const float4& b,
 RISE_HD inline double world_range(const RayState& r, double t)
 {
     const double s = t - kSurfaceBackoff;
-    const double px = r.o[0] + r.dn[0] * s, py = r.o[1] + r.dn[1] * s, pz = r.o[2] + r.dn[2] * s;
-    const double vx = px - r.o[0], vy = py - r.o[1], vz = pz - r.o[2];
+    const double ox = r.ox, oy = r.oy, oz = r.oz;
+    const double px = ox + dn0(r) * s, py = oy + dn1(r) * s, pz = oz + dn2(r) * s;
+    const double vx = px - ox, vy = py - oy, vz = pz - oz;
     return sqrt(vx * vx + vy * vy + vz * vz);
 }
 
@@ -135,7 +146,7 @@ RISE_HD inline unsigned byte_of(unsigned w, int k) { return (w >> (8 * k)) & 0xf
 
 // Tests the eight children of one node.  Returns the permuted internal-hit
 // mask (bit k <-> slot k ^ oinv) and fills the node's leaf-triangle mask.
-RISE_HD inline unsigned test_node(const RayState& r, const Node8* __restrict__ nodes, unsigned ni, float tmax,
+RISE_HD inline unsigned test_node(const RayState& r, const NodeSlot* __restrict__ nodes, unsigned ni, float tmax,
                                      unsigned& child_base, unsigned& imask, unsigned& tri_base, unsigned& tmask)
 {
     const uint4* p = reinterpret_cast<const uint4*>(nodes + ni);
@@ -211,101 +222,164 @@ struct PrivStack {
     RISE_HD inline bool empty() const { return sp == 0; }
 };
 
+// Traversal state of one closest-hit query: the current node group (base,
+// 8-bit hit mask permuted by the ray octant, 8-bit internal mask), the best
+// certified hit and the box limit derived from it.
+struct ClosestState {
+    unsigned gbase, ghits, gimask;
+    unsigned best_leaf;
+    int best_id;
+    float tlim;
+    double best_t;
+};
+
+RISE_HD inline void closest_begin(const RayState& r, ClosestState& s)
+{
+    s.gbase = 0;
+    s.ghits = 1u << (0u ^ r.oinv);  // the root, in slot 0 of a virtual group
+    s.gimask = 1u;
+    s.best_leaf = 0xffffffffu;
+    s.best_id = -1;
+    s.tlim = INFINITY;
+    s.best_t = kRiseInfinity;
+}
+
+// One traversal step: visit the next node of the current group (near-to-far
+// by octant) and certify the triangles of its leaf children; when the group
+// is used up, pop the next one.  Returns false once the traversal is over.
+// Closest hit: smallest t, exact ties to the lowest caller id (the
+// reference's strict-closer guard, ...Specializations.h:172, plus the tie rule).
+template <bool COUNT, class Stack>
+RISE_HD inline bool closest_step(const RayState& r, const NodeSlot* __restrict__ nodes,
+                                 const Tri48* __restrict__ tris, Stack& st, ClosestState& s, unsigned& n_nodes,
+                                 unsigned& n_tris)
+{
+    unsigned tmask = 0, tbase = 0;
+    if (s.ghits) {
+        const unsigned k = __builtin_ctz(s.ghits);
+        s.ghits &= s.ghits - 1u;
+        const unsigned slot = k ^ r.oinv;
+        const unsigned ni = s.gbase + __builtin_popcount(s.gimask & ((1u << slot) - 1u));
+        if (s.ghits) st.push(make_uint2(s.gbase, s.ghits | (s.gimask << 8)));
+        if (COUNT) ++n_nodes;
+        unsigned cb, im;
+        s.ghits = test_node(r, nodes, ni, s.tlim, cb, im, tbase, tmask);
+        s.gbase = cb;
+        s.gimask = im;
+    }
+    while (tmask) {
+        const unsigned j = __builtin_ctz(tmask);
+        tmask &= tmask - 1u;
+        const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        if (COUNT) ++n_tris;
+        double tt;
+        if (certify(r, a, b, c, tt)) {
+            const int id = float_to_bits(a.w);
+            if (tt < s.best_t || (tt == s.best_t && id < s.best_id)) {
+                s.best_t = tt;
+                s.best_id = id;
+                s.best_leaf = tbase + j;
+                s.tlim = box_limit(r, s.best_t);
+            }
+        }
+    }
+    if (!s.ghits) {
+        if (st.empty()) return false;
+        const uint2 g = st.pop();
+        s.gbase = g.x;
+        s.ghits = g.y & 0xffu;
+        s.gimask = (g.y >> 8) & 0xffu;
+    }
+    return true;
+}
+
 // Closest-hit traversal.  Returns the caller id (-1 on miss), the best
 // parametric distance along dn in best_t and the leaf-order triangle index
 // of the hit in best_leaf.
 template <bool COUNT, class Stack>
-RISE_HD inline int trace_closest(const RayState& r, const Node8* __restrict__ nodes,
-                                    const Tri48* __restrict__ tris, Stack& st, double& best_t,
-                                    unsigned& best_leaf, unsigned& n_nodes, unsigned& n_tris)
+RISE_HD inline int trace_closest(const RayState& r, const NodeSlot* __restrict__ nodes,
+                                 const Tri48* __restrict__ tris, Stack& st, double& best_t, unsigned& best_leaf,
+                                 unsigned& n_nodes, unsigned& n_tris)
 {
-    best_t = kRiseInfinity;
-    int best_id = -1;
-    best_leaf = 0xffffffffu;
-    float tlim = INFINITY;
-    unsigned gbase = 0, ghits = 1u << (0u ^ r.oinv), gimask = 1u;
-    for (;;) {
-        unsigned tmask = 0, tbase = 0;
-        if (ghits) {
-            const unsigned k = __builtin_ctz(ghits);
-            ghits &= ghits - 1u;
-            const unsigned slot = k ^ r.oinv;
-            const unsigned ni = gbase + __builtin_popcount(gimask & ((1u << slot) - 1u));
-            if (ghits) st.push(make_uint2(gbase, ghits | (gimask << 8)));
-            if (COUNT) ++n_nodes;
-            unsigned cb, im;
-            ghits = test_node(r, nodes, ni, tlim, cb, im, tbase, tmask);
-            gbase = cb;
-            gimask = im;
-        }
-        while (tmask) {
-            const unsigned j = __builtin_ctz(tmask);
-            tmask &= tmask - 1u;
-            const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
-            const float4 a = tp[0], b = tp[1], c = tp[2];
-            if (COUNT) ++n_tris;
-            double tt;
-            if (certify(r, a, b, c, tt)) {
-                const int id = float_to_bits(a.w);
-                if (tt < best_t || (tt == best_t && id < best_id)) {
-                    best_t = tt;
-                    best_id = id;
-                    best_leaf = tbase + j;
-                    tlim = box_limit(r, best_t);
-                }
-            }
-        }
-        if (!ghits) {
-            if (st.empty()) break;
-            const uint2 g = st.pop();
-            gbase = g.x;
-            ghits = g.y & 0xffu;
-            gimask = (g.y >> 8) & 0xffu;
-        }
+    ClosestState s;
+    closest_begin(r, s);
+    while (closest_step<COUNT>(r, nodes, tris, st, s, n_nodes, n_tris)) {
     }
-    return best_id;
+    best_t = s.best_t;
+    best_leaf = s.best_leaf;
+    return s.best_id;
 }
 
-// Any-hit traversal: true iff some triangle certifies with
-// kNearZero < t < how_far2 (...Specializations.h:517-519).
-template <bool COUNT, class Stack>
-RISE_HD inline bool trace_any(const RayState& r, const Node8* __restrict__ nodes, const Tri48* __restrict__ tris,
-                                 Stack& st, double how_far2, unsigned& n_nodes, unsigned& n_tris)
+// Any-hit state: the current group, the box limit and the result.
+struct AnyState {
+    unsigned gbase, ghits, gimask;
+    bool hit;
+    float tlim;
+    double how_far2;
+};
+
+RISE_HD inline void any_begin(const RayState& r, double how_far2, AnyState& s)
 {
-    const float tlim = box_limit(r, how_far2);
-    unsigned gbase = 0, ghits = 1u << (0u ^ r.oinv), gimask = 1u;
-    for (;;) {
-        unsigned tmask = 0, tbase = 0;
-        if (ghits) {
-            const unsigned k = __builtin_ctz(ghits);
-            ghits &= ghits - 1u;
-            const unsigned slot = k ^ r.oinv;
-            const unsigned ni = gbase + __builtin_popcount(gimask & ((1u << slot) - 1u));
-            if (ghits) st.push(make_uint2(gbase, ghits | (gimask << 8)));
-            if (COUNT) ++n_nodes;
-            unsigned cb, im;
-            ghits = test_node(r, nodes, ni, tlim, cb, im, tbase, tmask);
-            gbase = cb;
-            gimask = im;
-        }
-        while (tmask) {
-            const unsigned j = __builtin_ctz(tmask);
-            tmask &= tmask - 1u;
-            const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
-            const float4 a = tp[0], b = tp[1], c = tp[2];
-            if (COUNT) ++n_tris;
-            double tt;
-            if (certify(r, a, b, c, tt) && tt > kNearZero && tt < how_far2) return true;
-        }
-        if (!ghits) {
-            if (st.empty()) break;
-            const uint2 g = st.pop();
-            gbase = g.x;
-            ghits = g.y & 0xffu;
-            gimask = (g.y >> 8) & 0xffu;
+    s.gbase = 0;
+    s.ghits = 1u << (0u ^ r.oinv);
+    s.gimask = 1u;
+    s.hit = false;
+    s.how_far2 = how_far2;
+    s.tlim = box_limit(r, how_far2);
+}
+
+// One any-hit step; true iff some triangle certifies with
+// kNearZero < t < how_far2 (...Specializations.h:517-519).  Returns false once
+// the query is decided (a hit, or the tree exhausted).
+template <bool COUNT, class Stack>
+RISE_HD inline bool any_step(const RayState& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
+                             Stack& st, AnyState& s, unsigned& n_nodes, unsigned& n_tris)
+{
+    unsigned tmask = 0, tbase = 0;
+    if (s.ghits) {
+        const unsigned k = __builtin_ctz(s.ghits);
+        s.ghits &= s.ghits - 1u;
+        const unsigned slot = k ^ r.oinv;
+        const unsigned ni = s.gbase + __builtin_popcount(s.gimask & ((1u << slot) - 1u));
+        if (s.ghits) st.push(make_uint2(s.gbase, s.ghits | (s.gimask << 8)));
+        if (COUNT) ++n_nodes;
+        unsigned cb, im;
+        s.ghits = test_node(r, nodes, ni, s.tlim, cb, im, tbase, tmask);
+        s.gbase = cb;
+        s.gimask = im;
+    }
+    while (tmask) {
+        const unsigned j = __builtin_ctz(tmask);
+        tmask &= tmask - 1u;
+        const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        if (COUNT) ++n_tris;
+        double tt;
+        if (certify(r, a, b, c, tt) && tt > kNearZero && tt < s.how_far2) {
+            s.hit = true;
+            return false;
         }
     }
-    return false;
+    if (!s.ghits) {
+        if (st.empty()) return false;
+        const uint2 g = st.pop();
+        s.gbase = g.x;
+        s.ghits = g.y & 0xffu;
+        s.gimask = (g.y >> 8) & 0xffu;
+    }
+    return true;
+}
+
+template <bool COUNT, class Stack>
+RISE_HD inline bool trace_any(const RayState& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
+                              Stack& st, double how_far2, unsigned& n_nodes, unsigned& n_tris)
+{
+    AnyState s;
+    any_begin(r, how_far2, s);
+    while (any_step<COUNT>(r, nodes, tris, st, s, n_nodes, n_tris)) {
+    }
+    return s.hit;
 }
 
 // Object.cpp:472-493: the world limit dHowFar becomes dHowFar2 = |d| * dHowFar

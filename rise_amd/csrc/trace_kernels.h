@@ -10,9 +10,10 @@
 namespace rise_hip {
 
 struct TraceScene {
-    const Node8* nodes;
+    const NodeSlot* nodes;
     const Tri48* tris;
     int depth;  // wide-node levels; selects the stack variant
+    unsigned long long* fetch_counter;  // device counter for dynamic ray fetch (one per in-flight call)
 };
 
 // Closest hit: rays SoA (ox, oy, oz, dx, dy, dz), n each.  counters: NULL, or

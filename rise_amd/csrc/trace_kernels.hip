@@ -8,12 +8,19 @@
 #include "trace_core.h"
 #include "trace_kernels.h"
 
+#include <algorithm>
+
 namespace rise_hip {
 
 namespace {
 
-constexpr int kBlock = 128;
-constexpr int kLdsStack = 16;
+#ifndef RISE_PERSISTENT
+#define RISE_PERSISTENT 1
+#endif
+#ifndef RISE_TRACE_BLOCK
+#define RISE_TRACE_BLOCK 128
+#endif
+constexpr int kBlock = RISE_TRACE_BLOCK;
 
 __device__ inline void load_ray(const float* __restrict__ rays, int64_t n, int64_t i, float& ox, float& oy, float& oz,
                                 float& dx, float& dy, float& dz)
@@ -35,14 +42,15 @@ __device__ inline void flush_counts(unsigned long long* counters, unsigned nn, u
     }
 }
 
-template <bool COUNT, bool LDS>
-__global__ __launch_bounds__(kBlock) void closest_kernel(const Node8* __restrict__ nodes,
+// STACK: node-group entries per lane held in LDS; 0 selects a private array.
+template <bool COUNT, int STACK>
+__global__ __launch_bounds__(kBlock) void closest_kernel(const NodeSlot* __restrict__ nodes,
                                                          const Tri48* __restrict__ tris,
                                                          const float* __restrict__ rays, int64_t n,
                                                          int32_t* __restrict__ prim_id, float* __restrict__ t_out,
                                                          unsigned long long* counters)
 {
-    __shared__ uint2 lds[LDS ? kLdsStack * kBlock : 1];
+    __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     float ox, oy, oz, dx, dy, dz;
@@ -53,8 +61,8 @@ __global__ __launch_bounds__(kBlock) void closest_kernel(const Node8* __restrict
     double best;
     unsigned leaf;
     int id;
-    if (LDS) {
-        LdsStack<kLdsStack> st{lds + threadIdx.x, kBlock, 0};
+    if (STACK > 0) {
+        LdsStack<STACK> st{lds + threadIdx.x, kBlock, 0};
         id = trace_closest<COUNT>(r, nodes, tris, st, best, leaf, nn, nt);
     } else {
         PrivStack st;
@@ -66,13 +74,13 @@ __global__ __launch_bounds__(kBlock) void closest_kernel(const Node8* __restrict
     flush_counts<COUNT>(counters, nn, nt);
 }
 
-template <bool COUNT, bool LDS>
-__global__ __launch_bounds__(kBlock) void any_kernel(const Node8* __restrict__ nodes, const Tri48* __restrict__ tris,
+template <bool COUNT, int STACK>
+__global__ __launch_bounds__(kBlock) void any_kernel(const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
                                                      const float* __restrict__ rays,
                                                      const float* __restrict__ tmax, int64_t n,
                                                      uint8_t* __restrict__ occluded, unsigned long long* counters)
 {
-    __shared__ uint2 lds[LDS ? kLdsStack * kBlock : 1];
+    __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     float ox, oy, oz, dx, dy, dz;
@@ -83,8 +91,8 @@ __global__ __launch_bounds__(kBlock) void any_kernel(const Node8* __restrict__ n
     const double how_far2 = scale_how_far(r, how_far);
     unsigned nn = 0, nt = 0;
     bool occ;
-    if (LDS) {
-        LdsStack<kLdsStack> st{lds + threadIdx.x, kBlock, 0};
+    if (STACK > 0) {
+        LdsStack<STACK> st{lds + threadIdx.x, kBlock, 0};
         occ = trace_any<COUNT>(r, nodes, tris, st, how_far2, nn, nt);
     } else {
         PrivStack st;
@@ -97,30 +105,175 @@ __global__ __launch_bounds__(kBlock) void any_kernel(const Node8* __restrict__ n
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+#ifndef RISE_REFILL
+#define RISE_REFILL 32
+#endif
+
+__device__ inline unsigned long long shfl_u64(unsigned long long v, int src)
+{
+    const unsigned lo = (unsigned)__shfl((int)(unsigned)v, src);
+    const unsigned hi = (unsigned)__shfl((int)(unsigned)(v >> 32), src);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Wave-wide dynamic ray fetch (persistent waves): lanes whose ray is done
+// take new ray indices from a global counter, in one atomic per wave, once
+// at least RISE_REFILL lanes are idle or no lane is busy.  A wave exits when
+// the counter is past the last ray and none of its lanes holds a ray.
+// Returns the new ray index for an idle lane, -1 if none was taken, -2 once
+// the rays are exhausted for this lane.
+__device__ inline long long refill(bool idle, bool busy, unsigned long long* next, int64_t n)
+{
+    const unsigned long long im = __ballot(idle);
+    const unsigned long long bm = __ballot(busy);
+    const int nidle = __popcll(im);
+    if (nidle == 0 || (nidle < RISE_REFILL && bm != 0ull)) return -1;
+    const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(im >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)im, 0u));
+    const int leader = __ffsll((unsigned long long)im) - 1;
+    unsigned long long base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(next, (unsigned long long)nidle);
+    base = shfl_u64(base, leader);
+    if (!idle) return -1;
+    const unsigned long long idx = base + rank;
+    return idx < (unsigned long long)n ? (long long)idx : -2;
+}
+
+template <bool COUNT, int STACK>
+__global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __restrict__ nodes,
+                                                             const Tri48* __restrict__ tris,
+                                                             const float* __restrict__ rays, int64_t n,
+                                                             int32_t* __restrict__ prim_id,
+                                                             float* __restrict__ t_out, unsigned long long* counters,
+                                                             unsigned long long* next)
+{
+    __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
+    LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
+    PrivStack pst;
+    pst.sp = 0;
+    RayState r;
+    ClosestState s;
+    long long ray = -1;  // -1 idle, -2 exhausted
+    unsigned nn = 0, nt = 0;
+    for (;;) {
+        const long long got = refill(ray == -1, ray >= 0, next, n);
+        if (ray == -1 && got != -1) {
+            ray = got;
+            if (ray >= 0) {
+                float ox, oy, oz, dx, dy, dz;
+                load_ray(rays, n, ray, ox, oy, oz, dx, dy, dz);
+                init_ray(r, ox, oy, oz, dx, dy, dz);
+                closest_begin(r, s);
+                lst.sp = 0;
+                pst.sp = 0;
+            }
+        }
+        if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
+        if (ray >= 0) {
+            const bool more = STACK > 0 ? closest_step<COUNT>(r, nodes, tris, lst, s, nn, nt)
+                                        : closest_step<COUNT>(r, nodes, tris, pst, s, nn, nt);
+            if (!more) {
+                prim_id[ray] = s.best_id;
+                t_out[ray] = s.best_id >= 0 ? (float)world_range(r, s.best_t) : INFINITY;
+                ray = -1;
+            }
+        }
+    }
+    flush_counts<COUNT>(counters, nn, nt);
+}
+
+template <bool COUNT, int STACK>
+__global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restrict__ nodes,
+                                                         const Tri48* __restrict__ tris,
+                                                         const float* __restrict__ rays,
+                                                         const float* __restrict__ tmax, int64_t n,
+                                                         uint8_t* __restrict__ occluded, unsigned long long* counters,
+                                                         unsigned long long* next)
+{
+    __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
+    LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
+    PrivStack pst;
+    pst.sp = 0;
+    RayState r;
+    AnyState s;
+    long long ray = -1;
+    unsigned nn = 0, nt = 0;
+    for (;;) {
+        const long long got = refill(ray == -1, ray >= 0, next, n);
+        if (ray == -1 && got != -1) {
+            ray = got;
+            if (ray >= 0) {
+                float ox, oy, oz, dx, dy, dz;
+                load_ray(rays, n, ray, ox, oy, oz, dx, dy, dz);
+                init_ray(r, ox, oy, oz, dx, dy, dz);
+                const double how_far = tmax ? (double)tmax[ray] : kRiseInfinity;
+                any_begin(r, scale_how_far(r, how_far), s);
+                lst.sp = 0;
+                pst.sp = 0;
+            }
+        }
+        if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
+        if (ray >= 0) {
+            const bool more = STACK > 0 ? any_step<COUNT>(r, nodes, tris, lst, s, nn, nt)
+                                        : any_step<COUNT>(r, nodes, tris, pst, s, nn, nt);
+            if (!more) {
+                occluded[ray] = s.hit ? 1 : 0;
+                ray = -1;
+            }
+        }
+    }
+    flush_counts<COUNT>(counters, nn, nt);
+}
+
+__global__ void reset_counter(unsigned long long* c) { *c = 0ull; }
+
+// Waves resident at once for a persistent kernel: CUs x blocks per CU.
+template <class K>
+unsigned persistent_grid(K kernel, int64_t n)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    const int64_t want = (int64_t)cus * per_cu;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)grid_for(n)));
+}
+
 }  // namespace
 
+// The node-group stack holds at most one entry per level below the root, so
+// the smallest LDS stack that fits the tree keeps occupancy highest (LDS is
+// what limits waves per SIMD here).
 hipError_t launch_closest(const TraceScene& s, const float* rays, int64_t n, int32_t* prim_id, float* t,
                           unsigned long long* counters, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    // The node-group stack holds at most one entry per level below the root.
-    const bool lds = s.depth - 1 <= kLdsStack;
+    const int need = s.depth - 1;
     const dim3 grid(grid_for(n)), block(kBlock);
+#if RISE_PERSISTENT
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
+#define RISE_LAUNCH_CLOSEST(C, S)                                                                               \
+    hipLaunchKernelGGL((closest_persistent<C, S>), dim3(persistent_grid(closest_persistent<C, S>, n)), block, 0, \
+                       stream, s.nodes, s.tris, rays, n, prim_id, t, counters, s.fetch_counter)
+#else
+#define RISE_LAUNCH_CLOSEST(C, S) \
+    hipLaunchKernelGGL((closest_kernel<C, S>), grid, block, 0, stream, s.nodes, s.tris, rays, n, prim_id, t, counters)
+#endif
     if (counters) {
-        if (lds)
-            hipLaunchKernelGGL((closest_kernel<true, true>), grid, block, 0, stream, s.nodes, s.tris, rays, n, prim_id, t,
-                               counters);
-        else
-            hipLaunchKernelGGL((closest_kernel<true, false>), grid, block, 0, stream, s.nodes, s.tris, rays, n, prim_id,
-                               t, counters);
+        if (need <= 8) RISE_LAUNCH_CLOSEST(true, 8);
+        else if (need <= 16) RISE_LAUNCH_CLOSEST(true, 16);
+        else RISE_LAUNCH_CLOSEST(true, 0);
     } else {
-        if (lds)
-            hipLaunchKernelGGL((closest_kernel<false, true>), grid, block, 0, stream, s.nodes, s.tris, rays, n, prim_id,
-                               t, counters);
-        else
-            hipLaunchKernelGGL((closest_kernel<false, false>), grid, block, 0, stream, s.nodes, s.tris, rays, n,
-                               prim_id, t, counters);
+        if (need <= 8) RISE_LAUNCH_CLOSEST(false, 8);
+        else if (need <= 16) RISE_LAUNCH_CLOSEST(false, 16);
+        else RISE_LAUNCH_CLOSEST(false, 0);
     }
+#undef RISE_LAUNCH_CLOSEST
     return hipGetLastError();
 }
 
@@ -128,23 +281,27 @@ hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax,
                       unsigned long long* counters, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    const bool lds = s.depth - 1 <= kLdsStack;
+    const int need = s.depth - 1;
     const dim3 grid(grid_for(n)), block(kBlock);
+#if RISE_PERSISTENT
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
+#define RISE_LAUNCH_ANY(C, S)                                                                                    \
+    hipLaunchKernelGGL((any_persistent<C, S>), dim3(persistent_grid(any_persistent<C, S>, n)), block, 0, stream, \
+                       s.nodes, s.tris, rays, tmax, n, occluded, counters, s.fetch_counter)
+#else
+#define RISE_LAUNCH_ANY(C, S) \
+    hipLaunchKernelGGL((any_kernel<C, S>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n, occluded, counters)
+#endif
     if (counters) {
-        if (lds)
-            hipLaunchKernelGGL((any_kernel<true, true>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n, occluded,
-                               counters);
-        else
-            hipLaunchKernelGGL((any_kernel<true, false>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n,
-                               occluded, counters);
+        if (need <= 8) RISE_LAUNCH_ANY(true, 8);
+        else if (need <= 16) RISE_LAUNCH_ANY(true, 16);
+        else RISE_LAUNCH_ANY(true, 0);
     } else {
-        if (lds)
-            hipLaunchKernelGGL((any_kernel<false, true>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n,
-                               occluded, counters);
-        else
-            hipLaunchKernelGGL((any_kernel<false, false>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n,
-                               occluded, counters);
+        if (need <= 8) RISE_LAUNCH_ANY(false, 8);
+        else if (need <= 16) RISE_LAUNCH_ANY(false, 16);
+        else RISE_LAUNCH_ANY(false, 0);
     }
+#undef RISE_LAUNCH_ANY
     return hipGetLastError();
 }
 
