@@ -202,7 +202,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(PtSceneView sc, PtParams
     }
 }
 
-// One bounce of IntegrateFromHitHWSS for the active queue.
+// One bounce of IntegrateFromHitHWSS for the active queue.  Every lane of a
+// wave runs the same loop trips (the queue bound is uniform per block) so the
+// compaction ballots see all lanes; each shadow record is written as soon as
+// it is made, which keeps one record live at a time.
 __global__ __launch_bounds__(kShadeBlock) void k_shade(PtSceneView sc, PtParams P, Paths S, int depth)
 {
     const unsigned n = S.cnt_ext[depth];
@@ -212,99 +215,98 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(PtSceneView sc, PtParams 
     unsigned* cnt_sh = S.cnt_sh + depth;
     const unsigned base_dim = 32u * (unsigned)(16 + depth);
     const unsigned stride = gridDim.x * blockDim.x;
-    // The loop bound is uniform per wave so every lane reaches the ballots.
     for (unsigned b = blockIdx.x * blockDim.x; b < n; b += stride) {
         const unsigned i = b + threadIdx.x;
-        const bool active = i < n;
-        unsigned slot = 0;
-        bool cont = false;
-        bool sh[kHwss] = {false, false, false, false};
-        float sval[kHwss] = {0, 0, 0, 0};
-        f3 sorg[kHwss], sdir[kHwss];
-        float stm[kHwss] = {0, 0, 0, 0};
-        f3 new_o = mk3(0, 0, 0), new_d = mk3(0, 0, 0);
-        if (active) {
+        unsigned slot = 0, leaf = 0xffffffffu;
+        if (i < n) {
             slot = qin[i];
-            const unsigned leaf = S.hit_leaf[slot];
-            if (leaf != 0xffffffffu) {
-                const unsigned path = S.path_id[slot];
-                const f3 o = mk3(S.ox[slot], S.oy[slot], S.oz[slot]);
-                const f3 d = mk3(S.dx[slot], S.dy[slot], S.dz[slot]);
-                float thr[kHwss], res[kHwss], lam[kHwss];
-                for (int w = 0; w < kHwss; ++w) {
-                    thr[w] = S.thr[w * P.cap + slot];
-                    res[w] = S.res[w * P.cap + slot];
-                }
-                wavelengths(S.lambda0[slot], P.lambda_begin, P.lambda_end, lam);
+            leaf = S.hit_leaf[slot];
+        }
+        const bool valid = leaf != 0xffffffffu;  // a miss ends the path (no environment)
+
+        // Hit record (TriangleMeshGeometryIndexed + Object layers).
+        Surf h;
+        DevObject ob{};
+        DevMaterial mat{};
+        unsigned path = 0;
+        f3 o = mk3(0, 0, 0);
+        float thr[kHwss] = {0, 0, 0, 0}, lam[kHwss] = {0, 0, 0, 0};
+        if (valid) {
+            path = S.path_id[slot];
+            o = mk3(S.ox[slot], S.oy[slot], S.oz[slot]);
+            const f3 d = mk3(S.dx[slot], S.dy[slot], S.dz[slot]);
+            for (int w = 0; w < kHwss; ++w) thr[w] = S.thr[w * P.cap + slot];
+            wavelengths(S.lambda0[slot], P.lambda_begin, P.lambda_end, lam);
+            const float4* tp = reinterpret_cast<const float4*>(sc.tris + leaf);
+            const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+            const f3 v0 = mk3(ta.x, ta.y, ta.z), v1 = mk3(tb.x, tb.y, tb.z), v2 = mk3(tc.x, tc.y, tc.z);
+            h.object = float_to_bits(tb.w);
+            ob = sc.objects[h.object];
+            h.material = ob.material;
+            mat = sc.materials[ob.material];
+            h.p = o + d * S.hit_t[slot];
+            h.dir = d;
+            const f3 fn = norm3(cross3(v1 - v0, v2 - v0));
+            h.n = fn;
+            h.ng = fn;
+            if (ob.double_sided) {  // TriangleMeshGeometryIndexed.cpp:171-188
+                if (dot3(h.n, d) > 0.0f) h.n = -h.n;
+                if (dot3(h.ng, d) > 0.0f) h.ng = -h.ng;
+            }
+            h.frame = onb_from_w(h.n);
+
+            // Emission with MIS against NEE (PathTracingIntegrator.cpp:4422-4515).
+            if (mat.emit_spd >= 0) {
                 const float bsdf_pdf = S.pdf[slot];
-
-                // Hit record (TriangleMeshGeometryIndexed + Object layers).
-                const float4* tp = reinterpret_cast<const float4*>(sc.tris + leaf);
-                const float4 ta = tp[0], tb = tp[1], tc = tp[2];
-                const f3 v0 = mk3(ta.x, ta.y, ta.z), v1 = mk3(tb.x, tb.y, tb.z), v2 = mk3(tc.x, tc.y, tc.z);
-                Surf h;
-                h.object = float_to_bits(tb.w);
-                const DevObject ob = sc.objects[h.object];
-                h.material = ob.material;
-                const DevMaterial mat = sc.materials[ob.material];
-                const float t = S.hit_t[slot];
-                h.p = o + d * t;
-                h.dir = d;
-                const f3 fn = norm3(cross3(v1 - v0, v2 - v0));
-                h.n = fn;
-                h.ng = fn;
-                if (ob.double_sided) {  // TriangleMeshGeometryIndexed.cpp:171-188
-                    if (dot3(h.n, d) > 0.0f) h.n = -h.n;
-                    if (dot3(h.ng, d) > 0.0f) h.ng = -h.ng;
-                }
-                h.frame = onb_from_w(h.n);
-
-                // Emission with MIS against NEE (PathTracingIntegrator.cpp:4422-4515).
-                if (mat.emit_spd >= 0) {
-                    float mis = 1.0f;
-                    if (bsdf_pdf > 0.0f && ob.luminary >= 0) {
-                        const DevLuminary L = sc.lums[ob.luminary];
-                        const float cos_l = fabsf(dot3(d, h.ng));
-                        if (L.area > 0.0f && cos_l > 0.0f) {
-                            const float dist = len3(h.p - o);
-                            const float ps = L.pdf_select > 0.0f ? L.pdf_select : 1.0f;
-                            const float p_nee = ps * (dist * dist) / (L.area * cos_l);
-                            mis = power_heuristic(bsdf_pdf, p_nee);
-                        }
+                float mis = 1.0f;
+                if (bsdf_pdf > 0.0f && ob.luminary >= 0) {
+                    const DevLuminary L = sc.lums[ob.luminary];
+                    const float cos_l = fabsf(dot3(d, h.ng));
+                    if (L.area > 0.0f && cos_l > 0.0f) {
+                        const float dist = len3(h.p - o);
+                        const float ps = L.pdf_select > 0.0f ? L.pdf_select : 1.0f;
+                        const float p_nee = ps * (dist * dist) / (L.area * cos_l);
+                        mis = power_heuristic(bsdf_pdf, p_nee);
                     }
-                    for (int w = 0; w < kHwss; ++w) res[w] += thr[w] * (emitted(sc, mat, -d, h.ng, lam[w]) * mis);
                 }
+                for (int w = 0; w < kHwss; ++w)
+                    S.res[w * P.cap + slot] += thr[w] * (emitted(sc, mat, -d, h.ng, lam[w]) * mis);
+            }
+        }
 
-                // Next-event estimation, one light sample per wavelength
-                // (PathTracingIntegrator.cpp:4521-4552 -> LightSampler::EvaluateDirectLightingNM,
-                // Lights/LightSampler.cpp:1987-2269).
-                if (sc.n_lums > 0) {
-                    for (int w = 0; w < kHwss; ++w) {
-                        const unsigned dim = base_dim + 4u * (unsigned)w;
-                        const float usel = rng_u(P.seed, path, dim);
-                        int li = 0;
-                        while (li < sc.n_lums - 1 && !(usel < sc.lums[li].cdf_select)) ++li;
-                        if (li == ob.luminary) continue;  // the shading object itself (:2061)
-                        const DevLuminary L = sc.lums[li];
-                        const float ux = rng_u(P.seed, path, dim + 1), uy = rng_u(P.seed, path, dim + 2),
-                                    uz = rng_u(P.seed, path, dim + 3);
-                        // UniformRandomPoint: first CDF entry >= uz, then PointOnTriangle
-                        // (TriangleMeshGeometryIndexed.cpp:207-218, GeometricUtilities.cpp:543-560).
-                        int k = 0;
-                        while (k < L.tri_count - 1 && sc.lum_cdf[L.tri_first + k] < uz) ++k;
-                        const float4* lp = reinterpret_cast<const float4*>(sc.tris + sc.lum_tris[L.tri_first + k]);
-                        const float4 la = lp[0], lb = lp[1], lc = lp[2];
-                        const f3 a0 = mk3(la.x, la.y, la.z), a1 = mk3(lb.x, lb.y, lb.z), a2 = mk3(lc.x, lc.y, lc.z);
-                        const float sa = sqrtf(1.0f - ux);
-                        const float al = 1.0f - sa, be = sa * uy, ga = 1.0f - al - be;
-                        const f3 pl = a1 * al + a2 * be + a0 * ga;
-                        const f3 ln = norm3(cross3(a1 - a0, a2 - a0));
-                        f3 to = pl - h.p;
-                        const float dist = len3(to);
-                        if (dist > 0.0f) to = to * (1.0f / dist);
-                        const float cos_s = dot3(to, h.n);
-                        const float cos_l = dot3(-to, ln);
-                        if (cos_l <= 0.0f || cos_s <= 0.0f) continue;
+        // Next-event estimation, one light sample per wavelength
+        // (PathTracingIntegrator.cpp:4521-4552 -> LightSampler::EvaluateDirectLightingNM,
+        // Lights/LightSampler.cpp:1987-2269).
+        for (int w = 0; w < kHwss; ++w) {
+            bool has = false;
+            float v = 0.0f, stm = 0.0f;
+            f3 sorg = mk3(0, 0, 0), to = mk3(0, 0, 0);
+            if (valid && sc.n_lums > 0) {
+                const unsigned dim = base_dim + 4u * (unsigned)w;
+                const float usel = rng_u(P.seed, path, dim);
+                int li = 0;
+                while (li < sc.n_lums - 1 && !(usel < sc.lums[li].cdf_select)) ++li;
+                if (li != ob.luminary) {  // never the shading object itself (:2061)
+                    const DevLuminary L = sc.lums[li];
+                    const float ux = rng_u(P.seed, path, dim + 1), uy = rng_u(P.seed, path, dim + 2),
+                                uz = rng_u(P.seed, path, dim + 3);
+                    // UniformRandomPoint: first CDF entry >= uz, then PointOnTriangle
+                    // (TriangleMeshGeometryIndexed.cpp:207-218, GeometricUtilities.cpp:543-560).
+                    int k = 0;
+                    while (k < L.tri_count - 1 && sc.lum_cdf[L.tri_first + k] < uz) ++k;
+                    const float4* lp = reinterpret_cast<const float4*>(sc.tris + sc.lum_tris[L.tri_first + k]);
+                    const float4 la = lp[0], lb = lp[1], lc = lp[2];
+                    const f3 a0 = mk3(la.x, la.y, la.z), a1 = mk3(lb.x, lb.y, lb.z), a2 = mk3(lc.x, lc.y, lc.z);
+                    const float sa = sqrtf(1.0f - ux);
+                    const float al = 1.0f - sa, be = sa * uy, ga = 1.0f - al - be;
+                    const f3 pl = a1 * al + a2 * be + a0 * ga;
+                    const f3 ln = norm3(cross3(a1 - a0, a2 - a0));
+                    to = pl - h.p;
+                    const float dist = len3(to);
+                    if (dist > 0.0f) to = to * (1.0f / dist);
+                    const float cos_s = dot3(to, h.n);
+                    const float cos_l = dot3(-to, ln);
+                    if (cos_l > 0.0f && cos_s > 0.0f) {
                         const DevMaterial lm = sc.materials[L.material];
                         const float le = emitted(sc, lm, -to, ln, lam[w]);
                         const float geom = L.area * cos_l / (dist * dist);
@@ -313,106 +315,103 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(PtSceneView sc, PtParams 
                         const float p_bsdf = lambert_pdf(h, to);
                         if (p_bsdf > 0.0f) contrib *= power_heuristic(p_light, p_bsdf);
                         contrib /= L.pdf_select;
-                        const float v = thr[w] * contrib;
+                        v = thr[w] * contrib;
                         if (v > 0.0f) {
-                            sh[w] = true;
-                            sval[w] = v;
-                            sorg[w] = offset_ray(h.p, dot3(to, h.ng) > 0.0f ? h.ng : -h.ng);
-                            sdir[w] = to;
-                            stm[w] = dist - 0.001f;  // LightSampler.cpp:2206
+                            has = true;
+                            sorg = offset_ray(h.p, dot3(to, h.ng) > 0.0f ? h.ng : -h.ng);
+                            stm = dist - 0.001f;  // LightSampler.cpp:2206
                         }
                     }
                 }
+            }
+            const unsigned si = wave_push(cnt_sh, has);
+            if (has) {
+                S.sox[si] = sorg.x;
+                S.soy[si] = sorg.y;
+                S.soz[si] = sorg.z;
+                S.sdx[si] = to.x;
+                S.sdy[si] = to.y;
+                S.sdz[si] = to.z;
+                S.stmax[si] = stm;
+                S.sval[si] = v;
+                S.sslot[si] = slot * 4u + (unsigned)w;
+            }
+        }
 
-                // LambertianSPF::ScatterNM at the hero wavelength (Materials/LambertianSPF.cpp:90-131).
-                const float ua = rng_u(P.seed, path, base_dim + 16), ub = rng_u(P.seed, path, base_dim + 17);
-                Onb fr = h.frame;
-                const bool back = dot3(d, h.frame.w) > kNearZeroF;
-                if (back) {
-                    fr.w = -fr.w;
-                    fr.u = -fr.u;
+        // LambertianSPF::ScatterNM at the hero wavelength (Materials/LambertianSPF.cpp:90-131),
+        // companions, Russian roulette, bounce caps.
+        bool cont = false;
+        f3 new_o = mk3(0, 0, 0), nd = mk3(0, 0, 0);
+        if (valid) {
+            const float ua = rng_u(P.seed, path, base_dim + 16), ub = rng_u(P.seed, path, base_dim + 17);
+            Onb fr = h.frame;
+            const bool back = dot3(h.dir, h.frame.w) > kNearZeroF;
+            if (back) {
+                fr.w = -fr.w;
+                fr.u = -fr.u;
+            }
+            const float phi = kTwoPi * ua, st = sqrtf(ub), ct = sqrtf(1.0f - ub);
+            nd = onb_xform(fr, mk3(cosf(phi) * st, sinf(phi) * st, ct));
+            const float pdf = fabsf(dot3(nd, h.frame.w)) * kInvPi;
+            const f3 neff = back ? -h.frame.w : h.frame.w;
+            const f3 g = geom_against_ray(h, neff);
+            if (dot3(nd, g) > 0.0f) {
+                float sc4[kHwss];
+                sc4[0] = spd_value(sc, mat.refl_spd, lam[0]);
+                for (int w = 1; w < kHwss; ++w) {
+                    // Companion weight: BSDF value * |cos| / pdf (:4693-4716).
+                    float cw = lambert_value(sc, h, mat.refl_spd, nd, lam[w]) * fabsf(dot3(nd, h.n));
+                    if (pdf > 0.0f) cw /= pdf;
+                    sc4[w] = cw > 0.0f ? cw : 0.0f;
                 }
-                const float phi = kTwoPi * ua, st = sqrtf(ub), ct = sqrtf(1.0f - ub);
-                const f3 nd = onb_xform(fr, mk3(cosf(phi) * st, sinf(phi) * st, ct));
-                const float pdf = fabsf(dot3(nd, h.frame.w)) * kInvPi;
-                const f3 neff = back ? -h.frame.w : h.frame.w;
-                const f3 g = geom_against_ray(h, neff);
-                if (dot3(nd, g) > 0.0f) {
-                    float sc4[kHwss];
-                    sc4[0] = spd_value(sc, mat.refl_spd, lam[0]);
-                    for (int w = 1; w < kHwss; ++w) {
-                        // Companion weight: BSDF value * |cos| / pdf (:4693-4716).
-                        float cw = lambert_value(sc, h, mat.refl_spd, nd, lam[w]) * fabsf(dot3(nd, h.n));
-                        if (pdf > 0.0f) cw /= pdf;
-                        sc4[w] = cw > 0.0f ? cw : 0.0f;
-                    }
-                    // Russian roulette over the max across wavelengths (:4721-4768,
-                    // PathTransportUtilities.h:86-110).
-                    float mprev = fabsf(thr[0]), mcurr = fabsf(thr[0] * sc4[0]);
-                    for (int w = 1; w < kHwss; ++w) {
-                        mprev = fmaxf(mprev, fabsf(thr[w]));
-                        mcurr = fmaxf(mcurr, fabsf(thr[w] * sc4[w]));
-                    }
-                    bool go = mcurr > kNearZeroF;
-                    if (go && depth >= P.rr_min_depth) {
-                        const float p = fminf(1.0f, mcurr / fmaxf(mprev, P.rr_threshold));
-                        if (rng_u(P.seed, path, base_dim + 19) >= p) {
-                            go = false;
-                        } else if (p > 0.0f && p < 1.0f) {
-                            const float inv = 1.0f / p;
-                            for (int w = 0; w < kHwss; ++w) sc4[w] *= inv;
-                        }
-                    }
-                    // Bounce caps (PathTransportUtilities.h:170-210).
-                    const unsigned nb = S.bounces[slot] + 1u;
-                    if (go && (int)nb > P.max_diffuse) go = false;
-                    if (go) {
-                        for (int w = 0; w < kHwss; ++w) S.thr[w * P.cap + slot] = thr[w] * sc4[w];
-                        S.pdf[slot] = pdf;
-                        S.bounces[slot] = nb;
-                        new_o = offset_ray(h.p, dot3(nd, h.ng) > 0.0f ? h.ng : -h.ng);
-                        new_d = nd;
-                        cont = true;
+                // Russian roulette over the max across wavelengths (:4721-4768,
+                // PathTransportUtilities.h:86-110).
+                float mprev = fabsf(thr[0]), mcurr = fabsf(thr[0] * sc4[0]);
+                for (int w = 1; w < kHwss; ++w) {
+                    mprev = fmaxf(mprev, fabsf(thr[w]));
+                    mcurr = fmaxf(mcurr, fabsf(thr[w] * sc4[w]));
+                }
+                bool go = mcurr > kNearZeroF;
+                if (go && depth >= P.rr_min_depth) {
+                    const float p = fminf(1.0f, mcurr / fmaxf(mprev, P.rr_threshold));
+                    if (rng_u(P.seed, path, base_dim + 19) >= p) {
+                        go = false;
+                    } else if (p > 0.0f && p < 1.0f) {
+                        const float inv = 1.0f / p;
+                        for (int w = 0; w < kHwss; ++w) sc4[w] *= inv;
                     }
                 }
-                for (int w = 0; w < kHwss; ++w) S.res[w * P.cap + slot] = res[w];
-                // Runaway guard for the next bounce (:4096-4110).
-                if (cont) {
+                // Bounce caps (PathTransportUtilities.h:170-210).
+                const unsigned nb = S.bounces[slot] + 1u;
+                if (go && (int)nb > P.max_diffuse) go = false;
+                if (go) {
+                    // Runaway guard for the next bounce (:4096-4110).
                     float mx = 0.0f;
                     bool bad = false;
                     for (int w = 0; w < kHwss; ++w) {
-                        const float v = S.thr[w * P.cap + slot];
-                        if (!(fabsf(v) <= 3.0e38f)) bad = true;
-                        mx = fmaxf(mx, fabsf(v));
+                        const float t = thr[w] * sc4[w];
+                        if (!(fabsf(t) <= 3.0e38f)) bad = true;
+                        mx = fmaxf(mx, fabsf(t));
+                        S.thr[w * P.cap + slot] = t;
                     }
-                    if (bad || mx > 1e6f) cont = false;
+                    if (!bad && mx <= 1e6f) {
+                        S.pdf[slot] = pdf;
+                        S.bounces[slot] = nb;
+                        new_o = offset_ray(h.p, dot3(nd, h.ng) > 0.0f ? h.ng : -h.ng);
+                        cont = true;
+                    }
                 }
             }
         }
-        // Compact the continuation and shadow queues.
         const unsigned qi = wave_push(cnt_next, cont);
         if (cont) {
             qout[qi] = slot;
             S.ox[slot] = new_o.x;
             S.oy[slot] = new_o.y;
             S.oz[slot] = new_o.z;
-            S.dx[slot] = new_d.x;
-            S.dy[slot] = new_d.y;
-            S.dz[slot] = new_d.z;
-        }
-        for (int w = 0; w < kHwss; ++w) {
-            const unsigned si = wave_push(cnt_sh, sh[w]);
-            if (sh[w]) {
-                S.sox[si] = sorg[w].x;
-                S.soy[si] = sorg[w].y;
-                S.soz[si] = sorg[w].z;
-                S.sdx[si] = sdir[w].x;
-                S.sdy[si] = sdir[w].y;
-                S.sdz[si] = sdir[w].z;
-                S.stmax[si] = stm[w];
-                S.sval[si] = sval[w];
-                S.sslot[si] = slot * 4u + (unsigned)w;
-            }
+            S.dx[slot] = nd.x;
+            S.dy[slot] = nd.y;
+            S.dz[slot] = nd.z;
         }
     }
 }
@@ -542,8 +541,8 @@ rise_hip_status upload(rise_hip_pt_scene* s, const std::vector<T>& v, const T** 
 
 void free_scene(rise_hip_pt_scene* s)
 {
-    for (void* p : s->allocs) hipFree(p);
-    if (s->work) hipFree(s->work);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->work) (void)hipFree(s->work);
     delete s;
 }
 
@@ -808,7 +807,7 @@ extern "C" rise_hip_status rise_hip_render(rise_hip_pt_scene* scene, const rise_
     Paths S{};
     const size_t need = carve(nullptr, cap, npx, nullptr);
     if (need > scene->work_bytes) {
-        if (scene->work) hipFree(scene->work);
+        if (scene->work) (void)hipFree(scene->work);
         scene->work = nullptr;
         scene->work_bytes = 0;
         const hipError_t e = hipMalloc(&scene->work, need);

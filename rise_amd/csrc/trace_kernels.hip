@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void any_kernel(const NodeSlot* __restrict_
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 #ifndef RISE_REFILL
-#define RISE_REFILL 32
+#define RISE_REFILL 28
 #endif
 
 __device__ inline unsigned long long shfl_u64(unsigned long long v, int src)
