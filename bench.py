@@ -146,6 +146,105 @@ def render_leg(args, cpu_enabled: bool):
     return out
 
 
+def batch_leg(args):
+    """Batched rendering: 64 requests over 8 seeded Cornell-style scenes with
+    10k-500k added triangles, resolutions {128..1024}^2, spp {1, 4, 16, 64},
+    depth 6.  Timed three ways on one stream: every request rendered alone
+    with rise_hip_render, back to back (the single-scene path); the whole
+    set as one batch launch; and that launch replayed from a captured graph.
+    Every batched film is checked bit-for-bit against the film of the same
+    request rendered alone."""
+    import numpy as np
+    import torch
+
+    from rise_amd import pt_scene
+    from rise_amd.render import PathTracingSpectralRasterizer, RenderBatch
+
+    t0 = time.perf_counter()
+    scenes, reqs = pt_scene.batch_workload()
+    rasts = [PathTracingSpectralRasterizer(s) for s in scenes]
+    setup_s = time.perf_counter() - t0
+    n_req = len(reqs)
+    paths = sum(r.width * r.height * r.spp for _, r in reqs)
+    log(f"batch: {n_req} requests, {paths / 1e6:.1f} M paths, scenes {[s.n_tris for s in scenes]} "
+        f"(setup {setup_s:.1f} s)")
+
+    batch = RenderBatch([(rasts[k], r) for k, r in reqs])
+    batch.launch()
+    st = batch.stats()
+    rays = st["extension_rays"] + st["shadow_rays"]
+    steps = max(1, args.batch_steps)
+
+    # Single-scene path: one rise_hip_render per request, back to back.
+    alone = []
+    for (k, r), f in zip(reqs, batch.films):
+        rasts[k].rasterize_scene(r, film=torch.empty_like(f), stats=False)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for (k, r), f in zip(reqs, batch.films):
+        film, _, _ = rasts[k].rasterize_scene(r, film=torch.empty_like(f), stats=False)
+        alone.append(film)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - w0
+    identical = all(torch.equal(a, f) for a, f in zip(alone, batch.films))
+    del alone
+
+    # One batch launch per step.
+    lat = []
+    for _ in range(steps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        batch.launch()
+        b.record()
+        torch.cuda.synchronize()
+        lat.append(a.elapsed_time(b))
+    batch_ms = float(np.mean(lat))
+
+    # The same launch captured in a graph.
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        batch.launch(side)
+    torch.cuda.synchronize()
+    glat = []
+    for _ in range(steps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        glat.append(a.elapsed_time(b))
+    graph_ms = float(np.mean(glat))
+    del g
+    best_ms = min(batch_ms, graph_ms)
+    out = {
+        "workload": "64 render requests over 8 seeded Cornell-style scenes + 10k-500k random tris, "
+        "res {128,256,512,1024}^2, spp {1,4,16,64}, depth 6",
+        "requests": n_req,
+        "paths": int(paths),
+        "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
+        "sequential_single_scene": {"s": round(seq_s, 4), "requests_per_s": round(n_req / seq_s, 2),
+                                    "mrays_per_s": round(rays / seq_s / 1e6, 1)},
+        "batched": {"ms": round(batch_ms, 3), "requests_per_s": round(n_req / (batch_ms / 1e3), 2),
+                    "mrays_per_s": round(rays / (batch_ms / 1e3) / 1e6, 1),
+                    "paths_per_s": round(paths / (batch_ms / 1e3), 1)},
+        "graph_replay": {"ms": round(graph_ms, 3), "requests_per_s": round(n_req / (graph_ms / 1e3), 2),
+                         "latency_ms_p50": round(float(np.percentile(glat, 50)), 3),
+                         "latency_ms_p99": round(float(np.percentile(glat, 99)), 3)},
+        "speedup_vs_sequential": round(seq_s / (best_ms / 1e3), 3),
+        "films_identical_alone_vs_batched": bool(identical),
+    }
+    log(f"batch: sequential {seq_s * 1e3:.1f} ms, batched {batch_ms:.1f} ms, graph {graph_ms:.1f} ms, "
+        f"{out['batched']['mrays_per_s']} Mrays/s, identical films {identical}")
+    batch.release()
+    for r in rasts:
+        r.release()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +255,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true", help="run only the 1 GiB copy kernel")
     ap.add_argument("--no-render", action="store_true", help="skip the Cornell-box render leg")
     ap.add_argument("--render-steps", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering leg")
+    ap.add_argument("--batch-steps", type=int, default=3)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -271,6 +372,7 @@ def main() -> int:
             log("oracle/_ref not built: no CPU baseline")
 
     render = None if args.no_render else render_leg(args, cpu_enabled=not args.no_cpu_baseline)
+    batched = None if args.no_batch else batch_leg(args)
 
     traffic = load_traffic()
     line = {
@@ -301,7 +403,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "closest_kernel",
+            "kernel": "closest_persistent",
             "bytes_per_ray": round(b_ray, 2),
             "node_record_bytes": node_bytes,
             "nodes_per_ray": round(nodes_per_ray, 3),
@@ -310,6 +412,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "render": render,
+        "batch": batched,
     }
     om.release()
     print(json.dumps(line), flush=True)

@@ -106,10 +106,42 @@ typedef struct rise_hip_render_stats {
  *   Rendering/ProgressiveFilm.h:36-60, before the XYZ -> RGB conversion).
  * film_xyz_sq: device, optional (NULL): per-pixel mean of squared XYZ.
  * stats: host, optional (NULL); when given the call waits for the stream.
- * The scene's work buffers are reused between calls: calls on one scene
- * must not overlap. */
+ * The scene keeps the work buffers and tables of its last call and reuses
+ * them while the request is unchanged; a changed request waits for the
+ * previous call's stream before replacing them. */
 rise_hip_status rise_hip_render(rise_hip_pt_scene* scene, const rise_hip_render_desc* desc, float* film_xyz,
                                 float* film_xyz_sq, rise_hip_render_stats* stats, hipStream_t stream);
+
+/* ---- Batches: several renders (scenes, image sizes, sample counts) in one
+ * launch sequence ----
+ *
+ * Replaces a loop of Job::Rasterize calls (Job.cpp:9268), one per job, each
+ * running IRasterizer::RasterizeScene to completion: here the requests share
+ * one pool of paths in flight, so small renders fill the GPU together.  Each request's film is identical to
+ * what rise_hip_render writes for it alone. */
+typedef struct rise_hip_render_request {
+    rise_hip_pt_scene* scene;
+    rise_hip_render_desc desc;
+    float* film_xyz;     /* device, width*height*3 floats (as rise_hip_render) */
+    float* film_xyz_sq;  /* device, optional */
+} rise_hip_render_request;
+
+typedef struct rise_hip_batch rise_hip_batch;
+
+/* Plans the batch (pass layout, work buffers, device tables).  Synchronous.
+ * The scenes must outlive the batch; the films are written by each launch. */
+rise_hip_status rise_hip_batch_create(const rise_hip_render_request* requests, size_t n_requests,
+                                      rise_hip_batch** out_batch);
+
+/* Enqueues the whole batch on `stream`: kernel launches only (no copies, no
+ * allocation, no host synchronization), so the call may be captured in a
+ * hipGraph and replayed.  Launches of one batch must not overlap. */
+rise_hip_status rise_hip_batch_launch(rise_hip_batch* batch, hipStream_t stream);
+
+/* Ray and path counts of the last launch (waits for `stream`). */
+rise_hip_status rise_hip_batch_get_stats(rise_hip_batch* batch, hipStream_t stream, rise_hip_render_stats* out_stats);
+
+rise_hip_status rise_hip_batch_release(rise_hip_batch* batch);
 
 #ifdef __cplusplus
 }

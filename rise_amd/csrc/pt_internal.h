@@ -1,0 +1,22 @@
+// pt_internal.h -- library-internal definition of a path-tracing scene.
+#pragma once
+
+#include "pt_device.h"
+
+#include <cstdint>
+#include <vector>
+
+struct rise_hip_batch;
+
+struct rise_hip_pt_scene {
+    int device = 0;
+    std::vector<void*> allocs;      // device tables owned by the scene
+    rise_hip::PtSceneView view{};
+    int64_t n_tris = 0;
+    bool has_glossy = false;        // a material scatters glossily (counts against max_glossy_bounce)
+    bool has_transmission = false;  // a material transmits (counts against max_transmission_bounce)
+    rise_hip_batch* solo = nullptr;  // cached batch for rise_hip_render, rebuilt when the request changes
+};
+
+// Frees the scene's cached single-request batch (pt_render.hip).
+void release_solo_batch(rise_hip_pt_scene* s);

@@ -1,0 +1,219 @@
+// pt_scene.hip -- spectral path-tracing scenes: BVH, material, spectrum and
+// luminary tables resident in HBM (rise_hip_pt_scene_create / _release).
+#include "rise_hip_render.h"
+
+#include "api_common.h"
+#include "bvh_build.h"
+#include "pt_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace rise_hip;
+
+// ---------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------
+
+namespace {
+
+template <class T>
+rise_hip_status upload(rise_hip_pt_scene* s, const std::vector<T>& v, const T** out)
+{
+    const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail_hip(e, "hipMalloc (scene)");
+    s->allocs.push_back(p);
+    if (!v.empty()) {
+        e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail_hip(e, "hipMemcpy (scene)");
+    }
+    *out = static_cast<const T*>(p);
+    return RISE_HIP_OK;
+}
+
+void free_scene(rise_hip_pt_scene* s)
+{
+    release_solo_batch(s);
+    for (void* p : s->allocs) (void)hipFree(p);
+    delete s;
+}
+
+}  // namespace
+
+extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* objects, size_t n_objects,
+                                                    const rise_hip_material_desc* materials, size_t n_materials,
+                                                    const rise_hip_spd_desc* spds, size_t n_spds,
+                                                    rise_hip_pt_scene** out_scene)
+{
+    clear_error();
+    if (!out_scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "out_scene is NULL");
+    *out_scene = nullptr;
+    if ((n_objects && !objects) || (n_materials && !materials) || (n_spds && !spds))
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL array with a nonzero count");
+    if (n_objects > 0x7fffffff || n_materials > 0x7fffffff || n_spds > 0x7fffffff)
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "too many objects, materials or spectra");
+    // Spectra.
+    std::vector<DevSpd> dspd(n_spds);
+    std::vector<float> amps;
+    for (size_t i = 0; i < n_spds; ++i) {
+        const rise_hip_spd_desc& d = spds[i];
+        if (d.n <= 0 || !d.amplitudes || !(d.lambda_end > d.lambda_begin))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "spectrum " + std::to_string(i) + " is empty or inverted");
+        dspd[i].begin = d.lambda_begin;
+        dspd[i].end = d.lambda_end;
+        dspd[i].n = d.n;
+        dspd[i].inv_delta = (float)((double)d.n / ((double)d.lambda_end - (double)d.lambda_begin));
+        dspd[i].offset = (int)amps.size();
+        amps.insert(amps.end(), d.amplitudes, d.amplitudes + d.n);
+    }
+    // Materials.
+    std::vector<DevMaterial> dmat(n_materials);
+    for (size_t i = 0; i < n_materials; ++i) {
+        const rise_hip_material_desc& m = materials[i];
+        if (m.type != RISE_HIP_MATERIAL_LAMBERTIAN)
+            return fail(RISE_HIP_ERR_NOT_SUPPORTED, "material " + std::to_string(i) + ": unsupported type");
+        if (m.reflectance_spd < 0 || (size_t)m.reflectance_spd >= n_spds || m.emission_spd >= (int)n_spds ||
+            m.emission_spd < -1)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "material " + std::to_string(i) + ": spectrum index out of range");
+        dmat[i] = DevMaterial{m.type, m.reflectance_spd, m.emission_spd, m.emission_scale};
+    }
+    // Objects -> one triangle list.
+    int64_t total = 0;
+    for (size_t o = 0; o < n_objects; ++o) {
+        if (objects[o].n_tris < 0 || (objects[o].n_tris > 0 && !objects[o].vertices))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "object " + std::to_string(o) + ": bad triangle buffer");
+        if (objects[o].material < 0 || (size_t)objects[o].material >= n_materials)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "object " + std::to_string(o) + ": material out of range");
+        total += objects[o].n_tris;
+    }
+    if (total > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "more than 2^31-1 triangles");
+    std::vector<float> verts((size_t)total * 9);
+    std::vector<int32_t> tri_obj((size_t)total);
+    {
+        int64_t off = 0;
+        for (size_t o = 0; o < n_objects; ++o) {
+            const int64_t k = objects[o].n_tris;
+            if (k) std::memcpy(verts.data() + 9 * off, objects[o].vertices, (size_t)k * 9 * sizeof(float));
+            for (int64_t i = 0; i < k; ++i) tri_obj[off + i] = (int32_t)o;
+            off += k;
+        }
+    }
+    WideBVH bvh;
+    std::string err;
+    try {
+        if (!build_wide_bvh(verts.data(), total, nullptr, BuildParams(), bvh, err))
+            return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+    }
+    // Object id in v1.w; inverse map global id -> leaf index.
+    std::vector<uint32_t> leaf_of((size_t)total);
+    for (size_t l = 0; l < bvh.tris.size(); ++l) {
+        int32_t id;
+        std::memcpy(&id, &bvh.tris[l].v0[3], 4);
+        const int32_t ob = tri_obj[(size_t)id];
+        std::memcpy(&bvh.tris[l].v1[3], &ob, 4);
+        leaf_of[(size_t)id] = (uint32_t)l;
+    }
+    // Luminaries (LuminaryManager + LightSampler::Prepare): emitting objects,
+    // area CDF over their triangles, selection weight = mean exitance * scale * area.
+    std::vector<DevObject> dobj(n_objects);
+    std::vector<DevLuminary> lums;
+    std::vector<uint32_t> lum_tris;
+    std::vector<float> lum_cdf;
+    std::vector<double> weights;
+    {
+        int64_t off = 0;
+        for (size_t o = 0; o < n_objects; ++o) {
+            const int64_t k = objects[o].n_tris;
+            dobj[o] = DevObject{objects[o].material, objects[o].double_sided ? 1 : 0, -1, 0};
+            const DevMaterial& m = dmat[objects[o].material];
+            if (m.emit_spd >= 0 && k > 0) {
+                std::vector<double> areas((size_t)k);
+                double tot = 0;
+                for (int64_t i = 0; i < k; ++i) {
+                    const float* f = verts.data() + 9 * (off + i);
+                    const double ax = (double)f[3] - f[0], ay = (double)f[4] - f[1], az = (double)f[5] - f[2];
+                    const double bx = (double)f[6] - f[0], by = (double)f[7] - f[1], bz = (double)f[8] - f[2];
+                    const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+                    areas[(size_t)i] = std::sqrt(cx * cx + cy * cy + cz * cz) * 0.5;
+                    tot += areas[(size_t)i];
+                }
+                const DevSpd& e = dspd[(size_t)m.emit_spd];
+                double mean = 0;
+                for (int j = 0; j < e.n; ++j) mean += amps[(size_t)(e.offset + j)];
+                mean /= e.n;
+                const double wgt = mean * m.emit_scale * tot;
+                if (tot > 0 && wgt > 0) {
+                    DevLuminary L{};
+                    L.object = (int)o;
+                    L.tri_first = (int)lum_tris.size();
+                    L.tri_count = (int)k;
+                    L.area = (float)tot;
+                    L.material = objects[o].material;
+                    double acc = 0;
+                    for (int64_t i = 0; i < k; ++i) {
+                        acc += areas[(size_t)i] / tot;
+                        lum_tris.push_back(leaf_of[(size_t)(off + i)]);
+                        lum_cdf.push_back((float)acc);
+                    }
+                    dobj[o].luminary = (int)lums.size();
+                    lums.push_back(L);
+                    weights.push_back(wgt);
+                }
+            }
+            off += k;
+        }
+        double wt = 0;
+        for (double w : weights) wt += w;
+        double acc = 0;
+        for (size_t i = 0; i < lums.size(); ++i) {
+            acc += weights[i];
+            lums[i].pdf_select = (float)(weights[i] / wt);
+            lums[i].cdf_select = (float)(acc / wt);
+        }
+        if (!lums.empty()) lums.back().cdf_select = 1.0f;
+    }
+
+    rise_hip_pt_scene* s = new (std::nothrow) rise_hip_pt_scene;
+    if (!s) return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    hipError_t he = hipGetDevice(&s->device);
+    if (he != hipSuccess) {
+        delete s;
+        return fail_hip(he, "hipGetDevice");
+    }
+    rise_hip_status st = RISE_HIP_OK;
+    PtSceneView& v = s->view;
+    if (st == RISE_HIP_OK) st = upload(s, bvh.nodes, &v.nodes);
+    if (st == RISE_HIP_OK) st = upload(s, bvh.tris, &v.tris);
+    if (st == RISE_HIP_OK) st = upload(s, dobj, &v.objects);
+    if (st == RISE_HIP_OK) st = upload(s, dmat, &v.materials);
+    if (st == RISE_HIP_OK) st = upload(s, dspd, &v.spds);
+    if (st == RISE_HIP_OK) st = upload(s, amps, &v.spd_amp);
+    if (st == RISE_HIP_OK) st = upload(s, lums, &v.lums);
+    if (st == RISE_HIP_OK) st = upload(s, lum_tris, &v.lum_tris);
+    if (st == RISE_HIP_OK) st = upload(s, lum_cdf, &v.lum_cdf);
+    if (st != RISE_HIP_OK) {
+        free_scene(s);
+        return st;
+    }
+    v.depth = bvh.stats.depth;
+    v.n_lums = (int)lums.size();
+    s->n_tris = total;
+    *out_scene = s;
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_pt_scene_release(rise_hip_pt_scene* scene)
+{
+    clear_error();
+    if (scene) free_scene(scene);
+    return RISE_HIP_OK;
+}

@@ -172,3 +172,52 @@ def furnace_box(albedo: float = 0.5, radiance: float = 1.0, max_bounce: int = 64
     r = RenderSettings(width=16, height=16, spp=64, max_diffuse_bounce=max_bounce, max_glossy_bounce=0,
                        max_transmission_bounce=0)
     return s, r
+
+
+def cloud_cornell(seed: int, n_extra: int):
+    """A Cornell-box-style scene with `n_extra` seeded random triangles added:
+    the box of ``cornell_box()`` with per-scene albedos, plus a cloud of small
+    double-sided triangles (centroids uniform over the upper-middle of the
+    box, edge vectors Gaussian with a scale that keeps the cloud's total area
+    near 1.5 m^2 whatever its count).  Returns (Scene, RenderSettings) with
+    depth 6."""
+    rng = np.random.default_rng(seed)
+    s, r = cornell_box()
+    s.spds[0] = Spd.flat(float(rng.uniform(0.5, 0.8)))
+    s.spds[1] = Spd.rgb_bands(*rng.uniform(0.05, 0.7, 3))
+    s.spds[2] = Spd.rgb_bands(*rng.uniform(0.05, 0.7, 3))
+    s.spds.append(Spd.rgb_bands(*rng.uniform(0.2, 0.8, 3)))
+    s.materials.append(Material(len(s.spds) - 1))
+    if n_extra > 0:
+        c = rng.uniform((-0.8, 0.7, -0.8), (0.8, 1.7, 0.8), (n_extra, 3))
+        sigma = math.sqrt(3.0 / n_extra)
+        e1 = rng.normal(0.0, sigma, (n_extra, 3))
+        e2 = rng.normal(0.0, sigma, (n_extra, 3))
+        v0 = c - (e1 + e2) / 3.0
+        tris = np.concatenate([v0, v0 + e1, v0 + e2], 1).astype(np.float32)
+        s.objects.insert(len(s.objects) - 1, Object(tris, len(s.materials) - 1, double_sided=True))
+    r.max_diffuse_bounce = 6
+    r.max_glossy_bounce = 6
+    r.max_transmission_bounce = 6
+    r.seed = seed
+    return s, r
+
+
+def batch_workload(seed: int = 4, n_instances: int = 64, n_scenes: int = 8):
+    """The batched-rendering workload: `n_scenes` seeded ``cloud_cornell``
+    scenes with 10k-500k added triangles (log-uniform), and `n_instances`
+    render requests, each over a scene drawn uniformly, at a resolution drawn
+    from {128, 256, 512, 1024}^2 and spp from {1, 4, 16, 64}, depth 6.
+    Returns (scenes, [(scene index, RenderSettings)])."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(1e4), math.log(5e5), n_scenes)).astype(np.int64)
+    scenes = [cloud_cornell(seed * 1000 + k, int(counts[k]))[0] for k in range(n_scenes)]
+    reqs = []
+    for i in range(n_instances):
+        k = int(rng.integers(n_scenes))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.choice([1, 4, 16, 64]))
+        r = RenderSettings(width=res, height=res, spp=spp, max_diffuse_bounce=6, max_glossy_bounce=6,
+                           max_transmission_bounce=6, seed=seed * 100000 + i)
+        reqs.append((k, r))
+    return scenes, reqs

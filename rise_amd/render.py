@@ -85,12 +85,25 @@ class RenderStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RenderRequest(ctypes.Structure):
+    _fields_ = [
+        ("scene", ctypes.c_void_p),
+        ("desc", RenderDesc),
+        ("film_xyz", ctypes.c_void_p),
+        ("film_xyz_sq", ctypes.c_void_p),
+    ]
+
+
 _P = ctypes.c_void_p
 _SIG = {
     "rise_hip_pt_scene_create": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.POINTER(_P)],
                                  ctypes.c_int),
     "rise_hip_pt_scene_release": ([_P], ctypes.c_int),
     "rise_hip_render": ([_P, ctypes.POINTER(RenderDesc), _P, _P, ctypes.POINTER(RenderStats), _P], ctypes.c_int),
+    "rise_hip_batch_create": ([_P, ctypes.c_size_t, ctypes.POINTER(_P)], ctypes.c_int),
+    "rise_hip_batch_launch": ([_P, _P], ctypes.c_int),
+    "rise_hip_batch_get_stats": ([_P, _P, ctypes.POINTER(RenderStats)], ctypes.c_int),
+    "rise_hip_batch_release": ([_P], ctypes.c_int),
 }
 
 
@@ -175,7 +188,7 @@ class PathTracingSpectralRasterizer:
         if want_sq and film_sq is None:
             film_sq = torch.empty_like(film)
         st = RenderStats() if stats else None
-        s = torch.cuda.current_stream().cuda_stream if stream is None else int(getattr(stream, "cuda_stream", stream))
+        s = _stream_handle(stream)
         _lib.check(_lib_render().rise_hip_render(self._h, ctypes.byref(d), film.data_ptr(),
                                                  film_sq.data_ptr() if want_sq else None,
                                                  ctypes.byref(st) if stats else None, s))
@@ -184,6 +197,66 @@ class PathTracingSpectralRasterizer:
     def release(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib_render().rise_hip_pt_scene_release(self._h)
+            self._h = _P(None)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def _stream_handle(stream) -> int:
+    import torch
+
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+class RenderBatch:
+    """Several renders planned as one launch sequence (rise_hip_batch_*).
+
+    ``items`` is a list of (PathTracingSpectralRasterizer, RenderSettings) or
+    (rasterizer, settings, spp, spp_offset, spp_total).  Films are allocated
+    here (``films[k]``: mean XYZ (H, W, 3); ``films_sq[k]`` when want_sq) and
+    rewritten by every ``launch``; each is identical to what
+    ``rasterize_scene`` gives for that item alone.  ``launch`` only enqueues
+    kernels, so it may be captured in a CUDA/HIP graph."""
+
+    def __init__(self, items, want_sq: bool = False):
+        import torch
+
+        L = _lib_render()
+        self.items = list(items)
+        self.films, self.films_sq = [], []
+        reqs = (RenderRequest * max(1, len(self.items)))()
+        for k, it in enumerate(self.items):
+            rast, settings = it[0], it[1]
+            spp, spp_offset, spp_total = (list(it[2:]) + [None, 0, None][len(it) - 2:])[:3]
+            film = torch.empty((settings.height, settings.width, 3), dtype=torch.float32, device="cuda")
+            sq = torch.empty_like(film) if want_sq else None
+            self.films.append(film)
+            self.films_sq.append(sq)
+            reqs[k].scene = rast._h.value
+            reqs[k].desc = render_desc(settings, rast.scene, spp, spp_offset, spp_total)
+            reqs[k].film_xyz = film.data_ptr()
+            reqs[k].film_xyz_sq = sq.data_ptr() if want_sq else None
+        out = _P(None)
+        _lib.check(L.rise_hip_batch_create(reqs, len(self.items), ctypes.byref(out)))
+        self._h = out
+
+    def launch(self, stream=None) -> None:
+        _lib.check(_lib_render().rise_hip_batch_launch(self._h, _stream_handle(stream)))
+
+    def stats(self, stream=None) -> dict:
+        st = RenderStats()
+        _lib.check(_lib_render().rise_hip_batch_get_stats(self._h, _stream_handle(stream), ctypes.byref(st)))
+        return st.as_dict()
+
+    def release(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib_render().rise_hip_batch_release(self._h)
             self._h = _P(None)
 
     def __del__(self):

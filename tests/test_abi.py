@@ -35,6 +35,26 @@ def test_null_arguments_rejected():
     assert L.rise_hip_scene_upload(ctypes.byref(desc), 1, ctypes.byref(out)) == 1
 
 
+def test_render_and_batch_arguments_rejected():
+    from rise_amd import render
+
+    L = render._lib_render()
+    out = ctypes.c_void_p()
+    assert L.rise_hip_batch_create(None, 0, ctypes.byref(out)) == 1
+    assert b"empty batch" in L.rise_hip_last_error()
+    assert L.rise_hip_batch_create(None, 3, None) == 1
+    # A request without a scene is rejected during planning, before any device call.
+    reqs = (render.RenderRequest * 1)()
+    reqs[0].desc.width = reqs[0].desc.height = reqs[0].desc.spp = reqs[0].desc.spp_total = 4
+    assert L.rise_hip_batch_create(reqs, 1, ctypes.byref(out)) == 1
+    assert b"request 0" in L.rise_hip_last_error() and not out.value
+    assert L.rise_hip_batch_launch(None, None) == 1
+    assert L.rise_hip_batch_get_stats(None, None, None) == 1
+    assert L.rise_hip_batch_release(None) == 0
+    assert L.rise_hip_render(None, None, None, None, None, None) == 1
+    assert L.rise_hip_pt_scene_release(None) == 0
+
+
 def test_cpp_adapter_demo_links():
     # The C++ host adapters (rise_amd/host) build and load the library; with no
     # arguments the demo prints its usage without touching the GPU.
