@@ -169,7 +169,18 @@ def batch_leg(args):
     log(f"batch: {n_req} requests, {paths / 1e6:.1f} M paths, scenes {[s.n_tris for s in scenes]} "
         f"(setup {setup_s:.1f} s)")
 
-    batch = RenderBatch([(rasts[k], r) for k, r in reqs])
+    items = [(rasts[k], r) for k, r in reqs]
+    tuning = None
+    tune_log = None
+    if not args.no_autotune:
+        from rise_amd import autotune
+
+        t1 = time.perf_counter()
+        tuning, results = autotune.tune(items, proxy_spp=1, budget_s=90.0)
+        tune_log = {"proxy": "the same 64 requests at 1 spp", "seconds": round(time.perf_counter() - t1, 1),
+                    "candidates": [{**tn, "ms": round(ms, 3)} for tn, ms in results]}
+        log(f"autotune: best {tuning} ({tune_log['seconds']} s)")
+    batch = RenderBatch(items, tuning=tuning)
     batch.launch()
     st = batch.stats()
     rays = st["extension_rays"] + st["shadow_rays"]
@@ -235,6 +246,8 @@ def batch_leg(args):
                          "latency_ms_p50": round(float(np.percentile(glat, 50)), 3),
                          "latency_ms_p99": round(float(np.percentile(glat, 99)), 3)},
         "speedup_vs_sequential": round(seq_s / (best_ms / 1e3), 3),
+        "tuning": batch.tuning(),
+        "autotune": tune_log,
         "films_identical_alone_vs_batched": bool(identical),
     }
     log(f"batch: sequential {seq_s * 1e3:.1f} ms, batched {batch_ms:.1f} ms, graph {graph_ms:.1f} ms, "
@@ -257,6 +270,7 @@ def main() -> int:
     ap.add_argument("--render-steps", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering leg")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true", help="batched leg with default launch shapes")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -36,15 +36,21 @@ typedef struct rise_hip_spd_desc {
 } rise_hip_spd_desc;
 
 typedef enum rise_hip_material_type {
-    RISE_HIP_MATERIAL_LAMBERTIAN = 0 /* LambertianMaterial (+ LambertianLuminaireMaterial emitter) */
+    RISE_HIP_MATERIAL_LAMBERTIAN = 0, /* LambertianMaterial (+ LambertianLuminaireMaterial emitter) */
+    RISE_HIP_MATERIAL_GGX = 1,        /* GGXMaterial, conductor Fresnel, isotropic roughness
+                                         (Materials/GGXBRDF.cpp, GGXSPF.cpp) */
+    RISE_HIP_MATERIAL_DIELECTRIC = 2  /* DielectricMaterial, smooth (Materials/DielectricSPF.cpp) */
 } rise_hip_material_type;
 
 typedef struct rise_hip_material_desc {
     int32_t type;            /* rise_hip_material_type */
-    int32_t reflectance_spd; /* index into the SPD list */
-    int32_t emission_spd;    /* -1: not an emitter (LambertianEmitter otherwise) */
+    int32_t reflectance_spd; /* Lambertian reflectance; GGX diffuse `rd` (-1: none) */
+    int32_t emission_spd;    /* -1: not an emitter (LambertianEmitter otherwise; Lambertian only) */
     float emission_scale;    /* LambertianEmitter scale: L = spd / pi * scale */
-    float reserved[4];
+    int32_t specular_spd;    /* GGX specular tint `rs` */
+    int32_t ior_spd;         /* GGX conductor n; dielectric index of refraction */
+    int32_t extinction_spd;  /* GGX conductor k; dielectric `tau`, transmittance per unit length (-1: none) */
+    float alpha;             /* GGX roughness (alphax = alphay) */
 } rise_hip_material_desc;
 
 /* A world-space triangle mesh with one material (an Object with identity
@@ -117,8 +123,8 @@ rise_hip_status rise_hip_render(rise_hip_pt_scene* scene, const rise_hip_render_
  *
  * Replaces a loop of Job::Rasterize calls (Job.cpp:9268), one per job, each
  * running IRasterizer::RasterizeScene to completion: here the requests share
- * one pool of paths in flight, so small renders fill the GPU together.  Each request's film is identical to
- * what rise_hip_render writes for it alone. */
+ * one pool of paths in flight, so small renders fill the GPU together.  Each
+ * request's film is identical to what rise_hip_render writes for it alone. */
 typedef struct rise_hip_render_request {
     rise_hip_pt_scene* scene;
     rise_hip_render_desc desc;
@@ -132,6 +138,25 @@ typedef struct rise_hip_batch rise_hip_batch;
  * The scenes must outlive the batch; the films are written by each launch. */
 rise_hip_status rise_hip_batch_create(const rise_hip_render_request* requests, size_t n_requests,
                                       rise_hip_batch** out_batch);
+
+/* Launch-shape choices of a batch (0 everywhere: defaults).  They change
+ * speed only: films are the same under every tuning. */
+typedef struct rise_hip_batch_tuning {
+    int32_t max_paths;    /* paths in flight per pass (0: 4194304); at least one chunk (<= 1048576 paths) */
+    int32_t trace_block;  /* workgroup size of the traversal kernels: 64, 128 or 256 (0: 128) */
+    int32_t stack;        /* traversal stack: 0 auto (LDS, sized by tree depth), 8 or 16 LDS entries per
+                             lane when the trees fit, -1 private memory */
+    int32_t persistent;   /* traversal schedule: 0 auto (persistent waves for scenes of >= 4096
+                             triangles), 1 persistent waves with dynamic fetch, 2 one ray per thread */
+} rise_hip_batch_tuning;
+
+/* rise_hip_batch_create with explicit tuning (NULL: defaults).  No
+ * counterpart in the reference: its CPU scheduler has no launch shapes. */
+rise_hip_status rise_hip_batch_create_tuned(const rise_hip_render_request* requests, size_t n_requests,
+                                            const rise_hip_batch_tuning* tuning, rise_hip_batch** out_batch);
+
+/* The tuning a batch resolved to (auto choices made concrete). */
+rise_hip_status rise_hip_batch_get_tuning(const rise_hip_batch* batch, rise_hip_batch_tuning* out_tuning);
 
 /* Enqueues the whole batch on `stream`: kernel launches only (no copies, no
  * allocation, no host synchronization), so the call may be captured in a

@@ -153,7 +153,8 @@ class _RoSpd(ctypes.Structure):
 
 class _RoMaterial(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("reflect_spd", ctypes.c_int32), ("emit_spd", ctypes.c_int32),
-                ("emit_scale", ctypes.c_double)]
+                ("emit_scale", ctypes.c_double), ("spec_spd", ctypes.c_int32), ("ior_spd", ctypes.c_int32),
+                ("ext_spd", ctypes.c_int32), ("alpha", ctypes.c_double)]
 
 
 class _RoScene(ctypes.Structure):
@@ -189,7 +190,8 @@ def render(scene, settings, s0: int = 0, s1: int | None = None, spp_total: int |
         spds[k] = _RoSpd(float(np.float32(s.begin)), float(np.float32(s.end)), a.size, a.ctypes.data)
     mats = (_RoMaterial * len(scene.materials))()
     for k, m in enumerate(scene.materials):
-        mats[k] = _RoMaterial(m.kind, m.reflectance, m.emission, float(np.float32(m.emission_scale)))
+        mats[k] = _RoMaterial(m.kind, m.reflectance, m.emission, float(np.float32(m.emission_scale)), m.specular,
+                              m.ior, m.extinction, float(np.float32(m.alpha)))
     om = np.array([o.material for o in scene.objects], np.int32)
     ods = np.array([1 if o.double_sided else 0 for o in scene.objects], np.uint8)
     keep += [om, ods]
@@ -249,3 +251,46 @@ def cie_y_integral(a: float, b: float) -> float:
     L.rise_oracle_cie_y_integral.argtypes = [ctypes.c_double, ctypes.c_double]
     L.rise_oracle_cie_y_integral.restype = ctypes.c_double
     return L.rise_oracle_cie_y_integral(a, b)
+
+
+# ---- BSDF math (pt_bsdf.c) --------------------------------------------------
+
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+_BSDF_SIG = {
+    "rise_oracle_ggx_lookup_ess": ([_D, _D], _D),
+    "rise_oracle_ggx_lookup_eavg": ([_D], _D),
+    "rise_oracle_ggx_ms_z": ([_D], _D),
+    "rise_oracle_ggx_sample_ms_cos": ([_D, _D], _D),
+    "rise_oracle_ggx_ms_pdf": ([_D, _D, _D], _D),
+    "rise_oracle_fresnel_conductor": ([_D, _D, _D, _D], _D),
+    "rise_oracle_fresnel_avg": ([_D, _D, _D], _D),
+    "rise_oracle_ggx_fms": ([_D, _D], _D),
+    "rise_oracle_ggx_d": ([_D, _D, _DP], _D),
+    "rise_oracle_ggx_g1": ([_D, _D, _DP], _D),
+    "rise_oracle_ggx_g2": ([_D, _D, _DP, _DP], _D),
+    "rise_oracle_vndf_sample": ([_DP, _DP, _D, _D, _D, _D, _DP], None),
+    "rise_oracle_vndf_pdf": ([_DP, _DP, _DP, _D, _D], _D),
+    "rise_oracle_ggx_value": ([_DP, _DP], _D),
+    "rise_oracle_ggx_pdf": ([_DP, _DP], _D),
+    "rise_oracle_ggx_scatter": ([_DP, _D, _D, _D, _DP], None),
+    "rise_oracle_dielectric_reflectance": ([_DP, _DP, _DP, _D, _D], _D),
+    "rise_oracle_refract": ([_DP, _D, _D, _DP], ctypes.c_int),
+}
+
+
+def bsdf_lib() -> ctypes.CDLL:
+    """liboracle.so with the BSDF-math entry points typed; array arguments
+    take ``vec(...)`` (contiguous float64 buffers)."""
+    L = oracle_lib()
+    for name, (a, r) in _BSDF_SIG.items():
+        fn = getattr(L, name)
+        fn.argtypes = a
+        fn.restype = r
+    return L
+
+
+def vec(*xs) -> ctypes.Array:
+    """A C double array for the BSDF-math entry points."""
+    flat = np.asarray(xs, np.float64).ravel()
+    return (ctypes.c_double * flat.size)(*flat.tolist())

@@ -1,11 +1,13 @@
 /*
  * pt_oracle.c -- FP64 restatement of RISE's HWSS spectral path tracer for
- * Lambertian surfaces and Lambertian area luminaires.  TEST INFRASTRUCTURE
+ * Lambertian, GGX conductor and smooth dielectric surfaces and Lambertian
+ * area luminaires.  TEST INFRASTRUCTURE
  * ONLY.  See pt_oracle.h for the reference lines followed and the random
  * number contract.  Intersections are brute force over all triangles with
  * the ray-cast oracle's semantics (rt_oracle.c), ties to the lowest id.
  */
 #include "pt_oracle.h"
+#include "pt_bsdf.h"
 #include "rt_oracle.h"
 
 #include <math.h>
@@ -377,6 +379,226 @@ static double emitted(const prep* P, const ro_material* m, v3 out, v3 N, double 
     return spd_value(&P->s->spds[m->emit_spd], nm) * INV_PI * m->emit_scale;
 }
 
+/* ---- materials --------------------------------------------------------- */
+
+enum { MAT_LAMBERT = 0, MAT_GGX = 1, MAT_DIELECTRIC = 2 };
+
+static double spd_or(const prep* P, int32_t i, double nm, double fallback)
+{
+    return i >= 0 ? spd_value(&P->s->spds[i], nm) : fallback;
+}
+
+static rb_v3 rv(v3 a) { rb_v3 r = {a.x, a.y, a.z}; return r; }
+static v3 vr(rb_v3 a) { return mk(a.x, a.y, a.z); }
+
+static rb_ggx ggx_at(const prep* P, const hit* h, double nm, double amb)
+{
+    const ro_material* m = h->mat;
+    rb_ggx g;
+    g.dir = rv(h->dir);
+    g.onb.u = rv(h->frame.u);
+    g.onb.v = rv(h->frame.v);
+    g.onb.w = rv(h->frame.w);
+    g.ng = rv(h->ng);
+    g.rd = spd_or(P, m->reflect_spd, nm, 0.0);
+    g.rs = spd_or(P, m->spec_spd, nm, 0.0);
+    g.alpha = m->alpha;
+    g.ior = spd_or(P, m->ior_spd, nm, 1.0);
+    g.ext = spd_or(P, m->ext_spd, nm, 0.0);
+    g.amb = amb;
+    return g;
+}
+
+/* IBSDF::valueNM of the hit's material (the dielectric has none). */
+static double bsdf_value(const prep* P, const hit* h, v3 vin, double nm, double amb)
+{
+    if (h->mat->type == MAT_LAMBERT) return lambert_value(P, h, vin, nm);
+    if (h->mat->type == MAT_GGX) {
+        const rb_ggx g = ggx_at(P, h, nm, amb);
+        return rb_ggx_value(&g, rv(vin));
+    }
+    return 0.0;
+}
+
+/* ISPF::PdfNM. */
+static double bsdf_pdf(const prep* P, const hit* h, v3 wo, double nm, double amb)
+{
+    if (h->mat->type == MAT_LAMBERT) return lambert_pdf(h, wo);
+    if (h->mat->type == MAT_GGX) {
+        const rb_ggx g = ggx_at(P, h, nm, amb);
+        return rb_ggx_pdf(&g, rv(wo));
+    }
+    return 0.0;
+}
+
+/* ISPF::ScatterNM for the non-delta materials: one ray or none.  u1, u2
+ * direction, u3 lobe choice. */
+static rb_sample bsdf_scatter(const prep* P, const hit* h, double nm, double amb, double u1, double u2, double u3)
+{
+    rb_sample out = {0, {0, 0, 0}, 0, 0, 0};
+    if (h->mat->type == MAT_LAMBERT) {
+        /* LambertianSPF::ScatterNM (LambertianSPF.cpp:90-131). */
+        onb fr = h->frame;
+        const int back = dot(h->dir, h->frame.w) > NZ;
+        if (back) {
+            fr.w = neg(fr.w);
+            fr.u = neg(fr.u);
+        }
+        const double phi = TWO_PI * u1, st = sqrt(u2), ct = sqrt(1.0 - u2);
+        const v3 nd = onb_xform(&fr, mk(cos(phi) * st, sin(phi) * st, ct));
+        const v3 neff = back ? neg(h->frame.w) : h->frame.w;
+        const v3 g = geom_against_ray(h, neff);
+        if (!(dot(nd, g) > 0)) return out;
+        out.ok = 1;
+        out.dir = rv(nd);
+        out.pdf = fabs(dot(nd, h->frame.w)) * INV_PI;
+        out.kray = spd_value(&P->s->spds[h->mat->reflect_spd], nm);
+        out.type = 0;
+        return out;
+    }
+    if (h->mat->type == MAT_GGX) {
+        const rb_ggx g = ggx_at(P, h, nm, amb);
+        return rb_ggx_scatter(&g, u3, u1, u2);
+    }
+    return out;
+}
+
+/* ---- index-of-refraction stack (IORStack.h) ------------------------------- */
+
+#define IOR_CAP 4 /* air + 3 nested objects; deeper pushes are dropped */
+
+typedef struct {
+    int n;
+    int32_t obj[IOR_CAP];
+    double ior[IOR_CAP];
+} iorstack;
+
+static void ior_init(iorstack* s)
+{
+    s->n = 1;
+    s->obj[0] = -1;
+    s->ior[0] = 1.0;
+}
+
+static double ior_top(const iorstack* s) { return s->ior[s->n - 1]; }
+
+static int ior_contains(const iorstack* s, int32_t o)
+{
+    for (int i = s->n - 1; i >= 0; --i)
+        if (s->obj[i] == o) return 1;
+    return 0;
+}
+
+static void ior_push(iorstack* s, int32_t o, double ior)
+{
+    if (s->n < IOR_CAP) {
+        s->obj[s->n] = o;
+        s->ior[s->n] = ior;
+        s->n++;
+    }
+}
+
+static void ior_pop(iorstack* s, int32_t o)
+{
+    if (s->n <= 1) return;
+    for (int i = s->n - 1; i >= 1; --i)
+        if (s->obj[i] == o) {
+            for (int k = i; k < s->n - 1; ++k) {
+                s->obj[k] = s->obj[k + 1];
+                s->ior[k] = s->ior[k + 1];
+            }
+            s->n--;
+            return;
+        }
+}
+
+/* DielectricSPF::ScatterNM + GenerateScatteredRay for a smooth dielectric
+ * (DielectricSPF.cpp:120-287, 395-432), then ScatteredRayContainer::
+ * RandomlySelect by krayNM (ScatteredRayContainer.cpp:43-92) with the
+ * integrator's select probability (PathTracingIntegrator.cpp:2650-2670).
+ * Returns 0 when no ray is produced; else the direction, the throughput
+ * factor kray / selectProb, the lobe type and the stack the ray carries. */
+static int dielectric_scatter(const prep* P, const hit* h, double nm, const iorstack* st, double u_select,
+                              v3* dir_out, double* weight, int* type, iorstack* st_out)
+{
+    const ro_material* m = h->mat;
+    const int inside = ior_contains(st, h->object);
+    const double rindex = spd_or(P, m->ior_spd, nm, 1.33);
+    double k_refr = 1.0;
+    if (inside) {
+        const double dist = mag(sub(h->p, h->origin));
+        k_refr = pow(spd_or(P, m->ext_spd, nm, 1.0), dist);
+    }
+    const v3 w = h->frame.w;
+    const v3 neff = inside ? neg(w) : w;
+    const v3 graw = (dot(h->ng, h->ng) > 1e-12) ? h->ng : neff;
+    const v3 gn = (dot(graw, neff) >= 0) ? graw : neg(graw);
+    rb_v3 refr = rv(h->dir);
+    double ref;
+    iorstack st_refr = *st;
+    if (inside) {
+        iorstack exit_st = *st;
+        ior_pop(&exit_st, h->object);
+        const double exit_ior = ior_top(&exit_st);
+        if (rb_refract(rv(neg(w)), rindex, exit_ior, &refr)) {
+            ior_pop(&st_refr, h->object);
+            ref = rb_dielectric_reflectance(rv(h->dir), refr, rv(neg(w)), rindex, exit_ior);
+        } else {
+            ref = 1.0;
+        }
+    } else {
+        if (rb_refract(rv(w), ior_top(st), rindex, &refr)) {
+            ref = rb_dielectric_reflectance(rv(h->dir), refr, rv(w), ior_top(st), rindex);
+            ior_push(&st_refr, h->object, rindex);
+        } else {
+            ref = 1.0;
+        }
+    }
+    int has_refl = 1, has_refr = 1;
+    v3 refl = vr(rb_reflect(rv(h->dir), rv(inside ? w : neg(w))));
+    if (dot(refl, gn) <= 0) {
+        if (ref >= 1.0) refl = vr(rb_reflect(rv(h->dir), rv(gn)));
+        else has_refl = 0;
+    }
+    const v3 rd = vr(refr);
+    if (!inside && dot(rd, w) > -NZ) has_refr = 0;
+    else if (inside && dot(rd, w) < NZ) has_refr = 0;
+    /* Container order: the refracted ray first, then the reflected one. */
+    double kr[2];
+    v3 dr[2];
+    int ty[2], cnt = 0;
+    iorstack sts[2];
+    if (has_refr && ref < 1.0) {
+        kr[cnt] = k_refr * (1.0 - ref);
+        dr[cnt] = rd;
+        ty[cnt] = 2;
+        sts[cnt] = st_refr;
+        cnt++;
+    }
+    if (has_refl && ref != 0.0) {
+        kr[cnt] = ref;
+        dr[cnt] = refl;
+        ty[cnt] = 1;
+        sts[cnt] = *st;
+        cnt++;
+    }
+    if (cnt == 0) return 0;
+    int pick = 0;
+    double select_prob = 1.0;
+    if (cnt == 2) {
+        const double total = kr[0] + kr[1];
+        if (!(total > NZ)) return 0;
+        pick = (u_select < kr[0] / total) ? 0 : 1;
+        if (kr[pick] > NZ) select_prob = kr[pick] / total;
+    }
+    if (select_prob < NZ) return 0;
+    *dir_out = dr[pick];
+    *weight = kr[pick] / select_prob;
+    *type = ty[pick];
+    *st_out = sts[pick];
+    return 1;
+}
+
 /* Luminary selection by CDF inversion over the weights; pdf = w / total. */
 static int select_luminary(const prep* P, double u, double* pdf)
 {
@@ -398,7 +620,7 @@ static double lum_pdf_select(const prep* P, int lum)
 }
 
 /* EvaluateDirectLightingNM for area luminaries (LightSampler.cpp:1987-2269). */
-static double direct_nm(prep* P, const hit* h, double nm, uint64_t seed, uint32_t path, uint32_t dim0,
+static double direct_nm(prep* P, const hit* h, double nm, double amb, uint64_t seed, uint32_t path, uint32_t dim0,
                         uint64_t* n_shadow)
 {
     if (P->n_lums == 0) return 0.0;
@@ -432,13 +654,111 @@ static double direct_nm(prep* P, const hit* h, double nm, uint64_t seed, uint32_
     const ro_material* lm = &P->s->materials[P->s->object_material[L->object]];
     const double le = emitted(P, lm, neg(to), ln, nm);
     const double geom = L->area * cos_l / (dist * dist);
-    double contrib = le * cos_s * geom * lambert_value(P, h, to, nm);
+    double contrib = le * cos_s * geom * bsdf_value(P, h, to, nm, amb);
     if (L->area > 0 && cos_l > 0) {
         const double p_light = pdf_alias * (dist * dist) / (L->area * cos_l);
-        const double p_bsdf = lambert_pdf(h, to);
+        const double p_bsdf = bsdf_pdf(P, h, to, nm, amb);
         if (p_bsdf > 0) contrib *= power_heuristic(p_light, p_bsdf);
     }
     return contrib / pdf_alias;
+}
+
+/* Emission at a hit, weighted against NEE by the power heuristic when the
+ * ray came from a non-delta scatter (PathTracingIntegrator.cpp:4422-4515). */
+static double emission_mis(const prep* P, const hit* h, double nm, double bsdf_pdf)
+{
+    if (h->mat->emit_spd < 0) return 0.0;
+    double e = emitted(P, h->mat, neg(h->dir), h->ng, nm);
+    const int li = P->lum_of_object[h->object];
+    if (bsdf_pdf > 0 && li >= 0) {
+        const double area = P->lums[li].area;
+        const double cos_l = fabs(dot(h->dir, h->ng));
+        if (area > 0 && cos_l > 0) {
+            const double dist = mag(sub(h->p, h->origin));
+            double ps = lum_pdf_select(P, li);
+            if (ps <= 0) ps = 1.0;
+            const double p_nee = ps * (dist * dist) / (area * cos_l);
+            e *= power_heuristic(bsdf_pdf, p_nee);
+        }
+    }
+    return e;
+}
+
+/* PropagateBounceLimits (PathTransportUtilities.h:170-210): 1 if the scatter
+ * type's cap is exceeded. */
+static int over_cap(const ro_scene* s, int type, int* diffuse, int* glossy, int* transmission)
+{
+    if (type == 0) return ++*diffuse > s->max_diffuse_bounce;
+    if (type == 1) return ++*glossy > s->max_glossy_bounce;
+    return ++*transmission > s->max_transmission_bounce;
+}
+
+#define NM_LANE_DIMS (1u << 20)
+
+/* IntegrateFromHitNM: the single-wavelength loop a path continues with after
+ * a dielectric hit, starting at that hit (PathTracingIntegrator.cpp:1561-3377).
+ * Returns the radiance of lane `lane` (wavelength nm) relative to the
+ * throughput at the hit. */
+static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, double importance, double bsdf_pdf,
+                      int diffuse, int glossy, int transmission, uint64_t seed, uint32_t path, uint64_t* n_ext,
+                      uint64_t* n_shadow)
+{
+    const ro_scene* s = P->s;
+    double result = 0.0, thr = 1.0;
+    iorstack st;
+    ior_init(&st);
+    hit h = *h0;
+    v3 org = h0->origin, dir = h0->dir;
+    for (int depth = depth0; depth < MAX_PATH_DEPTH; ++depth) {
+        if (!isfinite(thr) || fabs(thr) > 1e6) break;
+        const uint32_t base = 32u * (uint32_t)(16 + depth) + NM_LANE_DIMS * (uint32_t)(lane + 1);
+        if (depth != depth0) {
+            double t;
+            ++*n_ext;
+            const int64_t tri = closest(P, org, dir, &t);
+            if (tri < 0) break;
+            make_hit(P, org, dir, tri, t, &h);
+        }
+        const double amb = ior_top(&st);
+        result += thr * emission_mis(P, &h, nm, bsdf_pdf);
+        if (h.mat->type == MAT_DIELECTRIC) {
+            v3 nd;
+            double wgt;
+            int type;
+            iorstack nst;
+            if (!dielectric_scatter(P, &h, nm, &st, rise_oracle_rng(seed, path, base + 18), &nd, &wgt, &type, &nst))
+                break;
+            if (over_cap(s, type, &diffuse, &glossy, &transmission)) break;
+            importance *= fabs(wgt);
+            thr *= wgt;
+            bsdf_pdf = 0.0;
+            st = nst;
+            org = add(h.p, mul(nd, 1e-8));
+            dir = nd;
+            continue;
+        }
+        result += thr * direct_nm(P, &h, nm, amb, seed, path, base, n_shadow);
+        const rb_sample sc = bsdf_scatter(P, &h, nm, amb, rise_oracle_rng(seed, path, base + 16),
+                                          rise_oracle_rng(seed, path, base + 17), rise_oracle_rng(seed, path, base + 18));
+        if (!sc.ok) break;
+        double scat = sc.kray;
+        if (fabs(scat) <= NZ) break;
+        if (depth >= s->rr_min_depth) {
+            const double prev = importance > s->rr_threshold ? importance : s->rr_threshold;
+            double p = importance * fabs(scat) / prev;
+            if (p > 1.0) p = 1.0;
+            if (rise_oracle_rng(seed, path, base + 19) >= p) break;
+            if (p > 0 && p < 1.0) scat /= p;
+        }
+        importance *= fabs(scat);
+        if (over_cap(s, sc.type, &diffuse, &glossy, &transmission)) break;
+        thr *= scat;
+        bsdf_pdf = sc.pdf;
+        const v3 nd = vr(sc.dir);
+        org = add(h.p, mul(nd, 1e-8));
+        dir = nd;
+    }
+    return result;
 }
 
 /* One camera sample: IntegratePixelSpectral + IntegrateRayHWSS +
@@ -470,8 +790,8 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
 
     double result[HWSS_N] = {0, 0, 0, 0};
     double thr[HWSS_N] = {1, 1, 1, 1};
-    double bsdf_pdf = 0.0;
-    int diffuse = 0;
+    double bsdf_pdf = 0.0, importance = 1.0;
+    int diffuse = 0, glossy = 0, transmission = 0;
     uint64_t n_ext = 0, n_shadow = 0;
 
     for (int depth = 0; depth < MAX_PATH_DEPTH; ++depth) {
@@ -492,51 +812,33 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         hit h;
         make_hit(P, org, dir, tri, t, &h);
 
-        /* Emission with MIS (PathTracingIntegrator.cpp:4422-4515). */
-        if (h.mat->emit_spd >= 0) {
-            for (int w = 0; w < HWSS_N; ++w) {
-                double e = emitted(P, h.mat, neg(h.dir), h.ng, lam[w]);
-                const int li = P->lum_of_object[h.object];
-                if (bsdf_pdf > 0 && li >= 0) {
-                    const double area = P->lums[li].area;
-                    const double cos_l = fabs(dot(h.dir, h.ng));
-                    if (area > 0 && cos_l > 0) {
-                        const double dist = mag(sub(h.p, h.origin));
-                        double ps = lum_pdf_select(P, li);
-                        if (ps <= 0) ps = 1.0;
-                        const double p_nee = ps * (dist * dist) / (area * cos_l);
-                        e *= power_heuristic(bsdf_pdf, p_nee);
-                    }
-                }
-                result[w] += thr[w] * e;
-            }
+        /* A material without a BSDF (the dielectric): every wavelength goes on
+         * as its own single-wavelength path (PathTracingIntegrator.cpp:4348-4365). */
+        if (h.mat->type == MAT_DIELECTRIC) {
+            for (int w = 0; w < HWSS_N; ++w)
+                result[w] += thr[w] * nm_path(P, &h, lam[w], w, depth, importance, bsdf_pdf, diffuse, glossy,
+                                              transmission, seed, path, &n_ext, &n_shadow);
+            break;
         }
+
+        /* Emission with MIS (PathTracingIntegrator.cpp:4422-4515). */
+        for (int w = 0; w < HWSS_N; ++w) result[w] += thr[w] * emission_mis(P, &h, lam[w], bsdf_pdf);
         /* Next-event estimation, one light sample per wavelength
          * (PathTracingIntegrator.cpp:4521-4552). */
         for (int w = 0; w < HWSS_N; ++w)
-            result[w] += thr[w] * direct_nm(P, &h, lam[w], seed, path, base + 4u * (uint32_t)w, &n_shadow);
+            result[w] += thr[w] * direct_nm(P, &h, lam[w], 1.0, seed, path, base + 4u * (uint32_t)w, &n_shadow);
 
-        /* LambertianSPF::ScatterNM at the hero wavelength (LambertianSPF.cpp:90-131). */
-        const double ua = rise_oracle_rng(seed, path, base + 16), ub = rise_oracle_rng(seed, path, base + 17);
-        onb fr = h.frame;
-        const int back = dot(h.dir, h.frame.w) > NZ;
-        if (back) {
-            fr.w = neg(fr.w);
-            fr.u = neg(fr.u);
-        }
-        const double phi = TWO_PI * ua, st = sqrt(ub), ct = sqrt(1.0 - ub);
-        const v3 nd = onb_xform(&fr, mk(cos(phi) * st, sin(phi) * st, ct));
-        const double pdf = fabs(dot(nd, h.frame.w)) * INV_PI;
-        const v3 neff = back ? neg(h.frame.w) : h.frame.w;
-        const v3 g = geom_against_ray(&h, neff);
-        if (!(dot(nd, g) > 0)) break;
+        /* Hero ScatterNM, then companions at the hero's direction (:4620-4718). */
+        const rb_sample hs = bsdf_scatter(P, &h, lam[0], 1.0, rise_oracle_rng(seed, path, base + 16),
+                                          rise_oracle_rng(seed, path, base + 17), rise_oracle_rng(seed, path, base + 18));
+        if (!hs.ok) break;
+        const v3 nd = vr(hs.dir);
         double sc[HWSS_N];
-        sc[0] = spd_value(&s->spds[h.mat->reflect_spd], lam[0]);
+        sc[0] = hs.kray;
         for (int w = 1; w < HWSS_N; ++w) {
-            /* Companion: BSDF value * |cos| / pdf (PathTracingIntegrator.cpp:4693-4716). */
-            double cw = lambert_value(P, &h, nd, lam[w]);
+            double cw = bsdf_value(P, &h, nd, lam[w], 1.0);
             cw *= fabs(dot(nd, h.n));
-            if (pdf > 0) cw /= pdf;
+            if (hs.pdf > 0) cw /= hs.pdf;
             sc[w] = cw > 0 ? cw : 0;
         }
         /* Russian roulette over the max across wavelengths (:4721-4768). */
@@ -554,10 +856,11 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
             if (p > 0 && p < 1.0)
                 for (int w = 0; w < HWSS_N; ++w) sc[w] *= 1.0 / p;
         }
-        /* Bounce limit (PathTransportUtilities.h:170-210). */
-        if (++diffuse > s->max_diffuse_bounce) break;
+        importance *= fabs(sc[0]);
+        /* Bounce limits (PathTransportUtilities.h:170-210). */
+        if (over_cap(s, hs.type, &diffuse, &glossy, &transmission)) break;
         for (int w = 0; w < HWSS_N; ++w) thr[w] *= sc[w];
-        bsdf_pdf = pdf;
+        bsdf_pdf = hs.pdf;
         /* currentRay = scattered ray from ptIntersection; Advance(1e-8). */
         org = add(h.p, mul(nd, 1e-8));
         dir = nd;
@@ -619,7 +922,10 @@ int rise_oracle_render(const ro_scene* s, int32_t s0, int32_t s1, double* xyz_su
         const int32_t mi = s->object_material[o];
         if (mi < 0 || mi >= s->n_materials) return -1;
         const ro_material* m = &s->materials[mi];
-        if (m->type != 0 || m->reflect_spd < 0 || m->reflect_spd >= s->n_spds || m->emit_spd >= s->n_spds) return -1;
+        if (m->type < 0 || m->type > 2 || m->reflect_spd >= s->n_spds || m->emit_spd >= s->n_spds ||
+            m->spec_spd >= s->n_spds || m->ior_spd >= s->n_spds || m->ext_spd >= s->n_spds)
+            return -1;
+        if (m->type == 0 && m->reflect_spd < 0) return -1;
     }
     prep P;
     prepare(&P, s);

@@ -4,10 +4,15 @@
  * baseline leg use it as the checker or as the timed CPU port; the product
  * path never links it).
  *
- * Restates, for Lambertian surfaces and Lambertian area luminaires (paths
- * relative to the reference's src/Library/):
+ * Restates, for Lambertian, GGX conductor (pt_bsdf.c) and smooth dielectric
+ * surfaces and Lambertian area luminaires (paths relative to the reference's
+ * src/Library/):
  *   Rendering/PathTracingSpectralRasterizer.cpp:121-198, 206-479  pixel/sample loop, XYZ film
  *   Shaders/PathTracingIntegrator.cpp:3939-4822, 4855-5126          IntegrateRayHWSS / IntegrateFromHitHWSS
+ *   Shaders/PathTracingIntegrator.cpp:1561-3377                     IntegrateFromHitNM (single-wavelength
+ *                                                                   loop after a dielectric hit)
+ *   Materials/ScatteredRayContainer.cpp:43-92                       lobe selection by krayNM
+ *   Utilities/IORStack.h                                            index-of-refraction stack
  *   Lights/LightSampler.cpp:1942-2362                               EvaluateDirectLightingNM (area luminaries)
  *   Materials/LambertianBRDF.cpp:44-93, LambertianSPF.cpp:90-172    BRDF, sampling, pdf
  *   Materials/LambertianEmitter.cpp:82-89                           one-sided emission
@@ -25,6 +30,11 @@
  * a counter-based hash of (seed, path, dimension) with the same stream
  * layout, the same function the HIP renderer uses, so both follow the same
  * random decisions and their per-sample results can be compared directly.
+ * Dimensions per bounce (stream 16 + depth, base 32 * stream): +4w..+4w+3
+ * the light sample of wavelength w, +16 and +17 the BSDF direction, +18 the
+ * lobe choice, +19 Russian roulette.  After a dielectric hit the path goes on
+ * as four single-wavelength paths; lane w adds (w + 1) << 20 to every
+ * dimension of its bounces.
  */
 #ifndef RISE_PT_ORACLE_H
 #define RISE_PT_ORACLE_H
@@ -42,10 +52,14 @@ typedef struct ro_spd {
 } ro_spd;
 
 typedef struct ro_material {
-    int32_t type;          /* 0 = Lambertian */
-    int32_t reflect_spd;   /* reflectance SPD index */
+    int32_t type;          /* 0 Lambertian, 1 GGX conductor, 2 smooth dielectric */
+    int32_t reflect_spd;   /* Lambertian reflectance / GGX diffuse (-1: none) */
     int32_t emit_spd;      /* -1: not an emitter */
     double emit_scale;     /* LambertianEmitter scale */
+    int32_t spec_spd;      /* GGX specular tint */
+    int32_t ior_spd;       /* GGX conductor n / dielectric index of refraction */
+    int32_t ext_spd;       /* GGX conductor k / dielectric transmittance per unit length (-1: 1) */
+    double alpha;          /* GGX roughness (isotropic) */
 } ro_material;
 
 typedef struct ro_scene {

@@ -24,11 +24,17 @@ struct DevSpd {
     int pad[3];
 };
 
+enum : int { kMatLambert = 0, kMatGgx = 1, kMatDielectric = 2 };
+
 struct DevMaterial {
     int type;
-    int refl_spd;
+    int refl_spd;   // Lambertian reflectance / GGX diffuse (-1: none)
     int emit_spd;   // -1: none
     float emit_scale;
+    int spec_spd;   // GGX specular tint
+    int ior_spd;    // GGX conductor n / dielectric index (-1: the reference's defaults)
+    int ext_spd;    // GGX conductor k / dielectric transmittance per unit length (-1: none)
+    float alpha;    // GGX roughness
 };
 
 struct DevObject {

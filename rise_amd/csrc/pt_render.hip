@@ -30,7 +30,9 @@
 
 #include "api_common.h"
 #include "cie_table.h"
+#include "pt_ggx.h"
 #include "pt_internal.h"
+#include "wave_fetch.h"
 
 #include <hip/hip_runtime.h>
 
@@ -54,7 +56,7 @@ constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 256;
 constexpr int64_t kChunkPaths = int64_t(1) << 20;   // paths per chunk (at most)
 constexpr int64_t kMaxPaths = int64_t(1) << 22;     // paths in flight per pass (at most)
-constexpr int kCounters = 2 * (kMaxPathDepth + 1);  // extension counts, then shadow counts, per pass
+constexpr int kCounters = 5 * (kMaxPathDepth + 1);  // extension, shadow, then 3 binned-hit counts, per pass
 
 // ColorUtils::CIE_Y_Integral (Utilities/Color/ColorUtils.cpp:327-350).
 double cie_y_integral(double a, double b)
@@ -102,25 +104,34 @@ struct PassEntry {
     int req, p0, np, s0, ns, slot0, film0, pad;
 };
 
-// Path state, structure of arrays, `cap` entries each.
+// Path state, structure of arrays.  A path is one slot; its rays are kept per
+// lane (4 * cap entries, index lane * cap + slot): a path whose wavelengths
+// share one ray uses lane 0, and after a dielectric hit lane w carries
+// wavelength w on its own.
 struct Paths {
     int cap;
-    float *ox, *oy, *oz, *dx, *dy, *dz;
-    float* thr;      // 4 * cap, wavelength-major
-    float* res;      // 4 * cap
-    float* lambda0;
-    float* pdf;      // bsdfPdf of the ray being traced (0: camera or delta)
+    float *ox, *oy, *oz, *dx, *dy, *dz;  // 4 * cap
+    float* thr;      // 4 * cap, per wavelength
+    float* res;      // 4 * cap, per wavelength
+    float* lambda0;  // cap
+    float* pdf;      // 4 * cap: bsdfPdf of the ray being traced (0: camera or delta)
+    float* imp;      // 4 * cap: importance (product of scatter weights)
+    unsigned* state; // 4 * cap: bounce counts (bits 0-7 diffuse, 8-15 glossy, 16-23 transmission), IOR-stack depth (24-25)
+    int* ior_obj;    // 3 * 4 * cap: objects on the IOR stack above the ambient air
+    unsigned* flags; // cap: kModeNm once the wavelengths go on as separate lanes
     unsigned* path_id;
     unsigned* req;
-    unsigned* bounces;   // diffuse count (PropagateBounceLimits)
-    unsigned* hit_leaf;
-    float* hit_t;
-    unsigned* queue[2];
+    unsigned* hit_leaf;  // 4 * cap
+    float* hit_t;        // 4 * cap
+    unsigned* queue[2];  // 4 * cap entries, slot * 4 + lane
+    unsigned* mqueue[3]; // hits binned by material kind, 4 * cap entries each
     // shadow records, 4 * cap
     float *sox, *soy, *soz, *sdx, *sdy, *sdz, *stmax, *sval;
     unsigned* sslot;     // slot * 4 + wavelength
     unsigned* cnt_ext;   // [kMaxPathDepth + 1] of the current pass
     unsigned* cnt_sh;    // [kMaxPathDepth + 1]
+    unsigned* cnt_mat;   // [3][kMaxPathDepth + 1]: binned hits per material kind
+    unsigned long long* fetch;  // [2 * (kMaxPathDepth + 1)]: work counters of the persistent traversal kernels
 };
 
 // Wave-wide compaction: one atomic per wave, ranks from the ballot.
@@ -196,66 +207,374 @@ __global__ void k_generate(const ReqInfo* __restrict__ reqs, const PassEntry* __
         }
         S.lambda0[i] = R.lambda_begin + ul * (R.lambda_end - R.lambda_begin);
         S.pdf[i] = 0.0f;
+        S.imp[i] = 1.0f;
+        S.state[i] = 0u;
+        S.flags[i] = 0u;
         S.path_id[i] = path;
         S.req[i] = (unsigned)E.req;
-        S.bounces[i] = 0u;
-        S.queue[0][i] = (unsigned)i;
+        S.queue[0][i] = (unsigned)i * 4u;
     }
 }
 
 // Closest hit for the active queue (ObjectManager::IntersectRay).
-template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+// PERSIST: persistent waves take queue entries with a wave-wide fetch
+// (wave_fetch.h) and step one traversal iteration at a time, so lanes that
+// finish early are refilled; otherwise one thread traces one ray to the end
+// (grid-stride), which costs less per ray when trees are shallow.
+template <int STACK, int BLOCK, bool PERSIST>
+__global__ __launch_bounds__(BLOCK) void k_extend(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
-    __shared__ uint2 lds[STACK > 0 ? STACK * kTraceBlock : 1];
+    __shared__ uint2 lds[STACK > 0 ? STACK * BLOCK : 1];
+    LdsStack<STACK> lst{lds + threadIdx.x, BLOCK, 0};
+    PrivStack pst;
+    pst.sp = 0;
     const unsigned n = S.cnt_ext[depth];
     const unsigned* q = S.queue[depth & 1];
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const unsigned slot = q[i];
-        const unsigned rq = S.req[slot];
-        const NodeSlot* nodes = reqs[rq].sc.nodes;
-        const Tri48* tris = reqs[rq].sc.tris;
-        RayState r;
-        init_ray(r, S.ox[slot], S.oy[slot], S.oz[slot], S.dx[slot], S.dy[slot], S.dz[slot]);
-        double best;
-        unsigned leaf, nn = 0, nt = 0;
-        int id;
-        if (STACK > 0) {
-            LdsStack<STACK> st{lds + threadIdx.x, kTraceBlock, 0};
-            id = trace_closest<false>(r, nodes, tris, st, best, leaf, nn, nt);
-        } else {
-            PrivStack st;
-            st.sp = 0;
-            id = trace_closest<false>(r, nodes, tris, st, best, leaf, nn, nt);
+    if constexpr (!PERSIST) {
+        for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            const unsigned code = q[i];
+            const unsigned slot = (code & 3u) * (unsigned)S.cap + (code >> 2);
+            const ReqInfo& R = reqs[S.req[code >> 2]];
+            RayState r;
+            init_ray(r, S.ox[slot], S.oy[slot], S.oz[slot], S.dx[slot], S.dy[slot], S.dz[slot]);
+            double best;
+            unsigned leaf, nn = 0, nt = 0;
+            lst.sp = 0;
+            pst.sp = 0;
+            const int id = STACK > 0 ? trace_closest<false>(r, R.sc.nodes, R.sc.tris, lst, best, leaf, nn, nt)
+                                     : trace_closest<false>(r, R.sc.nodes, R.sc.tris, pst, best, leaf, nn, nt);
+            S.hit_leaf[slot] = id >= 0 ? leaf : 0xffffffffu;
+            S.hit_t[slot] = id >= 0 ? (float)best : 0.0f;
         }
-        S.hit_leaf[slot] = id >= 0 ? leaf : 0xffffffffu;
-        S.hit_t[slot] = id >= 0 ? (float)best : 0.0f;
+        return;
+    }
+    unsigned long long* next = S.fetch + depth;
+    RayState r;
+    ClosestState s;
+    const NodeSlot* nodes = nullptr;
+    const Tri48* tris = nullptr;
+    unsigned slot = 0, nn = 0, nt = 0;  // slot: lane * cap + path slot of the current item
+    long long item = -1;  // -1 idle, -2 exhausted
+    for (;;) {
+        const long long got = refill(item == -1, item >= 0, next, n);
+        if (item == -1 && got != -1) {
+            item = got;
+            if (item >= 0) {
+                const unsigned code = q[item];
+                slot = (code & 3u) * (unsigned)S.cap + (code >> 2);
+                const ReqInfo& R = reqs[S.req[code >> 2]];
+                nodes = R.sc.nodes;
+                tris = R.sc.tris;
+                init_ray(r, S.ox[slot], S.oy[slot], S.oz[slot], S.dx[slot], S.dy[slot], S.dz[slot]);
+                closest_begin(r, s);
+                lst.sp = 0;
+                pst.sp = 0;
+            }
+        }
+        if (__ballot(item >= 0) == 0ull && __ballot(item == -1) == 0ull) break;
+        if (item >= 0) {
+            const bool more = STACK > 0 ? closest_step<false>(r, nodes, tris, lst, s, nn, nt)
+                                        : closest_step<false>(r, nodes, tris, pst, s, nn, nt);
+            if (!more) {
+                S.hit_leaf[slot] = s.best_id >= 0 ? s.best_leaf : 0xffffffffu;
+                S.hit_t[slot] = s.best_id >= 0 ? (float)s.best_t : 0.0f;
+                item = -1;
+            }
+        }
     }
 }
 
-// One bounce of IntegrateFromHitHWSS for the active queue.  Every lane of a
-// wave runs the same loop trips (the queue bound is uniform per block) so the
-// compaction ballots see all lanes; each shadow record is written as soon as
-// it is made, which keeps one record live at a time.
-__global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+// ---- materials at one wavelength ------------------------------------------------
+
+constexpr GgxTables make_ggx_tables()
+{
+    GgxTables t{};
+    constexpr double eavg[kLut] = {RISE_GGX_EAVG_VALUES};
+    constexpr double glx[21] = {RISE_GL21_NODES};
+    constexpr double glw[21] = {RISE_GL21_WEIGHTS};
+    const std::array<float, kLut> z = make_row_z();
+    for (int i = 0; i < kLut * kLut; ++i) t.ess[i] = (float)kEssTable[i];
+    for (int i = 0; i < kLut; ++i) {
+        t.eavg[i] = (float)eavg[i];
+        t.row_z[i] = z[i];
+    }
+    for (int i = 0; i < 21; ++i) {
+        t.gl_x[i] = (float)glx[i];
+        t.gl_w[i] = (float)glw[i];
+    }
+    return t;
+}
+
+__constant__ GgxTables c_ggx = make_ggx_tables();
+
+// Painter defaults of the scene-file chunks when a spectrum is absent:
+// ggx_material ior 2.45, extinction 3.45; dielectric_material ior 1.33, tau
+// none (ChunkParserRegistry.cpp:3206-3207, 3835-3836).
+__device__ inline float spd_or(const PtSceneView& sc, int spd, float nm, float fallback)
+{
+    return spd >= 0 ? spd_value(sc, spd, nm) : fallback;
+}
+
+__device__ inline GgxParams ggx_params(const PtSceneView& sc, const DevMaterial& m, float nm, float amb)
+{
+    return GgxParams{spd_or(sc, m.refl_spd, nm, 0.0f), spd_or(sc, m.spec_spd, nm, 0.0f), m.alpha,
+                     spd_or(sc, m.ior_spd, nm, 2.45f), spd_or(sc, m.ext_spd, nm, 3.45f), amb};
+}
+
+// IBSDF::valueNM of a non-delta material of kind MAT (the dielectric has no BSDF).
+template <int MAT>
+__device__ inline float bsdf_value(const PtSceneView& sc, const DevMaterial& m, const Surf& h, f3 v, float nm,
+                                   float amb)
+{
+    if (MAT == kMatLambert) return lambert_value(sc, h, m.refl_spd, v, nm);
+    if (MAT == kMatGgx) return ggx_value(c_ggx, h, ggx_params(sc, m, nm, amb), v);
+    return 0.0f;
+}
+
+// ISPF::PdfNM.
+template <int MAT>
+__device__ inline float bsdf_pdf(const PtSceneView& sc, const DevMaterial& m, const Surf& h, f3 wo, float nm,
+                                 float amb)
+{
+    if (MAT == kMatLambert) return lambert_pdf(h, wo);
+    if (MAT == kMatGgx) return ggx_pdf(c_ggx, h, ggx_params(sc, m, nm, amb), wo);
+    return 0.0f;
+}
+
+// ISPF::ScatterNM of a non-delta material: one ray or none.
+template <int MAT>
+__device__ inline ScatterOut bsdf_scatter(const PtSceneView& sc, const DevMaterial& m, const Surf& h, float nm,
+                                          float amb, float u1, float u2, float u3)
+{
+    if (MAT == kMatGgx) return ggx_scatter(c_ggx, h, ggx_params(sc, m, nm, amb), u3, u1, u2);
+    // LambertianSPF::ScatterNM (Materials/LambertianSPF.cpp:90-131).
+    ScatterOut out{false, mk3(0, 0, 0), 0.0f, 0.0f, 0};
+    Onb fr = h.frame;
+    const bool back = dot3(h.dir, h.frame.w) > kNearZeroF;
+    if (back) {
+        fr.w = -fr.w;
+        fr.u = -fr.u;
+    }
+    const float phi = kTwoPi * u1, st = sqrtf(u2), ct = sqrtf(1.0f - u2);
+    const f3 nd = onb_xform(fr, mk3(cosf(phi) * st, sinf(phi) * st, ct));
+    const f3 g = geom_against_ray(h, back ? -h.frame.w : h.frame.w);
+    if (dot3(nd, g) > 0.0f)
+        out = ScatterOut{true, nd, spd_value(sc, m.refl_spd, nm), fabsf(dot3(nd, h.frame.w)) * kInvPi, 0};
+    return out;
+}
+
+// Index-of-refraction stack of one single-wavelength path (IORStack.h):
+// objects above the ambient air; the index of an entry is its dielectric's
+// index at the path's wavelength.  At most 3 nested objects; a deeper push is
+// dropped.
+struct IorStack {
+    int n;
+    int obj[3];
+};
+
+__device__ inline float ior_of(const PtSceneView& sc, int obj, float nm)
+{
+    return spd_or(sc, sc.materials[sc.objects[obj].material].ior_spd, nm, 1.33f);
+}
+
+__device__ inline float ior_top(const PtSceneView& sc, const IorStack& s, float nm)
+{
+    return s.n > 0 ? ior_of(sc, s.obj[s.n - 1], nm) : 1.0f;
+}
+
+__device__ inline bool ior_contains(const IorStack& s, int o)
+{
+    bool c = false;
+    for (int i = 0; i < 3; ++i) c = c || (i < s.n && s.obj[i] == o);
+    return c;
+}
+
+__device__ inline void ior_push(IorStack& s, int o)
+{
+    if (s.n < 3) s.obj[s.n++] = o;
+}
+
+__device__ inline void ior_pop(IorStack& s, int o)
+{
+    int at = -1;
+    for (int i = 0; i < 3; ++i)
+        if (i < s.n && s.obj[i] == o) at = i;
+    if (at < 0) return;
+    for (int i = at; i < 2; ++i) s.obj[i] = s.obj[i + 1];
+    s.n--;
+}
+
+// DielectricSPF::ScatterNM + GenerateScatteredRay, smooth (DielectricSPF.cpp:
+// 120-287, 395-432), then ScatteredRayContainer::RandomlySelect by krayNM
+// (ScatteredRayContainer.cpp:43-92) and the integrator's division by the
+// select probability (PathTracingIntegrator.cpp:2650-2670).  `from` is the
+// origin of the ray that hit (for absorption over the segment inside).
+__device__ inline bool dielectric_scatter(const PtSceneView& sc, const DevMaterial& m, const Surf& h, f3 from,
+                                          float nm, const IorStack& st, float u_select, f3& dir, float& weight,
+                                          int& type, IorStack& st_out)
+{
+    const bool inside = ior_contains(st, h.object);
+    const float rindex = spd_or(sc, m.ior_spd, nm, 1.33f);
+    const float k_refr = inside ? powf(spd_or(sc, m.ext_spd, nm, 1.0f), len3(h.p - from)) : 1.0f;
+    const f3 w = h.frame.w;
+    const f3 neff = inside ? -w : w;
+    const f3 graw = dot3(h.ng, h.ng) > 1e-12f ? h.ng : neff;
+    const f3 gn = dot3(graw, neff) >= 0.0f ? graw : -graw;
+    f3 refr = h.dir;
+    float ref;
+    IorStack st_refr = st;
+    if (inside) {
+        IorStack exit_st = st;
+        ior_pop(exit_st, h.object);
+        const float exit_ior = ior_top(sc, exit_st, nm);
+        if (refract3(-w, rindex, exit_ior, refr)) {
+            ior_pop(st_refr, h.object);
+            ref = dielectric_reflectance(h.dir, refr, -w, rindex, exit_ior);
+        } else {
+            ref = 1.0f;
+        }
+    } else {
+        const float top = ior_top(sc, st, nm);
+        if (refract3(w, top, rindex, refr)) {
+            ref = dielectric_reflectance(h.dir, refr, w, top, rindex);
+            ior_push(st_refr, h.object);
+        } else {
+            ref = 1.0f;
+        }
+    }
+    bool has_refl = true, has_refr = true;
+    f3 refl = reflect3(h.dir, inside ? w : -w);
+    if (dot3(refl, gn) <= 0.0f) {
+        if (ref >= 1.0f) refl = reflect3(h.dir, gn);
+        else has_refl = false;
+    }
+    if (!inside && dot3(refr, w) > -kNearZeroF) has_refr = false;
+    else if (inside && dot3(refr, w) < kNearZeroF) has_refr = false;
+    const bool a = has_refr && ref < 1.0f;   // the refracted ray is added first
+    const bool b = has_refl && ref != 0.0f;
+    if (!a && !b) return false;
+    const float ka = k_refr * (1.0f - ref), kb = ref;
+    bool pick_a = a;
+    float select = 1.0f;
+    if (a && b) {
+        const float total = ka + kb;
+        if (!(total > kNearZeroF)) return false;
+        pick_a = u_select < ka / total;
+        const float kp = pick_a ? ka : kb;
+        if (kp > kNearZeroF) select = kp / total;
+    }
+    if (select < kNearZeroF) return false;
+    if (pick_a) {
+        dir = refr;
+        weight = ka / select;
+        type = 2;
+        st_out = st_refr;
+    } else {
+        dir = refl;
+        weight = kb / select;
+        type = 1;
+        st_out = st;
+    }
+    return true;
+}
+
+// PropagateBounceLimits (PathTransportUtilities.h:170-210) on the packed
+// counters (bits 0-7 diffuse, 8-15 glossy, 16-23 transmission): false when
+// the scatter type's cap is exceeded.
+__device__ inline bool bump_bounces(unsigned& st, int type, const ReqInfo& R)
+{
+    const int shift = 8 * type;
+    const unsigned c = ((st >> shift) & 0xffu) + 1u;
+    const int cap = type == 0 ? R.max_diffuse : (type == 1 ? R.max_glossy : R.max_transmission);
+    if ((int)c > cap) return false;
+    st = (st & ~(0xffu << shift)) | (c << shift);
+    return true;
+}
+
+// Bins the extension queue's hits by material kind (misses end their path),
+// so each shading kernel runs one material's code over a dense queue.
+__global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
     const unsigned n = S.cnt_ext[depth];
-    const unsigned* qin = S.queue[depth & 1];
+    const unsigned* q = S.queue[depth & 1];
+    unsigned* cnt = S.cnt_mat + depth;
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned b = blockIdx.x * blockDim.x; b < n; b += stride) {
+        const unsigned i = b + threadIdx.x;
+        int kind = -1;
+        unsigned code = 0;
+        if (i < n) {
+            code = q[i];
+            const unsigned leaf = S.hit_leaf[(code & 3u) * (unsigned)S.cap + (code >> 2)];
+            if (leaf != 0xffffffffu) {
+                const PtSceneView& sc = reqs[S.req[code >> 2]].sc;
+                const int obj = float_to_bits(reinterpret_cast<const float4*>(sc.tris + leaf)[1].w);
+                kind = sc.materials[sc.objects[obj].material].type;
+            }
+        }
+        for (int m = 0; m < 3; ++m) {
+            const unsigned at = wave_push(cnt + m * (kMaxPathDepth + 1), kind == m);
+            if (kind == m) S.mqueue[m][at] = code;
+        }
+    }
+}
+
+constexpr unsigned kLaneDims = 1u << 20;  // dimension offset of single-wavelength lane w: (w + 1) << 20
+constexpr unsigned kModeNm = 1u;          // flags bit: the path continues as single-wavelength lanes
+
+// Writes lane w's next ray, offset off the surface toward its side, and
+// appends it to the next extension queue at `at`.
+__device__ inline void write_continuation(Paths& S, const Surf& h, unsigned slot, unsigned w, unsigned* at, f3 dir,
+                                          float pdf, float imp, unsigned bounces, const IorStack& st)
+{
+    const unsigned cap = (unsigned)S.cap, lw = w * cap + slot;
+    *at = slot * 4u + w;
+    const f3 no = offset_ray(h.p, dot3(dir, h.ng) > 0.0f ? h.ng : -h.ng);
+    S.ox[lw] = no.x;
+    S.oy[lw] = no.y;
+    S.oz[lw] = no.z;
+    S.dx[lw] = dir.x;
+    S.dy[lw] = dir.y;
+    S.dz[lw] = dir.z;
+    S.pdf[lw] = pdf;
+    S.imp[lw] = imp;
+    S.state[lw] = (bounces & 0x00ffffffu) | ((unsigned)st.n << 24);
+    for (int k = 0; k < 3; ++k)
+        if (k < st.n) S.ior_obj[k * 4 * cap + lw] = st.obj[k];
+}
+
+// One bounce for the active queue: IntegrateFromHitHWSS
+// (PathTracingIntegrator.cpp:3939-4822) for paths still sharing one ray, and
+// the single-wavelength loop IntegrateFromHitNM (:1561-3377) for paths that
+// hit a dielectric, whose wavelengths then continue as separate lanes.  Every
+// thread of a wave runs the same loop trips (the queue bound is uniform per
+// block) so the compaction ballots see all lanes; each shadow record is
+// written as soon as it is made, which keeps one record live at a time.
+template <int MAT>
+__global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+{
+    const unsigned n = S.cnt_mat[MAT * (kMaxPathDepth + 1) + depth];
+    const unsigned* qin = S.mqueue[MAT];
     unsigned* qout = S.queue[(depth + 1) & 1];
     unsigned* cnt_next = S.cnt_ext + depth + 1;
     unsigned* cnt_sh = S.cnt_sh + depth;
     const unsigned base_dim = 32u * (unsigned)(16 + depth);
     const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned cap = (unsigned)S.cap;
     for (unsigned b = blockIdx.x * blockDim.x; b < n; b += stride) {
         const unsigned i = b + threadIdx.x;
-        unsigned slot = 0, leaf = 0xffffffffu;
+        unsigned slot = 0, lane = 0, li = 0, leaf = 0xffffffffu;
         if (i < n) {
-            slot = qin[i];
-            leaf = S.hit_leaf[slot];
+            const unsigned code = qin[i];
+            slot = code >> 2;
+            lane = code & 3u;
+            li = lane * cap + slot;
+            leaf = S.hit_leaf[li];
         }
-        const bool valid = leaf != 0xffffffffu;  // a miss ends the path (no environment)
-        const ReqInfo* R = reqs + (valid ? S.req[slot] : 0u);
-        const PtSceneView& sc = R->sc;
+        const bool valid = i < n;  // binned entries are hits on a material of kind MAT
+        const ReqInfo& R = reqs[valid ? S.req[slot] : 0u];
+        const PtSceneView& sc = R.sc;
+        const bool nm = valid && (S.flags[slot] & kModeNm);
 
         // Hit record (TriangleMeshGeometryIndexed + Object layers).
         Surf h;
@@ -263,13 +582,20 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
         DevMaterial mat{};
         unsigned path = 0;
         f3 o = mk3(0, 0, 0);
-        float thr[kHwss] = {0, 0, 0, 0}, lam[kHwss] = {0, 0, 0, 0};
+        float lam[kHwss] = {0, 0, 0, 0};
+        float pdf_in = 0.0f, imp = 1.0f;
+        unsigned bst = 0;
+        IorStack ist{0, {0, 0, 0}};
         if (valid) {
             path = S.path_id[slot];
-            o = mk3(S.ox[slot], S.oy[slot], S.oz[slot]);
-            const f3 d = mk3(S.dx[slot], S.dy[slot], S.dz[slot]);
-            for (int w = 0; w < kHwss; ++w) thr[w] = S.thr[w * S.cap + slot];
-            wavelengths(S.lambda0[slot], R->lambda_begin, R->lambda_end, lam);
+            o = mk3(S.ox[li], S.oy[li], S.oz[li]);
+            const f3 d = mk3(S.dx[li], S.dy[li], S.dz[li]);
+            wavelengths(S.lambda0[slot], R.lambda_begin, R.lambda_end, lam);
+            pdf_in = S.pdf[li];
+            imp = S.imp[li];
+            bst = S.state[li];
+            ist.n = (int)((bst >> 24) & 3u);
+            for (int k = 0; k < 3; ++k) ist.obj[k] = k < ist.n ? S.ior_obj[k * 4 * cap + li] : 0;
             const float4* tp = reinterpret_cast<const float4*>(sc.tris + leaf);
             const float4 ta = tp[0], tb = tp[1], tc = tp[2];
             const f3 v0 = mk3(ta.x, ta.y, ta.z), v1 = mk3(tb.x, tb.y, tb.z), v2 = mk3(tc.x, tc.y, tc.z);
@@ -277,7 +603,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             ob = sc.objects[h.object];
             h.material = ob.material;
             mat = sc.materials[ob.material];
-            h.p = o + d * S.hit_t[slot];
+            h.p = o + d * S.hit_t[li];
             h.dir = d;
             const f3 fn = norm3(cross3(v1 - v0, v2 - v0));
             h.n = fn;
@@ -287,42 +613,46 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 if (dot3(h.ng, d) > 0.0f) h.ng = -h.ng;
             }
             h.frame = onb_from_w(h.n);
+        }
+        // A material without a BSDF hit by a shared ray: every wavelength goes
+        // on as its own lane from this hit (:4348-4365).
+        const bool branch = valid && !nm && MAT == kMatDielectric;
+        const bool surface = valid && MAT != kMatDielectric;  // emission, NEE, BSDF sampling
+        const float amb = nm ? ior_top(sc, ist, lam[lane]) : 1.0f;
 
-            // Emission with MIS against NEE (PathTracingIntegrator.cpp:4422-4515).
-            if (mat.emit_spd >= 0) {
-                const float bsdf_pdf = S.pdf[slot];
-                float mis = 1.0f;
-                if (bsdf_pdf > 0.0f && ob.luminary >= 0) {
-                    const DevLuminary L = sc.lums[ob.luminary];
-                    const float cos_l = fabsf(dot3(d, h.ng));
-                    if (L.area > 0.0f && cos_l > 0.0f) {
-                        const float dist = len3(h.p - o);
-                        const float ps = L.pdf_select > 0.0f ? L.pdf_select : 1.0f;
-                        const float p_nee = ps * (dist * dist) / (L.area * cos_l);
-                        mis = power_heuristic(bsdf_pdf, p_nee);
-                    }
+        // Emission with MIS against NEE (:4422-4515).
+        if (MAT == kMatLambert && valid && mat.emit_spd >= 0) {
+            float mis = 1.0f;
+            if (pdf_in > 0.0f && ob.luminary >= 0) {
+                const DevLuminary L = sc.lums[ob.luminary];
+                const float cos_l = fabsf(dot3(h.dir, h.ng));
+                if (L.area > 0.0f && cos_l > 0.0f) {
+                    const float dist = len3(h.p - o);
+                    const float ps = L.pdf_select > 0.0f ? L.pdf_select : 1.0f;
+                    mis = power_heuristic(pdf_in, ps * (dist * dist) / (L.area * cos_l));
                 }
-                for (int w = 0; w < kHwss; ++w)
-                    S.res[w * S.cap + slot] += thr[w] * (emitted(sc, mat, -d, h.ng, lam[w]) * mis);
             }
+            for (int w = 0; w < kHwss; ++w)
+                if (!nm || (unsigned)w == lane)
+                    S.res[w * cap + slot] += S.thr[w * cap + slot] * (emitted(sc, mat, -h.dir, h.ng, lam[w]) * mis);
         }
 
-        // Next-event estimation, one light sample per wavelength
-        // (PathTracingIntegrator.cpp:4521-4552 -> LightSampler::EvaluateDirectLightingNM,
-        // Lights/LightSampler.cpp:1987-2269).
+        // Next-event estimation: one light sample per wavelength of a shared
+        // ray, one for a single-wavelength lane (:4521-4552 and the NM twin ->
+        // LightSampler::EvaluateDirectLightingNM, Lights/LightSampler.cpp:1987-2269).
         for (int w = 0; w < kHwss; ++w) {
             bool has = false;
             float v = 0.0f, stm = 0.0f;
             f3 sorg = mk3(0, 0, 0), to = mk3(0, 0, 0);
-            if (valid && sc.n_lums > 0) {
-                const unsigned dim = base_dim + 4u * (unsigned)w;
-                const float usel = rng_u(R->seed, path, dim);
-                int li = 0;
-                while (li < sc.n_lums - 1 && !(usel < sc.lums[li].cdf_select)) ++li;
-                if (li != ob.luminary) {  // never the shading object itself (:2061)
-                    const DevLuminary L = sc.lums[li];
-                    const float ux = rng_u(R->seed, path, dim + 1), uy = rng_u(R->seed, path, dim + 2),
-                                uz = rng_u(R->seed, path, dim + 3);
+            if (surface && sc.n_lums > 0 && (!nm || (unsigned)w == lane)) {
+                const unsigned dim = nm ? base_dim + kLaneDims * (lane + 1u) : base_dim + 4u * (unsigned)w;
+                const float usel = rng_u(R.seed, path, dim);
+                int li_ = 0;
+                while (li_ < sc.n_lums - 1 && !(usel < sc.lums[li_].cdf_select)) ++li_;
+                if (li_ != ob.luminary) {  // never the shading object itself (:2061)
+                    const DevLuminary L = sc.lums[li_];
+                    const float ux = rng_u(R.seed, path, dim + 1), uy = rng_u(R.seed, path, dim + 2),
+                                uz = rng_u(R.seed, path, dim + 3);
                     // UniformRandomPoint: first CDF entry >= uz, then PointOnTriangle
                     // (TriangleMeshGeometryIndexed.cpp:207-218, GeometricUtilities.cpp:543-560).
                     int k = 0;
@@ -343,12 +673,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                         const DevMaterial lm = sc.materials[L.material];
                         const float le = emitted(sc, lm, -to, ln, lam[w]);
                         const float geom = L.area * cos_l / (dist * dist);
-                        float contrib = le * cos_s * geom * lambert_value(sc, h, mat.refl_spd, to, lam[w]);
+                        float contrib = le * cos_s * geom * bsdf_value<MAT>(sc, mat, h, to, lam[w], amb);
                         const float p_light = L.pdf_select * (dist * dist) / (L.area * cos_l);
-                        const float p_bsdf = lambert_pdf(h, to);
+                        const float p_bsdf = bsdf_pdf<MAT>(sc, mat, h, to, lam[w], amb);
                         if (p_bsdf > 0.0f) contrib *= power_heuristic(p_light, p_bsdf);
                         contrib /= L.pdf_select;
-                        v = thr[w] * contrib;
+                        v = S.thr[w * cap + slot] * contrib;
                         if (v > 0.0f) {
                             has = true;
                             sorg = offset_ray(h.p, dot3(to, h.ng) > 0.0f ? h.ng : -h.ng);
@@ -371,80 +701,117 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             }
         }
 
-        // LambertianSPF::ScatterNM at the hero wavelength (Materials/LambertianSPF.cpp:90-131),
-        // companions, Russian roulette, bounce caps.
-        bool cont = false;
-        f3 new_o = mk3(0, 0, 0), nd = mk3(0, 0, 0);
-        if (valid) {
-            const float ua = rng_u(R->seed, path, base_dim + 16), ub = rng_u(R->seed, path, base_dim + 17);
-            Onb fr = h.frame;
-            const bool back = dot3(h.dir, h.frame.w) > kNearZeroF;
-            if (back) {
-                fr.w = -fr.w;
-                fr.u = -fr.u;
-            }
-            const float phi = kTwoPi * ua, st = sqrtf(ub), ct = sqrtf(1.0f - ub);
-            nd = onb_xform(fr, mk3(cosf(phi) * st, sinf(phi) * st, ct));
-            const float pdf = fabsf(dot3(nd, h.frame.w)) * kInvPi;
-            const f3 neff = back ? -h.frame.w : h.frame.w;
-            const f3 g = geom_against_ray(h, neff);
-            if (dot3(nd, g) > 0.0f) {
-                float sc4[kHwss];
-                sc4[0] = spd_value(sc, mat.refl_spd, lam[0]);
-                for (int w = 1; w < kHwss; ++w) {
-                    // Companion weight: BSDF value * |cos| / pdf (:4693-4716).
-                    float cw = lambert_value(sc, h, mat.refl_spd, nd, lam[w]) * fabsf(dot3(nd, h.n));
-                    if (pdf > 0.0f) cw /= pdf;
-                    sc4[w] = cw > 0.0f ? cw : 0.0f;
-                }
-                // Russian roulette over the max across wavelengths (:4721-4768,
-                // PathTransportUtilities.h:86-110).
-                float mprev = fabsf(thr[0]), mcurr = fabsf(thr[0] * sc4[0]);
-                for (int w = 1; w < kHwss; ++w) {
-                    mprev = fmaxf(mprev, fabsf(thr[w]));
-                    mcurr = fmaxf(mcurr, fabsf(thr[w] * sc4[w]));
-                }
-                bool go = mcurr > kNearZeroF;
-                if (go && depth >= R->rr_min_depth) {
-                    const float p = fminf(1.0f, mcurr / fmaxf(mprev, R->rr_threshold));
-                    if (rng_u(R->seed, path, base_dim + 19) >= p) {
-                        go = false;
-                    } else if (p > 0.0f && p < 1.0f) {
-                        const float inv = 1.0f / p;
-                        for (int w = 0; w < kHwss; ++w) sc4[w] *= inv;
+        // Scattering and the continuation: a shared ray goes on as lane 0, a
+        // single-wavelength lane as itself; a dielectric continues each lane.
+        if constexpr (MAT != kMatDielectric) {
+            bool cont = false;
+            f3 cdir = mk3(0, 0, 0);
+            float cpdf = 0.0f, cimp = 0.0f;
+            unsigned cst = 0u;
+            const unsigned cw = nm ? lane : 0u;
+            if (surface && !nm) {
+                // Hero ScatterNM, companions at the hero's direction (:4620-4718).
+                const ScatterOut hs = bsdf_scatter<MAT>(sc, mat, h, lam[0], 1.0f, rng_u(R.seed, path, base_dim + 16),
+                                                        rng_u(R.seed, path, base_dim + 17),
+                                                        rng_u(R.seed, path, base_dim + 18));
+                if (hs.ok) {
+                    float thr[kHwss], sc4[kHwss];
+                    for (int w = 0; w < kHwss; ++w) thr[w] = S.thr[w * cap + slot];
+                    sc4[0] = hs.kray;
+                    for (int w = 1; w < kHwss; ++w) {
+                        float cwt = bsdf_value<MAT>(sc, mat, h, hs.dir, lam[w], 1.0f) * fabsf(dot3(hs.dir, h.n));
+                        if (hs.pdf > 0.0f) cwt /= hs.pdf;
+                        sc4[w] = cwt > 0.0f ? cwt : 0.0f;
+                    }
+                    // Russian roulette over the max across wavelengths (:4721-4768).
+                    float mprev = fabsf(thr[0]), mcurr = fabsf(thr[0] * sc4[0]);
+                    for (int w = 1; w < kHwss; ++w) {
+                        mprev = fmaxf(mprev, fabsf(thr[w]));
+                        mcurr = fmaxf(mcurr, fabsf(thr[w] * sc4[w]));
+                    }
+                    bool go = mcurr > kNearZeroF;
+                    if (go && depth >= R.rr_min_depth) {
+                        const float p = fminf(1.0f, mcurr / fmaxf(mprev, R.rr_threshold));
+                        if (rng_u(R.seed, path, base_dim + 19) >= p) {
+                            go = false;
+                        } else if (p > 0.0f && p < 1.0f) {
+                            const float inv = 1.0f / p;
+                            for (int w = 0; w < kHwss; ++w) sc4[w] *= inv;
+                        }
+                    }
+                    unsigned nb = bst;
+                    if (go && !bump_bounces(nb, hs.type, R)) go = false;
+                    if (go) {
+                        // Runaway guard for the next bounce (:4096-4110).
+                        float mx = 0.0f;
+                        bool bad = false;
+                        for (int w = 0; w < kHwss; ++w) {
+                            const float t = thr[w] * sc4[w];
+                            if (!(fabsf(t) <= 3.0e38f)) bad = true;
+                            mx = fmaxf(mx, fabsf(t));
+                            S.thr[w * cap + slot] = t;
+                        }
+                        if (!bad && mx <= 1e6f) {
+                            cont = true;
+                            cdir = hs.dir;
+                            cpdf = hs.pdf;
+                            cimp = imp * fabsf(sc4[0]);
+                            cst = nb;
+                        }
                     }
                 }
-                // Bounce caps (PathTransportUtilities.h:170-210).
-                const unsigned nb = S.bounces[slot] + 1u;
-                if (go && (int)nb > R->max_diffuse) go = false;
+            } else if (surface && nm) {
+                // IntegrateFromHitNM PART 3 at the lane's wavelength (:2905-3290).
+                const unsigned ld = base_dim + kLaneDims * (lane + 1u);
+                const ScatterOut so = bsdf_scatter<MAT>(sc, mat, h, lam[lane], amb, rng_u(R.seed, path, ld + 16),
+                                                        rng_u(R.seed, path, ld + 17), rng_u(R.seed, path, ld + 18));
+                float scat = so.kray;
+                bool go = so.ok && fabsf(scat) > kNearZeroF;
+                if (go && depth >= R.rr_min_depth) {
+                    const float p = fminf(1.0f, imp * fabsf(scat) / fmaxf(imp, R.rr_threshold));
+                    if (rng_u(R.seed, path, ld + 19) >= p) go = false;
+                    else if (p > 0.0f && p < 1.0f) scat /= p;
+                }
+                unsigned nb = bst;
+                if (go && !bump_bounces(nb, so.type, R)) go = false;
                 if (go) {
-                    // Runaway guard for the next bounce (:4096-4110).
-                    float mx = 0.0f;
-                    bool bad = false;
-                    for (int w = 0; w < kHwss; ++w) {
-                        const float t = thr[w] * sc4[w];
-                        if (!(fabsf(t) <= 3.0e38f)) bad = true;
-                        mx = fmaxf(mx, fabsf(t));
-                        S.thr[w * S.cap + slot] = t;
-                    }
-                    if (!bad && mx <= 1e6f) {
-                        S.pdf[slot] = pdf;
-                        S.bounces[slot] = nb;
-                        new_o = offset_ray(h.p, dot3(nd, h.ng) > 0.0f ? h.ng : -h.ng);
+                    const float t = S.thr[lane * cap + slot] * scat;
+                    S.thr[lane * cap + slot] = t;
+                    if (fabsf(t) <= 1e6f) {
                         cont = true;
+                        cdir = so.dir;
+                        cpdf = so.pdf;
+                        cimp = imp * fabsf(scat);
+                        cst = nb;
                     }
                 }
             }
-        }
-        const unsigned qi = wave_push(cnt_next, cont);
-        if (cont) {
-            qout[qi] = slot;
-            S.ox[slot] = new_o.x;
-            S.oy[slot] = new_o.y;
-            S.oz[slot] = new_o.z;
-            S.dx[slot] = nd.x;
-            S.dy[slot] = nd.y;
-            S.dz[slot] = nd.z;
+            const unsigned qi = wave_push(cnt_next, cont);
+            if (cont) write_continuation(S, h, slot, cw, qout + qi, cdir, cpdf, cimp, cst, ist);
+        } else {
+            // A dielectric: for a lane already on its own, and for each wavelength of
+            // a shared ray that branches here (:2634-2760; no NEE or RR at this vertex).
+            for (int w = 0; w < kHwss; ++w) {
+                bool cont = false;
+                f3 nd = mk3(0, 0, 0);
+                float wgt = 0.0f;
+                int type = 0;
+                IorStack nst = ist;
+                unsigned nb = bst;
+                if (valid && (!nm || (unsigned)w == lane)) {
+                    const unsigned ld = base_dim + kLaneDims * ((unsigned)w + 1u);
+                    if (dielectric_scatter(sc, mat, h, o, lam[w], ist, rng_u(R.seed, path, ld + 18), nd, wgt, type,
+                                           nst) &&
+                        bump_bounces(nb, type, R)) {
+                        const float t = S.thr[w * cap + slot] * wgt;
+                        S.thr[w * cap + slot] = t;
+                        cont = fabsf(t) <= 1e6f;
+                    }
+                }
+                const unsigned qi = wave_push(cnt_next, cont);
+                if (cont) write_continuation(S, h, slot, (unsigned)w, qout + qi, nd, 0.0f, imp * fabsf(wgt), nb, nst);
+            }
+            if (branch) S.flags[slot] |= kModeNm;
         }
     }
 }
@@ -452,32 +819,93 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
 // Shadow rays (LightSampler::ShadowOccludedNM -> ObjectManager::IntersectShadowRay);
 // adds each unoccluded contribution to its path and wavelength.  There is one
 // record per (path, wavelength) and bounce, so the additions never collide.
-template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_connect(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+// The same two traversal schedules as k_extend.
+template <int STACK, int BLOCK, bool PERSIST>
+__global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
-    __shared__ uint2 lds[STACK > 0 ? STACK * kTraceBlock : 1];
+    __shared__ uint2 lds[STACK > 0 ? STACK * BLOCK : 1];
+    LdsStack<STACK> lst{lds + threadIdx.x, BLOCK, 0};
+    PrivStack pst;
+    pst.sp = 0;
     const unsigned n = S.cnt_sh[depth];
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const unsigned code = S.sslot[i];
-        const unsigned slot = code >> 2, w = code & 3u;
-        const unsigned rq = S.req[slot];
-        const NodeSlot* nodes = reqs[rq].sc.nodes;
-        const Tri48* tris = reqs[rq].sc.tris;
-        RayState r;
-        init_ray(r, S.sox[i], S.soy[i], S.soz[i], S.sdx[i], S.sdy[i], S.sdz[i]);
-        const double how_far2 = scale_how_far(r, (double)S.stmax[i]);
-        unsigned nn = 0, nt = 0;
-        bool occ;
-        if (STACK > 0) {
-            LdsStack<STACK> st{lds + threadIdx.x, kTraceBlock, 0};
-            occ = trace_any<false>(r, nodes, tris, st, how_far2, nn, nt);
-        } else {
-            PrivStack st;
-            st.sp = 0;
-            occ = trace_any<false>(r, nodes, tris, st, how_far2, nn, nt);
+    if constexpr (!PERSIST) {
+        for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            const unsigned code = S.sslot[i];
+            const ReqInfo& R = reqs[S.req[code >> 2]];
+            RayState r;
+            init_ray(r, S.sox[i], S.soy[i], S.soz[i], S.sdx[i], S.sdy[i], S.sdz[i]);
+            unsigned nn = 0, nt = 0;
+            lst.sp = 0;
+            pst.sp = 0;
+            const double how_far2 = scale_how_far(r, (double)S.stmax[i]);
+            const bool occ = STACK > 0 ? trace_any<false>(r, R.sc.nodes, R.sc.tris, lst, how_far2, nn, nt)
+                                       : trace_any<false>(r, R.sc.nodes, R.sc.tris, pst, how_far2, nn, nt);
+            if (!occ) S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[i];
         }
-        if (!occ) S.res[w * S.cap + slot] += S.sval[i];
+        return;
     }
+    unsigned long long* next = S.fetch + (kMaxPathDepth + 1) + depth;
+    RayState r;
+    AnyState s;
+    const NodeSlot* nodes = nullptr;
+    const Tri48* tris = nullptr;
+    unsigned nn = 0, nt = 0;
+    long long item = -1;
+    for (;;) {
+        const long long got = refill(item == -1, item >= 0, next, n);
+        if (item == -1 && got != -1) {
+            item = got;
+            if (item >= 0) {
+                const unsigned slot = S.sslot[item] >> 2;
+                const ReqInfo& R = reqs[S.req[slot]];
+                nodes = R.sc.nodes;
+                tris = R.sc.tris;
+                init_ray(r, S.sox[item], S.soy[item], S.soz[item], S.sdx[item], S.sdy[item], S.sdz[item]);
+                any_begin(r, scale_how_far(r, (double)S.stmax[item]), s);
+                lst.sp = 0;
+                pst.sp = 0;
+            }
+        }
+        if (__ballot(item >= 0) == 0ull && __ballot(item == -1) == 0ull) break;
+        if (item >= 0) {
+            const bool more = STACK > 0 ? any_step<false>(r, nodes, tris, lst, s, nn, nt)
+                                        : any_step<false>(r, nodes, tris, pst, s, nn, nt);
+            if (!more) {
+                if (!s.hit) {
+                    const unsigned code = S.sslot[item];
+                    S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[item];
+                }
+                item = -1;
+            }
+        }
+    }
+}
+
+// Kernel tables over the traversal variants: stack (LDS 8, LDS 16, private),
+// workgroup size (64, 128, 256), schedule (persistent or one ray per thread).
+using TraceKernel = void (*)(const ReqInfo*, Paths, int);
+
+template <int S, int B, bool P>
+struct ExtendOf {
+    static constexpr TraceKernel fn = k_extend<S, B, P>;
+};
+template <int S, int B, bool P>
+struct ConnectOf {
+    static constexpr TraceKernel fn = k_connect<S, B, P>;
+};
+
+template <template <int, int, bool> class K, bool P>
+TraceKernel pick_block(int stack, int block)
+{
+    if (stack == 8) return block == 64 ? K<8, 64, P>::fn : block == 256 ? K<8, 256, P>::fn : K<8, 128, P>::fn;
+    if (stack == 16) return block == 64 ? K<16, 64, P>::fn : block == 256 ? K<16, 256, P>::fn : K<16, 128, P>::fn;
+    return block == 64 ? K<0, 64, P>::fn : block == 256 ? K<0, 256, P>::fn : K<0, 128, P>::fn;
+}
+
+template <template <int, int, bool> class K>
+TraceKernel pick_trace(int stack, int block, bool persist)
+{
+    return persist ? pick_block<K, true>(stack, block) : pick_block<K, false>(stack, block);
 }
 
 // Per-pixel FP32 sums with a Kahan compensation term, samples in index order.
@@ -559,23 +987,27 @@ size_t carve(char* base, int64_t cap, int passes, Paths* P, unsigned** counters)
     const size_t c = (size_t)cap, c4 = 4 * c;
     Paths S{};
     S.cap = (int)cap;
-    S.ox = (float*)take(c * 4);
-    S.oy = (float*)take(c * 4);
-    S.oz = (float*)take(c * 4);
-    S.dx = (float*)take(c * 4);
-    S.dy = (float*)take(c * 4);
-    S.dz = (float*)take(c * 4);
+    S.ox = (float*)take(c4 * 4);
+    S.oy = (float*)take(c4 * 4);
+    S.oz = (float*)take(c4 * 4);
+    S.dx = (float*)take(c4 * 4);
+    S.dy = (float*)take(c4 * 4);
+    S.dz = (float*)take(c4 * 4);
     S.thr = (float*)take(c4 * 4);
     S.res = (float*)take(c4 * 4);
     S.lambda0 = (float*)take(c * 4);
-    S.pdf = (float*)take(c * 4);
+    S.pdf = (float*)take(c4 * 4);
+    S.imp = (float*)take(c4 * 4);
+    S.state = (unsigned*)take(c4 * 4);
+    S.ior_obj = (int*)take(3 * c4 * 4);
+    S.flags = (unsigned*)take(c * 4);
     S.path_id = (unsigned*)take(c * 4);
     S.req = (unsigned*)take(c * 4);
-    S.bounces = (unsigned*)take(c * 4);
-    S.hit_leaf = (unsigned*)take(c * 4);
-    S.hit_t = (float*)take(c * 4);
-    S.queue[0] = (unsigned*)take(c * 4);
-    S.queue[1] = (unsigned*)take(c * 4);
+    S.hit_leaf = (unsigned*)take(c4 * 4);
+    S.hit_t = (float*)take(c4 * 4);
+    S.queue[0] = (unsigned*)take(c4 * 4);
+    S.queue[1] = (unsigned*)take(c4 * 4);
+    for (int m = 0; m < 3; ++m) S.mqueue[m] = (unsigned*)take(c4 * 4);
     S.sox = (float*)take(c4 * 4);
     S.soy = (float*)take(c4 * 4);
     S.soz = (float*)take(c4 * 4);
@@ -586,9 +1018,23 @@ size_t carve(char* base, int64_t cap, int passes, Paths* P, unsigned** counters)
     S.sval = (float*)take(c4 * 4);
     S.sslot = (unsigned*)take(c4 * 4);
     unsigned* cnt = (unsigned*)take(sizeof(unsigned) * (size_t)kCounters * (size_t)std::max(1, passes));
+    S.fetch = (unsigned long long*)take(sizeof(unsigned long long) * 2 * (kMaxPathDepth + 1));
     if (P) *P = S;
     if (counters) *counters = cnt;
     return off;
+}
+
+// Workgroups of `kernel` resident at once on the current device.
+template <class K>
+unsigned resident_grid(K kernel, int block)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    return (unsigned)(cus * per_cu);
 }
 
 unsigned grid_for(int64_t n, int block, unsigned max_blocks)
@@ -690,6 +1136,11 @@ struct rise_hip_batch {
     int n_req = 0;
     int bounces = 1;                   // bounce iterations per pass
     int stack = 0;                     // LDS stack entries per lane (0: private array)
+    int trace_block = 128;             // workgroup size of the traversal kernels
+    bool persist = true;               // persistent traversal waves (else one ray per thread)
+    TraceKernel ext_fn = nullptr, con_fn = nullptr;
+    unsigned grid_ext = 1, grid_con = 1;  // workgroups of the traversal kernels
+    unsigned mat_mask = 0;                // material kinds present in the batch's scenes
     int64_t cap = 0;                   // path pool size
     std::vector<rise_hip_render_request> requests;
     std::vector<PassEntry> entries;    // all passes, concatenated
@@ -742,8 +1193,15 @@ bool same_requests(const rise_hip_batch* b, const rise_hip_render_request* reqs,
 
 // Plans the passes and uploads the tables, reusing b's device buffers when
 // they are large enough.  The caller has made sure no launch of b is in flight.
-rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, size_t n)
+rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, size_t n,
+                     const rise_hip_batch_tuning* tuning)
 {
+    rise_hip_batch_tuning tn{};
+    if (tuning) tn = *tuning;
+    if (tn.max_paths < 0 || (tn.trace_block != 0 && tn.trace_block != 64 && tn.trace_block != 128 &&
+                             tn.trace_block != 256) ||
+        (tn.stack != 0 && tn.stack != 8 && tn.stack != 16 && tn.stack != -1) || tn.persistent < 0 || tn.persistent > 2)
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad tuning values");
     if (n == 0 || !reqs) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "empty batch");
     if (n > (size_t)0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "too many requests");
     b->requests.clear();
@@ -751,6 +1209,8 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     std::vector<std::vector<PassEntry>> chunks(n);
     int64_t total_paths = 0, max_chunk = 0;
     int bounces = 1, depth = 1;
+    int64_t max_tris = 0;
+    unsigned mat_mask = 0;
     std::vector<int64_t> req_px(n, 0);
     size_t film_floats = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -769,9 +1229,12 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
                          (sc->has_transmission ? D.max_transmission_bounce : 0);
         bounces = std::max(bounces, std::min(kMaxPathDepth, hits));
         depth = std::max(depth, reqs[k].scene->view.depth);
+        mat_mask |= reqs[k].scene->material_kinds;
+        max_tris = std::max<int64_t>(max_tris, reqs[k].scene->n_tris);
         film_floats += 12 * (size_t)req_px[k];
     }
-    const int64_t cap = std::max<int64_t>(max_chunk, std::min<int64_t>(kMaxPaths, total_paths));
+    const int64_t want = tn.max_paths > 0 ? std::min<int64_t>(tn.max_paths, kMaxPaths) : kMaxPaths;
+    const int64_t cap = std::max<int64_t>(max_chunk, std::min<int64_t>(want, total_paths));
 
     // Passes: each takes the next chunk of every request, in request order, while they fit.
     std::vector<PassEntry> entries;
@@ -841,9 +1304,28 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     b->pass_paths.swap(pass_paths);
     b->pass_film.swap(pass_film);
     b->req_px.swap(req_px);
+    // Traversal: the node-group stack holds at most one entry per level below
+    // the root; the smallest LDS stack that fits keeps occupancy highest.
     const int need_stack = depth - 1;
-    b->stack = need_stack <= 8 ? 8 : (need_stack <= 16 ? 16 : 0);
+    const int auto_stack = need_stack <= 8 ? 8 : (need_stack <= 16 ? 16 : 0);
+    if (tn.stack == 8 && need_stack <= 8) b->stack = 8;
+    else if (tn.stack == 16 && need_stack <= 16) b->stack = 16;
+    else if (tn.stack == -1) b->stack = 0;
+    else b->stack = auto_stack;
+    b->trace_block = tn.trace_block ? tn.trace_block : 128;
+    b->persist = tn.persistent == 1 || (tn.persistent == 0 && max_tris >= 4096);
+    b->ext_fn = pick_trace<ExtendOf>(b->stack, b->trace_block, b->persist);
+    b->con_fn = pick_trace<ConnectOf>(b->stack, b->trace_block, b->persist);
+    if (b->persist) {
+        // Never more workgroups than queue entries of a full pass.
+        b->grid_ext = std::min(resident_grid(b->ext_fn, b->trace_block), grid_for(4 * cap, b->trace_block, 1u << 30));
+        b->grid_con = std::min(resident_grid(b->con_fn, b->trace_block), grid_for(4 * cap, b->trace_block, 1u << 30));
+    } else {
+        b->grid_ext = grid_for(4 * cap, b->trace_block, 8192);
+        b->grid_con = grid_for(4 * cap, b->trace_block, 8192);
+    }
     b->bounces = bounces;
+    b->mat_mask = mat_mask;
     b->n_req = (int)n;
     b->requests.assign(reqs, reqs + n);
     b->launched = false;
@@ -853,8 +1335,8 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
 // Kernel launches of a planned batch (no host synchronization, no copies).
 hipError_t launch(rise_hip_batch* b, hipStream_t stream)
 {
-    const unsigned gt = grid_for(b->cap, kTraceBlock, 4096);
-    const unsigned gs = grid_for(b->cap, kShadeBlock, 2048);
+    const unsigned ge = b->grid_ext, gc = b->grid_con;
+    const unsigned gs = grid_for(4 * b->cap, kShadeBlock, 2048);
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)b->film_floats, 256, 4096)), dim3(256), 0, stream, b->d_film,
                        b->film_floats);
     const int passes = (int)b->pass_paths.size();
@@ -862,23 +1344,25 @@ hipError_t launch(rise_hip_batch* b, hipStream_t stream)
         Paths S = b->S;
         S.cnt_ext = b->counters + (size_t)p * kCounters;
         S.cnt_sh = S.cnt_ext + (kMaxPathDepth + 1);
+        S.cnt_mat = S.cnt_ext + 2 * (kMaxPathDepth + 1);
         const PassEntry* ent = b->d_entries + b->pass_first[p];
         const int n_ent = b->pass_first[p + 1] - b->pass_first[p];
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, S.cnt_ext, kCounters);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<unsigned*>(S.fetch),
+                           4 * (kMaxPathDepth + 1));
         hipLaunchKernelGGL(k_generate, dim3(grid_for(b->pass_paths[p], 256, 4096)), dim3(256), 0, stream, b->d_info,
                            ent, n_ent, b->pass_paths[p], S);
         for (int d = 0; d < b->bounces; ++d) {
-            switch (b->stack) {
-            case 8: hipLaunchKernelGGL(k_extend<8>, dim3(gt), dim3(kTraceBlock), 0, stream, b->d_info, S, d); break;
-            case 16: hipLaunchKernelGGL(k_extend<16>, dim3(gt), dim3(kTraceBlock), 0, stream, b->d_info, S, d); break;
-            default: hipLaunchKernelGGL(k_extend<0>, dim3(gt), dim3(kTraceBlock), 0, stream, b->d_info, S, d); break;
-            }
-            hipLaunchKernelGGL(k_shade, dim3(gs), dim3(kShadeBlock), 0, stream, b->d_info, S, d);
-            switch (b->stack) {
-            case 8: hipLaunchKernelGGL(k_connect<8>, dim3(gt), dim3(kTraceBlock), 0, stream, b->d_info, S, d); break;
-            case 16: hipLaunchKernelGGL(k_connect<16>, dim3(gt), dim3(kTraceBlock), 0, stream, b->d_info, S, d); break;
-            default: hipLaunchKernelGGL(k_connect<0>, dim3(gt), dim3(kTraceBlock), 0, stream, b->d_info, S, d); break;
-            }
+            void* args[] = {&b->d_info, &S, &d};
+            hipLaunchKernel((const void*)b->ext_fn, dim3(ge), dim3(b->trace_block), args, 0, stream);
+            hipLaunchKernelGGL(k_bin, dim3(grid_for(4 * b->cap, 256, 2048)), dim3(256), 0, stream, b->d_info, S, d);
+            if (b->mat_mask & (1u << kMatLambert))
+                hipLaunchKernelGGL(k_shade<kMatLambert>, dim3(gs), dim3(kShadeBlock), 0, stream, b->d_info, S, d);
+            if (b->mat_mask & (1u << kMatGgx))
+                hipLaunchKernelGGL(k_shade<kMatGgx>, dim3(gs), dim3(kShadeBlock), 0, stream, b->d_info, S, d);
+            if (b->mat_mask & (1u << kMatDielectric))
+                hipLaunchKernelGGL(k_shade<kMatDielectric>, dim3(gs), dim3(kShadeBlock), 0, stream, b->d_info, S, d);
+            hipLaunchKernel((const void*)b->con_fn, dim3(gc), dim3(b->trace_block), args, 0, stream);
         }
         hipLaunchKernelGGL(k_film, dim3(grid_for(b->pass_film[p], 256, 4096)), dim3(256), 0, stream, b->d_info, ent,
                            n_ent, b->pass_film[p], S);
@@ -922,20 +1406,37 @@ void release_solo_batch(rise_hip_pt_scene* s)
     }
 }
 
-extern "C" rise_hip_status rise_hip_batch_create(const rise_hip_render_request* requests, size_t n_requests,
-                                                 rise_hip_batch** out_batch)
+extern "C" rise_hip_status rise_hip_batch_create_tuned(const rise_hip_render_request* requests, size_t n_requests,
+                                                       const rise_hip_batch_tuning* tuning, rise_hip_batch** out_batch)
 {
     clear_error();
     if (!out_batch) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "out_batch is NULL");
     *out_batch = nullptr;
     rise_hip_batch* b = new (std::nothrow) rise_hip_batch;
     if (!b) return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
-    const rise_hip_status st = plan(b, requests, n_requests);
+    const rise_hip_status st = plan(b, requests, n_requests, tuning);
     if (st != RISE_HIP_OK) {
         free_batch(b);
         return st;
     }
     *out_batch = b;
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_batch_create(const rise_hip_render_request* requests, size_t n_requests,
+                                                 rise_hip_batch** out_batch)
+{
+    return rise_hip_batch_create_tuned(requests, n_requests, nullptr, out_batch);
+}
+
+extern "C" rise_hip_status rise_hip_batch_get_tuning(const rise_hip_batch* batch, rise_hip_batch_tuning* out)
+{
+    clear_error();
+    if (!batch || !out) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    out->max_paths = (int32_t)batch->cap;
+    out->trace_block = batch->trace_block;
+    out->stack = batch->stack > 0 ? batch->stack : -1;
+    out->persistent = batch->persist ? 1 : 2;
     return RISE_HIP_OK;
 }
 
@@ -985,7 +1486,7 @@ extern "C" rise_hip_status rise_hip_render(rise_hip_pt_scene* scene, const rise_
             const hipError_t e = hipStreamSynchronize(b->last_stream);
             if (e != hipSuccess) return fail_hip(e, "previous render");
         }
-        const rise_hip_status st = plan(b, &q, 1);
+        const rise_hip_status st = plan(b, &q, 1, nullptr);
         if (st != RISE_HIP_OK) {
             b->requests.clear();
             return st;

@@ -75,14 +75,28 @@ extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* 
     }
     // Materials.
     std::vector<DevMaterial> dmat(n_materials);
+    bool has_glossy = false, has_transmission = false;
+    unsigned material_kinds = 0;
     for (size_t i = 0; i < n_materials; ++i) {
         const rise_hip_material_desc& m = materials[i];
-        if (m.type != RISE_HIP_MATERIAL_LAMBERTIAN)
-            return fail(RISE_HIP_ERR_NOT_SUPPORTED, "material " + std::to_string(i) + ": unsupported type");
-        if (m.reflectance_spd < 0 || (size_t)m.reflectance_spd >= n_spds || m.emission_spd >= (int)n_spds ||
-            m.emission_spd < -1)
-            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "material " + std::to_string(i) + ": spectrum index out of range");
-        dmat[i] = DevMaterial{m.type, m.reflectance_spd, m.emission_spd, m.emission_scale};
+        const std::string who = "material " + std::to_string(i) + ": ";
+        if (m.type != RISE_HIP_MATERIAL_LAMBERTIAN && m.type != RISE_HIP_MATERIAL_GGX &&
+            m.type != RISE_HIP_MATERIAL_DIELECTRIC)
+            return fail(RISE_HIP_ERR_NOT_SUPPORTED, who + "unsupported type");
+        auto bad = [&](int32_t k, bool required) { return k < -1 || k >= (int)n_spds || (required && k < 0); };
+        if (bad(m.reflectance_spd, m.type == RISE_HIP_MATERIAL_LAMBERTIAN) || bad(m.emission_spd, false) ||
+            bad(m.specular_spd, m.type == RISE_HIP_MATERIAL_GGX) || bad(m.ior_spd, false) ||
+            bad(m.extinction_spd, false))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "spectrum index out of range");
+        if (m.type != RISE_HIP_MATERIAL_LAMBERTIAN && m.emission_spd >= 0)
+            return fail(RISE_HIP_ERR_NOT_SUPPORTED, who + "only Lambertian materials emit");
+        if (m.type == RISE_HIP_MATERIAL_GGX && !(m.alpha >= 0.0f))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "negative or NaN roughness");
+        dmat[i] = DevMaterial{m.type,        m.reflectance_spd, m.emission_spd,    m.emission_scale,
+                              m.specular_spd, m.ior_spd,         m.extinction_spd, m.alpha};
+        material_kinds |= 1u << m.type;
+        if (m.type == RISE_HIP_MATERIAL_GGX) has_glossy = true;
+        if (m.type == RISE_HIP_MATERIAL_DIELECTRIC) has_glossy = has_transmission = true;
     }
     // Objects -> one triangle list.
     int64_t total = 0;
@@ -207,6 +221,9 @@ extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* 
     v.depth = bvh.stats.depth;
     v.n_lums = (int)lums.size();
     s->n_tris = total;
+    s->has_glossy = has_glossy;
+    s->has_transmission = has_transmission;
+    s->material_kinds = material_kinds;
     *out_scene = s;
     return RISE_HIP_OK;
 }

@@ -38,12 +38,28 @@ class Spd:
         return Spd(begin, end, np.array([b, g, r], np.float32))
 
 
+LAMBERTIAN, GGX, DIELECTRIC = 0, 1, 2
+
+
 @dataclass
 class Material:
-    reflectance: int            # SPD index
-    emission: int = -1          # SPD index or -1
+    """A material; SPD indices refer to Scene.spds (-1: none).
+
+    * LAMBERTIAN (``lambertian_material``): reflectance, optional emission
+      (``lambertian_luminaire_material``: L = spd / pi * emission_scale).
+    * GGX (``ggx_material``, conductor Fresnel mode, isotropic): reflectance =
+      diffuse ``rd`` (-1: none), specular = ``rs``, ior / extinction = the
+      conductor's n and k, alpha = roughness.
+    * DIELECTRIC (``dielectric_material``, smooth): ior, extinction = ``tau``
+      (transmittance per unit length, -1: none)."""
+    reflectance: int = -1
+    emission: int = -1
     emission_scale: float = 1.0
-    kind: int = 0               # 0 = Lambertian
+    kind: int = LAMBERTIAN
+    specular: int = -1
+    ior: int = -1
+    extinction: int = -1
+    alpha: float = 0.15
 
 
 @dataclass
@@ -221,3 +237,84 @@ def batch_workload(seed: int = 4, n_instances: int = 64, n_scenes: int = 8):
                            max_transmission_bounce=6, seed=seed * 100000 + i)
         reqs.append((k, r))
     return scenes, reqs
+
+
+def icosphere(center=(0.0, 0.0, 0.0), radius: float = 1.0, subdiv: int = 2) -> np.ndarray:
+    """Icosahedron subdivided `subdiv` times and projected to the sphere:
+    20 * 4^subdiv triangles (n, 9) float32, outward face normals."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius + np.asarray(center, np.float64)
+    tris = np.array([np.concatenate([V[a], V[b], V[c]]) for a, b, c in faces], np.float32)
+    # orient outward
+    cen = tris.reshape(-1, 3, 3).mean(1) - np.asarray(center, np.float32)
+    n = np.cross(tris[:, 3:6] - tris[:, 0:3], tris[:, 6:9] - tris[:, 0:3])
+    flip = np.sum(n * cen, 1) < 0
+    tris[flip] = tris[flip].reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9)
+    return tris
+
+
+def furnace_with(kind: int, radiance: float = 1.0, alpha: float = 0.3, ior: float = 1.5, max_bounce: int = 64):
+    """A closed emitting box (radiance `radiance`, albedo 0, every face its own
+    luminary) around a sphere of radius 0.5 at the origin: a smooth dielectric
+    (index `ior`, no absorption) or a GGX mirror-like conductor (rd = 0,
+    rs = 1, n = 1e-3, k = 1e6, so F ~ 1, roughness `alpha`).  Seen through the
+    camera inside the box the sphere has radiance = `radiance` x its albedo,
+    which is 1 for the dielectric and ~1 for the compensated GGX lobe."""
+    s = Scene()
+    s.spds = [Spd.flat(0.0), Spd.flat(radiance), Spd.flat(1.0), Spd.flat(ior), Spd.flat(1e-3), Spd.flat(1e6)]
+    s.materials = [Material(0, emission=1, emission_scale=math.pi)]
+    tris = box((-2, -2, -2), (2, 2, 2))
+    tris = np.ascontiguousarray(tris.reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9))
+    s.objects = [Object(tris[2 * f: 2 * f + 2].copy(), 0) for f in range(6)]
+    if kind == DIELECTRIC:
+        s.materials.append(Material(kind=DIELECTRIC, ior=3))
+    else:
+        s.materials.append(Material(kind=GGX, reflectance=-1, specular=2, ior=4, extinction=5, alpha=alpha))
+    s.objects.append(Object(icosphere((0, 0, 0), 0.5, 3), 1))
+    s.camera = Camera(position=(0.0, 0.0, 1.8), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=20.0)
+    r = RenderSettings(width=12, height=12, spp=64, max_diffuse_bounce=max_bounce, max_glossy_bounce=max_bounce,
+                       max_transmission_bounce=max_bounce)
+    return s, r
+
+
+def cornell_materials(glass_subdiv: int = 2):
+    """The Cornell box with the tall block made a GGX conductor (gold-like
+    n, k over three bands, roughness 0.25, white diffuse 0 / specular 1) and a
+    smooth glass sphere (index 1.5, radius 0.3, 20 * 4^glass_subdiv
+    triangles) on the floor in front of the short block."""
+    s, r = cornell_box()
+    s.spds += [
+        Spd.flat(1.0),                                   # 4 specular tint
+        Spd(380.0, 720.0, np.array([1.3, 0.6, 0.2], np.float32)),   # 5 conductor n (blue, green, red bands)
+        Spd(380.0, 720.0, np.array([1.9, 2.4, 3.2], np.float32)),   # 6 conductor k
+        Spd.flat(1.5),                                   # 7 glass index
+    ]
+    s.materials += [
+        Material(kind=GGX, reflectance=-1, specular=4, ior=5, extinction=6, alpha=0.25),   # 4 gold-like
+        Material(kind=DIELECTRIC, ior=7),                                                  # 5 glass
+    ]
+    s.objects[4] = Object(s.objects[4].tris, 4)
+    s.objects.insert(5, Object(icosphere((0.35, 0.3, 0.85), 0.3, glass_subdiv), 5))
+    return s, r

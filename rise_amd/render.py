@@ -32,7 +32,10 @@ class MaterialDesc(ctypes.Structure):
         ("reflectance_spd", ctypes.c_int32),
         ("emission_spd", ctypes.c_int32),
         ("emission_scale", ctypes.c_float),
-        ("reserved", ctypes.c_float * 4),
+        ("specular_spd", ctypes.c_int32),
+        ("ior_spd", ctypes.c_int32),
+        ("extinction_spd", ctypes.c_int32),
+        ("alpha", ctypes.c_float),
     ]
 
 
@@ -94,6 +97,18 @@ class RenderRequest(ctypes.Structure):
     ]
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("max_paths", ctypes.c_int32),
+        ("trace_block", ctypes.c_int32),
+        ("stack", ctypes.c_int32),
+        ("persistent", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _SIG = {
     "rise_hip_pt_scene_create": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.POINTER(_P)],
@@ -101,6 +116,9 @@ _SIG = {
     "rise_hip_pt_scene_release": ([_P], ctypes.c_int),
     "rise_hip_render": ([_P, ctypes.POINTER(RenderDesc), _P, _P, ctypes.POINTER(RenderStats), _P], ctypes.c_int),
     "rise_hip_batch_create": ([_P, ctypes.c_size_t, ctypes.POINTER(_P)], ctypes.c_int),
+    "rise_hip_batch_create_tuned": ([_P, ctypes.c_size_t, ctypes.POINTER(BatchTuning), ctypes.POINTER(_P)],
+                                    ctypes.c_int),
+    "rise_hip_batch_get_tuning": ([_P, ctypes.POINTER(BatchTuning)], ctypes.c_int),
     "rise_hip_batch_launch": ([_P, _P], ctypes.c_int),
     "rise_hip_batch_get_stats": ([_P, _P, ctypes.POINTER(RenderStats)], ctypes.c_int),
     "rise_hip_batch_release": ([_P], ctypes.c_int),
@@ -161,6 +179,10 @@ class PathTracingSpectralRasterizer:
             mats[k].reflectance_spd = m.reflectance
             mats[k].emission_spd = m.emission
             mats[k].emission_scale = m.emission_scale
+            mats[k].specular_spd = m.specular
+            mats[k].ior_spd = m.ior
+            mats[k].extinction_spd = m.extinction
+            mats[k].alpha = m.alpha
         objs = (ObjectDesc * max(1, len(scene.objects)))()
         for k, o in enumerate(scene.objects):
             t = np.ascontiguousarray(o.tris, np.float32).reshape(-1, 9)
@@ -222,9 +244,10 @@ class RenderBatch:
     here (``films[k]``: mean XYZ (H, W, 3); ``films_sq[k]`` when want_sq) and
     rewritten by every ``launch``; each is identical to what
     ``rasterize_scene`` gives for that item alone.  ``launch`` only enqueues
-    kernels, so it may be captured in a CUDA/HIP graph."""
+    kernels, so it may be captured in a CUDA/HIP graph.  ``tuning`` sets the
+    launch shapes (rise_hip_batch_tuning fields; see rise_amd.autotune)."""
 
-    def __init__(self, items, want_sq: bool = False):
+    def __init__(self, items, want_sq: bool = False, tuning: dict | None = None):
         import torch
 
         L = _lib_render()
@@ -243,8 +266,15 @@ class RenderBatch:
             reqs[k].film_xyz = film.data_ptr()
             reqs[k].film_xyz_sq = sq.data_ptr() if want_sq else None
         out = _P(None)
-        _lib.check(L.rise_hip_batch_create(reqs, len(self.items), ctypes.byref(out)))
+        tn = BatchTuning(**(tuning or {}))
+        _lib.check(L.rise_hip_batch_create_tuned(reqs, len(self.items), ctypes.byref(tn), ctypes.byref(out)))
         self._h = out
+
+    def tuning(self) -> dict:
+        """The launch shapes this batch resolved to."""
+        t = BatchTuning()
+        _lib.check(_lib_render().rise_hip_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
 
     def launch(self, stream=None) -> None:
         _lib.check(_lib_render().rise_hip_batch_launch(self._h, _stream_handle(stream)))

@@ -132,3 +132,28 @@ def test_batch_matches_oracle(cuda):
     batch.release()
     a.release()
     b_.release()
+
+
+def test_batch_tuning_does_not_change_films(cuda):
+    import torch
+
+    log_case("batch under three launch tunings")
+    scenes, reqs = _mixed_requests()
+    ref = RenderBatch(reqs)
+    ref.launch()
+    torch.cuda.synchronize()
+    films = [f.cpu().numpy() for f in ref.films]
+    ref.release()
+    for tn in ({"persistent": 2, "trace_block": 64, "max_paths": 1 << 20},
+               {"persistent": 1, "trace_block": 256, "stack": -1},
+               {"persistent": 1, "trace_block": 128, "stack": 16, "max_paths": 3 << 20}):
+        b = RenderBatch(reqs, tuning=tn)
+        got = b.tuning()
+        assert got["trace_block"] == tn["trace_block"] and got["persistent"] == tn["persistent"]
+        b.launch()
+        torch.cuda.synchronize()
+        for k, f in enumerate(films):
+            assert np.array_equal(b.films[k].cpu().numpy(), f), (tn, k)
+        b.release()
+    for s in scenes:
+        s.release()

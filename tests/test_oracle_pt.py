@@ -91,3 +91,22 @@ def test_two_seeds_agree_statistically():
     frac = np.mean(np.abs(z[..., 1]) < 3.0)
     assert frac >= 0.97, frac
     assert abs(ma[..., 1].mean() / mb[..., 1].mean() - 1) < 0.02
+
+
+def test_oracle_dielectric_furnace():
+    # A lossless glass sphere inside a uniformly emitting box (radiance 1, albedo 0): every
+    # path ends on the box with weight R + T = 1 through the single-wavelength branch.
+    s, r = pt_scene.furnace_with(pt_scene.DIELECTRIC)
+    sm, _, cnt = oracle.render(s, r)
+    y = sm[..., 1] / r.spp
+    assert abs(y.mean() - 1.0) < 0.005, y.mean()
+    assert cnt[0] > cnt[2]  # refracted and reflected continuations were traced
+
+
+def test_oracle_ggx_furnace():
+    # A GGX sphere with F ~ 1 in the same box: radiance = the lobe's albedo, ~1 with the
+    # multiscatter compensation.
+    s, r = pt_scene.furnace_with(pt_scene.GGX, alpha=0.5)
+    sm, _, _ = oracle.render(s, r)
+    y = sm[..., 1] / r.spp
+    assert abs(y.mean() - 1.0) < 0.03, y.mean()
