@@ -74,6 +74,20 @@ rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* objects, si
 
 rise_hip_status rise_hip_pt_scene_release(rise_hip_pt_scene* scene);
 
+/* The spectral sky radiance map of the scene-file chunk `hosek_wilkie_skylight`
+ * (Job::AddHosekWilkieSkylight, Job.cpp:2717-2750; the reference's model is
+ * Preetham 1999, Utilities/HosekWilkieSkyModel.cpp) without its matched
+ * directional sun light.  Rays leaving the scene see it; renders and batches
+ * created afterwards use it (NULL removes it). */
+typedef struct rise_hip_sky_desc {
+    float solar_elevation_deg;  /* solar_elevation (45) */
+    float solar_azimuth_deg;    /* solar_azimuth (0) */
+    float turbidity;            /* turbidity (3) */
+    float sky_scale;            /* sky_intensity_scale (1) */
+} rise_hip_sky_desc;
+
+rise_hip_status rise_hip_pt_scene_set_sky(rise_hip_pt_scene* scene, const rise_hip_sky_desc* sky);
+
 /* Pinhole camera: vertical field of view, square pixels, pixel row 0 at the top. */
 typedef struct rise_hip_camera_desc {
     float position[3];
@@ -97,6 +111,8 @@ typedef struct rise_hip_render_desc {
     float rr_threshold;
     uint64_t seed;
     rise_hip_camera_desc camera;
+    int32_t sky_background;          /* camera rays that miss see the sky (radiance_background) */
+    int32_t reserved;
 } rise_hip_render_desc;
 
 typedef struct rise_hip_render_stats {

@@ -169,6 +169,9 @@ class _RoScene(ctypes.Structure):
         ("max_diffuse_bounce", ctypes.c_int32), ("max_glossy_bounce", ctypes.c_int32),
         ("max_transmission_bounce", ctypes.c_int32), ("rr_min_depth", ctypes.c_int32),
         ("rr_threshold", ctypes.c_double), ("seed", ctypes.c_uint64),
+        ("sky_enabled", ctypes.c_int32), ("sky_background", ctypes.c_int32),
+        ("sky_elevation_deg", ctypes.c_double), ("sky_azimuth_deg", ctypes.c_double),
+        ("sky_turbidity", ctypes.c_double), ("sky_scale", ctypes.c_double),
     ]
 
 
@@ -221,6 +224,13 @@ def render(scene, settings, s0: int = 0, s1: int | None = None, spp_total: int |
     rs.rr_min_depth = settings.rr_min_depth
     rs.rr_threshold = float(np.float32(settings.rr_threshold))
     rs.seed = settings.seed
+    if scene.sky is not None:
+        rs.sky_enabled = 1
+        rs.sky_elevation_deg = float(np.float32(scene.sky.elevation_deg))
+        rs.sky_azimuth_deg = float(np.float32(scene.sky.azimuth_deg))
+        rs.sky_turbidity = float(np.float32(scene.sky.turbidity))
+        rs.sky_scale = float(np.float32(scene.sky.scale))
+    rs.sky_background = 1 if settings.sky_background else 0
     H, W = settings.height, settings.width
     sm = np.zeros((H, W, 3), np.float64)
     sq = np.zeros((H, W, 3), np.float64)

@@ -148,6 +148,102 @@ static double power_heuristic(double pa, double pb)
     return a2 / (a2 + pb * pb);
 }
 
+
+/* ---- sky (HosekWilkieSkyModel.cpp: Preetham 1999) -------------------------- */
+
+typedef struct { double A, B, C, D, E; } perez;
+
+typedef struct {
+    int on;
+    double theta_s, cos_theta_s, sun[3], yz, xz, yzc, scale;
+    perez pY, px, py;
+} sky_model;
+
+static perez perez_of(double T, const double t[5], const double c[5])
+{
+    perez p = {t[0] * T + c[0], t[1] * T + c[1], t[2] * T + c[2], t[3] * T + c[3], t[4] * T + c[4]};
+    return p;
+}
+
+static void sky_init(sky_model* m, const ro_scene* s)
+{
+    memset(m, 0, sizeof(*m));
+    if (!s->sky_enabled) return;
+    m->on = 1;
+    const double el = s->sky_elevation_deg * (M_PI / 180.0), az = s->sky_azimuth_deg * (M_PI / 180.0);
+    const double T = s->sky_turbidity;
+    /* Preetham 1999 Table 2 (as HosekWilkieSkyModel.cpp:31-53 states them). */
+    static const double Yt[5] = {0.1787, -0.3554, -0.0227, 0.1206, -0.0670};
+    static const double Yc[5] = {-1.4630, 0.4275, 5.3251, -2.5771, 0.3703};
+    static const double xt[5] = {-0.0193, -0.0665, -0.0004, -0.0641, -0.0033};
+    static const double xc[5] = {-0.2592, 0.0008, 0.2125, -0.8989, 0.0452};
+    static const double yt[5] = {-0.0167, -0.0950, -0.0079, -0.0441, -0.0109};
+    static const double yc[5] = {-0.2608, 0.0092, 0.2102, -1.6537, 0.0529};
+    double ts = M_PI / 2.0 - el;
+    if (ts < 0) ts = 0;
+    if (ts > M_PI / 2.0) ts = M_PI / 2.0;
+    m->theta_s = ts;
+    m->cos_theta_s = cos(ts);
+    m->sun[0] = cos(el) * sin(az);
+    m->sun[1] = sin(el);
+    m->sun[2] = cos(el) * cos(az);
+    m->pY = perez_of(T, Yt, Yc);
+    m->px = perez_of(T, xt, xc);
+    m->py = perez_of(T, yt, yc);
+    const double chi = (4.0 / 9.0 - T / 120.0) * (M_PI - 2.0 * ts);
+    m->yz = (4.0453 * T - 4.9710) * tan(chi) - 0.2155 * T + 2.4192;
+    if (m->yz < 0) m->yz = 0;
+    const double t1 = ts, t2 = ts * ts, t3 = t2 * ts;
+    m->xz = (0.00166 * t3 - 0.00375 * t2 + 0.00209 * t1 + 0) * T * T +
+            (-0.02903 * t3 + 0.06377 * t2 - 0.03202 * t1 + 0.00394) * T +
+            (0.11693 * t3 - 0.21196 * t2 + 0.06052 * t1 + 0.25886);
+    m->yzc = (0.00275 * t3 - 0.00610 * t2 + 0.00317 * t1 + 0) * T * T +
+             (-0.04214 * t3 + 0.08970 * t2 - 0.04153 * t1 + 0.00516) * T +
+             (0.15346 * t3 - 0.26756 * t2 + 0.06670 * t1 + 0.26688);
+    m->scale = s->sky_scale;
+}
+
+static double perez_f(const perez* c, double cos_theta, double gamma)
+{
+    const double sc = cos_theta > 1e-3 ? cos_theta : 1e-3;
+    const double t1 = 1.0 + c->A * exp(c->B / sc);
+    const double cg = cos(gamma);
+    const double t2 = 1.0 + c->C * exp(c->D * gamma) + c->E * cg * cg;
+    return t1 * t2;
+}
+
+/* SampleRadiance (HosekWilkieSkyModel.cpp:220-256) times the map's scale. */
+static double sky_radiance(const sky_model* m, v3 d, double nm)
+{
+    if (!m->on) return 0.0;
+    if (nm < 380.0 || nm > 780.0) return 0.0;
+    d = normalize(d);
+    if (d.y <= 0) return 0.0;
+    const double ct = d.y > 0 ? d.y : 0;
+    double ds = d.x * m->sun[0] + d.y * m->sun[1] + d.z * m->sun[2];
+    if (ds < -1) ds = -1;
+    if (ds > 1) ds = 1;
+    const double g = acos(ds);
+    const double Y = m->yz * perez_f(&m->pY, ct, g) / perez_f(&m->pY, 1.0, m->theta_s);
+    const double x = m->xz * perez_f(&m->px, ct, g) / perez_f(&m->px, 1.0, m->theta_s);
+    const double y = m->yzc * perez_f(&m->py, ct, g) / perez_f(&m->py, 1.0, m->theta_s);
+    if (Y < 0) return 0.0;
+    if (y < 1e-6) return 0.0;
+    const double X = (x / y) * Y, Z = ((1.0 - x - y) / y) * Y;
+    double obs[3];
+    if (!rise_oracle_xyz_from_nm(nm, obs)) return 0.0;
+    const double den = obs[0] + obs[1] + obs[2];
+    if (den < 1e-9) return 0.0;
+    return (X * obs[0] + Y * obs[1] + Z * obs[2]) / den * m->scale;
+}
+
+double rise_oracle_sky_radiance(const ro_scene* scene, const double dir[3], double nm)
+{
+    sky_model m;
+    sky_init(&m, scene);
+    return sky_radiance(&m, mk(dir[0], dir[1], dir[2]), nm);
+}
+
 /* ---- scene preparation --------------------------------------------------- */
 
 typedef struct {
@@ -167,6 +263,7 @@ typedef struct {
     double lum_total;
     int32_t* lum_of_object; /* -1 if not a luminary */
     double y_norm;
+    sky_model sky;
     /* work distribution */
     int32_t s0, s1;
     double* xyz_sum;
@@ -244,6 +341,7 @@ static int prepare(prep* P, const ro_scene* s)
     }
     const double ky = rise_oracle_cie_y_integral(s->lambda_begin, s->lambda_end);
     P->y_norm = ky > NZ ? (s->lambda_end - s->lambda_begin) / ky : 1.0;
+    sky_init(&P->sky, s);
     return 0;
 }
 
@@ -716,7 +814,12 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
             double t;
             ++*n_ext;
             const int64_t tri = closest(P, org, dir, &t);
-            if (tri < 0) break;
+            if (tri < 0) {
+                /* Escape: the radiance map at this wavelength, full weight (no
+                 * environment light sample to weigh against). */
+                result += thr * sky_radiance(&P->sky, dir, nm);
+                break;
+            }
             make_hit(P, org, dir, tri, t, &h);
         }
         const double amb = ior_top(&st);
@@ -808,7 +911,13 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         double t;
         ++n_ext;
         const int64_t tri = closest(P, org, dir, &t);
-        if (tri < 0) break; /* no environment in these scenes */
+        if (tri < 0) {
+            /* Escape (PathTracingIntegrator.cpp:4230-4290, 5070-5110): the sky at each
+             * wavelength; a camera ray sees it only when the map is the background. */
+            if (depth > 0 || s->sky_background)
+                for (int w = 0; w < HWSS_N; ++w) result[w] += thr[w] * sky_radiance(&P->sky, dir, lam[w]);
+            break;
+        }
         hit h;
         make_hit(P, org, dir, tri, t, &h);
 

@@ -23,6 +23,8 @@
  *   Utilities/GeometricUtilities.cpp:85-98, 543-568                 cosine sampling, point on triangle
  *   Utilities/OrthonormalBasis3D.cpp:37-56, 138-144                 ONB from W
  *   Objects/Object.cpp:641-643                                      hit point back-off
+ *   Utilities/HosekWilkieSkyModel.cpp:20-301, Rendering/HosekWilkieSpectralRadianceMap.cpp
+ *                                                                   sky radiance on escape
  * Random numbers: the reference draws per-dimension samples from an
  * Owen-scrambled Sobol sequence (stream 16 + depth, 32 dimensions per
  * stream) and hero wavelength / pixel jitter from a Mersenne Twister; it is
@@ -81,6 +83,11 @@ typedef struct ro_scene {
     int32_t rr_min_depth;
     double rr_threshold;
     uint64_t seed;
+    /* Sky radiance map (HosekWilkieSpectralRadianceMap: the Preetham model,
+     * Utilities/HosekWilkieSkyModel.cpp), seen by rays that leave the scene. */
+    int32_t sky_enabled;
+    int32_t sky_background;     /* camera rays that miss see the sky */
+    double sky_elevation_deg, sky_azimuth_deg, sky_turbidity, sky_scale;
 } ro_scene;
 
 /* Counter-based uniform in [0,1) with 24 random bits (exactly representable
@@ -96,6 +103,9 @@ int rise_oracle_render(const ro_scene* scene, int32_t s0, int32_t s1, double* xy
 
 /* XYZFromNM (ColorUtils.cpp:313-325); returns 0 outside 380-780 nm. */
 int rise_oracle_xyz_from_nm(double nm, double xyz[3]);
+
+/* Sky radiance at wavelength nm in direction dir (unit), scale included. */
+double rise_oracle_sky_radiance(const ro_scene* scene, const double dir[3], double nm);
 
 /* CIE_Y_Integral (ColorUtils.cpp:327-350). */
 double rise_oracle_cie_y_integral(double a, double b);

@@ -55,6 +55,18 @@ struct DevLuminary {
     int pad;
 };
 
+// Sky radiance map (HosekWilkieSpectralRadianceMap over the Preetham model,
+// Utilities/HosekWilkieSkyModel.cpp): Perez coefficients for Y, x, y, the
+// zenith values divided by the zenith Perez term, the sun direction, scale.
+struct DevSky {
+    int on;
+    float theta_s;
+    float sun[3];
+    float pz[3][5];   // A..E for Y, x, y
+    float zen[3];     // Yz / F_Y(zenith), xz / F_x(zenith), yz / F_y(zenith)
+    float scale;
+};
+
 struct PtSceneView {
     const NodeSlot* nodes;
     const Tri48* tris;      // leaf order; v0.w = global id, v1.w = object id
@@ -67,6 +79,7 @@ struct PtSceneView {
     const DevLuminary* lums;
     const unsigned* lum_tris;   // leaf-order triangle indices
     const float* lum_cdf;       // area CDF per luminary
+    DevSky sky;
 };
 
 // ---- random numbers: counter-based hash ------------------------------------

@@ -83,6 +83,32 @@ __device__ inline bool xyz_from_nm(float nm, float& X, float& Y, float& Z)
     return true;
 }
 
+// HosekWilkieSkyModel::SampleRadiance (Utilities/HosekWilkieSkyModel.cpp:
+// 172-256) times the map's scale: Perez-weighted xyY in direction d (unit),
+// turned into a radiance at nm by the colour-matching functions.
+__device__ inline float sky_radiance(const DevSky& k, f3 d, float nm)
+{
+    if (!k.on || nm < 380.0f || nm > 780.0f) return 0.0f;
+    d = norm3(d);
+    if (d.y <= 0.0f) return 0.0f;
+    const float ds = fmaxf(-1.0f, fminf(1.0f, d.x * k.sun[0] + d.y * k.sun[1] + d.z * k.sun[2]));
+    const float g = acosf(ds), cg = cosf(g);
+    const float sc = fmaxf(1e-3f, d.y);
+    float v[3];
+    for (int c = 0; c < 3; ++c) {
+        const float* p = k.pz[c];
+        v[c] = k.zen[c] * (1.0f + p[0] * expf(p[1] / sc)) * (1.0f + p[2] * expf(p[3] * g) + p[4] * cg * cg);
+    }
+    const float Y = v[0], x = v[1], y = v[2];
+    if (Y < 0.0f || y < 1e-6f) return 0.0f;
+    const float X = (x / y) * Y, Z = ((1.0f - x - y) / y) * Y;
+    float ox, oy, oz;
+    if (!xyz_from_nm(nm, ox, oy, oz)) return 0.0f;
+    const float den = ox + oy + oz;
+    if (den < 1e-9f) return 0.0f;
+    return (X * ox + Y * oy + Z * oz) / den * k.scale;
+}
+
 // One request: scene tables, render parameters, film accumulators, output.
 struct ReqInfo {
     PtSceneView sc;
@@ -90,6 +116,7 @@ struct ReqInfo {
     int max_diffuse, max_glossy, max_transmission, rr_min_depth;
     float rr_threshold, lambda_begin, lambda_end;
     float xyz_scale;  // mYNormalization / N (PathTracingSpectralRasterizer.cpp:168-173)
+    int sky_background;  // camera rays that miss see the sky
     unsigned long long seed;
     float cam_pos[3], cam_f[3], cam_r[3], cam_u[3];
     float cam_h;      // 2 tan(fov / 2)
@@ -491,6 +518,9 @@ __device__ inline bool bump_bounces(unsigned& st, int type, const ReqInfo& R)
     return true;
 }
 
+constexpr unsigned kLaneDims = 1u << 20;  // dimension offset of single-wavelength lane w: (w + 1) << 20
+constexpr unsigned kModeNm = 1u;          // flags bit: the path continues as single-wavelength lanes
+
 // Bins the extension queue's hits by material kind (misses end their path),
 // so each shading kernel runs one material's code over a dense queue.
 __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
@@ -505,11 +535,25 @@ __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
         unsigned code = 0;
         if (i < n) {
             code = q[i];
-            const unsigned leaf = S.hit_leaf[(code & 3u) * (unsigned)S.cap + (code >> 2)];
+            const unsigned slot = code >> 2, lane = code & 3u, li = lane * (unsigned)S.cap + slot;
+            const unsigned leaf = S.hit_leaf[li];
+            const ReqInfo& R = reqs[S.req[slot]];
+            const PtSceneView& sc = R.sc;
             if (leaf != 0xffffffffu) {
-                const PtSceneView& sc = reqs[S.req[code >> 2]].sc;
                 const int obj = float_to_bits(reinterpret_cast<const float4*>(sc.tris + leaf)[1].w);
                 kind = sc.materials[sc.objects[obj].material].type;
+            } else if (sc.sky.on && (depth > 0 || R.sky_background)) {
+                // Escape (PathTracingIntegrator.cpp:4230-4290, 5070-5110 and the
+                // single-wavelength twin): the sky at each wavelength the ray
+                // carries, full weight (no environment light sample to weigh
+                // against).
+                float lam[kHwss];
+                wavelengths(S.lambda0[slot], R.lambda_begin, R.lambda_end, lam);
+                const f3 d = mk3(S.dx[li], S.dy[li], S.dz[li]);
+                const bool nm = S.flags[slot] & kModeNm;
+                for (int w = 0; w < kHwss; ++w)
+                    if (!nm || (unsigned)w == lane)
+                        S.res[w * S.cap + slot] += S.thr[w * S.cap + slot] * sky_radiance(sc.sky, d, lam[w]);
             }
         }
         for (int m = 0; m < 3; ++m) {
@@ -519,8 +563,6 @@ __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
     }
 }
 
-constexpr unsigned kLaneDims = 1u << 20;  // dimension offset of single-wavelength lane w: (w + 1) << 20
-constexpr unsigned kModeNm = 1u;          // flags bit: the path continues as single-wavelength lanes
 
 // Writes lane w's next ray, offset off the surface toward its side, and
 // appends it to the next extension queue at `at`.
@@ -1082,6 +1124,7 @@ rise_hip_status make_info(const rise_hip_render_request& q, ReqInfo& I)
     const double ynorm = ky > 1e-12 ? ((double)D.lambda_end - D.lambda_begin) / ky : 1.0;
     I.xyz_scale = (float)(ynorm / kHwss);
     I.seed = D.seed;
+    I.sky_background = D.sky_background;
     const double px = D.camera.position[0], py = D.camera.position[1], pz = D.camera.position[2];
     double f[3] = {D.camera.look_at[0] - px, D.camera.look_at[1] - py, D.camera.look_at[2] - pz};
     const double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);

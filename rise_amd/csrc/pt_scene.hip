@@ -234,3 +234,60 @@ extern "C" rise_hip_status rise_hip_pt_scene_release(rise_hip_pt_scene* scene)
     if (scene) free_scene(scene);
     return RISE_HIP_OK;
 }
+
+// Job::AddHosekWilkieSkylight without the sun light: HosekWilkieSkyModel's
+// constructor (Utilities/HosekWilkieSkyModel.cpp:98-150) evaluated once in
+// FP64; the per-direction part runs on the device (sky_radiance).
+extern "C" rise_hip_status rise_hip_pt_scene_set_sky(rise_hip_pt_scene* scene, const rise_hip_sky_desc* sky)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    release_solo_batch(scene);  // a cached plan holds a copy of the scene tables
+    DevSky& d = scene->view.sky;
+    d = DevSky{};
+    if (!sky) return RISE_HIP_OK;
+    if (!(sky->turbidity >= 1.0f && sky->turbidity <= 64.0f) || !(sky->sky_scale >= 0.0f))
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad sky turbidity or scale");
+    const double el = sky->solar_elevation_deg * (M_PI / 180.0), az = sky->solar_azimuth_deg * (M_PI / 180.0);
+    const double T = sky->turbidity;
+    // Preetham 1999 Table 2: each coefficient is t * T + c.
+    static const double kT[3][5] = {{0.1787, -0.3554, -0.0227, 0.1206, -0.0670},
+                                    {-0.0193, -0.0665, -0.0004, -0.0641, -0.0033},
+                                    {-0.0167, -0.0950, -0.0079, -0.0441, -0.0109}};
+    static const double kC[3][5] = {{-1.4630, 0.4275, 5.3251, -2.5771, 0.3703},
+                                    {-0.2592, 0.0008, 0.2125, -0.8989, 0.0452},
+                                    {-0.2608, 0.0092, 0.2102, -1.6537, 0.0529}};
+    const double ts = std::max(0.0, std::min(M_PI / 2.0, M_PI / 2.0 - el));
+    double p[3][5];
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 5; ++k) p[c][k] = kT[c][k] * T + kC[c][k];
+    // Zenith luminance (kcd/m^2) and chromaticity (Preetham 1999 section 4.4).
+    const double chi = (4.0 / 9.0 - T / 120.0) * (M_PI - 2.0 * ts);
+    double yz = (4.0453 * T - 4.9710) * std::tan(chi) - 0.2155 * T + 2.4192;
+    if (yz < 0) yz = 0;
+    const double t1 = ts, t2 = ts * ts, t3 = t2 * ts;
+    const double xz = (0.00166 * t3 - 0.00375 * t2 + 0.00209 * t1) * T * T +
+                      (-0.02903 * t3 + 0.06377 * t2 - 0.03202 * t1 + 0.00394) * T +
+                      (0.11693 * t3 - 0.21196 * t2 + 0.06052 * t1 + 0.25886);
+    const double yc = (0.00275 * t3 - 0.00610 * t2 + 0.00317 * t1) * T * T +
+                      (-0.04214 * t3 + 0.08970 * t2 - 0.04153 * t1 + 0.00516) * T +
+                      (0.15346 * t3 - 0.26756 * t2 + 0.06670 * t1 + 0.26688);
+    // Perez F at the zenith (cos theta = 1, gamma = theta_s): the ratio's denominator.
+    auto perez = [&](int c, double cos_theta, double gamma) {
+        const double sc = std::max(1e-3, cos_theta);
+        const double cg = std::cos(gamma);
+        return (1.0 + p[c][0] * std::exp(p[c][1] / sc)) * (1.0 + p[c][2] * std::exp(p[c][3] * gamma) + p[c][4] * cg * cg);
+    };
+    const double zv[3] = {yz, xz, yc};
+    d.on = 1;
+    d.theta_s = (float)ts;
+    d.sun[0] = (float)(std::cos(el) * std::sin(az));
+    d.sun[1] = (float)std::sin(el);
+    d.sun[2] = (float)(std::cos(el) * std::cos(az));
+    for (int c = 0; c < 3; ++c) {
+        for (int k = 0; k < 5; ++k) d.pz[c][k] = (float)p[c][k];
+        d.zen[c] = (float)(zv[c] / perez(c, 1.0, ts));
+    }
+    d.scale = sky->sky_scale;
+    return RISE_HIP_OK;
+}

@@ -78,6 +78,17 @@ class Camera:
 
 
 @dataclass
+class Sky:
+    """The spectral sky radiance map of ``hosek_wilkie_skylight``
+    (HosekWilkieSpectralRadianceMap; the reference's model is Preetham 1999):
+    sun elevation / azimuth in degrees, turbidity, radiance scale."""
+    elevation_deg: float = 45.0
+    azimuth_deg: float = 0.0
+    turbidity: float = 3.0
+    scale: float = 1.0
+
+
+@dataclass
 class RenderSettings:
     width: int
     height: int
@@ -90,6 +101,7 @@ class RenderSettings:
     rr_min_depth: int = 3
     rr_threshold: float = 0.05
     seed: int = 1
+    sky_background: bool = True   # camera rays that leave the scene see the sky
 
 
 @dataclass
@@ -98,6 +110,7 @@ class Scene:
     materials: list = field(default_factory=list)
     objects: list = field(default_factory=list)
     camera: Camera | None = None
+    sky: Sky | None = None
 
     def flat(self):
         """All triangles (n, 9) float32 and their object index (n,) int32."""
@@ -317,4 +330,31 @@ def cornell_materials(glass_subdiv: int = 2):
     ]
     s.objects[4] = Object(s.objects[4].tris, 4)
     s.objects.insert(5, Object(icosphere((0.35, 0.3, 0.85), 0.3, glass_subdiv), 5))
+    return s, r
+
+
+def sky_spheres(n_side: int = 8, seed: int = 3):
+    """A ground quad under the spectral sky with a grid of small spheres on
+    it, alternately GGX conductors (roughness drawn from U[0.05, 0.5]) and
+    smooth glass (index 1.5): a reduced version of the terrain-and-spheres
+    workload for parity tests.  No area lights: the sky is the only source."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.spds = [Spd.flat(0.4), Spd.flat(1.0), Spd(380.0, 720.0, np.array([1.3, 0.6, 0.2], np.float32)),
+              Spd(380.0, 720.0, np.array([1.9, 2.4, 3.2], np.float32)), Spd.flat(1.5)]
+    s.materials = [Material(0), Material(kind=DIELECTRIC, ior=4)]
+    s.objects = [Object(quad((-6, 0, -6), (-6, 0, 6), (6, 0, 6), (6, 0, -6)), 0)]
+    for i in range(n_side):
+        for j in range(n_side):
+            c = (-4 + 8 * (i + 0.5) / n_side, 0.35, -4 + 8 * (j + 0.5) / n_side)
+            if (i + j) % 2 == 0:
+                s.materials.append(Material(kind=GGX, reflectance=-1, specular=1, ior=2, extinction=3,
+                                            alpha=float(rng.uniform(0.05, 0.5))))
+                s.objects.append(Object(icosphere(c, 0.35, 1), len(s.materials) - 1))
+            else:
+                s.objects.append(Object(icosphere(c, 0.35, 1), 1))
+    s.camera = Camera(position=(0.0, 4.0, 9.0), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=45.0)
+    s.sky = Sky(elevation_deg=40.0, azimuth_deg=30.0, turbidity=3.0, scale=0.05)
+    r = RenderSettings(width=64, height=48, spp=16, max_diffuse_bounce=8, max_glossy_bounce=8,
+                       max_transmission_bounce=8)
     return s, r

@@ -48,6 +48,15 @@ class ObjectDesc(ctypes.Structure):
     ]
 
 
+class SkyDesc(ctypes.Structure):
+    _fields_ = [
+        ("solar_elevation_deg", ctypes.c_float),
+        ("solar_azimuth_deg", ctypes.c_float),
+        ("turbidity", ctypes.c_float),
+        ("sky_scale", ctypes.c_float),
+    ]
+
+
 class CameraDesc(ctypes.Structure):
     _fields_ = [
         ("position", ctypes.c_float * 3),
@@ -73,6 +82,8 @@ class RenderDesc(ctypes.Structure):
         ("rr_threshold", ctypes.c_float),
         ("seed", ctypes.c_uint64),
         ("camera", CameraDesc),
+        ("sky_background", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -114,6 +125,7 @@ _SIG = {
     "rise_hip_pt_scene_create": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.POINTER(_P)],
                                  ctypes.c_int),
     "rise_hip_pt_scene_release": ([_P], ctypes.c_int),
+    "rise_hip_pt_scene_set_sky": ([_P, ctypes.POINTER(SkyDesc)], ctypes.c_int),
     "rise_hip_render": ([_P, ctypes.POINTER(RenderDesc), _P, _P, ctypes.POINTER(RenderStats), _P], ctypes.c_int),
     "rise_hip_batch_create": ([_P, ctypes.c_size_t, ctypes.POINTER(_P)], ctypes.c_int),
     "rise_hip_batch_create_tuned": ([_P, ctypes.c_size_t, ctypes.POINTER(BatchTuning), ctypes.POINTER(_P)],
@@ -155,6 +167,7 @@ def render_desc(settings: RenderSettings, scene: Scene, spp: int | None = None, 
     d.camera.look_at[:] = c.look_at
     d.camera.up[:] = c.up
     d.camera.fov_deg = c.fov_deg
+    d.sky_background = 1 if settings.sky_background else 0
     return d
 
 
@@ -195,6 +208,9 @@ class PathTracingSpectralRasterizer:
         _lib.check(L.rise_hip_pt_scene_create(objs, len(scene.objects), mats, len(scene.materials), spds,
                                               len(scene.spds), ctypes.byref(out)))
         self._h = out
+        if scene.sky is not None:
+            sd = SkyDesc(scene.sky.elevation_deg, scene.sky.azimuth_deg, scene.sky.turbidity, scene.sky.scale)
+            _lib.check(L.rise_hip_pt_scene_set_sky(self._h, ctypes.byref(sd)))
         self._keep = []
 
     def rasterize_scene(self, settings: RenderSettings, spp: int | None = None, spp_offset: int = 0,

@@ -80,3 +80,25 @@ def test_gpu_materials_statistical_parity(cuda):
     z = (g - o) / np.sqrt(vg + vo + 1e-30)
     frac = float(np.mean(np.abs(z[..., 1]) < 3.29))
     assert frac >= 0.99, frac
+
+
+def test_gpu_sky_scene_matches_oracle(cuda):
+    # Ground, GGX and glass spheres under the spectral sky (the only light): escapes
+    # from shared and single-wavelength rays add the sky at their wavelengths.
+    log_case("sky + GGX + glass spheres 48x36 16spp same streams")
+    s, r = pt_scene.sky_spheres(n_side=6)
+    r.width, r.height, r.spp = 48, 36, 16
+    g, _, st = _render(s, r)
+    sm, _, cnt = oracle.render(s, r)
+    o = sm / r.spp
+    floor = 0.01 * np.abs(o[..., 1]).mean()
+    rel = np.abs(g - o) / (np.abs(o) + floor)
+    frac = float(np.mean(np.all(rel <= 1e-2, axis=-1)))
+    assert frac >= 0.97, frac
+    assert abs(g[..., 1].mean() / o[..., 1].mean() - 1) < 2e-3
+    assert abs(st["extension_rays"] / cnt[0] - 1) < 1e-2
+    # background off: camera escapes are black, the rest unchanged
+    r.sky_background = False
+    g0, _, _ = _render(s, r)
+    assert np.all(g0 <= g + 1e-6)
+    assert g0[..., 1].mean() < g[..., 1].mean()

@@ -13,6 +13,7 @@
 import math
 
 import numpy as np
+import pytest
 
 import oracle
 from rise_amd import pt_scene
@@ -110,3 +111,67 @@ def test_oracle_ggx_furnace():
     sm, _, _ = oracle.render(s, r)
     y = sm[..., 1] / r.spp
     assert abs(y.mean() - 1.0) < 0.03, y.mean()
+
+
+def _sky_scene(elev=45.0, az=0.0, turb=3.0, scale=1.0):
+    s, r = pt_scene.cornell_box()
+    s.sky = pt_scene.Sky(elev, az, turb, scale)
+    return s, r
+
+
+def _sky(s, d, nm):
+    import ctypes
+
+    L = oracle.oracle_lib()
+    L.rise_oracle_sky_radiance.restype = ctypes.c_double
+    # build the ro_scene the same way render does, through a 1x1 render's parameters
+    rs = oracle._RoScene()
+    rs.sky_enabled = 1
+    rs.sky_elevation_deg, rs.sky_azimuth_deg = s.sky.elevation_deg, s.sky.azimuth_deg
+    rs.sky_turbidity, rs.sky_scale = s.sky.turbidity, s.sky.scale
+    v = (ctypes.c_double * 3)(*d)
+    L.rise_oracle_sky_radiance.argtypes = [ctypes.POINTER(oracle._RoScene), ctypes.c_void_p, ctypes.c_double]
+    return L.rise_oracle_sky_radiance(ctypes.byref(rs), ctypes.cast(v, ctypes.c_void_p), nm)
+
+
+def test_oracle_sky_invariants():
+    # tests/HosekWilkieReferenceTest.cpp: zenith and horizon positive, ground zero,
+    # a clear sky bluer than a hazy one; plus outside-band zero, symmetry about the
+    # sun's vertical plane, linearity in the scale, brightest toward the sun.
+    s, _ = _sky_scene(85.0, 0.0, 2.0)
+    zen = _sky(s, (0, 1, 0), 550)
+    hor = _sky(s, (1, 0.01, 0), 550)
+    assert zen > 0 and hor > 0 and math.isfinite(zen) and math.isfinite(hor)
+    assert _sky(s, (0, -1, 0), 550) == 0.0
+    assert _sky(s, (0, 1, 0), 379.0) == 0.0 and _sky(s, (0, 1, 0), 781.0) == 0.0
+    clear, _ = _sky_scene(45.0, 0.0, 2.0)
+    hazy, _ = _sky_scene(45.0, 0.0, 8.0)
+    br = lambda sk: _sky(sk, (0, 1, 0), 450) / _sky(sk, (0, 1, 0), 650)  # noqa: E731
+    assert br(clear) > br(hazy)
+    s, _ = _sky_scene(30.0, 40.0, 3.0)
+    sun = (math.cos(math.radians(30)) * math.sin(math.radians(40)), math.sin(math.radians(30)),
+           math.cos(math.radians(30)) * math.cos(math.radians(40)))
+    # mirror a direction across the vertical plane containing the sun
+    n = np.cross(np.array(sun), np.array([0.0, 1.0, 0.0]))
+    n /= np.linalg.norm(n)
+    d = np.array([0.3, 0.5, -0.2])
+    d /= np.linalg.norm(d)
+    m = d - 2 * np.dot(d, n) * n
+    assert _sky(s, tuple(d), 520) == pytest.approx(_sky(s, tuple(m), 520), rel=1e-12)
+    assert _sky(s, sun, 520) > _sky(s, (-sun[0], sun[1], -sun[2]), 520)
+    s2, _ = _sky_scene(30.0, 40.0, 3.0, 2.5)
+    assert _sky(s2, tuple(d), 520) == pytest.approx(2.5 * _sky(s, tuple(d), 520), rel=1e-12)
+
+
+def test_oracle_sky_seen_by_camera():
+    # A camera ray that leaves the scene sees the sky at each of its wavelengths; with
+    # the background off it sees black.
+    s, r = pt_scene.sky_spheres()
+    s.camera = pt_scene.Camera((0, 1, 0), (0, 2, 0.01), (0, 0, 1), 5.0)   # straight up, nothing above
+    r.width = r.height = 4
+    r.spp = 64
+    sm, _, _ = oracle.render(s, r)
+    assert sm[..., 1].mean() > 0
+    r.sky_background = False
+    sm0, _, _ = oracle.render(s, r)
+    assert np.all(sm0 == 0)
