@@ -146,6 +146,125 @@ def render_leg(args, cpu_enabled: bool):
     return out
 
 
+def terrain_leg(args):
+    """Terrain-and-spheres spectral path tracing: a 2048x1024 value-noise
+    heightfield (4,188,162 triangles) with 64 GGX-conductor and glass
+    icospheres (327,680 triangles) under the spectral sky, 1920x1080, 64 spp,
+    depth 8.  Rays = extension + shadow queries actually traced."""
+    import numpy as np
+    import torch
+
+    from rise_amd import pt_scene
+    from rise_amd.render import PathTracingSpectralRasterizer
+
+    t0 = time.perf_counter()
+    scene, rs = pt_scene.terrain_spheres()
+    pt = PathTracingSpectralRasterizer(scene)
+    setup = time.perf_counter() - t0
+    film, _, stats = pt.rasterize_scene(rs)  # warm-up, plans the render, counts rays
+    torch.cuda.synchronize()
+    rays = stats["extension_rays"] + stats["shadow_rays"]
+    lat = []
+    for _ in range(max(1, args.terrain_steps)):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        pt.rasterize_scene(rs, film=film, stats=False)
+        b.record()
+        torch.cuda.synchronize()
+        lat.append(a.elapsed_time(b))
+    ms = float(np.mean(lat))
+    y = film[..., 1].float().mean().item()
+    out = {
+        "workload": "terrain 2048x1024 value noise (4,188,162 tris) + 64 icospheres (GGX alpha U[0.05,0.5] / glass "
+        "1.5), spectral sky, 1920x1080, 64 spp, depth 8",
+        "n_tris": int(scene.n_tris),
+        "ms_per_render": round(ms, 1),
+        "mrays_per_s": round(rays / (ms / 1e3) / 1e6, 1),
+        "paths_per_s": round(stats["paths"] / (ms / 1e3), 1),
+        "rays_per_render": {"extension": stats["extension_rays"], "shadow": stats["shadow_rays"]},
+        "mean_Y": round(y, 6),
+        "setup_s": round(setup, 1),
+    }
+    log(f"terrain: {ms:.0f} ms, {out['mrays_per_s']} Mrays/s, {out['paths_per_s']:.3g} paths/s "
+        f"({scene.n_tris} tris, setup {setup:.1f} s)")
+    pt.release()
+    return out
+
+
+def serving_leg(args):
+    """Serving: 16 resident scenes of 50k-2M triangles (Lambertian, GGX and
+    glass), a Poisson stream (200 requests/s) of 64 render requests at
+    128^2-1024^2 and 1-64 spp.  The server admits requests as they arrive and
+    batches them continuously; latency is submit-to-finish per request.
+    Compared with the same requests rendered one at a time with
+    rise_hip_render, back to back."""
+    import numpy as np
+    import torch
+
+    from rise_amd import pt_scene
+    from rise_amd.render import PathTracingSpectralRasterizer, RenderServer
+
+    t0 = time.perf_counter()
+    scenes, reqs = pt_scene.serving_workload(n_requests=args.serving_requests)
+    rasts = [PathTracingSpectralRasterizer(s) for s in scenes]
+    setup = time.perf_counter() - t0
+    paths = sum(r.width * r.height * r.spp for _, _, r in reqs)
+    log(f"serving: {len(reqs)} requests, {paths / 1e6:.1f} M paths over {len(scenes)} scenes "
+        f"({sum(s.n_tris for s in scenes) / 1e6:.1f} M tris, setup {setup:.1f} s)")
+    sv = RenderServer()
+    for _, k, r in reqs[:4]:   # warm-up: plans kernels and the allocator
+        sv.submit(rasts[k], pt_scene.RenderSettings(**{**r.__dict__, "width": 32, "height": 32, "spp": 1}))
+    sv.drain()
+    torch.cuda.synchronize()
+    st0 = sv.stats()
+    lat = {}
+    pending = list(reqs)
+    t_start = time.perf_counter()
+    submitted = 0
+    while len(lat) < len(reqs):
+        now = time.perf_counter() - t_start
+        while pending and pending[0][0] <= now:
+            _, k, r = pending.pop(0)
+            tk, _, _ = sv.submit(rasts[k], r)
+            submitted += 1
+        for tk, l in sv.poll():
+            lat[tk] = l
+        if pending and not sv.stats()["admitted"]:
+            time.sleep(max(0.0, min(0.001, pending[0][0] - (time.perf_counter() - t_start))))
+    makespan = time.perf_counter() - t_start
+    st = sv.stats()
+    rays = (st["extension_rays"] - st0["extension_rays"]) + (st["shadow_rays"] - st0["shadow_rays"])
+    lats = np.array(sorted(lat.values())) * 1e3
+    sv.release()
+    # The same requests one at a time through the single-request path.
+    w0 = time.perf_counter()
+    for _, k, r in reqs:
+        rasts[k].rasterize_scene(r, stats=False)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - w0
+    out = {
+        "workload": "Poisson 200 req/s, 64 requests over 16 resident scenes (50k-2M tris; Lambertian/GGX/glass), "
+        "res {128,256,512,1024}^2, spp {1,4,16,64}, depth 6",
+        "requests": len(reqs),
+        "paths": int(paths),
+        "makespan_s": round(makespan, 3),
+        "requests_per_s": round(len(reqs) / makespan, 2),
+        "mrays_per_s": round(rays / makespan / 1e6, 1),
+        "paths_per_s": round(paths / makespan, 1),
+        "latency_ms_p50": round(float(np.percentile(lats, 50)), 1),
+        "latency_ms_p99": round(float(np.percentile(lats, 99)), 1),
+        "passes": st["passes"] - st0["passes"],
+        "sequential_single_request": {"s": round(seq_s, 3), "requests_per_s": round(len(reqs) / seq_s, 2)},
+        "arrival_span_s": round(reqs[-1][0], 3),
+    }
+    log(f"serving: makespan {makespan:.2f} s, {out['requests_per_s']} req/s, {out['mrays_per_s']} Mrays/s, "
+        f"p50 {out['latency_ms_p50']} ms, p99 {out['latency_ms_p99']} ms; sequential {seq_s:.2f} s")
+    for r in rasts:
+        r.release()
+    return out
+
+
 def batch_leg(args):
     """Batched rendering: 64 requests over 8 seeded Cornell-style scenes with
     10k-500k added triangles, resolutions {128..1024}^2, spp {1, 4, 16, 64},
@@ -271,6 +390,10 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering leg")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true", help="batched leg with default launch shapes")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the terrain-and-spheres render leg")
+    ap.add_argument("--terrain-steps", type=int, default=1)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-requests", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -387,6 +510,8 @@ def main() -> int:
 
     render = None if args.no_render else render_leg(args, cpu_enabled=not args.no_cpu_baseline)
     batched = None if args.no_batch else batch_leg(args)
+    terrain = None if args.no_terrain else terrain_leg(args)
+    serving = None if args.no_serving else serving_leg(args)
 
     traffic = load_traffic()
     line = {
@@ -427,6 +552,8 @@ def main() -> int:
         "parity": parity,
         "render": render,
         "batch": batched,
+        "terrain": terrain,
+        "serving": serving,
     }
     om.release()
     print(json.dumps(line), flush=True)

@@ -184,6 +184,54 @@ rise_hip_status rise_hip_batch_get_stats(rise_hip_batch* batch, hipStream_t stre
 
 rise_hip_status rise_hip_batch_release(rise_hip_batch* batch);
 
+/* ---- Serving: continuous batching of render requests ----
+ *
+ * Replaces the reference's one-job-at-a-time rendering (Job::Rasterize,
+ * Job.cpp:9268, called per request by its front ends) for a stream of
+ * requests: scenes stay resident on the device (rise_hip_pt_scene), requests
+ * are submitted at any time, admitted while their film accumulators (48 B
+ * per pixel) fit the HBM budget, and share every pass with the requests
+ * already rendering.  Each film is bit-identical to rise_hip_render of that
+ * request alone.  All work runs on the stream given at creation. */
+typedef struct rise_hip_server_config {
+    int64_t hbm_budget_bytes;     /* film accumulators of admitted requests (0: 8 GiB); a request larger
+                                     than the budget is still admitted when nothing else is */
+    int32_t max_requests;         /* admitted at once (0: 256, at most 256) */
+    int32_t reserved;
+    rise_hip_batch_tuning tuning; /* launch shapes (max_paths: the path pool) */
+} rise_hip_server_config;
+
+typedef struct rise_hip_server_stats {
+    int64_t submitted, completed, admitted, waiting;
+    int64_t passes, paths, extension_rays, shadow_rays;  /* over completed passes */
+    int64_t hbm_in_use;                                  /* film accumulators now allocated */
+} rise_hip_server_stats;
+
+typedef struct rise_hip_server rise_hip_server;
+
+rise_hip_status rise_hip_server_create(const rise_hip_server_config* config, hipStream_t stream,
+                                       rise_hip_server** out_server);
+
+/* Queues a request (host call; nothing waits).  The scene and the films must
+ * stay valid until the request is reported done. */
+rise_hip_status rise_hip_server_submit(rise_hip_server* server, const rise_hip_render_request* request,
+                                       uint64_t* out_ticket);
+
+/* Makes progress without blocking: retires finished passes, admits waiting
+ * requests, launches passes (up to 4 in flight).  Reports up to `capacity`
+ * finished requests: their tickets and submit-to-finish latency in ns. */
+rise_hip_status rise_hip_server_poll(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
+                                     size_t capacity, size_t* n_done);
+
+/* As poll, but blocks until every submitted request has finished. */
+rise_hip_status rise_hip_server_drain(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
+                                      size_t capacity, size_t* n_done);
+
+rise_hip_status rise_hip_server_get_stats(const rise_hip_server* server, rise_hip_server_stats* out_stats);
+
+/* Waits for the stream, then frees everything (unfinished requests are dropped). */
+rise_hip_status rise_hip_server_release(rise_hip_server* server);
+
 #ifdef __cplusplus
 }
 #endif

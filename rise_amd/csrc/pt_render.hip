@@ -38,8 +38,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -1068,7 +1072,7 @@ size_t carve(char* base, int64_t cap, int passes, Paths* P, unsigned** counters)
 
 // Workgroups of `kernel` resident at once on the current device.
 template <class K>
-unsigned resident_grid(K kernel, int block)
+unsigned resident_grid_uncached(K kernel, int block)
 {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -1079,10 +1083,58 @@ unsigned resident_grid(K kernel, int block)
     return (unsigned)(cus * per_cu);
 }
 
+template <class K>
+unsigned resident_grid(K kernel, int block)
+{
+    // Occupancy per (kernel, block size, device), queried once.
+    static std::mutex mu;
+    static std::vector<std::tuple<const void*, int, int, unsigned>> cache;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const auto& c : cache)
+            if (std::get<0>(c) == (const void*)kernel && std::get<1>(c) == block && std::get<2>(c) == cur)
+                return std::get<3>(c);
+    }
+    const unsigned g = resident_grid_uncached(kernel, block);
+    std::lock_guard<std::mutex> lk(mu);
+    cache.emplace_back((const void*)kernel, block, cur, g);
+    return g;
+}
+
+
 unsigned grid_for(int64_t n, int block, unsigned max_blocks)
 {
     const int64_t g = (n + block - 1) / block;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, max_blocks));
+}
+
+// Traversal launch shape: the node-group stack holds at most one entry per
+// level below the root, and the smallest LDS stack that fits keeps occupancy
+// highest; persistent waves pay off on deep trees, one ray per thread on
+// shallow ones.
+void choose_traversal(const rise_hip_batch_tuning& tn, int depth, int64_t max_tris, int64_t cap, int& stack,
+                      int& block, bool& persist, TraceKernel& ext_fn, TraceKernel& con_fn, unsigned& ge, unsigned& gc)
+{
+    const int need_stack = depth - 1;
+    const int auto_stack = need_stack <= 8 ? 8 : (need_stack <= 16 ? 16 : 0);
+    if (tn.stack == 8 && need_stack <= 8) stack = 8;
+    else if (tn.stack == 16 && need_stack <= 16) stack = 16;
+    else if (tn.stack == -1) stack = 0;
+    else stack = auto_stack;
+    block = tn.trace_block ? tn.trace_block : 128;
+    persist = tn.persistent == 1 || (tn.persistent == 0 && max_tris >= 4096);
+    ext_fn = pick_trace<ExtendOf>(stack, block, persist);
+    con_fn = pick_trace<ConnectOf>(stack, block, persist);
+    if (persist) {
+        // Never more workgroups than queue entries of a full pass.
+        ge = std::min(resident_grid(ext_fn, block), grid_for(4 * cap, block, 1u << 30));
+        gc = std::min(resident_grid(con_fn, block), grid_for(4 * cap, block, 1u << 30));
+    } else {
+        ge = grid_for(4 * cap, block, 8192);
+        gc = grid_for(4 * cap, block, 8192);
+    }
 }
 
 rise_hip_status validate(const rise_hip_render_request& q, size_t k)
@@ -1347,26 +1399,8 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     b->pass_paths.swap(pass_paths);
     b->pass_film.swap(pass_film);
     b->req_px.swap(req_px);
-    // Traversal: the node-group stack holds at most one entry per level below
-    // the root; the smallest LDS stack that fits keeps occupancy highest.
-    const int need_stack = depth - 1;
-    const int auto_stack = need_stack <= 8 ? 8 : (need_stack <= 16 ? 16 : 0);
-    if (tn.stack == 8 && need_stack <= 8) b->stack = 8;
-    else if (tn.stack == 16 && need_stack <= 16) b->stack = 16;
-    else if (tn.stack == -1) b->stack = 0;
-    else b->stack = auto_stack;
-    b->trace_block = tn.trace_block ? tn.trace_block : 128;
-    b->persist = tn.persistent == 1 || (tn.persistent == 0 && max_tris >= 4096);
-    b->ext_fn = pick_trace<ExtendOf>(b->stack, b->trace_block, b->persist);
-    b->con_fn = pick_trace<ConnectOf>(b->stack, b->trace_block, b->persist);
-    if (b->persist) {
-        // Never more workgroups than queue entries of a full pass.
-        b->grid_ext = std::min(resident_grid(b->ext_fn, b->trace_block), grid_for(4 * cap, b->trace_block, 1u << 30));
-        b->grid_con = std::min(resident_grid(b->con_fn, b->trace_block), grid_for(4 * cap, b->trace_block, 1u << 30));
-    } else {
-        b->grid_ext = grid_for(4 * cap, b->trace_block, 8192);
-        b->grid_con = grid_for(4 * cap, b->trace_block, 8192);
-    }
+    choose_traversal(tn, depth, max_tris, cap, b->stack, b->trace_block, b->persist, b->ext_fn, b->con_fn,
+                     b->grid_ext, b->grid_con);
     b->bounces = bounces;
     b->mat_mask = mat_mask;
     b->n_req = (int)n;
@@ -1376,39 +1410,61 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
 }
 
 // Kernel launches of a planned batch (no host synchronization, no copies).
+// Launch shape of one pass: kernel variants and grids.
+struct PassShape {
+    TraceKernel ext_fn, con_fn;
+    unsigned ge, gc, block;
+    int64_t cap;
+    int bounces;
+    unsigned mat_mask;
+};
+
+// Kernels of one pass over `n_ent` chunks at `ent` (paths and film threads
+// given), counters of the pass in S.cnt_*: generate, the bounce loop, film.
+void launch_pass(const ReqInfo* d_info, Paths S, const PassEntry* ent, int n_ent, int paths, int film,
+                 const PassShape& sh, hipStream_t stream)
+{
+    const unsigned gs = grid_for(4 * sh.cap, kShadeBlock, 2048);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, S.cnt_ext, kCounters);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<unsigned*>(S.fetch),
+                       4 * (kMaxPathDepth + 1));
+    hipLaunchKernelGGL(k_generate, dim3(grid_for(paths, 256, 4096)), dim3(256), 0, stream, d_info, ent, n_ent, paths,
+                       S);
+    for (int d = 0; d < sh.bounces; ++d) {
+        void* args[] = {(void*)&d_info, &S, &d};
+        hipLaunchKernel((const void*)sh.ext_fn, dim3(sh.ge), dim3(sh.block), args, 0, stream);
+        hipLaunchKernelGGL(k_bin, dim3(grid_for(4 * sh.cap, 256, 2048)), dim3(256), 0, stream, d_info, S, d);
+        if (sh.mat_mask & (1u << kMatLambert))
+            hipLaunchKernelGGL(k_shade<kMatLambert>, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
+        if (sh.mat_mask & (1u << kMatGgx))
+            hipLaunchKernelGGL(k_shade<kMatGgx>, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
+        if (sh.mat_mask & (1u << kMatDielectric))
+            hipLaunchKernelGGL(k_shade<kMatDielectric>, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
+        hipLaunchKernel((const void*)sh.con_fn, dim3(sh.gc), dim3(sh.block), args, 0, stream);
+    }
+    hipLaunchKernelGGL(k_film, dim3(grid_for(film, 256, 4096)), dim3(256), 0, stream, d_info, ent, n_ent, film, S);
+}
+
+Paths pass_counters(Paths S, unsigned* block)
+{
+    S.cnt_ext = block;
+    S.cnt_sh = block + (kMaxPathDepth + 1);
+    S.cnt_mat = block + 2 * (kMaxPathDepth + 1);
+    return S;
+}
+
+// Kernel launches of a planned batch (no host synchronization, no copies).
 hipError_t launch(rise_hip_batch* b, hipStream_t stream)
 {
-    const unsigned ge = b->grid_ext, gc = b->grid_con;
-    const unsigned gs = grid_for(4 * b->cap, kShadeBlock, 2048);
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)b->film_floats, 256, 4096)), dim3(256), 0, stream, b->d_film,
                        b->film_floats);
+    const PassShape sh{b->ext_fn, b->con_fn, b->grid_ext, b->grid_con, (unsigned)b->trace_block,
+                       b->cap,    b->bounces, b->mat_mask};
     const int passes = (int)b->pass_paths.size();
     for (int p = 0; p < passes; ++p) {
-        Paths S = b->S;
-        S.cnt_ext = b->counters + (size_t)p * kCounters;
-        S.cnt_sh = S.cnt_ext + (kMaxPathDepth + 1);
-        S.cnt_mat = S.cnt_ext + 2 * (kMaxPathDepth + 1);
-        const PassEntry* ent = b->d_entries + b->pass_first[p];
-        const int n_ent = b->pass_first[p + 1] - b->pass_first[p];
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, S.cnt_ext, kCounters);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<unsigned*>(S.fetch),
-                           4 * (kMaxPathDepth + 1));
-        hipLaunchKernelGGL(k_generate, dim3(grid_for(b->pass_paths[p], 256, 4096)), dim3(256), 0, stream, b->d_info,
-                           ent, n_ent, b->pass_paths[p], S);
-        for (int d = 0; d < b->bounces; ++d) {
-            void* args[] = {&b->d_info, &S, &d};
-            hipLaunchKernel((const void*)b->ext_fn, dim3(ge), dim3(b->trace_block), args, 0, stream);
-            hipLaunchKernelGGL(k_bin, dim3(grid_for(4 * b->cap, 256, 2048)), dim3(256), 0, stream, b->d_info, S, d);
-            if (b->mat_mask & (1u << kMatLambert))
-                hipLaunchKernelGGL(k_shade<kMatLambert>, dim3(gs), dim3(kShadeBlock), 0, stream, b->d_info, S, d);
-            if (b->mat_mask & (1u << kMatGgx))
-                hipLaunchKernelGGL(k_shade<kMatGgx>, dim3(gs), dim3(kShadeBlock), 0, stream, b->d_info, S, d);
-            if (b->mat_mask & (1u << kMatDielectric))
-                hipLaunchKernelGGL(k_shade<kMatDielectric>, dim3(gs), dim3(kShadeBlock), 0, stream, b->d_info, S, d);
-            hipLaunchKernel((const void*)b->con_fn, dim3(gc), dim3(b->trace_block), args, 0, stream);
-        }
-        hipLaunchKernelGGL(k_film, dim3(grid_for(b->pass_film[p], 256, 4096)), dim3(256), 0, stream, b->d_info, ent,
-                           n_ent, b->pass_film[p], S);
+        const Paths S = pass_counters(b->S, b->counters + (size_t)p * kCounters);
+        launch_pass(b->d_info, S, b->d_entries + b->pass_first[p], b->pass_first[p + 1] - b->pass_first[p],
+                    b->pass_paths[p], b->pass_film[p], sh, stream);
     }
     for (int k = 0; k < b->n_req; ++k)
         hipLaunchKernelGGL(k_resolve, dim3(grid_for(3 * b->req_px[k], 256, 2048)), dim3(256), 0, stream, b->d_info, k);
@@ -1538,5 +1594,379 @@ extern "C" rise_hip_status rise_hip_render(rise_hip_pt_scene* scene, const rise_
     const hipError_t e = launch(b, stream);
     if (e != hipSuccess) return fail_hip(e, "render launch");
     if (stats) return read_stats(b, stream, stats);
+    return RISE_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching of render requests
+// ---------------------------------------------------------------------------
+//
+// Requests are submitted at any time and admitted while their film
+// accumulators fit the HBM budget and a request slot is free.  Each pass takes
+// the next chunk of every admitted request (the chunks of rise_hip_batch), so
+// a request that arrives while others render shares the next pass with them.
+// Passes are launched ahead on the server's stream (a ring of kRing passes in
+// flight), each with its chunk table staged in pinned memory; a request is
+// done once the pass holding its last chunk, and its resolve, have run.  The
+// chunking and the per-pixel summation order are those of rise_hip_render, so
+// every film is bit-identical to rendering that request alone.
+
+namespace {
+
+constexpr int kRing = 4;          // passes in flight
+constexpr int kServerSlots = 256; // admitted requests at most
+
+struct ServerReq {
+    uint64_t ticket = 0;
+    rise_hip_render_request q{};
+    std::vector<PassEntry> chunks;
+    size_t next = 0;
+    int slot = -1;
+    float* acc = nullptr;          // film accumulators, 12 floats per pixel
+    size_t acc_bytes = 0;
+    int last_ring = -1;            // ring slot of the pass that holds the last chunk
+    uint64_t last_pass = 0;        // sequence number of that pass
+    int64_t submit_ns = 0;
+    int64_t px = 0;
+    int hits = 1;                  // bounce iterations this request needs
+    int depth = 1;                 // tree depth of its scene
+    int64_t tris = 0;
+    unsigned kinds = 0;
+};
+
+int64_t now_ns()
+{
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+struct rise_hip_server {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t budget = 0, used = 0;
+    int max_slots = kServerSlots;
+    rise_hip_batch_tuning tuning{};
+    int64_t cap = 0;
+    void* work = nullptr;
+    Paths S{};
+    unsigned* d_counters = nullptr;  // kRing blocks of kCounters
+    unsigned* h_counters = nullptr;  // pinned copies of the same
+    ReqInfo* d_info = nullptr;       // kServerSlots
+    ReqInfo* h_info = nullptr;       // pinned staging, kServerSlots
+    PassEntry* d_ent = nullptr;      // kRing * kServerSlots
+    PassEntry* h_ent = nullptr;      // pinned staging
+    hipEvent_t ev[kRing] = {};
+    bool busy[kRing] = {};
+    int pass_bounces[kRing] = {};
+    int64_t pass_paths[kRing] = {};
+    uint64_t pass_seq[kRing] = {};
+    uint64_t seq = 0;                // passes launched
+    uint64_t retired_upto = 0;       // every pass with sequence < this has completed
+    int ring_next = 0;
+    uint64_t next_ticket = 1;
+    std::deque<ServerReq> waiting;
+    std::vector<ServerReq> active;   // admission order
+    std::vector<int> free_slots;
+    std::vector<std::pair<uint64_t, int64_t>> done;  // (ticket, latency ns) not yet reported
+    rise_hip_server_stats stats{};
+};
+
+namespace {
+
+void server_free(rise_hip_server* sv)
+{
+    if (!sv) return;
+    for (auto& r : sv->active)
+        if (r.acc) (void)hipFreeAsync(r.acc, sv->stream);
+    (void)hipStreamSynchronize(sv->stream);
+    for (int k = 0; k < kRing; ++k)
+        if (sv->ev[k]) (void)hipEventDestroy(sv->ev[k]);
+    (void)hipFree(sv->work);
+    (void)hipFree(sv->d_counters);
+    (void)hipFree(sv->d_info);
+    (void)hipFree(sv->d_ent);
+    (void)hipHostFree(sv->h_counters);
+    (void)hipHostFree(sv->h_info);
+    (void)hipHostFree(sv->h_ent);
+    delete sv;
+}
+
+// Marks completed passes and requests; counts their rays.
+void server_retire(rise_hip_server* sv, bool block)
+{
+    for (int k = 0; k < kRing; ++k) {
+        if (!sv->busy[k]) continue;
+        const hipError_t e = block ? hipEventSynchronize(sv->ev[k]) : hipEventQuery(sv->ev[k]);
+        if (e != hipSuccess) continue;
+        sv->busy[k] = false;
+        const unsigned* c = sv->h_counters + (size_t)k * kCounters;
+        for (int d = 0; d < sv->pass_bounces[k]; ++d) {
+            sv->stats.extension_rays += c[d];
+            sv->stats.shadow_rays += c[kMaxPathDepth + 1 + d];
+        }
+        sv->stats.paths += sv->pass_paths[k];
+    }
+    // Passes complete in launch order on one stream: the oldest still busy bounds retired_upto.
+    uint64_t lim = sv->seq;
+    for (int k = 0; k < kRing; ++k)
+        if (sv->busy[k]) lim = std::min(lim, sv->pass_seq[k]);
+    sv->retired_upto = lim;
+    const int64_t t = now_ns();
+    for (size_t i = 0; i < sv->active.size();) {
+        ServerReq& r = sv->active[i];
+        if (r.next == r.chunks.size() && r.last_ring >= 0 && r.last_pass < sv->retired_upto) {
+            sv->done.emplace_back(r.ticket, t - r.submit_ns);
+            sv->used -= (int64_t)r.acc_bytes;
+            sv->free_slots.push_back(r.slot);
+            sv->stats.completed += 1;
+            sv->active.erase(sv->active.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    }
+}
+
+// Admits waiting requests in arrival order while memory and slots allow.
+rise_hip_status server_admit(rise_hip_server* sv)
+{
+    while (!sv->waiting.empty() && !sv->free_slots.empty()) {
+        ServerReq& r = sv->waiting.front();
+        if (sv->used + (int64_t)r.acc_bytes > sv->budget && !sv->active.empty()) break;
+        const int slot = sv->free_slots.back();
+        ReqInfo info;
+        rise_hip_status st = make_info(r.q, info);
+        if (st != RISE_HIP_OK) return st;
+        hipError_t e = hipMallocAsync((void**)&r.acc, r.acc_bytes, sv->stream);
+        if (e != hipSuccess) return fail_hip(e, "film accumulators");
+        const size_t f = 3 * (size_t)r.px;
+        info.fsum = r.acc;
+        info.fcomp = r.acc + f;
+        info.fsq = r.acc + 2 * f;
+        info.fsqc = r.acc + 3 * f;
+        sv->h_info[slot] = info;
+        e = hipMemcpyAsync(sv->d_info + slot, sv->h_info + slot, sizeof(ReqInfo), hipMemcpyHostToDevice, sv->stream);
+        if (e != hipSuccess) return fail_hip(e, "request table");
+        hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)(4 * f), 256, 4096)), dim3(256), 0, sv->stream, r.acc, 4 * f);
+        sv->free_slots.pop_back();
+        r.slot = slot;
+        for (PassEntry& c : r.chunks) c.req = slot;
+        sv->used += (int64_t)r.acc_bytes;
+        sv->active.push_back(std::move(r));
+        sv->waiting.pop_front();
+    }
+    return RISE_HIP_OK;
+}
+
+// Launches passes while the ring has room and admitted requests have chunks left.
+rise_hip_status server_launch(rise_hip_server* sv)
+{
+    for (;;) {
+        const int k = sv->ring_next;
+        if (sv->busy[k]) return RISE_HIP_OK;
+        PassEntry* ent = sv->h_ent + (size_t)k * kServerSlots;
+        int n_ent = 0;
+        int64_t paths = 0, film = 0, tris = 0;
+        int bounces = 1, depth = 1;
+        unsigned kinds = 0;
+        std::vector<ServerReq*> finishing;
+        for (ServerReq& r : sv->active) {
+            if (r.next >= r.chunks.size()) continue;
+            PassEntry e = r.chunks[r.next];
+            const int64_t pp = (int64_t)e.np * e.ns;
+            if (paths + pp > sv->cap) continue;
+            e.slot0 = (int)paths;
+            e.film0 = (int)film;
+            paths += pp;
+            film += e.np;
+            ent[n_ent++] = e;
+            r.next++;
+            bounces = std::max(bounces, r.hits);
+            depth = std::max(depth, r.depth);
+            tris = std::max(tris, r.tris);
+            kinds |= r.kinds;
+            if (r.next == r.chunks.size()) finishing.push_back(&r);
+        }
+        if (n_ent == 0) return RISE_HIP_OK;
+        PassShape sh{};
+        int stack = 0, block = 128;
+        bool persist = true;
+        choose_traversal(sv->tuning, depth, tris, sv->cap, stack, block, persist, sh.ext_fn, sh.con_fn, sh.ge, sh.gc);
+        sh.block = (unsigned)block;
+        sh.cap = sv->cap;
+        sh.bounces = bounces;
+        sh.mat_mask = kinds;
+        PassEntry* d_ent = sv->d_ent + (size_t)k * kServerSlots;
+        hipError_t e = hipMemcpyAsync(d_ent, ent, (size_t)n_ent * sizeof(PassEntry), hipMemcpyHostToDevice, sv->stream);
+        if (e != hipSuccess) return fail_hip(e, "pass table");
+        unsigned* cnt = sv->d_counters + (size_t)k * kCounters;
+        launch_pass(sv->d_info, pass_counters(sv->S, cnt), d_ent, n_ent, (int)paths, (int)film, sh, sv->stream);
+        for (ServerReq* r : finishing) {
+            hipLaunchKernelGGL(k_resolve, dim3(grid_for(3 * r->px, 256, 2048)), dim3(256), 0, sv->stream, sv->d_info,
+                               r->slot);
+            e = hipFreeAsync(r->acc, sv->stream);
+            if (e != hipSuccess) return fail_hip(e, "film accumulators");
+            r->acc = nullptr;
+            r->last_ring = k;
+            r->last_pass = sv->seq;
+        }
+        e = hipMemcpyAsync(sv->h_counters + (size_t)k * kCounters, cnt, kCounters * sizeof(unsigned),
+                           hipMemcpyDeviceToHost, sv->stream);
+        if (e == hipSuccess) e = hipEventRecord(sv->ev[k], sv->stream);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) return fail_hip(e, "pass launch");
+        sv->busy[k] = true;
+        sv->pass_bounces[k] = bounces;
+        sv->pass_paths[k] = paths;
+        sv->pass_seq[k] = sv->seq++;
+        sv->stats.passes += 1;
+        sv->ring_next = (k + 1) % kRing;
+    }
+}
+
+rise_hip_status server_report(rise_hip_server* sv, uint64_t* tickets, int64_t* latency_ns, size_t cap,
+                              size_t* n_done)
+{
+    size_t n = 0;
+    while (n < cap && !sv->done.empty()) {
+        if (tickets) tickets[n] = sv->done.front().first;
+        if (latency_ns) latency_ns[n] = sv->done.front().second;
+        sv->done.erase(sv->done.begin());
+        ++n;
+    }
+    if (n_done) *n_done = n;
+    return RISE_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" rise_hip_status rise_hip_server_create(const rise_hip_server_config* config, hipStream_t stream,
+                                                  rise_hip_server** out_server)
+{
+    clear_error();
+    if (!out_server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "out_server is NULL");
+    *out_server = nullptr;
+    rise_hip_server_config cfg{};
+    if (config) cfg = *config;
+    if (cfg.hbm_budget_bytes < 0 || cfg.max_requests < 0 || cfg.max_requests > kServerSlots)
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad server configuration");
+    if (cfg.tuning.max_paths < 0 || cfg.tuning.persistent < 0 || cfg.tuning.persistent > 2 ||
+        (cfg.tuning.trace_block != 0 && cfg.tuning.trace_block != 64 && cfg.tuning.trace_block != 128 &&
+         cfg.tuning.trace_block != 256) ||
+        (cfg.tuning.stack != 0 && cfg.tuning.stack != 8 && cfg.tuning.stack != 16 && cfg.tuning.stack != -1))
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad tuning values");
+    rise_hip_server* sv = new (std::nothrow) rise_hip_server;
+    if (!sv) return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    sv->stream = stream;
+    sv->budget = cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : (int64_t(8) << 30);
+    sv->max_slots = cfg.max_requests > 0 ? cfg.max_requests : kServerSlots;
+    sv->tuning = cfg.tuning;
+    sv->cap = cfg.tuning.max_paths > 0 ? std::max<int64_t>(kChunkPaths, std::min<int64_t>(cfg.tuning.max_paths, kMaxPaths))
+                                       : kMaxPaths;
+    hipError_t e = hipGetDevice(&sv->device);
+    const size_t need = carve(nullptr, sv->cap, 1, nullptr, nullptr);
+    if (e == hipSuccess) e = hipMalloc(&sv->work, need);
+    if (e == hipSuccess) e = hipMalloc((void**)&sv->d_counters, sizeof(unsigned) * kCounters * kRing);
+    if (e == hipSuccess) e = hipMalloc((void**)&sv->d_info, sizeof(ReqInfo) * kServerSlots);
+    if (e == hipSuccess) e = hipMalloc((void**)&sv->d_ent, sizeof(PassEntry) * kServerSlots * kRing);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->h_counters, sizeof(unsigned) * kCounters * kRing, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->h_info, sizeof(ReqInfo) * kServerSlots, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->h_ent, sizeof(PassEntry) * kServerSlots * kRing, 0);
+    for (int k = 0; k < kRing && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&sv->ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        server_free(sv);
+        return fail_hip(e, "server allocation");
+    }
+    carve((char*)sv->work, sv->cap, 1, &sv->S, nullptr);
+    for (int k = sv->max_slots - 1; k >= 0; --k) sv->free_slots.push_back(k);
+    *out_server = sv;
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_server_submit(rise_hip_server* server, const rise_hip_render_request* request,
+                                                  uint64_t* out_ticket)
+{
+    clear_error();
+    if (!server || !request) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    rise_hip_status st = validate(*request, 0);
+    if (st != RISE_HIP_OK) return st;
+    {
+        ReqInfo probe;
+        st = make_info(*request, probe);  // camera checks, so admission cannot fail later
+        if (st != RISE_HIP_OK) return st;
+    }
+    ServerReq r;
+    r.q = *request;
+    r.ticket = server->next_ticket++;
+    r.submit_ns = now_ns();
+    const rise_hip_render_desc& D = request->desc;
+    r.px = (int64_t)D.width * D.height;
+    r.acc_bytes = 12 * sizeof(float) * (size_t)r.px;
+    chunks_of(D, 0, r.chunks);
+    const rise_hip_pt_scene* sc = request->scene;
+    r.hits = std::min(kMaxPathDepth, 1 + D.max_diffuse_bounce + (sc->has_glossy ? D.max_glossy_bounce : 0) +
+                                         (sc->has_transmission ? D.max_transmission_bounce : 0));
+    r.depth = sc->view.depth;
+    r.tris = sc->n_tris;
+    r.kinds = sc->material_kinds;
+    for (const PassEntry& c : r.chunks)
+        if ((int64_t)c.np * c.ns > server->cap) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "chunk exceeds path pool");
+    if (out_ticket) *out_ticket = r.ticket;
+    server->waiting.push_back(std::move(r));
+    server->stats.submitted += 1;
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_server_poll(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
+                                                size_t capacity, size_t* n_done)
+{
+    clear_error();
+    if (!server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "server is NULL");
+    server_retire(server, false);
+    rise_hip_status st = server_admit(server);
+    if (st == RISE_HIP_OK) st = server_launch(server);
+    if (st != RISE_HIP_OK) return st;
+    return server_report(server, tickets, latency_ns, capacity, n_done);
+}
+
+extern "C" rise_hip_status rise_hip_server_drain(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
+                                                 size_t capacity, size_t* n_done)
+{
+    clear_error();
+    if (!server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "server is NULL");
+    while (!server->waiting.empty() || !server->active.empty()) {
+        server_retire(server, false);
+        rise_hip_status st = server_admit(server);
+        if (st == RISE_HIP_OK) st = server_launch(server);
+        if (st != RISE_HIP_OK) return st;
+        if (!server->active.empty() || !server->waiting.empty()) {
+            // wait for the oldest pass in flight before going round again
+            int oldest = -1;
+            for (int k = 0; k < kRing; ++k)
+                if (server->busy[k] && (oldest < 0 || server->pass_seq[k] < server->pass_seq[oldest])) oldest = k;
+            if (oldest >= 0) (void)hipEventSynchronize(server->ev[oldest]);
+            else if (server->waiting.empty() && server->active.empty()) break;
+        }
+    }
+    server_retire(server, true);
+    return server_report(server, tickets, latency_ns, capacity, n_done);
+}
+
+extern "C" rise_hip_status rise_hip_server_get_stats(const rise_hip_server* server, rise_hip_server_stats* out)
+{
+    clear_error();
+    if (!server || !out) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = server->stats;
+    out->admitted = (int64_t)server->active.size();
+    out->waiting = (int64_t)server->waiting.size();
+    out->hbm_in_use = server->used;
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_server_release(rise_hip_server* server)
+{
+    clear_error();
+    server_free(server);
     return RISE_HIP_OK;
 }

@@ -255,6 +255,18 @@ def batch_workload(seed: int = 4, n_instances: int = 64, n_scenes: int = 8):
 def icosphere(center=(0.0, 0.0, 0.0), radius: float = 1.0, subdiv: int = 2) -> np.ndarray:
     """Icosahedron subdivided `subdiv` times and projected to the sphere:
     20 * 4^subdiv triangles (n, 9) float32, outward face normals."""
+    unit = _unit_icosphere(subdiv)
+    c = np.tile(np.asarray(center, np.float64), 3)
+    return np.ascontiguousarray((unit * radius + c).astype(np.float32))
+
+
+_ICO_CACHE: dict = {}
+
+
+def _unit_icosphere(subdiv: int) -> np.ndarray:
+    if subdiv in _ICO_CACHE:
+        return _ICO_CACHE[subdiv]
+    center, radius = (0.0, 0.0, 0.0), 1.0
     t = (1.0 + 5 ** 0.5) / 2.0
     verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
              (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -279,12 +291,13 @@ def icosphere(center=(0.0, 0.0, 0.0), radius: float = 1.0, subdiv: int = 2) -> n
             nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         faces = nf
     V = np.array(verts) * radius + np.asarray(center, np.float64)
-    tris = np.array([np.concatenate([V[a], V[b], V[c]]) for a, b, c in faces], np.float32)
+    tris = np.array([np.concatenate([V[a], V[b], V[c]]) for a, b, c in faces], np.float64)
     # orient outward
-    cen = tris.reshape(-1, 3, 3).mean(1) - np.asarray(center, np.float32)
+    cen = tris.reshape(-1, 3, 3).mean(1)
     n = np.cross(tris[:, 3:6] - tris[:, 0:3], tris[:, 6:9] - tris[:, 0:3])
     flip = np.sum(n * cen, 1) < 0
     tris[flip] = tris[flip].reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9)
+    _ICO_CACHE[subdiv] = tris
     return tris
 
 
@@ -358,3 +371,116 @@ def sky_spheres(n_side: int = 8, seed: int = 3):
     r = RenderSettings(width=64, height=48, spp=16, max_diffuse_bounce=8, max_glossy_bounce=8,
                        max_transmission_bounce=8)
     return s, r
+
+
+def value_noise(nx: int, ny: int, octaves: int = 6, seed: int = 7) -> np.ndarray:
+    """Seeded value noise on an nx x ny grid: the sum over octaves k of
+    bilinearly interpolated random lattices of 4 * 2^k cells, amplitude 2^-k,
+    normalized to [-1, 1]."""
+    rng = np.random.default_rng(seed)
+    u = np.linspace(0.0, 1.0, nx)
+    v = np.linspace(0.0, 1.0, ny)
+    h = np.zeros((ny, nx))
+    amp, total = 1.0, 0.0
+    for k in range(octaves):
+        cells = 4 * (1 << k)
+        lat = rng.uniform(-1.0, 1.0, (cells + 2, cells + 2))
+        x = u * cells
+        y = v * cells
+        x0 = np.floor(x).astype(int)
+        y0 = np.floor(y).astype(int)
+        fx = x - x0
+        fy = y - y0
+        fx = fx * fx * (3 - 2 * fx)
+        fy = fy * fy * (3 - 2 * fy)
+        a = lat[np.ix_(y0, x0)]
+        b = lat[np.ix_(y0, x0 + 1)]
+        c = lat[np.ix_(y0 + 1, x0)]
+        d = lat[np.ix_(y0 + 1, x0 + 1)]
+        top = a + (b - a) * fx[None, :]
+        bot = c + (d - c) * fx[None, :]
+        h += amp * (top + (bot - top) * fy[:, None])
+        total += amp
+        amp *= 0.5
+    return h / total
+
+
+def heightfield(nx: int, ny: int, extent=(20.0, 10.0), amplitude: float = 0.2, seed: int = 7) -> np.ndarray:
+    """Triangulated heightfield over an nx x ny vertex grid centred at the
+    origin (x across, z along, y up): 2 (nx-1)(ny-1) triangles, upward normals."""
+    h = amplitude * value_noise(nx, ny, 6, seed)
+    xs = np.linspace(-extent[0] / 2, extent[0] / 2, nx)
+    zs = np.linspace(-extent[1] / 2, extent[1] / 2, ny)
+    X, Z = np.meshgrid(xs, zs)
+    P = np.stack([X, h, Z], -1).astype(np.float32)
+    p00, p10, p01, p11 = P[:-1, :-1], P[:-1, 1:], P[1:, :-1], P[1:, 1:]
+    # (p00, p01, p10) and (p10, p01, p11): normals +y for x across, z along
+    t1 = np.concatenate([p00, p01, p10], -1).reshape(-1, 9)
+    t2 = np.concatenate([p10, p01, p11], -1).reshape(-1, 9)
+    return np.ascontiguousarray(np.concatenate([t1, t2], 0))
+
+
+def terrain_spheres(nx: int = 2048, ny: int = 1024, subdiv: int = 4, seed: int = 7):
+    """The terrain-and-spheres workload: a 2048 x 1024 heightfield of seeded
+    6-octave value noise (amplitude 0.2; 4,188,162 triangles) with 64
+    icospheres (subdivision 4, 5,120 triangles each) resting on it, alternately
+    GGX conductors (roughness U[0.05, 0.5]) and smooth glass (index 1.5),
+    under the spectral sky; 1920 x 1080, 64 spp, depth 8."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.spds = [Spd.rgb_bands(0.25, 0.3, 0.18), Spd.flat(1.0), Spd(380.0, 720.0, np.array([1.3, 0.6, 0.2], np.float32)),
+              Spd(380.0, 720.0, np.array([1.9, 2.4, 3.2], np.float32)), Spd.flat(1.5)]
+    s.materials = [Material(0), Material(kind=DIELECTRIC, ior=4)]
+    s.objects = [Object(heightfield(nx, ny, seed=seed), 0)]
+    for i in range(8):
+        for j in range(8):
+            c = (-8.0 + 16.0 * (i + 0.5) / 8, 0.55, -4.0 + 8.0 * (j + 0.5) / 8)
+            if (i + j) % 2 == 0:
+                s.materials.append(Material(kind=GGX, reflectance=-1, specular=1, ior=2, extinction=3,
+                                            alpha=float(rng.uniform(0.05, 0.5))))
+                s.objects.append(Object(icosphere(c, 0.4, subdiv), len(s.materials) - 1))
+            else:
+                s.objects.append(Object(icosphere(c, 0.4, subdiv), 1))
+    s.camera = Camera(position=(0.0, 5.0, 12.0), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=50.0)
+    s.sky = Sky(elevation_deg=35.0, azimuth_deg=40.0, turbidity=3.0, scale=2e-4)
+    r = RenderSettings(width=1920, height=1080, spp=64, max_diffuse_bounce=8, max_glossy_bounce=8,
+                       max_transmission_bounce=8)
+    return s, r
+
+
+def serving_workload(seed: int = 5, n_requests: int = 64, n_scenes: int = 16, rate: float = 200.0):
+    """The serving workload: `n_scenes` resident scenes of 50k-2M triangles
+    (log-uniform; Cornell-style boxes with a seeded triangle cloud, every third
+    cloud a GGX conductor, every third scene also holding a glass sphere and a
+    GGX block) and a Poisson stream of `n_requests` render requests at `rate`
+    per second, each over a scene drawn uniformly, at a resolution drawn from
+    {128, 256, 512, 1024}^2 and spp from {1, 4, 16, 64}.
+    Returns (scenes, [(arrival s, scene index, RenderSettings)])."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(5e4), math.log(2e6), n_scenes)).astype(np.int64)
+    scenes = []
+    for k in range(n_scenes):
+        s, _ = cloud_cornell(seed * 1000 + k, int(counts[k]))
+        if k % 3 == 1:
+            s.spds += [Spd.flat(1.0), Spd.flat(0.2), Spd.flat(3.0)]
+            s.materials.append(Material(kind=GGX, reflectance=-1, specular=len(s.spds) - 3, ior=len(s.spds) - 2,
+                                        extinction=len(s.spds) - 1, alpha=0.3))
+            s.objects[-2] = Object(s.objects[-2].tris, len(s.materials) - 1, double_sided=True)
+        elif k % 3 == 2:
+            s.spds += [Spd.flat(1.0), Spd(380.0, 720.0, np.array([1.3, 0.6, 0.2], np.float32)),
+                       Spd(380.0, 720.0, np.array([1.9, 2.4, 3.2], np.float32)), Spd.flat(1.5)]
+            n = len(s.spds)
+            s.materials += [Material(kind=GGX, reflectance=-1, specular=n - 4, ior=n - 3, extinction=n - 2, alpha=0.25),
+                            Material(kind=DIELECTRIC, ior=n - 1)]
+            s.objects[4] = Object(s.objects[4].tris, len(s.materials) - 2)
+            s.objects.insert(len(s.objects) - 1, Object(icosphere((0.35, 0.3, 0.85), 0.3, 2), len(s.materials) - 1))
+        scenes.append(s)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    reqs = []
+    for i in range(n_requests):
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.choice([1, 4, 16, 64]))
+        r = RenderSettings(width=res, height=res, spp=spp, max_diffuse_bounce=6, max_glossy_bounce=6,
+                           max_transmission_bounce=6, seed=seed * 100000 + i)
+        reqs.append((float(t[i]), int(rng.integers(n_scenes)), r))
+    return scenes, reqs

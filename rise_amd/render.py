@@ -120,6 +120,23 @@ class BatchTuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("max_requests", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("tuning", BatchTuning),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "admitted", "waiting", "passes", "paths",
+                                              "extension_rays", "shadow_rays", "hbm_in_use")]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _SIG = {
     "rise_hip_pt_scene_create": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.POINTER(_P)],
@@ -134,6 +151,12 @@ _SIG = {
     "rise_hip_batch_launch": ([_P, _P], ctypes.c_int),
     "rise_hip_batch_get_stats": ([_P, _P, ctypes.POINTER(RenderStats)], ctypes.c_int),
     "rise_hip_batch_release": ([_P], ctypes.c_int),
+    "rise_hip_server_create": ([ctypes.POINTER(ServerConfig), _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "rise_hip_server_submit": ([_P, ctypes.POINTER(RenderRequest), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "rise_hip_server_poll": ([_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "rise_hip_server_drain": ([_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "rise_hip_server_get_stats": ([_P, ctypes.POINTER(ServerStats)], ctypes.c_int),
+    "rise_hip_server_release": ([_P], ctypes.c_int),
 }
 
 
@@ -304,6 +327,78 @@ class RenderBatch:
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib_render().rise_hip_batch_release(self._h)
             self._h = _P(None)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class RenderServer:
+    """Continuous-batching render server (rise_hip_server_*): scenes stay
+    resident, requests are submitted at any time and share passes with the
+    requests already rendering.  ``submit`` returns a ticket and the film
+    tensor the request will write; ``poll`` / ``drain`` report finished
+    tickets with their submit-to-finish latency in seconds."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, max_requests: int = 0, tuning: dict | None = None, stream=None):
+        import numpy as np  # noqa: F401
+
+        self._stream = stream
+        cfg = ServerConfig(hbm_budget_bytes, max_requests, 0, BatchTuning(**(tuning or {})))
+        out = _P(None)
+        _lib.check(_lib_render().rise_hip_server_create(ctypes.byref(cfg), _stream_handle(stream), ctypes.byref(out)))
+        self._h = out
+        self._keep = {}
+        self._tk = (ctypes.c_uint64 * 512)()
+        self._lat = (ctypes.c_int64 * 512)()
+
+    def submit(self, rast, settings, spp=None, spp_offset: int = 0, spp_total=None, want_sq: bool = False):
+        import torch
+
+        film = torch.empty((settings.height, settings.width, 3), dtype=torch.float32, device="cuda")
+        sq = torch.empty_like(film) if want_sq else None
+        q = RenderRequest()
+        q.scene = rast._h.value
+        q.desc = render_desc(settings, rast.scene, spp, spp_offset, spp_total)
+        q.film_xyz = film.data_ptr()
+        q.film_xyz_sq = sq.data_ptr() if want_sq else None
+        t = ctypes.c_uint64()
+        _lib.check(_lib_render().rise_hip_server_submit(self._h, ctypes.byref(q), ctypes.byref(t)))
+        self._keep[t.value] = (rast, film, sq)
+        return t.value, film, sq
+
+    def _collect(self, fn) -> list:
+        n = ctypes.c_size_t()
+        _lib.check(fn(self._h, self._tk, self._lat, 512, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            out.append((int(self._tk[i]), self._lat[i] * 1e-9))
+            self._keep.pop(int(self._tk[i]), None)
+        return out
+
+    def poll(self) -> list:
+        return self._collect(_lib_render().rise_hip_server_poll)
+
+    def drain(self) -> list:
+        done = []
+        while True:
+            got = self._collect(_lib_render().rise_hip_server_drain)
+            done += got
+            if len(got) < 512:
+                return done
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _lib.check(_lib_render().rise_hip_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def release(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib_render().rise_hip_server_release(self._h)
+            self._h = _P(None)
+        self._keep = {}
 
     def __del__(self):
         try:

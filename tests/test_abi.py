@@ -53,6 +53,16 @@ def test_render_and_batch_arguments_rejected():
     assert L.rise_hip_batch_release(None) == 0
     assert L.rise_hip_render(None, None, None, None, None, None) == 1
     assert L.rise_hip_pt_scene_release(None) == 0
+    # Server: bad configurations and NULL arguments are rejected before any device call.
+    cfg = render.ServerConfig(-1, 0, 0, render.BatchTuning())
+    assert L.rise_hip_server_create(ctypes.byref(cfg), None, ctypes.byref(out)) == 1
+    cfg = render.ServerConfig(0, 0, 0, render.BatchTuning(0, 96, 0, 0))
+    assert L.rise_hip_server_create(ctypes.byref(cfg), None, ctypes.byref(out)) == 1
+    assert b"tuning" in L.rise_hip_last_error()
+    assert L.rise_hip_server_create(None, None, None) == 1
+    assert L.rise_hip_server_submit(None, None, None) == 1
+    assert L.rise_hip_server_poll(None, None, None, 0, None) == 1
+    assert L.rise_hip_server_release(None) == 0
 
 
 def test_cpp_adapter_demo_links():
