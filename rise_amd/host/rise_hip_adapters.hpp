@@ -112,6 +112,42 @@ public:
         return (int)materials_.size() - 1;
     }
 
+    // GGXMaterial with conductor Fresnel (ggx_material: rd, rs, alphax = alphay, ior, extinction).
+    int AddGGXMaterial(int diffuse_spd, int specular_spd, float alpha, int ior_spd, int extinction_spd)
+    {
+        rise_hip_material_desc m{};
+        m.type = RISE_HIP_MATERIAL_GGX;
+        m.reflectance_spd = diffuse_spd;
+        m.emission_spd = -1;
+        m.specular_spd = specular_spd;
+        m.ior_spd = ior_spd;
+        m.extinction_spd = extinction_spd;
+        m.alpha = alpha;
+        materials_.push_back(m);
+        return (int)materials_.size() - 1;
+    }
+
+    // DielectricMaterial, smooth (dielectric_material: ior, tau).
+    int AddDielectricMaterial(int ior_spd, int tau_spd = -1)
+    {
+        rise_hip_material_desc m{};
+        m.type = RISE_HIP_MATERIAL_DIELECTRIC;
+        m.reflectance_spd = -1;
+        m.emission_spd = -1;
+        m.specular_spd = -1;
+        m.ior_spd = ior_spd;
+        m.extinction_spd = tau_spd;
+        materials_.push_back(m);
+        return (int)materials_.size() - 1;
+    }
+
+    // Job::AddHosekWilkieSkylight without the sun light; applied at PrepareForRendering.
+    void SetSky(const rise_hip_sky_desc& sky)
+    {
+        sky_ = sky;
+        has_sky_ = true;
+    }
+
     void AddObject(const float* vertices, int64_t n_tris, int material, bool double_sided = false)
     {
         objects_.push_back({vertices, n_tris, material, double_sided ? 1 : 0});
@@ -122,10 +158,14 @@ public:
         for (size_t i = 0; i < spds_.size(); ++i) spds_[i].amplitudes = amps_[i].data();
         rise_hip_pt_scene_release(scene_);
         scene_ = nullptr;
-        return check(rise_hip_pt_scene_create(objects_.data(), objects_.size(), materials_.data(), materials_.size(),
-                                              spds_.data(), spds_.size(), &scene_),
-                     "PrepareForRendering");
+        if (!check(rise_hip_pt_scene_create(objects_.data(), objects_.size(), materials_.data(), materials_.size(),
+                                            spds_.data(), spds_.size(), &scene_),
+                   "PrepareForRendering"))
+            return false;
+        return !has_sky_ || check(rise_hip_pt_scene_set_sky(scene_, &sky_), "PrepareForRendering (sky)");
     }
+
+    rise_hip_pt_scene* Scene() const { return scene_; }
 
     // RasterizeScene: renders desc into film_xyz (device, W*H*3 mean XYZ).
     bool RasterizeScene(const rise_hip_render_desc& desc, float* film_xyz, hipStream_t stream,
@@ -139,7 +179,55 @@ private:
     std::vector<rise_hip_spd_desc> spds_;
     std::vector<rise_hip_material_desc> materials_;
     std::vector<rise_hip_object_desc> objects_;
+    rise_hip_sky_desc sky_{};
+    bool has_sky_ = false;
     rise_hip_pt_scene* scene_ = nullptr;
+};
+
+// A continuous-batching render server over resident scenes: a job runner
+// submits each incoming render and polls for finished ones.
+class HipRenderServer {
+public:
+    explicit HipRenderServer(hipStream_t stream, int64_t hbm_budget_bytes = 0)
+    {
+        rise_hip_server_config cfg{};
+        cfg.hbm_budget_bytes = hbm_budget_bytes;
+        ok_ = check(rise_hip_server_create(&cfg, stream, &server_), "HipRenderServer");
+    }
+    HipRenderServer(const HipRenderServer&) = delete;
+    HipRenderServer& operator=(const HipRenderServer&) = delete;
+    ~HipRenderServer() { rise_hip_server_release(server_); }
+
+    bool Ok() const { return ok_; }
+
+    bool Submit(const HipSpectralRasterizer& scene, const rise_hip_render_desc& desc, float* film_xyz,
+                uint64_t& ticket)
+    {
+        rise_hip_render_request q{};
+        q.scene = scene.Scene();
+        q.desc = desc;
+        q.film_xyz = film_xyz;
+        return check(rise_hip_server_submit(server_, &q, &ticket), "Submit");
+    }
+
+    // Finished tickets since the last call (non-blocking, or all when wait is set).
+    bool Finished(std::vector<uint64_t>& tickets, bool wait)
+    {
+        uint64_t t[64];
+        int64_t lat[64];
+        size_t n = 0;
+        do {
+            if (!check(wait ? rise_hip_server_drain(server_, t, lat, 64, &n) : rise_hip_server_poll(server_, t, lat, 64, &n),
+                       "Finished"))
+                return false;
+            tickets.insert(tickets.end(), t, t + n);
+        } while (n == 64);
+        return true;
+    }
+
+private:
+    rise_hip_server* server_ = nullptr;
+    bool ok_ = false;
 };
 
 }  // namespace rise_hip_host
