@@ -264,6 +264,7 @@ typedef struct {
     int32_t* lum_of_object; /* -1 if not a luminary */
     double y_norm;
     sky_model sky;
+    int has_media;   /* some object has an interior medium (bSceneHasObjectMedia) */
     /* work distribution */
     int32_t s0, s1;
     double* xyz_sum;
@@ -342,6 +343,9 @@ static int prepare(prep* P, const ro_scene* s)
     const double ky = rise_oracle_cie_y_integral(s->lambda_begin, s->lambda_end);
     P->y_norm = ky > NZ ? (s->lambda_end - s->lambda_begin) / ky : 1.0;
     sky_init(&P->sky, s);
+    /* LightSampler::Prepare's MediaScan (LightSampler.cpp:767-785). */
+    if (s->object_medium)
+        for (int o = 0; o < s->n_objects; ++o) P->has_media |= s->object_medium[o] >= 0;
     return 0;
 }
 
@@ -379,9 +383,11 @@ static int64_t closest(const prep* P, v3 o, v3 dn, double* t_out)
 }
 
 /* Any hit in (NEARZERO, dHowFar) after the Object layer's normalization
- * (Object.cpp:652-712). */
+ * (Object.cpp:652-712), over the objects that cast shadows
+ * (ObjectManager.cpp:119-124). */
 static int occluded(const prep* P, v3 o, v3 d, double how_far)
 {
+    const uint8_t* casts = P->s->object_casts_shadows;
     const v3 dn = normalize(d);
     const double m = mag(d);
     const double factor = (m > NZ) ? m : 1.0;
@@ -393,6 +399,7 @@ static int occluded(const prep* P, v3 o, v3 d, double how_far)
         const double eA[3] = {f[3] - f[0], f[4] - f[1], f[5] - f[2]};
         const double eB[3] = {f[6] - f[0], f[7] - f[1], f[8] - f[2]};
         double t;
+        if (casts && !casts[P->s->tri_object[k]]) continue;
         if (rise_oracle_ray_triangle(oo, dd, f, eA, eB, &t, NULL, NULL) && t > NZ && t < how_far2) return 1;
     }
     return 0;
@@ -697,6 +704,142 @@ static int dielectric_scatter(const prep* P, const hit* h, double nm, const iors
     return 1;
 }
 
+/* ---- participating media ------------------------------------------------ */
+
+#define MAX_VOLUME_BOUNCE 64 /* StabilityConfig maxVolumeBounce */
+
+typedef struct {
+    double sigma_t, sigma_s;
+} medium_coeff;
+
+/* ColorMath::Luminance (ColorMath.h:124-129). */
+static double luminance3(const double c[3]) { return 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]; }
+
+/* HomogeneousMedium::GetCoefficientsNM without spectral functions
+ * (HomogeneousMedium.cpp:95-115): the same at every wavelength. */
+static medium_coeff medium_nm(const ro_medium* m)
+{
+    const double t[3] = {m->sigma_a[0] + m->sigma_s[0], m->sigma_a[1] + m->sigma_s[1], m->sigma_a[2] + m->sigma_s[2]};
+    medium_coeff c;
+    c.sigma_t = luminance3(t);
+    c.sigma_s = luminance3(m->sigma_s);
+    return c;
+}
+
+/* MediumTracking::GetCurrentMediumWithObject (MediumTracking.h): the top
+ * object of the IOR stack if it has an interior medium (no global medium). */
+static int32_t medium_object(const prep* P, const iorstack* st)
+{
+    const int32_t top = st->obj[st->n - 1];
+    if (top < 0 || !P->s->object_medium || P->s->object_medium[top] < 0) return -1;
+    return top;
+}
+
+double rise_oracle_hg_eval(double cos_theta, double g)
+{
+    /* HenyeyGreensteinPhaseFunction::EvaluateWithG (:20-33). */
+    const double g2 = g * g;
+    const double denom = 1.0 + g2 - 2.0 * g * cos_theta;
+    if (denom <= 0.0) return 0.0;
+    return (1.0 - g2) / (4.0 * M_PI * denom * sqrt(denom));
+}
+
+/* Vector3Ops::Perpendicular (VectorsOps.h:164-188). */
+static v3 perpendicular(v3 v)
+{
+    const double x = fabs(v.x), y = fabs(v.y), z = fabs(v.z);
+    if (x < y && x < z) return mk(0.0, v.z, -v.y);
+    if (y < z) return mk(v.z, 0.0, -v.x);
+    return mk(v.y, -v.x, 0.0);
+}
+
+/* GeometricUtilities::Perturb (GeometricUtilities.cpp:100-118): vec in the
+ * basis U = vec, V = perpendicular, W = U x V (OrthonormalBasis3D::
+ * CreateFromU), rotated by `down` about the basis Y axis, then by `around`
+ * about its X axis (MatricesOps.h:274-303, row-vector transforms), and back. */
+static v3 perturb(v3 vec, double down, double around)
+{
+    const v3 U = normalize(vec), V = normalize(perpendicular(vec)), W = cross(U, V);
+    const double a0 = dot(U, vec), a1 = dot(V, vec), a2 = dot(W, vec);
+    const double cd = cos(down), sd = sin(down), ca = cos(around), sa = sin(around);
+    const double r0 = cd * a0 + sd * a2, r1 = a1, r2 = -sd * a0 + cd * a2;
+    const double q0 = r0, q1 = ca * r1 - sa * r2, q2 = sa * r1 + ca * r2;
+    return add(add(mul(U, q0), mul(V, q1)), mul(W, q2));
+}
+
+void rise_oracle_hg_sample(const double wo[3], double g, double u1, double u2, double out[3])
+{
+    /* HenyeyGreensteinPhaseFunction::SampleWithG (:35-56). */
+    double cos_theta;
+    if (fabs(g) < 1e-10) {
+        cos_theta = 1.0 - 2.0 * u1;
+    } else {
+        const double inner = (1.0 - g * g) / (1.0 - g + 2.0 * g * u1);
+        cos_theta = (1.0 / (2.0 * g)) * (1.0 + g * g - inner * inner);
+    }
+    cos_theta = fmax(-1.0, fmin(1.0, cos_theta));
+    const v3 d = normalize(perturb(mk(wo[0], wo[1], wo[2]), acos(cos_theta), 2.0 * M_PI * u2));
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
+
+/* EvalShadowTransmittanceNM (LightSampler.cpp:369-470): walks the shadow
+ * ray boundary by boundary (closest hits over every object from 1e-5 past
+ * the last boundary, at most 16), keeping a stack of the media entered
+ * (front face by the geometric normal: push; back face: remove) and
+ * multiplying the innermost medium's transmittance over each segment. */
+static double shadow_tr(const prep* P, v3 o, v3 d, double max_dist, int32_t origin_obj)
+{
+    const ro_scene* s = P->s;
+    double tr = 1.0;
+    if (max_dist <= 0 || !P->has_media) return tr;
+    int32_t stk[4];
+    int cnt = 0;
+    if (origin_obj >= 0) stk[cnt++] = origin_obj;
+    double seg = 0.0;
+#define TOP_SIGMA_T (medium_nm(&s->media[s->object_medium[stk[cnt - 1]]]).sigma_t)
+    for (int step = 0; step < 16 && seg < max_dist; ++step) {
+        const double cs = seg + 1e-5;
+        if (cs >= max_dist) break;
+        const v3 co = add(o, mul(d, cs));
+        const double cmax = max_dist - cs;
+        double t;
+        const int64_t tri = closest(P, co, d, &t);
+        if (tri < 0 || t >= cmax) {
+            const double rem = max_dist - seg;
+            if (rem > 0 && cnt > 0) tr *= exp(-TOP_SIGMA_T * rem);
+            seg = max_dist;
+            break;
+        }
+        hit h;
+        make_hit(P, co, d, tri, t, &h);
+        const double bd = cs + t;
+        const double len = bd - seg;
+        if (len > 0 && cnt > 0) tr *= exp(-TOP_SIGMA_T * len);
+        if (s->object_medium[h.object] >= 0) {
+            if (dot(h.ng, d) < 0) {
+                if (cnt < 4) stk[cnt++] = h.object;
+            } else {
+                for (int i = 0; i < cnt; ++i)
+                    if (stk[i] == h.object) {
+                        for (int j = i; j < cnt - 1; ++j) stk[j] = stk[j + 1];
+                        --cnt;
+                        break;
+                    }
+            }
+        }
+        seg = bd;
+        if (tr < 1e-6) return 0.0;
+    }
+    if (seg < max_dist) {
+        const double rem = max_dist - seg;
+        if (rem > 0 && cnt > 0) tr *= exp(-TOP_SIGMA_T * rem);
+    }
+#undef TOP_SIGMA_T
+    return tr;
+}
+
 /* Luminary selection by CDF inversion over the weights; pdf = w / total. */
 static int select_luminary(const prep* P, double u, double* pdf)
 {
@@ -717,9 +860,10 @@ static double lum_pdf_select(const prep* P, int lum)
     return P->lums[lum].weight / P->lum_total;
 }
 
-/* EvaluateDirectLightingNM for area luminaries (LightSampler.cpp:1987-2269). */
-static double direct_nm(prep* P, const hit* h, double nm, double amb, uint64_t seed, uint32_t path, uint32_t dim0,
-                        uint64_t* n_shadow)
+/* EvaluateDirectLightingNM for area luminaries (LightSampler.cpp:1987-2269);
+ * mobj: the object whose medium holds the shading point (-1: none). */
+static double direct_nm(prep* P, const hit* h, double nm, double amb, int32_t mobj, uint64_t seed, uint32_t path,
+                        uint32_t dim0, uint64_t* n_shadow)
 {
     if (P->n_lums == 0) return 0.0;
     const int self = P->lum_of_object[h->object];
@@ -753,10 +897,55 @@ static double direct_nm(prep* P, const hit* h, double nm, double amb, uint64_t s
     const double le = emitted(P, lm, neg(to), ln, nm);
     const double geom = L->area * cos_l / (dist * dist);
     double contrib = le * cos_s * geom * bsdf_value(P, h, to, nm, amb);
+    contrib *= shadow_tr(P, h->p, to, dist, mobj);
     if (L->area > 0 && cos_l > 0) {
         const double p_light = pdf_alias * (dist * dist) / (L->area * cos_l);
         const double p_bsdf = bsdf_pdf(P, h, to, nm, amb);
         if (p_bsdf > 0) contrib *= power_heuristic(p_light, p_bsdf);
+    }
+    return contrib / pdf_alias;
+}
+
+/* MediumTransport::EvaluateInScatteringNM (MediumTransport.cpp:139-174) ->
+ * EvaluateDirectLightingNM with isVolumeScatter: cosSurface = 1, the phase
+ * function as BSDF and as MIS pdf, no shading object -- hence no self
+ * exclusion and no occlusion test (LightSampler.cpp:2200-2211 tests only for
+ * a shading object) -- and the transmittance walk from inside mobj's medium. */
+static double direct_volume(prep* P, v3 p, v3 wo, double g, int32_t mobj, double nm, uint64_t seed, uint32_t path,
+                            uint32_t dim0, uint64_t* n_shadow)
+{
+    if (P->n_lums == 0) return 0.0;
+    double pdf_alias;
+    const int li = select_luminary(P, rise_oracle_rng(seed, path, dim0), &pdf_alias);
+    if (li < 0) return 0.0;
+    const luminary* L = &P->lums[li];
+    const double ux = rise_oracle_rng(seed, path, dim0 + 1);
+    const double uy = rise_oracle_rng(seed, path, dim0 + 2);
+    const double uz = rise_oracle_rng(seed, path, dim0 + 3);
+    int64_t k = 0;
+    while (k < L->n - 1 && L->cdf[k] < uz) ++k;
+    v3 a, b, c;
+    tri_verts(P, L->tris[k], &a, &b, &c);
+    const double sa = sqrt(1.0 - ux);
+    const double alpha = 1.0 - sa, beta = sa * uy, gamma = 1.0 - alpha - beta;
+    const v3 pl = mk(alpha * b.x + beta * c.x + gamma * a.x, alpha * b.y + beta * c.y + gamma * a.y,
+                     alpha * b.z + beta * c.z + gamma * a.z);
+    const v3 ln = normalize(cross(sub(b, a), sub(c, a)));
+    v3 to = sub(pl, p);
+    const double dist = mag(to);
+    if (dist > 0.0) to = mul(to, 1.0 / dist);
+    const double cos_l = dot(neg(to), ln);
+    if (cos_l <= 0) return 0.0;
+    ++*n_shadow;
+    const ro_material* lm = &P->s->materials[P->s->object_material[L->object]];
+    const double le = emitted(P, lm, neg(to), ln, nm);
+    const double geom = L->area * cos_l / (dist * dist);
+    const double phase = rise_oracle_hg_eval(dot(to, wo), g);
+    double contrib = le * 1.0 * geom * phase;
+    contrib *= shadow_tr(P, p, to, dist, mobj);
+    if (L->area > 0 && cos_l > 0) {
+        const double p_light = pdf_alias * (dist * dist) / (L->area * cos_l);
+        if (phase > 0) contrib *= power_heuristic(p_light, phase);
     }
     return contrib / pdf_alias;
 }
@@ -801,6 +990,7 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
                       int diffuse, int glossy, int transmission, uint64_t seed, uint32_t path, uint64_t* n_ext,
                       uint64_t* n_shadow)
 {
+    int vb = 0; /* volume bounces (none before the branch: the shared loop has no medium) */
     const ro_scene* s = P->s;
     double result = 0.0, thr = 1.0;
     iorstack st;
@@ -814,6 +1004,64 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
             double t;
             ++*n_ext;
             const int64_t tri = closest(P, org, dir, &t);
+            /* Medium transport (:1686-1925): free flight at this wavelength
+             * (SampleDistanceWithEquiangularMIS_NM without positional lights,
+             * :364-386 -> HomogeneousMedium::SampleDistanceNM, :150-172). */
+            const int32_t mobj = medium_object(P, &st);
+            if (mobj >= 0) {
+                const ro_medium* M = &s->media[s->object_medium[mobj]];
+                const medium_coeff mc = medium_nm(M);
+                const double max_dist = tri >= 0 ? t : RISE_ORACLE_INFINITY;
+                int scattered = 0;
+                double tm = max_dist;
+                if (mc.sigma_t > 0.0) {
+                    const double xi = rise_oracle_rng(seed, path, base + 20);
+                    const double tt = -log(fmax(1.0 - xi, 1e-30)) / mc.sigma_t;
+                    if (tt < max_dist) {
+                        scattered = 1;
+                        tm = tt;
+                    }
+                }
+                if (scattered && vb < MAX_VOLUME_BOUNCE) {
+                    /* Volume scatter: weight Tr sigma_s / (sigma_t Tr), in-scattering
+                     * NEE, phase sampling, Russian roulette at depth + volume
+                     * bounces; the ray goes on from the scatter point. */
+                    const v3 sp = add(org, mul(dir, tm));
+                    const double tr = exp(-mc.sigma_t * tm);
+                    double mw = 0.0;
+                    if (mc.sigma_t > 0 && tr > 0) mw = tr * mc.sigma_s / (mc.sigma_t * tr);
+                    if (!(mw > 0)) break;
+                    thr *= mw;
+                    const double ld = direct_volume(P, sp, dir, M->g, mobj, nm, seed, path, base, n_shadow);
+                    if (ld > 0) result += thr * ld;
+                    const double wo[3] = {dir.x, dir.y, dir.z};
+                    double wi[3];
+                    rise_oracle_hg_sample(wo, M->g, rise_oracle_rng(seed, path, base + 16),
+                                          rise_oracle_rng(seed, path, base + 17), wi);
+                    const v3 nd = mk(wi[0], wi[1], wi[2]);
+                    const double pp = rise_oracle_hg_eval(dot(dir, nd), M->g);
+                    if (pp <= NZ) break;
+                    thr *= rise_oracle_hg_eval(dot(dir, nd), M->g) / pp;
+                    if (depth + vb >= s->rr_min_depth) {
+                        const double prev = importance > s->rr_threshold ? importance : s->rr_threshold;
+                        double p = fabs(thr) / prev;
+                        if (p > 1.0) p = 1.0;
+                        if (rise_oracle_rng(seed, path, base + 19) >= p) break;
+                        if (p > 0 && p < 1.0) thr /= p;
+                    }
+                    org = sp;
+                    dir = nd;
+                    bsdf_pdf = pp;
+                    ++vb;
+                    continue;
+                } else if (!scattered) {
+                    /* Survival: Tr / pSurvival, the no-scatter pdf being the
+                     * transmittance itself floored at 1e-30 (IMedium.h:235-251). */
+                    const double tr = exp(-mc.sigma_t * max_dist);
+                    const double ps = fmax(exp(-mc.sigma_t * max_dist), 1e-30);
+                    thr *= tr / ps;
+                }
+            }
             if (tri < 0) {
                 /* Escape: the radiance map at this wavelength, full weight (no
                  * environment light sample to weigh against). */
@@ -840,7 +1088,7 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
             dir = nd;
             continue;
         }
-        result += thr * direct_nm(P, &h, nm, amb, seed, path, base, n_shadow);
+        result += thr * direct_nm(P, &h, nm, amb, medium_object(P, &st), seed, path, base, n_shadow);
         const rb_sample sc = bsdf_scatter(P, &h, nm, amb, rise_oracle_rng(seed, path, base + 16),
                                           rise_oracle_rng(seed, path, base + 17), rise_oracle_rng(seed, path, base + 18));
         if (!sc.ok) break;
@@ -935,7 +1183,7 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         /* Next-event estimation, one light sample per wavelength
          * (PathTracingIntegrator.cpp:4521-4552). */
         for (int w = 0; w < HWSS_N; ++w)
-            result[w] += thr[w] * direct_nm(P, &h, lam[w], 1.0, seed, path, base + 4u * (uint32_t)w, &n_shadow);
+            result[w] += thr[w] * direct_nm(P, &h, lam[w], 1.0, -1, seed, path, base + 4u * (uint32_t)w, &n_shadow);
 
         /* Hero ScatterNM, then companions at the hero's direction (:4620-4718). */
         const rb_sample hs = bsdf_scatter(P, &h, lam[0], 1.0, rise_oracle_rng(seed, path, base + 16),
@@ -1035,6 +1283,8 @@ int rise_oracle_render(const ro_scene* s, int32_t s0, int32_t s1, double* xyz_su
             m->spec_spd >= s->n_spds || m->ior_spd >= s->n_spds || m->ext_spd >= s->n_spds)
             return -1;
         if (m->type == 0 && m->reflect_spd < 0) return -1;
+        if (s->object_medium && (s->object_medium[o] >= s->n_media || (s->object_medium[o] >= 0 && !s->media)))
+            return -1;
     }
     prep P;
     prepare(&P, s);
@@ -1059,4 +1309,15 @@ int rise_oracle_render(const ro_scene* s, int32_t s0, int32_t s1, double* xyz_su
     pthread_mutex_destroy(&P.lock);
     release(&P);
     return 0;
+}
+
+double rise_oracle_shadow_transmittance(const ro_scene* scene, const double o[3], const double d[3], double max_dist,
+                                        int32_t origin_object, double nm)
+{
+    (void)nm; /* RGB media have the same coefficients at every wavelength */
+    prep P;
+    prepare(&P, scene);
+    const double tr = shadow_tr(&P, mk(o[0], o[1], o[2]), mk(d[0], d[1], d[2]), max_dist, origin_object);
+    release(&P);
+    return tr;
 }

@@ -25,6 +25,11 @@
  *   Objects/Object.cpp:641-643                                      hit point back-off
  *   Utilities/HosekWilkieSkyModel.cpp:20-301, Rendering/HosekWilkieSpectralRadianceMap.cpp
  *                                                                   sky radiance on escape
+ *   Materials/HomogeneousMedium.cpp:95-240                          coefficients, free flight, Tr
+ *   Materials/HenyeyGreensteinPhaseFunction.cpp:20-110              phase function (GeometricUtilities.cpp:100-118 Perturb)
+ *   Shaders/PathTracingIntegrator.cpp:364-530, 1686-1925            medium transport in the NM loop
+ *   Utilities/MediumTracking.h, Utilities/MediumTransport.cpp:139-174  current medium, in-scattering NEE
+ *   Lights/LightSampler.cpp:140-470                                 shadow-ray transmittance walk
  * Random numbers: the reference draws per-dimension samples from an
  * Owen-scrambled Sobol sequence (stream 16 + depth, 32 dimensions per
  * stream) and hero wavelength / pixel jitter from a Mersenne Twister; it is
@@ -64,6 +69,15 @@ typedef struct ro_material {
     double alpha;          /* GGX roughness (isotropic) */
 } ro_material;
 
+/* HomogeneousMedium with RGB coefficients and a Henyey-Greenstein phase
+ * function.  Its single-wavelength coefficients are the luminances of the
+ * RGB triples at every wavelength (GetCoefficientsNM, HomogeneousMedium.cpp:
+ * 95-115): sigma_t = Y(sigma_a + sigma_s), sigma_s = Y(sigma_s). */
+typedef struct ro_medium {
+    double sigma_a[3], sigma_s[3];
+    double g;  /* HG asymmetry */
+} ro_medium;
+
 typedef struct ro_scene {
     const float* tris;          /* n_tris * 9 */
     int64_t n_tris;
@@ -88,6 +102,13 @@ typedef struct ro_scene {
     int32_t sky_enabled;
     int32_t sky_background;     /* camera rays that miss see the sky */
     double sky_elevation_deg, sky_azimuth_deg, sky_turbidity, sky_scale;
+    /* Participating media inside objects (standard_object interior_medium),
+     * entered through a dielectric boundary; objects that do not cast
+     * shadows (casts_shadows FALSE) are skipped by shadow rays.  NULL: none. */
+    const int32_t* object_medium;        /* medium index per object, -1: none */
+    const uint8_t* object_casts_shadows; /* NULL: every object casts shadows */
+    const ro_medium* media;
+    int32_t n_media;
 } ro_scene;
 
 /* Counter-based uniform in [0,1) with 24 random bits (exactly representable
@@ -106,6 +127,17 @@ int rise_oracle_xyz_from_nm(double nm, double xyz[3]);
 
 /* Sky radiance at wavelength nm in direction dir (unit), scale included. */
 double rise_oracle_sky_radiance(const ro_scene* scene, const double dir[3], double nm);
+
+/* HenyeyGreensteinPhaseFunction: Evaluate at cos(theta) and Sample around
+ * wo from two uniforms (out: unit direction). */
+double rise_oracle_hg_eval(double cos_theta, double g);
+void rise_oracle_hg_sample(const double wo[3], double g, double u1, double u2, double out[3]);
+
+/* Transmittance along a shadow ray at wavelength nm from o (unit dir d) over
+ * maxDist, starting inside the medium of object origin_object (-1: none):
+ * the boundary walk of EvalShadowTransmittanceNM. */
+double rise_oracle_shadow_transmittance(const ro_scene* scene, const double o[3], const double d[3], double max_dist,
+                                        int32_t origin_object, double nm);
 
 /* CIE_Y_Integral (ColorUtils.cpp:327-350). */
 double rise_oracle_cie_y_integral(double a, double b);

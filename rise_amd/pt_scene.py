@@ -64,9 +64,27 @@ class Material:
 
 @dataclass
 class Object:
+    """A triangle-mesh object (``standard_object``): `medium` indexes
+    Scene.media for its interior (``interior_medium``, -1: none; entered
+    through a dielectric boundary); `casts_shadows` False lets shadow rays
+    through it (``casts_shadows FALSE``), the usual setting for a medium's
+    shell.  Medium shells must be single-sided and closed, with outward
+    normals: shadow rays enter and leave media by the face orientation."""
     tris: np.ndarray            # (n, 9) float32
     material: int
     double_sided: bool = False
+    medium: int = -1
+    casts_shadows: bool = True
+
+
+@dataclass
+class Medium:
+    """``homogeneous_medium`` with an ``hg`` phase function: RGB absorption
+    and scattering coefficients (per unit length; the single-wavelength
+    coefficients are their luminances) and the Henyey-Greenstein asymmetry g."""
+    sigma_a: tuple = (0.05, 0.05, 0.05)
+    sigma_s: tuple = (0.5, 0.5, 0.5)
+    g: float = 0.3
 
 
 @dataclass
@@ -111,6 +129,7 @@ class Scene:
     objects: list = field(default_factory=list)
     camera: Camera | None = None
     sky: Sky | None = None
+    media: list = field(default_factory=list)
 
     def flat(self):
         """All triangles (n, 9) float32 and their object index (n,) int32."""
@@ -322,6 +341,46 @@ def furnace_with(kind: int, radiance: float = 1.0, alpha: float = 0.3, ior: floa
     s.camera = Camera(position=(0.0, 0.0, 1.8), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=20.0)
     r = RenderSettings(width=12, height=12, spp=64, max_diffuse_bounce=max_bounce, max_glossy_bounce=max_bounce,
                        max_transmission_bounce=max_bounce)
+    return s, r
+
+
+def medium_shell(lo, hi, medium: int, material: int) -> Object:
+    """A closed box shell (outward normals) holding medium `medium`, with a
+    dielectric boundary `material` and no shadows of its own."""
+    return Object(box(lo, hi), material, medium=medium, casts_shadows=False)
+
+
+def furnace_medium(sigma_a=(0.0, 0.0, 0.0), sigma_s=(0.8, 0.8, 0.8), g: float = 0.3, half: float = 0.5,
+                   max_bounce: int = 64):
+    """A closed emitting box (radiance 1, albedo 0, every face its own
+    luminary) around a cube of half-size `half` at the origin filled with a
+    homogeneous medium behind an index-matched (index 1) dielectric shell.
+    A purely scattering medium leaves the uniform radiance field unchanged
+    (radiance 1 everywhere); a purely absorbing one shows exp(-sigma_t * chord)."""
+    s = Scene()
+    s.spds = [Spd.flat(0.0), Spd.flat(1.0)]
+    s.materials = [Material(0, emission=1, emission_scale=math.pi), Material(kind=DIELECTRIC, ior=1)]
+    tris = box((-2, -2, -2), (2, 2, 2))
+    tris = np.ascontiguousarray(tris.reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9))
+    s.objects = [Object(tris[2 * f: 2 * f + 2].copy(), 0) for f in range(6)]
+    s.media = [Medium(tuple(sigma_a), tuple(sigma_s), g)]
+    s.objects.append(medium_shell((-half, -half, -half), (half, half, half), 0, 1))
+    s.camera = Camera(position=(0.0, 0.0, 1.8), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=30.0)
+    r = RenderSettings(width=12, height=12, spp=64, max_diffuse_bounce=max_bounce, max_glossy_bounce=max_bounce,
+                       max_transmission_bounce=max_bounce)
+    return s, r
+
+
+def cornell_fog(sigma_a=(0.05, 0.05, 0.05), sigma_s=(0.5, 0.5, 0.5), g: float = 0.3):
+    """The Cornell box with a floating box of homogeneous medium (sigma_s 0.5,
+    sigma_a 0.05, g 0.3 by default) behind an index-matched dielectric shell
+    that casts no shadow, above the short block: x in [0, 0.9], y in [0.7, 1.7],
+    z in [-0.5, 0.6]."""
+    s, r = cornell_box()
+    s.spds.append(Spd.flat(1.0))
+    s.materials.append(Material(kind=DIELECTRIC, ior=len(s.spds) - 1))
+    s.media = [Medium(tuple(sigma_a), tuple(sigma_s), g)]
+    s.objects.insert(5, medium_shell((0.0, 0.7, -0.5), (0.9, 1.7, 0.6), 0, len(s.materials) - 1))
     return s, r
 
 
