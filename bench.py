@@ -244,7 +244,8 @@ def serving_leg(args):
     torch.cuda.synchronize()
     seq_s = time.perf_counter() - w0
     out = {
-        "workload": "Poisson 200 req/s, 64 requests over 16 resident scenes (50k-2M tris; Lambertian/GGX/glass), "
+        "workload": "Poisson 200 req/s, 64 requests over 16 resident scenes (50k-2M tris; Lambertian/GGX/glass; "
+        "half with a ground-fog box, homogeneous sigma_s 0.5 sigma_a 0.05 HG g 0.3), "
         "res {128,256,512,1024}^2, spp {1,4,16,64}, depth 6",
         "requests": len(reqs),
         "paths": int(paths),

@@ -72,6 +72,38 @@ rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* objects, si
                                          const rise_hip_spd_desc* spds, size_t n_spds,
                                          rise_hip_pt_scene** out_scene);
 
+/* Participating media (row "media" of the scene format): the scene-file
+ * chunk `homogeneous_medium` with `phase hg g` (Job::AddHomogeneousMedium;
+ * Materials/HomogeneousMedium.cpp, Materials/HenyeyGreensteinPhaseFunction.cpp).
+ * RGB coefficients per unit length; the spectral renderer uses their
+ * luminances at every wavelength (HomogeneousMedium::GetCoefficientsNM). */
+typedef struct rise_hip_medium_desc {
+    float sigma_a[3];  /* absorption */
+    float sigma_s[3];  /* scattering */
+    float hg_g;        /* Henyey-Greenstein asymmetry, in (-1, 1) */
+    int32_t reserved;  /* 0 */
+} rise_hip_medium_desc;
+
+/* Per object: `standard_object`'s `interior_medium` (index into the media,
+ * -1: none) and `casts_shadows` (0: shadow rays pass through it, the usual
+ * setting for a medium's shell).  A medium is entered through a dielectric
+ * boundary (the index-of-refraction stack, Utilities/MediumTracking.h); its
+ * shell must be a closed single-sided mesh with outward normals, since shadow
+ * rays enter and leave media by face orientation (LightSampler.cpp:300-330). */
+typedef struct rise_hip_object_media {
+    int32_t interior_medium;
+    int32_t casts_shadows;
+} rise_hip_object_media;
+
+/* rise_hip_pt_scene_create with participating media: `object_media` has one
+ * entry per object (NULL: no media, every object casts shadows). */
+rise_hip_status rise_hip_pt_scene_create_with_media(const rise_hip_object_desc* objects, size_t n_objects,
+                                                    const rise_hip_material_desc* materials, size_t n_materials,
+                                                    const rise_hip_spd_desc* spds, size_t n_spds,
+                                                    const rise_hip_medium_desc* media, size_t n_media,
+                                                    const rise_hip_object_media* object_media,
+                                                    rise_hip_pt_scene** out_scene);
+
 rise_hip_status rise_hip_pt_scene_release(rise_hip_pt_scene* scene);
 
 /* The spectral sky radiance map of the scene-file chunk `hosek_wilkie_skylight`

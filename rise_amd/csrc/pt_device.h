@@ -41,8 +41,21 @@ struct DevObject {
     int material;
     int double_sided;
     int luminary;   // -1: not a luminary
+    int medium;     // interior medium, -1: none
+};
+
+// HomogeneousMedium with RGB coefficients and a Henyey-Greenstein phase
+// function, at one wavelength: the luminances of sigma_a + sigma_s and of
+// sigma_s (GetCoefficientsNM, Materials/HomogeneousMedium.cpp:95-115), the same
+// at every wavelength.
+struct DevMedium {
+    float sigma_t, sigma_s, g;
     int pad;
 };
+
+// Tri48 v2.w bit 0: the triangle's object casts no shadow (skipped by the
+// any-hit query of shadow rays, ObjectManager.cpp:119-124).
+constexpr unsigned kTriNoShadow = 1u;
 
 struct DevLuminary {
     int object;
@@ -80,6 +93,13 @@ struct PtSceneView {
     const unsigned* lum_tris;   // leaf-order triangle indices
     const float* lum_cdf;       // area CDF per luminary
     DevSky sky;
+    // Participating media: table, and a BVH over the shells of the objects
+    // holding one (v0.w = index, v1.w = object id) for the shadow-ray
+    // transmittance walk.  has_media = 0: none.
+    const DevMedium* media;
+    const NodeSlot* mnodes;
+    const Tri48* mtris;
+    int has_media;
 };
 
 // ---- random numbers: counter-based hash ------------------------------------

@@ -16,6 +16,7 @@ struct rise_hip_pt_scene {
     bool has_glossy = false;        // a material scatters glossily (counts against max_glossy_bounce)
     bool has_transmission = false;  // a material transmits (counts against max_transmission_bounce)
     unsigned material_kinds = 0;    // bit k: a material of kind k (rise_hip_material_type) is present
+    bool has_media = false;         // some object holds a participating medium
     rise_hip_batch* solo = nullptr;  // cached batch for rise_hip_render, rebuilt when the request changes
 };
 

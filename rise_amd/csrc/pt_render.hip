@@ -60,7 +60,10 @@ constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 256;
 constexpr int64_t kChunkPaths = int64_t(1) << 20;   // paths per chunk (at most)
 constexpr int64_t kMaxPaths = int64_t(1) << 22;     // paths in flight per pass (at most)
-constexpr int kCounters = 5 * (kMaxPathDepth + 1);  // extension, shadow, then 3 binned-hit counts, per pass
+constexpr int kCounters = 6 * (kMaxPathDepth + 1);  // extension, shadow, then 4 binned-hit counts, per pass
+constexpr int kBins = 4;                            // hit bins: the 3 material kinds, then volume scatters
+constexpr int kMatVolume = 3;                       // bin of the lanes that scatter inside a medium
+constexpr int kMaxVolumeBounce = 64;                // StabilityConfig maxVolumeBounce (StabilityConfig.h:112)
 
 // ColorUtils::CIE_Y_Integral (Utilities/Color/ColorUtils.cpp:327-350).
 double cie_y_integral(double a, double b)
@@ -142,7 +145,8 @@ struct PassEntry {
 struct Paths {
     int cap;
     float *ox, *oy, *oz, *dx, *dy, *dz;  // 4 * cap
-    float* thr;      // 4 * cap, per wavelength
+    float* thr;      // 4 * cap, per wavelength; a single-wavelength lane's is relative to its branch
+    float* thr0;     // 4 * cap: a single-wavelength lane's throughput at the dielectric hit it branched at
     float* res;      // 4 * cap, per wavelength
     float* lambda0;  // cap
     float* pdf;      // 4 * cap: bsdfPdf of the ray being traced (0: camera or delta)
@@ -155,13 +159,17 @@ struct Paths {
     unsigned* hit_leaf;  // 4 * cap
     float* hit_t;        // 4 * cap
     unsigned* queue[2];  // 4 * cap entries, slot * 4 + lane
-    unsigned* mqueue[3]; // hits binned by material kind, 4 * cap entries each
-    // shadow records, 4 * cap
+    unsigned* mqueue[kBins]; // hits binned by material kind (and volume scatters), 4 * cap entries each
+    // shadow records, 4 * cap: origin, unit direction, distance to the light point, value
     float *sox, *soy, *soz, *sdx, *sdy, *sdz, *stmax, *sval;
     unsigned* sslot;     // slot * 4 + wavelength
+    // Media only (null when no scene of the pass holds a medium):
+    unsigned char* vb;   // 4 * cap: volume bounces of the lane
+    unsigned* sinfo;     // per shadow record: kShadowInScatter | (object whose medium holds the origin) + 1
+    unsigned* svis;      // per shadow record: 1 once found unoccluded, awaiting the transmittance walk
     unsigned* cnt_ext;   // [kMaxPathDepth + 1] of the current pass
     unsigned* cnt_sh;    // [kMaxPathDepth + 1]
-    unsigned* cnt_mat;   // [3][kMaxPathDepth + 1]: binned hits per material kind
+    unsigned* cnt_mat;   // [kBins][kMaxPathDepth + 1]: binned hits per material kind, then volume scatters
     unsigned long long* fetch;  // [2 * (kMaxPathDepth + 1)]: work counters of the persistent traversal kernels
 };
 
@@ -243,6 +251,7 @@ __global__ void k_generate(const ReqInfo* __restrict__ reqs, const PassEntry* __
         S.flags[i] = 0u;
         S.path_id[i] = path;
         S.req[i] = (unsigned)E.req;
+        if (S.vb) S.vb[i] = 0;
         S.queue[0][i] = (unsigned)i * 4u;
     }
 }
@@ -525,14 +534,36 @@ __device__ inline bool bump_bounces(unsigned& st, int type, const ReqInfo& R)
 constexpr unsigned kLaneDims = 1u << 20;  // dimension offset of single-wavelength lane w: (w + 1) << 20
 constexpr unsigned kModeNm = 1u;          // flags bit: the path continues as single-wavelength lanes
 
+constexpr unsigned kShadowInScatter = 0x80000000u;  // shadow record of an in-scattering NEE sample
+
+// The object on top of a lane's IOR stack when it holds a medium, else -1
+// (MediumTracking::GetCurrentMediumWithObject, Utilities/MediumTracking.h;
+// no global medium).
+__device__ inline int lane_medium_object(const PtSceneView& sc, const Paths& S, unsigned li)
+{
+    if (!sc.has_media) return -1;
+    const int n = (int)((S.state[li] >> 24) & 3u);
+    if (n == 0) return -1;
+    const int top = S.ior_obj[(n - 1) * 4 * S.cap + li];
+    return sc.objects[top].medium >= 0 ? top : -1;
+}
+
 // Bins the extension queue's hits by material kind (misses end their path),
-// so each shading kernel runs one material's code over a dense queue.
+// so each shading kernel runs one material's code over a dense queue.  A
+// single-wavelength lane inside a medium first samples a free-flight
+// distance (PathTracingIntegrator.cpp:1686-1712 ->
+// SampleDistanceWithEquiangularMIS_NM without positional lights, :364-386,
+// -> HomogeneousMedium::SampleDistanceNM, HomogeneousMedium.cpp:150-172): a
+// scatter before the hit goes to the volume bin with its distance in hit_t;
+// otherwise the lane takes the survival weight Tr / pSurvival (:1895-1910,
+// IMedium.h:235-251) and goes on to the surface or the escape.
 __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
     const unsigned n = S.cnt_ext[depth];
     const unsigned* q = S.queue[depth & 1];
     unsigned* cnt = S.cnt_mat + depth;
     const unsigned stride = gridDim.x * blockDim.x;
+    const int bins = S.vb ? kBins : 3;
     for (unsigned b = blockIdx.x * blockDim.x; b < n; b += stride) {
         const unsigned i = b + threadIdx.x;
         int kind = -1;
@@ -543,7 +574,35 @@ __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
             const unsigned leaf = S.hit_leaf[li];
             const ReqInfo& R = reqs[S.req[slot]];
             const PtSceneView& sc = R.sc;
-            if (leaf != 0xffffffffu) {
+            const bool nm = S.flags[slot] & kModeNm;
+            bool volume = false;
+            const int mobj = (nm && S.vb) ? lane_medium_object(sc, S, li) : -1;
+            if (mobj >= 0) {
+                const DevMedium M = sc.media[sc.objects[mobj].medium];
+                const bool hit = leaf != 0xffffffffu;
+                const float maxd = hit ? S.hit_t[li] : INFINITY;
+                bool scattered = false;
+                float tm = 0.0f;
+                if (M.sigma_t > 0.0f) {
+                    const unsigned dim = 32u * (unsigned)(16 + depth) + kLaneDims * (lane + 1u) + 20u;
+                    const float xi = rng_u(R.seed, S.path_id[slot], dim);
+                    const float tt = -logf(fmaxf(1.0f - xi, 1e-30f)) / M.sigma_t;
+                    if (tt < maxd) {
+                        scattered = true;
+                        tm = tt;
+                    }
+                }
+                if (scattered && (int)S.vb[li] < kMaxVolumeBounce) {
+                    volume = true;
+                    S.hit_t[li] = tm;
+                } else if (!scattered && M.sigma_t > 0.0f) {  // sigma_t = 0: Tr = pSurvival = 1
+                    const float tr = expf(-M.sigma_t * maxd);
+                    S.thr[li] *= tr / fmaxf(tr, 1e-30f);
+                }
+            }
+            if (volume) {
+                kind = kMatVolume;
+            } else if (leaf != 0xffffffffu) {
                 const int obj = float_to_bits(reinterpret_cast<const float4*>(sc.tris + leaf)[1].w);
                 kind = sc.materials[sc.objects[obj].material].type;
             } else if (sc.sky.on && (depth > 0 || R.sky_background)) {
@@ -554,13 +613,12 @@ __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
                 float lam[kHwss];
                 wavelengths(S.lambda0[slot], R.lambda_begin, R.lambda_end, lam);
                 const f3 d = mk3(S.dx[li], S.dy[li], S.dz[li]);
-                const bool nm = S.flags[slot] & kModeNm;
                 for (int w = 0; w < kHwss; ++w)
-                    if (!nm || (unsigned)w == lane)
-                        S.res[w * S.cap + slot] += S.thr[w * S.cap + slot] * sky_radiance(sc.sky, d, lam[w]);
+                    if (!nm) S.res[w * S.cap + slot] += S.thr[w * S.cap + slot] * sky_radiance(sc.sky, d, lam[w]);
+                if (nm) S.res[li] += S.thr0[li] * (S.thr[li] * sky_radiance(sc.sky, d, lam[lane]));
             }
         }
-        for (int m = 0; m < 3; ++m) {
+        for (int m = 0; m < bins; ++m) {
             const unsigned at = wave_push(cnt + m * (kMaxPathDepth + 1), kind == m);
             if (kind == m) S.mqueue[m][at] = code;
         }
@@ -568,14 +626,20 @@ __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 }
 
 
-// Writes lane w's next ray, offset off the surface toward its side, and
-// appends it to the next extension queue at `at`.
-__device__ inline void write_continuation(Paths& S, const Surf& h, unsigned slot, unsigned w, unsigned* at, f3 dir,
-                                          float pdf, float imp, unsigned bounces, const IorStack& st)
+// Origin of a ray leaving a surface: offset off it toward the ray's side.
+__device__ inline f3 surface_origin(const Surf& h, f3 dir)
+{
+    return offset_ray(h.p, dot3(dir, h.ng) > 0.0f ? h.ng : -h.ng);
+}
+
+// Writes lane w's next ray from `no` and appends it to the next extension
+// queue at `at`.
+__device__ inline void write_continuation(Paths& S, f3 no, unsigned slot, unsigned w, unsigned* at, f3 dir, float pdf,
+                                          float imp, unsigned bounces, const IorStack& st, unsigned vbn)
 {
     const unsigned cap = (unsigned)S.cap, lw = w * cap + slot;
     *at = slot * 4u + w;
-    const f3 no = offset_ray(h.p, dot3(dir, h.ng) > 0.0f ? h.ng : -h.ng);
+    if (S.vb) S.vb[lw] = (unsigned char)vbn;
     S.ox[lw] = no.x;
     S.oy[lw] = no.y;
     S.oz[lw] = no.z;
@@ -630,7 +694,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
         f3 o = mk3(0, 0, 0);
         float lam[kHwss] = {0, 0, 0, 0};
         float pdf_in = 0.0f, imp = 1.0f;
-        unsigned bst = 0;
+        unsigned bst = 0, vbn = 0;
+        int mobj = -1;  // object whose medium holds the hit point (single-wavelength lanes)
         IorStack ist{0, {0, 0, 0}};
         if (valid) {
             path = S.path_id[slot];
@@ -642,6 +707,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             bst = S.state[li];
             ist.n = (int)((bst >> 24) & 3u);
             for (int k = 0; k < 3; ++k) ist.obj[k] = k < ist.n ? S.ior_obj[k * 4 * cap + li] : 0;
+            if (nm && S.vb) {
+                vbn = S.vb[li];
+                mobj = lane_medium_object(sc, S, li);
+            }
             const float4* tp = reinterpret_cast<const float4*>(sc.tris + leaf);
             const float4 ta = tp[0], tb = tp[1], tc = tp[2];
             const f3 v0 = mk3(ta.x, ta.y, ta.z), v1 = mk3(tb.x, tb.y, tb.z), v2 = mk3(tc.x, tc.y, tc.z);
@@ -679,8 +748,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 }
             }
             for (int w = 0; w < kHwss; ++w)
-                if (!nm || (unsigned)w == lane)
-                    S.res[w * cap + slot] += S.thr[w * cap + slot] * (emitted(sc, mat, -h.dir, h.ng, lam[w]) * mis);
+                if (!nm) S.res[w * cap + slot] += S.thr[w * cap + slot] * (emitted(sc, mat, -h.dir, h.ng, lam[w]) * mis);
+            if (nm) S.res[li] += S.thr0[li] * (S.thr[li] * (emitted(sc, mat, -h.dir, h.ng, lam[lane]) * mis));
         }
 
         // Next-event estimation: one light sample per wavelength of a shared
@@ -724,11 +793,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                         const float p_bsdf = bsdf_pdf<MAT>(sc, mat, h, to, lam[w], amb);
                         if (p_bsdf > 0.0f) contrib *= power_heuristic(p_light, p_bsdf);
                         contrib /= L.pdf_select;
-                        v = S.thr[w * cap + slot] * contrib;
+                        v = nm ? S.thr0[li] * (S.thr[li] * contrib) : S.thr[w * cap + slot] * contrib;
                         if (v > 0.0f) {
                             has = true;
                             sorg = offset_ray(h.p, dot3(to, h.ng) > 0.0f ? h.ng : -h.ng);
-                            stm = dist - 0.001f;  // LightSampler.cpp:2206
+                            stm = dist;
                         }
                     }
                 }
@@ -744,6 +813,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 S.stmax[si] = stm;
                 S.sval[si] = v;
                 S.sslot[si] = slot * 4u + (unsigned)w;
+                if (S.sinfo) {
+                    S.sinfo[si] = (unsigned)(mobj + 1);
+                    S.svis[si] = 0u;
+                }
             }
         }
 
@@ -833,7 +906,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 }
             }
             const unsigned qi = wave_push(cnt_next, cont);
-            if (cont) write_continuation(S, h, slot, cw, qout + qi, cdir, cpdf, cimp, cst, ist);
+            if (cont) write_continuation(S, surface_origin(h, cdir), slot, cw, qout + qi, cdir, cpdf, cimp, cst, ist, vbn);
         } else {
             // A dielectric: for a lane already on its own, and for each wavelength of
             // a shared ray that branches here (:2634-2760; no NEE or RR at this vertex).
@@ -849,23 +922,297 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                     if (dielectric_scatter(sc, mat, h, o, lam[w], ist, rng_u(R.seed, path, ld + 18), nd, wgt, type,
                                            nst) &&
                         bump_bounces(nb, type, R)) {
-                        const float t = S.thr[w * cap + slot] * wgt;
+                        // A shared ray's wavelength starts its own loop at throughput 1
+                        // (PathTracingIntegrator.cpp:1593), its shared throughput kept aside.
+                        if (!nm) S.thr0[w * cap + slot] = S.thr[w * cap + slot];
+                        const float t = (nm ? S.thr[w * cap + slot] : 1.0f) * wgt;
                         S.thr[w * cap + slot] = t;
                         cont = fabsf(t) <= 1e6f;
                     }
                 }
                 const unsigned qi = wave_push(cnt_next, cont);
-                if (cont) write_continuation(S, h, slot, (unsigned)w, qout + qi, nd, 0.0f, imp * fabsf(wgt), nb, nst);
+                if (cont)
+                    write_continuation(S, surface_origin(h, nd), slot, (unsigned)w, qout + qi, nd, 0.0f, imp * fabsf(wgt),
+                                       nb, nst, vbn);
             }
             if (branch) S.flags[slot] |= kModeNm;
         }
     }
 }
 
-// Shadow rays (LightSampler::ShadowOccludedNM -> ObjectManager::IntersectShadowRay);
-// adds each unoccluded contribution to its path and wavelength.  There is one
-// record per (path, wavelength) and bounce, so the additions never collide.
-// The same two traversal schedules as k_extend.
+
+// ---- participating media --------------------------------------------------
+
+// HenyeyGreensteinPhaseFunction::EvaluateWithG (Materials/
+// HenyeyGreensteinPhaseFunction.cpp:20-33); Pdf is the same value.
+__device__ inline float hg_eval(float c, float g)
+{
+    const float g2 = g * g;
+    const float den = 1.0f + g2 - 2.0f * g * c;
+    if (den <= 0.0f) return 0.0f;
+    return (1.0f - g2) / (2.0f * kTwoPi * den * sqrtf(den));
+}
+
+// Vector3Ops::Perpendicular (Utilities/Math3D/VectorsOps.h:164-188).
+__device__ inline f3 perpendicular3(f3 v)
+{
+    const float x = fabsf(v.x), y = fabsf(v.y), z = fabsf(v.z);
+    if (x < y && x < z) return mk3(0.0f, v.z, -v.y);
+    if (y < z) return mk3(v.z, 0.0f, -v.x);
+    return mk3(v.y, -v.x, 0.0f);
+}
+
+// HenyeyGreensteinPhaseFunction::SampleWithG (:35-56): cos(theta) by inverting
+// the HG CDF, then GeometricUtilities::Perturb (GeometricUtilities.cpp:100-118)
+// -- wo in the basis U = wo, V = Perpendicular(wo), W = U x V, rotated by
+// theta about the basis Y axis and by phi about its X axis (row-vector
+// matrices, Utilities/Math3D/MatricesOps.h:274-303) -- and back.
+__device__ inline f3 hg_sample(f3 wo, float g, float u1, float u2)
+{
+    float ct;
+    if (fabsf(g) < 1e-10f) {
+        ct = 1.0f - 2.0f * u1;
+    } else {
+        const float inner = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+        ct = (1.0f / (2.0f * g)) * (1.0f + g * g - inner * inner);
+    }
+    ct = fmaxf(-1.0f, fminf(1.0f, ct));
+    const float th = acosf(ct), ph = kTwoPi * u2;
+    const f3 U = norm3(wo), V = norm3(perpendicular3(wo)), W = cross3(U, V);
+    const float a0 = dot3(U, wo), a1 = dot3(V, wo), a2 = dot3(W, wo);
+    const float cd = cosf(th), sd = sinf(th), ca = cosf(ph), sa = sinf(ph);
+    const float r0 = cd * a0 + sd * a2, r1 = a1, r2 = -sd * a0 + cd * a2;
+    return norm3(U * r0 + V * (ca * r1 - sa * r2) + W * (sa * r1 + ca * r2));
+}
+
+// A volume scatter of a single-wavelength lane (PathTracingIntegrator.cpp:
+// 1714-1889): weight Tr sigma_s / (sigma_t Tr); in-scattering NEE
+// (MediumTransport::EvaluateInScatteringNM, Utilities/MediumTransport.cpp:
+// 139-174, -> EvaluateDirectLightingNM with isVolumeScatter: cosSurface 1,
+// the phase function as BSDF and MIS pdf, no shading object, so no
+// self-exclusion and no occlusion test -- its record only takes the
+// transmittance walk); HG continuation from the scatter point with
+// bsdfPdf = the phase pdf; Russian roulette at depth + volume bounces
+// (:1862-1881), which leaves the importance unchanged.
+__global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+{
+    const unsigned n = S.cnt_mat[kMatVolume * (kMaxPathDepth + 1) + depth];
+    const unsigned* qin = S.mqueue[kMatVolume];
+    unsigned* qout = S.queue[(depth + 1) & 1];
+    unsigned* cnt_next = S.cnt_ext + depth + 1;
+    unsigned* cnt_sh = S.cnt_sh + depth;
+    const unsigned base_dim = 32u * (unsigned)(16 + depth);
+    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned cap = (unsigned)S.cap;
+    for (unsigned b = blockIdx.x * blockDim.x; b < n; b += stride) {
+        const unsigned i = b + threadIdx.x;
+        const bool valid = i < n;
+        unsigned slot = 0, lane = 0, li = 0;
+        if (valid) {
+            const unsigned code = qin[i];
+            slot = code >> 2;
+            lane = code & 3u;
+            li = lane * cap + slot;
+        }
+        const ReqInfo& R = reqs[valid ? S.req[slot] : 0u];
+        const PtSceneView& sc = R.sc;
+        bool alive = false;
+        f3 p = mk3(0, 0, 0), d = mk3(0, 0, 0);
+        float nm = 0.0f, thr = 0.0f, imp = 1.0f;
+        unsigned path = 0, bst = 0, vbn = 0;
+        int mobj = -1;
+        IorStack ist{0, {0, 0, 0}};
+        DevMedium M{};
+        if (valid) {
+            path = S.path_id[slot];
+            d = mk3(S.dx[li], S.dy[li], S.dz[li]);
+            const float tm = S.hit_t[li];
+            p = mk3(S.ox[li], S.oy[li], S.oz[li]) + d * tm;
+            float lam[kHwss];
+            wavelengths(S.lambda0[slot], R.lambda_begin, R.lambda_end, lam);
+            nm = lam[lane];
+            imp = S.imp[li];
+            bst = S.state[li];
+            vbn = S.vb[li];
+            ist.n = (int)((bst >> 24) & 3u);
+            for (int k = 0; k < 3; ++k) ist.obj[k] = k < ist.n ? S.ior_obj[k * 4 * cap + li] : 0;
+            mobj = lane_medium_object(sc, S, li);  // binned here only with a medium on top
+            M = sc.media[sc.objects[mobj].medium];
+            const float tr = expf(-M.sigma_t * tm);
+            float mw = 0.0f;
+            if (M.sigma_t > 0.0f && tr > 0.0f) mw = tr * M.sigma_s / (M.sigma_t * tr);
+            if (mw > 0.0f) {
+                thr = S.thr[li] * mw;
+                S.thr[li] = thr;
+                alive = true;
+            }
+        }
+        const unsigned ld = base_dim + kLaneDims * (lane + 1u);
+
+        // In-scattering NEE at the lane's wavelength.
+        bool has = false;
+        float v = 0.0f, dist = 0.0f;
+        f3 to = mk3(0, 0, 0);
+        if (alive && sc.n_lums > 0) {
+            const float usel = rng_u(R.seed, path, ld);
+            int l = 0;
+            while (l < sc.n_lums - 1 && !(usel < sc.lums[l].cdf_select)) ++l;
+            const DevLuminary L = sc.lums[l];
+            const float ux = rng_u(R.seed, path, ld + 1), uy = rng_u(R.seed, path, ld + 2),
+                        uz = rng_u(R.seed, path, ld + 3);
+            int k = 0;
+            while (k < L.tri_count - 1 && sc.lum_cdf[L.tri_first + k] < uz) ++k;
+            const float4* lp = reinterpret_cast<const float4*>(sc.tris + sc.lum_tris[L.tri_first + k]);
+            const float4 la = lp[0], lb = lp[1], lc = lp[2];
+            const f3 a0 = mk3(la.x, la.y, la.z), a1 = mk3(lb.x, lb.y, lb.z), a2 = mk3(lc.x, lc.y, lc.z);
+            const float sa = sqrtf(1.0f - ux);
+            const float al = 1.0f - sa, be = sa * uy, ga = 1.0f - al - be;
+            const f3 pl = a1 * al + a2 * be + a0 * ga;
+            const f3 ln = norm3(cross3(a1 - a0, a2 - a0));
+            to = pl - p;
+            dist = len3(to);
+            if (dist > 0.0f) to = to * (1.0f / dist);
+            const float cos_l = dot3(-to, ln);
+            if (cos_l > 0.0f) {
+                const float le = emitted(sc, sc.materials[L.material], -to, ln, nm);
+                const float ph = hg_eval(dot3(to, d), M.g);
+                float contrib = le * 1.0f * (L.area * cos_l / (dist * dist)) * ph;
+                const float p_light = L.pdf_select * (dist * dist) / (L.area * cos_l);
+                if (ph > 0.0f) contrib *= power_heuristic(p_light, ph);
+                contrib /= L.pdf_select;
+                v = S.thr0[li] * (thr * contrib);
+                has = v > 0.0f;
+            }
+        }
+        const unsigned si = wave_push(cnt_sh, has);
+        if (has) {
+            S.sox[si] = p.x;
+            S.soy[si] = p.y;
+            S.soz[si] = p.z;
+            S.sdx[si] = to.x;
+            S.sdy[si] = to.y;
+            S.sdz[si] = to.z;
+            S.stmax[si] = dist;
+            S.sval[si] = v;
+            S.sslot[si] = slot * 4u + lane;
+            S.sinfo[si] = kShadowInScatter | (unsigned)(mobj + 1);
+            S.svis[si] = 1u;
+        }
+
+        // Phase sampling, Russian roulette, continuation.
+        bool cont = false;
+        f3 nd = mk3(0, 0, 0);
+        float pp = 0.0f;
+        if (alive) {
+            nd = hg_sample(d, M.g, rng_u(R.seed, path, ld + 16), rng_u(R.seed, path, ld + 17));
+            pp = hg_eval(dot3(d, nd), M.g);
+            if (pp > kNearZeroF) {
+                float t = thr * (hg_eval(dot3(d, nd), M.g) / pp);
+                bool go = true;
+                if (depth + (int)vbn >= R.rr_min_depth) {
+                    const float q = fminf(1.0f, fabsf(t) / fmaxf(imp, R.rr_threshold));
+                    if (rng_u(R.seed, path, ld + 19) >= q) go = false;
+                    else if (q > 0.0f && q < 1.0f) t /= q;
+                }
+                if (go) {
+                    S.thr[li] = t;
+                    cont = fabsf(t) <= 1e6f;  // runaway guard of the next bounce (:1660-1683)
+                }
+            }
+        }
+        const unsigned qi = wave_push(cnt_next, cont);
+        if (cont) write_continuation(S, p, slot, lane, qout + qi, nd, pp, imp, bst, ist, vbn + 1u);
+    }
+}
+
+// EvalShadowTransmittanceNM (Lights/LightSampler.cpp:369-470): the shadow
+// ray from o (unit d) over max_dist, starting inside the medium of object
+// `origin` (-1: none), walks boundary by boundary -- closest hits from 1e-5
+// past the last boundary, at most 16 -- keeping a stack of at most 4 media
+// (front face by the geometric normal: enter; back face: leave) and
+// multiplying the innermost medium's transmittance over each segment.  The
+// walk here intersects only the shells of objects holding a medium: in the
+// reference, crossings of other objects only split segments (the same
+// transmittance product) and count toward the 16 steps.
+__device__ inline float shadow_walk(const PtSceneView& sc, f3 o, f3 d, float max_dist, int origin)
+{
+    float tr = 1.0f;
+    if (!(max_dist > 0.0f)) return tr;
+    int stk[4] = {origin, -1, -1, -1};
+    int cnt = origin >= 0 ? 1 : 0;
+    float seg = 0.0f;
+    PrivStack st;
+    for (int step = 0; step < 16 && seg < max_dist; ++step) {
+        const float cs = seg + 1e-5f;
+        if (cs >= max_dist) break;
+        const f3 co = o + d * cs;
+        const float cmax = max_dist - cs;
+        RayState r;
+        init_ray(r, co.x, co.y, co.z, d.x, d.y, d.z);
+        st.sp = 0;
+        double best;
+        unsigned leaf, nn = 0, nt = 0;
+        const int id = trace_closest<false>(r, sc.mnodes, sc.mtris, st, best, leaf, nn, nt);
+        const float sig = cnt > 0 ? sc.media[sc.objects[stk[cnt - 1]].medium].sigma_t : 0.0f;
+        const float t = (float)best;
+        if (id < 0 || t >= cmax) {
+            const float rem = max_dist - seg;
+            if (rem > 0.0f && cnt > 0) tr *= expf(-sig * rem);
+            seg = max_dist;
+            break;
+        }
+        const float4* tp = reinterpret_cast<const float4*>(sc.mtris + leaf);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        const int obj = float_to_bits(b.w);
+        f3 ng = cross3(mk3(b.x - a.x, b.y - a.y, b.z - a.z), mk3(c.x - a.x, c.y - a.y, c.z - a.z));
+        if (sc.objects[obj].double_sided && dot3(ng, d) > 0.0f) ng = -ng;  // TriangleMeshGeometryIndexed.cpp:171-188
+        const float bd = cs + t;
+        const float len = bd - seg;
+        if (len > 0.0f && cnt > 0) tr *= expf(-sig * len);
+        if (dot3(ng, d) < 0.0f) {
+            if (cnt < 4) stk[cnt++] = obj;
+        } else {
+            int at = -1;
+            for (int k = 0; k < 4; ++k)
+                if (at < 0 && k < cnt && stk[k] == obj) at = k;
+            if (at >= 0) {
+                for (int k = 0; k < 3; ++k)
+                    if (k >= at) stk[k] = stk[k + 1];
+                --cnt;
+            }
+        }
+        seg = bd;
+        if (tr < 1e-6f) return 0.0f;
+    }
+    if (seg < max_dist && cnt > 0) {
+        const float rem = max_dist - seg;
+        if (rem > 0.0f) tr *= expf(-sc.media[sc.objects[stk[cnt - 1]].medium].sigma_t * rem);
+    }
+    return tr;
+}
+
+// Adds the unoccluded shadow records of scenes with media, times the
+// transmittance of the walk (LightSampler.cpp:2222-2228).
+__global__ void k_walk(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+{
+    const unsigned n = S.cnt_sh[depth];
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (!S.svis[i]) continue;
+        const unsigned code = S.sslot[i];
+        const ReqInfo& R = reqs[S.req[code >> 2]];
+        const int origin = (int)(S.sinfo[i] & ~kShadowInScatter) - 1;
+        const float tr = shadow_walk(R.sc, mk3(S.sox[i], S.soy[i], S.soz[i]), mk3(S.sdx[i], S.sdy[i], S.sdz[i]),
+                                     S.stmax[i], origin);
+        S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[i] * tr;
+    }
+}
+
+// Shadow rays (LightSampler::ShadowOccludedNM -> ObjectManager::IntersectShadowRay)
+// to 0.001 short of the light point (LightSampler.cpp:2206); adds each
+// unoccluded contribution to its path and wavelength, or, in a scene with
+// media, marks it for the transmittance walk.  In-scattering records take no
+// occlusion test.  There is one record per (path, wavelength) and bounce, so
+// the additions never collide.  The same two traversal schedules as k_extend.
 template <int STACK, int BLOCK, bool PERSIST>
 __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
@@ -876,6 +1223,7 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
     const unsigned n = S.cnt_sh[depth];
     if constexpr (!PERSIST) {
         for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            if (S.sinfo && (S.sinfo[i] & kShadowInScatter)) continue;
             const unsigned code = S.sslot[i];
             const ReqInfo& R = reqs[S.req[code >> 2]];
             RayState r;
@@ -883,10 +1231,13 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
             unsigned nn = 0, nt = 0;
             lst.sp = 0;
             pst.sp = 0;
-            const double how_far2 = scale_how_far(r, (double)S.stmax[i]);
+            const double how_far2 = scale_how_far(r, (double)(S.stmax[i] - 0.001f));
             const bool occ = STACK > 0 ? trace_any<false>(r, R.sc.nodes, R.sc.tris, lst, how_far2, nn, nt)
                                        : trace_any<false>(r, R.sc.nodes, R.sc.tris, pst, how_far2, nn, nt);
-            if (!occ) S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[i];
+            if (!occ) {
+                if (R.sc.has_media) S.svis[i] = 1u;
+                else S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[i];
+            }
         }
         return;
     }
@@ -895,19 +1246,23 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
     AnyState s;
     const NodeSlot* nodes = nullptr;
     const Tri48* tris = nullptr;
+    bool media = false;
     unsigned nn = 0, nt = 0;
     long long item = -1;
     for (;;) {
         const long long got = refill(item == -1, item >= 0, next, n);
         if (item == -1 && got != -1) {
             item = got;
-            if (item >= 0) {
+            if (item >= 0 && S.sinfo && (S.sinfo[item] & kShadowInScatter)) {
+                item = -1;  // no occlusion test; the walk adds it
+            } else if (item >= 0) {
                 const unsigned slot = S.sslot[item] >> 2;
                 const ReqInfo& R = reqs[S.req[slot]];
                 nodes = R.sc.nodes;
                 tris = R.sc.tris;
+                media = R.sc.has_media;
                 init_ray(r, S.sox[item], S.soy[item], S.soz[item], S.sdx[item], S.sdy[item], S.sdz[item]);
-                any_begin(r, scale_how_far(r, (double)S.stmax[item]), s);
+                any_begin(r, scale_how_far(r, (double)(S.stmax[item] - 0.001f)), s);
                 lst.sp = 0;
                 pst.sp = 0;
             }
@@ -919,7 +1274,8 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
             if (!more) {
                 if (!s.hit) {
                     const unsigned code = S.sslot[item];
-                    S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[item];
+                    if (media) S.svis[item] = 1u;
+                    else S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[item];
                 }
                 item = -1;
             }
@@ -1022,7 +1378,7 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Carves the path pool for `cap` paths plus `passes` counter blocks from
 // `base` (NULL: sizes only); returns the size.
-size_t carve(char* base, int64_t cap, int passes, Paths* P, unsigned** counters)
+size_t carve(char* base, int64_t cap, int passes, bool media, Paths* P, unsigned** counters)
 {
     size_t off = 0;
     auto take = [&](size_t bytes) -> char* {
@@ -1040,6 +1396,7 @@ size_t carve(char* base, int64_t cap, int passes, Paths* P, unsigned** counters)
     S.dy = (float*)take(c4 * 4);
     S.dz = (float*)take(c4 * 4);
     S.thr = (float*)take(c4 * 4);
+    S.thr0 = (float*)take(c4 * 4);
     S.res = (float*)take(c4 * 4);
     S.lambda0 = (float*)take(c * 4);
     S.pdf = (float*)take(c4 * 4);
@@ -1054,6 +1411,12 @@ size_t carve(char* base, int64_t cap, int passes, Paths* P, unsigned** counters)
     S.queue[0] = (unsigned*)take(c4 * 4);
     S.queue[1] = (unsigned*)take(c4 * 4);
     for (int m = 0; m < 3; ++m) S.mqueue[m] = (unsigned*)take(c4 * 4);
+    if (media) {
+        S.mqueue[kMatVolume] = (unsigned*)take(c4 * 4);
+        S.vb = (unsigned char*)take(c4);
+        S.sinfo = (unsigned*)take(c4 * 4);
+        S.svis = (unsigned*)take(c4 * 4);
+    }
     S.sox = (float*)take(c4 * 4);
     S.soy = (float*)take(c4 * 4);
     S.soz = (float*)take(c4 * 4);
@@ -1306,6 +1669,7 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     int bounces = 1, depth = 1;
     int64_t max_tris = 0;
     unsigned mat_mask = 0;
+    bool has_media = false;
     std::vector<int64_t> req_px(n, 0);
     size_t film_floats = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -1320,11 +1684,14 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
         total_paths += req_px[k] * D.spp;
         // A path makes at most 1 + (sum of the caps of the lobe kinds its scene has) hits.
         const rise_hip_pt_scene* sc = reqs[k].scene;
+        // Each volume scatter takes one more bounce, up to kMaxVolumeBounce.
         const int hits = 1 + D.max_diffuse_bounce + (sc->has_glossy ? D.max_glossy_bounce : 0) +
-                         (sc->has_transmission ? D.max_transmission_bounce : 0);
+                         (sc->has_transmission ? D.max_transmission_bounce : 0) +
+                         (sc->has_media ? kMaxVolumeBounce : 0);
         bounces = std::max(bounces, std::min(kMaxPathDepth, hits));
         depth = std::max(depth, reqs[k].scene->view.depth);
-        mat_mask |= reqs[k].scene->material_kinds;
+        mat_mask |= reqs[k].scene->material_kinds | (sc->has_media ? 1u << kMatVolume : 0u);
+        has_media = has_media || sc->has_media;
         max_tris = std::max<int64_t>(max_tris, reqs[k].scene->n_tris);
         film_floats += 12 * (size_t)req_px[k];
     }
@@ -1364,7 +1731,7 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     // Device buffers.
     hipError_t e = hipGetDevice(&b->device);
     if (e != hipSuccess) return fail_hip(e, "hipGetDevice");
-    const size_t need = carve(nullptr, cap, passes, nullptr, nullptr);
+    const size_t need = carve(nullptr, cap, passes, has_media, nullptr, nullptr);
     if (need > b->work_bytes || !b->work) {
         (void)hipFree(b->work);
         b->work = nullptr;
@@ -1392,7 +1759,7 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     if (e != hipSuccess) return fail_hip(e, "batch tables");
 
     b->cap = cap;
-    carve((char*)b->work, cap, passes, &b->S, &b->counters);
+    carve((char*)b->work, cap, passes, has_media, &b->S, &b->counters);
     b->film_floats = film_floats;
     b->entries.swap(entries);
     b->pass_first.swap(pass_first);
@@ -1440,7 +1807,11 @@ void launch_pass(const ReqInfo* d_info, Paths S, const PassEntry* ent, int n_ent
             hipLaunchKernelGGL(k_shade<kMatGgx>, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
         if (sh.mat_mask & (1u << kMatDielectric))
             hipLaunchKernelGGL(k_shade<kMatDielectric>, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
+        if (sh.mat_mask & (1u << kMatVolume))
+            hipLaunchKernelGGL(k_volume, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
         hipLaunchKernel((const void*)sh.con_fn, dim3(sh.gc), dim3(sh.block), args, 0, stream);
+        if (sh.mat_mask & (1u << kMatVolume))
+            hipLaunchKernelGGL(k_walk, dim3(grid_for(4 * sh.cap, 256, 2048)), dim3(256), 0, stream, d_info, S, d);
     }
     hipLaunchKernelGGL(k_film, dim3(grid_for(film, 256, 4096)), dim3(256), 0, stream, d_info, ent, n_ent, film, S);
 }
@@ -1865,7 +2236,7 @@ extern "C" rise_hip_status rise_hip_server_create(const rise_hip_server_config* 
     sv->cap = cfg.tuning.max_paths > 0 ? std::max<int64_t>(kChunkPaths, std::min<int64_t>(cfg.tuning.max_paths, kMaxPaths))
                                        : kMaxPaths;
     hipError_t e = hipGetDevice(&sv->device);
-    const size_t need = carve(nullptr, sv->cap, 1, nullptr, nullptr);
+    const size_t need = carve(nullptr, sv->cap, 1, true, nullptr, nullptr);
     if (e == hipSuccess) e = hipMalloc(&sv->work, need);
     if (e == hipSuccess) e = hipMalloc((void**)&sv->d_counters, sizeof(unsigned) * kCounters * kRing);
     if (e == hipSuccess) e = hipMalloc((void**)&sv->d_info, sizeof(ReqInfo) * kServerSlots);
@@ -1878,7 +2249,7 @@ extern "C" rise_hip_status rise_hip_server_create(const rise_hip_server_config* 
         server_free(sv);
         return fail_hip(e, "server allocation");
     }
-    carve((char*)sv->work, sv->cap, 1, &sv->S, nullptr);
+    carve((char*)sv->work, sv->cap, 1, true, &sv->S, nullptr);
     for (int k = sv->max_slots - 1; k >= 0; --k) sv->free_slots.push_back(k);
     *out_server = sv;
     return RISE_HIP_OK;
@@ -1906,10 +2277,11 @@ extern "C" rise_hip_status rise_hip_server_submit(rise_hip_server* server, const
     chunks_of(D, 0, r.chunks);
     const rise_hip_pt_scene* sc = request->scene;
     r.hits = std::min(kMaxPathDepth, 1 + D.max_diffuse_bounce + (sc->has_glossy ? D.max_glossy_bounce : 0) +
-                                         (sc->has_transmission ? D.max_transmission_bounce : 0));
+                                         (sc->has_transmission ? D.max_transmission_bounce : 0) +
+                                         (sc->has_media ? kMaxVolumeBounce : 0));
     r.depth = sc->view.depth;
     r.tris = sc->n_tris;
-    r.kinds = sc->material_kinds;
+    r.kinds = sc->material_kinds | (sc->has_media ? 1u << kMatVolume : 0u);
     for (const PassEntry& c : r.chunks)
         if ((int64_t)c.np * c.ns > server->cap) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "chunk exceeds path pool");
     if (out_ticket) *out_ticket = r.ticket;

@@ -52,13 +52,47 @@ extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* 
                                                     const rise_hip_spd_desc* spds, size_t n_spds,
                                                     rise_hip_pt_scene** out_scene)
 {
+    return rise_hip_pt_scene_create_with_media(objects, n_objects, materials, n_materials, spds, n_spds, nullptr, 0,
+                                               nullptr, out_scene);
+}
+
+extern "C" rise_hip_status rise_hip_pt_scene_create_with_media(
+    const rise_hip_object_desc* objects, size_t n_objects, const rise_hip_material_desc* materials,
+    size_t n_materials, const rise_hip_spd_desc* spds, size_t n_spds, const rise_hip_medium_desc* media,
+    size_t n_media, const rise_hip_object_media* object_media, rise_hip_pt_scene** out_scene)
+{
     clear_error();
     if (!out_scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "out_scene is NULL");
     *out_scene = nullptr;
-    if ((n_objects && !objects) || (n_materials && !materials) || (n_spds && !spds))
+    if ((n_objects && !objects) || (n_materials && !materials) || (n_spds && !spds) || (n_media && !media))
         return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL array with a nonzero count");
-    if (n_objects > 0x7fffffff || n_materials > 0x7fffffff || n_spds > 0x7fffffff)
-        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "too many objects, materials or spectra");
+    if (n_objects > 0x7fffffff || n_materials > 0x7fffffff || n_spds > 0x7fffffff || n_media > 0x7fffffff)
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "too many objects, materials, spectra or media");
+    // Media: luminances of the RGB coefficients (HomogeneousMedium::GetCoefficientsNM).
+    std::vector<DevMedium> dmed(n_media);
+    for (size_t i = 0; i < n_media; ++i) {
+        const rise_hip_medium_desc& m = media[i];
+        const std::string who = "medium " + std::to_string(i) + ": ";
+        for (int c = 0; c < 3; ++c)
+            if (!(m.sigma_a[c] >= 0.0f && m.sigma_s[c] >= 0.0f && std::isfinite(m.sigma_a[c]) &&
+                  std::isfinite(m.sigma_s[c])))
+                return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "coefficients must be finite and non-negative");
+        if (!(m.hg_g > -1.0f && m.hg_g < 1.0f))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "Henyey-Greenstein g must lie in (-1, 1)");
+        if (m.reserved != 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "reserved field must be 0");
+        double t[3];
+        for (int c = 0; c < 3; ++c) t[c] = (double)m.sigma_a[c] + (double)m.sigma_s[c];
+        dmed[i].sigma_t = (float)(0.2126 * t[0] + 0.7152 * t[1] + 0.0722 * t[2]);
+        dmed[i].sigma_s = (float)(0.2126 * (double)m.sigma_s[0] + 0.7152 * (double)m.sigma_s[1] +
+                                  0.0722 * (double)m.sigma_s[2]);
+        dmed[i].g = m.hg_g;
+        dmed[i].pad = 0;
+    }
+    if (object_media)
+        for (size_t o = 0; o < n_objects; ++o)
+            if (object_media[o].interior_medium < -1 || object_media[o].interior_medium >= (int64_t)n_media)
+                return fail(RISE_HIP_ERR_INVALID_ARGUMENT,
+                            "object " + std::to_string(o) + ": interior medium out of range");
     // Spectra.
     std::vector<DevSpd> dspd(n_spds);
     std::vector<float> amps;
@@ -127,14 +161,47 @@ extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* 
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }
-    // Object id in v1.w; inverse map global id -> leaf index.
+    // Object id in v1.w, shadow flag in v2.w; inverse map global id -> leaf index.
     std::vector<uint32_t> leaf_of((size_t)total);
     for (size_t l = 0; l < bvh.tris.size(); ++l) {
         int32_t id;
         std::memcpy(&id, &bvh.tris[l].v0[3], 4);
         const int32_t ob = tri_obj[(size_t)id];
         std::memcpy(&bvh.tris[l].v1[3], &ob, 4);
+        const uint32_t flags = (object_media && !object_media[ob].casts_shadows) ? kTriNoShadow : 0u;
+        std::memcpy(&bvh.tris[l].v2[3], &flags, 4);
         leaf_of[(size_t)id] = (uint32_t)l;
+    }
+    // The shells of the objects holding a medium, for the shadow-ray
+    // transmittance walk (LightSampler.cpp:369-470).
+    WideBVH mbvh;
+    bool has_media = false;
+    if (object_media) {
+        std::vector<float> mverts;
+        std::vector<int32_t> mobj;
+        int64_t off = 0;
+        for (size_t o = 0; o < n_objects; ++o) {
+            const int64_t k = objects[o].n_tris;
+            if (object_media[o].interior_medium >= 0 && k > 0) {
+                mverts.insert(mverts.end(), verts.begin() + 9 * off, verts.begin() + 9 * (off + k));
+                mobj.insert(mobj.end(), (size_t)k, (int32_t)o);
+            }
+            off += k;
+        }
+        if (!mobj.empty()) {
+            has_media = true;
+            try {
+                if (!build_wide_bvh(mverts.data(), (int64_t)mobj.size(), nullptr, BuildParams(), mbvh, err))
+                    return fail(RISE_HIP_ERR_BUILD, "medium-boundary BVH build failed: " + err);
+            } catch (const std::bad_alloc&) {
+                return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+            }
+            for (Tri48& t : mbvh.tris) {
+                int32_t id;
+                std::memcpy(&id, &t.v0[3], 4);
+                std::memcpy(&t.v1[3], &mobj[(size_t)id], 4);
+            }
+        }
     }
     // Luminaries (LuminaryManager + LightSampler::Prepare): emitting objects,
     // area CDF over their triangles, selection weight = mean exitance * scale * area.
@@ -147,7 +214,8 @@ extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* 
         int64_t off = 0;
         for (size_t o = 0; o < n_objects; ++o) {
             const int64_t k = objects[o].n_tris;
-            dobj[o] = DevObject{objects[o].material, objects[o].double_sided ? 1 : 0, -1, 0};
+            dobj[o] = DevObject{objects[o].material, objects[o].double_sided ? 1 : 0, -1,
+                                object_media ? object_media[o].interior_medium : -1};
             const DevMaterial& m = dmat[objects[o].material];
             if (m.emit_spd >= 0 && k > 0) {
                 std::vector<double> areas((size_t)k);
@@ -214,6 +282,9 @@ extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* 
     if (st == RISE_HIP_OK) st = upload(s, lums, &v.lums);
     if (st == RISE_HIP_OK) st = upload(s, lum_tris, &v.lum_tris);
     if (st == RISE_HIP_OK) st = upload(s, lum_cdf, &v.lum_cdf);
+    if (st == RISE_HIP_OK && has_media) st = upload(s, dmed, &v.media);
+    if (st == RISE_HIP_OK && has_media) st = upload(s, mbvh.nodes, &v.mnodes);
+    if (st == RISE_HIP_OK && has_media) st = upload(s, mbvh.tris, &v.mtris);
     if (st != RISE_HIP_OK) {
         free_scene(s);
         return st;
@@ -224,6 +295,8 @@ extern "C" rise_hip_status rise_hip_pt_scene_create(const rise_hip_object_desc* 
     s->has_glossy = has_glossy;
     s->has_transmission = has_transmission;
     s->material_kinds = material_kinds;
+    v.has_media = has_media ? 1 : 0;
+    s->has_media = has_media;
     *out_scene = s;
     return RISE_HIP_OK;
 }

@@ -330,7 +330,8 @@ RISE_HD inline void any_begin(const RayState& r, double how_far2, AnyState& s)
 }
 
 // One any-hit step; true iff some triangle certifies with
-// kNearZero < t < how_far2 (...Specializations.h:517-519).  Returns false once
+// kNearZero < t < how_far2 (...Specializations.h:517-519) and casts shadows
+// (bit 0 of v2.w clear; ObjectManager.cpp:119-124).  Returns false once
 // the query is decided (a hit, or the tree exhausted).
 template <bool COUNT, class Stack>
 RISE_HD inline bool any_step(const RayState& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
@@ -356,7 +357,7 @@ RISE_HD inline bool any_step(const RayState& r, const NodeSlot* __restrict__ nod
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
         double tt;
-        if (certify(r, a, b, c, tt) && tt > kNearZero && tt < s.how_far2) {
+        if (certify(r, a, b, c, tt) && tt > kNearZero && tt < s.how_far2 && !(float_to_bits(c.w) & 1u)) {
             s.hit = true;
             return false;
         }

@@ -148,9 +148,27 @@ public:
         has_sky_ = true;
     }
 
-    void AddObject(const float* vertices, int64_t n_tris, int material, bool double_sided = false)
+    // Job::AddHomogeneousMedium with `phase hg g`: RGB absorption and scattering per unit length.
+    int AddHomogeneousMedium(const float sigma_a[3], const float sigma_s[3], float hg_g)
+    {
+        rise_hip_medium_desc m{};
+        for (int c = 0; c < 3; ++c) {
+            m.sigma_a[c] = sigma_a[c];
+            m.sigma_s[c] = sigma_s[c];
+        }
+        m.hg_g = hg_g;
+        media_.push_back(m);
+        return (int)media_.size() - 1;
+    }
+
+    // standard_object: a triangle mesh with its material, interior_medium (-1: none)
+    // and casts_shadows.
+    void AddObject(const float* vertices, int64_t n_tris, int material, bool double_sided = false,
+                   int interior_medium = -1, bool casts_shadows = true)
     {
         objects_.push_back({vertices, n_tris, material, double_sided ? 1 : 0});
+        object_media_.push_back({interior_medium, casts_shadows ? 1 : 0});
+        if (interior_medium >= 0 || !casts_shadows) use_media_ = true;
     }
 
     bool PrepareForRendering()
@@ -158,8 +176,10 @@ public:
         for (size_t i = 0; i < spds_.size(); ++i) spds_[i].amplitudes = amps_[i].data();
         rise_hip_pt_scene_release(scene_);
         scene_ = nullptr;
-        if (!check(rise_hip_pt_scene_create(objects_.data(), objects_.size(), materials_.data(), materials_.size(),
-                                            spds_.data(), spds_.size(), &scene_),
+        if (!check(rise_hip_pt_scene_create_with_media(objects_.data(), objects_.size(), materials_.data(),
+                                                       materials_.size(), spds_.data(), spds_.size(), media_.data(),
+                                                       media_.size(), use_media_ ? object_media_.data() : nullptr,
+                                                       &scene_),
                    "PrepareForRendering"))
             return false;
         return !has_sky_ || check(rise_hip_pt_scene_set_sky(scene_, &sky_), "PrepareForRendering (sky)");
@@ -179,6 +199,9 @@ private:
     std::vector<rise_hip_spd_desc> spds_;
     std::vector<rise_hip_material_desc> materials_;
     std::vector<rise_hip_object_desc> objects_;
+    std::vector<rise_hip_medium_desc> media_;
+    std::vector<rise_hip_object_media> object_media_;
+    bool use_media_ = false;
     rise_hip_sky_desc sky_{};
     bool has_sky_ = false;
     rise_hip_pt_scene* scene_ = nullptr;

@@ -511,10 +511,12 @@ def serving_workload(seed: int = 5, n_requests: int = 64, n_scenes: int = 16, ra
     """The serving workload: `n_scenes` resident scenes of 50k-2M triangles
     (log-uniform; Cornell-style boxes with a seeded triangle cloud, every third
     cloud a GGX conductor, every third scene also holding a glass sphere and a
-    GGX block) and a Poisson stream of `n_requests` render requests at `rate`
-    per second, each over a scene drawn uniformly, at a resolution drawn from
-    {128, 256, 512, 1024}^2 and spp from {1, 4, 16, 64}.
-    Returns (scenes, [(arrival s, scene index, RenderSettings)])."""
+    GGX block, every other scene a box of ground fog -- homogeneous medium
+    sigma_s 0.5, sigma_a 0.05, g 0.3 behind an index-matched shell that casts
+    no shadow, over the floor up to y = 0.6) and a Poisson stream of
+    `n_requests` render requests at `rate` per second, each over a scene drawn
+    uniformly, at a resolution drawn from {128, 256, 512, 1024}^2 and spp from
+    {1, 4, 16, 64}.  Returns (scenes, [(arrival s, scene index, RenderSettings)])."""
     rng = np.random.default_rng(seed)
     counts = np.exp(rng.uniform(math.log(5e4), math.log(2e6), n_scenes)).astype(np.int64)
     scenes = []
@@ -533,6 +535,12 @@ def serving_workload(seed: int = 5, n_requests: int = 64, n_scenes: int = 16, ra
                             Material(kind=DIELECTRIC, ior=n - 1)]
             s.objects[4] = Object(s.objects[4].tris, len(s.materials) - 2)
             s.objects.insert(len(s.objects) - 1, Object(icosphere((0.35, 0.3, 0.85), 0.3, 2), len(s.materials) - 1))
+        if k % 2 == 0:
+            s.spds.append(Spd.flat(1.0))
+            s.materials.append(Material(kind=DIELECTRIC, ior=len(s.spds) - 1))
+            s.media = [Medium((0.05, 0.05, 0.05), (0.5, 0.5, 0.5), 0.3)]
+            s.objects.insert(len(s.objects) - 1,
+                             medium_shell((-0.95, 0.02, -0.95), (0.95, 0.6, 0.95), 0, len(s.materials) - 1))
         scenes.append(s)
     t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     reqs = []

@@ -48,6 +48,19 @@ class ObjectDesc(ctypes.Structure):
     ]
 
 
+class MediumDesc(ctypes.Structure):
+    _fields_ = [
+        ("sigma_a", ctypes.c_float * 3),
+        ("sigma_s", ctypes.c_float * 3),
+        ("hg_g", ctypes.c_float),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class ObjectMedia(ctypes.Structure):
+    _fields_ = [("interior_medium", ctypes.c_int32), ("casts_shadows", ctypes.c_int32)]
+
+
 class SkyDesc(ctypes.Structure):
     _fields_ = [
         ("solar_elevation_deg", ctypes.c_float),
@@ -141,6 +154,8 @@ _P = ctypes.c_void_p
 _SIG = {
     "rise_hip_pt_scene_create": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.POINTER(_P)],
                                  ctypes.c_int),
+    "rise_hip_pt_scene_create_with_media": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P,
+                                             ctypes.c_size_t, _P, ctypes.POINTER(_P)], ctypes.c_int),
     "rise_hip_pt_scene_release": ([_P], ctypes.c_int),
     "rise_hip_pt_scene_set_sky": ([_P, ctypes.POINTER(SkyDesc)], ctypes.c_int),
     "rise_hip_render": ([_P, ctypes.POINTER(RenderDesc), _P, _P, ctypes.POINTER(RenderStats), _P], ctypes.c_int),
@@ -228,8 +243,22 @@ class PathTracingSpectralRasterizer:
             objs[k].material = o.material
             objs[k].double_sided = 1 if o.double_sided else 0
         out = _P(None)
-        _lib.check(L.rise_hip_pt_scene_create(objs, len(scene.objects), mats, len(scene.materials), spds,
-                                              len(scene.spds), ctypes.byref(out)))
+        if scene.media or not all(o.casts_shadows for o in scene.objects):
+            med = (MediumDesc * max(1, len(scene.media)))()
+            for k, m in enumerate(scene.media):
+                med[k].sigma_a[:] = m.sigma_a
+                med[k].sigma_s[:] = m.sigma_s
+                med[k].hg_g = m.g
+            om = (ObjectMedia * max(1, len(scene.objects)))()
+            for k, o in enumerate(scene.objects):
+                om[k].interior_medium = o.medium
+                om[k].casts_shadows = 1 if o.casts_shadows else 0
+            _lib.check(L.rise_hip_pt_scene_create_with_media(objs, len(scene.objects), mats, len(scene.materials),
+                                                             spds, len(scene.spds), med, len(scene.media), om,
+                                                             ctypes.byref(out)))
+        else:
+            _lib.check(L.rise_hip_pt_scene_create(objs, len(scene.objects), mats, len(scene.materials), spds,
+                                                  len(scene.spds), ctypes.byref(out)))
         self._h = out
         if scene.sky is not None:
             sd = SkyDesc(scene.sky.elevation_deg, scene.sky.azimuth_deg, scene.sky.turbidity, scene.sky.scale)

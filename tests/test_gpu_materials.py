@@ -87,6 +87,8 @@ def test_gpu_sky_scene_matches_oracle(cuda):
     # from shared and single-wavelength rays add the sky at their wavelengths.
     log_case("sky + GGX + glass spheres 48x36 16spp same streams")
     s, r = pt_scene.sky_spheres(n_side=6)
+    # aim higher than the scene's default so the top rows see the sky above the horizon
+    s.camera = pt_scene.Camera(position=(0.0, 4.0, 9.0), look_at=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=45.0)
     r.width, r.height, r.spp = 48, 36, 16
     g, _, st = _render(s, r)
     sm, _, cnt = oracle.render(s, r)
