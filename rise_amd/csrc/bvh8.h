@@ -55,7 +55,8 @@ static_assert(sizeof(Tri48) == 48, "Tri48 must be 48 bytes");
 // ((s&1)?-x:+x, (s&2)?-y:+y, (s&4)?-z:+z) from the node centre.  A ray whose
 // direction sign bits form octant o visits slot (k ^ (~o & 7)) at step k, which
 // is near-to-far to first order.
-constexpr int kMaxLeafTris = 4;
+constexpr int kMaxLeafTris = 4;   // leaves of the binary build (SAH)
+constexpr int kMaxLeafCount = 7;  // triangles of one wide-node leaf slot (3 bits of meta)
 constexpr int kMaxStack = 64;  // node-group stack entries; depth is checked at build
 
 }  // namespace rise_hip

@@ -51,8 +51,9 @@ struct Box {
 struct Node2 {
     Box box;
     int32_t left = -1;   // internal: children left, left + 1
-    uint32_t first = 0;  // leaf: first index into idx
+    uint32_t first = 0;  // first index into idx of the subtree's triangles
     uint32_t count = 0;  // leaf: > 0
+    uint32_t span = 0;   // triangles under the node: idx[first, first + span)
 };
 
 struct Bvh2Builder {
@@ -73,6 +74,7 @@ struct Bvh2Builder {
         nodes[ni].left = -1;
         nodes[ni].first = first;
         nodes[ni].count = count;
+        nodes[ni].span = count;
     }
 
     void build(int64_t ni, uint32_t first, uint32_t last)
@@ -191,6 +193,8 @@ struct Bvh2Builder {
         const int64_t l = alloc_pair();
         nodes[ni].left = (int32_t)l;
         nodes[ni].count = 0;
+        nodes[ni].first = first;
+        nodes[ni].span = count;
         const bool spawn = count > 65536 && spawn_budget.fetch_sub(1) > 0;
         if (spawn) {
             auto fut = std::async(std::launch::async, [this, l, first, mid] { build(l, first, mid); });
@@ -248,7 +252,8 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
         err = "triangle count out of range";
         return false;
     }
-    if (params.max_leaf < 1 || params.max_leaf > kMaxLeafTris || params.bins < 2) {
+    if (params.max_leaf < 1 || params.max_leaf > kMaxLeafTris || params.bins < 2 || params.collapse_leaf < 0 ||
+        params.collapse_leaf > kMaxLeafTris) {
         err = "invalid build parameters";
         return false;
     }
@@ -412,17 +417,21 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
                 chi[k][ns] = cn.box.hi[k];
             }
             slot_list[ns++] = s;
-            if (cn.left >= 0) {
+            // A small subtree that would become a wide node of its own is
+            // stored as one leaf of all its triangles instead.
+            const bool as_leaf = cn.left < 0 || (cn.span <= (uint32_t)params.collapse_leaf && tri_off + cn.span <= 32);
+            const uint32_t leaf_n = cn.left < 0 ? cn.count : cn.span;
+            if (!as_leaf) {
                 node.imask |= (uint8_t)(1u << s);
                 node.meta[s] = 1;
                 ++n_internal;
             } else {
-                if (cn.count < 1 || cn.count > (uint32_t)kMaxLeafTris || tri_off + cn.count > 32) {
+                if (leaf_n < 1 || leaf_n > (uint32_t)kMaxLeafCount || tri_off + leaf_n > 32) {
                     err = "internal error: leaf layout exceeds node limits";
                     return false;
                 }
-                node.meta[s] = (uint8_t)((cn.count << 5) | tri_off);
-                for (uint32_t t = 0; t < cn.count; ++t) {
+                node.meta[s] = (uint8_t)((leaf_n << 5) | tri_off);
+                for (uint32_t t = 0; t < leaf_n; ++t) {
                     const uint32_t p = b.idx[cn.first + t];
                     const float* f = tris + 9 * (size_t)p;
                     Tri48 tr;
@@ -437,7 +446,7 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
                     tr.v2[3] = 0.0f;
                     out.tris.push_back(tr);
                 }
-                tri_off += cn.count;
+                tri_off += leaf_n;
             }
         }
         node.child_base = child_base;
@@ -445,7 +454,7 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
         // Internal children: allocate in slot order.
         for (int s = 0; s < 8; ++s) {
             const int c = child_in[s];
-            if (c < 0 || !(node.imask & (1u << s))) continue;
+            if (c < 0 || !(node.imask & (1u << s))) continue;  // empty or stored as a leaf
             const uint32_t idx8 = (uint32_t)out.nodes.size();
             out.nodes.emplace_back();
             queue.push_back({kids[c], idx8, pd.level + 1});

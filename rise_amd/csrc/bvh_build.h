@@ -14,11 +14,21 @@
 #include <string>
 #include <vector>
 
+#ifndef RISE_COLLAPSE_LEAF
+#define RISE_COLLAPSE_LEAF 0
+#endif
+
 namespace rise_hip {
 
 struct BuildParams {
     int bins = 32;              // SAH bins per axis (reference: 32)
     int max_leaf = kMaxLeafTris;
+    // Collapse: a BVH2 subtree of at most this many triangles that would
+    // become a wide node of its own is stored as one leaf (<= kMaxLeafTris,
+    // so a node's leaves always fit its 32 triangle slots).  0: never.  On
+    // the 1M-triangle soup, 2 and 3 cut node visits by 3% and 10% but add 39%
+    // and 170% FP64 triangle tests, and trace 4% and 22% slower on MI355X.
+    int collapse_leaf = RISE_COLLAPSE_LEAF;
     float sah_traversal = 1.0f;  // reference mesh costs (TriangleMeshGeometryIndexed.cpp:529-534)
     float sah_intersect = 1.0f;
     int threads = 0;            // 0 = hardware concurrency
