@@ -16,7 +16,7 @@
 //   film                per-pixel FP32 Kahan sums over the chunk's samples
 // and one resolve per request at the end.  Every path carries its request
 // index, so paths of different scenes and image sizes share the queues.
-// Queues are compacted with a wave-wide ballot and one atomic per wave; every
+// Queues are compacted with wave ballots and one atomic per workgroup; every
 // count lives in device memory and every kernel is a grid-stride loop over
 // it, so a batch is a fixed sequence of kernel launches with no host
 // synchronization (capturable in a hipGraph).  A pixel's samples are summed
@@ -57,7 +57,7 @@ __constant__ float c_cie_z[81] = {RISE_CIE_Z_VALUES};
 const double h_cie_y[81] = {RISE_CIE_Y_VALUES};
 
 constexpr int kTraceBlock = 128;
-constexpr int kShadeBlock = 256;
+constexpr int kShadeBlock = 512;  // shading / binning workgroups: 8 waves share one queue atomic
 constexpr int64_t kChunkPaths = int64_t(1) << 20;   // paths per chunk (at most)
 constexpr int64_t kMaxPaths = int64_t(1) << 22;     // paths in flight per pass (at most)
 constexpr int kCounters = 6 * (kMaxPathDepth + 1);  // extension, shadow, then 4 binned-hit counts, per pass
@@ -133,9 +133,12 @@ struct ReqInfo {
 
 // One chunk of a pass: pixels [p0, p0 + np) x samples [s0, s0 + ns) of
 // request `req`, in pool slots [slot0, slot0 + np * ns) (pixel-major), and
-// film threads [film0, film0 + np).
+// film threads [film0, film0 + nf).  A pass may hold several chunks of one
+// request (contiguous, in chunk order); a chunk whose pixel block an earlier
+// chunk of the pass already covers has nf = 0, and that chunk's film
+// threads add both, in sample order.
 struct PassEntry {
-    int req, p0, np, s0, ns, slot0, film0, pad;
+    int req, p0, np, s0, ns, slot0, film0, nf;
 };
 
 // Path state, structure of arrays.  A path is one slot; its rays are kept per
@@ -173,19 +176,53 @@ struct Paths {
     unsigned long long* fetch;  // [2 * (kMaxPathDepth + 1)]: work counters of the persistent traversal kernels
 };
 
-// Wave-wide compaction: one atomic per wave, ranks from the ballot.
-__device__ inline unsigned wave_push(unsigned* counter, bool pred)
-{
-    const unsigned long long m = __ballot(pred);
-    if (m == 0ull) return 0xffffffffu;
-    const unsigned rank =
-        __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    unsigned base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
-    base = __shfl(base, leader);
-    return pred ? base + rank : 0xffffffffu;
-}
+// Workgroup-wide compaction: a ballot per wave and one atomic per workgroup,
+// so a queue counter sees one atomic per BLOCK threads (same-address atomics
+// serialize in L2, and one per wave made them the shading kernels' bottleneck).  Every thread of the block calls it
+// at the same point: the grid-stride loops run one trip count per block.
+// Consecutive calls alternate between two LDS slots, so each needs two
+// barriers.  Positions within a queue carry no meaning (each path's result is
+// summed per slot and per pixel in sample order), so the order of
+// reservations does not change any film.
+template <int BLOCK, int NB>
+struct BlockPush {
+    static constexpr int W = BLOCK / 64;
+    unsigned* lds;  // 2 * NB * (W + 1) entries
+    int flip;
+    // Reserves one entry in queue `kind` (0..NB-1; -1: none) for this thread;
+    // counters[k] counts queue k.  Returns the index, or 0xffffffff.
+    __device__ inline unsigned operator()(unsigned* const* counters, int kind)
+    {
+        unsigned* b = lds + flip * NB * (W + 1);
+        flip ^= 1;
+        const unsigned wave = threadIdx.x >> 6;
+        unsigned rank = 0;
+        for (int k = 0; k < NB; ++k) {
+            const unsigned long long m = __ballot(kind == k);
+            if (kind == k)
+                rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (__lane_id() == 0) b[k * (W + 1) + wave] = (unsigned)__popcll(m);
+        }
+        __syncthreads();
+        if (threadIdx.x < NB) {
+            unsigned* c = b + threadIdx.x * (W + 1);
+            unsigned total = 0;
+            for (int w = 0; w < W; ++w) {
+                const unsigned n = c[w];
+                c[w] = total;
+                total += n;
+            }
+            c[W] = total ? atomicAdd(counters[threadIdx.x], total) : 0u;
+        }
+        __syncthreads();
+        return kind >= 0 ? b[kind * (W + 1) + W] + b[kind * (W + 1) + wave] + rank : 0xffffffffu;
+    }
+    __device__ inline unsigned operator()(unsigned* counter, bool pred)
+    {
+        unsigned* const cs[1] = {counter};
+        return (*this)(cs, pred ? 0 : -1);
+    }
+};
 
 // Index of the last entry whose field KEY is <= x (entries sorted by KEY,
 // the first one 0).
@@ -295,8 +332,10 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const ReqInfo* __restrict__ re
     const Tri48* tris = nullptr;
     unsigned slot = 0, nn = 0, nt = 0;  // slot: lane * cap + path slot of the current item
     long long item = -1;  // -1 idle, -2 exhausted
-    for (;;) {
-        const long long got = refill(item == -1, item >= 0, next, n);
+    const int64_t grid_threads = (int64_t)gridDim.x * blockDim.x;
+    for (bool first_pass = true;; first_pass = false) {
+        // The first item of every thread is assigned statically; later ones come from the counter.
+        const long long got = first_pass ? first_item(n) : refill(item == -1, item >= 0, next, n, grid_threads);
         if (item == -1 && got != -1) {
             item = got;
             if (item >= 0) {
@@ -557,13 +596,15 @@ __device__ inline int lane_medium_object(const PtSceneView& sc, const Paths& S, 
 // scatter before the hit goes to the volume bin with its distance in hit_t;
 // otherwise the lane takes the survival weight Tr / pSurvival (:1895-1910,
 // IMedium.h:235-251) and goes on to the surface or the escape.
-__global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+__global__ __launch_bounds__(kShadeBlock) void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
+    __shared__ unsigned lds[2 * kBins * (kShadeBlock / 64 + 1)];
+    BlockPush<kShadeBlock, kBins> push{lds, 0};
     const unsigned n = S.cnt_ext[depth];
     const unsigned* q = S.queue[depth & 1];
-    unsigned* cnt = S.cnt_mat + depth;
+    unsigned* const cnt[kBins] = {S.cnt_mat + depth, S.cnt_mat + (kMaxPathDepth + 1) + depth,
+                                  S.cnt_mat + 2 * (kMaxPathDepth + 1) + depth, S.cnt_mat + 3 * (kMaxPathDepth + 1) + depth};
     const unsigned stride = gridDim.x * blockDim.x;
-    const int bins = S.vb ? kBins : 3;
     for (unsigned b = blockIdx.x * blockDim.x; b < n; b += stride) {
         const unsigned i = b + threadIdx.x;
         int kind = -1;
@@ -618,10 +659,8 @@ __global__ void k_bin(const ReqInfo* __restrict__ reqs, Paths S, int depth)
                 if (nm) S.res[li] += S.thr0[li] * (S.thr[li] * sky_radiance(sc.sky, d, lam[lane]));
             }
         }
-        for (int m = 0; m < bins; ++m) {
-            const unsigned at = wave_push(cnt + m * (kMaxPathDepth + 1), kind == m);
-            if (kind == m) S.mqueue[m][at] = code;
-        }
+        const unsigned at = push(cnt, kind);  // kind kMatVolume only when the pass has media
+        if (kind >= 0) S.mqueue[kind][at] = code;
     }
 }
 
@@ -663,6 +702,8 @@ __device__ inline void write_continuation(Paths& S, f3 no, unsigned slot, unsign
 template <int MAT>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
+    __shared__ unsigned lds[2 * (kShadeBlock / 64 + 1)];
+    BlockPush<kShadeBlock, 1> push{lds, 0};
     const unsigned n = S.cnt_mat[MAT * (kMaxPathDepth + 1) + depth];
     const unsigned* qin = S.mqueue[MAT];
     unsigned* qout = S.queue[(depth + 1) & 1];
@@ -802,7 +843,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                     }
                 }
             }
-            const unsigned si = wave_push(cnt_sh, has);
+            const unsigned si = push(cnt_sh, has);
             if (has) {
                 S.sox[si] = sorg.x;
                 S.soy[si] = sorg.y;
@@ -905,7 +946,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                     }
                 }
             }
-            const unsigned qi = wave_push(cnt_next, cont);
+            const unsigned qi = push(cnt_next, cont);
             if (cont) write_continuation(S, surface_origin(h, cdir), slot, cw, qout + qi, cdir, cpdf, cimp, cst, ist, vbn);
         } else {
             // A dielectric: for a lane already on its own, and for each wavelength of
@@ -930,7 +971,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                         cont = fabsf(t) <= 1e6f;
                     }
                 }
-                const unsigned qi = wave_push(cnt_next, cont);
+                const unsigned qi = push(cnt_next, cont);
                 if (cont)
                     write_continuation(S, surface_origin(h, nd), slot, (unsigned)w, qout + qi, nd, 0.0f, imp * fabsf(wgt),
                                        nb, nst, vbn);
@@ -996,6 +1037,8 @@ __device__ inline f3 hg_sample(f3 wo, float g, float u1, float u2)
 // (:1862-1881), which leaves the importance unchanged.
 __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
+    __shared__ unsigned lds[2 * (kShadeBlock / 64 + 1)];
+    BlockPush<kShadeBlock, 1> push{lds, 0};
     const unsigned n = S.cnt_mat[kMatVolume * (kMaxPathDepth + 1) + depth];
     const unsigned* qin = S.mqueue[kMatVolume];
     unsigned* qout = S.queue[(depth + 1) & 1];
@@ -1084,7 +1127,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restric
                 has = v > 0.0f;
             }
         }
-        const unsigned si = wave_push(cnt_sh, has);
+        const unsigned si = push(cnt_sh, has);
         if (has) {
             S.sox[si] = p.x;
             S.soy[si] = p.y;
@@ -1120,7 +1163,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restric
                 }
             }
         }
-        const unsigned qi = wave_push(cnt_next, cont);
+        const unsigned qi = push(cnt_next, cont);
         if (cont) write_continuation(S, p, slot, lane, qout + qi, nd, pp, imp, bst, ist, vbn + 1u);
     }
 }
@@ -1249,8 +1292,10 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
     bool media = false;
     unsigned nn = 0, nt = 0;
     long long item = -1;
-    for (;;) {
-        const long long got = refill(item == -1, item >= 0, next, n);
+    const int64_t grid_threads = (int64_t)gridDim.x * blockDim.x;
+    for (bool first_pass = true;; first_pass = false) {
+        // The first item of every thread is assigned statically; later ones come from the counter.
+        const long long got = first_pass ? first_item(n) : refill(item == -1, item >= 0, next, n, grid_threads);
         if (item == -1 && got != -1) {
             item = got;
             if (item >= 0 && S.sinfo && (S.sinfo[item] & kShadowInScatter)) {
@@ -1323,7 +1368,8 @@ __global__ void k_film(const ReqInfo* __restrict__ reqs, const PassEntry* __rest
                        Paths S)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_film; i += gridDim.x * blockDim.x) {
-        const PassEntry E = ent[find_entry<&PassEntry::film0>(ent, n_ent, i)];
+        const int e0 = find_entry<&PassEntry::film0>(ent, n_ent, i);
+        const PassEntry E = ent[e0];
         const ReqInfo& R = reqs[E.req];
         const int local = i - E.film0;
         const int pix = E.p0 + local;
@@ -1334,8 +1380,13 @@ __global__ void k_film(const ReqInfo* __restrict__ reqs, const PassEntry* __rest
             q[k] = R.fsq[3 * pix + k];
             qc[k] = R.fsqc[3 * pix + k];
         }
-        for (int j = 0; j < E.ns; ++j) {
-            const int slot = E.slot0 + local * E.ns + j;
+        // This chunk and the later chunks of the same pixel block in this pass,
+        // in chunk (= sample) order.
+        for (int e = e0; e < n_ent && ent[e].req == E.req; ++e) {
+          const PassEntry F = ent[e];
+          if (F.p0 != E.p0) continue;
+          for (int j = 0; j < F.ns; ++j) {
+            const int slot = F.slot0 + local * F.ns + j;
             float lam[kHwss];
             wavelengths(S.lambda0[slot], R.lambda_begin, R.lambda_end, lam);
             float X = 0.0f, Y = 0.0f, Z = 0.0f;
@@ -1353,6 +1404,7 @@ __global__ void k_film(const ReqInfo* __restrict__ reqs, const PassEntry* __rest
                 kahan_add(s[k], c[k], v[k]);
                 kahan_add(q[k], qc[k], v[k] * v[k]);
             }
+          }
         }
         for (int k = 0; k < 3; ++k) {
             R.fsum[3 * pix + k] = s[k];
@@ -1583,6 +1635,59 @@ void chunks_of(const rise_hip_render_desc& D, int req, std::vector<PassEntry>& o
         }
 }
 
+// The chunks still to render of one request.
+struct PassSource {
+    const std::vector<PassEntry>* chunks;
+    size_t* next;
+};
+
+// Fills one pass from the sources: round-robin, each round taking the next
+// chunk of every source in turn while it fits in `cap` paths and the pass has
+// fewer than `max_entries` chunks, so every source gets one chunk before any
+// gets a second.  A source's chunks are emitted together in chunk order;
+// film threads go to the first chunk of each pixel block (PassEntry).
+// Returns the paths and film threads of the pass; `taken[k]` = chunks of
+// source k in it.
+void fill_pass(const std::vector<PassSource>& src, int64_t cap, size_t max_entries, std::vector<PassEntry>& out,
+               int64_t& paths, int64_t& film, std::vector<int>& taken)
+{
+    std::vector<std::vector<PassEntry>> per(src.size());
+    taken.assign(src.size(), 0);
+    paths = 0;
+    size_t count = 0;
+    for (bool added = true; added;) {
+        added = false;
+        for (size_t k = 0; k < src.size() && count < max_entries; ++k) {
+            const std::vector<PassEntry>& ch = *src[k].chunks;
+            size_t& nx = *src[k].next;
+            if (nx >= ch.size()) continue;
+            const int64_t pp = (int64_t)ch[nx].np * ch[nx].ns;
+            if (paths + pp > cap) continue;
+            per[k].push_back(ch[nx]);
+            paths += pp;
+            ++nx;
+            ++count;
+            ++taken[k];
+            added = true;
+        }
+    }
+    int64_t slot = 0;
+    film = 0;
+    for (size_t k = 0; k < src.size(); ++k) {
+        const size_t first = out.size();
+        for (PassEntry e : per[k]) {
+            bool covered = false;
+            for (size_t j = first; j < out.size(); ++j) covered = covered || out[j].p0 == e.p0;
+            e.slot0 = (int)slot;
+            e.film0 = (int)film;
+            e.nf = covered ? 0 : e.np;
+            slot += (int64_t)e.np * e.ns;
+            film += e.nf;
+            out.push_back(e);
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1698,30 +1803,21 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     const int64_t want = tn.max_paths > 0 ? std::min<int64_t>(tn.max_paths, kMaxPaths) : kMaxPaths;
     const int64_t cap = std::max<int64_t>(max_chunk, std::min<int64_t>(want, total_paths));
 
-    // Passes: each takes the next chunk of every request, in request order, while they fit.
+    // Passes: round-robin over the requests (fill_pass) until every chunk is placed.
     std::vector<PassEntry> entries;
     std::vector<int> pass_first(1, 0), pass_paths, pass_film;
     std::vector<size_t> next(n, 0);
+    std::vector<PassSource> src(n);
+    for (size_t k = 0; k < n; ++k) src[k] = PassSource{&chunks[k], &next[k]};
     size_t left = 0;
     for (const auto& c : chunks) left += c.size();
+    std::vector<int> taken;
     while (left > 0) {
-        int64_t paths = 0;
-        int64_t film = 0;
+        int64_t paths = 0, film = 0;
         const size_t first = entries.size();
-        for (size_t k = 0; k < n; ++k) {
-            if (next[k] >= chunks[k].size()) continue;
-            PassEntry e = chunks[k][next[k]];
-            const int64_t pp = (int64_t)e.np * e.ns;
-            if (paths + pp > cap) continue;
-            e.slot0 = (int)paths;
-            e.film0 = (int)film;
-            paths += pp;
-            film += e.np;
-            entries.push_back(e);
-            ++next[k];
-            --left;
-        }
+        fill_pass(src, cap, SIZE_MAX, entries, paths, film, taken);
         if (entries.size() == first) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "internal error: empty pass");
+        left -= entries.size() - first;
         pass_first.push_back((int)entries.size());
         pass_paths.push_back((int)paths);
         pass_film.push_back((int)film);
@@ -1800,7 +1896,7 @@ void launch_pass(const ReqInfo* d_info, Paths S, const PassEntry* ent, int n_ent
     for (int d = 0; d < sh.bounces; ++d) {
         void* args[] = {(void*)&d_info, &S, &d};
         hipLaunchKernel((const void*)sh.ext_fn, dim3(sh.ge), dim3(sh.block), args, 0, stream);
-        hipLaunchKernelGGL(k_bin, dim3(grid_for(4 * sh.cap, 256, 2048)), dim3(256), 0, stream, d_info, S, d);
+        hipLaunchKernelGGL(k_bin, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
         if (sh.mat_mask & (1u << kMatLambert))
             hipLaunchKernelGGL(k_shade<kMatLambert>, dim3(gs), dim3(kShadeBlock), 0, stream, d_info, S, d);
         if (sh.mat_mask & (1u << kMatGgx))
@@ -1986,6 +2082,7 @@ namespace {
 
 constexpr int kRing = 4;          // passes in flight
 constexpr int kServerSlots = 256; // admitted requests at most
+constexpr int kPassEntries = 1024; // chunks per server pass at most
 
 struct ServerReq {
     uint64_t ticket = 0;
@@ -2026,7 +2123,7 @@ struct rise_hip_server {
     unsigned* h_counters = nullptr;  // pinned copies of the same
     ReqInfo* d_info = nullptr;       // kServerSlots
     ReqInfo* h_info = nullptr;       // pinned staging, kServerSlots
-    PassEntry* d_ent = nullptr;      // kRing * kServerSlots
+    PassEntry* d_ent = nullptr;      // kRing * kPassEntries
     PassEntry* h_ent = nullptr;      // pinned staging
     hipEvent_t ev[kRing] = {};
     bool busy[kRing] = {};
@@ -2136,30 +2233,29 @@ rise_hip_status server_launch(rise_hip_server* sv)
     for (;;) {
         const int k = sv->ring_next;
         if (sv->busy[k]) return RISE_HIP_OK;
-        PassEntry* ent = sv->h_ent + (size_t)k * kServerSlots;
-        int n_ent = 0;
+        PassEntry* ent = sv->h_ent + (size_t)k * kPassEntries;
         int64_t paths = 0, film = 0, tris = 0;
         int bounces = 1, depth = 1;
         unsigned kinds = 0;
         std::vector<ServerReq*> finishing;
-        for (ServerReq& r : sv->active) {
-            if (r.next >= r.chunks.size()) continue;
-            PassEntry e = r.chunks[r.next];
-            const int64_t pp = (int64_t)e.np * e.ns;
-            if (paths + pp > sv->cap) continue;
-            e.slot0 = (int)paths;
-            e.film0 = (int)film;
-            paths += pp;
-            film += e.np;
-            ent[n_ent++] = e;
-            r.next++;
+        std::vector<PassSource> src;
+        src.reserve(sv->active.size());
+        for (ServerReq& r : sv->active) src.push_back(PassSource{&r.chunks, &r.next});
+        std::vector<PassEntry> pass;
+        std::vector<int> taken;
+        fill_pass(src, sv->cap, kPassEntries, pass, paths, film, taken);
+        const int n_ent = (int)pass.size();
+        if (n_ent == 0) return RISE_HIP_OK;
+        std::copy(pass.begin(), pass.end(), ent);
+        for (size_t i = 0; i < sv->active.size(); ++i) {
+            if (!taken[i]) continue;
+            ServerReq& r = sv->active[i];
             bounces = std::max(bounces, r.hits);
             depth = std::max(depth, r.depth);
             tris = std::max(tris, r.tris);
             kinds |= r.kinds;
             if (r.next == r.chunks.size()) finishing.push_back(&r);
         }
-        if (n_ent == 0) return RISE_HIP_OK;
         PassShape sh{};
         int stack = 0, block = 128;
         bool persist = true;
@@ -2168,7 +2264,7 @@ rise_hip_status server_launch(rise_hip_server* sv)
         sh.cap = sv->cap;
         sh.bounces = bounces;
         sh.mat_mask = kinds;
-        PassEntry* d_ent = sv->d_ent + (size_t)k * kServerSlots;
+        PassEntry* d_ent = sv->d_ent + (size_t)k * kPassEntries;
         hipError_t e = hipMemcpyAsync(d_ent, ent, (size_t)n_ent * sizeof(PassEntry), hipMemcpyHostToDevice, sv->stream);
         if (e != hipSuccess) return fail_hip(e, "pass table");
         unsigned* cnt = sv->d_counters + (size_t)k * kCounters;
@@ -2240,10 +2336,10 @@ extern "C" rise_hip_status rise_hip_server_create(const rise_hip_server_config* 
     if (e == hipSuccess) e = hipMalloc(&sv->work, need);
     if (e == hipSuccess) e = hipMalloc((void**)&sv->d_counters, sizeof(unsigned) * kCounters * kRing);
     if (e == hipSuccess) e = hipMalloc((void**)&sv->d_info, sizeof(ReqInfo) * kServerSlots);
-    if (e == hipSuccess) e = hipMalloc((void**)&sv->d_ent, sizeof(PassEntry) * kServerSlots * kRing);
+    if (e == hipSuccess) e = hipMalloc((void**)&sv->d_ent, sizeof(PassEntry) * kPassEntries * kRing);
     if (e == hipSuccess) e = hipHostMalloc((void**)&sv->h_counters, sizeof(unsigned) * kCounters * kRing, 0);
     if (e == hipSuccess) e = hipHostMalloc((void**)&sv->h_info, sizeof(ReqInfo) * kServerSlots, 0);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->h_ent, sizeof(PassEntry) * kServerSlots * kRing, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->h_ent, sizeof(PassEntry) * kPassEntries * kRing, 0);
     for (int k = 0; k < kRing && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&sv->ev[k], hipEventDisableTiming);
     if (e != hipSuccess) {
         server_free(sv);

@@ -122,8 +122,10 @@ __global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __r
     ClosestState s;
     long long ray = -1;  // -1 idle, -2 exhausted
     unsigned nn = 0, nt = 0;
-    for (;;) {
-        const long long got = refill(ray == -1, ray >= 0, next, n);
+    const int64_t grid_threads = (int64_t)gridDim.x * blockDim.x;
+    for (bool first_pass = true;; first_pass = false) {
+        // The first item of every thread is assigned statically; later ones come from the counter.
+        const long long got = first_pass ? first_item(n) : refill(ray == -1, ray >= 0, next, n, grid_threads);
         if (ray == -1 && got != -1) {
             ray = got;
             if (ray >= 0) {
@@ -165,8 +167,10 @@ __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restr
     AnyState s;
     long long ray = -1;
     unsigned nn = 0, nt = 0;
-    for (;;) {
-        const long long got = refill(ray == -1, ray >= 0, next, n);
+    const int64_t grid_threads = (int64_t)gridDim.x * blockDim.x;
+    for (bool first_pass = true;; first_pass = false) {
+        // The first item of every thread is assigned statically; later ones come from the counter.
+        const long long got = first_pass ? first_item(n) : refill(ray == -1, ray >= 0, next, n, grid_threads);
         if (ray == -1 && got != -1) {
             ray = got;
             if (ray >= 0) {
