@@ -58,6 +58,9 @@ const double h_cie_y[81] = {RISE_CIE_Y_VALUES};
 
 constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 512;  // shading / binning workgroups: 8 waves share one queue atomic
+#ifndef RISE_SHADE_GRID
+#define RISE_SHADE_GRID 2048      // workgroups of the grid-stride shading / binning launches (at most)
+#endif
 constexpr int64_t kChunkPaths = int64_t(1) << 20;   // paths per chunk (at most)
 constexpr int64_t kMaxPaths = int64_t(1) << 22;     // paths in flight per pass (at most)
 constexpr int kCounters = 6 * (kMaxPathDepth + 1);  // extension, shadow, then 4 binned-hit counts, per pass
@@ -1887,7 +1890,7 @@ struct PassShape {
 void launch_pass(const ReqInfo* d_info, Paths S, const PassEntry* ent, int n_ent, int paths, int film,
                  const PassShape& sh, hipStream_t stream)
 {
-    const unsigned gs = grid_for(4 * sh.cap, kShadeBlock, 2048);
+    const unsigned gs = grid_for(4 * sh.cap, kShadeBlock, RISE_SHADE_GRID);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, S.cnt_ext, kCounters);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<unsigned*>(S.fetch),
                        4 * (kMaxPathDepth + 1));

@@ -2,6 +2,7 @@
 // Built by tests (hipcc host compile, linked against librise_hip.so).
 //   adapter_demo raycast   closest hit for a grid of rays through a triangle
 //   adapter_demo render    a tiny spectral render of a lit floor
+//   adapter_demo scene F   a scene file loaded by LoadAsciiScene, rendered at 64x64, 4 spp
 // Prints "ok ..." and exits 0 on success.
 #include "rise_hip_adapters.hpp"
 
@@ -103,10 +104,33 @@ static int render()
     return 0;
 }
 
+static int scene(const char* path)
+{
+    HipSpectralRasterizer r;
+    rise_hip_render_desc d{};
+    if (!r.LoadAsciiScene(path, &d)) return 1;
+    d.width = d.height = 64;
+    d.spp = d.spp_total = 4;
+    float* film = nullptr;
+    if (hipMalloc((void**)&film, 64 * 64 * 3 * 4) != hipSuccess) return 2;
+    rise_hip_render_stats st{};
+    if (!r.RasterizeScene(d, film, nullptr, &st)) return 3;
+    std::vector<float> h(64 * 64 * 3);
+    hipMemcpy(h.data(), film, h.size() * 4, hipMemcpyDeviceToHost);
+    hipFree(film);
+    double y = 0;
+    for (int i = 0; i < 64 * 64; ++i) y += h[3 * i + 1];
+    y /= 64 * 64;
+    if (!(y > 0) || st.paths != 64 * 64 * 4) return 4;
+    std::printf("ok scene meanY %.4f paths %lld\n", y, (long long)st.paths);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc > 1 && std::strcmp(argv[1], "raycast") == 0) return raycast();
     if (argc > 1 && std::strcmp(argv[1], "render") == 0) return render();
-    std::fprintf(stderr, "usage: adapter_demo raycast|render\n");
+    if (argc > 2 && std::strcmp(argv[1], "scene") == 0) return scene(argv[2]);
+    std::fprintf(stderr, "usage: adapter_demo raycast|render|scene <file>\n");
     return 64;
 }

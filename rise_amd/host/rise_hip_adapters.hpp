@@ -16,6 +16,7 @@
 
 #include "rise_hip.h"
 #include "rise_hip_render.h"
+#include "rise_hip_scene_file.h"
 
 #include <cstdint>
 #include <cstdio>
@@ -183,6 +184,21 @@ public:
                    "PrepareForRendering"))
             return false;
         return !has_sky_ || check(rise_hip_pt_scene_set_sky(scene_, &sky_), "PrepareForRendering (sky)");
+    }
+
+    // Job::LoadAsciiSceneViaCst (Job.h:2843): reads a native scene file and
+    // makes its scene resident; *desc receives the render its camera, film
+    // and rasterizer chunks describe.  Replaces anything added before.
+    bool LoadAsciiScene(const char* path, rise_hip_render_desc* desc)
+    {
+        rise_hip_scene_file* f = nullptr;
+        if (!check(rise_hip_scene_file_parse(path, &f), "LoadAsciiScene")) return false;
+        rise_hip_pt_scene_release(scene_);
+        scene_ = nullptr;
+        bool ok = check(rise_hip_scene_file_upload(f, &scene_), "LoadAsciiScene (upload)");
+        if (ok && desc) ok = check(rise_hip_scene_file_get_render_desc(f, desc), "LoadAsciiScene (render)");
+        rise_hip_scene_file_release(f);
+        return ok;
     }
 
     rise_hip_pt_scene* Scene() const { return scene_; }

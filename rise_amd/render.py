@@ -265,6 +265,16 @@ class PathTracingSpectralRasterizer:
             _lib.check(L.rise_hip_pt_scene_set_sky(self._h, ctypes.byref(sd)))
         self._keep = []
 
+    @classmethod
+    def from_handle(cls, handle, scene: Scene) -> "PathTracingSpectralRasterizer":
+        """Wraps a scene made resident elsewhere (rise_hip_scene_file_upload);
+        `scene` supplies the camera."""
+        self = cls.__new__(cls)
+        self.scene = scene
+        self._keep = []
+        self._h = handle
+        return self
+
     def rasterize_scene(self, settings: RenderSettings, spp: int | None = None, spp_offset: int = 0,
                         spp_total: int | None = None, want_sq: bool = False, stream=None, stats: bool = True,
                         film=None, film_sq=None):
