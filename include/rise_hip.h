@@ -73,7 +73,7 @@ typedef struct rise_hip_scene_stats {
     int32_t depth;          /* wide-node levels */
     int32_t node_bytes;     /* 80 */
     int32_t tri_bytes;      /* 48 */
-    int32_t reserved;
+    int32_t bvh_source;     /* 0: built here; 1: from a .risemesh BVH2 cache; 2: from a scene cache */
     double build_ms;        /* host build time */
     double sah_cost;        /* SAH cost of the binary build / root area */
     int64_t device_bytes;   /* HBM held by the scene */

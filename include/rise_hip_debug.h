@@ -12,6 +12,7 @@
 #define RISE_HIP_DEBUG_H
 
 #include "rise_hip.h"
+#include "rise_hip_risemesh.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -25,6 +26,14 @@ rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t n_tris, con
                                           const float* tmax, int64_t n, int32_t mode, int32_t* prim_id,
                                           float* t, uint8_t* occluded, uint64_t* counters,
                                           rise_hip_scene_stats* stats);
+
+/* The same over a .risemesh file's triangles (ids = file order), the 8-wide
+ * tree collapsed from the file's BVH2 cache when it is valid
+ * (stats->bvh_source = 1), else built. */
+rise_hip_status rise_hip_debug_trace_host_risemesh(const rise_hip_risemesh* mesh, const float* rays_soa,
+                                                   const float* tmax, int64_t n, int32_t mode, int32_t* prim_id,
+                                                   float* t, uint8_t* occluded, uint64_t* counters,
+                                                   rise_hip_scene_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -27,12 +27,16 @@ extern "C" {
 #endif
 
 /* A step spectrum: n equal bins over [lambda_begin, lambda_end]
- * (SpectralPacket, Utilities/Color/SpectralPacket.h:109-124, 285-297). */
+ * (SpectralPacket, Utilities/Color/SpectralPacket.h:109-124, 285-297).
+ * With n = RISE_HIP_SPD_SIGMOID it is instead an RGB-uplifted sigmoid
+ * spectrum (RGBSigmoidPolynomial, see rise_hip_rgb_uplift.h): amplitudes =
+ * {c0, c1, c2, scale}, range fixed at [380, 780] nm. */
+#define RISE_HIP_SPD_SIGMOID (-1)
 typedef struct rise_hip_spd_desc {
     float lambda_begin;
     float lambda_end;
     int32_t n;
-    const float* amplitudes; /* n values */
+    const float* amplitudes; /* n values (sigmoid: 4) */
 } rise_hip_spd_desc;
 
 typedef enum rise_hip_material_type {

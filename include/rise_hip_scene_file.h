@@ -9,12 +9,17 @@
  * parsers they follow:
  *   spectral_painter (:1086-1170)          -> a spectrum (SpectralPacket of the
  *                                            control points, Job.cpp:1557-1582)
+ *   uniformcolor_painter (:1005-1040)      -> the RGB-uplifted sigmoid spectrum
+ *                                            (colorspace Rec709RGB_Linear, RISERGB
+ *                                            or sRGB; see rise_hip_rgb_uplift.h)
  *   lambertian_material (:3090)            lambertian_luminaire_material (:3373)
  *   ggx_material (:3826; conductor Fresnel, isotropic, not emissive)
  *   dielectric_material (:3200; smooth: scattering >= 1e6)
  *   homogeneous_medium (:6142; RGB coefficients, phase "hg g" or "isotropic")
  *   hosek_wilkie_skylight (:6931; create_sun FALSE)
  *   rawmesh2_geometry (:5104; Geometry/TriangleMeshLoaderRAW2.cpp format)
+ *   risemesh_geometry (:5137; .risemesh v1-v5, its BVH2 cache used for the
+ *                      object's subtree; see rise_hip_risemesh.h)
  *   clippedplane_geometry (:5004; as two triangles)
  *   standard_object (:6411; position / orientation / scale, interior_medium,
  *                    casts_shadows)

@@ -96,7 +96,9 @@ class RefMesh:
     """The reference's TriangleMeshGeometryIndexed (BVH4 + float filter +
     double certifier) built from its own sources (oracle/_ref)."""
 
-    def __init__(self, tris: np.ndarray, variant: str = "ieee"):
+    def __init__(self, tris: np.ndarray | None, variant: str = "ieee", risemesh: str | None = None):
+        """Builds the mesh from `tris`, or (tris None) loads the .risemesh file
+        `risemesh` with the reference's Deserialize."""
         name = "libriseref_ieee.so" if variant == "ieee" else "libriseref.so"
         L = _load(REF_DIR / name)
         L.rise_ref_mesh_create.argtypes = [_P, _I64]
@@ -104,9 +106,29 @@ class RefMesh:
         L.rise_ref_mesh_release.argtypes = [_P]
         L.rise_ref_closest.argtypes = [_P, _P, _I64, _P, _P, _P, ctypes.c_int]
         L.rise_ref_any.argtypes = [_P, _P, _P, _I64, _P, ctypes.c_int]
+        L.rise_ref_risemesh_save.argtypes = [_P, ctypes.c_char_p]
+        L.rise_ref_risemesh_save.restype = ctypes.c_int
+        L.rise_ref_risemesh_load.argtypes = [ctypes.c_char_p]
+        L.rise_ref_risemesh_load.restype = _P
+        L.rise_ref_bvh2_nodes.argtypes = [_P]
+        L.rise_ref_bvh2_nodes.restype = ctypes.c_uint
         self._L = L
-        self._tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
-        self._h = L.rise_ref_mesh_create(self._tris.ctypes.data, self._tris.shape[0])
+        if tris is None:
+            self._tris = None
+            self._h = L.rise_ref_risemesh_load(str(risemesh).encode())
+        else:
+            self._tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+            self._h = L.rise_ref_mesh_create(self._tris.ctypes.data, self._tris.shape[0])
+
+    def loaded_bvh2_nodes(self) -> int:
+        """BVH2 node count of a mesh loaded from a file (its cache's count when
+        the reference took the cache)."""
+        return int(self._L.rise_ref_bvh2_nodes(self._h))
+
+    def save_risemesh(self, path) -> None:
+        """TriangleMeshGeometryIndexed::Serialize to a .risemesh file."""
+        if self._L.rise_ref_risemesh_save(self._h, str(path).encode()) != 0:
+            raise RuntimeError(f"reference Serialize to {path} failed")
 
     def closest(self, rays: np.ndarray, threads: int = 0):
         rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -189,7 +211,8 @@ def _ro_scene(scene, settings, spp_total, keep):
     for k, s in enumerate(scene.spds):
         a = np.ascontiguousarray(s.amp, np.float64)
         keep.append(a)
-        spds[k] = _RoSpd(float(np.float32(s.begin)), float(np.float32(s.end)), a.size, a.ctypes.data)
+        n = -1 if getattr(s, "sigmoid", False) else a.size
+        spds[k] = _RoSpd(float(np.float32(s.begin)), float(np.float32(s.end)), n, a.ctypes.data)
     mats = (_RoMaterial * len(scene.materials))()
     for k, m in enumerate(scene.materials):
         mats[k] = _RoMaterial(m.kind, m.reflectance, m.emission, float(np.float32(m.emission_scale)), m.specular,

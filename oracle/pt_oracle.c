@@ -87,6 +87,15 @@ double rise_oracle_cie_y_integral(double a, double b)
 /* SpectralPacket::ValueAtNM (SpectralPacket.h:285-297): step bins. */
 static double spd_value(const ro_spd* s, double nm)
 {
+    if (s->n < 0) {
+        /* Sigmoid-polynomial spectrum, RGBSigmoidPolynomial::Eval
+         * (Utilities/Color/RGBSigmoidPolynomial.h:55-69): amp = c0, c1, c2,
+         * scale; zero outside [380, 780] nm. */
+        if (nm < 380.0 || nm > 780.0) return 0.0;
+        const double t = (nm - 380.0) / 400.0;
+        const double x = s->amp[0] * t * t + s->amp[1] * t + s->amp[2];
+        return s->amp[3] * (0.5 + x / (2.0 * sqrt(1.0 + x * x)));
+    }
     if (nm < s->lambda_begin || nm > s->lambda_end) return 0.0;
     const double delta = (s->lambda_end - s->lambda_begin) / (double)s->n;
     int idx = (int)((nm - s->lambda_begin) / delta);
@@ -332,8 +341,14 @@ static int prepare(prep* P, const ro_scene* s)
          * exitance * area). */
         const ro_spd* e = &s->spds[m->emit_spd];
         double mean = 0;
-        for (int k = 0; k < e->n; ++k) mean += e->amp[k];
-        mean /= (double)e->n;
+        if (e->n < 0) {
+            /* sigmoid spectrum: mean over the 81 CIE sample wavelengths */
+            for (int k = 0; k < 81; ++k) mean += spd_value(e, 380.0 + 5.0 * k);
+            mean /= 81.0;
+        } else {
+            for (int k = 0; k < e->n; ++k) mean += e->amp[k];
+            mean /= (double)e->n;
+        }
         L.weight = mean * m->emit_scale * total;
         if (!(L.weight > 0)) continue;
         P->lum_of_object[o] = P->n_lums;

@@ -54,8 +54,8 @@ extern "C" {
 
 typedef struct ro_spd {
     double lambda_begin, lambda_end; /* SpectralPacket range */
-    int32_t n;                       /* number of step bins */
-    const double* amp;               /* n amplitudes */
+    int32_t n;                       /* number of step bins; -1: sigmoid polynomial */
+    const double* amp;               /* n amplitudes (sigmoid: c0, c1, c2, scale) */
 } ro_spd;
 
 typedef struct ro_material {

@@ -22,6 +22,7 @@
 
 #include "Geometry/TriangleMeshGeometryIndexed.h"
 #include "Intersection/RayIntersectionGeometric.h"
+#include "Utilities/MemoryBuffer.h"
 
 #include <atomic>
 #include <condition_variable>
@@ -38,6 +39,12 @@ namespace {
 
 struct RefMesh {
     Implementation::TriangleMeshGeometryIndexed* mesh;
+};
+
+// Reads the protected BVH pointer, to tell a loaded cache from a rebuild.
+struct MeshProbe : public Implementation::TriangleMeshGeometryIndexed {
+    MeshProbe() : Implementation::TriangleMeshGeometryIndexed(false, true) {}
+    unsigned bvh2_nodes() const { return pPtrBVH ? (unsigned)pPtrBVH->numNodes() : 0u; }
 };
 
 // Persistent worker pool: RISE renders with long-lived pool threads
@@ -213,6 +220,38 @@ void rise_ref_any(void* handle, const float* rays, const float* tmax, int64_t n,
         if ((how_far != RISE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
         occluded[i] = h->mesh->IntersectRay_IntersectionOnly(Ray(o, dn), how_far2, true, true) ? 1 : 0;
     });
+}
+
+// Writes the mesh with TriangleMeshGeometryIndexed::Serialize (.risemesh v5,
+// with its BVH2 cache block).  Returns 0 on success.
+int rise_ref_risemesh_save(void* handle, const char* path)
+{
+    RefMesh* h = (RefMesh*)handle;
+    Implementation::MemoryBuffer* mb = new Implementation::MemoryBuffer();
+    h->mesh->Serialize(*mb);
+    const bool ok = mb->DumpToFileToCursor(path);
+    mb->release();
+    return ok ? 0 : 1;
+}
+
+// Loads a .risemesh file with TriangleMeshGeometryIndexed::Deserialize (which
+// takes the BVH2 cache when it is valid).  Triangle ids in ray queries come
+// from the texture coordinate's u, as for rise_ref_mesh_create.
+void* rise_ref_risemesh_load(const char* path)
+{
+    RefMesh* h = new RefMesh;
+    h->mesh = new MeshProbe();
+    Implementation::MemoryBuffer* mb = new Implementation::MemoryBuffer(path);
+    h->mesh->Deserialize(*mb);
+    mb->release();
+    return h;
+}
+
+// Nodes of the BVH2 of a mesh from rise_ref_risemesh_load (equal to the
+// file's BVH2 block when Deserialize took the cache).
+unsigned rise_ref_bvh2_nodes(void* handle)
+{
+    return static_cast<MeshProbe*>(((RefMesh*)handle)->mesh)->bvh2_nodes();
 }
 
 }  // extern "C"

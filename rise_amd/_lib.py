@@ -49,7 +49,7 @@ class SceneStats(ctypes.Structure):
         ("depth", ctypes.c_int32),
         ("node_bytes", ctypes.c_int32),
         ("tri_bytes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("bvh_source", ctypes.c_int32),
         ("build_ms", ctypes.c_double),
         ("sah_cost", ctypes.c_double),
         ("device_bytes", ctypes.c_int64),

@@ -14,4 +14,9 @@ rise_hip_status fail(rise_hip_status s, const std::string& msg);
 rise_hip_status fail_hip(hipError_t e, const char* what);
 void clear_error();
 
+struct WideBVH;
+// Uploads a built wide BVH as a ray-cast scene (rise_hip_scene_upload's
+// device part); bvh_source goes to the scene's statistics.
+rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene);
+
 }  // namespace rise_hip

@@ -54,6 +54,7 @@ struct Node2 {
     uint32_t first = 0;  // first index into idx of the subtree's triangles
     uint32_t count = 0;  // leaf: > 0
     uint32_t span = 0;   // triangles under the node: idx[first, first + span)
+    uint8_t axis = 0;    // internal: split axis
 };
 
 struct Bvh2Builder {
@@ -182,6 +183,7 @@ struct Bvh2Builder {
             const float e0 = cbox.hi[0] - cbox.lo[0], e1 = cbox.hi[1] - cbox.lo[1], e2 = cbox.hi[2] - cbox.lo[2];
             if (e1 > e0) a = 1;
             if (e2 > std::max(e0, e1)) a = 2;
+            best_axis = a;
             mid = first + count / 2;
             std::nth_element(idx.begin() + first, idx.begin() + mid, idx.begin() + last,
                              [&](uint32_t x, uint32_t y) {
@@ -195,6 +197,7 @@ struct Bvh2Builder {
         nodes[ni].count = 0;
         nodes[ni].first = first;
         nodes[ni].span = count;
+        nodes[ni].axis = (uint8_t)(best_axis >= 0 ? best_axis : 0);
         const bool spawn = count > 65536 && spawn_budget.fetch_sub(1) > 0;
         if (spawn) {
             auto fut = std::async(std::launch::async, [this, l, first, mid] { build(l, first, mid); });
@@ -243,11 +246,10 @@ void decode_child_box(const Node8& n, int s, float lo[3], float hi[3])
     hi[2] = std::fmaf((float)n.qhiz[s], sz, n.pz);
 }
 
-bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const BuildParams& params,
-                    WideBVH& out, std::string& err)
+namespace {
+
+bool check_params(const BuildParams& params, int64_t n, std::string& err)
 {
-    const auto t0 = std::chrono::steady_clock::now();
-    out = WideBVH();
     if (n < 0 || n > (int64_t)0x7fffffff) {
         err = "triangle count out of range";
         return false;
@@ -257,17 +259,22 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
         err = "invalid build parameters";
         return false;
     }
-    if (n == 0) {
-        // Empty scene: a single node with no children; every ray misses.
-        NodeSlot root;
-        std::memset(&root, 0, sizeof(root));
-        out.nodes.push_back(root);
-        out.stats.nodes = 1;
-        out.stats.depth = 1;
-        return true;
-    }
+    return true;
+}
 
-    Bvh2Builder b(params);
+void empty_scene(WideBVH& out)
+{
+    // Empty scene: a single node with no children; every ray misses.
+    NodeSlot root;
+    std::memset(&root, 0, sizeof(root));
+    out.nodes.push_back(root);
+    out.stats.nodes = 1;
+    out.stats.depth = 1;
+}
+
+// Primitive boxes, padded outward, and centroids; the scene bounds.
+bool prepare_prims(const float* tris, int64_t n, Bvh2Builder& b, WideBVH& out, std::string& err)
+{
     b.pbox.resize((size_t)n);
     b.cent.resize((size_t)n * 3);
     b.idx.resize((size_t)n);
@@ -309,20 +316,329 @@ bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const Buil
         out.scene_lo[k] = scene.lo[k];
         out.scene_hi[k] = scene.hi[k];
     }
+    return true;
+}
 
+int threads_of(const BuildParams& params)
+{
+    return params.threads > 0 ? params.threads : (int)std::thread::hardware_concurrency();
+}
+
+}  // namespace
+
+// Checks a binary tree over `count` primitives (from a file: anything may be
+// wrong): node 0 the root, children in range and each node reached once,
+// prims a permutation, every prim position under exactly one leaf.
+bool check_bvh2(const Bvh2Tree& t, uint32_t count, std::string& err)
+{
+    const size_t nn = t.nodes.size();
+    if (nn == 0 || t.prims.size() != count) {
+        err = "node or primitive count does not match the mesh";
+        return false;
+    }
+    std::vector<uint8_t> seen(count, 0);
+    for (uint32_t p : t.prims) {
+        if (p >= count || seen[p]) {
+            err = "primitive order is not a permutation";
+            return false;
+        }
+        seen[p] = 1;
+    }
+    std::fill(seen.begin(), seen.end(), 0);
+    std::vector<uint8_t> reached(nn, 0);
+    std::vector<uint32_t> stack{0};
+    reached[0] = 1;
+    while (!stack.empty()) {
+        const Bvh2Node& nd = t.nodes[stack.back()];
+        stack.pop_back();
+        if (nd.count == 0) {
+            const uint64_t l = nd.first_or_left;
+            if (l == 0 || l + 1 >= nn || reached[l] || reached[l + 1]) {
+                err = "bad child index";
+                return false;
+            }
+            reached[l] = reached[l + 1] = 1;
+            stack.push_back((uint32_t)l);
+            stack.push_back((uint32_t)l + 1);
+        } else {
+            if ((uint64_t)nd.first_or_left + nd.count > count) {
+                err = "leaf out of range";
+                return false;
+            }
+            for (uint32_t i = nd.first_or_left; i < nd.first_or_left + nd.count; ++i) {
+                if (seen[i]) {
+                    err = "a primitive lies under two leaves";
+                    return false;
+                }
+                seen[i] = 1;
+            }
+        }
+    }
+    for (uint8_t v : seen)
+        if (!v) {
+            err = "a primitive lies under no leaf";
+            return false;
+        }
+    return true;
+}
+
+namespace {
+
+// Copies tree `t` over idx[first, first + count) into b.nodes from `base`
+// and refits its boxes to the (padded) primitive boxes bottom-up, so a tree
+// built in another space (an object's space before its transform) stays
+// conservative.  Returns the root's index.
+int64_t import_bvh2(Bvh2Builder& b, const Bvh2Tree& t, uint32_t first, int64_t base)
+{
+    const size_t nn = t.nodes.size();
+    for (size_t j = 0; j < t.prims.size(); ++j) b.idx[first + j] = first + t.prims[j];
+    for (size_t i = 0; i < nn; ++i) {
+        const Bvh2Node& src = t.nodes[i];
+        Node2& d = b.nodes[(size_t)(base + (int64_t)i)];
+        if (src.count == 0) {
+            d.left = (int32_t)(base + src.first_or_left);
+            d.count = 0;
+            d.axis = src.axis;
+        } else {
+            d.left = -1;
+            d.first = first + src.first_or_left;
+            d.count = src.count;
+            d.span = src.count;
+        }
+    }
+    // Leaves above kMaxLeafTris triangles (the reference's builder makes a few
+    // when its split fails) become subtrees of halves.
+    for (size_t i = 0; i < nn; ++i) {
+        const int64_t root = base + (int64_t)i;
+        if (b.nodes[(size_t)root].left >= 0 || b.nodes[(size_t)root].count <= (uint32_t)kMaxLeafTris) continue;
+        std::vector<int64_t> todo{root};
+        while (!todo.empty()) {
+            const int64_t ni = todo.back();
+            todo.pop_back();
+            const uint32_t f = b.nodes[(size_t)ni].first, c = b.nodes[(size_t)ni].count;
+            if (c <= (uint32_t)kMaxLeafTris) continue;
+            const int64_t l = b.alloc_pair();
+            const uint32_t h = c / 2;
+            b.make_leaf(l, f, h);
+            b.make_leaf(l + 1, f + h, c - h);
+            Node2& nd = b.nodes[(size_t)ni];
+            nd.left = (int32_t)l;
+            nd.count = 0;
+            nd.axis = 0;
+            todo.push_back(l);
+            todo.push_back(l + 1);
+        }
+    }
+    // Post-order refit (explicit stack: the tree may be deep).
+    std::vector<std::pair<int64_t, bool>> st{{base, false}};
+    while (!st.empty()) {
+        auto [ni, expanded] = st.back();
+        st.pop_back();
+        Node2& nd = b.nodes[(size_t)ni];
+        if (nd.left < 0) {
+            nd.box.reset();
+            for (uint32_t i = nd.first; i < nd.first + nd.count; ++i) nd.box.grow(b.pbox[b.idx[i]]);
+            continue;
+        }
+        if (!expanded) {
+            st.push_back({ni, true});
+            st.push_back({nd.left, false});
+            st.push_back({nd.left + 1, false});
+            continue;
+        }
+        const Node2& l = b.nodes[(size_t)nd.left];
+        const Node2& r = b.nodes[(size_t)nd.left + 1];
+        nd.box = l.box;
+        nd.box.grow(r.box);
+        nd.first = std::min(l.first, r.first);
+        // A subtree whose triangles are not one contiguous run is never
+        // collapsed into a single leaf.
+        const bool contiguous = (l.first + l.span == r.first) || (r.first + r.span == l.first);
+        nd.span = contiguous ? l.span + r.span : 0xffffffffu;
+    }
+    return base;
+}
+
+}  // namespace
+
+static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, const BuildParams& params,
+                          Bvh2Builder& b, int64_t n2, WideBVH& out, std::string& err,
+                          std::chrono::steady_clock::time_point t0);
+
+bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const BuildParams& params,
+                    WideBVH& out, std::string& err)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    out = WideBVH();
+    if (!check_params(params, n, err)) return false;
+    if (n == 0) {
+        empty_scene(out);
+        return true;
+    }
+    Bvh2Builder b(params);
+    if (!prepare_prims(tris, n, b, out, err)) return false;
     b.nodes.resize((size_t)(2 * n + 1));
-    int threads = params.threads > 0 ? params.threads : (int)std::thread::hardware_concurrency();
-    b.spawn_budget = std::max(0, 4 * threads);
+    b.spawn_budget = std::max(0, 4 * threads_of(params));
+    b.build(0, 0, (uint32_t)n);
+    return collapse_wide(tris, n, ids, params, b, b.next_node.load(), out, err, t0);
+}
+
+bool build_bvh2(const float* tris, int64_t n, const BuildParams& params, Bvh2Tree& out, float scene_lo[3],
+                float scene_hi[3], std::string& err)
+{
+    out = Bvh2Tree();
+    if (!check_params(params, n, err)) return false;
+    if (n == 0) return true;
+    Bvh2Builder b(params);
+    WideBVH tmp;
+    if (!prepare_prims(tris, n, b, tmp, err)) return false;
+    for (int k = 0; k < 3; ++k) {
+        scene_lo[k] = tmp.scene_lo[k];
+        scene_hi[k] = tmp.scene_hi[k];
+    }
+    b.nodes.resize((size_t)(2 * n + 1));
+    b.spawn_budget = std::max(0, 4 * threads_of(params));
     b.build(0, 0, (uint32_t)n);
     const int64_t n2 = b.next_node.load();
+    // Node 0 is the root and pairs are allocated from 1, so the layout is
+    // already the reference's (children at left, left + 1).
+    out.nodes.resize((size_t)n2);
+    for (int64_t i = 0; i < n2; ++i) {
+        const Node2& s = b.nodes[(size_t)i];
+        Bvh2Node& d = out.nodes[(size_t)i];
+        for (int k = 0; k < 3; ++k) {
+            d.lo[k] = s.box.lo[k];
+            d.hi[k] = s.box.hi[k];
+        }
+        d.first_or_left = s.left >= 0 ? (uint32_t)s.left : s.first;
+        d.count = s.left >= 0 ? 0 : (uint16_t)s.count;
+        d.axis = s.left >= 0 ? s.axis : 0;
+        d.pad = 0;
+    }
+    out.prims = b.idx;
+    return true;
+}
+
+bool build_wide_bvh_parts(const float* tris, int64_t n, const int32_t* ids, const std::vector<Bvh2Part>& parts,
+                          const BuildParams& params, WideBVH& out, std::string& err)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    out = WideBVH();
+    if (!check_params(params, n, err)) return false;
+    int64_t covered = 0;
+    size_t extra = 0;
+    for (const Bvh2Part& p : parts) {
+        if (p.first != covered || p.count < 0) {
+            err = "parts must cover the triangles in order";
+            return false;
+        }
+        covered += p.count;
+        if (p.tree) extra += p.tree->nodes.size();
+    }
+    if (covered != n) {
+        err = "parts must cover the triangles in order";
+        return false;
+    }
+    if (n == 0) {
+        empty_scene(out);
+        return true;
+    }
+    Bvh2Builder b(params);
+    if (!prepare_prims(tris, n, b, out, err)) return false;
+    // Room for SAH-built parts, imported trees, leaves split on import and the top level.
+    b.nodes.resize((size_t)(4 * n + 1) + extra + 2 * parts.size());
+    b.spawn_budget = std::max(0, 4 * threads_of(params));
+    // Subtree of every non-empty part: imported or built.
+    std::vector<int64_t> roots;
+    for (const Bvh2Part& p : parts) {
+        if (p.count == 0) continue;
+        if (p.tree) {
+            if (!check_bvh2(*p.tree, (uint32_t)p.count, err)) {
+                err = "binary tree of triangles " + std::to_string(p.first) + "+" + std::to_string(p.count) + ": " + err;
+                return false;
+            }
+            const int64_t base = b.next_node.fetch_add((int64_t)p.tree->nodes.size());
+            roots.push_back(import_bvh2(b, *p.tree, (uint32_t)p.first, base));
+            out.stats.imported_tris += p.count;
+        } else {
+            const int64_t r = b.next_node.fetch_add(1);
+            b.build(r, (uint32_t)p.first, (uint32_t)(p.first + p.count));
+            roots.push_back(r);
+        }
+    }
+    // Top level over the part roots: median splits by box centre on the
+    // widest axis (there are few parts); a single part becomes the root.
+    struct Top {
+        Bvh2Builder& b;
+        void make(int64_t ni, std::vector<int64_t> list)
+        {
+            if (list.size() == 1) {
+                b.nodes[(size_t)ni] = b.nodes[(size_t)list[0]];
+                return;
+            }
+            Box cb, box;
+            cb.reset();
+            box.reset();
+            for (int64_t r : list) {
+                const Box& x = b.nodes[(size_t)r].box;
+                float c[3] = {0.5f * x.lo[0] + 0.5f * x.hi[0], 0.5f * x.lo[1] + 0.5f * x.hi[1],
+                              0.5f * x.lo[2] + 0.5f * x.hi[2]};
+                cb.grow(c);
+                box.grow(x);
+            }
+            int a = 0;
+            const float e0 = cb.hi[0] - cb.lo[0], e1 = cb.hi[1] - cb.lo[1], e2 = cb.hi[2] - cb.lo[2];
+            if (e1 > e0) a = 1;
+            if (e2 > std::max(e0, e1)) a = 2;
+            const size_t mid = list.size() / 2;
+            std::nth_element(list.begin(), list.begin() + (std::ptrdiff_t)mid, list.end(), [&](int64_t x, int64_t y) {
+                const Box& bx = b.nodes[(size_t)x].box;
+                const Box& by = b.nodes[(size_t)y].box;
+                const float cx = bx.lo[a] + bx.hi[a], cy = by.lo[a] + by.hi[a];
+                return cx < cy || (cx == cy && x < y);
+            });
+            const int64_t l = b.alloc_pair();
+            std::vector<int64_t> left(list.begin(), list.begin() + (std::ptrdiff_t)mid);
+            std::vector<int64_t> right(list.begin() + (std::ptrdiff_t)mid, list.end());
+            Node2& nd = b.nodes[(size_t)ni];
+            nd.box = box;
+            nd.left = (int32_t)l;
+            nd.count = 0;
+            nd.first = 0;
+            nd.span = 0xffffffffu;
+            nd.axis = (uint8_t)a;
+            make(l, std::move(left));
+            make(l + 1, std::move(right));
+        }
+    };
+    Top{b}.make(0, roots);
+    return collapse_wide(tris, n, ids, params, b, b.next_node.load(), out, err, t0);
+}
+
+static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, const BuildParams& params,
+                          Bvh2Builder& b, int64_t n2, WideBVH& out, std::string& err,
+                          std::chrono::steady_clock::time_point t0)
+{
     b.nodes.resize((size_t)n2);
-    out.stats.bvh2_nodes = n2;
     {
+        // Over the nodes reachable from the root (a tree assembled from parts
+        // leaves the copied part roots behind).
         const double ra = b.nodes[0].box.area();
         double c = 0;
-        for (const Node2& nd : b.nodes)
+        int64_t reach = 0;
+        std::vector<int32_t> st{0};
+        while (!st.empty()) {
+            const Node2& nd = b.nodes[(size_t)st.back()];
+            st.pop_back();
+            ++reach;
             c += (nd.left < 0 ? params.sah_intersect * nd.count : params.sah_traversal) * nd.box.area();
+            if (nd.left >= 0) {
+                st.push_back(nd.left);
+                st.push_back(nd.left + 1);
+            }
+        }
         out.stats.sah_cost = ra > 0 ? c / ra : 0;
+        out.stats.bvh2_nodes = reach;
     }
 
     // Collapse to 8-wide nodes in breadth-first order.

@@ -41,6 +41,7 @@ struct BuildStats {
     int depth = 0;             // wide-node levels (root = 1)
     double build_ms = 0.0;
     double sah_cost = 0.0;     // SAH cost of the BVH2, normalized by root area
+    int64_t imported_tris = 0; // triangles under binary trees taken from the caller
 };
 
 struct WideBVH {
@@ -55,6 +56,49 @@ struct WideBVH {
 // Returns false and fills err on failure (too deep, bad input).
 bool build_wide_bvh(const float* tris, int64_t n, const int32_t* ids, const BuildParams& params,
                     WideBVH& out, std::string& err);
+
+// A binary tree in the reference's on-disk BVH2 layout (Acceleration/BVH.h:87-96,
+// 32 bytes per node): node 0 is the root, an internal node's children are
+// nodes first_or_left and first_or_left + 1, a leaf holds prims[first_or_left,
+// first_or_left + count); prims[] are indices into the caller's triangles.
+struct Bvh2Node {
+    float lo[3];
+    float hi[3];
+    uint32_t first_or_left;
+    uint16_t count;  // 0: internal
+    uint8_t axis;
+    uint8_t pad;
+};
+static_assert(sizeof(Bvh2Node) == 32, "Bvh2Node must be 32 bytes");
+
+struct Bvh2Tree {
+    std::vector<Bvh2Node> nodes;
+    std::vector<uint32_t> prims;
+};
+
+// The binned-SAH binary stage alone (boxes padded as in the wide build);
+// scene_lo / scene_hi receive the triangles' bounds.
+bool build_bvh2(const float* tris, int64_t n, const BuildParams& params, Bvh2Tree& out, float scene_lo[3],
+                float scene_hi[3], std::string& err);
+
+// Checks a binary tree over `count` triangles: structure, and that prims is a
+// permutation covered once by leaves.
+bool check_bvh2(const Bvh2Tree& tree, uint32_t count, std::string& err);
+
+// A range of the input triangles and, optionally, a binary tree over it.
+struct Bvh2Part {
+    int64_t first;
+    int64_t count;
+    const Bvh2Tree* tree;  // NULL: built by SAH
+};
+
+// Wide build whose binary stage takes the given trees for ranges of the input
+// instead of building them (their boxes are refit to the triangles, so a tree
+// built before an object's transform stays valid); parts without a tree are
+// built by SAH, and a top level joins the parts.  parts must cover [0, n) in
+// order.  Trees are checked (check_bvh2); a bad one fails the build.
+bool build_wide_bvh_parts(const float* tris, int64_t n, const int32_t* ids, const std::vector<Bvh2Part>& parts,
+                          const BuildParams& params, WideBVH& out, std::string& err);
 
 // Decoded child box of slot s (exactly the arithmetic the kernels use).
 void decode_child_box(const Node8& n, int s, float lo[3], float hi[3]);

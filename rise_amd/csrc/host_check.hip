@@ -4,6 +4,7 @@
 
 #include "api_common.h"
 #include "bvh_build.h"
+#include "risemesh_internal.h"
 #include "trace_core.h"
 
 #include <atomic>
@@ -29,17 +30,35 @@ struct HostStack {
 
 }  // namespace
 
+namespace {
+
+rise_hip_status check_trace_args(const float* rays, int64_t n, int32_t mode, int32_t* prim_id, float* t,
+                                 uint8_t* occluded)
+{
+    if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative count");
+    if (n > 0 && !rays) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (mode == 0 && n > 0 && (!prim_id || !t)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if (mode == 1 && n > 0 && !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if (mode != 0 && mode != 1) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
+    return RISE_HIP_OK;
+}
+
+rise_hip_status trace_host(const WideBVH& bvh, int64_t n_tris, int32_t bvh_source, const float* rays,
+                           const float* tmax, int64_t n, int32_t mode, int32_t* prim_id, float* t,
+                           uint8_t* occluded, uint64_t* counters, rise_hip_scene_stats* stats);
+
+}  // namespace
+
 extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t n_tris, const float* rays,
                                                      const float* tmax, int64_t n, int32_t mode, int32_t* prim_id,
                                                      float* t, uint8_t* occluded, uint64_t* counters,
                                                      rise_hip_scene_stats* stats)
 {
     clear_error();
-    if (n_tris < 0 || n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative count");
-    if ((n_tris > 0 && !tris) || (n > 0 && !rays)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
-    if (mode == 0 && n > 0 && (!prim_id || !t)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
-    if (mode == 1 && n > 0 && !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
-    if (mode != 0 && mode != 1) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
+    if (n_tris < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative count");
+    if (n_tris > 0 && !tris) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    rise_hip_status st = check_trace_args(rays, n, mode, prim_id, t, occluded);
+    if (st != RISE_HIP_OK) return st;
     WideBVH bvh;
     std::string err;
     try {
@@ -48,6 +67,42 @@ extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t 
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
     }
+    return trace_host(bvh, n_tris, 0, rays, tmax, n, mode, prim_id, t, occluded, counters, stats);
+}
+
+extern "C" rise_hip_status rise_hip_debug_trace_host_risemesh(const rise_hip_risemesh* mesh, const float* rays,
+                                                              const float* tmax, int64_t n, int32_t mode,
+                                                              int32_t* prim_id, float* t, uint8_t* occluded,
+                                                              uint64_t* counters, rise_hip_scene_stats* stats)
+{
+    clear_error();
+    if (!mesh) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL mesh");
+    rise_hip_status st = check_trace_args(rays, n, mode, prim_id, t, occluded);
+    if (st != RISE_HIP_OK) return st;
+    WideBVH bvh;
+    std::string err;
+    int64_t n_tris = 0;
+    try {
+        const std::vector<double> d = risemesh_triangles_f64(mesh);
+        std::vector<float> tris(d.size());
+        for (size_t i = 0; i < d.size(); ++i) tris[i] = (float)d[i];
+        n_tris = (int64_t)(tris.size() / 9);
+        const std::vector<Bvh2Part> parts{{0, n_tris, risemesh_tree(mesh)}};
+        if (!build_wide_bvh_parts(tris.data(), n_tris, nullptr, parts, BuildParams(), bvh, err))
+            return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    return trace_host(bvh, n_tris, bvh.stats.imported_tris > 0 ? 1 : 0, rays, tmax, n, mode, prim_id, t, occluded,
+                      counters, stats);
+}
+
+namespace {
+
+rise_hip_status trace_host(const WideBVH& bvh, int64_t n_tris, int32_t bvh_source, const float* rays,
+                           const float* tmax, int64_t n, int32_t mode, int32_t* prim_id, float* t,
+                           uint8_t* occluded, uint64_t* counters, rise_hip_scene_stats* stats)
+{
     if (stats) {
         *stats = rise_hip_scene_stats{};
         stats->n_tris = n_tris;
@@ -58,6 +113,7 @@ extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t 
         stats->tri_bytes = (int32_t)sizeof(Tri48);
         stats->build_ms = bvh.stats.build_ms;
         stats->sah_cost = bvh.stats.sah_cost;
+        stats->bvh_source = bvh_source;
     }
     const NodeSlot* nodes = bvh.nodes.data();
     const Tri48* trs = bvh.tris.data();
@@ -108,3 +164,5 @@ extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t 
     }
     return RISE_HIP_OK;
 }
+
+}  // namespace

@@ -178,6 +178,14 @@ RISE_HD inline float spd_value(const PtSceneView& s, int spd, float nm)
 {
     const DevSpd d = s.spds[spd];
     if (nm < d.begin || nm > d.end) return 0.0f;
+    if (d.n < 0) {
+        // Sigmoid-polynomial spectrum (RGBSigmoidPolynomial::Eval): amplitudes
+        // hold c0, c1, c2 and a scale; the range is [380, 780] nm.
+        const float* c = s.spd_amp + d.offset;
+        const float t = (nm - 380.0f) * (1.0f / 400.0f);
+        const float x = (c[0] * t + c[1]) * t + c[2];
+        return c[3] * (0.5f + x / (2.0f * sqrtf(1.0f + x * x)));
+    }
     int idx = (int)((nm - d.begin) * d.inv_delta);
     if (idx >= d.n) idx = d.n - 1;
     if (idx < 0) idx = 0;

@@ -22,3 +22,15 @@ struct rise_hip_pt_scene {
 
 // Frees the scene's cached single-request batch (pt_render.hip).
 void release_solo_batch(rise_hip_pt_scene* s);
+
+namespace rise_hip {
+struct Bvh2Tree;
+// rise_hip_pt_scene_create_with_media with, optionally, a binary tree per
+// object (object_trees[o] over that object's triangles, NULL: built), as a
+// .risemesh file's BVH2 cache provides (pt_scene.hip).
+rise_hip_status pt_scene_create(const rise_hip_object_desc* objects, size_t n_objects,
+                                const rise_hip_material_desc* materials, size_t n_materials,
+                                const rise_hip_spd_desc* spds, size_t n_spds, const rise_hip_medium_desc* media,
+                                size_t n_media, const rise_hip_object_media* object_media,
+                                const Bvh2Tree* const* object_trees, rise_hip_pt_scene** out_scene);
+}  // namespace rise_hip

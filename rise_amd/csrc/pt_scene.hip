@@ -5,6 +5,7 @@
 #include "api_common.h"
 #include "bvh_build.h"
 #include "pt_internal.h"
+#include "rise_hip_rgb_uplift.h"
 
 #include <hip/hip_runtime.h>
 
@@ -61,6 +62,17 @@ extern "C" rise_hip_status rise_hip_pt_scene_create_with_media(
     size_t n_materials, const rise_hip_spd_desc* spds, size_t n_spds, const rise_hip_medium_desc* media,
     size_t n_media, const rise_hip_object_media* object_media, rise_hip_pt_scene** out_scene)
 {
+    return pt_scene_create(objects, n_objects, materials, n_materials, spds, n_spds, media, n_media, object_media,
+                           nullptr, out_scene);
+}
+
+rise_hip_status rise_hip::pt_scene_create(const rise_hip_object_desc* objects, size_t n_objects,
+                                          const rise_hip_material_desc* materials, size_t n_materials,
+                                          const rise_hip_spd_desc* spds, size_t n_spds,
+                                          const rise_hip_medium_desc* media, size_t n_media,
+                                          const rise_hip_object_media* object_media,
+                                          const Bvh2Tree* const* object_trees, rise_hip_pt_scene** out_scene)
+{
     clear_error();
     if (!out_scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "out_scene is NULL");
     *out_scene = nullptr;
@@ -98,6 +110,21 @@ extern "C" rise_hip_status rise_hip_pt_scene_create_with_media(
     std::vector<float> amps;
     for (size_t i = 0; i < n_spds; ++i) {
         const rise_hip_spd_desc& d = spds[i];
+        if (d.n == RISE_HIP_SPD_SIGMOID) {
+            if (!d.amplitudes)
+                return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "spectrum " + std::to_string(i) + ": NULL coefficients");
+            for (int k = 0; k < 4; ++k)
+                if (!std::isfinite(d.amplitudes[k]))
+                    return fail(RISE_HIP_ERR_INVALID_ARGUMENT,
+                                "spectrum " + std::to_string(i) + ": non-finite sigmoid coefficient or scale");
+            dspd[i].begin = 380.0f;
+            dspd[i].end = 780.0f;
+            dspd[i].n = RISE_HIP_SPD_SIGMOID;
+            dspd[i].inv_delta = 0.0f;
+            dspd[i].offset = (int)amps.size();
+            amps.insert(amps.end(), d.amplitudes, d.amplitudes + 4);
+            continue;
+        }
         if (d.n <= 0 || !d.amplitudes || !(d.lambda_end > d.lambda_begin))
             return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "spectrum " + std::to_string(i) + " is empty or inverted");
         dspd[i].begin = d.lambda_begin;
@@ -155,9 +182,21 @@ extern "C" rise_hip_status rise_hip_pt_scene_create_with_media(
     }
     WideBVH bvh;
     std::string err;
+    bool any_tree = false;
+    for (size_t o = 0; object_trees && o < n_objects; ++o) any_tree = any_tree || object_trees[o];
     try {
-        if (!build_wide_bvh(verts.data(), total, nullptr, BuildParams(), bvh, err))
+        if (any_tree) {
+            std::vector<Bvh2Part> parts;
+            int64_t off = 0;
+            for (size_t o = 0; o < n_objects; ++o) {
+                parts.push_back({off, objects[o].n_tris, object_trees[o]});
+                off += objects[o].n_tris;
+            }
+            if (!build_wide_bvh_parts(verts.data(), total, nullptr, parts, BuildParams(), bvh, err))
+                return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+        } else if (!build_wide_bvh(verts.data(), total, nullptr, BuildParams(), bvh, err)) {
             return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+        }
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }
@@ -230,8 +269,15 @@ extern "C" rise_hip_status rise_hip_pt_scene_create_with_media(
                 }
                 const DevSpd& e = dspd[(size_t)m.emit_spd];
                 double mean = 0;
-                for (int j = 0; j < e.n; ++j) mean += amps[(size_t)(e.offset + j)];
-                mean /= e.n;
+                if (e.n == RISE_HIP_SPD_SIGMOID) {
+                    // Mean over the 81 CIE sample wavelengths, 380..780 nm.
+                    const float* c = &amps[(size_t)e.offset];
+                    for (int j = 0; j < 81; ++j) mean += rise_hip_sigmoid_spectrum_eval(c, c[3], 380.0 + 5.0 * j);
+                    mean /= 81.0;
+                } else {
+                    for (int j = 0; j < e.n; ++j) mean += amps[(size_t)(e.offset + j)];
+                    mean /= e.n;
+                }
                 const double wgt = mean * m.emit_scale * tot;
                 if (tot > 0 && wgt > 0) {
                     DevLuminary L{};

@@ -101,6 +101,15 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }
 
+    return upload_scene(bvh, total, 0, out_scene);
+}
+
+}  // extern "C"
+
+namespace rise_hip {
+
+rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene)
+{
     rise_hip_scene* s = new (std::nothrow) rise_hip_scene;
     if (!s) return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
     hipError_t e = hipGetDevice(&s->device);
@@ -133,9 +142,14 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
     s->stats.build_ms = bvh.stats.build_ms;
     s->stats.sah_cost = bvh.stats.sah_cost;
     s->stats.device_bytes = (int64_t)(nb + tb);
+    s->stats.bvh_source = bvh_source;
     *out_scene = s;
     return RISE_HIP_OK;
 }
+
+}  // namespace rise_hip
+
+extern "C" {
 
 rise_hip_status rise_hip_scene_release(rise_hip_scene* scene)
 {

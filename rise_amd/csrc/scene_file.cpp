@@ -7,6 +7,10 @@
 #include "rise_hip_scene_file.h"
 
 #include "api_common.h"
+#include "bvh_build.h"
+#include "pt_internal.h"
+#include "risemesh_internal.h"
+#include "rise_hip_rgb_uplift.h"
 
 #include <algorithm>
 #include <cmath>
@@ -14,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <utility>
@@ -21,8 +26,14 @@
 
 using namespace rise_hip;
 
+struct RiseMeshDeleter {
+    void operator()(rise_hip_risemesh* m) const { rise_hip_risemesh_release(m); }
+};
+using RiseMeshPtr = std::shared_ptr<rise_hip_risemesh>;
+
 struct rise_hip_scene_file {
     std::vector<std::vector<float>> meshes;   // world-space triangles per object (9 floats each)
+    std::vector<RiseMeshPtr> object_risemesh; // per object: the .risemesh its geometry came from, if any
     std::vector<rise_hip_object_desc> objects;
     std::vector<rise_hip_object_media> object_media;
     std::vector<rise_hip_material_desc> materials;
@@ -236,13 +247,18 @@ struct Builder {
     rise_hip_scene_file& F;
     std::string base_dir;
     std::map<std::string, int> spd_of_painter;
-    std::map<std::string, std::string> rgb_painters;  // uniformcolor_painter names (unsupported when used)
+    struct RgbPainter {
+        double rgb[3];  // linear Rec.709
+        int spd = -1;   // uplifted spectrum, made on first use
+    };
+    std::map<std::string, RgbPainter> rgb_painters;
     std::map<double, int> flat_spd;
     std::map<std::string, int> material_of;
     std::map<std::string, int> medium_of;
     struct Mesh {
         std::vector<double> v;  // object space, 9 per triangle
         bool double_sided = false;
+        RiseMeshPtr risemesh;   // the file of a risemesh_geometry (its BVH2 cache)
     };
     std::map<std::string, Mesh> meshes;
     bool have_camera = false, have_film = false, have_rasterizer = false;
@@ -284,10 +300,47 @@ struct Builder {
         if (parse_double(v, d)) return flat(d);
         auto it = spd_of_painter.find(v);
         if (it != spd_of_painter.end()) return it->second;
-        if (rgb_painters.count(v))
-            error_at(c, "`" + key + "` names the RGB painter `" + v +
-                            "`; the spectral path needs spectral painters (the RGB-to-spectrum table is not available)");
+        auto ri = rgb_painters.find(v);
+        if (ri != rgb_painters.end()) {
+            // RGBAlbedoSpectrum::FromRGB (UniformColorPainter.h:38-52): the
+            // sigmoid spectrum of the table entry, scale 1.
+            if (ri->second.spd < 0) {
+                const rise_hip_rgb_lut* lut = nullptr;
+                float cf[3];
+                if (rise_hip_rgb_lut_default(&lut) != RISE_HIP_OK ||
+                    rise_hip_rgb_lut_lookup(lut, ri->second.rgb, cf) != RISE_HIP_OK)
+                    error_at(c, std::string(rise_hip_last_error()));
+                F.amps.push_back({cf[0], cf[1], cf[2], 1.0f});
+                rise_hip_spd_desc d{};
+                d.lambda_begin = 380.0f;
+                d.lambda_end = 780.0f;
+                d.n = RISE_HIP_SPD_SIGMOID;
+                F.spds.push_back(d);
+                ri->second.spd = (int)F.spds.size() - 1;
+            }
+            return ri->second.spd;
+        }
         error_at(c, "`" + key + "` names an unknown painter `" + v + "`");
+    }
+
+    // :1005-1040 (uniformcolor_painter) and Job::AddUniformColorPainter
+    // (Job.cpp:2454-2485): `color` R G B in `colorspace` Rec709RGB_Linear
+    // (default; RISEPel is linear Rec.709), RISERGB (the same) or sRGB
+    // (decoded by ColorUtils::SRGBTransferFunctionInverse, ColorUtils.cpp:34-45).
+    void uniform_painter(const Chunk& c)
+    {
+        double col[3] = {0, 0, 0};
+        get_vec3(c, "color", col);
+        const std::string cs = unquote(c.get("colorspace") ? *c.get("colorspace") : "Rec709RGB_Linear");
+        RgbPainter p;
+        for (int k = 0; k < 3; ++k) {
+            double v = col[k];
+            if (cs == "sRGB") v = v == 0 ? 0.0 : v <= 0.03928 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4);
+            else if (cs != "Rec709RGB_Linear" && cs != "RISERGB")
+                error_at(c, "colorspace `" + cs + "` is not supported (Rec709RGB_Linear, RISERGB or sRGB)");
+            p.rgb[k] = v;
+        }
+        rgb_painters[unquote(c.get("name") ? *c.get("name") : "noname")] = p;
     }
 
     // Parsers/ChunkParserRegistry.cpp:1086-1170 and Job::AddSpectralColorPainter
@@ -508,6 +561,24 @@ struct Builder {
         meshes[name] = std::move(m);
     }
 
+    // :5137 (risemesh_geometry, Job::AddRISEMeshTriangleMeshGeometry): a
+    // .risemesh file; double-sidedness comes from the file, `face_normals`
+    // is what the GPU path shades with anyway.  The file's BVH2 cache, when
+    // valid, replaces the SAH build of this object's subtree.
+    void risemesh(const Chunk& c)
+    {
+        const std::string name = unquote(c.get("name") ? *c.get("name") : "noname");
+        const std::string fn = resolve(base_dir, c.get("file") ? *c.get("file") : "none");
+        rise_hip_risemesh* raw = nullptr;
+        if (rise_hip_risemesh_read(fn.c_str(), &raw) != RISE_HIP_OK)
+            error_at(c, std::string(rise_hip_last_error()));
+        Mesh m;
+        m.risemesh = RiseMeshPtr(raw, RiseMeshDeleter());
+        m.v = risemesh_triangles_f64(raw);
+        m.double_sided = risemesh_double_sided(raw);
+        meshes[name] = std::move(m);
+    }
+
     // :5004 (ClippedPlaneGeometry), stored as the triangles (a, b, c) and
     // (a, c, d); its `doublesided` defaults to TRUE.
     void clippedplane(const Chunk& c)
@@ -567,6 +638,7 @@ struct Builder {
             w[i + 2] = (float)(z + pos[2]);
         }
         F.meshes.push_back(std::move(w));
+        F.object_risemesh.push_back(gi->second.risemesh);
         rise_hip_object_desc od{};
         od.n_tris = (int64_t)(F.meshes.back().size() / 9);
         od.material = mi->second;
@@ -664,7 +736,7 @@ struct Builder {
         for (const Chunk& c : chunks) {
             const std::string& k = c.keyword;
             if (k == "spectral_painter") spectral_painter(c);
-            else if (k == "uniformcolor_painter") rgb_painters[unquote(c.get("name") ? *c.get("name") : "noname")] = k;
+            else if (k == "uniformcolor_painter") uniform_painter(c);
             else if (k == "lambertian_material") lambertian(c);
             else if (k == "lambertian_luminaire_material") luminaire(c);
             else if (k == "ggx_material") ggx(c);
@@ -673,6 +745,7 @@ struct Builder {
             else if (k == "hosek_wilkie_skylight") sky(c);
             else if (k == "rawmesh2_geometry") rawmesh2(c);
             else if (k == "clippedplane_geometry") clippedplane(c);
+            else if (k == "risemesh_geometry") risemesh(c);
             else if (k == "standard_object") object(c);
             else if (k == "pinhole_camera") camera(c);
             else if (k == "film") film(c);
@@ -762,9 +835,13 @@ extern "C" rise_hip_status rise_hip_scene_file_upload(const rise_hip_scene_file*
 {
     clear_error();
     if (!f || !out_scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
-    rise_hip_status st = rise_hip_pt_scene_create_with_media(
-        f->objects.data(), f->objects.size(), f->materials.data(), f->materials.size(), f->spds.data(),
-        f->spds.size(), f->media.data(), f->media.size(), f->any_media ? f->object_media.data() : nullptr, out_scene);
+    std::vector<const Bvh2Tree*> trees(f->objects.size(), nullptr);
+    for (size_t o = 0; o < trees.size(); ++o)
+        if (f->object_risemesh[o]) trees[o] = risemesh_tree(f->object_risemesh[o].get());
+    rise_hip_status st = pt_scene_create(f->objects.data(), f->objects.size(), f->materials.data(),
+                                         f->materials.size(), f->spds.data(), f->spds.size(), f->media.data(),
+                                         f->media.size(), f->any_media ? f->object_media.data() : nullptr,
+                                         trees.data(), out_scene);
     if (st != RISE_HIP_OK || !f->has_sky) return st;
     st = rise_hip_pt_scene_set_sky(*out_scene, &f->sky);
     if (st != RISE_HIP_OK) {

@@ -25,9 +25,14 @@ import numpy as np
 
 @dataclass
 class Spd:
+    """A step spectrum (n bins over [begin, end]) or, with sigmoid True, a
+    sigmoid-polynomial spectrum from RGB uplift: amp = (c0, c1, c2, scale),
+    S = scale * sigmoid(c0 t^2 + c1 t + c2), t = (nm - 380) / 400, zero
+    outside [380, 780] nm."""
     begin: float
     end: float
     amp: np.ndarray
+    sigmoid: bool = False
 
     @staticmethod
     def flat(v: float, begin: float = 380.0, end: float = 720.0) -> "Spd":

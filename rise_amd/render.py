@@ -222,7 +222,7 @@ class PathTracingSpectralRasterizer:
             self._keep.append(a)
             spds[k].lambda_begin = s.begin
             spds[k].lambda_end = s.end
-            spds[k].n = a.size
+            spds[k].n = -1 if s.sigmoid else a.size
             spds[k].amplitudes = a.ctypes.data
         mats = (MaterialDesc * max(1, len(scene.materials)))()
         for k, m in enumerate(scene.materials):

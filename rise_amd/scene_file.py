@@ -100,8 +100,10 @@ class SceneFile:
         media = _array(p[4], MediumDesc, info["n_media"])
         s = Scene()
         for d in spds:
-            amp = np.ctypeslib.as_array(ctypes.cast(d.amplitudes, ctypes.POINTER(ctypes.c_float)), (d.n,)).copy()
-            s.spds.append(Spd(_f(d.lambda_begin), _f(d.lambda_end), amp))
+            sig = d.n == -1
+            amp = np.ctypeslib.as_array(ctypes.cast(d.amplitudes, ctypes.POINTER(ctypes.c_float)),
+                                        (4 if sig else d.n,)).copy()
+            s.spds.append(Spd(_f(d.lambda_begin), _f(d.lambda_end), amp, sigmoid=sig))
         for m in mats:
             s.materials.append(Material(reflectance=m.reflectance_spd, emission=m.emission_spd,
                                         emission_scale=_f(m.emission_scale), kind=m.type,

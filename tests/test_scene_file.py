@@ -160,8 +160,7 @@ pinhole_camera
 
 @pytest.mark.parametrize("extra, needle", [
     ("bogus_chunk\n{\n\tname x\n}\n", "bogus_chunk (line 24)"),
-    ("uniformcolor_painter\n{\n\tname c\n\tcolor 1 0 0\n}\nlambertian_material\n{\n\tname m2\n\treflectance c\n}\n",
-     "RGB painter"),
+    ("uniformcolor_painter\n{\n\tname c\n\tcolor 1 0 0\n\tcolorspace ROMMRGB_Linear\n}\n", "colorspace"),
     ("hosek_wilkie_skylight\n{\n\tturbidity 3\n}\n", "create_sun FALSE"),
     ("dielectric_material\n{\n\tname d\n\tior 1.5\n}\n", "smooth dielectrics"),
     ("ggx_material\n{\n\tname g\n\trs 1\n\talphax 0.1\n\talphay 0.3\n}\n", "anisotropic"),
