@@ -25,8 +25,9 @@
  *                    casts_shadows)
  *   pinhole_camera (:4282)   film (:4814)
  *   pathtracing_spectral_rasterizer (:8894)
- * Chunks with no effect on the spectral path tracer's image (file outputs,
- * shaders and shader ops, the other rasterizers' settings) are skipped;
+ *   file_rasterizeroutput (:9207; see rise_hip_output.h)
+ * Chunks with no effect on the spectral path tracer's image (shaders and
+ * shader ops, the other rasterizers' settings) are skipped;
  * any other chunk, and any parameter value of a supported chunk that the
  * GPU path cannot represent, is an error naming the chunk and line.
  * Parsing needs no GPU.
@@ -34,6 +35,7 @@
 #ifndef RISE_HIP_SCENE_FILE_H
 #define RISE_HIP_SCENE_FILE_H
 
+#include "rise_hip_output.h"
 #include "rise_hip_render.h"
 
 #ifdef __cplusplus
@@ -50,6 +52,8 @@ typedef struct rise_hip_scene_file_info {
     int64_t n_tris;
     int32_t has_sky;
     int32_t n_skipped_chunks;  /* chunks skipped as irrelevant to the image */
+    int32_t n_outputs;         /* file_rasterizeroutput chunks */
+    int32_t reserved;
 } rise_hip_scene_file_info;
 
 /* Reads and checks a scene file (relative mesh and spectrum files resolve
@@ -69,6 +73,13 @@ rise_hip_status rise_hip_scene_file_get_tables(const rise_hip_scene_file* file,
 
 /* The render the file's camera, film and rasterizer chunks describe. */
 rise_hip_status rise_hip_scene_file_get_render_desc(const rise_hip_scene_file* file, rise_hip_render_desc* out_desc);
+
+/* Output `index` (0 .. n_outputs-1) of the file's file_rasterizeroutput
+ * chunks (FileRasterizerOutput, Parsers/ChunkParserRegistry.cpp:9207-9325):
+ * its description for rise_hip_output_write and its path pattern (owned by
+ * `file`; the file name is pattern + "." + rise_hip_output_extension). */
+rise_hip_status rise_hip_scene_file_get_output(const rise_hip_scene_file* file, int32_t index,
+                                               rise_hip_output_desc* desc, const char** pattern);
 
 /* Builds the device scene (rise_hip_pt_scene_create_with_media, then
  * rise_hip_pt_scene_set_sky when the file has a sky). */

@@ -23,6 +23,10 @@
 #include "Geometry/TriangleMeshGeometryIndexed.h"
 #include "Intersection/RayIntersectionGeometric.h"
 #include "Utilities/MemoryBuffer.h"
+#include "RasterImages/HDRWriter.h"
+#include "RasterImages/PPMWriter.h"
+#include "RasterImages/TGAWriter.h"
+#include "Rendering/DisplayTransformWriter.h"
 
 #include <atomic>
 #include <condition_variable>
@@ -252,6 +256,57 @@ void* rise_ref_risemesh_load(const char* path)
 unsigned rise_ref_bvh2_nodes(void* handle)
 {
     return static_cast<MeshProbe*>(((RefMesh*)handle)->mesh)->bvh2_nodes();
+}
+
+// Writes an XYZ film (h x w x 3 floats, row y = raster row y) with the
+// reference's own colour pipeline and writers, as FileRasterizerOutput /
+// FrameEncoderBase::Encode drive them: each pixel RISEColor(RISEPel(XYZPel),
+// alpha 1); for LDR formats a DisplayTransformWriter when the transform is
+// not `none` or the exposure not 0.  type: 0 TGA, 1 PPM, 3 HDR; color_space:
+// 0 linear Rec.709, 1 sRGB.  Returns 0 on success.
+int rise_ref_write_image(const float* xyz, int w, int h, int type, int color_space, double exposure_ev,
+                         int display_transform, const char* path)
+{
+    Implementation::MemoryBuffer* mb = new Implementation::MemoryBuffer();
+    const COLOR_SPACE cs = color_space == 0 ? eColorSpace_Rec709RGB_Linear : eColorSpace_sRGB;
+    IRasterImageWriter* w0 = nullptr;
+    if (type == 0) w0 = new Implementation::TGAWriter(*mb, cs);
+    else if (type == 1) w0 = new Implementation::PPMWriter(*mb, cs);
+    else if (type == 3) w0 = new Implementation::HDRWriter(*mb, cs);
+    else {
+        mb->release();
+        return 2;
+    }
+    IRasterImageWriter* wr = w0;
+    Implementation::DisplayTransformWriter* dtw = nullptr;
+    if (type != 3 && (display_transform != 0 || exposure_ev != 0.0)) {
+        dtw = new Implementation::DisplayTransformWriter(*w0, exposure_ev, (DISPLAY_TRANSFORM)display_transform);
+        wr = dtw;
+    }
+    wr->BeginWrite(w, h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float* p = xyz + 3 * ((size_t)y * w + x);
+            const XYZPel X(p[0], p[1], p[2]);
+            wr->WriteColor(RISEColor(RISEPel(X), 1.0), x, y);
+        }
+    wr->EndWrite();
+    const bool ok = mb->DumpToFileToCursor(path);
+    if (dtw) dtw->release();
+    w0->release();
+    mb->release();
+    return ok ? 0 : 1;
+}
+
+// RISEPel(XYZPel): ColorUtils::XYZtoRec709RGB with its gamut clip, n pixels.
+void rise_ref_xyz_to_rec709(const float* xyz, int64_t n, double* rgb)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const RISEPel p(XYZPel(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+        rgb[3 * i] = p.r;
+        rgb[3 * i + 1] = p.g;
+        rgb[3 * i + 2] = p.b;
+    }
 }
 
 }  // extern "C"

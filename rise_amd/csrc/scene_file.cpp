@@ -11,6 +11,7 @@
 #include "pt_internal.h"
 #include "risemesh_internal.h"
 #include "rise_hip_rgb_uplift.h"
+#include "rise_hip_output.h"
 
 #include <algorithm>
 #include <cmath>
@@ -45,6 +46,8 @@ struct rise_hip_scene_file {
     rise_hip_render_desc desc{};
     int skipped = 0;
     bool any_media = false;
+    std::vector<rise_hip_output_desc> outputs;
+    std::vector<std::string> output_patterns;
 };
 
 namespace {
@@ -714,6 +717,40 @@ struct Builder {
         have_rasterizer = true;
     }
 
+    // :9207-9325 (FileRasterizerOutput): pattern, type (TGA default), bpp (8),
+    // color_space (sRGB), exposure, display_transform (ACES for LDR types,
+    // none for HDR), exr_with_alpha (TRUE).  `multiple` (one file per
+    // animation frame) has no meaning for a single render and is ignored.
+    void file_output(const Chunk& c)
+    {
+        rise_hip_output_desc d{};
+        const std::string t = unquote(c.get("type") ? *c.get("type") : "TGA");
+        if (t == "TGA") d.format = RISE_HIP_IMAGE_TGA;
+        else if (t == "PPM") d.format = RISE_HIP_IMAGE_PPM;
+        else if (t == "PNG") d.format = RISE_HIP_IMAGE_PNG;
+        else if (t == "HDR") d.format = RISE_HIP_IMAGE_HDR;
+        else if (t == "EXR") d.format = RISE_HIP_IMAGE_EXR;
+        else error_at(c, "output type `" + t + "` is not supported (TGA, PPM, PNG, HDR, EXR)");
+        d.bpp = (int32_t)get_double(c, "bpp", 8.0);
+        const std::string cs = unquote(c.get("color_space") ? *c.get("color_space") : "sRGB");
+        if (cs == "sRGB") d.color_space = RISE_HIP_COLOR_SRGB;
+        else if (cs == "Rec709RGB_Linear") d.color_space = RISE_HIP_COLOR_REC709_LINEAR;
+        else error_at(c, "color_space `" + cs + "` is not supported (sRGB, Rec709RGB_Linear)");
+        d.exposure_ev = get_double(c, "exposure", 0.0);
+        d.display_transform = RISE_HIP_DISPLAY_DEFAULT;
+        if (c.has("display_transform")) {
+            const std::string dt = unquote(*c.get("display_transform"));
+            static const char* names[] = {"none", "reinhard", "aces", "agx", "hable"};
+            d.display_transform = -2;
+            for (int k = 0; k < 5; ++k)
+                if (dt == names[k]) d.display_transform = k;
+            if (d.display_transform == -2) error_at(c, "unknown display_transform `" + dt + "`");
+        }
+        d.with_alpha = get_bool(c, "exr_with_alpha", true) ? 1 : 0;
+        F.outputs.push_back(d);
+        F.output_patterns.push_back(unquote(c.get("pattern") ? *c.get("pattern") : "none"));
+    }
+
     void run(const std::vector<Chunk>& chunks)
     {
         // Defaults of a render the file does not fully describe.
@@ -730,7 +767,7 @@ struct Builder {
         F.desc.camera.up[1] = 1.0f;
         F.desc.camera.fov_deg = 30.0f;
         F.desc.sky_background = 1;
-        static const char* kSkip[] = {"file_rasterizeroutput", "standard_shader", "pathtracing_shaderop",
+        static const char* kSkip[] = {"standard_shader", "pathtracing_shaderop",
                                       "directlighting_shaderop", "ambientocclusion_shaderop", "simple_shader",
                                       "advanced_shader", "global_options", "pixelpel_rasterizer"};
         for (const Chunk& c : chunks) {
@@ -750,6 +787,7 @@ struct Builder {
             else if (k == "pinhole_camera") camera(c);
             else if (k == "film") film(c);
             else if (k == "pathtracing_spectral_rasterizer") rasterizer(c);
+            else if (k == "file_rasterizeroutput") file_output(c);
             else if (std::find_if(std::begin(kSkip), std::end(kSkip), [&](const char* s) { return k == s; }) !=
                          std::end(kSkip) ||
                      k.find("shaderop") != std::string::npos)
@@ -800,6 +838,7 @@ extern "C" rise_hip_status rise_hip_scene_file_get_info(const rise_hip_scene_fil
     for (const auto& o : f->objects) out->n_tris += o.n_tris;
     out->has_sky = f->has_sky ? 1 : 0;
     out->n_skipped_chunks = f->skipped;
+    out->n_outputs = (int32_t)f->outputs.size();
     return RISE_HIP_OK;
 }
 
@@ -828,6 +867,18 @@ extern "C" rise_hip_status rise_hip_scene_file_get_render_desc(const rise_hip_sc
     clear_error();
     if (!f || !out) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = f->desc;
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_scene_file_get_output(const rise_hip_scene_file* f, int32_t index,
+                                                          rise_hip_output_desc* desc, const char** pattern)
+{
+    clear_error();
+    if (!f) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL scene file");
+    if (index < 0 || (size_t)index >= f->outputs.size())
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "output index out of range");
+    if (desc) *desc = f->outputs[(size_t)index];
+    if (pattern) *pattern = f->output_patterns[(size_t)index].c_str();
     return RISE_HIP_OK;
 }
 

@@ -18,6 +18,7 @@ import numpy as np
 
 from . import _lib
 from .pt_scene import Camera, Material, Medium, Object, RenderSettings, Scene, Sky, Spd
+from .output import OutputDesc, extension
 from .render import (MaterialDesc, MediumDesc, ObjectDesc, ObjectMedia, PathTracingSpectralRasterizer, RenderDesc,
                      SkyDesc, SpdDesc)
 
@@ -31,10 +32,12 @@ class SceneFileInfo(ctypes.Structure):
         ("n_tris", ctypes.c_int64),
         ("has_sky", ctypes.c_int32),
         ("n_skipped_chunks", ctypes.c_int32),
+        ("n_outputs", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 _P = ctypes.c_void_p
@@ -45,6 +48,8 @@ _SIG = {
     "rise_hip_scene_file_get_tables": ([_P, _PP, _PP, _PP, _PP, _PP, _PP], ctypes.c_int),
     "rise_hip_scene_file_get_render_desc": ([_P, ctypes.POINTER(RenderDesc)], ctypes.c_int),
     "rise_hip_scene_file_upload": ([_P, _PP], ctypes.c_int),
+    "rise_hip_scene_file_get_output": ([_P, ctypes.c_int32, ctypes.POINTER(OutputDesc),
+                                        ctypes.POINTER(ctypes.c_char_p)], ctypes.c_int),
     "rise_hip_scene_file_release": ([_P], ctypes.c_int),
 }
 
@@ -132,6 +137,17 @@ class SceneFile:
                               max_transmission_bounce=d.max_transmission_bounce, rr_min_depth=d.rr_min_depth,
                               rr_threshold=_f(d.rr_threshold), seed=int(d.seed),
                               sky_background=bool(d.sky_background))
+
+    def outputs(self) -> list:
+        """[(OutputDesc, file name)] of the file's file_rasterizeroutput chunks."""
+        out = []
+        names = {0: "TGA", 1: "PPM", 2: "PNG", 3: "HDR", 6: "EXR"}
+        for i in range(self.info()["n_outputs"]):
+            d, pat = OutputDesc(), ctypes.c_char_p()
+            _lib.check(_lib_scene_file().rise_hip_scene_file_get_output(self._h, i, ctypes.byref(d),
+                                                                         ctypes.byref(pat)))
+            out.append((d, f"{pat.value.decode()}.{extension(names[d.format])}"))
+        return out
 
     def rasterizer(self) -> PathTracingSpectralRasterizer:
         """The scene made resident by rise_hip_scene_file_upload."""

@@ -73,7 +73,11 @@ def test_materials_media_sky_and_settings():
     f = SceneFile(SCENES / "features.RISEscene")
     info = f.info()
     assert info == {"n_objects": 4, "n_materials": 4, "n_spds": 5, "n_media": 2, "n_tris": 38, "has_sky": 1,
-                    "n_skipped_chunks": 2}
+                    "n_skipped_chunks": 1, "n_outputs": 1}
+    (d, name), = f.outputs()
+    # type EXR; bpp, colour space, display transform and alpha at the parser's defaults
+    assert name == "rendered/features.exr"
+    assert (d.format, d.bpp, d.color_space, d.display_transform, d.exposure_ev, d.with_alpha) == (6, 8, 1, -1, 0.0, 1)
     s, r = f.scene(), f.settings()
     ground, gold, glass, shell = s.materials
     assert ground.kind == pt_scene.LAMBERTIAN and ground.reflectance == 0 and ground.emission == -1
