@@ -101,19 +101,23 @@ __global__ void k_output(const float* __restrict__ film, int64_t n, out::Params 
 
 // ---- framing --------------------------------------------------------------
 
-uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n)
-{
-    static uint32_t table[256];
-    static bool init = false;
-    if (!init) {
+struct Crc32Table {
+    uint32_t t[256];
+    Crc32Table()
+    {
         for (uint32_t k = 0; k < 256; ++k) {
             uint32_t v = k;
             for (int j = 0; j < 8; ++j) v = (v & 1) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
-            table[k] = v;
+            t[k] = v;
         }
-        init = true;
     }
-    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xff] ^ (c >> 8);
+};
+
+// CRC-32 (IEEE 802.3, reflected), as PNG chunks carry it.
+uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n)
+{
+    static const Crc32Table table;  // initialized once, thread-safely
+    for (size_t i = 0; i < n; ++i) c = table.t[(c ^ p[i]) & 0xff] ^ (c >> 8);
     return c;
 }
 

@@ -201,9 +201,19 @@ std::vector<Chunk> read_chunks(const std::string& path)
             cur.params.emplace_back(key, val);
             continue;
         }
-        if (s[0] == '>' || s[0] == '!' || s[0] == '$')
-            throw ParseError{"line " + std::to_string(ln) + ": commands and macros of the retired scene language are "
-                                                           "not supported"};
+        if (s[0] == '>') {
+            // Job::LoadAsciiSceneViaCst accepts only the render-neutral
+            // commands (`> echo`, `> set accelerator`, Job.h:2835-2842) and
+            // refuses the rest; so does this reader.
+            std::istringstream is(s.substr(1));
+            std::string cmd, arg;
+            is >> cmd >> arg;
+            if (cmd == "echo" || (cmd == "set" && arg == "accelerator")) continue;
+            throw ParseError{"line " + std::to_string(ln) + ": `> " + cmd +
+                             "` commands are not supported (only `> echo` and `> set accelerator` are render-neutral)"};
+        }
+        if (s[0] == '!' || s[0] == '$')
+            throw ParseError{"line " + std::to_string(ln) + ": macros of the retired scene language are not supported"};
         cur = Chunk{};
         cur.keyword = s;
         cur.line = ln;

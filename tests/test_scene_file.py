@@ -173,6 +173,7 @@ pinhole_camera
     ("rawmesh2_geometry\n{\n\tname r\n\tfile does_not_exist.raw2\n}\n", "cannot open"),
     ("homogeneous_medium\n{\n\tname h\n\tphase schlick 0.2\n}\n", "phase"),
     ("> run other.RISEscene\n", "not supported"),
+    ("> echo hello\n> set accelerator bvh\nbogus\n{\n}\n", "bogus (line 26)"),  # neutral commands pass
     ("film\n{\n\twidth 8\n", "unterminated"),
 ])
 def test_errors_name_the_problem(tmp_path, extra, needle):
