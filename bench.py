@@ -192,6 +192,100 @@ def terrain_leg(args):
     return out
 
 
+def widened_leg(args, tris):
+    """The stages either side of the hot path, each timed on its own:
+    * .risemesh: writing the 1M-triangle soup with its BVH2 cache, reading it,
+      and the 8-wide build from the file's cache vs from scratch (host);
+    * RGB uplift: generating the 64^3 Jakob-Hanika table on all host threads;
+    * scene file: parsing the Cornell-box scene file and uploading its scene;
+    * output: the per-pixel output kernel on a 1920x1080 film (HIP events on
+      its stream; 12 B read + 4 B written per pixel against the HBM peak) and
+      the whole PNG write (GPU resolve, copy, host framing, file)."""
+    import tempfile
+    from pathlib import Path
+
+    import numpy as np
+    import torch
+
+    from rise_amd.output import output_desc, resolve, write_image
+    from rise_amd.rgb_uplift import RgbLut
+    from rise_amd.risemesh import RiseMesh, write_risemesh
+    from rise_amd.scene_file import SceneFile
+
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        # warm the upload path (first device allocations) on a small mesh
+        write_risemesh(Path(td) / "small.risemesh", tris[:1000], with_bvh_cache=True)
+        RiseMesh(Path(td) / "small.risemesh").object_manager().release()
+        p = Path(td) / "soup.risemesh"
+        t0 = time.perf_counter()
+        write_risemesh(p, tris, with_bvh_cache=True)
+        t1 = time.perf_counter()
+        m = RiseMesh(p)
+        t2 = time.perf_counter()
+        om = m.object_manager()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        st = om.stats()
+        om.release()
+        write_risemesh(Path(td) / "plain.risemesh", tris, with_bvh_cache=False)
+        m2 = RiseMesh(Path(td) / "plain.risemesh")
+        t4 = time.perf_counter()
+        om2 = m2.object_manager()
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        st2 = om2.stats()
+        om2.release()
+        out["risemesh"] = {"n_tris": int(tris.shape[0]), "write_with_cache_ms": round(1e3 * (t1 - t0), 1),
+                           "read_ms": round(1e3 * (t2 - t1), 1),
+                           "scene_from_cache_ms": round(1e3 * (t3 - t2), 1), "bvh_source_cached": st["bvh_source"],
+                           "scene_from_scratch_ms": round(1e3 * (t5 - t4), 1),
+                           "bvh_source_plain": st2["bvh_source"]}
+        log(f"risemesh: write {out['risemesh']['write_with_cache_ms']} ms, read {out['risemesh']['read_ms']} ms, "
+            f"scene from cache {out['risemesh']['scene_from_cache_ms']} ms vs from scratch "
+            f"{out['risemesh']['scene_from_scratch_ms']} ms")
+    lut = RgbLut.generate(64)
+    out["rgb_lut"] = {"resolution": 64, "generate_ms": round(lut.stats["build_ms"], 1),
+                      "converged": lut.stats["converged"], "failed": lut.stats["failed"]}
+    lut.release()
+    log(f"rgb table: 64^3 generated in {out['rgb_lut']['generate_ms']} ms")
+    t0 = time.perf_counter()
+    sf = SceneFile(Path(__file__).resolve().parent / "tests" / "golden" / "scenes" / "cornell.RISEscene")
+    t1 = time.perf_counter()
+    pt = sf.rasterizer()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    pt.release()
+    out["scene_file"] = {"parse_ms": round(1e3 * (t1 - t0), 2), "upload_ms": round(1e3 * (t2 - t1), 2)}
+    g = torch.Generator(device="cuda").manual_seed(3)
+    film = torch.rand((1080, 1920, 3), device="cuda", generator=g) * 2.0
+    px = torch.empty((1080, 1920, 4), dtype=torch.uint8, device="cuda")
+    d = output_desc("PNG", 8, "sRGB", "aces")
+    s = torch.cuda.current_stream()
+    for _ in range(3):
+        resolve(film, px, d, s)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    a.record(s)
+    for _ in range(reps):
+        resolve(film, px, d, s)
+    b.record(s)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    bytes_px = 16
+    gbs = 1920 * 1080 * bytes_px / (ms / 1e3) / 1e9
+    with tempfile.TemporaryDirectory() as td:
+        t0 = time.perf_counter()
+        write_image(film, Path(td) / "f.png", d)
+        png_ms = 1e3 * (time.perf_counter() - t0)
+    out["output"] = {"film": "1920x1080", "kernel_ms": round(ms, 4), "bytes_per_pixel": bytes_px,
+                     "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / 8000.0, 3),
+                     "png_write_ms": round(png_ms, 1)}
+    log(f"output: kernel {ms * 1e3:.1f} us per 1080p film ({gbs:.0f} GB/s, {gbs / 8000:.1%} of HBM peak), "
+        f"PNG file {png_ms:.1f} ms")
+    return out
+
+
 def serving_leg(args):
     """Serving: 16 resident scenes of 50k-2M triangles (Lambertian, GGX and
     glass), a Poisson stream (200 requests/s) of 64 render requests at
@@ -395,6 +489,8 @@ def main() -> int:
     ap.add_argument("--terrain-steps", type=int, default=1)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--no-widened", action="store_true",
+                    help="skip the mesh-file / RGB-table / scene-file / output-stage timings")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -513,6 +609,7 @@ def main() -> int:
     batched = None if args.no_batch else batch_leg(args)
     terrain = None if args.no_terrain else terrain_leg(args)
     serving = None if args.no_serving else serving_leg(args)
+    widened = None if args.no_widened else widened_leg(args, tris)
 
     traffic = load_traffic()
     line = {
@@ -555,6 +652,7 @@ def main() -> int:
         "batch": batched,
         "terrain": terrain,
         "serving": serving,
+        "widened": widened,
     }
     om.release()
     print(json.dumps(line), flush=True)

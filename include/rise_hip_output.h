@@ -63,6 +63,14 @@ typedef struct rise_hip_output_desc {
 rise_hip_status rise_hip_output_write(const float* film_xyz, int32_t width, int32_t height,
                                       const rise_hip_output_desc* desc, const char* path, void* stream);
 
+/* The per-pixel stage alone, device to device on `stream` (no copies, waits
+ * or allocations, so it can be captured in a graph): writes width x height
+ * pixels of 4 bytes (TGA / PPM / PNG 8-bit: R, G, B, A; HDR: R, G, B, E),
+ * 8 bytes (PNG 16-bit: four native-endian uint16) or 12 bytes (EXR: three
+ * floats) to `out`, row-major, film row order. */
+rise_hip_status rise_hip_output_resolve(const float* film_xyz, int32_t width, int32_t height,
+                                        const rise_hip_output_desc* desc, void* out, void* stream);
+
 /* The same from a film in host memory, resolved on host threads with the
  * same per-pixel code (no GPU). */
 rise_hip_status rise_hip_output_write_host(const float* film_xyz, int32_t width, int32_t height,

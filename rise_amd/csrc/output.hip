@@ -368,6 +368,22 @@ extern "C" rise_hip_status rise_hip_output_write(const float* film, int32_t w, i
     }
 }
 
+extern "C" rise_hip_status rise_hip_output_resolve(const float* film, int32_t w, int32_t h,
+                                                   const rise_hip_output_desc* desc, void* out, void* stream)
+{
+    clear_error();
+    if (!film || !out) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL film or output");
+    Plan pl{};
+    rise_hip_status st = make_plan(desc, w, h, pl);
+    if (st != RISE_HIP_OK) return st;
+    const int64_t n = (int64_t)w * h;
+    const int block = 256;
+    const unsigned grid = (unsigned)std::min<int64_t>((n + block - 1) / block, 8192);
+    hipLaunchKernelGGL(k_output, dim3(grid), dim3(block), 0, (hipStream_t)stream, film, n, pl.p, pl.mode, out);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "output resolve launch");
+}
+
 extern "C" rise_hip_status rise_hip_output_write_host(const float* film, int32_t w, int32_t h,
                                                       const rise_hip_output_desc* desc, const char* path)
 {

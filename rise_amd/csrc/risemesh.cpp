@@ -26,7 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
-#include <map>
+#include <unordered_map>
 #include <new>
 #include <string>
 #include <tuple>
@@ -324,15 +324,36 @@ extern "C" rise_hip_status rise_hip_risemesh_write(const char* path, const float
     try {
         for (int64_t i = 0; i < 9 * n_tris; ++i)
             if (!std::isfinite(tris[i])) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "non-finite vertex coordinate");
-        // Exact-match vertex and coordinate deduplication.
-        std::map<std::tuple<float, float, float>, uint32_t> pmap;
-        std::map<std::pair<float, float>, uint32_t> cmap;
+        // Exact-match vertex and coordinate deduplication (by bit pattern).
+        struct Key3Hash {
+            size_t operator()(const std::tuple<uint32_t, uint32_t, uint32_t>& k) const
+            {
+                uint64_t h = (uint64_t)std::get<0>(k) * 0x9E3779B97F4A7C15ull;
+                h ^= (uint64_t)std::get<1>(k) * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+                h ^= (uint64_t)std::get<2>(k) * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+                return (size_t)h;
+            }
+        };
+        struct Key2Hash {
+            size_t operator()(const std::pair<uint32_t, uint32_t>& k) const
+            {
+                return (size_t)(((uint64_t)k.first << 32 | k.second) * 0x9E3779B97F4A7C15ull);
+            }
+        };
+        auto bits = [](float f) {
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            return f == 0.0f ? 0u : u;  // +0 and -0 are one vertex
+        };
+        std::unordered_map<std::tuple<uint32_t, uint32_t, uint32_t>, uint32_t, Key3Hash> pmap;
+        std::unordered_map<std::pair<uint32_t, uint32_t>, uint32_t, Key2Hash> cmap;
+        pmap.reserve((size_t)n_tris * 3);
         std::vector<double> points, coords;
         std::vector<uint32_t> idx((size_t)n_tris * 9, 0);
         for (int64_t t = 0; t < n_tris; ++t)
             for (int k = 0; k < 3; ++k) {
                 const float* v = tris + 9 * t + 3 * k;
-                auto key = std::make_tuple(v[0], v[1], v[2]);
+                auto key = std::make_tuple(bits(v[0]), bits(v[1]), bits(v[2]));
                 auto it = pmap.find(key);
                 if (it == pmap.end()) {
                     it = pmap.emplace(key, (uint32_t)(points.size() / 3)).first;
@@ -341,7 +362,7 @@ extern "C" rise_hip_status rise_hip_risemesh_write(const char* path, const float
                 idx[9 * t + 3 * k] = it->second;
                 if (uvs) {
                     const float* c = uvs + 6 * t + 2 * k;
-                    auto ck = std::make_pair(c[0], c[1]);
+                    auto ck = std::make_pair(bits(c[0]), bits(c[1]));
                     auto ci = cmap.find(ck);
                     if (ci == cmap.end()) {
                         ci = cmap.emplace(ck, (uint32_t)(coords.size() / 2)).first;

@@ -39,7 +39,19 @@ _SIG = {
     "rise_hip_output_write_host": ([_P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(OutputDesc),
                                     ctypes.c_char_p], ctypes.c_int),
     "rise_hip_output_extension": ([ctypes.c_int32], ctypes.c_char_p),
+    "rise_hip_output_resolve": ([_P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(OutputDesc), _P, _P],
+                                ctypes.c_int),
 }
+
+
+def resolve(film, out, desc: OutputDesc, stream=None) -> None:
+    """The per-pixel stage alone on the GPU: film (H, W, 3) float32 CUDA tensor
+    into `out` (a CUDA tensor of the format's pixel bytes)."""
+    import torch
+
+    h, w = film.shape[:2]
+    s = torch.cuda.current_stream().cuda_stream if stream is None else int(getattr(stream, "cuda_stream", stream))
+    _lib.check(_lib_output().rise_hip_output_resolve(film.data_ptr(), w, h, ctypes.byref(desc), out.data_ptr(), s))
 
 
 def _lib_output():

@@ -44,6 +44,24 @@ def test_gpu_output_matches_reference_fixtures(cuda, tmp_path):
         assert _body((tmp_path / name).read_bytes(), ext) == _body((OUT / name).read_bytes(), ext), name
 
 
+def test_gpu_resolve_device_to_device(cuda):
+    import torch
+
+    from rise_amd.output import resolve
+
+    log_case("output resolve device to device")
+    f = np.load(OUT / "film.npy")
+    film = torch.from_numpy(f).to(cuda).contiguous()
+    px = torch.empty((f.shape[0], f.shape[1], 4), dtype=torch.uint8, device=cuda)
+    resolve(film, px, output_desc("PPM", 8, "sRGB", "none", 0.0))
+    torch.cuda.synchronize()
+    ppm = (OUT / "ppm_srgb_none_+0.0.ppm").read_bytes()
+    body = np.frombuffer(ppm[len(b"P6\n37 23\n255\n"):], np.uint8).reshape(f.shape[0], f.shape[1], 3)
+    got = px.cpu().numpy()
+    np.testing.assert_array_equal(got[..., :3], body)
+    assert np.all(got[..., 3] == 255)
+
+
 @pytest.mark.parametrize("fmt,bpp", [("PNG", 8), ("PNG", 16), ("EXR", 16), ("EXR", 32)])
 def test_gpu_png_exr_equal_host(cuda, tmp_path, fmt, bpp):
     import torch
