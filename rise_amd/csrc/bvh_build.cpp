@@ -654,9 +654,12 @@ static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, cons
     out.nodes.emplace_back();
     queue.push_back({0, 0, 1});
     int max_level = 1;
-    for (size_t qi = 0; qi < queue.size(); ++qi) {
-        const Pending pd = queue[qi];
-        max_level = std::max(max_level, pd.level);
+    // One wide node from binary node pd.n2 into onodes[pd.n8]; its internal
+    // children get contiguous slots appended to onodes and are queued on q,
+    // its leaf triangles are appended to otris.
+    auto emit = [&](const Pending& pd, std::vector<NodeSlot>& onodes, std::vector<Tri48>& otris,
+                    std::vector<Pending>& q, int& lvl, std::string& emsg) -> bool {
+        lvl = std::max(lvl, pd.level);
         int32_t kids[8];
         int nk = 0;
         const Node2& root2 = b.nodes[pd.n2];
@@ -717,8 +720,8 @@ static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, cons
         std::memset(&node, 0, sizeof(node));
         // Internal children get contiguous node indices in slot order; leaf
         // triangles are appended contiguously in slot order.
-        uint32_t child_base = (uint32_t)out.nodes.size();
-        uint32_t tri_base = (uint32_t)out.tris.size();
+        uint32_t child_base = (uint32_t)onodes.size();
+        uint32_t tri_base = (uint32_t)otris.size();
         int n_internal = 0;
         uint32_t tri_off = 0;
         float clo[3][8], chi[3][8];
@@ -743,7 +746,7 @@ static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, cons
                 ++n_internal;
             } else {
                 if (leaf_n < 1 || leaf_n > (uint32_t)kMaxLeafCount || tri_off + leaf_n > 32) {
-                    err = "internal error: leaf layout exceeds node limits";
+                    emsg = "internal error: leaf layout exceeds node limits";
                     return false;
                 }
                 node.meta[s] = (uint8_t)((leaf_n << 5) | tri_off);
@@ -760,7 +763,7 @@ static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, cons
                     std::memcpy(&tr.v0[3], &id, 4);
                     tr.v1[3] = 0.0f;
                     tr.v2[3] = 0.0f;
-                    out.tris.push_back(tr);
+                    otris.push_back(tr);
                 }
                 tri_off += leaf_n;
             }
@@ -771,9 +774,9 @@ static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, cons
         for (int s = 0; s < 8; ++s) {
             const int c = child_in[s];
             if (c < 0 || !(node.imask & (1u << s))) continue;  // empty or stored as a leaf
-            const uint32_t idx8 = (uint32_t)out.nodes.size();
-            out.nodes.emplace_back();
-            queue.push_back({kids[c], idx8, pd.level + 1});
+            const uint32_t idx8 = (uint32_t)onodes.size();
+            onodes.emplace_back();
+            q.push_back({kids[c], idx8, pd.level + 1});
         }
         (void)n_internal;
 
@@ -799,7 +802,7 @@ static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, cons
             }
             for (;;) {
                 if (e > 127) {
-                    err = "internal error: child box quantization overflow";
+                    emsg = "internal error: child box quantization overflow";
                     return false;
                 }
                 if (quantize_axis(pp[k], e, clo[k], chi[k], ns, ql[k], qh[k])) break;
@@ -826,7 +829,69 @@ static bool collapse_wide(const float* tris, int64_t n, const int32_t* ids, cons
             node.qhiy[s] = qh[1][i];
             node.qhiz[s] = qh[2][i];
         }
-        static_cast<Node8&>(out.nodes[pd.n8]) = node;
+        static_cast<Node8&>(onodes[pd.n8]) = node;
+        return true;
+    };
+    // Breadth-first from the root until the frontier holds kFrontier subtrees,
+    // then every frontier subtree is collapsed on its own (in parallel) and
+    // the pieces are appended in frontier order: the layout depends only on
+    // the tree, never on the thread count.
+    constexpr size_t kFrontier = 256;
+    size_t qi = 0;
+    for (; qi < queue.size() && queue.size() - qi < kFrontier; ++qi) {
+        const Pending pd = queue[qi];
+        if (!emit(pd, out.nodes, out.tris, queue, max_level, err)) return false;
+    }
+    const size_t n_items = queue.size() - qi;
+    if (n_items > 0) {
+        struct Piece {
+            std::vector<NodeSlot> nodes;
+            std::vector<Tri48> tris;
+            int level = 1;
+            std::string err;
+            bool ok = true;
+        };
+        std::vector<Piece> pieces(n_items);
+        std::atomic<size_t> next_item{0};
+        auto work = [&]() {
+            for (size_t k; (k = next_item.fetch_add(1)) < n_items;) {
+                Piece& P = pieces[k];
+                std::vector<Pending> lq{{queue[qi + k].n2, 0u, queue[qi + k].level}};
+                P.nodes.emplace_back();
+                for (size_t j = 0; j < lq.size() && P.ok; ++j) {
+                    const Pending pd = lq[j];
+                    P.ok = emit(pd, P.nodes, P.tris, lq, P.level, P.err);
+                }
+            }
+        };
+        const int nt = std::max(1, std::min(threads_of(params), (int)n_items));
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+        for (size_t k = 0; k < n_items; ++k) {
+            Piece& P = pieces[k];
+            if (!P.ok) {
+                err = P.err;
+                return false;
+            }
+            // Local node 0 takes the slot its parent reserved; local j >= 1
+            // goes to nbase + j - 1 (a child is never local node 0).
+            const uint32_t nbase = (uint32_t)out.nodes.size(), tbase = (uint32_t)out.tris.size();
+            for (size_t j = 0; j < P.nodes.size(); ++j) {
+                Node8 nd = static_cast<const Node8&>(P.nodes[j]);
+                nd.child_base = nbase + nd.child_base - 1u;
+                nd.tri_base += tbase;
+                if (j == 0) {
+                    static_cast<Node8&>(out.nodes[queue[qi + k].n8]) = nd;
+                } else {
+                    out.nodes.emplace_back();
+                    static_cast<Node8&>(out.nodes.back()) = nd;
+                }
+            }
+            out.tris.insert(out.tris.end(), P.tris.begin(), P.tris.end());
+            max_level = std::max(max_level, P.level);
+        }
     }
     if (max_level > kMaxStack) {
         err = "BVH too deep for the traversal stack (" + std::to_string(max_level) + " levels)";
