@@ -2,7 +2,8 @@
 // Built by tests (hipcc host compile, linked against librise_hip.so).
 //   adapter_demo raycast   closest hit for a grid of rays through a triangle
 //   adapter_demo render    a tiny spectral render of a lit floor
-//   adapter_demo scene F   a scene file loaded by LoadAsciiScene, rendered at 64x64, 4 spp
+//   adapter_demo scene F   a scene file loaded by LoadAsciiScene, rendered at 64x64, 4 spp, written
+//                          through its file_rasterizeroutput chunk to /tmp/adapter_demo_output.img
 // Prints "ok ..." and exits 0 on success.
 #include "rise_hip_adapters.hpp"
 
@@ -108,7 +109,8 @@ static int scene(const char* path)
 {
     HipSpectralRasterizer r;
     rise_hip_render_desc d{};
-    if (!r.LoadAsciiScene(path, &d)) return 1;
+    std::vector<std::pair<rise_hip_output_desc, std::string>> outs;
+    if (!r.LoadAsciiScene(path, &d, &outs)) return 1;
     d.width = d.height = 64;
     d.spp = d.spp_total = 4;
     float* film = nullptr;
@@ -117,12 +119,15 @@ static int scene(const char* path)
     if (!r.RasterizeScene(d, film, nullptr, &st)) return 3;
     std::vector<float> h(64 * 64 * 3);
     hipMemcpy(h.data(), film, h.size() * 4, hipMemcpyDeviceToHost);
+    // the scene's own output chunk, written next to the working directory's temp files
+    const bool wrote = !outs.empty() && HipSpectralRasterizer::OutputImage(film, 64, 64, outs[0].first,
+                                                                           "/tmp/adapter_demo_output.img", nullptr);
     hipFree(film);
     double y = 0;
     for (int i = 0; i < 64 * 64; ++i) y += h[3 * i + 1];
     y /= 64 * 64;
-    if (!(y > 0) || st.paths != 64 * 64 * 4) return 4;
-    std::printf("ok scene meanY %.4f paths %lld\n", y, (long long)st.paths);
+    if (!(y > 0) || st.paths != 64 * 64 * 4 || !wrote) return 4;
+    std::printf("ok scene meanY %.4f paths %lld output %s\n", y, (long long)st.paths, outs[0].second.c_str());
     return 0;
 }
 

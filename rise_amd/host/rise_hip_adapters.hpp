@@ -17,11 +17,13 @@
 #include "rise_hip.h"
 #include "rise_hip_render.h"
 #include "rise_hip_scene_file.h"
+#include "rise_hip_output.h"
 
 #include <cstdint>
 #include <cstdio>
 #include <functional>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace rise_hip_host {
@@ -188,8 +190,11 @@ public:
 
     // Job::LoadAsciiSceneViaCst (Job.h:2843): reads a native scene file and
     // makes its scene resident; *desc receives the render its camera, film
-    // and rasterizer chunks describe.  Replaces anything added before.
-    bool LoadAsciiScene(const char* path, rise_hip_render_desc* desc)
+    // and rasterizer chunks describe, `outputs` (optional) its
+    // file_rasterizeroutput chunks as (description, file name) pairs.
+    // Replaces anything added before.
+    bool LoadAsciiScene(const char* path, rise_hip_render_desc* desc,
+                        std::vector<std::pair<rise_hip_output_desc, std::string>>* outputs = nullptr)
     {
         rise_hip_scene_file* f = nullptr;
         if (!check(rise_hip_scene_file_parse(path, &f), "LoadAsciiScene")) return false;
@@ -197,8 +202,24 @@ public:
         scene_ = nullptr;
         bool ok = check(rise_hip_scene_file_upload(f, &scene_), "LoadAsciiScene (upload)");
         if (ok && desc) ok = check(rise_hip_scene_file_get_render_desc(f, desc), "LoadAsciiScene (render)");
+        rise_hip_scene_file_info info{};
+        if (ok && outputs) ok = check(rise_hip_scene_file_get_info(f, &info), "LoadAsciiScene (info)");
+        for (int32_t i = 0; ok && outputs && i < info.n_outputs; ++i) {
+            rise_hip_output_desc od{};
+            const char* pattern = nullptr;
+            ok = check(rise_hip_scene_file_get_output(f, i, &od, &pattern), "LoadAsciiScene (output)");
+            if (ok) outputs->push_back({od, std::string(pattern) + "." + rise_hip_output_extension(od.format)});
+        }
         rise_hip_scene_file_release(f);
         return ok;
+    }
+
+    // FileRasterizerOutput::OutputImage: writes a rendered film (device,
+    // W*H*3 mean XYZ) to `path` as `od` describes.
+    static bool OutputImage(const float* film_xyz, int width, int height, const rise_hip_output_desc& od,
+                            const char* path, hipStream_t stream)
+    {
+        return check(rise_hip_output_write(film_xyz, width, height, &od, path, stream), "OutputImage");
     }
 
     rise_hip_pt_scene* Scene() const { return scene_; }
