@@ -14,7 +14,7 @@ import itertools
 import time
 
 
-def candidates(persistent=(1, 2), trace_block=(64, 128, 256), max_paths=(1 << 21, 1 << 22), stack=(0,)):
+def candidates(persistent=(1, 2), trace_block=(64, 128, 256), max_paths=(1 << 24, 1 << 25), stack=(0,)):
     for p, b, m, s in itertools.product(persistent, trace_block, max_paths, stack):
         yield {"persistent": p, "trace_block": b, "max_paths": m, "stack": s}
 

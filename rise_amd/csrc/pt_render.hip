@@ -62,7 +62,10 @@ constexpr int kShadeBlock = 512;  // shading / binning workgroups: 8 waves share
 #define RISE_SHADE_GRID 2048      // workgroups of the grid-stride shading / binning launches (at most)
 #endif
 constexpr int64_t kChunkPaths = int64_t(1) << 20;   // paths per chunk (at most)
-constexpr int64_t kMaxPaths = int64_t(1) << 22;     // paths in flight per pass (at most)
+#ifndef RISE_MAX_PATHS_LOG2
+#define RISE_MAX_PATHS_LOG2 25
+#endif
+constexpr int64_t kMaxPaths = int64_t(1) << RISE_MAX_PATHS_LOG2;  // paths in flight per pass (at most)
 constexpr int kCounters = 6 * (kMaxPathDepth + 1);  // extension, shadow, then 4 binned-hit counts, per pass
 constexpr int kBins = 4;                            // hit bins: the 3 material kinds, then volume scatters
 constexpr int kMatVolume = 3;                       // bin of the lanes that scatter inside a medium
