@@ -66,6 +66,20 @@ rise_hip_status rise_hip_risemesh_upload(const rise_hip_risemesh* mesh, rise_hip
 
 rise_hip_status rise_hip_risemesh_release(rise_hip_risemesh* mesh);
 
+/* Scene cache: the resident ray-cast scene's built 8-wide tree (nodes and
+ * triangles, copied back from the device) saved to `path`, and read back and
+ * uploaded with no build at all (stats.bvh_source = 2).  No counterpart in the
+ * reference, whose .risemesh cache holds the binary tree and re-derives its
+ * 4-wide tree on load; for callers that re-create large resident scenes
+ * (servers restarting, scenes evicted and re-admitted).  Layout: "RHBVH8\0\0",
+ * uint32 version 1, uint32 node bytes (80), uint32 depth, uint32 reserved,
+ * uint64 node count, uint64 triangle count, the nodes, the 48-byte
+ * triangles.  A file is checked before upload: sizes, every child and
+ * triangle range in bounds, children after their parent, depth within the
+ * traversal stack. */
+rise_hip_status rise_hip_scene_cache_write(const rise_hip_scene* scene, const char* path);
+rise_hip_status rise_hip_scene_cache_read(const char* path, rise_hip_scene** out_scene);
+
 #ifdef __cplusplus
 }
 #endif

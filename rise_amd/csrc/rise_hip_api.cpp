@@ -1,5 +1,6 @@
 // rise_hip_api.cpp -- C ABI: scene upload and batch ray queries.
 #include "rise_hip.h"
+#include "rise_hip_risemesh.h"
 
 #include "api_common.h"
 #include "bvh_build.h"
@@ -8,6 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <fstream>
 #include <cstring>
 #include <new>
 #include <string>
@@ -225,3 +229,123 @@ rise_hip_status rise_hip_intersect_any_counted(rise_hip_scene* scene, const floa
 }
 
 }  // extern "C"
+
+namespace {
+
+struct CacheHeader {
+    char magic[8];
+    uint32_t version, node_bytes, depth, reserved;
+    uint64_t n_nodes, n_tris;
+};
+static_assert(sizeof(CacheHeader) == 40, "cache header layout");
+constexpr char kCacheMagic[8] = {'R', 'H', 'B', 'V', 'H', '8', 0, 0};
+
+// Bounds and shape of a wide tree read from a file: every internal child and
+// leaf triangle range inside the arrays, children after their parent (so the
+// traversal cannot loop), depth within the traversal stack.
+bool check_wide(const std::vector<NodeSlot>& nodes, uint64_t n_tris, int& depth, std::string& err)
+{
+    const uint64_t nn = nodes.size();
+    std::vector<uint8_t> level(nn, 0);
+    level[0] = 1;
+    depth = 1;
+    for (uint64_t i = 0; i < nn; ++i) {
+        const Node8& n = nodes[i];
+        if (level[i] == 0) continue;  // unreachable from the root: never traversed
+        const unsigned internal = (unsigned)__builtin_popcount(n.imask);
+        if (internal > 0 && (n.child_base <= i || (uint64_t)n.child_base + internal > nn)) {
+            err = "node " + std::to_string(i) + ": child range out of bounds";
+            return false;
+        }
+        for (unsigned k = 0; k < internal; ++k) {
+            level[n.child_base + k] = (uint8_t)std::min(255, level[i] + 1);
+            depth = std::max(depth, level[i] + 1);
+        }
+        for (int s = 0; s < 8; ++s) {
+            if (n.meta[s] == 0 || ((n.imask >> s) & 1u)) continue;
+            const uint64_t cnt = n.meta[s] >> 5, off = n.meta[s] & 31u;
+            if (cnt == 0 || (uint64_t)n.tri_base + off + cnt > n_tris) {
+                err = "node " + std::to_string(i) + ": leaf triangles out of bounds";
+                return false;
+            }
+        }
+    }
+    if (depth > kMaxStack) {
+        err = "tree deeper than the traversal stack";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" rise_hip_status rise_hip_scene_cache_write(const rise_hip_scene* scene, const char* path)
+{
+    clear_error();
+    if (!scene || !path) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    const uint64_t nn = (uint64_t)scene->stats.n_nodes, nt = (uint64_t)scene->stats.n_tris;
+    std::vector<NodeSlot> nodes;
+    std::vector<Tri48> tris;
+    try {
+        nodes.resize(nn);
+        tris.resize(nt);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    hipError_t e = hipSetDevice(scene->device);
+    if (e == hipSuccess) e = hipMemcpy(nodes.data(), scene->d_nodes, nn * sizeof(NodeSlot), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nt) e = hipMemcpy(tris.data(), scene->d_tris, nt * sizeof(Tri48), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "scene cache copy");
+    CacheHeader h{};
+    std::memcpy(h.magic, kCacheMagic, 8);
+    h.version = 1;
+    h.node_bytes = (uint32_t)sizeof(NodeSlot);
+    h.depth = (uint32_t)scene->stats.depth;
+    h.n_nodes = nn;
+    h.n_tris = nt;
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, std::string("cannot create ") + path);
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(nodes.data(), sizeof(NodeSlot), nn, f) == nn &&
+              std::fwrite(tris.data(), sizeof(Tri48), nt, f) == nt;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? RISE_HIP_OK : fail(RISE_HIP_ERR_INVALID_ARGUMENT, std::string("cannot write ") + path);
+}
+
+extern "C" rise_hip_status rise_hip_scene_cache_read(const char* path, rise_hip_scene** out_scene)
+{
+    clear_error();
+    if (!path || !out_scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out_scene = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+    const uint64_t size = (uint64_t)f.tellg();
+    f.seekg(0);
+    CacheHeader h{};
+    if (size < sizeof(h) || !f.read((char*)&h, sizeof(h)) || std::memcmp(h.magic, kCacheMagic, 8) != 0 ||
+        h.version != 1)
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, std::string(path) + ": not a scene cache (version 1)");
+    if (h.node_bytes != sizeof(NodeSlot) || h.n_nodes == 0 || h.n_tris > 0x7fffffffull ||
+        h.n_nodes > (size - sizeof(h)) / sizeof(NodeSlot) ||
+        size != sizeof(h) + h.n_nodes * sizeof(NodeSlot) + h.n_tris * sizeof(Tri48))
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, std::string(path) + ": sizes do not match the file");
+    WideBVH bvh;
+    std::string err;
+    int depth = 0;
+    try {
+        bvh.nodes.resize(h.n_nodes);
+        bvh.tris.resize(h.n_tris);
+        if (!f.read((char*)bvh.nodes.data(), (std::streamsize)(h.n_nodes * sizeof(NodeSlot))) ||
+            !f.read((char*)bvh.tris.data(), (std::streamsize)(h.n_tris * sizeof(Tri48))))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, std::string(path) + ": file truncated");
+        if (!check_wide(bvh.nodes, h.n_tris, depth, err))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, std::string(path) + ": " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    bvh.stats.nodes = (int64_t)h.n_nodes;
+    bvh.stats.tris = (int64_t)h.n_tris;
+    bvh.stats.depth = depth;
+    bvh.stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return upload_scene(bvh, (int64_t)h.n_tris, 2, out_scene);
+}

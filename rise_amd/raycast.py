@@ -11,6 +11,7 @@ whole batch of rays resident in device memory.  Errors raise
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -104,6 +105,23 @@ class ObjectManager:
             )
         _lib.check(st)
         return occluded
+
+    def save_cache(self, path) -> None:
+        """Saves the built 8-wide tree (rise_hip_scene_cache_write)."""
+        L = _lib.lib()
+        L.rise_hip_scene_cache_write.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        _lib.check(L.rise_hip_scene_cache_write(self.handle, os.fsencode(path)))
+
+    @classmethod
+    def load_cache(cls, path) -> "ObjectManager":
+        """A resident scene from a saved tree, with no build (rise_hip_scene_cache_read)."""
+        L = _lib.lib()
+        L.rise_hip_scene_cache_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+        om = cls()
+        out = ctypes.c_void_p(None)
+        _lib.check(L.rise_hip_scene_cache_read(os.fsencode(path), ctypes.byref(out)))
+        om._scene = out
+        return om
 
     def release(self) -> None:
         if self._scene.value:

@@ -193,3 +193,34 @@ def test_scene_file_risemesh_geometry_matches_raw2():
     for x, y in zip(a.objects, b.objects):
         np.testing.assert_array_equal(x.tris, y.tris)
         assert (x.material, x.double_sided) == (y.material, y.double_sided)
+
+
+def _cache_bytes(nodes: bytes, n_nodes: int, n_tris: int, tris: bytes = b"", magic=b"RHBVH8\0\0", node_bytes=80):
+    return magic + struct.pack("<IIII", 1, node_bytes, 1, 0) + struct.pack("<QQ", n_nodes, n_tris) + nodes + tris
+
+
+def _node(imask=0, child_base=0, tri_base=0, meta=(0,) * 8):
+    # origin, exponents, imask, child_base, tri_base, meta, 48 quantized bounds
+    return (struct.pack("<fffbbbB", 0, 0, 0, 0, 0, 0, imask) + struct.pack("<II", child_base, tri_base)
+            + bytes(meta) + bytes(48))
+
+
+def test_scene_cache_files_are_checked(tmp_path):
+    """Malformed scene-cache files fail before anything reaches the device."""
+    from rise_amd.raycast import ObjectManager
+
+    cases = {
+        "magic": _cache_bytes(_node(), 1, 0, magic=b"RHBVH9\0\0"),
+        "node size": _cache_bytes(_node(), 1, 0, node_bytes=128),
+        "sizes": _cache_bytes(_node(), 2, 0),
+        "child range": _cache_bytes(_node(imask=1, child_base=1, meta=(1,) + (0,) * 7), 1, 0),
+        "child before parent": _cache_bytes(_node(imask=1, child_base=0, meta=(1,) + (0,) * 7) + _node(), 2, 0),
+        "leaf range": _cache_bytes(_node(meta=((2 << 5) | 0,) + (0,) * 7), 1, 1, tris=bytes(48)),
+    }
+    want = {"magic": "not a scene cache", "node size": "sizes", "sizes": "sizes", "child range": "child range",
+            "child before parent": "child range", "leaf range": "leaf triangles"}
+    for name, data in cases.items():
+        p = tmp_path / f"{name.replace(' ', '_')}.rhbvh"
+        p.write_bytes(data)
+        with pytest.raises(RiseHipError, match=want[name]):
+            ObjectManager.load_cache(p)
