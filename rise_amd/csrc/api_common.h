@@ -14,6 +14,11 @@ rise_hip_status fail(rise_hip_status s, const std::string& msg);
 rise_hip_status fail_hip(hipError_t e, const char* what);
 void clear_error();
 
+// Host threads for the library's own pools (BVH build, table generation,
+// host-side checks): OMP_NUM_THREADS when set, else the hardware threads, at
+// most 16 either way, so a pool never outgrows the CPU share of one GPU.
+int host_threads();
+
 struct WideBVH;
 // Uploads a built wide BVH as a ray-cast scene (rise_hip_scene_upload's
 // device part); bvh_source goes to the scene's statistics.

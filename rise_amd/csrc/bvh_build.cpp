@@ -2,6 +2,8 @@
 // See bvh_build.h for the reference lines this follows.
 #include "bvh_build.h"
 
+#include "api_common.h"
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -321,7 +323,7 @@ bool prepare_prims(const float* tris, int64_t n, Bvh2Builder& b, WideBVH& out, s
 
 int threads_of(const BuildParams& params)
 {
-    return params.threads > 0 ? params.threads : (int)std::thread::hardware_concurrency();
+    return params.threads > 0 ? params.threads : host_threads();
 }
 
 }  // namespace

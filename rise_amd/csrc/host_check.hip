@@ -152,7 +152,7 @@ rise_hip_status trace_host(const WideBVH& bvh, int64_t n_tris, int32_t bvh_sourc
         cn += ln;
         ct += lt;
     };
-    const int nth = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = host_threads();
     std::vector<std::thread> th;
     for (int k = 1; k < nth; ++k) th.emplace_back(worker);
     worker();

@@ -395,7 +395,7 @@ extern "C" rise_hip_status rise_hip_output_write_host(const float* film, int32_t
     const int64_t n = (int64_t)w * h;
     try {
         std::vector<uint8_t> px((size_t)n * pl.bytes_per_px);
-        const int nt = std::max(1u, std::thread::hardware_concurrency());
+        const int nt = host_threads();
         std::vector<std::thread> th;
         for (int t = 0; t < nt; ++t)
             th.emplace_back([&, t] {

@@ -254,7 +254,7 @@ extern "C" rise_hip_status rise_hip_rgb_lut_generate(int32_t resolution, int32_t
             for (int p; (p = next.fetch_add(1)) < planes;) solve_plane(p / resolution, p % resolution, resolution,
                                                                        L->coeffs.data(), ps[(size_t)p]);
         };
-        int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+        int nt = threads > 0 ? threads : host_threads();
         nt = std::max(1, std::min(nt, planes));
         std::vector<std::thread> th;
         for (int k = 1; k < nt; ++k) th.emplace_back(worker);

@@ -8,7 +8,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
+#include <cstdlib>
+#include <thread>
 #include <chrono>
 #include <cstdio>
 #include <fstream>
@@ -53,6 +56,16 @@ rise_hip_status fail_hip(hipError_t e, const char* what)
 }
 
 void clear_error() { g_last_error.clear(); }
+
+int host_threads()
+{
+    int n = (int)std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+        const int v = std::atoi(e);
+        if (v > 0) n = v;
+    }
+    return std::max(1, std::min(n, 16));
+}
 
 }  // namespace rise_hip
 
