@@ -509,8 +509,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU only: this build has no multi-GPU path "
-            "(the film/request sharding across GPUs is described in DESIGN.md).",
+            "bench.py measures one GPU only: there is no multi-GPU path (a render would shard by image "
+            "tiles or sample ranges, serving by whole requests).",
             file=sys.stderr,
         )
         return 2
