@@ -16,6 +16,7 @@
 
 #include "rise_hip.h"
 #include "rise_hip_render.h"
+#include "rise_hip_risemesh.h"
 #include "rise_hip_scene_file.h"
 #include "rise_hip_output.h"
 
@@ -77,6 +78,16 @@ public:
                             hipStream_t stream) const
     {
         return check(rise_hip_intersect_any(scene_, rays_soa, tmax, n, occluded, stream), "IntersectShadowRay");
+    }
+
+    // The built 8-wide tree saved to / reloaded from a scene cache file
+    // (rise_hip_scene_cache_*): a reload replaces PrepareForRendering with no build.
+    bool SaveCache(const char* path) const { return check(rise_hip_scene_cache_write(scene_, path), "SaveCache"); }
+    bool LoadCache(const char* path)
+    {
+        rise_hip_scene_release(scene_);
+        scene_ = nullptr;
+        return check(rise_hip_scene_cache_read(path, &scene_), "LoadCache");
     }
 
     bool GetStats(rise_hip_scene_stats& out) const
