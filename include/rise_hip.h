@@ -91,6 +91,27 @@ rise_hip_status rise_hip_scene_release(rise_hip_scene* scene);
 
 rise_hip_status rise_hip_scene_get_stats(const rise_hip_scene* scene, rise_hip_scene_stats* out_stats);
 
+/* Arithmetic of the reference build whose hits the ray queries reproduce bit
+ * for bit.  The hit decision is the reference's leaf test
+ * (Acceleration/BVH.h:1237-1301): the float pre-filter
+ * BVH::MollerTrumboreFloat (BVH.h:1111-1167) on the float normalized ray,
+ * then the double certifier RayTriangleIntersection
+ * (Intersection/RayTriangleIntersection.cpp:21-68) on the ray normalized by
+ * Vector3Ops::Normalize (Object.cpp:414-425).
+ *   RISE_HIP_ARITH_IEEE: every operation rounded on its own, in the order the
+ *     reference's source writes it (the default).
+ *   RISE_HIP_ARITH_X86_FMA: as the reference's Linux build compiles it
+ *     (g++ -O3 -march=x86-64-v3, products contracted into fused multiply-adds:
+ *     a*b - c*d -> fma(a, b, -(c*d)), a*b + c*d + e*f -> fma(e, f, fma(a, b, c*d))).
+ * Applies to rise_hip_intersect_* calls made after it returns.  No
+ * counterpart in the reference (a property of how it is compiled). */
+typedef enum rise_hip_arith {
+    RISE_HIP_ARITH_IEEE = 0,
+    RISE_HIP_ARITH_X86_FMA = 1
+} rise_hip_arith;
+
+rise_hip_status rise_hip_scene_set_arithmetic(rise_hip_scene* scene, rise_hip_arith arith);
+
 /* ------------------------------------------------------------------------ */
 /* Batch ray queries                                                        */
 /* ------------------------------------------------------------------------ */

@@ -18,10 +18,12 @@
 extern "C" {
 #endif
 
-/* mode 0: closest hit (fills prim_id, t); mode 1: any hit (fills occluded;
- * tmax may be NULL).  All buffers are host memory; rays_soa as in
- * rise_hip_intersect_closest.  counters (optional, 2 uint64) receive nodes
- * visited and triangles certified; stats (optional) the build statistics. */
+/* mode bit 0 clear: closest hit (fills prim_id, t); set: any hit (fills
+ * occluded; tmax may be NULL).  mode bit 1 set: RISE_HIP_ARITH_X86_FMA
+ * arithmetic, else RISE_HIP_ARITH_IEEE.  All buffers are host memory;
+ * rays_soa as in rise_hip_intersect_closest.  counters (optional, 2 uint64)
+ * receive nodes visited and leaf triangles tested; stats (optional) the
+ * build statistics. */
 rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t n_tris, const float* rays_soa,
                                           const float* tmax, int64_t n, int32_t mode, int32_t* prim_id,
                                           float* t, uint8_t* occluded, uint64_t* counters,

@@ -48,8 +48,10 @@ def oracle_lib() -> ctypes.CDLL:
         L.rise_oracle_ray_triangle.argtypes = [_P] * 5 + [_P, _P, _P]
         L.rise_oracle_ray_triangle.restype = ctypes.c_int
         L.rise_oracle_normalize.argtypes = [_P, _P]
-        L.rise_oracle_closest_brute.argtypes = [_P, _I64, _P, _I64, _P, _P, _P, ctypes.c_int]
-        L.rise_oracle_any_brute.argtypes = [_P, _I64, _P, _P, _I64, _P, ctypes.c_int]
+        L.rise_oracle_closest_brute.argtypes = [_P, _I64, _P, _I64, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int]
+        L.rise_oracle_any_brute.argtypes = [_P, _I64, _P, _P, _I64, _P, ctypes.c_int, ctypes.c_int]
+        L.rise_oracle_filter_float.argtypes = [_P] * 5 + [ctypes.c_float]
+        L.rise_oracle_filter_float.restype = ctypes.c_int
         _oracle = L
     return _oracle
 
@@ -66,8 +68,24 @@ def ray_triangle(o, d, v0, eA, eB):
     return bool(hit), float(out[0]), float(out[1]), float(out[2])
 
 
-def closest(tris: np.ndarray, rays: np.ndarray, threads: int = 0):
-    """Brute-force closest hit.  Returns (prim_id int32, t float32, t64 float64)."""
+NO_FILTER = 1
+X86_FMA = 2
+
+
+def _opts(no_filter: bool, arith: str) -> int:
+    if arith not in ("ieee", "fast"):
+        raise ValueError("arith is 'ieee' or 'fast'")
+    return (NO_FILTER if no_filter else 0) | (X86_FMA if arith == "fast" else 0)
+
+
+def closest(tris: np.ndarray, rays: np.ndarray, threads: int = 0, flags: bool = False, no_filter: bool = False,
+            arith: str = "ieee"):
+    """Brute-force closest hit.  Returns (prim_id int32, t float32, t64 float64),
+    plus, with flags=True, a uint8 array marking rays whose result depends on
+    traversal order (the winner fails the float filter's distance test against
+    its own t).  no_filter=True drops the float pre-filter (certifier only).
+    arith: "ieee" (every operation rounded) or "fast" (the reference's Linux
+    build's fused multiply-adds)."""
     L = oracle_lib()
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -75,12 +93,15 @@ def closest(tris: np.ndarray, rays: np.ndarray, threads: int = 0):
     pid = np.empty(n, dtype=np.int32)
     t = np.empty(n, dtype=np.float32)
     t64 = np.empty(n, dtype=np.float64)
+    fl = np.empty(n, dtype=np.uint8) if flags else None
     L.rise_oracle_closest_brute(tris.ctypes.data, tris.shape[0], rays.ctypes.data, n, pid.ctypes.data,
-                                t.ctypes.data, t64.ctypes.data, threads)
-    return pid, t, t64
+                                t.ctypes.data, t64.ctypes.data, None if fl is None else fl.ctypes.data,
+                                _opts(no_filter, arith), threads)
+    return (pid, t, t64, fl) if flags else (pid, t, t64)
 
 
-def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, threads: int = 0) -> np.ndarray:
+def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, threads: int = 0,
+            no_filter: bool = False, arith: str = "ieee") -> np.ndarray:
     L = oracle_lib()
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -88,8 +109,16 @@ def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, 
     occ = np.empty(n, dtype=np.uint8)
     tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
     L.rise_oracle_any_brute(tris.ctypes.data, tris.shape[0], rays.ctypes.data,
-                            None if tm is None else tm.ctypes.data, n, occ.ctypes.data, threads)
+                            None if tm is None else tm.ctypes.data, n, occ.ctypes.data, _opts(no_filter, arith),
+                            threads)
     return occ
+
+
+def filter_float(o, d, p0, e1, e2, best: float) -> bool:
+    """BVH::MollerTrumboreFloat on float32 inputs."""
+    L = oracle_lib()
+    arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, p0, e1, e2)]
+    return bool(L.rise_oracle_filter_float(*[a.ctypes.data for a in arrs], ctypes.c_float(best)))
 
 
 class RefMesh:

@@ -377,17 +377,31 @@ static void release(prep* P)
 
 /* ---- ray queries (rt_oracle semantics, inlined for speed) ---------------- */
 
-/* Closest hit for a unit-direction ray; returns tri or -1 and the MT t. */
+/* The float pre-filter's record and ray (BVH.h:431-467, :1212-1225): float
+ * vertex, float edges, float origin and direction. */
+static int filter_tri(const double* f, const float fo[3], const float fd[3], float best)
+{
+    const float p0[3] = {(float)f[0], (float)f[1], (float)f[2]};
+    const float e1[3] = {(float)f[3] - p0[0], (float)f[4] - p0[1], (float)f[5] - p0[2]};
+    const float e2[3] = {(float)f[6] - p0[0], (float)f[7] - p0[1], (float)f[8] - p0[2]};
+    return rise_oracle_filter_float(fo, fd, p0, e1, e2, best);
+}
+
+/* Closest hit for a unit-direction ray; returns tri or -1 and the MT t.
+ * Each triangle passes the float pre-filter against the running best
+ * (Bvh4Leaf_Geometric, BVH.h:1237-1254) before the double certifier. */
 static int64_t closest(const prep* P, v3 o, v3 dn, double* t_out)
 {
     double best = RISE_ORACLE_INFINITY;
     int64_t id = -1;
     const double oo[3] = {o.x, o.y, o.z}, dd[3] = {dn.x, dn.y, dn.z};
+    const float fo[3] = {(float)o.x, (float)o.y, (float)o.z}, fd[3] = {(float)dn.x, (float)dn.y, (float)dn.z};
     for (int64_t k = 0; k < P->s->n_tris; ++k) {
         const double* f = P->v + 9 * k;
         const double eA[3] = {f[3] - f[0], f[4] - f[1], f[5] - f[2]};
         const double eB[3] = {f[6] - f[0], f[7] - f[1], f[8] - f[2]};
         double t;
+        if (!filter_tri(f, fo, fd, (float)best)) continue;
         if (rise_oracle_ray_triangle(oo, dd, f, eA, eB, &t, NULL, NULL) && t < best) {
             best = t;
             id = k;
@@ -409,12 +423,14 @@ static int occluded(const prep* P, v3 o, v3 d, double how_far)
     double how_far2 = how_far;
     if ((how_far != RISE_ORACLE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
     const double oo[3] = {o.x, o.y, o.z}, dd[3] = {dn.x, dn.y, dn.z};
+    const float fo[3] = {(float)o.x, (float)o.y, (float)o.z}, fd[3] = {(float)dn.x, (float)dn.y, (float)dn.z};
     for (int64_t k = 0; k < P->s->n_tris; ++k) {
         const double* f = P->v + 9 * k;
         const double eA[3] = {f[3] - f[0], f[4] - f[1], f[5] - f[2]};
         const double eB[3] = {f[6] - f[0], f[7] - f[1], f[8] - f[2]};
         double t;
         if (casts && !casts[P->s->tri_object[k]]) continue;
+        if (!filter_tri(f, fo, fd, (float)how_far2)) continue;
         if (rise_oracle_ray_triangle(oo, dd, f, eA, eB, &t, NULL, NULL) && t > NZ && t < how_far2) return 1;
     }
     return 0;

@@ -16,11 +16,37 @@
 
 #define NZ RISE_ORACLE_NEARZERO
 
+/* The two arithmetic models (RISE_ORACLE_X86_FMA in opts): every operation
+ * rounded on its own, or the contraction g++ -O3 -march=x86-64-v3 applies to
+ * the reference (a*b - c*d -> fma(a, b, -(c*d)); a*b + c*d + e*f ->
+ * fma(e, f, fma(a, b, c*d)); read from its compiled BVH::IntersectRay4,
+ * RayTriangleIntersection and Vector3Ops code).  This file is built with
+ * -ffp-contract=off, so only the explicit fma calls fuse. */
+static inline double sp64(int x, double a, double b, double c, double d)
+{
+    return x ? fma(a, b, -(c * d)) : a * b - c * d;
+}
+static inline double d364(int x, double a0, double b0, double a1, double b1, double a2, double b2)
+{
+    return x ? fma(a2, b2, fma(a0, b0, a1 * b1)) : a0 * b0 + a1 * b1 + a2 * b2;
+}
+static inline float sp32(int x, float a, float b, float c, float d)
+{
+    return x ? fmaf(a, b, -(c * d)) : a * b - c * d;
+}
+static inline float d332(int x, float a0, float b0, float a1, float b1, float a2, float b2)
+{
+    return x ? fmaf(a2, b2, fmaf(a0, b0, a1 * b1)) : a0 * b0 + a1 * b1 + a2 * b2;
+}
+
 /* Vector3Ops::Normalize (Utilities/Math3D/VectorsOps.h:229-238):
  * mag = sqrt(x*x + y*y + z*z); if mag > 0, v * (1/mag). */
-void rise_oracle_normalize(const double v[3], double out[3])
+static void normalize_x(int x, const double v[3], double out[3]);
+void rise_oracle_normalize(const double v[3], double out[3]) { normalize_x(0, v, out); }
+
+static void normalize_x(int x, const double v[3], double out[3])
 {
-    const double mag = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double mag = sqrt(d364(x, v[0], v[0], v[1], v[1], v[2], v[2]));
     if (mag > 0.0) {
         const double ov = 1.0 / mag;
         out[0] = v[0] * ov;
@@ -37,30 +63,29 @@ void rise_oracle_normalize(const double v[3], double out[3])
  * Cross = (a.y*b.z-a.z*b.y, a.z*b.x-a.x*b.z, a.x*b.y-a.y*b.x) and
  * Dot = a.x*b.x + a.y*b.y + a.z*b.z (VectorsOps.h:191-200), evaluated left
  * to right. */
-int rise_oracle_ray_triangle(const double o[3], const double d[3],
-                             const double v0[3], const double eA[3], const double eB[3],
-                             double* t, double* alpha, double* beta)
+static int ray_triangle_x(int x, const double o[3], const double d[3], const double v0[3], const double eA[3],
+                          const double eB[3], double* t, double* alpha, double* beta)
 {
-    const double p0 = d[1] * eB[2] - d[2] * eB[1];
-    const double p1 = d[2] * eB[0] - d[0] * eB[2];
-    const double p2 = d[0] * eB[1] - d[1] * eB[0];
-    const double det = eA[0] * p0 + eA[1] * p1 + eA[2] * p2;
+    const double p0 = sp64(x, d[1], eB[2], d[2], eB[1]);
+    const double p1 = sp64(x, d[2], eB[0], d[0], eB[2]);
+    const double p2 = sp64(x, d[0], eB[1], d[1], eB[0]);
+    const double det = d364(x, eA[0], p0, eA[1], p1, eA[2], p2);
     if (det > -NZ && det < NZ)
         return 0;
     const double oodet = 1.0 / det;
     const double t0 = o[0] - v0[0];
     const double t1 = o[1] - v0[1];
     const double t2 = o[2] - v0[2];
-    const double a = (t0 * p0 + t1 * p1 + t2 * p2) * oodet;
+    const double a = d364(x, t0, p0, t1, p1, t2, p2) * oodet;
     if ((a < 0.0 - NZ) || (a > 1.0 + NZ))
         return 0;
-    const double q0 = t1 * eA[2] - t2 * eA[1];
-    const double q1 = t2 * eA[0] - t0 * eA[2];
-    const double q2 = t0 * eA[1] - t1 * eA[0];
-    const double b = (d[0] * q0 + d[1] * q1 + d[2] * q2) * oodet;
+    const double q0 = sp64(x, t1, eA[2], t2, eA[1]);
+    const double q1 = sp64(x, t2, eA[0], t0, eA[2]);
+    const double q2 = sp64(x, t0, eA[1], t1, eA[0]);
+    const double b = d364(x, d[0], q0, d[1], q1, d[2], q2) * oodet;
     if ((b < 0.0 - NZ) || ((a + b) > 1.0 + NZ))
         return 0;
-    const double r = (eB[0] * q0 + eB[1] * q1 + eB[2] * q2) * oodet;
+    const double r = d364(x, eB[0], q0, eB[1], q1, eB[2], q2) * oodet;
     if (r >= NZ) {
         *t = r;
         if (alpha) *alpha = a;
@@ -70,12 +95,62 @@ int rise_oracle_ray_triangle(const double o[3], const double d[3],
     return 0;
 }
 
+int rise_oracle_ray_triangle(const double o[3], const double d[3],
+                             const double v0[3], const double eA[3], const double eB[3],
+                             double* t, double* alpha, double* beta)
+{
+    return ray_triangle_x(0, o, d, v0, eA, eB, t, alpha, beta);
+}
+
+/* BVH::MollerTrumboreFloat (Acceleration/BVH.h:1111-1167): the float
+ * pre-filter every leaf triangle passes before the double certifier
+ * (Bvh4Leaf_*, BVH.h:1237-1301).  p0, e1 = v1 - v0, e2 = v2 - v0 are float
+ * (BuildFastFilter, BVH.h:431-467); o and d are the float origin and the
+ * float normalized direction (PrepRayFloat, BVH.h:1212-1225); best is
+ * (float) of the current range.  Every operation is one IEEE float
+ * operation in the reference's order (this file is built without
+ * contraction).  Returns 1 when the triangle may be hit. */
+static int filter_x(int x, const float o[3], const float d[3], const float p0[3], const float e1[3],
+                    const float e2[3], float best)
+{
+    const float eps = 1e-5f;
+    const float pv0 = sp32(x, d[1], e2[2], d[2], e2[1]);
+    const float pv1 = sp32(x, d[2], e2[0], d[0], e2[2]);
+    const float pv2 = sp32(x, d[0], e2[1], d[1], e2[0]);
+    const float det = d332(x, e1[0], pv0, e1[1], pv1, e1[2], pv2);
+    if (fabsf(det) < 1e-20f)
+        return 0;
+    const float inv = 1.0f / det;
+    const float tv0 = o[0] - p0[0], tv1 = o[1] - p0[1], tv2 = o[2] - p0[2];
+    const float u = d332(x, tv0, pv0, tv1, pv1, tv2, pv2) * inv;
+    if (u < -eps || u > 1.0f + eps)
+        return 0;
+    const float qv0 = sp32(x, tv1, e1[2], tv2, e1[1]);
+    const float qv1 = sp32(x, tv2, e1[0], tv0, e1[2]);
+    const float qv2 = sp32(x, tv0, e1[1], tv1, e1[0]);
+    const float v = d332(x, d[0], qv0, d[1], qv1, d[2], qv2) * inv;
+    if (v < -eps || u + v > 1.0f + eps)
+        return 0;
+    const float t = d332(x, e2[0], qv0, e2[1], qv1, e2[2], qv2) * inv;
+    if (t < -eps || t > best + eps)
+        return 0;
+    return 1;
+}
+
+int rise_oracle_filter_float(const float o[3], const float d[3], const float p0[3], const float e1[3],
+                             const float e2[3], float best)
+{
+    return filter_x(0, o, d, p0, e1, e2, best);
+}
+
 typedef struct {
     double v0[3], eA[3], eB[3];
+    float p0[3], e1[3], e2[3];
 } tri64;
 
 /* Edges in double from the (float-valued) vertices, as
- * TriangleMeshGeometryIndexedSpecializations.h:141-142 computes them. */
+ * TriangleMeshGeometryIndexedSpecializations.h:141-142 computes them, and
+ * the float filter record of BuildFastFilter (BVH.h:457-459). */
 static tri64* expand_tris(const float* tris, int64_t n)
 {
     tri64* out = (tri64*)malloc(sizeof(tri64) * (size_t)(n > 0 ? n : 1));
@@ -86,6 +161,9 @@ static tri64* expand_tris(const float* tris, int64_t n)
             out[i].v0[k] = a;
             out[i].eA[k] = b - a;
             out[i].eB[k] = c - a;
+            out[i].p0[k] = f[k];
+            out[i].e1[k] = f[3 + k] - f[k];
+            out[i].e2[k] = f[6 + k] - f[k];
         }
     }
     return out;
@@ -100,7 +178,9 @@ typedef struct {
     int32_t* prim_id;
     float* t;
     double* t64;
+    uint8_t* flags;
     uint8_t* occluded;
+    int opts;
     int64_t next;
     pthread_mutex_t lock;
 } job;
@@ -125,6 +205,16 @@ static void load_ray(const float* rays, int64_t n, int64_t i, double o[3], doubl
     d[2] = rays[5 * n + i];
 }
 
+/* PrepRayFloat (BVH.h:1212-1225): the float origin and float direction of
+ * the object-space ray (normalized in double first). */
+static void float_ray(const double o[3], const double dn[3], float fo[3], float fd[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        fo[k] = (float)o[k];
+        fd[k] = (float)dn[k];
+    }
+}
+
 static void* closest_worker(void* arg)
 {
     job* j = (job*)arg;
@@ -137,13 +227,20 @@ static void* closest_worker(void* arg)
             load_ray(j->rays, j->n_rays, i, o, dw);
             /* Object.cpp:414-425: object-space direction normalized in
              * double (identity transform leaves o and d unchanged). */
-            rise_oracle_normalize(dw, dn);
+            const int x = (j->opts & RISE_ORACLE_X86_FMA) != 0;
+            const int nof = (j->opts & RISE_ORACLE_NO_FILTER) != 0;
+            normalize_x(x, dw, dn);
+            float fo[3], fd[3];
+            float_ray(o, dn, fo, fd);
             double best = RISE_ORACLE_INFINITY;
             int64_t best_id = -1;
             for (int64_t k = 0; k < j->n_tris; ++k) {
                 double tt;
-                if (rise_oracle_ray_triangle(o, dn, j->tris[k].v0, j->tris[k].eA, j->tris[k].eB,
-                                             &tt, NULL, NULL)) {
+                /* The filter's distance test runs against +inf here; the
+                 * winner's own test is checked below. */
+                if (!nof && !filter_x(x, fo, fd, j->tris[k].p0, j->tris[k].e1, j->tris[k].e2, INFINITY))
+                    continue;
+                if (ray_triangle_x(x, o, dn, j->tris[k].v0, j->tris[k].eA, j->tris[k].eB, &tt, NULL, NULL)) {
                     /* Strict-closer commit (...Specializations.h:172); k
                      * ascends, so an exact tie keeps the lowest id. */
                     if (tt < best) {
@@ -152,16 +249,25 @@ static void* closest_worker(void* arg)
                     }
                 }
             }
+            if (j->flags) {
+                /* In a traversal the filter compares the float t with the best
+                 * range found so far, which is never below the winner's own
+                 * t.  The winner therefore passes in every traversal order iff
+                 * it passes against its own t; otherwise the result depends on
+                 * the order and the ray is flagged. */
+                const tri64* w = best_id >= 0 ? &j->tris[best_id] : NULL;
+                j->flags[i] = (w && !nof && !filter_x(x, fo, fd, w->p0, w->e1, w->e2, (float)best)) ? 1 : 0;
+            }
             if (best_id >= 0) {
                 /* Object.cpp:641-643: back off by SURFACE_INTERSEC_ERROR,
                  * then the world range |p - origin| (PointAtLength =
                  * origin + dir*s, Utilities/Ray.h:91-94). */
                 const double s2 = best - 1e-12;
-                const double px = o[0] + dn[0] * s2;
-                const double py = o[1] + dn[1] * s2;
-                const double pz = o[2] + dn[2] * s2;
+                const double px = x ? fma(dn[0], s2, o[0]) : o[0] + dn[0] * s2;
+                const double py = x ? fma(dn[1], s2, o[1]) : o[1] + dn[1] * s2;
+                const double pz = x ? fma(dn[2], s2, o[2]) : o[2] + dn[2] * s2;
                 const double vx = px - o[0], vy = py - o[1], vz = pz - o[2];
-                const double range = sqrt(vx * vx + vy * vy + vz * vz);
+                const double range = sqrt(d364(x, vx, vx, vy, vy, vz, vz));
                 j->prim_id[i] = (int32_t)best_id;
                 j->t[i] = (float)range;
                 if (j->t64) j->t64[i] = range;
@@ -185,22 +291,30 @@ static void* any_worker(void* arg)
         for (int64_t i = s; i < e; ++i) {
             double o[3], dw[3], dn[3];
             load_ray(j->rays, j->n_rays, i, o, dw);
-            rise_oracle_normalize(dw, dn);
+            const int x = (j->opts & RISE_ORACLE_X86_FMA) != 0;
+            const int nof = (j->opts & RISE_ORACLE_NO_FILTER) != 0;
+            normalize_x(x, dw, dn);
             /* Object.cpp:472-493: factor = |M^-1 d| (identity -> |d|);
              * dHowFar is scaled unless it is the RISE_INFINITY sentinel and
              * factor >= 1. */
-            const double mag = sqrt(dw[0] * dw[0] + dw[1] * dw[1] + dw[2] * dw[2]);
+            const double mag = sqrt(d364(x, dw[0], dw[0], dw[1], dw[1], dw[2], dw[2]));
             const double factor = (mag > NZ) ? mag : 1.0;
             const double how_far = j->tmax ? (double)j->tmax[i] : RISE_ORACLE_INFINITY;
             double how_far2 = how_far;
             if ((how_far != RISE_ORACLE_INFINITY) || (factor < 1.0))
                 how_far2 = factor * how_far;
+            float fo[3], fd[3];
+            float_ray(o, dn, fo, fd);
+            /* Bvh4Leaf_IntersectionOnly (BVH.h:1278-1301): the filter runs
+             * against (float)dHowFar2. */
+            const float fbest = (float)how_far2;
             uint8_t occ = 0;
             for (int64_t k = 0; k < j->n_tris && !occ; ++k) {
                 double tt;
+                if (!nof && !filter_x(x, fo, fd, j->tris[k].p0, j->tris[k].e1, j->tris[k].e2, fbest))
+                    continue;
                 /* ...Specializations.h:517-519 */
-                if (rise_oracle_ray_triangle(o, dn, j->tris[k].v0, j->tris[k].eA, j->tris[k].eB,
-                                             &tt, NULL, NULL) &&
+                if (ray_triangle_x(x, o, dn, j->tris[k].v0, j->tris[k].eA, j->tris[k].eB, &tt, NULL, NULL) &&
                     tt > NZ && tt < how_far2)
                     occ = 1;
             }
@@ -227,7 +341,8 @@ static void run(job* j, void* (*fn)(void*), int n_threads)
 
 void rise_oracle_closest_brute(const float* tris, int64_t n_tris,
                                const float* rays, int64_t n_rays,
-                               int32_t* prim_id, float* t, double* t64, int n_threads)
+                               int32_t* prim_id, float* t, double* t64, uint8_t* flags,
+                               int opts, int n_threads)
 {
     job j;
     memset(&j, 0, sizeof(j));
@@ -239,13 +354,15 @@ void rise_oracle_closest_brute(const float* tris, int64_t n_tris,
     j.prim_id = prim_id;
     j.t = t;
     j.t64 = t64;
+    j.flags = flags;
+    j.opts = opts;
     run(&j, closest_worker, n_threads);
     free(tt);
 }
 
 void rise_oracle_any_brute(const float* tris, int64_t n_tris,
                            const float* rays, const float* tmax, int64_t n_rays,
-                           uint8_t* occluded, int n_threads)
+                           uint8_t* occluded, int opts, int n_threads)
 {
     job j;
     memset(&j, 0, sizeof(j));
@@ -256,6 +373,7 @@ void rise_oracle_any_brute(const float* tris, int64_t n_tris,
     j.tmax = tmax;
     j.n_rays = n_rays;
     j.occluded = occluded;
+    j.opts = opts;
     run(&j, any_worker, n_threads);
     free(tt);
 }

@@ -14,6 +14,10 @@
  *     "exact-t ties go to the lowest triangle id" (the reference resolves
  *     ties by traversal order).
  *   - Any-hit: ...Specializations.h:476-519 (hit iff NEARZERO < t < dHowFar).
+ *   - Leaf pre-filter: BVH::MollerTrumboreFloat (Acceleration/BVH.h:1111-1167)
+ *     on BuildFastFilter's float records (BVH.h:431-467) and PrepRayFloat's
+ *     float ray (BVH.h:1212-1225), applied before the certifier as
+ *     Bvh4Leaf_* do (BVH.h:1237-1301).
  *   - Object layer for an identity transform: Objects/Object.cpp:414-425
  *     (direction normalized in double via Vector3Ops::Normalize,
  *     Utilities/Math3D/VectorsOps.h:229-238) and Object.cpp:641-643
@@ -27,7 +31,11 @@
  * Pinning: every function here is checked in tests/test_oracle_rt.py against
  * the reference's own KATs (tests/RayTriangleIntersectionTest.cpp), the
  * reference's BVH4 path compiled from its sources (oracle/_ref), and
- * closed-form hits on axis-aligned triangles.
+ * closed-form hits on axis-aligned triangles.  The filter is pinned by rays
+ * aimed within 3e-5 (barycentric) of sliver-triangle edges, where it rejects
+ * hits the certifier would accept: the oracle must give the reference's ids
+ * on every such ray (live against oracle/_ref, and the committed fixture
+ * tests/golden/raycast_filter.npz), and without the filter it must not.
  */
 #ifndef RISE_RT_ORACLE_H
 #define RISE_RT_ORACLE_H
@@ -42,6 +50,12 @@ extern "C" {
 #define RISE_ORACLE_NEARZERO 1e-12
 #define RISE_ORACLE_INFINITY 1.7976931348623158e+308
 
+/* opts bits of the brute-force queries. */
+#define RISE_ORACLE_NO_FILTER 1 /* the double certifier alone, no float pre-filter */
+#define RISE_ORACLE_X86_FMA 2   /* the reference's Linux build's contraction (g++ -O3
+                                   -march=x86-64-v3): a*b - c*d -> fma(a, b, -(c*d)),
+                                   a*b + c*d + e*f -> fma(e, f, fma(a, b, c*d)) */
+
 /* RayTriangleIntersection (RayTriangleIntersection.cpp:21-68).  Returns 1 on
  * hit and fills t/alpha/beta.  edgeA = v1 - v0, edgeB = v2 - v0. */
 int rise_oracle_ray_triangle(const double o[3], const double d[3],
@@ -51,22 +65,31 @@ int rise_oracle_ray_triangle(const double o[3], const double d[3],
 /* Vector3Ops::Normalize (VectorsOps.h:229-238). */
 void rise_oracle_normalize(const double v[3], double out[3]);
 
+/* BVH::MollerTrumboreFloat (BVH.h:1111-1167).  Returns 1 when the triangle
+ * (float p0, e1 = v1 - v0, e2 = v2 - v0) passes for the float ray (o, d)
+ * against the float range `best`. */
+int rise_oracle_filter_float(const float o[3], const float d[3], const float p0[3], const float e1[3],
+                             const float e2[3], float best);
+
 /* Closest hit over all triangles, brute force.
  * tris: n_tris * 9 floats (v0.xyz, v1.xyz, v2.xyz).
  * rays: SoA, 6 arrays of n_rays floats each (ox, oy, oz, dx, dy, dz).
  * Output: prim_id (-1 on miss) and t = world distance (float), +inf on miss.
  * Optional t64 (may be NULL) receives the double world distance.
- * n_threads <= 0 means one per online CPU. */
+ * Optional flags (may be NULL): 1 where the result depends on traversal
+ * order (the winner fails the filter's distance test against its own t).
+ * opts: RISE_ORACLE_* bits.  n_threads <= 0 means one per online CPU. */
 void rise_oracle_closest_brute(const float* tris, int64_t n_tris,
                                const float* rays, int64_t n_rays,
-                               int32_t* prim_id, float* t, double* t64, int n_threads);
+                               int32_t* prim_id, float* t, double* t64, uint8_t* flags,
+                               int opts, int n_threads);
 
 /* Any-hit over all triangles, brute force.  tmax: per-ray world distance
  * limit (dHowFar); NULL means RISE_INFINITY.  occluded[i] = 1 iff some
  * triangle has NEARZERO < t < dHowFar2 (Object.cpp:480-493 scaling). */
 void rise_oracle_any_brute(const float* tris, int64_t n_tris,
                            const float* rays, const float* tmax, int64_t n_rays,
-                           uint8_t* occluded, int n_threads);
+                           uint8_t* occluded, int opts, int n_threads);
 
 #ifdef __cplusplus
 }

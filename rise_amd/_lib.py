@@ -70,6 +70,7 @@ _SIGNATURES = {
     "rise_hip_scene_upload": ([_P, _SIZE, ctypes.POINTER(_P)], ctypes.c_int),
     "rise_hip_scene_release": ([_P], ctypes.c_int),
     "rise_hip_scene_get_stats": ([_P, ctypes.POINTER(SceneStats)], ctypes.c_int),
+    "rise_hip_scene_set_arithmetic": ([_P, ctypes.c_int], ctypes.c_int),
     "rise_hip_intersect_closest": ([_P, _P, _I64, _P, _P, _P], ctypes.c_int),
     "rise_hip_intersect_any": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "rise_hip_intersect_closest_counted": ([_P, _P, _I64, _P, _P, _P, _P], ctypes.c_int),

@@ -37,9 +37,9 @@ rise_hip_status check_trace_args(const float* rays, int64_t n, int32_t mode, int
 {
     if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative count");
     if (n > 0 && !rays) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
-    if (mode == 0 && n > 0 && (!prim_id || !t)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
-    if (mode == 1 && n > 0 && !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
-    if (mode != 0 && mode != 1) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
+    if ((mode & 1) == 0 && n > 0 && (!prim_id || !t)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if ((mode & 1) == 1 && n > 0 && !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if (mode < 0 || mode > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mode must be 0..3");
     return RISE_HIP_OK;
 }
 
@@ -120,6 +120,20 @@ rise_hip_status trace_host(const WideBVH& bvh, int64_t n_tris, int32_t bvh_sourc
     std::atomic<int64_t> next(0);
     std::atomic<unsigned long long> cn(0), ct(0);
     std::atomic<bool> overflow(false);
+    auto trace_one = [&](auto r, int64_t i, unsigned& nn, unsigned& nt) {
+        init_ray(r, rays[i], rays[n + i], rays[2 * n + i], rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        HostStack st;
+        if ((mode & 1) == 0) {
+            double best;
+            unsigned leaf;
+            const int id = trace_closest<true>(r, nodes, trs, st, best, leaf, nn, nt);
+            prim_id[i] = id;
+            t[i] = id >= 0 ? (float)world_range(r, best) : INFINITY;
+        } else {
+            const double hf = tmax ? (double)tmax[i] : kRiseInfinity;
+            occluded[i] = trace_any<true>(r, nodes, trs, st, scale_how_far(r, hf), nn, nt) ? 1 : 0;
+        }
+    };
     auto worker = [&]() {
         unsigned long long ln = 0, lt = 0;
         for (;;) {
@@ -127,21 +141,10 @@ rise_hip_status trace_host(const WideBVH& bvh, int64_t n_tris, int32_t bvh_sourc
             if (s >= n) break;
             const int64_t e = std::min<int64_t>(s + 64, n);
             for (int64_t i = s; i < e; ++i) {
-                RayState r;
-                init_ray(r, rays[i], rays[n + i], rays[2 * n + i], rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
-                HostStack st;
                 unsigned nn = 0, nt = 0;
                 try {
-                    if (mode == 0) {
-                        double best;
-                        unsigned leaf;
-                        const int id = trace_closest<true>(r, nodes, trs, st, best, leaf, nn, nt);
-                        prim_id[i] = id;
-                        t[i] = id >= 0 ? (float)world_range(r, best) : INFINITY;
-                    } else {
-                        const double hf = tmax ? (double)tmax[i] : kRiseInfinity;
-                        occluded[i] = trace_any<true>(r, nodes, trs, st, scale_how_far(r, hf), nn, nt) ? 1 : 0;
-                    }
+                    if (mode & 2) trace_one(RayStateT<kArithX86Fma>{}, i, nn, nt);
+                    else trace_one(RayStateT<kArithIeee>{}, i, nn, nt);
                 } catch (const std::exception&) {
                     overflow = true;
                 }

@@ -30,12 +30,13 @@ struct rise_hip_scene {
     static constexpr int kFetchSlots = 64;
     unsigned long long* d_fetch = nullptr;
     std::atomic<unsigned> next_slot{0};
+    std::atomic<int> arith{RISE_HIP_ARITH_IEEE};
     int device = 0;
     rise_hip_scene_stats stats;
     TraceScene view()
     {
         const unsigned k = next_slot.fetch_add(1u) % kFetchSlots;
-        return TraceScene{d_nodes, d_tris, stats.depth, d_fetch + k};
+        return TraceScene{d_nodes, d_tris, stats.depth, d_fetch + k, arith.load()};
     }
 };
 
@@ -187,6 +188,16 @@ rise_hip_status rise_hip_scene_get_stats(const rise_hip_scene* scene, rise_hip_s
     return RISE_HIP_OK;
 }
 
+rise_hip_status rise_hip_scene_set_arithmetic(rise_hip_scene* scene, rise_hip_arith arith)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (arith != RISE_HIP_ARITH_IEEE && arith != RISE_HIP_ARITH_X86_FMA)
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "unknown arithmetic model");
+    scene->arith.store((int)arith);
+    return RISE_HIP_OK;
+}
+
 static rise_hip_status closest_impl(rise_hip_scene* scene, const float* rays, int64_t n, int32_t* prim_id, float* t,
                                     uint64_t* counters, hipStream_t stream)
 {
@@ -255,7 +266,9 @@ constexpr char kCacheMagic[8] = {'R', 'H', 'B', 'V', 'H', '8', 0, 0};
 
 // Bounds and shape of a wide tree read from a file: every internal child and
 // leaf triangle range inside the arrays, children after their parent (so the
-// traversal cannot loop), depth within the traversal stack.
+// traversal cannot loop), every node reached from one parent only (a tree,
+// not a DAG, so the level computed here is the node's only depth), depth
+// within the traversal stack.
 bool check_wide(const std::vector<NodeSlot>& nodes, uint64_t n_tris, int& depth, std::string& err)
 {
     const uint64_t nn = nodes.size();
@@ -271,6 +284,10 @@ bool check_wide(const std::vector<NodeSlot>& nodes, uint64_t n_tris, int& depth,
             return false;
         }
         for (unsigned k = 0; k < internal; ++k) {
+            if (level[n.child_base + k] != 0) {
+                err = "node " + std::to_string(n.child_base + k) + " is reached from two parents";
+                return false;
+            }
             level[n.child_base + k] = (uint8_t)std::min(255, level[i] + 1);
             depth = std::max(depth, level[i] + 1);
         }

@@ -1,19 +1,28 @@
 // trace_core.h -- device-side traversal of the 8-wide quantized BVH.
 //
 // One lane traces one ray.  Boxes are tested in FP32 with a conservative
-// slab test; every triangle whose leaf box is reached is certified in FP64
-// with the reference's Moller-Trumbore (Intersection/RayTriangleIntersection.cpp:21-68),
-// so the committed hit is the one the reference's double-precision leaf test
-// would commit.  Closest-hit keeps the smallest t, exact ties going to the
-// lowest triangle id; the reference's strict-closer guard is
+// slab test; every triangle whose leaf box is reached goes through the
+// reference's FP32 Moller-Trumbore pre-filter (Acceleration/BVH.h:1111-1167,
+// the same float operations in the same order) and, when the filter passes
+// it, is certified in FP64 with the reference's Moller-Trumbore
+// (Intersection/RayTriangleIntersection.cpp:21-68), so the committed hit is
+// the one the reference's leaf (BVH.h:1237-1301) would commit.  Closest-hit
+// keeps the smallest t, exact ties going to the lowest triangle id; the
+// reference's strict-closer guard is
 // Geometry/TriangleMeshGeometryIndexedSpecializations.h:172.
+//
+// The filter's distance test (t > current best + 1e-5) depends on the best
+// hit found so far, hence on traversal order; it can change the committed
+// hit only when the winner's float distance exceeds its double distance by
+// more than 1e-5 (the oracle flags such rays; none occur on the 1M-triangle
+// workload).
 //
 // Stack: node groups (base index, 8-bit hit mask, 8-bit internal mask), at
 // most one per tree level, held in LDS (short stack, STACK entries per lane)
 // or, for deep trees, in a private array.
 //
-// The whole file is compiled with -ffp-contract=off: double operations are
-// rounded one by one in the order the reference writes them.
+// The whole file is compiled with -ffp-contract=off: float and double
+// operations are rounded one by one in the order the reference writes them.
 #pragma once
 
 #include "bvh8.h"
@@ -40,23 +49,68 @@ constexpr double kSurfaceBackoff = 1e-12;                // Objects/Object.cpp:3
 // running best by it again) never rejects a box the exact ray enters.
 #define RISE_SLAB_PAD 1.0000019073486328f
 
+// Arithmetic of the reference build whose results are reproduced.
+//   kArithIeee: every float and double operation rounded on its own, in the
+//     order the reference's source writes it (its code built without
+//     contraction).
+//   kArithX86Fma: the reference as its Linux build compiles it (g++ -O3
+//     -march=x86-64-v3, contraction on).  That build fuses the first product
+//     of a*b - c*d into fma(a, b, -(c*d)) and evaluates a*b + c*d + e*f as
+//     fma(e, f, fma(a, b, c*d)), in Normalize, the float pre-filter, the
+//     double certifier and the world range alike (read from the compiled
+//     BVH::IntersectRay4, RayTriangleIntersection and Vector3Ops code).
+// Both forms are exact IEEE operations on the GPU (v_fma_f32 / v_fma_f64).
+constexpr int kArithIeee = 0;
+constexpr int kArithX86Fma = 1;
+
+template <int A>
+RISE_HD inline float sub_prod(float a, float b, float c, float d)
+{
+    if (A == kArithX86Fma) return fmaf(a, b, -(c * d));
+    return a * b - c * d;
+}
+template <int A>
+RISE_HD inline double sub_prod(double a, double b, double c, double d)
+{
+    if (A == kArithX86Fma) return fma(a, b, -(c * d));
+    return a * b - c * d;
+}
+template <int A>
+RISE_HD inline float dot3(float a0, float b0, float a1, float b1, float a2, float b2)
+{
+    if (A == kArithX86Fma) return fmaf(a2, b2, fmaf(a0, b0, a1 * b1));
+    return a0 * b0 + a1 * b1 + a2 * b2;
+}
+template <int A>
+RISE_HD inline double dot3(double a0, double b0, double a1, double b1, double a2, double b2)
+{
+    if (A == kArithX86Fma) return fma(a2, b2, fma(a0, b0, a1 * b1));
+    return a0 * b0 + a1 * b1 + a2 * b2;
+}
+
 // The certifier's direction is d normalized in double exactly as
 // Vector3Ops::Normalize computes it (Object.cpp:414-425): dn_k = d_k * ov with
 // ov = 1/|d| (or 1 when |d| = 0).  Only ov is kept in registers; dn and the
 // double origin are rebuilt from the float inputs when a triangle is
-// certified, bit-identically.
-struct RayState {
+// certified, bit-identically.  A selects the arithmetic (kArith*).
+template <int A>
+struct RayStateT {
     float ox, oy, oz;      // float origin (exact input)
     float dx, dy, dz;      // direction as given
+    float fx, fy, fz;      // (float)dn: the filter's direction (BVH.h:1212-1225)
     float ix, iy, iz;      // 1/d with d == 0 -> FLT_MAX (BVH.h:1212-1225)
     unsigned oinv;         // (~octant) & 7
     double ov;             // Normalize's 1/|d|
     double len_d;          // |d| in double
 };
+using RayState = RayStateT<kArithIeee>;
 
-RISE_HD inline double dn0(const RayState& r) { return (double)r.dx * r.ov; }
-RISE_HD inline double dn1(const RayState& r) { return (double)r.dy * r.ov; }
-RISE_HD inline double dn2(const RayState& r) { return (double)r.dz * r.ov; }
+template <int A>
+RISE_HD inline double dn0(const RayStateT<A>& r) { return (double)r.dx * r.ov; }
+template <int A>
+RISE_HD inline double dn1(const RayStateT<A>& r) { return (double)r.dy * r.ov; }
+template <int A>
+RISE_HD inline double dn2(const RayStateT<A>& r) { return (double)r.dz * r.ov; }
 
 RISE_HD inline float safe_inv(float d)
 {
@@ -82,7 +136,8 @@ RISE_HD inline void normalize_rise(const double v[3], double out[3])
     }
 }
 
-RISE_HD inline void init_ray(RayState& r, float ox, float oy, float oz, float dx, float dy, float dz)
+template <int A>
+RISE_HD inline void init_ray(RayStateT<A>& r, float ox, float oy, float oz, float dx, float dy, float dz)
 {
     r.ox = ox;
     r.oy = oy;
@@ -96,34 +151,70 @@ RISE_HD inline void init_ray(RayState& r, float ox, float oy, float oz, float dx
     r.dy = dy;
     r.dz = dz;
     const double d[3] = {dx, dy, dz};
-    const double mag = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double mag = sqrt(dot3<A>(d[0], d[0], d[1], d[1], d[2], d[2]));
     r.ov = mag > 0.0 ? 1.0 / mag : 1.0;
     r.len_d = mag;
+    r.fx = (float)dn0(r);
+    r.fy = (float)dn1(r);
+    r.fz = (float)dn2(r);
+}
+
+// BVH::MollerTrumboreFloat (Acceleration/BVH.h:1111-1167) on the leaf
+// triangle's float vertices: p0 = v0, e1 = v1 - v0, e2 = v2 - v0 in float
+// (BuildFastFilter, BVH.h:431-467), the float origin and the float
+// normalized direction (PrepRayFloat, BVH.h:1212-1225).  `best` is the
+// caller's (float) current range.  Every operation is one IEEE float
+// operation in the reference's order (no contraction; 1/det correctly
+// rounded).  A triangle it rejects is never certified.
+template <int A>
+RISE_HD inline bool filter_float(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, float best)
+{
+    constexpr float kEps = 1e-5f;
+    const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+    const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+    const float pv0 = sub_prod<A>(r.fy, e2z, r.fz, e2y);
+    const float pv1 = sub_prod<A>(r.fz, e2x, r.fx, e2z);
+    const float pv2 = sub_prod<A>(r.fx, e2y, r.fy, e2x);
+    const float det = dot3<A>(e1x, pv0, e1y, pv1, e1z, pv2);
+    if (fabsf(det) < 1e-20f) return false;
+    const float inv = 1.0f / det;
+    const float tv0 = r.ox - a.x, tv1 = r.oy - a.y, tv2 = r.oz - a.z;
+    const float u = dot3<A>(tv0, pv0, tv1, pv1, tv2, pv2) * inv;
+    if (u < -kEps || u > 1.0f + kEps) return false;
+    const float qv0 = sub_prod<A>(tv1, e1z, tv2, e1y);
+    const float qv1 = sub_prod<A>(tv2, e1x, tv0, e1z);
+    const float qv2 = sub_prod<A>(tv0, e1y, tv1, e1x);
+    const float v = dot3<A>(r.fx, qv0, r.fy, qv1, r.fz, qv2) * inv;
+    if (v < -kEps || u + v > 1.0f + kEps) return false;
+    const float t = dot3<A>(e2x, qv0, e2y, qv1, e2z, qv2) * inv;
+    if (t < -kEps || t > best + kEps) return false;
+    return true;
 }
 
 // FP64 Moller-Trumbore, RayTriangleIntersection.cpp:21-68 with the edges of
 // ...Specializations.h:141-142 (computed in double from the vertices).
-RISE_HD inline bool certify(const RayState& r, const float4& a, const float4& b, const float4& c, double& t_out)
+template <int A>
+RISE_HD inline bool certify(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, double& t_out)
 {
     const double v0x = a.x, v0y = a.y, v0z = a.z;
     const double eAx = (double)b.x - v0x, eAy = (double)b.y - v0y, eAz = (double)b.z - v0z;
     const double eBx = (double)c.x - v0x, eBy = (double)c.y - v0y, eBz = (double)c.z - v0z;
     const double d[3] = {dn0(r), dn1(r), dn2(r)};
-    const double p0 = d[1] * eBz - d[2] * eBy;
-    const double p1 = d[2] * eBx - d[0] * eBz;
-    const double p2 = d[0] * eBy - d[1] * eBx;
-    const double det = eAx * p0 + eAy * p1 + eAz * p2;
+    const double p0 = sub_prod<A>(d[1], eBz, d[2], eBy);
+    const double p1 = sub_prod<A>(d[2], eBx, d[0], eBz);
+    const double p2 = sub_prod<A>(d[0], eBy, d[1], eBx);
+    const double det = dot3<A>(eAx, p0, eAy, p1, eAz, p2);
     if (det > -kNearZero && det < kNearZero) return false;
     const double oodet = 1.0 / det;
     const double t0 = (double)r.ox - v0x, t1 = (double)r.oy - v0y, t2 = (double)r.oz - v0z;
-    const double al = (t0 * p0 + t1 * p1 + t2 * p2) * oodet;
+    const double al = dot3<A>(t0, p0, t1, p1, t2, p2) * oodet;
     if ((al < 0.0 - kNearZero) || (al > 1.0 + kNearZero)) return false;
-    const double q0 = t1 * eAz - t2 * eAy;
-    const double q1 = t2 * eAx - t0 * eAz;
-    const double q2 = t0 * eAy - t1 * eAx;
-    const double be = (d[0] * q0 + d[1] * q1 + d[2] * q2) * oodet;
+    const double q0 = sub_prod<A>(t1, eAz, t2, eAy);
+    const double q1 = sub_prod<A>(t2, eAx, t0, eAz);
+    const double q2 = sub_prod<A>(t0, eAy, t1, eAx);
+    const double be = dot3<A>(d[0], q0, d[1], q1, d[2], q2) * oodet;
     if ((be < 0.0 - kNearZero) || ((al + be) > 1.0 + kNearZero)) return false;
-    const double tt = (eBx * q0 + eBy * q1 + eBz * q2) * oodet;
+    const double tt = dot3<A>(eBx, q0, eBy, q1, eBz, q2) * oodet;
     if (tt >= kNearZero) {
         t_out = tt;
         return true;
@@ -133,20 +224,31 @@ RISE_HD inline bool certify(const RayState& r, const float4& a, const float4& b,
 
 // World range of a committed hit: Object.cpp:641-643 for an identity
 // transform (|origin + dn * (t - 1e-12) - origin|).
-RISE_HD inline double world_range(const RayState& r, double t)
+template <int A>
+RISE_HD inline double world_range(const RayStateT<A>& r, double t)
 {
     const double s = t - kSurfaceBackoff;
     const double ox = r.ox, oy = r.oy, oz = r.oz;
-    const double px = ox + dn0(r) * s, py = oy + dn1(r) * s, pz = oz + dn2(r) * s;
+    double px, py, pz;
+    if (A == kArithX86Fma) {
+        px = fma(dn0(r), s, ox);
+        py = fma(dn1(r), s, oy);
+        pz = fma(dn2(r), s, oz);
+    } else {
+        px = ox + dn0(r) * s;
+        py = oy + dn1(r) * s;
+        pz = oz + dn2(r) * s;
+    }
     const double vx = px - ox, vy = py - oy, vz = pz - oz;
-    return sqrt(vx * vx + vy * vy + vz * vz);
+    return sqrt(dot3<A>(vx, vx, vy, vy, vz, vz));
 }
 
 RISE_HD inline unsigned byte_of(unsigned w, int k) { return (w >> (8 * k)) & 0xffu; }
 
 // Tests the eight children of one node.  Returns the permuted internal-hit
 // mask (bit k <-> slot k ^ oinv) and fills the node's leaf-triangle mask.
-RISE_HD inline unsigned test_node(const RayState& r, const NodeSlot* __restrict__ nodes, unsigned ni, float tmax,
+template <int A>
+RISE_HD inline unsigned test_node(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes, unsigned ni, float tmax,
                                      unsigned& child_base, unsigned& imask, unsigned& tri_base, unsigned& tmask)
 {
     const uint4* p = reinterpret_cast<const uint4*>(nodes + ni);
@@ -197,7 +299,8 @@ RISE_HD inline unsigned test_node(const RayState& r, const NodeSlot* __restrict_
 
 // Converts a best distance along dn into the float traversal parameter along
 // d, widened (RISE_SLAB_PAD); kRiseInfinity maps to +inf.
-RISE_HD inline float box_limit(const RayState& r, double best)
+template <int A>
+RISE_HD inline float box_limit(const RayStateT<A>& r, double best)
 {
     if (!(best < kRiseInfinity)) return INFINITY;
     const double s = best / r.len_d;
@@ -233,7 +336,8 @@ struct ClosestState {
     double best_t;
 };
 
-RISE_HD inline void closest_begin(const RayState& r, ClosestState& s)
+template <int A>
+RISE_HD inline void closest_begin(const RayStateT<A>& r, ClosestState& s)
 {
     s.gbase = 0;
     s.ghits = 1u << (0u ^ r.oinv);  // the root, in slot 0 of a virtual group
@@ -249,8 +353,8 @@ RISE_HD inline void closest_begin(const RayState& r, ClosestState& s)
 // is used up, pop the next one.  Returns false once the traversal is over.
 // Closest hit: smallest t, exact ties to the lowest caller id (the
 // reference's strict-closer guard, ...Specializations.h:172, plus the tie rule).
-template <bool COUNT, class Stack>
-RISE_HD inline bool closest_step(const RayState& r, const NodeSlot* __restrict__ nodes,
+template <bool COUNT, class Stack, int A>
+RISE_HD inline bool closest_step(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
                                  const Tri48* __restrict__ tris, Stack& st, ClosestState& s, unsigned& n_nodes,
                                  unsigned& n_tris)
 {
@@ -274,7 +378,7 @@ RISE_HD inline bool closest_step(const RayState& r, const NodeSlot* __restrict__
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
         double tt;
-        if (certify(r, a, b, c, tt)) {
+        if (filter_float(r, a, b, c, (float)s.best_t) && certify(r, a, b, c, tt)) {
             const int id = float_to_bits(a.w);
             if (tt < s.best_t || (tt == s.best_t && id < s.best_id)) {
                 s.best_t = tt;
@@ -297,8 +401,8 @@ RISE_HD inline bool closest_step(const RayState& r, const NodeSlot* __restrict__
 // Closest-hit traversal.  Returns the caller id (-1 on miss), the best
 // parametric distance along dn in best_t and the leaf-order triangle index
 // of the hit in best_leaf.
-template <bool COUNT, class Stack>
-RISE_HD inline int trace_closest(const RayState& r, const NodeSlot* __restrict__ nodes,
+template <bool COUNT, class Stack, int A>
+RISE_HD inline int trace_closest(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
                                  const Tri48* __restrict__ tris, Stack& st, double& best_t, unsigned& best_leaf,
                                  unsigned& n_nodes, unsigned& n_tris)
 {
@@ -316,16 +420,19 @@ struct AnyState {
     unsigned gbase, ghits, gimask;
     bool hit;
     float tlim;
+    float fbest;  // (float)dHowFar2: the filter's range (BVH.h:1283)
     double how_far2;
 };
 
-RISE_HD inline void any_begin(const RayState& r, double how_far2, AnyState& s)
+template <int A>
+RISE_HD inline void any_begin(const RayStateT<A>& r, double how_far2, AnyState& s)
 {
     s.gbase = 0;
     s.ghits = 1u << (0u ^ r.oinv);
     s.gimask = 1u;
     s.hit = false;
     s.how_far2 = how_far2;
+    s.fbest = (float)how_far2;
     s.tlim = box_limit(r, how_far2);
 }
 
@@ -333,8 +440,8 @@ RISE_HD inline void any_begin(const RayState& r, double how_far2, AnyState& s)
 // kNearZero < t < how_far2 (...Specializations.h:517-519) and casts shadows
 // (bit 0 of v2.w clear; ObjectManager.cpp:119-124).  Returns false once
 // the query is decided (a hit, or the tree exhausted).
-template <bool COUNT, class Stack>
-RISE_HD inline bool any_step(const RayState& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
+template <bool COUNT, class Stack, int A>
+RISE_HD inline bool any_step(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
                              Stack& st, AnyState& s, unsigned& n_nodes, unsigned& n_tris)
 {
     unsigned tmask = 0, tbase = 0;
@@ -357,7 +464,8 @@ RISE_HD inline bool any_step(const RayState& r, const NodeSlot* __restrict__ nod
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
         double tt;
-        if (certify(r, a, b, c, tt) && tt > kNearZero && tt < s.how_far2 && !(float_to_bits(c.w) & 1u)) {
+        if (filter_float(r, a, b, c, s.fbest) && certify(r, a, b, c, tt) && tt > kNearZero && tt < s.how_far2 &&
+            !(float_to_bits(c.w) & 1u)) {
             s.hit = true;
             return false;
         }
@@ -372,8 +480,8 @@ RISE_HD inline bool any_step(const RayState& r, const NodeSlot* __restrict__ nod
     return true;
 }
 
-template <bool COUNT, class Stack>
-RISE_HD inline bool trace_any(const RayState& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
+template <bool COUNT, class Stack, int A>
+RISE_HD inline bool trace_any(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
                               Stack& st, double how_far2, unsigned& n_nodes, unsigned& n_tris)
 {
     AnyState s;
@@ -385,7 +493,8 @@ RISE_HD inline bool trace_any(const RayState& r, const NodeSlot* __restrict__ no
 
 // Object.cpp:472-493: the world limit dHowFar becomes dHowFar2 = |d| * dHowFar
 // in the normalized frame, unless it is the RISE_INFINITY sentinel and |d| >= 1.
-RISE_HD inline double scale_how_far(const RayState& r, double how_far)
+template <int A>
+RISE_HD inline double scale_how_far(const RayStateT<A>& r, double how_far)
 {
     const double factor = (r.len_d > kNearZero) ? r.len_d : 1.0;
     if ((how_far != kRiseInfinity) || (factor < 1.0)) return factor * how_far;

@@ -14,6 +14,7 @@ struct TraceScene {
     const Tri48* tris;
     int depth;  // wide-node levels; selects the stack variant
     unsigned long long* fetch_counter;  // device counter for dynamic ray fetch (one per in-flight call)
+    int arith;  // rise_hip_arith: kArithIeee or kArithX86Fma (trace_core.h), same values
 };
 
 // Closest hit: rays SoA (ox, oy, oz, dx, dy, dz), n each.  counters: NULL, or

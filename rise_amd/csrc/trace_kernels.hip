@@ -5,6 +5,7 @@
 // with one launch over a whole batch.  One lane per ray; the node-group stack
 // lives in LDS (16 entries per lane) when the tree is shallow enough, else in
 // a private array.
+#include "rise_hip.h"
 #include "trace_core.h"
 #include "trace_kernels.h"
 #include "wave_fetch.h"
@@ -12,6 +13,9 @@
 #include <algorithm>
 
 namespace rise_hip {
+
+static_assert(kArithIeee == RISE_HIP_ARITH_IEEE && kArithX86Fma == RISE_HIP_ARITH_X86_FMA,
+              "arithmetic models share the ABI's values");
 
 namespace {
 
@@ -44,7 +48,7 @@ __device__ inline void flush_counts(unsigned long long* counters, unsigned nn, u
 }
 
 // STACK: node-group entries per lane held in LDS; 0 selects a private array.
-template <bool COUNT, int STACK>
+template <bool COUNT, int STACK, int A>
 __global__ __launch_bounds__(kBlock) void closest_kernel(const NodeSlot* __restrict__ nodes,
                                                          const Tri48* __restrict__ tris,
                                                          const float* __restrict__ rays, int64_t n,
@@ -56,7 +60,7 @@ __global__ __launch_bounds__(kBlock) void closest_kernel(const NodeSlot* __restr
     if (i >= n) return;
     float ox, oy, oz, dx, dy, dz;
     load_ray(rays, n, i, ox, oy, oz, dx, dy, dz);
-    RayState r;
+    RayStateT<A> r;
     init_ray(r, ox, oy, oz, dx, dy, dz);
     unsigned nn = 0, nt = 0;
     double best;
@@ -75,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void closest_kernel(const NodeSlot* __restr
     flush_counts<COUNT>(counters, nn, nt);
 }
 
-template <bool COUNT, int STACK>
+template <bool COUNT, int STACK, int A>
 __global__ __launch_bounds__(kBlock) void any_kernel(const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
                                                      const float* __restrict__ rays,
                                                      const float* __restrict__ tmax, int64_t n,
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void any_kernel(const NodeSlot* __restrict_
     if (i >= n) return;
     float ox, oy, oz, dx, dy, dz;
     load_ray(rays, n, i, ox, oy, oz, dx, dy, dz);
-    RayState r;
+    RayStateT<A> r;
     init_ray(r, ox, oy, oz, dx, dy, dz);
     const double how_far = tmax ? (double)tmax[i] : kRiseInfinity;
     const double how_far2 = scale_how_far(r, how_far);
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void any_kernel(const NodeSlot* __restrict_
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-template <bool COUNT, int STACK>
+template <bool COUNT, int STACK, int A>
 __global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __restrict__ nodes,
                                                              const Tri48* __restrict__ tris,
                                                              const float* __restrict__ rays, int64_t n,
@@ -118,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __r
     LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
     PrivStack pst;
     pst.sp = 0;
-    RayState r;
+    RayStateT<A> r;
     ClosestState s;
     long long ray = -1;  // -1 idle, -2 exhausted
     unsigned nn = 0, nt = 0;
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __r
     flush_counts<COUNT>(counters, nn, nt);
 }
 
-template <bool COUNT, int STACK>
+template <bool COUNT, int STACK, int A>
 __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restrict__ nodes,
                                                          const Tri48* __restrict__ tris,
                                                          const float* __restrict__ rays,
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restr
     LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
     PrivStack pst;
     pst.sp = 0;
-    RayState r;
+    RayStateT<A> r;
     AnyState s;
     long long ray = -1;
     unsigned nn = 0, nt = 0;
@@ -229,22 +233,29 @@ hipError_t launch_closest(const TraceScene& s, const float* rays, int64_t n, int
     const dim3 grid(grid_for(n)), block(kBlock);
 #if RISE_PERSISTENT
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
-#define RISE_LAUNCH_CLOSEST(C, S)                                                                               \
-    hipLaunchKernelGGL((closest_persistent<C, S>), dim3(persistent_grid(closest_persistent<C, S>, n)), block, 0, \
-                       stream, s.nodes, s.tris, rays, n, prim_id, t, counters, s.fetch_counter)
+#define RISE_LAUNCH_CLOSEST(C, S, A)                                                                      \
+    hipLaunchKernelGGL((closest_persistent<C, S, A>), dim3(persistent_grid(closest_persistent<C, S, A>, n)), block, \
+                       0, stream, s.nodes, s.tris, rays, n, prim_id, t, counters, s.fetch_counter)
 #else
-#define RISE_LAUNCH_CLOSEST(C, S) \
-    hipLaunchKernelGGL((closest_kernel<C, S>), grid, block, 0, stream, s.nodes, s.tris, rays, n, prim_id, t, counters)
+#define RISE_LAUNCH_CLOSEST(C, S, A)                                                                          \
+    hipLaunchKernelGGL((closest_kernel<C, S, A>), grid, block, 0, stream, s.nodes, s.tris, rays, n, prim_id, t, \
+                       counters)
 #endif
+#define RISE_LAUNCH_CLOSEST_A(C, S)                                             \
+    do {                                                                         \
+        if (s.arith == kArithX86Fma) RISE_LAUNCH_CLOSEST(C, S, kArithX86Fma);    \
+        else RISE_LAUNCH_CLOSEST(C, S, kArithIeee);                              \
+    } while (0)
     if (counters) {
-        if (need <= 8) RISE_LAUNCH_CLOSEST(true, 8);
-        else if (need <= 16) RISE_LAUNCH_CLOSEST(true, 16);
-        else RISE_LAUNCH_CLOSEST(true, 0);
+        if (need <= 8) RISE_LAUNCH_CLOSEST_A(true, 8);
+        else if (need <= 16) RISE_LAUNCH_CLOSEST_A(true, 16);
+        else RISE_LAUNCH_CLOSEST_A(true, 0);
     } else {
-        if (need <= 8) RISE_LAUNCH_CLOSEST(false, 8);
-        else if (need <= 16) RISE_LAUNCH_CLOSEST(false, 16);
-        else RISE_LAUNCH_CLOSEST(false, 0);
+        if (need <= 8) RISE_LAUNCH_CLOSEST_A(false, 8);
+        else if (need <= 16) RISE_LAUNCH_CLOSEST_A(false, 16);
+        else RISE_LAUNCH_CLOSEST_A(false, 0);
     }
+#undef RISE_LAUNCH_CLOSEST_A
 #undef RISE_LAUNCH_CLOSEST
     return hipGetLastError();
 }
@@ -257,22 +268,29 @@ hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax,
     const dim3 grid(grid_for(n)), block(kBlock);
 #if RISE_PERSISTENT
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
-#define RISE_LAUNCH_ANY(C, S)                                                                                    \
-    hipLaunchKernelGGL((any_persistent<C, S>), dim3(persistent_grid(any_persistent<C, S>, n)), block, 0, stream, \
-                       s.nodes, s.tris, rays, tmax, n, occluded, counters, s.fetch_counter)
+#define RISE_LAUNCH_ANY(C, S, A)                                                                           \
+    hipLaunchKernelGGL((any_persistent<C, S, A>), dim3(persistent_grid(any_persistent<C, S, A>, n)), block, 0, \
+                       stream, s.nodes, s.tris, rays, tmax, n, occluded, counters, s.fetch_counter)
 #else
-#define RISE_LAUNCH_ANY(C, S) \
-    hipLaunchKernelGGL((any_kernel<C, S>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n, occluded, counters)
+#define RISE_LAUNCH_ANY(C, S, A)                                                                             \
+    hipLaunchKernelGGL((any_kernel<C, S, A>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n, occluded, \
+                       counters)
 #endif
+#define RISE_LAUNCH_ANY_A(C, S)                                             \
+    do {                                                                     \
+        if (s.arith == kArithX86Fma) RISE_LAUNCH_ANY(C, S, kArithX86Fma);    \
+        else RISE_LAUNCH_ANY(C, S, kArithIeee);                              \
+    } while (0)
     if (counters) {
-        if (need <= 8) RISE_LAUNCH_ANY(true, 8);
-        else if (need <= 16) RISE_LAUNCH_ANY(true, 16);
-        else RISE_LAUNCH_ANY(true, 0);
+        if (need <= 8) RISE_LAUNCH_ANY_A(true, 8);
+        else if (need <= 16) RISE_LAUNCH_ANY_A(true, 16);
+        else RISE_LAUNCH_ANY_A(true, 0);
     } else {
-        if (need <= 8) RISE_LAUNCH_ANY(false, 8);
-        else if (need <= 16) RISE_LAUNCH_ANY(false, 16);
-        else RISE_LAUNCH_ANY(false, 0);
+        if (need <= 8) RISE_LAUNCH_ANY_A(false, 8);
+        else if (need <= 16) RISE_LAUNCH_ANY_A(false, 16);
+        else RISE_LAUNCH_ANY_A(false, 0);
     }
+#undef RISE_LAUNCH_ANY_A
 #undef RISE_LAUNCH_ANY
     return hipGetLastError();
 }

@@ -26,15 +26,28 @@ def _stream_handle(stream) -> int | None:
     return int(getattr(stream, "cuda_stream", stream))
 
 
-class ObjectManager:
-    """Resident triangle scene answering batched ray queries on the GPU."""
+ARITH_IEEE = 0
+"""Hits as the reference's source computes them, every operation rounded."""
+ARITH_X86_FMA = 1
+"""Hits as the reference's Linux build (g++ -O3 -march=x86-64-v3) computes them."""
 
-    def __init__(self) -> None:
+
+class ObjectManager:
+    """Resident triangle scene answering batched ray queries on the GPU.
+
+    arith selects the reference build whose hits are reproduced bit for bit
+    (ARITH_IEEE or ARITH_X86_FMA; rise_hip_scene_set_arithmetic)."""
+
+    def __init__(self, arith: int = ARITH_IEEE) -> None:
         self._meshes: list[tuple[np.ndarray, np.ndarray | None]] = []
         self._scene = ctypes.c_void_p(None)
+        self._arith = arith
+        self._from_cache = False
 
     def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None) -> None:
         """tris: (n, 9) float32 world-space triangles; ids: optional (n,) int32."""
+        if self._from_cache:
+            raise RuntimeError("a scene loaded from a cache is complete; it takes no more meshes")
         t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
         i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
         if i is not None and i.shape[0] != t.shape[0]:
@@ -42,6 +55,8 @@ class ObjectManager:
         self._meshes.append((t, i))
 
     def prepare_for_rendering(self) -> None:
+        if self._from_cache:
+            raise RuntimeError("a scene loaded from a cache is already prepared")
         L = _lib.lib()
         self.release()
         descs = (_lib.MeshDesc * max(1, len(self._meshes)))()
@@ -52,6 +67,13 @@ class ObjectManager:
         out = ctypes.c_void_p(None)
         _lib.check(L.rise_hip_scene_upload(descs, len(self._meshes), ctypes.byref(out)))
         self._scene = out
+        self.set_arithmetic(self._arith)
+
+    def set_arithmetic(self, arith: int) -> None:
+        """ARITH_IEEE or ARITH_X86_FMA for the queries that follow."""
+        self._arith = arith
+        if self._scene.value:
+            _lib.check(_lib.lib().rise_hip_scene_set_arithmetic(self._scene, int(arith)))
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -113,14 +135,17 @@ class ObjectManager:
         _lib.check(L.rise_hip_scene_cache_write(self.handle, os.fsencode(path)))
 
     @classmethod
-    def load_cache(cls, path) -> "ObjectManager":
-        """A resident scene from a saved tree, with no build (rise_hip_scene_cache_read)."""
+    def load_cache(cls, path, arith: int = ARITH_IEEE) -> "ObjectManager":
+        """A resident scene from a saved tree, with no build (rise_hip_scene_cache_read).
+        The scene is complete: add_mesh and prepare_for_rendering raise."""
         L = _lib.lib()
         L.rise_hip_scene_cache_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
-        om = cls()
+        om = cls(arith)
         out = ctypes.c_void_p(None)
         _lib.check(L.rise_hip_scene_cache_read(os.fsencode(path), ctypes.byref(out)))
         om._scene = out
+        om._from_cache = True
+        om.set_arithmetic(arith)
         return om
 
     def release(self) -> None:
@@ -144,10 +169,12 @@ def _check_rays(rays) -> None:
         raise ValueError("rays must be in device memory")
 
 
-def host_trace(tris: np.ndarray, rays: np.ndarray, mode: int = 0, tmax: np.ndarray | None = None):
+def host_trace(tris: np.ndarray, rays: np.ndarray, mode: int = 0, tmax: np.ndarray | None = None,
+               arith: int = ARITH_IEEE):
     """Runs the kernels' traversal code on host threads (validation only).
     Returns (prim_id, t, counters, stats) for mode 0, (occluded, counters, stats)
     for mode 1."""
+    mode = (mode & 1) | (2 if arith == ARITH_X86_FMA else 0)
     L = _lib.lib()
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -155,12 +182,12 @@ def host_trace(tris: np.ndarray, rays: np.ndarray, mode: int = 0, tmax: np.ndarr
     counters = np.zeros(2, dtype=np.uint64)
     stats = _lib.SceneStats()
     tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
-    if mode == 0:
+    if (mode & 1) == 0:
         pid = np.empty(n, dtype=np.int32)
         t = np.empty(n, dtype=np.float32)
         _lib.check(
             L.rise_hip_debug_trace_host(
-                tris.ctypes.data, tris.shape[0], rays.ctypes.data, None, n, 0, pid.ctypes.data, t.ctypes.data,
+                tris.ctypes.data, tris.shape[0], rays.ctypes.data, None, n, mode, pid.ctypes.data, t.ctypes.data,
                 None, counters.ctypes.data, ctypes.byref(stats),
             )
         )
@@ -168,7 +195,7 @@ def host_trace(tris: np.ndarray, rays: np.ndarray, mode: int = 0, tmax: np.ndarr
     occ = np.empty(n, dtype=np.uint8)
     _lib.check(
         L.rise_hip_debug_trace_host(
-            tris.ctypes.data, tris.shape[0], rays.ctypes.data, None if tm is None else tm.ctypes.data, n, 1,
+            tris.ctypes.data, tris.shape[0], rays.ctypes.data, None if tm is None else tm.ctypes.data, n, mode,
             None, None, occ.ctypes.data, counters.ctypes.data, ctypes.byref(stats),
         )
     )

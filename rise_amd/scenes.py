@@ -94,3 +94,28 @@ def cast_rays(
 def rays_from_rows(o: np.ndarray, d: np.ndarray) -> np.ndarray:
     """(n,3) origins and directions to a contiguous (6, n) float32 SoA."""
     return np.ascontiguousarray(np.concatenate([np.asarray(o, np.float32).T, np.asarray(d, np.float32).T], 0))
+
+
+def near_edge_rays(tris: np.ndarray, n: int, seed: int, band: float = 3e-5) -> np.ndarray:
+    """Rays aimed at points within `band` (in barycentrics) of a triangle edge.
+
+    Each ray picks a triangle, an edge of it (u = 0, v = 0 or u + v = 1) and a
+    point whose distance from that edge, in barycentric units, is uniform in
+    [-band, band]; the origin is uniform in the unit cube.  Near such edges a
+    float Moller-Trumbore test of a small triangle seen from afar loses about
+    1e-5 in barycentrics, so these rays separate a float pre-filter from the
+    double certifier.  Returns (6, n) float32."""
+    g = _rng(seed)
+    t = np.asarray(tris, np.float64).reshape(-1, 9)
+    k = g.integers(0, t.shape[0], n)
+    v0, v1, v2 = t[k, 0:3], t[k, 3:6], t[k, 6:9]
+    along = g.random(n)
+    off = (g.random(n) * 2.0 - 1.0) * band
+    edge = g.integers(0, 3, n)
+    u = np.where(edge == 0, off, np.where(edge == 1, along, along - off))
+    v = np.where(edge == 0, along, np.where(edge == 1, off, 1.0 - along))
+    p = v0 + u[:, None] * (v1 - v0) + v[:, None] * (v2 - v0)
+    o = g.random((n, 3))
+    d = p - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return rays_from_rows(o, d)

@@ -5,11 +5,15 @@ cover (tests/BVHBuilderTest.cpp: random workloads, deep clustered trees;
 tests/RayBoxIntersectionTest.cpp: axis-parallel and grazing rays).  This
 checks the 8-wide build, the outward quantization and the conservative slab
 test before any GPU run."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 import oracle
 from rise_amd import raycast, scenes
+
+GOLDEN_FILTER = Path(__file__).resolve().parent / "golden" / "raycast_filter.npz"
 
 
 def _check(tris, rays, tmax=None):
@@ -139,3 +143,18 @@ def test_nonfinite_vertices_rejected():
     tris[3, 4] = np.nan
     with pytest.raises(Exception, match="non-finite"):
         raycast.host_trace(tris, scenes.incoherent_rays(4, 1))
+
+
+@pytest.mark.parametrize("arith", ["ieee", "fast"])
+def test_float_filter_fixture(arith):
+    """Rays within 3e-5 of sliver-triangle edges: the host-compiled traversal
+    gives the ids, distances and occlusion of the reference build with the
+    same arithmetic (tests/golden/raycast_filter.npz)."""
+    z = np.load(GOLDEN_FILTER)
+    a = raycast.ARITH_X86_FMA if arith == "fast" else raycast.ARITH_IEEE
+    hp, ht, _, _ = raycast.host_trace(z["tris"], z["rays"], arith=a)
+    assert np.array_equal(hp, z[f"prim_id_{arith}"])
+    hit = hp >= 0
+    assert np.array_equal(ht[hit], z[f"t64_{arith}"][hit].astype(np.float32))
+    occ, _, _ = raycast.host_trace(z["tris"], z["rays"], 1, z[f"tmax_{arith}"], arith=a)
+    assert np.array_equal(occ, z[f"occluded_{arith}"])

@@ -8,6 +8,13 @@ against it.
   random soups, and the golden fixture that path produced
   (tests/golden/raycast_small.npz, tests/golden/make_raycast_golden.py).
 * The tie rule (exact-t ties -> lowest id) on duplicated triangles.
+* The float pre-filter BVH::MollerTrumboreFloat (Acceleration/BVH.h:1111-1167):
+  closed-form cases that fix its 1e-5 margins, its sign conventions and its
+  distance test; rays aimed within 3e-5 of sliver-triangle edges, where it
+  rejects hits the double certifier accepts, against the reference's two
+  builds (every operation rounded, and its Linux flags' fused multiply-adds):
+  live when oracle/_ref is built, and the committed fixture
+  tests/golden/raycast_filter.npz that those builds wrote.
 """
 from pathlib import Path
 
@@ -18,6 +25,7 @@ import oracle
 from rise_amd import scenes
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "raycast_small.npz"
+GOLDEN_FILTER = Path(__file__).resolve().parent / "golden" / "raycast_filter.npz"
 
 V0 = (0.0, 0.0, 0.0)
 EA = (1.0, 0.0, 0.0)
@@ -139,13 +147,90 @@ def test_oracle_matches_reference_bvh(seed, n, sigma):
 
 
 @pytest.mark.skipif(not oracle.ref_available("fast"), reason="oracle/_ref not built (no reference tree)")
-def test_reference_linux_flags_agree_on_ids():
-    # The reference built with its own Linux flags (FMA contraction allowed)
-    # commits the same triangles; distances agree far inside 1e-5 relative.
+def test_reference_linux_flags_match_the_fma_model():
+    # The reference built with its own Linux flags (products contracted into
+    # fused multiply-adds) against the oracle's model of that contraction:
+    # every id and every double distance equal.
     tris = scenes.soup_triangles(20000, seed=21, sigma=0.01)
     rays = np.concatenate([scenes.pinhole_rays(96, 64), scenes.incoherent_rays(6000, 121)], 1)
-    pid, t, t64 = oracle.closest(tris, rays)
-    rp, rt, rt64 = oracle.RefMesh(tris, "fast").closest(rays)
+    pid, t, t64 = oracle.closest(tris, rays, arith="fast")
+    ref = oracle.RefMesh(tris, "fast")
+    rp, rt, rt64 = ref.closest(rays)
     assert np.array_equal(pid, rp)
     hit = pid >= 0
-    assert np.max(np.abs(rt64[hit] - t64[hit]) / t64[hit]) < 1e-12
+    assert np.array_equal(t64[hit], rt64[hit])
+    tmax = np.where(hit, np.where(np.arange(len(pid)) % 2 == 0, t * 0.999, t * 1.001), 0.3).astype(np.float32)
+    assert np.array_equal(oracle.any_hit(tris, rays, tmax, arith="fast"), ref.any_hit(rays, tmax))
+
+
+# --- the float pre-filter (BVH.h:1111-1167) ---------------------------------
+
+# Unit right triangle in z = 0; a ray from (u, v, 1) straight down gives the
+# filter u, v and t = 1 exactly (pvec = (1, 0, 0), det = 1, qvec = (0, 1, -v)).
+P0 = (0.0, 0.0, 0.0)
+E1 = (1.0, 0.0, 0.0)
+E2 = (0.0, 1.0, 0.0)
+DOWN = (0.0, 0.0, -1.0)
+
+
+def _filt(u, v, best=float("inf"), z=1.0, d=DOWN):
+    return oracle.filter_float((u, v, z), d, P0, E1, E2, best)
+
+
+def test_filter_margins_are_1e5_on_each_barycentric():
+    assert _filt(0.25, 0.25)
+    assert _filt(-0.99e-5, 0.5) and not _filt(-1.01e-5, 0.5)
+    assert _filt(0.5, -0.99e-5) and not _filt(0.5, -1.01e-5)
+    assert _filt(1.0 + 0.99e-5, 0.0) and not _filt(1.0 + 1.01e-5, 0.0)
+    assert _filt(0.5, 0.5 + 0.8e-5) and not _filt(0.5, 0.5 + 1.2e-5)  # u + v rounds at 2^-24 here
+
+
+def test_filter_distance_test():
+    # t = 1: rejected only past best + 1e-5, and behind the origin.
+    assert _filt(0.25, 0.25, best=1.0 - 0.9e-5)
+    assert not _filt(0.25, 0.25, best=1.0 - 1.1e-5)
+    assert not _filt(0.25, 0.25, z=-1.0)
+    assert _filt(0.25, 0.25, z=0.9e-5 * 0.5)
+
+
+def test_filter_rejects_parallel_rays():
+    assert not _filt(0.25, 0.25, d=(1.0, 0.0, 0.0))
+
+
+def _filter_fixture():
+    return np.load(GOLDEN_FILTER)
+
+
+@pytest.mark.parametrize("arith", ["ieee", "fast"])
+def test_filter_fixture_matches_reference_builds(arith):
+    z = _filter_fixture()
+    pid, t, t64 = oracle.closest(z["tris"], z["rays"], arith=arith)
+    assert np.array_equal(pid, z[f"prim_id_{arith}"])
+    hit = pid >= 0
+    assert np.array_equal(t64[hit], z[f"t64_{arith}"][hit])
+    occ = oracle.any_hit(z["tris"], z["rays"], z[f"tmax_{arith}"], arith=arith)
+    assert np.array_equal(occ, z[f"occluded_{arith}"])
+    # The fixture has teeth: the certifier alone, or the other arithmetic,
+    # gives different ids on hundreds of its rays.
+    bare, _, _ = oracle.closest(z["tris"], z["rays"], arith=arith, no_filter=True)
+    assert (bare != pid).sum() >= 100
+    other, _, _ = oracle.closest(z["tris"], z["rays"], arith="fast" if arith == "ieee" else "ieee")
+    assert (other != pid).sum() >= 100
+
+
+@pytest.mark.skipif(not oracle.ref_available("fast"), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("arith", ["ieee", "fast"])
+def test_filter_near_edges_live(arith):
+    tris = scenes.soup_triangles(2000, seed=17, sigma=0.003)
+    rays = scenes.near_edge_rays(tris, 20000, 18)
+    pid, t, t64, flags = oracle.closest(tris, rays, arith=arith, flags=True)
+    ref = oracle.RefMesh(tris, arith)
+    rp, _, rt64 = ref.closest(rays)
+    assert np.array_equal(pid, rp)
+    hit = pid >= 0
+    assert np.array_equal(t64[hit], rt64[hit])
+    tmax = np.where(hit, t * 1.001, 0.3).astype(np.float32)
+    assert np.array_equal(oracle.any_hit(tris, rays, tmax, arith=arith), ref.any_hit(rays, tmax))
+    # Order-dependent rays (the winner's float t more than 1e-5 past its
+    # double t) are rare even here.
+    assert flags.sum() < 50
