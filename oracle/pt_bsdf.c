@@ -591,3 +591,11 @@ int rise_oracle_refract(const double n[3], double ni, double nt, double vin[3])
     }
     return ok;
 }
+
+void rise_oracle_reflect(const double vin[3], const double n[3], double out[3])
+{
+    const rb_v3 r = rb_reflect(mk(vin[0], vin[1], vin[2]), mk(n[0], n[1], n[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}

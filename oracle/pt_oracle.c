@@ -151,10 +151,39 @@ static v3 onb_xform(const onb* b, v3 a)
               b->u.z * a.x + b->v.z * a.y + b->w.z * a.z);
 }
 
+/* PathTransportUtilities::PowerHeuristic (PathTransportUtilities.h:251-258). */
 static double power_heuristic(double pa, double pb)
 {
     const double a2 = pa * pa;
     return a2 / (a2 + pb * pb);
+}
+
+/* PathTransportUtilities::EvaluateRussianRoulette (PathTransportUtilities.h:86-110):
+ * from depth min_depth on, survive with p = min(1, cur_max / max(prev_max,
+ * threshold)); a survivor's weight is divided by `survival` (p when 0 < p < 1,
+ * else 1). */
+typedef struct { int terminate; double survival; } rr_result;
+
+static rr_result russian_roulette(int depth, int min_depth, double threshold, double cur_max, double prev_max, double u)
+{
+    rr_result r = {0, 1.0};
+    if (depth >= min_depth) {
+        const double den = prev_max > threshold ? prev_max : threshold;
+        double p = cur_max / den;
+        if (p > 1.0) p = 1.0;
+        if (u >= p) r.terminate = 1;
+        else if (p > 0 && p < 1.0) r.survival = p;
+    }
+    return r;
+}
+
+/* SampledWavelengths::SampleEquidistant (SampledWavelengths.h:56-90): the hero
+ * uniform in [a, b), companions at equal offsets with wrap-around. */
+static void sample_wavelengths(double u, double a, double b, double lam[HWSS_N])
+{
+    const double range = b - a, delta = range / HWSS_N;
+    lam[0] = a + u * range;
+    for (int i = 1; i < HWSS_N; ++i) lam[i] = a + fmod(lam[0] - a + i * delta, range);
 }
 
 
@@ -1073,13 +1102,10 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
                     const double pp = rise_oracle_hg_eval(dot(dir, nd), M->g);
                     if (pp <= NZ) break;
                     thr *= rise_oracle_hg_eval(dot(dir, nd), M->g) / pp;
-                    if (depth + vb >= s->rr_min_depth) {
-                        const double prev = importance > s->rr_threshold ? importance : s->rr_threshold;
-                        double p = fabs(thr) / prev;
-                        if (p > 1.0) p = 1.0;
-                        if (rise_oracle_rng(seed, path, base + 19) >= p) break;
-                        if (p > 0 && p < 1.0) thr /= p;
-                    }
+                    const rr_result rr = russian_roulette(depth + vb, s->rr_min_depth, s->rr_threshold, fabs(thr),
+                                                          importance, rise_oracle_rng(seed, path, base + 19));
+                    if (rr.terminate) break;
+                    if (rr.survival < 1.0) thr /= rr.survival;
                     org = sp;
                     dir = nd;
                     bsdf_pdf = pp;
@@ -1125,13 +1151,10 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
         if (!sc.ok) break;
         double scat = sc.kray;
         if (fabs(scat) <= NZ) break;
-        if (depth >= s->rr_min_depth) {
-            const double prev = importance > s->rr_threshold ? importance : s->rr_threshold;
-            double p = importance * fabs(scat) / prev;
-            if (p > 1.0) p = 1.0;
-            if (rise_oracle_rng(seed, path, base + 19) >= p) break;
-            if (p > 0 && p < 1.0) scat /= p;
-        }
+        const rr_result rr = russian_roulette(depth, s->rr_min_depth, s->rr_threshold, importance * fabs(scat),
+                                              importance, rise_oracle_rng(seed, path, base + 19));
+        if (rr.terminate) break;
+        if (rr.survival < 1.0) scat /= rr.survival;
         importance *= fabs(scat);
         if (over_cap(s, sc.type, &diffuse, &glossy, &transmission)) break;
         thr *= scat;
@@ -1164,11 +1187,8 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
     v3 dir = normalize(add(add(f, mul(r, cx)), mul(u, cy)));
     v3 org = cp;
 
-    /* SampledWavelengths::SampleEquidistant (SampledWavelengths.h:79-106). */
-    const double a = s->lambda_begin, b = s->lambda_end, range = b - a, delta = range / HWSS_N;
     double lam[HWSS_N];
-    lam[0] = a + ul * range;
-    for (int i = 1; i < HWSS_N; ++i) lam[i] = a + fmod(lam[0] - a + i * delta, range);
+    sample_wavelengths(ul, s->lambda_begin, s->lambda_end, lam);
 
     double result[HWSS_N] = {0, 0, 0, 0};
     double thr[HWSS_N] = {1, 1, 1, 1};
@@ -1236,14 +1256,11 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
             if (fabs(thr[w] * sc[w]) > mcurr) mcurr = fabs(thr[w] * sc[w]);
         }
         if (mcurr <= NZ) break;
-        if (depth >= s->rr_min_depth) {
-            const double thrs = mprev > s->rr_threshold ? mprev : s->rr_threshold;
-            double p = mcurr / thrs;
-            if (p > 1.0) p = 1.0;
-            if (rise_oracle_rng(seed, path, base + 19) >= p) break;
-            if (p > 0 && p < 1.0)
-                for (int w = 0; w < HWSS_N; ++w) sc[w] *= 1.0 / p;
-        }
+        const rr_result rr = russian_roulette(depth, s->rr_min_depth, s->rr_threshold, mcurr, mprev,
+                                              rise_oracle_rng(seed, path, base + 19));
+        if (rr.terminate) break;
+        if (rr.survival < 1.0)
+            for (int w = 0; w < HWSS_N; ++w) sc[w] *= 1.0 / rr.survival;
         importance *= fabs(sc[0]);
         /* Bounce limits (PathTransportUtilities.h:170-210). */
         if (over_cap(s, hs.type, &diffuse, &glossy, &transmission)) break;
@@ -1351,4 +1368,29 @@ double rise_oracle_shadow_transmittance(const ro_scene* scene, const double o[3]
     const double tr = shadow_tr(&P, mk(o[0], o[1], o[2]), mk(d[0], d[1], d[2]), max_dist, origin_object);
     release(&P);
     return tr;
+}
+
+/* ---- entry points for the value-for-value comparison with the reference -- */
+
+void rise_oracle_russian_roulette(int depth, int min_depth, double threshold, double cur_max, double prev_max,
+                                  double u, double out[2])
+{
+    const rr_result r = russian_roulette(depth, min_depth, threshold, cur_max, prev_max, u);
+    out[0] = r.terminate ? 1.0 : 0.0;
+    out[1] = r.survival;
+}
+
+double rise_oracle_power_heuristic(double pa, double pb) { return power_heuristic(pa, pb); }
+
+void rise_oracle_sample_wavelengths(double u, double a, double b, double out[4]) { sample_wavelengths(u, a, b, out); }
+
+void rise_oracle_onb_from_w(const double w[3], double out[9])
+{
+    const onb o = onb_from_w(mk(w[0], w[1], w[2]));
+    const v3 k[3] = {o.u, o.v, o.w};
+    for (int i = 0; i < 3; ++i) {
+        out[3 * i] = k[i].x;
+        out[3 * i + 1] = k[i].y;
+        out[3 * i + 2] = k[i].z;
+    }
 }

@@ -207,15 +207,18 @@ def cornell_box():
     return s, r
 
 
-def furnace_box(albedo: float = 0.5, radiance: float = 1.0, max_bounce: int = 64):
+def furnace_box(albedo=0.5, radiance=1.0, max_bounce: int = 64):
     """White furnace: a closed box whose six faces (six objects, so each face is
-    its own luminary) all emit `radiance` (flat) and reflect `albedo` (flat).
-    From inside, L = Le / (1 - albedo) up to the albedo^(max_bounce + 1) term
-    (the last vertex keeps only the NEE share of its MIS pair), so with a
-    large cap the render checks MIS, NEE, Russian roulette and the
-    Y normalization (E[Y] = L for a flat spectrum) against a closed form."""
+    its own luminary) all emit `radiance` and reflect `albedo` (each a number
+    for a flat spectrum, or an Spd).  From inside, L(nm) = Le(nm) / (1 -
+    albedo(nm)) up to the albedo^(max_bounce + 1) term (the last vertex keeps
+    only the NEE share of its MIS pair), so with a large cap the render checks
+    MIS, NEE, Russian roulette and the Y normalization (E[Y] = L for a flat
+    spectrum; E[Y] = the ybar-weighted mean of L(nm) over the render's
+    wavelength range otherwise) against a closed form."""
     s = Scene()
-    s.spds = [Spd.flat(albedo), Spd.flat(radiance)]
+    s.spds = [albedo if isinstance(albedo, Spd) else Spd.flat(albedo),
+              radiance if isinstance(radiance, Spd) else Spd.flat(radiance)]
     s.materials = [Material(0, emission=1, emission_scale=math.pi)]
     tris = box((-1, -1, -1), (1, 1, 1))
     # inward normals: swap two vertices of every outward triangle

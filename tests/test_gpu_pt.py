@@ -6,7 +6,8 @@
   (Russian roulette, a grazing hit) flips between the two.  Bar: at least
   99% of pixels within 1e-3 relative (of the pixel, floored at 1% of the
   image mean), image mean within 1e-3, ray counts within 0.5%.
-* Closed forms: white furnace (L = Le / (1 - albedo)), emitter seen directly.
+* Closed forms: white furnace (L = Le / (1 - albedo)), the same with
+  step-spectrum albedo and emission, emitter seen directly.
 * Statistics: independent seeds, per-pixel z-test pass rate >= 99% at
   |z| < 3.29.
 * Pass splitting, progressive sample ranges and run-to-run determinism.
@@ -19,6 +20,7 @@ import pytest
 import oracle
 from rise_amd import pt_scene
 from rise_amd.render import PathTracingSpectralRasterizer
+from tests import pt_closed_forms
 from tests.conftest import log_case
 
 pytestmark = pytest.mark.gpu
@@ -66,6 +68,22 @@ def test_gpu_white_furnace(cuda):
     var = gq[..., 1].mean() - y * y
     se = math.sqrt(max(var, 0) / n)
     assert abs(y - 2.0) < 4 * se + 1e-3, (y, se)
+
+
+def test_gpu_spectral_furnace(cuda):
+    """Non-flat step spectra: every companion wavelength carries its own
+    albedo and emission (closed form in tests/pt_closed_forms.py)."""
+    log_case("spectral furnace (step-spectrum albedo and emission)")
+    rho, le = pt_closed_forms.spectral_furnace_spectra()
+    want = pt_closed_forms.furnace_y(rho, le)
+    s, r = pt_scene.furnace_box(albedo=rho, radiance=le, max_bounce=96)
+    r.spp = 1024
+    g, gq, _ = _render(s, r)
+    n = r.width * r.height * r.spp
+    y = g[..., 1].mean()
+    se = math.sqrt(max(gq[..., 1].mean() - y * y, 0) / n)
+    assert abs(y - want) < 4 * se + 1e-3 * want, (y, want, se)
+    assert abs(pt_closed_forms.furnace_y_hero_only(rho, le) - want) > 12 * se
 
 
 def test_gpu_emitter_seen_directly(cuda):
