@@ -70,6 +70,85 @@ def load_traffic():
         return None
 
 
+REL_RMSE_TARGET = 0.01
+CORNELL_REFERENCE = ROOT / "tests" / "golden" / "cornell_oracle_1024spp.npz"
+
+
+def _trials(pt, rs, spp: int, trials: int, seed0: int):
+    """`trials` independent renders of `rs` at `spp` (render seeds seed0, seed0 + 1, ...):
+    (films (K, H, W, 3) float64 on the GPU, ms per render, rays per render)."""
+    import dataclasses
+
+    import torch
+
+    films, rays = [], 0
+    for k in range(trials):
+        r = dataclasses.replace(rs, spp=spp, seed=seed0 + k)
+        film, _, st = pt.rasterize_scene(r, stats=(k == 0))
+        if k == 0:
+            rays = st["extension_rays"] + st["shadow_rays"]
+        films.append(film.double())
+    # Timed: the last request again (its plan is reused, as in a render loop).
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    pt.rasterize_scene(r, stats=False)
+    b.record()
+    torch.cuda.synchronize()
+    return torch.stack(films), float(a.elapsed_time(b)), int(rays)
+
+
+def _rel_noise(films) -> float:
+    """Relative RMSE of one render from K independent ones: sqrt(mean over pixels
+    and XYZ channels of the inter-render variance) / mean of the K-render mean.
+    Unbiased renders (checked against the oracle) make this the distance to
+    the converged image (RISE's K-trial measure, docs/UNIFIED_INTEGRATOR_BASELINES.md)."""
+    var = films.var(dim=0, unbiased=True)
+    return float(var.mean().sqrt() / films.mean(dim=0).mean())
+
+
+def image_quality(pt, rs, spp: int, trials: int, seed0: int, ref=None):
+    """Relative RMSE at `spp` from `trials` renders, the samples per pixel
+    that reach REL_RMSE_TARGET under the 1/N law, and a measurement at that
+    count (K renders, their own relative RMSE, time and Mrays/s).  With an
+    oracle reference (dict: xyz, var_y, spp): each render's relative RMSE
+    against it and the per-pixel z-test of the K-render mean (Y) against it
+    with pooled variance (the inter-render variance and the reference's
+    per-sample variance / its spp)."""
+    import math
+
+    import numpy as np
+    import torch
+
+    films, ms, rays = _trials(pt, rs, spp, trials, seed0)
+    rel = _rel_noise(films)
+    out = {"spp": spp, "trials": trials, "rel_rmse": round(rel, 5), "ms_per_render": round(ms, 3)}
+    if ref is not None:
+        rx = torch.from_numpy(ref["xyz"].astype(np.float64)).to(films.device)
+        per = ((films - rx) ** 2).mean(dim=(1, 2, 3)).sqrt() / rx.mean()
+        out["rel_rmse_vs_oracle"] = round(float(per.mean()), 5)
+        out["oracle_spp"] = int(ref["spp"])
+        gm = films.mean(dim=0)[..., 1]
+        gv = films.var(dim=0, unbiased=True)[..., 1] / trials
+        ov = torch.from_numpy(ref["var_y"].astype(np.float64)).to(films.device) / float(ref["spp"])
+        z = (gm - rx[..., 1]) / torch.sqrt(gv + ov + 1e-30)
+        out["z_test_pass_rate_Y"] = round(float((z.abs() < 3.29).double().mean()), 5)
+        out["mean_Y_rel_diff_vs_oracle"] = float(gm.mean() / rx[..., 1].mean() - 1.0)
+    need = int(math.ceil(spp * (rel / REL_RMSE_TARGET) ** 2 / 16.0) * 16)
+    out["spp_for_1pct"] = need
+    f1, ms1, rays1 = _trials(pt, rs, need, max(4, trials // 2), seed0 + 1000)
+    out["at_spp_for_1pct"] = {
+        "rel_rmse": round(_rel_noise(f1), 5),
+        "trials": int(f1.shape[0]),
+        "ms_per_render": round(ms1, 2),
+        "mrays_per_s": round(rays1 / (ms1 / 1e3) / 1e6, 1),
+        "renders_per_s": round(1e3 / ms1, 3),
+    }
+    del films, f1
+    torch.cuda.empty_cache()
+    return out
+
+
 def render_leg(args, cpu_enabled: bool):
     """Cornell-box spectral path tracing: 256x256, 16 spp, 4 hero wavelengths,
     diffuse depth 5, Russian roulette from depth 3.  Rays = extension + shadow
@@ -115,6 +194,14 @@ def render_leg(args, cpu_enabled: bool):
         "parity": None,
     }
     log(f"render: {ms:.2f} ms, {out['mrays_per_s']} Mrays/s, {out['paths_per_s']:.3g} paths/s")
+    if not args.no_quality:
+        ref = None
+        if CORNELL_REFERENCE.exists():
+            z = np.load(CORNELL_REFERENCE)
+            ref = {"xyz": z["xyz"], "var_y": z["var_y"], "spp": int(z["spp"])}
+        out["quality"] = image_quality(pt, rs, rs.spp, args.quality_trials, 5000, ref)
+        out["rel_rmse"] = out["quality"]["rel_rmse"]
+        log(f"render quality: {out['quality']}")
     if cpu_enabled:
         import oracle
 
@@ -188,6 +275,15 @@ def terrain_leg(args):
     }
     log(f"terrain: {ms:.0f} ms, {out['mrays_per_s']} Mrays/s, {out['paths_per_s']:.3g} paths/s "
         f"({scene.n_tris} tris, setup {setup:.1f} s)")
+    if not args.no_quality:
+        import dataclasses
+
+        small = dataclasses.replace(rs, width=480, height=270)
+        q = image_quality(pt, small, rs.spp, max(4, args.quality_trials // 2), 9000)
+        q["image"] = "480x270 (a quarter of 1920x1080 per axis), same camera"
+        out["quality"] = q
+        out["rel_rmse"] = q["rel_rmse"]
+        log(f"terrain quality: {q}")
     pt.release()
     return out
 
@@ -503,6 +599,8 @@ def main() -> int:
     ap.add_argument("--terrain-steps", type=int, default=1)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--no-quality", action="store_true", help="skip the relative-RMSE measurements of the render legs")
+    ap.add_argument("--quality-trials", type=int, default=16, help="independent renders per relative-RMSE estimate")
     ap.add_argument("--no-widened", action="store_true",
                     help="skip the mesh-file / RGB-table / scene-file / output-stage timings")
     args = ap.parse_args()

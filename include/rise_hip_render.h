@@ -132,6 +132,29 @@ typedef struct rise_hip_camera_desc {
     float fov_deg;
 } rise_hip_camera_desc;
 
+/* Random numbers of the bounce loops: every sampler.Get1D() of the
+ * reference's integrator draws from an Owen-scrambled Sobol sequence
+ * (Sampling/SobolSequence.h, Utilities/SobolSampler.h, ZSobolSampler.h) at
+ * the dimension the reference's consumption order gives it, per pixel
+ * (PathTracingSpectralRasterizer.cpp:229-345).
+ *   RISE_HIP_SAMPLER_ZSOBOL (0, the reference's default, blue_noise_sampler
+ *     TRUE): index Morton(x, y) * 2^ceil(log2 spp) + sample, and the sample's
+ *     position inside the pixel's block shuffled per dimension pair.
+ *   RISE_HIP_SAMPLER_SOBOL (1, blue_noise_sampler FALSE): index = sample,
+ *     seed HashCombine(x, y), shuffled the same way.
+ *   RISE_HIP_SAMPLER_ZSOBOL_UNSHUFFLED (2): the Z-order draws exactly as the
+ *     reference makes them.  Every dimension pair then reuses one index, and
+ *     the estimate is biased (about -1% on a spectral furnace); kept for
+ *     comparison with the reference's noise.
+ * The hero wavelength, the per-pixel seed of the pixel positions and
+ * free-flight distances, which the reference draws from a Mersenne Twister,
+ * come from a counter-based hash of (seed, path). */
+typedef enum rise_hip_sampler {
+    RISE_HIP_SAMPLER_ZSOBOL = 0,
+    RISE_HIP_SAMPLER_SOBOL = 1,
+    RISE_HIP_SAMPLER_ZSOBOL_UNSHUFFLED = 2
+} rise_hip_sampler;
+
 typedef struct rise_hip_render_desc {
     int32_t width;
     int32_t height;
@@ -148,7 +171,7 @@ typedef struct rise_hip_render_desc {
     uint64_t seed;
     rise_hip_camera_desc camera;
     int32_t sky_background;          /* camera rays that miss see the sky (radiance_background) */
-    int32_t reserved;
+    int32_t sampler;                 /* rise_hip_sampler: the bounce loops' random numbers */
 } rise_hip_render_desc;
 
 typedef struct rise_hip_render_stats {

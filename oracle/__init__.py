@@ -228,7 +228,7 @@ class _RoScene(ctypes.Structure):
         ("sky_elevation_deg", ctypes.c_double), ("sky_azimuth_deg", ctypes.c_double),
         ("sky_turbidity", ctypes.c_double), ("sky_scale", ctypes.c_double),
         ("object_medium", ctypes.c_void_p), ("object_casts_shadows", ctypes.c_void_p),
-        ("media", ctypes.c_void_p), ("n_media", ctypes.c_int32),
+        ("media", ctypes.c_void_p), ("n_media", ctypes.c_int32), ("sampler", ctypes.c_int32),
     ]
 
 
@@ -282,6 +282,7 @@ def _ro_scene(scene, settings, spp_total, keep):
         rs.sky_turbidity = float(np.float32(scene.sky.turbidity))
         rs.sky_scale = float(np.float32(scene.sky.scale))
     rs.sky_background = 1 if settings.sky_background else 0
+    rs.sampler = int(getattr(settings, "sampler", 0))
     if scene.media:
         omed = np.array([o.medium for o in scene.objects], np.int32)
         med = (_RoMedium * len(scene.media))()
@@ -318,6 +319,21 @@ def render(scene, settings, s0: int = 0, s1: int | None = None, spp_total: int |
     if rc != 0:
         raise ValueError("oracle render rejected the scene")
     return sm, sq, cnt
+
+
+def sobol_u32(index: int, dim: int, seed: int) -> int:
+    """SobolSequence::Sample as the 32-bit scrambled value."""
+    L = oracle_lib()
+    L.rise_oracle_sobol_u32.argtypes = [ctypes.c_uint32] * 3
+    L.rise_oracle_sobol_u32.restype = ctypes.c_uint32
+    return int(L.rise_oracle_sobol_u32(index, dim, seed))
+
+
+def hash_combine(a: int, b: int) -> int:
+    L = oracle_lib()
+    L.rise_oracle_hash_combine.argtypes = [ctypes.c_uint32] * 2
+    L.rise_oracle_hash_combine.restype = ctypes.c_uint32
+    return int(L.rise_oracle_hash_combine(a, b))
 
 
 def rng(seed: int, path: int, dim: int) -> float:

@@ -35,6 +35,177 @@ double rise_oracle_rng(uint64_t seed, uint32_t path, uint32_t dim)
     return (double)(x >> 40) * (1.0 / 16777216.0);
 }
 
+/* The same hash's top 32 bits: a per-pixel scramble seed. */
+static uint32_t rng_u32(uint64_t seed, uint32_t path, uint32_t dim)
+{
+    uint64_t x = ((((uint64_t)path) << 32) | dim) * 0x9E3779B97F4A7C15ull + seed;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (uint32_t)(x >> 32);
+}
+
+/* ---- Owen-scrambled Sobol (Sampling/SobolSequence.h:121-200) -------------- */
+
+static uint32_t reverse_bits(uint32_t n)
+{
+    n = (n << 16) | (n >> 16);
+    n = ((n & 0x00ff00ffu) << 8) | ((n & 0xff00ff00u) >> 8);
+    n = ((n & 0x0f0f0f0fu) << 4) | ((n & 0xf0f0f0f0u) >> 4);
+    n = ((n & 0x33333333u) << 2) | ((n & 0xccccccccu) >> 2);
+    n = ((n & 0x55555555u) << 1) | ((n & 0xaaaaaaaau) >> 1);
+    return n;
+}
+
+/* Sobol dimension 1: direction numbers d_0 = 2^31, d_{i+1} = d_i ^ (d_i >> 1). */
+static uint32_t sobol_dim1(uint32_t index)
+{
+    uint32_t v = 0, d = 1u << 31;
+    for (; index; index >>= 1, d ^= d >> 1)
+        if (index & 1u) v ^= d;
+    return v;
+}
+
+/* SobolSequence::HashCombine: golden-ratio mix, then the MurmurHash3 fmix32 finalizer. */
+uint32_t rise_oracle_hash_combine(uint32_t a, uint32_t b)
+{
+    a ^= b + 0x9e3779b9u + (a << 6) + (a >> 2);
+    a ^= a >> 16;
+    a *= 0x85ebca6bu;
+    a ^= a >> 13;
+    a *= 0xc2b2ae35u;
+    a ^= a >> 16;
+    return a;
+}
+
+/* Laine-Karras hash on the bit-reversed value (Burley 2020, Listing 1). */
+static uint32_t owen_scramble(uint32_t v, uint32_t seed)
+{
+    v = reverse_bits(v);
+    v ^= v * 0x3d20adeau;
+    v += seed;
+    v *= (seed >> 16) | 1u;
+    v ^= v * 0x05526c56u;
+    v ^= v * 0x53a22864u;
+    return reverse_bits(v);
+}
+
+/* SobolSequence::Sample as 32 bits: padded Sobol (dimension & 1), scrambled
+ * with HashCombine(seed, dimension). */
+uint32_t rise_oracle_sobol_u32(uint32_t index, uint32_t dim, uint32_t seed)
+{
+    const uint32_t v = (dim & 1u) ? sobol_dim1(index) : reverse_bits(index);
+    return owen_scramble(v, rise_oracle_hash_combine(seed, dim));
+}
+
+/* The sample in [0, 1), quantized to 24 bits as the HIP renderer's float draws
+ * are (the reference keeps 32; the difference is below 2^-24). */
+double rise_oracle_sobol(uint32_t index, uint32_t dim, uint32_t seed)
+{
+    return (double)(rise_oracle_sobol_u32(index, dim, seed) >> 8) * (1.0 / 16777216.0);
+}
+
+/* MortonCode::Part1By1 / Morton2D / RoundUpPow2 / Log2Int (Utilities/MortonCode.h). */
+static uint32_t part1by1(uint32_t x)
+{
+    x &= 0x0000ffffu;
+    x = (x ^ (x << 8)) & 0x00ff00ffu;
+    x = (x ^ (x << 4)) & 0x0f0f0f0fu;
+    x = (x ^ (x << 2)) & 0x33333333u;
+    x = (x ^ (x << 1)) & 0x55555555u;
+    return x;
+}
+
+static uint32_t log2_pow2_ceil(uint32_t v)
+{
+    uint32_t p = 1, l = 0;
+    while (p < v) {
+        p <<= 1;
+        ++l;
+    }
+    return l;
+}
+
+/* The random streams of one camera sample.  Sampler draws (every
+ * sampler.Get1D() of the bounce loops) come from the Owen-scrambled Sobol
+ * sequence at (index, scramble): PathTracingSpectralRasterizer::IntegratePixel
+ * (:229-345) takes index = (Morton(x, y) << log2(spp rounded up to a power of
+ * two)) | sample and pixel seed HashCombine(Morton, 0) for the Z-order sampler
+ * (the default, blue_noise_sampler), index = sample and seed HashCombine(x, y)
+ * for the plain one; the render seed, when not 0, is folded into the pixel
+ * seed with HashCombine so independent renders scramble differently.
+ *
+ * Shuffling: the reference pads with Sobol dimensions 0 and 1 at the SAME
+ * index for every dimension pair, so dimensions 0, 2, 4, ... are scrambles of
+ * one van der Corput point and are strongly dependent; with the Laine-Karras
+ * hash standing in for random Owen scrambling this is biased (-1.0% on the
+ * spectral furnace's closed form, tests/test_oracle_pt.py).  The default
+ * samplers therefore also shuffle the index per dimension pair, as Burley
+ * (JCGT 2020, section 4) does: the sample's position inside the pixel's block
+ * of 2^log2(spp) indices is Owen-scrambled with a per-pair seed, a permutation
+ * of the block, so each pixel still uses exactly its own block of the
+ * sequence.  SAMPLER_ZSOBOL_UNSHUFFLED keeps the reference's draws unchanged.
+ *
+ * The draws the reference takes from its Mersenne Twister (hero wavelength,
+ * the per-pixel seed of the pixel-position Sobol points, free-flight
+ * distances) come from the counter hash at (seed, path). */
+typedef struct {
+    uint64_t seed;
+    uint32_t path;
+    uint32_t index, scramble;
+    uint32_t mask; /* the pixel's index block (2^log2(spp) - 1); 0: no shuffle */
+} rstream;
+
+static rstream make_stream(const ro_scene* s, uint32_t px, uint32_t py, uint32_t si)
+{
+    rstream r;
+    r.seed = s->seed;
+    r.path = (uint32_t)(((uint64_t)py * (uint64_t)s->width + px) * (uint64_t)s->spp_total + si);
+    uint32_t pseed = rise_oracle_hash_combine(px, py);
+    const uint32_t l2 = log2_pow2_ceil((uint32_t)(s->spp_total > 0 ? s->spp_total : 1));
+    r.index = si;
+    r.mask = s->sampler == 2 ? 0u : (l2 >= 32 ? 0xffffffffu : (1u << l2) - 1u);
+    if (s->sampler != 1 && px <= 0xffffu && py <= 0xffffu) {
+        const uint32_t mi = part1by1(px) | (part1by1(py) << 1);
+        if (l2 < 32 && ((uint64_t)mi << l2) < (1ull << 32)) {
+            r.index = (mi << l2) | si;
+            pseed = rise_oracle_hash_combine(mi, 0u);
+        }
+    }
+    const uint32_t s32 = (uint32_t)s->seed ^ (uint32_t)(s->seed >> 32);
+    r.scramble = s32 ? rise_oracle_hash_combine(pseed, s32) : pseed;
+    return r;
+}
+
+static double u_sampler(const rstream* r, uint32_t dim)
+{
+    uint32_t index = r->index;
+    if (r->mask) {
+        const uint32_t pair_seed = rise_oracle_hash_combine(r->scramble, 0x51ed270bu + (dim >> 1));
+        index = (index & ~r->mask) | (owen_scramble(index, pair_seed) & r->mask);
+    }
+    return rise_oracle_sobol(index, dim, r->scramble);
+}
+static double u_indep(const rstream* r, uint32_t dim) { return rise_oracle_rng(r->seed, r->path, dim); }
+
+/* Sampler dimensions of one vertex, in the order the reference consumes them
+ * after StartStream(16 + depth) (PathTracingIntegrator.cpp:4112, :1686):
+ * per light sample 4 (alias-table pick, then the three numbers of
+ * UniformRandomPoint; LightSampler.cpp:2090-2215), then the BSDF's own draws
+ * (Lambertian 2: LambertianSPF.cpp:90-99; GGX 3: the lobe, then 2,
+ * GGXSPF.cpp:479-519; smooth dielectric 2, unused: DielectricSPF.cpp:421; the
+ * phase function 2: HenyeyGreensteinPhaseFunction.cpp:81-93), then the
+ * lobe choice (RandomlySelect, :4619 / :2670 / :2945) -- not at a volume
+ * scatter -- then Russian roulette (:4757 / :3224 / :1870) -- not at a
+ * dielectric.  A shared (four-wavelength) vertex takes four light samples,
+ * a single-wavelength one (IntegrateFromHitNM) one; every single-wavelength
+ * lane of a path restarts the same stream at each depth, so the lanes of one
+ * path share their sampler draws, as the reference's do. */
+#define NEE_DIMS 4u
+#define STREAM_BASE(depth) (32u * (uint32_t)(16 + (depth)))
+
 /* ---- CIE 1931 2-degree colour matching functions, 380-780 nm, 5 nm ------- */
 /* (ColorUtils.cpp:81; the z table is zero past 630 nm there, as here.) */
 static const double CIE_X[81] = {
@@ -922,18 +1093,18 @@ static double lum_pdf_select(const prep* P, int lum)
 
 /* EvaluateDirectLightingNM for area luminaries (LightSampler.cpp:1987-2269);
  * mobj: the object whose medium holds the shading point (-1: none). */
-static double direct_nm(prep* P, const hit* h, double nm, double amb, int32_t mobj, uint64_t seed, uint32_t path,
-                        uint32_t dim0, uint64_t* n_shadow)
+static double direct_nm(prep* P, const hit* h, double nm, double amb, int32_t mobj, const rstream* R, uint32_t dim0,
+                        uint64_t* n_shadow)
 {
     if (P->n_lums == 0) return 0.0;
     const int self = P->lum_of_object[h->object];
     double pdf_alias;
-    const int li = select_luminary(P, rise_oracle_rng(seed, path, dim0), &pdf_alias);
+    const int li = select_luminary(P, u_sampler(R, dim0), &pdf_alias);
     if (li < 0 || li == self) return 0.0;
     const luminary* L = &P->lums[li];
-    const double ux = rise_oracle_rng(seed, path, dim0 + 1);
-    const double uy = rise_oracle_rng(seed, path, dim0 + 2);
-    const double uz = rise_oracle_rng(seed, path, dim0 + 3);
+    const double ux = u_sampler(R, dim0 + 1);
+    const double uy = u_sampler(R, dim0 + 2);
+    const double uz = u_sampler(R, dim0 + 3);
     /* UniformRandomPoint (TriangleMeshGeometryIndexed.cpp:207-218): first
      * CDF entry >= uz, then PointOnTriangle (GeometricUtilities.cpp:543-560). */
     int64_t k = 0;
@@ -971,17 +1142,17 @@ static double direct_nm(prep* P, const hit* h, double nm, double amb, int32_t mo
  * function as BSDF and as MIS pdf, no shading object -- hence no self
  * exclusion and no occlusion test (LightSampler.cpp:2200-2211 tests only for
  * a shading object) -- and the transmittance walk from inside mobj's medium. */
-static double direct_volume(prep* P, v3 p, v3 wo, double g, int32_t mobj, double nm, uint64_t seed, uint32_t path,
+static double direct_volume(prep* P, v3 p, v3 wo, double g, int32_t mobj, double nm, const rstream* R,
                             uint32_t dim0, uint64_t* n_shadow)
 {
     if (P->n_lums == 0) return 0.0;
     double pdf_alias;
-    const int li = select_luminary(P, rise_oracle_rng(seed, path, dim0), &pdf_alias);
+    const int li = select_luminary(P, u_sampler(R, dim0), &pdf_alias);
     if (li < 0) return 0.0;
     const luminary* L = &P->lums[li];
-    const double ux = rise_oracle_rng(seed, path, dim0 + 1);
-    const double uy = rise_oracle_rng(seed, path, dim0 + 2);
-    const double uz = rise_oracle_rng(seed, path, dim0 + 3);
+    const double ux = u_sampler(R, dim0 + 1);
+    const double uy = u_sampler(R, dim0 + 2);
+    const double uz = u_sampler(R, dim0 + 3);
     int64_t k = 0;
     while (k < L->n - 1 && L->cdf[k] < uz) ++k;
     v3 a, b, c;
@@ -1040,14 +1211,32 @@ static int over_cap(const ro_scene* s, int type, int* diffuse, int* glossy, int*
     return ++*transmission > s->max_transmission_bounce;
 }
 
+/* Key offset of single-wavelength lane w's independent draws (free flight):
+ * (w + 1) << 20, so lanes do not share them. */
 #define NM_LANE_DIMS (1u << 20)
+
+/* The BSDF's sampler draws at a vertex whose BSDF dimensions start at s0;
+ * returns the dimension of its Russian-roulette draw. */
+static uint32_t bsdf_draws(const rstream* R, int type, uint32_t s0, double* u1, double* u2, double* u3)
+{
+    if (type == MAT_GGX) {
+        *u3 = u_sampler(R, s0);
+        *u1 = u_sampler(R, s0 + 1);
+        *u2 = u_sampler(R, s0 + 2);
+        return s0 + 4;
+    }
+    *u1 = u_sampler(R, s0);
+    *u2 = u_sampler(R, s0 + 1);
+    *u3 = 0.0;
+    return s0 + 3;
+}
 
 /* IntegrateFromHitNM: the single-wavelength loop a path continues with after
  * a dielectric hit, starting at that hit (PathTracingIntegrator.cpp:1561-3377).
  * Returns the radiance of lane `lane` (wavelength nm) relative to the
  * throughput at the hit. */
 static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, double importance, double bsdf_pdf,
-                      int diffuse, int glossy, int transmission, uint64_t seed, uint32_t path, uint64_t* n_ext,
+                      int diffuse, int glossy, int transmission, const rstream* R, uint64_t* n_ext,
                       uint64_t* n_shadow)
 {
     int vb = 0; /* volume bounces (none before the branch: the shared loop has no medium) */
@@ -1059,7 +1248,8 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
     v3 org = h0->origin, dir = h0->dir;
     for (int depth = depth0; depth < MAX_PATH_DEPTH; ++depth) {
         if (!isfinite(thr) || fabs(thr) > 1e6) break;
-        const uint32_t base = 32u * (uint32_t)(16 + depth) + NM_LANE_DIMS * (uint32_t)(lane + 1);
+        const uint32_t base = STREAM_BASE(depth);
+        const uint32_t lane_key = base + NM_LANE_DIMS * (uint32_t)(lane + 1);
         if (depth != depth0) {
             double t;
             ++*n_ext;
@@ -1075,7 +1265,7 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
                 int scattered = 0;
                 double tm = max_dist;
                 if (mc.sigma_t > 0.0) {
-                    const double xi = rise_oracle_rng(seed, path, base + 20);
+                    const double xi = u_indep(R, lane_key + 20);
                     const double tt = -log(fmax(1.0 - xi, 1e-30)) / mc.sigma_t;
                     if (tt < max_dist) {
                         scattered = 1;
@@ -1092,18 +1282,17 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
                     if (mc.sigma_t > 0 && tr > 0) mw = tr * mc.sigma_s / (mc.sigma_t * tr);
                     if (!(mw > 0)) break;
                     thr *= mw;
-                    const double ld = direct_volume(P, sp, dir, M->g, mobj, nm, seed, path, base, n_shadow);
+                    const double ld = direct_volume(P, sp, dir, M->g, mobj, nm, R, base, n_shadow);
                     if (ld > 0) result += thr * ld;
                     const double wo[3] = {dir.x, dir.y, dir.z};
                     double wi[3];
-                    rise_oracle_hg_sample(wo, M->g, rise_oracle_rng(seed, path, base + 16),
-                                          rise_oracle_rng(seed, path, base + 17), wi);
+                    rise_oracle_hg_sample(wo, M->g, u_sampler(R, base + NEE_DIMS), u_sampler(R, base + NEE_DIMS + 1), wi);
                     const v3 nd = mk(wi[0], wi[1], wi[2]);
                     const double pp = rise_oracle_hg_eval(dot(dir, nd), M->g);
                     if (pp <= NZ) break;
                     thr *= rise_oracle_hg_eval(dot(dir, nd), M->g) / pp;
                     const rr_result rr = russian_roulette(depth + vb, s->rr_min_depth, s->rr_threshold, fabs(thr),
-                                                          importance, rise_oracle_rng(seed, path, base + 19));
+                                                          importance, u_sampler(R, base + NEE_DIMS + 2));
                     if (rr.terminate) break;
                     if (rr.survival < 1.0) thr /= rr.survival;
                     org = sp;
@@ -1134,7 +1323,7 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
             double wgt;
             int type;
             iorstack nst;
-            if (!dielectric_scatter(P, &h, nm, &st, rise_oracle_rng(seed, path, base + 18), &nd, &wgt, &type, &nst))
+            if (!dielectric_scatter(P, &h, nm, &st, u_sampler(R, base + 2), &nd, &wgt, &type, &nst))
                 break;
             if (over_cap(s, type, &diffuse, &glossy, &transmission)) break;
             importance *= fabs(wgt);
@@ -1145,14 +1334,15 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
             dir = nd;
             continue;
         }
-        result += thr * direct_nm(P, &h, nm, amb, medium_object(P, &st), seed, path, base, n_shadow);
-        const rb_sample sc = bsdf_scatter(P, &h, nm, amb, rise_oracle_rng(seed, path, base + 16),
-                                          rise_oracle_rng(seed, path, base + 17), rise_oracle_rng(seed, path, base + 18));
+        result += thr * direct_nm(P, &h, nm, amb, medium_object(P, &st), R, base, n_shadow);
+        double u1, u2, u3;
+        const uint32_t rr_dim = bsdf_draws(R, h.mat->type, base + NEE_DIMS, &u1, &u2, &u3);
+        const rb_sample sc = bsdf_scatter(P, &h, nm, amb, u1, u2, u3);
         if (!sc.ok) break;
         double scat = sc.kray;
         if (fabs(scat) <= NZ) break;
         const rr_result rr = russian_roulette(depth, s->rr_min_depth, s->rr_threshold, importance * fabs(scat),
-                                              importance, rise_oracle_rng(seed, path, base + 19));
+                                              importance, u_sampler(R, rr_dim));
         if (rr.terminate) break;
         if (rr.survival < 1.0) scat /= rr.survival;
         importance *= fabs(scat);
@@ -1171,11 +1361,15 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
 static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_out[3])
 {
     const ro_scene* s = P->s;
-    const uint64_t seed = s->seed;
-    const uint32_t path = (uint32_t)(((uint64_t)py * s->width + px) * (uint64_t)s->spp_total + (uint64_t)si);
+    const rstream RS = make_stream(s, (uint32_t)px, (uint32_t)py, (uint32_t)si);
+    const rstream* R = &RS;
     /* Camera (stream 0): jitter, then hero wavelength. */
-    const double jx = rise_oracle_rng(seed, path, 0), jy = rise_oracle_rng(seed, path, 1);
-    const double ul = rise_oracle_rng(seed, path, 2);
+    /* Pixel position: SobolSampling2D::GenerateSamplePoints (Sampling/SobolSampling2D.cpp),
+     * Owen-scrambled Sobol points of the pixel's samples under one seed per
+     * pixel (the reference draws that seed from its Mersenne Twister). */
+    const uint32_t pix_seed = rng_u32(s->seed, (uint32_t)((uint64_t)py * s->width + px), 0xffffffffu);
+    const double jx = rise_oracle_sobol((uint32_t)si, 0, pix_seed), jy = rise_oracle_sobol((uint32_t)si, 1, pix_seed);
+    const double ul = u_indep(R, 2);
     const v3 cp = mk(s->cam_pos[0], s->cam_pos[1], s->cam_pos[2]);
     const v3 f = normalize(sub(mk(s->cam_look[0], s->cam_look[1], s->cam_look[2]), cp));
     const v3 r = normalize(cross(f, mk(s->cam_up[0], s->cam_up[1], s->cam_up[2])));
@@ -1205,7 +1399,7 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
             if (fabs(thr[w]) > mx) mx = fabs(thr[w]);
         }
         if (bad || mx > 1e6) break;
-        const uint32_t base = 32u * (uint32_t)(16 + depth);
+        const uint32_t base = STREAM_BASE(depth);
 
         double t;
         ++n_ext;
@@ -1225,7 +1419,7 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         if (h.mat->type == MAT_DIELECTRIC) {
             for (int w = 0; w < HWSS_N; ++w)
                 result[w] += thr[w] * nm_path(P, &h, lam[w], w, depth, importance, bsdf_pdf, diffuse, glossy,
-                                              transmission, seed, path, &n_ext, &n_shadow);
+                                              transmission, R, &n_ext, &n_shadow);
             break;
         }
 
@@ -1234,11 +1428,12 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         /* Next-event estimation, one light sample per wavelength
          * (PathTracingIntegrator.cpp:4521-4552). */
         for (int w = 0; w < HWSS_N; ++w)
-            result[w] += thr[w] * direct_nm(P, &h, lam[w], 1.0, -1, seed, path, base + 4u * (uint32_t)w, &n_shadow);
+            result[w] += thr[w] * direct_nm(P, &h, lam[w], 1.0, -1, R, base + NEE_DIMS * (uint32_t)w, &n_shadow);
 
         /* Hero ScatterNM, then companions at the hero's direction (:4620-4718). */
-        const rb_sample hs = bsdf_scatter(P, &h, lam[0], 1.0, rise_oracle_rng(seed, path, base + 16),
-                                          rise_oracle_rng(seed, path, base + 17), rise_oracle_rng(seed, path, base + 18));
+        double u1, u2, u3;
+        const uint32_t rr_dim = bsdf_draws(R, h.mat->type, base + HWSS_N * NEE_DIMS, &u1, &u2, &u3);
+        const rb_sample hs = bsdf_scatter(P, &h, lam[0], 1.0, u1, u2, u3);
         if (!hs.ok) break;
         const v3 nd = vr(hs.dir);
         double sc[HWSS_N];
@@ -1257,7 +1452,7 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         }
         if (mcurr <= NZ) break;
         const rr_result rr = russian_roulette(depth, s->rr_min_depth, s->rr_threshold, mcurr, mprev,
-                                              rise_oracle_rng(seed, path, base + 19));
+                                              u_sampler(R, rr_dim));
         if (rr.terminate) break;
         if (rr.survival < 1.0)
             for (int w = 0; w < HWSS_N; ++w) sc[w] *= 1.0 / rr.survival;

@@ -30,18 +30,25 @@
  *   Shaders/PathTracingIntegrator.cpp:364-530, 1686-1925            medium transport in the NM loop
  *   Utilities/MediumTracking.h, Utilities/MediumTransport.cpp:139-174  current medium, in-scattering NEE
  *   Lights/LightSampler.cpp:140-470                                 shadow-ray transmittance walk
- * Random numbers: the reference draws per-dimension samples from an
- * Owen-scrambled Sobol sequence (stream 16 + depth, 32 dimensions per
- * stream) and hero wavelength / pixel jitter from a Mersenne Twister; it is
- * not reproducible run to run.  The oracle instead draws every number from
- * a counter-based hash of (seed, path, dimension) with the same stream
- * layout, the same function the HIP renderer uses, so both follow the same
- * random decisions and their per-sample results can be compared directly.
- * Dimensions per bounce (stream 16 + depth, base 32 * stream): +4w..+4w+3
- * the light sample of wavelength w, +16 and +17 the BSDF direction, +18 the
- * lobe choice, +19 Russian roulette.  After a dielectric hit the path goes on
- * as four single-wavelength paths; lane w adds (w + 1) << 20 to every
- * dimension of its bounces.
+ * Random numbers: the reference draws every sampler.Get1D() of its bounce
+ * loops from an Owen-scrambled Sobol sequence (Sampling/SobolSequence.h,
+ * Utilities/SobolSampler.h, ZSobolSampler.h; stream 16 + depth, 32
+ * dimensions per stream), the pixel positions from Owen-scrambled Sobol
+ * points under a per-pixel seed (Sampling/SobolSampling2D.cpp), and the hero
+ * wavelength, that per-pixel seed and free-flight distances from a Mersenne
+ * Twister seeded per thread, so its renders are not reproducible.  The oracle
+ * draws the same Sobol numbers at the same dimensions, in the order the
+ * reference consumes them (pt_oracle.c: make_stream, NEE_DIMS, bsdf_draws),
+ * with the index shuffled per dimension pair unless the unshuffled sampler
+ * is selected, and takes the Mersenne-Twister draws from a counter-based hash
+ * of (seed, path, dimension).  The HIP renderer uses the same functions, so
+ * both follow the same random decisions and their per-sample results can be
+ * compared directly.  Per bounce (base 32 * (16 + depth)): a shared
+ * four-wavelength vertex takes +4w..+4w+3 for the light sample of wavelength
+ * w and its BSDF draws from +16; a single-wavelength vertex (after a
+ * dielectric hit every wavelength goes on as its own path) takes +0..+3 for
+ * its light sample and its BSDF draws from +4, or +0..+2 at a dielectric;
+ * the lanes of one path share these draws, as the reference's branches do.
  */
 #ifndef RISE_PT_ORACLE_H
 #define RISE_PT_ORACLE_H
@@ -109,11 +116,22 @@ typedef struct ro_scene {
     const uint8_t* object_casts_shadows; /* NULL: every object casts shadows */
     const ro_medium* media;
     int32_t n_media;
+    /* Sampler of the bounce loops: 0 Z-order Owen-scrambled Sobol (the
+     * reference's default, blue_noise_sampler), 1 per-pixel Owen-scrambled
+     * Sobol, both with the index shuffled per dimension pair; 2 the Z-order
+     * sampler exactly as the reference draws it (no shuffle). */
+    int32_t sampler;
 } ro_scene;
 
 /* Counter-based uniform in [0,1) with 24 random bits (exactly representable
  * in float): splitmix64 finalizer of key * golden + seed, key = path << 32 | dim. */
 double rise_oracle_rng(uint64_t seed, uint32_t path, uint32_t dim);
+
+/* SobolSequence::HashCombine and SobolSequence::Sample (as 32 bits, and in
+ * [0, 1) with 24 bits) (Sampling/SobolSequence.h:121-200). */
+uint32_t rise_oracle_hash_combine(uint32_t a, uint32_t b);
+uint32_t rise_oracle_sobol_u32(uint32_t index, uint32_t dim, uint32_t seed);
+double rise_oracle_sobol(uint32_t index, uint32_t dim, uint32_t seed);
 
 /* Renders samples [s0, s1) of every pixel.  xyz_sum (W*H*3) receives the
  * sum of per-sample XYZ, xyz_sq (optional) the sum of squares; counters

@@ -16,6 +16,8 @@
 //                                                 Utilities/PathTransportUtilities.h:86-258
 //   SampledWavelengths::SampleEquidistant         Utilities/Color/SampledWavelengths.h:56-90
 //   OrthonormalBasis3D::CreateFromW               Utilities/OrthonormalBasis3D.cpp
+//   SobolSequence::Sample / HashCombine           Sampling/SobolSequence.h:121-200
+//   MortonCode::Morton2D                          Utilities/MortonCode.h
 
 #include "Utilities/HosekWilkieSkyModel.h"
 #include "Utilities/MicrofacetEnergyLUT.h"
@@ -24,6 +26,8 @@
 #include "Utilities/OrthonormalBasis3D.h"
 #include "Utilities/PathTransportUtilities.h"
 #include "Utilities/Color/SampledWavelengths.h"
+#include "Utilities/MortonCode.h"
+#include "Sampling/SobolSequence.h"
 
 #include <cmath>
 
@@ -168,5 +172,14 @@ void rise_ref_onb_from_w(const double w[3], double onb[9])
     put(b.v(), onb + 3);
     put(b.w(), onb + 6);
 }
+
+double rise_ref_sobol_sample(uint32_t index, uint32_t dim, uint32_t seed)
+{
+    return SobolSequence::Sample(index, dim, seed);
+}
+
+uint32_t rise_ref_hash_combine(uint32_t a, uint32_t b) { return SobolSequence::HashCombine(a, b); }
+
+uint32_t rise_ref_morton2d(uint32_t x, uint32_t y) { return MortonCode::Morton2D(x, y); }
 
 }  // extern "C"

@@ -103,11 +103,11 @@ struct PtSceneView {
 };
 
 // ---- random numbers: counter-based hash ------------------------------------
-// u = top 24 bits of splitmix64(key * golden + seed), key = path << 32 | dim.
-// The oracle (oracle/pt_oracle.c) uses the same function, so both follow the
-// same random decisions.  Dimension layout: stream s covers dims 32s..32s+31;
-// the camera uses stream 0, bounce `depth` uses stream 16 + depth, as
-// SobolSampler::StartStream(16 + depth) does (PathTracingIntegrator.cpp:4112).
+// u = top 24 bits of splitmix64(key * golden + seed), key = path << 32 | dim:
+// the draws the reference takes from its Mersenne Twister (hero wavelength,
+// the pixel-position points' per-pixel seed, free-flight distances).  The
+// oracle (oracle/pt_oracle.c) uses the same function, so both follow the
+// same random decisions.
 RISE_HD inline float rng_u(unsigned long long seed, unsigned path, unsigned dim)
 {
     unsigned long long x = ((((unsigned long long)path) << 32) | dim) * 0x9E3779B97F4A7C15ull + seed;
@@ -118,6 +118,100 @@ RISE_HD inline float rng_u(unsigned long long seed, unsigned path, unsigned dim)
     x ^= x >> 31;
     return (float)(unsigned)(x >> 40) * (1.0f / 16777216.0f);
 }
+
+// The same hash's top 32 bits (a per-pixel scramble seed).
+RISE_HD inline unsigned rng_u32(unsigned long long seed, unsigned path, unsigned dim)
+{
+    unsigned long long x = ((((unsigned long long)path) << 32) | dim) * 0x9E3779B97F4A7C15ull + seed;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (unsigned)(x >> 32);
+}
+
+// ---- Owen-scrambled Sobol (Sampling/SobolSequence.h:121-200) ----------------
+// Every sampler.Get1D() of the reference's bounce loops.  Padded Sobol:
+// dimension d uses Sobol dimension d & 1 (van der Corput, or the (0, 2)
+// sequence's second dimension) scrambled by the Laine-Karras hash with seed
+// HashCombine(seed, d).  The oracle (oracle/pt_oracle.c) computes the same
+// numbers; both quantize to 24 bits, as rng_u does.
+RISE_HD inline unsigned reverse_bits32(unsigned n) { return __builtin_bitreverse32(n); }
+
+RISE_HD inline unsigned sobol_dim1(unsigned index)
+{
+    unsigned v = 0u, d = 1u << 31;
+    for (; index; index >>= 1, d ^= d >> 1)
+        if (index & 1u) v ^= d;
+    return v;
+}
+
+RISE_HD inline unsigned hash_combine(unsigned a, unsigned b)
+{
+    a ^= b + 0x9e3779b9u + (a << 6) + (a >> 2);
+    a ^= a >> 16;
+    a *= 0x85ebca6bu;
+    a ^= a >> 13;
+    a *= 0xc2b2ae35u;
+    a ^= a >> 16;
+    return a;
+}
+
+RISE_HD inline unsigned owen_scramble(unsigned v, unsigned seed)
+{
+    v = reverse_bits32(v);
+    v ^= v * 0x3d20adeau;
+    v += seed;
+    v *= (seed >> 16) | 1u;
+    v ^= v * 0x05526c56u;
+    v ^= v * 0x53a22864u;
+    return reverse_bits32(v);
+}
+
+RISE_HD inline float sobol_f(unsigned index, unsigned dim, unsigned seed)
+{
+    const unsigned v = (dim & 1u) ? sobol_dim1(index) : reverse_bits32(index);
+    return (float)(owen_scramble(v, hash_combine(seed, dim)) >> 8) * (1.0f / 16777216.0f);
+}
+
+// The sampler of one camera sample: Sobol index, scramble seed, and the
+// pixel's block of indices inside which the index is shuffled per dimension
+// pair (mask 0: no shuffle, the reference's own draws).  See sample_key in
+// pt_render.hip for how a path's key is made.
+struct SampleKey {
+    unsigned index, scramble, mask;
+};
+
+RISE_HD inline float sample_u(const SampleKey& k, unsigned dim)
+{
+    unsigned index = k.index;
+    if (k.mask) {
+        const unsigned pair_seed = hash_combine(k.scramble, 0x51ed270bu + (dim >> 1));
+        index = (index & ~k.mask) | (owen_scramble(index, pair_seed) & k.mask);
+    }
+    return sobol_f(index, dim, k.scramble);
+}
+
+// MortonCode::Part1By1 (Utilities/MortonCode.h).
+RISE_HD inline unsigned morton_part1by1(unsigned x)
+{
+    x &= 0x0000ffffu;
+    x = (x ^ (x << 8)) & 0x00ff00ffu;
+    x = (x ^ (x << 4)) & 0x0f0f0f0fu;
+    x = (x ^ (x << 2)) & 0x33333333u;
+    x = (x ^ (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// Sampler dimensions per vertex, in the reference's consumption order after
+// StartStream(16 + depth): 4 per light sample (alias pick, then the three
+// numbers of UniformRandomPoint), then the BSDF's draws (Lambertian 2; GGX 3:
+// the lobe, then 2; smooth dielectric 2, unused; the phase function 2), the
+// lobe choice (not at a volume scatter), Russian roulette (not at a
+// dielectric).
+constexpr unsigned kNeeDims = 4u;
+RISE_HD inline unsigned stream_base(int depth) { return 32u * (unsigned)(16 + depth); }
 
 // ---- small float vector ----------------------------------------------------
 struct f3 {

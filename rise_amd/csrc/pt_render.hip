@@ -131,6 +131,10 @@ struct ReqInfo {
     float xyz_scale;  // mYNormalization / N (PathTracingSpectralRasterizer.cpp:168-173)
     int sky_background;  // camera rays that miss see the sky
     unsigned long long seed;
+    // Sampler (sample_key): Z-order index, log2 of spp rounded up to a power
+    // of two, the shuffle mask (0: unshuffled), the render seed folded to 32 bits.
+    int sob_zorder;
+    unsigned sob_l2, sob_mask, seed32;
     float cam_pos[3], cam_f[3], cam_r[3], cam_u[3];
     float cam_h;      // 2 tan(fov / 2)
     float *fsum, *fcomp, *fsq, *fsqc;  // 3 * width * height each
@@ -244,6 +248,29 @@ __device__ inline int find_entry(const PassEntry* __restrict__ ent, int n, int x
     return lo;
 }
 
+// The sampler key of a path (pixel index * spp_total + sample):
+// PathTracingSpectralRasterizer::IntegratePixel (:229-345) with the Z-order
+// (ZSobolSampler) or plain (SobolSampler) index and per-pixel seed; the render
+// seed, when not 0, folded into the pixel seed with HashCombine.  The oracle's
+// make_stream (oracle/pt_oracle.c) computes the same key.
+__device__ inline SampleKey sample_key(const ReqInfo& R, unsigned path)
+{
+    const unsigned st = (unsigned)R.spp_total, w = (unsigned)R.width;
+    const unsigned pix = path / st, si = path - pix * st;
+    const unsigned py = pix / w, px = pix - py * w;
+    unsigned pseed = hash_combine(px, py);
+    SampleKey k{si, 0u, R.sob_mask};
+    if (R.sob_zorder && px <= 0xffffu && py <= 0xffffu) {
+        const unsigned mi = morton_part1by1(px) | (morton_part1by1(py) << 1);
+        if (R.sob_l2 < 32u && ((unsigned long long)mi << R.sob_l2) < (1ull << 32)) {
+            k.index = (mi << R.sob_l2) | si;
+            pseed = hash_combine(mi, 0u);
+        }
+    }
+    k.scramble = R.seed32 ? hash_combine(pseed, R.seed32) : pseed;
+    return k;
+}
+
 __global__ void k_fill(float* p, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -269,7 +296,11 @@ __global__ void k_generate(const ReqInfo* __restrict__ reqs, const PassEntry* __
         const int s = E.s0 + local % E.ns;
         const int px = pix % R.width, py = pix / R.width;
         const unsigned path = (unsigned)((unsigned long long)pix * (unsigned long long)R.spp_total + (unsigned)s);
-        const float jx = rng_u(R.seed, path, 0), jy = rng_u(R.seed, path, 1), ul = rng_u(R.seed, path, 2);
+        // Pixel position: SobolSampling2D::GenerateSamplePoints (Sampling/SobolSampling2D.cpp),
+        // Owen-scrambled Sobol points of the pixel's samples under one seed per pixel.
+        const unsigned pix_seed = rng_u32(R.seed, (unsigned)pix, 0xffffffffu);
+        const float jx = sobol_f((unsigned)s, 0u, pix_seed), jy = sobol_f((unsigned)s, 1u, pix_seed);
+        const float ul = rng_u(R.seed, path, 2);
         const float sx = (float)px + jx, sy = (float)py + jy;
         const float cx = (sx - 0.5f * (float)R.width) * R.cam_h / (float)R.height;
         const float cy = -(sy - 0.5f * (float)R.height) * R.cam_h / (float)R.height;
@@ -448,6 +479,25 @@ __device__ inline ScatterOut bsdf_scatter(const PtSceneView& sc, const DevMateri
     return out;
 }
 
+// The BSDF's sampler draws at a vertex whose BSDF dimensions start at s0, in
+// the reference's order (GGX: the lobe, then two, GGXSPF.cpp:479-519;
+// Lambertian: two, LambertianSPF.cpp:90-99), after them the lobe choice;
+// returns the dimension of the Russian-roulette draw.
+template <int MAT>
+__device__ inline unsigned bsdf_draws(const SampleKey& k, unsigned s0, float& u1, float& u2, float& u3)
+{
+    if (MAT == kMatGgx) {
+        u3 = sample_u(k, s0);
+        u1 = sample_u(k, s0 + 1);
+        u2 = sample_u(k, s0 + 2);
+        return s0 + 4;
+    }
+    u1 = sample_u(k, s0);
+    u2 = sample_u(k, s0 + 1);
+    u3 = 0.0f;
+    return s0 + 3;
+}
+
 // Index-of-refraction stack of one single-wavelength path (IORStack.h):
 // objects above the ambient air; the index of an entry is its dielectric's
 // index at the path's wavelength.  At most 3 nested objects; a deeper push is
@@ -576,7 +626,7 @@ __device__ inline bool bump_bounces(unsigned& st, int type, const ReqInfo& R)
     return true;
 }
 
-constexpr unsigned kLaneDims = 1u << 20;  // dimension offset of single-wavelength lane w: (w + 1) << 20
+constexpr unsigned kLaneDims = 1u << 20;  // key offset of lane w's independent draws (free flight): (w + 1) << 20
 constexpr unsigned kModeNm = 1u;          // flags bit: the path continues as single-wavelength lanes
 
 constexpr unsigned kShadowInScatter = 0x80000000u;  // shadow record of an in-scattering NEE sample
@@ -737,7 +787,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
         Surf h;
         DevObject ob{};
         DevMaterial mat{};
-        unsigned path = 0;
+        SampleKey key{0u, 0u, 0u};
         f3 o = mk3(0, 0, 0);
         float lam[kHwss] = {0, 0, 0, 0};
         float pdf_in = 0.0f, imp = 1.0f;
@@ -745,7 +795,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
         int mobj = -1;  // object whose medium holds the hit point (single-wavelength lanes)
         IorStack ist{0, {0, 0, 0}};
         if (valid) {
-            path = S.path_id[slot];
+            key = sample_key(R, S.path_id[slot]);
             o = mk3(S.ox[li], S.oy[li], S.oz[li]);
             const f3 d = mk3(S.dx[li], S.dy[li], S.dz[li]);
             wavelengths(S.lambda0[slot], R.lambda_begin, R.lambda_end, lam);
@@ -807,14 +857,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             float v = 0.0f, stm = 0.0f;
             f3 sorg = mk3(0, 0, 0), to = mk3(0, 0, 0);
             if (surface && sc.n_lums > 0 && (!nm || (unsigned)w == lane)) {
-                const unsigned dim = nm ? base_dim + kLaneDims * (lane + 1u) : base_dim + 4u * (unsigned)w;
-                const float usel = rng_u(R.seed, path, dim);
+                const unsigned dim = nm ? base_dim : base_dim + kNeeDims * (unsigned)w;
+                const float usel = sample_u(key, dim);
                 int li_ = 0;
                 while (li_ < sc.n_lums - 1 && !(usel < sc.lums[li_].cdf_select)) ++li_;
                 if (li_ != ob.luminary) {  // never the shading object itself (:2061)
                     const DevLuminary L = sc.lums[li_];
-                    const float ux = rng_u(R.seed, path, dim + 1), uy = rng_u(R.seed, path, dim + 2),
-                                uz = rng_u(R.seed, path, dim + 3);
+                    const float ux = sample_u(key, dim + 1), uy = sample_u(key, dim + 2), uz = sample_u(key, dim + 3);
                     // UniformRandomPoint: first CDF entry >= uz, then PointOnTriangle
                     // (TriangleMeshGeometryIndexed.cpp:207-218, GeometricUtilities.cpp:543-560).
                     int k = 0;
@@ -877,9 +926,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             const unsigned cw = nm ? lane : 0u;
             if (surface && !nm) {
                 // Hero ScatterNM, companions at the hero's direction (:4620-4718).
-                const ScatterOut hs = bsdf_scatter<MAT>(sc, mat, h, lam[0], 1.0f, rng_u(R.seed, path, base_dim + 16),
-                                                        rng_u(R.seed, path, base_dim + 17),
-                                                        rng_u(R.seed, path, base_dim + 18));
+                float u1, u2, u3;
+                const unsigned rr_dim = bsdf_draws<MAT>(key, base_dim + kHwss * kNeeDims, u1, u2, u3);
+                const ScatterOut hs = bsdf_scatter<MAT>(sc, mat, h, lam[0], 1.0f, u1, u2, u3);
                 if (hs.ok) {
                     float thr[kHwss], sc4[kHwss];
                     for (int w = 0; w < kHwss; ++w) thr[w] = S.thr[w * cap + slot];
@@ -898,7 +947,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                     bool go = mcurr > kNearZeroF;
                     if (go && depth >= R.rr_min_depth) {
                         const float p = fminf(1.0f, mcurr / fmaxf(mprev, R.rr_threshold));
-                        if (rng_u(R.seed, path, base_dim + 19) >= p) {
+                        if (sample_u(key, rr_dim) >= p) {
                             go = false;
                         } else if (p > 0.0f && p < 1.0f) {
                             const float inv = 1.0f / p;
@@ -928,14 +977,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 }
             } else if (surface && nm) {
                 // IntegrateFromHitNM PART 3 at the lane's wavelength (:2905-3290).
-                const unsigned ld = base_dim + kLaneDims * (lane + 1u);
-                const ScatterOut so = bsdf_scatter<MAT>(sc, mat, h, lam[lane], amb, rng_u(R.seed, path, ld + 16),
-                                                        rng_u(R.seed, path, ld + 17), rng_u(R.seed, path, ld + 18));
+                float u1, u2, u3;
+                const unsigned rr_dim = bsdf_draws<MAT>(key, base_dim + kNeeDims, u1, u2, u3);
+                const ScatterOut so = bsdf_scatter<MAT>(sc, mat, h, lam[lane], amb, u1, u2, u3);
                 float scat = so.kray;
                 bool go = so.ok && fabsf(scat) > kNearZeroF;
                 if (go && depth >= R.rr_min_depth) {
                     const float p = fminf(1.0f, imp * fabsf(scat) / fmaxf(imp, R.rr_threshold));
-                    if (rng_u(R.seed, path, ld + 19) >= p) go = false;
+                    if (sample_u(key, rr_dim) >= p) go = false;
                     else if (p > 0.0f && p < 1.0f) scat /= p;
                 }
                 unsigned nb = bst;
@@ -965,8 +1014,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 IorStack nst = ist;
                 unsigned nb = bst;
                 if (valid && (!nm || (unsigned)w == lane)) {
-                    const unsigned ld = base_dim + kLaneDims * ((unsigned)w + 1u);
-                    if (dielectric_scatter(sc, mat, h, o, lam[w], ist, rng_u(R.seed, path, ld + 18), nd, wgt, type,
+                    if (dielectric_scatter(sc, mat, h, o, lam[w], ist, sample_u(key, base_dim + 2u), nd, wgt, type,
                                            nst) &&
                         bump_bounces(nb, type, R)) {
                         // A shared ray's wavelength starts its own loop at throughput 1
@@ -1068,12 +1116,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restric
         bool alive = false;
         f3 p = mk3(0, 0, 0), d = mk3(0, 0, 0);
         float nm = 0.0f, thr = 0.0f, imp = 1.0f;
-        unsigned path = 0, bst = 0, vbn = 0;
+        unsigned bst = 0, vbn = 0;
+        SampleKey key{0u, 0u, 0u};
         int mobj = -1;
         IorStack ist{0, {0, 0, 0}};
         DevMedium M{};
         if (valid) {
-            path = S.path_id[slot];
+            key = sample_key(R, S.path_id[slot]);
             d = mk3(S.dx[li], S.dy[li], S.dz[li]);
             const float tm = S.hit_t[li];
             p = mk3(S.ox[li], S.oy[li], S.oz[li]) + d * tm;
@@ -1096,19 +1145,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restric
                 alive = true;
             }
         }
-        const unsigned ld = base_dim + kLaneDims * (lane + 1u);
-
-        // In-scattering NEE at the lane's wavelength.
+        // In-scattering NEE at the lane's wavelength (sampler dimensions as at a
+        // single-wavelength surface vertex: the light sample, then the phase
+        // function's two, then Russian roulette; PathTracingIntegrator.cpp:1760-1870).
         bool has = false;
         float v = 0.0f, dist = 0.0f;
         f3 to = mk3(0, 0, 0);
         if (alive && sc.n_lums > 0) {
-            const float usel = rng_u(R.seed, path, ld);
+            const float usel = sample_u(key, base_dim);
             int l = 0;
             while (l < sc.n_lums - 1 && !(usel < sc.lums[l].cdf_select)) ++l;
             const DevLuminary L = sc.lums[l];
-            const float ux = rng_u(R.seed, path, ld + 1), uy = rng_u(R.seed, path, ld + 2),
-                        uz = rng_u(R.seed, path, ld + 3);
+            const float ux = sample_u(key, base_dim + 1), uy = sample_u(key, base_dim + 2),
+                        uz = sample_u(key, base_dim + 3);
             int k = 0;
             while (k < L.tri_count - 1 && sc.lum_cdf[L.tri_first + k] < uz) ++k;
             const float4* lp = reinterpret_cast<const float4*>(sc.tris + sc.lum_tris[L.tri_first + k]);
@@ -1153,14 +1202,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restric
         f3 nd = mk3(0, 0, 0);
         float pp = 0.0f;
         if (alive) {
-            nd = hg_sample(d, M.g, rng_u(R.seed, path, ld + 16), rng_u(R.seed, path, ld + 17));
+            nd = hg_sample(d, M.g, sample_u(key, base_dim + kNeeDims), sample_u(key, base_dim + kNeeDims + 1));
             pp = hg_eval(dot3(d, nd), M.g);
             if (pp > kNearZeroF) {
                 float t = thr * (hg_eval(dot3(d, nd), M.g) / pp);
                 bool go = true;
                 if (depth + (int)vbn >= R.rr_min_depth) {
                     const float q = fminf(1.0f, fabsf(t) / fmaxf(imp, R.rr_threshold));
-                    if (rng_u(R.seed, path, ld + 19) >= q) go = false;
+                    if (sample_u(key, base_dim + kNeeDims + 2) >= q) go = false;
                     else if (q > 0.0f && q < 1.0f) t /= q;
                 }
                 if (go) {
@@ -1574,6 +1623,8 @@ rise_hip_status validate(const rise_hip_render_request& q, size_t k)
         return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "negative bounce cap or RR depth");
     if (!(D.camera.fov_deg > 0 && D.camera.fov_deg < 180))
         return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "bad field of view");
+    if (D.sampler < RISE_HIP_SAMPLER_ZSOBOL || D.sampler > RISE_HIP_SAMPLER_ZSOBOL_UNSHUFFLED)
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, who + "unknown sampler");
     return RISE_HIP_OK;
 }
 
@@ -1598,6 +1649,12 @@ rise_hip_status make_info(const rise_hip_render_request& q, ReqInfo& I)
     I.xyz_scale = (float)(ynorm / kHwss);
     I.seed = D.seed;
     I.sky_background = D.sky_background;
+    unsigned l2 = 0;
+    while (l2 < 32u && (1ull << l2) < (unsigned long long)std::max(1, D.spp_total)) ++l2;
+    I.sob_zorder = D.sampler != RISE_HIP_SAMPLER_SOBOL;
+    I.sob_l2 = l2;
+    I.sob_mask = D.sampler == RISE_HIP_SAMPLER_ZSOBOL_UNSHUFFLED ? 0u : (l2 >= 32u ? 0xffffffffu : (1u << l2) - 1u);
+    I.seed32 = (unsigned)D.seed ^ (unsigned)(D.seed >> 32);
     const double px = D.camera.position[0], py = D.camera.position[1], pz = D.camera.position[2];
     double f[3] = {D.camera.look_at[0] - px, D.camera.look_at[1] - py, D.camera.look_at[2] - pz};
     const double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);

@@ -721,6 +721,11 @@ struct Builder {
         F.desc.max_glossy_bounce = cap("max_glossy_bounce");
         F.desc.max_transmission_bounce = cap("max_transmission_bounce");
         F.desc.sky_background = get_bool(c, "radiance_background", true) ? 1 : 0;
+        // PixelFilterConfig (Utilities/PixelFilterConfig.h:80-97): the Sobol pixel
+        // sampler, with the Z-order (blue-noise) index unless blue_noise_sampler is off.
+        F.desc.sampler = get_bool(c, "blue_noise_sampler", true) ? RISE_HIP_SAMPLER_ZSOBOL : RISE_HIP_SAMPLER_SOBOL;
+        if (c.has("pixel_sampler") && lower(unquote(*c.get("pixel_sampler"))) != "sobol")
+            error_at(c, "only the sobol pixel_sampler is supported");
         const std::string pf = lower(unquote(c.get("pixel_filter") ? *c.get("pixel_filter") : "box"));
         if (pf != "box") error_at(c, "only the box pixel filter is supported");
         if (c.has("radiance_map")) error_at(c, "`radiance_map` is not supported (use hosek_wilkie_skylight)");

@@ -45,6 +45,14 @@ class Spd:
 
 LAMBERTIAN, GGX, DIELECTRIC = 0, 1, 2
 
+# Sampler of the bounce loops (rise_hip_render_desc.sampler): Z-order
+# Owen-scrambled Sobol, the reference's default (blue_noise_sampler TRUE), or
+# the per-pixel Owen-scrambled Sobol (blue_noise_sampler FALSE), both with the
+# sample index shuffled per dimension pair; or the Z-order sampler exactly as
+# the reference draws it (unshuffled: biased by about 1% on the spectral
+# furnace, see tests/test_oracle_pt.py).
+SAMPLER_ZSOBOL, SAMPLER_SOBOL, SAMPLER_ZSOBOL_UNSHUFFLED = 0, 1, 2
+
 
 @dataclass
 class Material:
@@ -125,6 +133,7 @@ class RenderSettings:
     rr_threshold: float = 0.05
     seed: int = 1
     sky_background: bool = True   # camera rays that leave the scene see the sky
+    sampler: int = 0              # SAMPLER_ZSOBOL (the default), SAMPLER_SOBOL or SAMPLER_ZSOBOL_UNSHUFFLED
 
 
 @dataclass
@@ -487,21 +496,23 @@ def heightfield(nx: int, ny: int, extent=(20.0, 10.0), amplitude: float = 0.2, s
     return np.ascontiguousarray(np.concatenate([t1, t2], 0))
 
 
-def terrain_spheres(nx: int = 2048, ny: int = 1024, subdiv: int = 4, seed: int = 7):
+def terrain_spheres(nx: int = 2048, ny: int = 1024, subdiv: int = 4, seed: int = 7, n_side: int = 8):
     """The terrain-and-spheres workload: a 2048 x 1024 heightfield of seeded
     6-octave value noise (amplitude 0.2; 4,188,162 triangles) with 64
     icospheres (subdivision 4, 5,120 triangles each) resting on it, alternately
     GGX conductors (roughness U[0.05, 0.5]) and smooth glass (index 1.5),
-    under the spectral sky; 1920 x 1080, 64 spp, depth 8."""
+    under the spectral sky; 1920 x 1080, 64 spp, depth 8.  Smaller grids,
+    subdivisions and n_side x n_side spheres give reduced scenes from the
+    same generator."""
     rng = np.random.default_rng(seed)
     s = Scene()
     s.spds = [Spd.rgb_bands(0.25, 0.3, 0.18), Spd.flat(1.0), Spd(380.0, 720.0, np.array([1.3, 0.6, 0.2], np.float32)),
               Spd(380.0, 720.0, np.array([1.9, 2.4, 3.2], np.float32)), Spd.flat(1.5)]
     s.materials = [Material(0), Material(kind=DIELECTRIC, ior=4)]
     s.objects = [Object(heightfield(nx, ny, seed=seed), 0)]
-    for i in range(8):
-        for j in range(8):
-            c = (-8.0 + 16.0 * (i + 0.5) / 8, 0.55, -4.0 + 8.0 * (j + 0.5) / 8)
+    for i in range(n_side):
+        for j in range(n_side):
+            c = (-8.0 + 16.0 * (i + 0.5) / n_side, 0.55, -4.0 + 8.0 * (j + 0.5) / n_side)
             if (i + j) % 2 == 0:
                 s.materials.append(Material(kind=GGX, reflectance=-1, specular=1, ior=2, extinction=3,
                                             alpha=float(rng.uniform(0.05, 0.5))))

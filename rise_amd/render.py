@@ -96,7 +96,7 @@ class RenderDesc(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("camera", CameraDesc),
         ("sky_background", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sampler", ctypes.c_int32),
     ]
 
 
@@ -206,6 +206,7 @@ def render_desc(settings: RenderSettings, scene: Scene, spp: int | None = None, 
     d.camera.up[:] = c.up
     d.camera.fov_deg = c.fov_deg
     d.sky_background = 1 if settings.sky_background else 0
+    d.sampler = int(settings.sampler)
     return d
 
 

@@ -136,7 +136,7 @@ class SceneFile:
                               max_glossy_bounce=d.max_glossy_bounce,
                               max_transmission_bounce=d.max_transmission_bounce, rr_min_depth=d.rr_min_depth,
                               rr_threshold=_f(d.rr_threshold), seed=int(d.seed),
-                              sky_background=bool(d.sky_background))
+                              sky_background=bool(d.sky_background), sampler=int(d.sampler))
 
     def outputs(self) -> list:
         """[(OutputDesc, file name)] of the file's file_rasterizeroutput chunks."""

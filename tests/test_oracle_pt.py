@@ -3,12 +3,18 @@
 * Colour: the CIE Y integral over 380-780 nm is the value the reference
   states for its table (~106.86, PixelBasedSpectralIntegratingRasterizer.cpp:58-62),
   XYZFromNM reproduces the table at its nodes and interpolates linearly.
-* Random numbers: uniform in [0, 1) with 24 bits, distinct per path and dim.
+* Random numbers: uniform in [0, 1) with 24 bits, distinct per path and dim;
+  the Owen-scrambled Sobol sampler's pixel blocks.
 * White furnace (closed form L = Le / (1 - albedo)) with MIS, NEE, Russian
   roulette and the Y normalization all active.
 * Directly viewed emitter: Y = Le (no MIS at the camera vertex).
 * Reciprocity of the image under a seed change (statistical agreement of
   two independent renders).
+* Finding: the reference's padded Sobol sampler, unshuffled, is biased on
+  the spectral furnace; the shuffled default is not.
+Standard errors of image means are taken from the spread of the per-pixel
+means (each pixel has its own scramble, so pixels are independent), since
+the per-sample variance misstates the error of quasi-random sampling.
 """
 import math
 
@@ -45,9 +51,9 @@ def test_rng_uniform_24_bit():
 
 
 def _y_stats(sm, sq, n):
-    y = sm[..., 1].sum() / n
-    var = sq[..., 1].sum() / n - y * y
-    return y, math.sqrt(max(var, 0.0) / n)
+    """Image mean of Y and its standard error from the per-pixel means."""
+    pm = sm[..., 1] / (n / sm[..., 1].size)
+    return float(pm.mean()), float(pm.std(ddof=1) / math.sqrt(pm.size))
 
 
 def test_white_furnace_closed_form():
@@ -81,6 +87,47 @@ def test_spectral_furnace_closed_form():
     y, se = _y_stats(sm, sq, r.width * r.height * r.spp)
     assert abs(y - want) < 4 * se + 1e-3 * want, (y, want, se)
     assert abs(bug - want) > 12 * se
+
+
+def test_unshuffled_sobol_is_biased_on_the_spectral_furnace():
+    """The reference's padded Sobol draws every dimension pair at the same
+    index (SobolSequence.h:180-200), so dimensions 0, 2, 4, ... are hash
+    scrambles of one van der Corput point.  Over eight renders the unshuffled
+    sampler misses the spectral furnace's closed form by about -1% (many
+    standard errors of the eight-render mean); the default sampler, which
+    shuffles the index per dimension pair, does not."""
+    rho, le = pt_closed_forms.spectral_furnace_spectra()
+    want = pt_closed_forms.furnace_y(rho, le)
+    out = {}
+    for sampler in (pt_scene.SAMPLER_ZSOBOL_UNSHUFFLED, pt_scene.SAMPLER_ZSOBOL):
+        ys = []
+        for seed in range(1, 9):
+            s, r = pt_scene.furnace_box(albedo=rho, radiance=le, max_bounce=96)
+            r.spp, r.seed, r.sampler = 64, seed, sampler
+            sm, sq, _ = oracle.render(s, r)
+            ys.append(_y_stats(sm, sq, r.width * r.height * r.spp)[0])
+        ys = np.array(ys)
+        out[sampler] = (ys.mean() / want - 1.0, (ys.mean() - want) / (ys.std(ddof=1) / math.sqrt(len(ys))))
+    bias, z = out[pt_scene.SAMPLER_ZSOBOL_UNSHUFFLED]
+    assert bias < -0.005 and z < -5, out
+    bias, z = out[pt_scene.SAMPLER_ZSOBOL]
+    assert abs(bias) < 0.003 and abs(z) < 4, out
+
+
+def test_sobol_sampler_uses_each_pixels_block():
+    """Z-order sampler: pixel (x, y) takes indices Morton(x, y) * 2^l2 + [0, spp);
+    with the shuffle the samples of a pixel are a permutation of that block, so
+    over all spp = 2^l2 samples the pixel's Sobol points are exactly those of the
+    block (its (0, 2)-net in dimensions 0 and 1)."""
+    # Dimensions 0 and 1 of the block's points form a (0, m, 2)-net: one point per
+    # elementary interval of area 1/2^m, here checked on the 2^m x 1 and 1 x 2^m grids.
+    m = 6
+    seed = oracle.hash_combine(37, 0)
+    x = np.array([oracle.sobol_u32(i, 0, seed) for i in range(5 << m, 6 << m)], np.uint64)
+    y = np.array([oracle.sobol_u32(i, 1, seed) for i in range(5 << m, 6 << m)], np.uint64)
+    assert len(np.unique(x >> (32 - m))) == 1 << m
+    assert len(np.unique(y >> (32 - m))) == 1 << m
+    assert len(np.unique(((x >> (32 - m // 2)) << (m // 2)) | (y >> (32 - m // 2)))) == 1 << m
 
 
 def test_emitter_seen_directly():

@@ -10,7 +10,9 @@ reference's own code, compiled from its sources into oracle/_ref
   and pdf, the hemispherical Fresnel average and F_ms (MicrofacetEnergyLUT.h);
 * Russian roulette and the power heuristic (PathTransportUtilities.h);
 * the HWSS wavelength sampler (SampledWavelengths.h);
-* OrthonormalBasis3D::CreateFromW.
+* OrthonormalBasis3D::CreateFromW;
+* the Owen-scrambled Sobol sample, HashCombine and the Morton code of the
+  sampler (SobolSequence.h, MortonCode.h), bit for bit.
 
 Tolerance: 1e-12 relative (both sides run in IEEE double without
 contraction); Russian roulette, the power heuristic and the wavelengths are
@@ -51,6 +53,9 @@ _SIG = {
     "rise_ref_power_heuristic": ([_D, _D], _D),
     "rise_ref_sample_equidistant": ([_D, _D, _D, _DP], None),
     "rise_ref_onb_from_w": ([_DP, _DP], None),
+    "rise_ref_sobol_sample": ([ctypes.c_uint32] * 3, _D),
+    "rise_ref_hash_combine": ([ctypes.c_uint32] * 2, ctypes.c_uint32),
+    "rise_ref_morton2d": ([ctypes.c_uint32] * 2, ctypes.c_uint32),
 }
 _ORACLE_SIG = {
     "rise_oracle_russian_roulette": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _DP], None),
@@ -213,3 +218,20 @@ def test_russian_roulette_power_heuristic_wavelengths_match_reference(ref, orc):
             ref.rise_ref_sample_equidistant(u, a0, b0, x)
             orc.rise_oracle_sample_wavelengths(u, a0, b0, y)
             assert x[:] == y[:]
+
+
+def test_sobol_sampler_matches_reference(ref):
+    g = np.random.Generator(np.random.PCG64(16))
+    idx = np.concatenate([np.arange(70), g.integers(0, 2 ** 32, 300)])
+    for i in idx:
+        for dim in (0, 1, 2, 3, 16, 17, 31, 32 * 16 + 5, 32 * 40 + 30):
+            seed = int(g.integers(0, 2 ** 32))
+            assert oracle.sobol_u32(int(i), dim, seed) / 2 ** 32 == ref.rise_ref_sobol_sample(int(i), dim, seed)
+    for a, b in g.integers(0, 2 ** 32, (200, 2)):
+        assert oracle.hash_combine(int(a), int(b)) == ref.rise_ref_hash_combine(int(a), int(b))
+    for x, y in g.integers(0, 2 ** 16, (200, 2)):
+        want = ref.rise_ref_morton2d(int(x), int(y))
+        got = 0
+        for k in range(16):
+            got |= ((int(x) >> k) & 1) << (2 * k) | ((int(y) >> k) & 1) << (2 * k + 1)
+        assert got == want
