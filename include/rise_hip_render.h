@@ -110,19 +110,43 @@ rise_hip_status rise_hip_pt_scene_create_with_media(const rise_hip_object_desc* 
 
 rise_hip_status rise_hip_pt_scene_release(rise_hip_pt_scene* scene);
 
-/* The spectral sky radiance map of the scene-file chunk `hosek_wilkie_skylight`
+/* The spectral sky of the scene-file chunk `hosek_wilkie_skylight`
  * (Job::AddHosekWilkieSkylight, Job.cpp:2717-2750; the reference's model is
- * Preetham 1999, Utilities/HosekWilkieSkyModel.cpp) without its matched
- * directional sun light.  Rays leaving the scene see it; renders and batches
- * created afterwards use it (NULL removes it). */
+ * Preetham 1999, Utilities/HosekWilkieSkyModel.cpp): a radiance map seen by
+ * rays leaving the scene and importance-sampled by next-event estimation
+ * (EnvironmentSampler, Rendering/EnvironmentSampler.cpp; LightSampler.cpp:
+ * 2278-2358), with BSDF-sampled escapes weighted against that sample by the
+ * power heuristic; and, when create_sun is set, its matched directional light
+ * `__hw_sun__` toward the sun with colour (1, 1, 1) and power
+ * sun_intensity_scale.  Renders and batches created afterwards use it (NULL
+ * removes both). */
 typedef struct rise_hip_sky_desc {
     float solar_elevation_deg;  /* solar_elevation (45) */
     float solar_azimuth_deg;    /* solar_azimuth (0) */
     float turbidity;            /* turbidity (3) */
     float sky_scale;            /* sky_intensity_scale (1) */
+    int32_t create_sun;         /* create_sun (the chunk's default is TRUE) */
+    float sun_intensity_scale;  /* sun_intensity_scale (3.14) */
 } rise_hip_sky_desc;
 
 rise_hip_status rise_hip_pt_scene_set_sky(rise_hip_pt_scene* scene, const rise_hip_sky_desc* sky);
+
+/* Directional lights (the scene-file chunk `directional_light`,
+ * Lights/DirectionalLight.cpp): light arriving from `direction` (toward the
+ * light; normalized here) with RGB `color` times `power`.  Next-event
+ * estimation evaluates each one deterministically at every surface vertex
+ * and wavelength as ComputeDirectLightingNM does (LightSampler.cpp:1968-2000):
+ * Rec.709 luminance of the colour x power x BSDF x cos, with a shadow ray to
+ * infinity; no random numbers, no MIS.  Replaces the scene's earlier list; a
+ * sky's matched sun comes on top.  At most 3 lights with a sun, 4 without. */
+typedef struct rise_hip_directional_light_desc {
+    float direction[3];
+    float color[3];
+    float power;
+} rise_hip_directional_light_desc;
+
+rise_hip_status rise_hip_pt_scene_set_directional_lights(rise_hip_pt_scene* scene,
+                                                         const rise_hip_directional_light_desc* lights, size_t n);
 
 /* Pinhole camera: vertical field of view, square pixels, pixel row 0 at the top. */
 typedef struct rise_hip_camera_desc {

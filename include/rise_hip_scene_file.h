@@ -53,7 +53,7 @@ typedef struct rise_hip_scene_file_info {
     int32_t has_sky;
     int32_t n_skipped_chunks;  /* chunks skipped as irrelevant to the image */
     int32_t n_outputs;         /* file_rasterizeroutput chunks */
-    int32_t reserved;
+    int32_t n_directional_lights;  /* directional_light chunks (a sky's sun not counted) */
 } rise_hip_scene_file_info;
 
 /* Reads and checks a scene file (relative mesh and spectrum files resolve
@@ -71,6 +71,11 @@ rise_hip_status rise_hip_scene_file_get_tables(const rise_hip_scene_file* file,
                                                const rise_hip_spd_desc** spds, const rise_hip_medium_desc** media,
                                                const rise_hip_sky_desc** sky);
 
+/* The file's directional_light chunks (n_directional_lights of them, owned by
+ * `file`), as rise_hip_pt_scene_set_directional_lights takes them. */
+rise_hip_status rise_hip_scene_file_get_directional_lights(const rise_hip_scene_file* file,
+                                                           const rise_hip_directional_light_desc** lights);
+
 /* The render the file's camera, film and rasterizer chunks describe. */
 rise_hip_status rise_hip_scene_file_get_render_desc(const rise_hip_scene_file* file, rise_hip_render_desc* out_desc);
 
@@ -82,7 +87,8 @@ rise_hip_status rise_hip_scene_file_get_output(const rise_hip_scene_file* file, 
                                                rise_hip_output_desc* desc, const char** pattern);
 
 /* Builds the device scene (rise_hip_pt_scene_create_with_media, then
- * rise_hip_pt_scene_set_sky when the file has a sky). */
+ * rise_hip_pt_scene_set_directional_lights and rise_hip_pt_scene_set_sky when
+ * the file has them). */
 rise_hip_status rise_hip_scene_file_upload(const rise_hip_scene_file* file, rise_hip_pt_scene** out_scene);
 
 rise_hip_status rise_hip_scene_file_release(rise_hip_scene_file* file);

@@ -229,6 +229,8 @@ class _RoScene(ctypes.Structure):
         ("sky_turbidity", ctypes.c_double), ("sky_scale", ctypes.c_double),
         ("object_medium", ctypes.c_void_p), ("object_casts_shadows", ctypes.c_void_p),
         ("media", ctypes.c_void_p), ("n_media", ctypes.c_int32), ("sampler", ctypes.c_int32),
+        ("sun_enabled", ctypes.c_int32), ("sun_power", ctypes.c_double), ("n_dir_lights", ctypes.c_int32),
+        ("dir_lights", ctypes.c_void_p),
     ]
 
 
@@ -281,6 +283,14 @@ def _ro_scene(scene, settings, spp_total, keep):
         rs.sky_azimuth_deg = float(np.float32(scene.sky.azimuth_deg))
         rs.sky_turbidity = float(np.float32(scene.sky.turbidity))
         rs.sky_scale = float(np.float32(scene.sky.scale))
+        rs.sun_enabled = 1 if getattr(scene.sky, "create_sun", False) else 0
+        rs.sun_power = float(np.float32(getattr(scene.sky, "sun_power", 0.0)))
+    lights = getattr(scene, "lights", [])
+    if lights:
+        dl = np.array([[*l.direction, *l.color, l.power] for l in lights], np.float32).astype(np.float64)
+        keep.append(dl)
+        rs.n_dir_lights = len(lights)
+        rs.dir_lights = dl.ctypes.data
     rs.sky_background = 1 if settings.sky_background else 0
     rs.sampler = int(getattr(settings, "sampler", 0))
     if scene.media:

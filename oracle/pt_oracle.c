@@ -453,6 +453,191 @@ double rise_oracle_sky_radiance(const ro_scene* scene, const double dir[3], doub
     return sky_radiance(&m, mk(dir[0], dir[1], dir[2]), nm);
 }
 
+/* ---- environment light sample (Rendering/EnvironmentSampler.cpp) ---------- */
+/* The importance table RayCaster builds over the sky map's painter
+ * (RayCaster.cpp:330-360, 64 x 64 cells; HosekWilkieSpectralRadianceMap.cpp
+ * HWAdapterPainter): cells laid out in the light-probe mapping, each weighted
+ * by the maximum Rec.709 component of the sky at the cell centre read as
+ * equirectangular coordinates, times the probe's sin(theta) / theta. */
+
+#define ENV_RES 64
+
+typedef struct {
+    int on;
+    double cond[ENV_RES][ENV_RES + 1];
+    double marg[ENV_RES + 1];
+} env_table;
+
+/* ColorUtils::LineIntersection2D / IsPointOnLineSegment / ClipAgainstGamutEdge /
+ * MoveXYZIntoRec709RGBGamut and the XYZ -> Rec.709 matrix (ColorUtils.cpp:150-222,
+ * 351-363; Color.cpp:214-220). */
+typedef struct { double x, y; } p2;
+static int line_isect2(p2 a0, p2 a1, p2 b0, p2 b1, p2* out)
+{
+    const double den = (b1.y - b0.y) * (a1.x - a0.x) - (b1.x - b0.x) * (a1.y - a0.y);
+    if (den > -NZ && den < NZ) return 0;
+    const double ua = ((b1.x - b0.x) * (a0.y - b0.y) - (b1.y - b0.y) * (a0.x - b0.x)) / den;
+    out->x = a0.x + ua * (a1.x - a0.x);
+    out->y = a0.y + ua * (a1.y - a0.y);
+    return 1;
+}
+static int on_seg2(p2 s0, p2 e, p2 q)
+{
+    const double v = (q.x - s0.x) / (e.x - s0.x);
+    return v >= NZ && v <= 1.0 - NZ;
+}
+static int clip_edge2(p2 c, p2 a, p2 b, p2 w, p2* out)
+{
+    if (!line_isect2(w, c, a, b, out)) return 0;
+    if (!on_seg2(a, b, *out)) return 0;
+    if (on_seg2(w, *out, c)) {
+        *out = c;
+        return 1;
+    }
+    return on_seg2(w, c, *out);
+}
+static void xyz_to_rec709(double X, double Y, double Z, double rgb[3])
+{
+    const p2 red = {0.640, 0.330}, green = {0.300, 0.600}, blue = {0.150, 0.060}, white = {0.3127, 0.3290};
+    const double ovs = 1.0 / (X + Y + Z);
+    const p2 c = {X * ovs, Y * ovs};
+    p2 hit = {0.0, 0.0};
+    if (clip_edge2(c, red, green, white, &hit) || clip_edge2(c, red, blue, white, &hit) ||
+        clip_edge2(c, blue, green, white, &hit)) {
+        const double prod = (1.0 / hit.y) * Y;
+        X = prod * hit.x;
+        Z = prod * (1 - hit.x - hit.y);
+    }
+    rgb[0] = 3.240479 * X + -1.537150 * Y + -0.498535 * Z;
+    rgb[1] = -0.969256 * X + 1.875992 * Y + 0.041556 * Z;
+    rgb[2] = 0.055648 * X + -0.204043 * Y + 1.057311 * Z;
+}
+
+/* HosekWilkieSkyModel::IntegrateRGB (HosekWilkieSkyModel.cpp:282-301). */
+static void sky_rgb(const sky_model* m, v3 d, double rgb[3])
+{
+    rgb[0] = rgb[1] = rgb[2] = 0.0;
+    if (d.y <= 0) return;
+    const double ct = d.y > 0 ? d.y : 0;
+    double ds = d.x * m->sun[0] + d.y * m->sun[1] + d.z * m->sun[2];
+    if (ds < -1) ds = -1;
+    if (ds > 1) ds = 1;
+    const double g = acos(ds);
+    const double Y = m->yz * perez_f(&m->pY, ct, g) / perez_f(&m->pY, 1.0, m->theta_s);
+    const double x = m->xz * perez_f(&m->px, ct, g) / perez_f(&m->px, 1.0, m->theta_s);
+    const double y = m->yzc * perez_f(&m->py, ct, g) / perez_f(&m->py, 1.0, m->theta_s);
+    if (Y < 0 || y < 1e-6) return;
+    xyz_to_rec709((x / y) * Y, Y, ((1.0 - x - y) / y) * Y, rgb);
+}
+
+/* EnvironmentSampler::Build (:150-246). */
+static void env_build(const sky_model* m, env_table* t)
+{
+    memset(t, 0, sizeof(*t));
+    if (!m->on) return;
+    for (int v = 0; v < ENV_RES; ++v) {
+        for (int u = 0; u < ENV_RES; ++u) {
+            const double s = ((double)u + 0.5) / ENV_RES, tc = ((double)v + 0.5) / ENV_RES;
+            const double phi = s * TWO_PI, th = tc * M_PI;
+            double rgb[3];
+            sky_rgb(m, mk(sin(th) * sin(phi), cos(th), sin(th) * cos(phi)), rgb);
+            double lum = rgb[0];
+            if (rgb[1] > lum) lum = rgb[1];
+            if (rgb[2] > lum) lum = rgb[2];
+            const double dx = s - 0.5, dy = -(tc - 0.5), r = sqrt(dx * dx + dy * dy), pt = r * TWO_PI;
+            double w = pt < 1e-6 ? 1.0 : sin(pt) / pt;
+            if (r > 0.5) w = 0.0;
+            t->cond[v][u + 1] = t->cond[v][u] + lum * w;
+        }
+    }
+    for (int v = 0; v < ENV_RES; ++v) t->marg[v + 1] = t->marg[v] + t->cond[v][ENV_RES];
+    const double total = t->marg[ENV_RES];
+    if (!(total > 0)) return;
+    for (int v = 0; v < ENV_RES; ++v) {
+        const double row = t->cond[v][ENV_RES];
+        if (row > 0)
+            for (int u = 0; u <= ENV_RES; ++u) t->cond[v][u] /= row;
+        t->cond[v][ENV_RES] = 1.0;
+    }
+    for (int v = 0; v <= ENV_RES; ++v) t->marg[v] /= total;
+    t->marg[ENV_RES] = 1.0;
+    t->on = 1;
+}
+
+/* EnvironmentSampler::SampleCDF (:118-148). */
+static unsigned env_search(const double* cdf, unsigned n, double xi, double* frac)
+{
+    unsigned lo = 0, hi = n;
+    while (lo + 1 < hi) {
+        const unsigned mid = (lo + hi) / 2;
+        if (cdf[mid] <= xi) lo = mid;
+        else hi = mid;
+    }
+    const double w = cdf[lo + 1] - cdf[lo];
+    *frac = w > 0 ? (xi - cdf[lo]) / w : 0.0;
+    return lo;
+}
+
+static double env_solid_angle(double pdf_st, double s, double tc)
+{
+    const double dx = s - 0.5, dy = -(tc - 0.5), r = sqrt(dx * dx + dy * dy), th = r * TWO_PI;
+    if (th < 1e-6 || r > 0.5) return pdf_st / (4.0 * M_PI);
+    const double st = sin(th);
+    if (st < 1e-10) return 0.0;
+    return pdf_st * th / ((4.0 * M_PI) * M_PI * st);
+}
+
+/* EnvironmentSampler::Sample (:248-312), identity map transform. */
+static v3 env_sample(const env_table* t, double u1, double u2, double* pdf)
+{
+    double fv, fu;
+    const unsigned v = env_search(t->marg, ENV_RES, u2, &fv);
+    const unsigned u = env_search(t->cond[v], ENV_RES, u1, &fu);
+    const double s = ((double)u + fu) / ENV_RES, tc = ((double)v + fv) / ENV_RES;
+    const double dx = s - 0.5, dy = -(tc - 0.5), rt = sqrt(dx * dx + dy * dy);
+    v3 d = mk(0, 0, -1);
+    if (!(rt < 1e-12)) {
+        const double th = rt * TWO_PI, sth = sin(th), cth = cos(th);
+        d = mk(dx * sth / rt, dy * sth / rt, -cth);
+    }
+    const double pm = (t->marg[v + 1] - t->marg[v]) * ENV_RES;
+    const double pc = (t->cond[v][u + 1] - t->cond[v][u]) * ENV_RES;
+    *pdf = env_solid_angle(pm * pc, s, tc);
+    return d;
+}
+
+/* EnvironmentSampler::Pdf (:314-355), identity map transform. */
+static double env_pdf(const env_table* t, v3 d)
+{
+    double s = 0.5, tc = 0.5;
+    const double den = sqrt(d.x * d.x + d.y * d.y);
+    if (!(den < 1e-12)) {
+        double c = -d.z;
+        if (c < -1) c = -1;
+        if (c > 1) c = 1;
+        const double r = 0.159154943 * acos(c) / den;
+        s = 0.5 + d.x * r;
+        tc = 0.5 - d.y * r;
+    }
+    s = s < 0 ? 0 : (s > 0.999999 ? 0.999999 : s);
+    tc = tc < 0 ? 0 : (tc > 0.999999 ? 0.999999 : tc);
+    unsigned u = (unsigned)(s * ENV_RES), v = (unsigned)(tc * ENV_RES);
+    if (u > ENV_RES - 1) u = ENV_RES - 1;
+    if (v > ENV_RES - 1) v = ENV_RES - 1;
+    const double pm = (t->marg[v + 1] - t->marg[v]) * ENV_RES;
+    const double pc = (t->cond[v][u + 1] - t->cond[v][u]) * ENV_RES;
+    return env_solid_angle(pm * pc, s, tc);
+}
+
+/* A directional light (Lights/DirectionalLight.cpp): unit direction toward it
+ * and Rec.709 luminance of its colour times its energy. */
+typedef struct {
+    v3 d;
+    double le;
+} dir_light;
+
+#define MAX_DIR_LIGHTS 4
+
 /* ---- scene preparation --------------------------------------------------- */
 
 typedef struct {
@@ -473,6 +658,9 @@ typedef struct {
     int32_t* lum_of_object; /* -1 if not a luminary */
     double y_norm;
     sky_model sky;
+    env_table* env;  /* environment light sample over the sky (NULL: none) */
+    dir_light dirs[MAX_DIR_LIGHTS];
+    int n_dirs;
     int has_media;   /* some object has an interior medium (bSceneHasObjectMedia) */
     /* work distribution */
     int32_t s0, s1;
@@ -558,6 +746,29 @@ static int prepare(prep* P, const ro_scene* s)
     const double ky = rise_oracle_cie_y_integral(s->lambda_begin, s->lambda_end);
     P->y_norm = ky > NZ ? (s->lambda_end - s->lambda_begin) / ky : 1.0;
     sky_init(&P->sky, s);
+    /* The environment sample's table over the sky (RayCaster.cpp:330-360; a
+     * black sky disables it), and the directional lights: the scene's, then the
+     * sky's matched sun (Job::AddHosekWilkieSkylight, Job.cpp:2734-2746). */
+    if (P->sky.on) {
+        P->env = (env_table*)malloc(sizeof(env_table));
+        if (!P->env) return -1;
+        env_build(&P->sky, P->env);
+        if (!P->env->on) {
+            free(P->env);
+            P->env = NULL;
+        }
+    }
+    for (int k = 0; k < s->n_dir_lights && P->n_dirs < MAX_DIR_LIGHTS; ++k) {
+        const double* l = s->dir_lights + 7 * k;
+        P->dirs[P->n_dirs].d = normalize(mk(l[0], l[1], l[2]));
+        P->dirs[P->n_dirs].le = (0.2126 * l[3] + 0.7152 * l[4] + 0.0722 * l[5]) * l[6];
+        ++P->n_dirs;
+    }
+    if (P->sky.on && s->sun_enabled && P->n_dirs < MAX_DIR_LIGHTS) {
+        P->dirs[P->n_dirs].d = normalize(mk(P->sky.sun[0], P->sky.sun[1], P->sky.sun[2]));
+        P->dirs[P->n_dirs].le = (0.2126 + 0.7152 + 0.0722) * s->sun_power;
+        ++P->n_dirs;
+    }
     /* LightSampler::Prepare's MediaScan (LightSampler.cpp:767-785). */
     if (s->object_medium)
         for (int o = 0; o < s->n_objects; ++o) P->has_media |= s->object_medium[o] >= 0;
@@ -573,6 +784,7 @@ static void release(prep* P)
     free(P->lums);
     free(P->lum_of_object);
     free(P->v);
+    free(P->env);
 }
 
 /* ---- ray queries (rt_oracle semantics, inlined for speed) ---------------- */
@@ -1091,10 +1303,15 @@ static double lum_pdf_select(const prep* P, int lum)
     return P->lums[lum].weight / P->lum_total;
 }
 
-/* EvaluateDirectLightingNM for area luminaries (LightSampler.cpp:1987-2269);
+/* Sampler dimensions of one light sample: the luminary pick and point when
+ * the scene has luminaries (an empty alias table draws nothing,
+ * LightSampler.cpp:2053-2056), then the environment sample's two. */
+static uint32_t light_dims(const prep* P) { return (P->n_lums > 0 ? NEE_DIMS : 0u) + (P->env ? 2u : 0u); }
+
+/* EvaluateDirectLightingNM's luminary sample (LightSampler.cpp:2002-2269);
  * mobj: the object whose medium holds the shading point (-1: none). */
-static double direct_nm(prep* P, const hit* h, double nm, double amb, int32_t mobj, const rstream* R, uint32_t dim0,
-                        uint64_t* n_shadow)
+static double lum_nm(prep* P, const hit* h, double nm, double amb, int32_t mobj, const rstream* R, uint32_t dim0,
+                     uint64_t* n_shadow)
 {
     if (P->n_lums == 0) return 0.0;
     const int self = P->lum_of_object[h->object];
@@ -1137,13 +1354,77 @@ static double direct_nm(prep* P, const hit* h, double nm, double amb, int32_t mo
     return contrib / pdf_alias;
 }
 
+/* The environment sample (LightSampler.cpp:2278-2358): a direction from the
+ * sky's importance table, the sky at nm, power heuristic against the BSDF's
+ * pdf, a shadow ray to infinity and the transmittance walk.  Counted and
+ * traced only when its value is positive. */
+static double env_nm(prep* P, const hit* h, double nm, double amb, int32_t mobj, const rstream* R, uint32_t d0,
+                     uint64_t* n_shadow)
+{
+    if (!P->env) return 0.0;
+    double pdf;
+    const v3 ed = env_sample(P->env, u_sampler(R, d0), u_sampler(R, d0 + 1), &pdf);
+    const double c = dot(ed, h->n);
+    if (!(c > 0 && pdf > 0)) return 0.0;
+    double contrib = sky_radiance(&P->sky, ed, nm) * bsdf_value(P, h, ed, nm, amb) * c / pdf;
+    const double pb = bsdf_pdf(P, h, ed, nm, amb);
+    if (pb > 0) contrib *= power_heuristic(pdf, pb);
+    if (!(contrib > 0)) return 0.0;
+    ++*n_shadow;
+    if (occluded(P, h->p, ed, RISE_ORACLE_INFINITY)) return 0.0;
+    return contrib * shadow_tr(P, h->p, ed, RISE_ORACLE_INFINITY, mobj);
+}
+
+/* The directional lights (LightSampler.cpp:1968-2000 ->
+ * DirectionalLight::ComputeDirectLightingNM, DirectionalLight.cpp:75-105):
+ * luminance x power x BSDF x cos, a shadow ray to infinity, no transmittance.
+ * `count`: count the shadow rays (a shared vertex traces one per light for its
+ * four wavelengths). */
+static double dirs_nm(prep* P, const hit* h, double nm, double amb, int count, uint64_t* n_shadow)
+{
+    double sum = 0.0;
+    for (int k = 0; k < P->n_dirs; ++k) {
+        const dir_light* L = &P->dirs[k];
+        const double c = dot(L->d, h->n);
+        if (!(c > 0)) continue;
+        const double v = L->le * bsdf_value(P, h, L->d, nm, amb) * c;
+        if (!(v > 0)) continue;
+        if (count) ++*n_shadow;
+        if (occluded(P, h->p, L->d, RISE_ORACLE_INFINITY)) continue;
+        sum += v;
+    }
+    return sum;
+}
+
+/* The weight of a BSDF-sampled ray that escapes to the sky, against the
+ * environment sample (PathTracingIntegrator.cpp:4259-4282): camera rays and
+ * rays leaving a delta lobe (pdf 0) keep full weight. */
+static double escape_mis(const prep* P, v3 dir, double bsdf_pdf)
+{
+    if (!P->env || !(bsdf_pdf > 0)) return 1.0;
+    const double pe = env_pdf(P->env, dir);
+    return pe > 0 ? power_heuristic(bsdf_pdf, pe) : 1.0;
+}
+
+/* EvaluateDirectLightingNM at a surface: the directional lights, one luminary
+ * sample, one environment sample (their sum; the HIP renderer keeps the
+ * three apart until the film). */
+static double direct_nm(prep* P, const hit* h, double nm, double amb, int32_t mobj, const rstream* R, uint32_t dim0,
+                        int count_dirs, uint64_t* n_shadow)
+{
+    const double lum = lum_nm(P, h, nm, amb, mobj, R, dim0, n_shadow);
+    const double env = env_nm(P, h, nm, amb, mobj, R, dim0 + (P->n_lums > 0 ? NEE_DIMS : 0u), n_shadow);
+    const double dirs = dirs_nm(P, h, nm, amb, count_dirs, n_shadow);
+    return lum + env + dirs;
+}
+
 /* MediumTransport::EvaluateInScatteringNM (MediumTransport.cpp:139-174) ->
  * EvaluateDirectLightingNM with isVolumeScatter: cosSurface = 1, the phase
  * function as BSDF and as MIS pdf, no shading object -- hence no self
  * exclusion and no occlusion test (LightSampler.cpp:2200-2211 tests only for
  * a shading object) -- and the transmittance walk from inside mobj's medium. */
-static double direct_volume(prep* P, v3 p, v3 wo, double g, int32_t mobj, double nm, const rstream* R,
-                            uint32_t dim0, uint64_t* n_shadow)
+static double lum_volume(prep* P, v3 p, v3 wo, double g, int32_t mobj, double nm, const rstream* R,
+                         uint32_t dim0, uint64_t* n_shadow)
 {
     if (P->n_lums == 0) return 0.0;
     double pdf_alias;
@@ -1179,6 +1460,41 @@ static double direct_volume(prep* P, v3 p, v3 wo, double g, int32_t mobj, double
         if (phase > 0) contrib *= power_heuristic(p_light, phase);
     }
     return contrib / pdf_alias;
+}
+
+/* In-scattering NEE: the luminary sample above, the environment sample
+ * (cos 1, the phase function as BSDF and MIS pdf, no occlusion test, the
+ * walk), and the directional lights (ComputeDirectLightingNM with the
+ * scatter point's normal set to the ray direction, MediumTransport.cpp:
+ * 160-175: cos against it, the phase function, occlusion-tested, no walk). */
+static double direct_volume(prep* P, v3 p, v3 wo, double g, int32_t mobj, double nm, const rstream* R,
+                            uint32_t dim0, uint64_t* n_shadow)
+{
+    double sum = lum_volume(P, p, wo, g, mobj, nm, R, dim0, n_shadow);
+    if (P->env) {
+        double pdf;
+        const uint32_t d0 = dim0 + (P->n_lums > 0 ? NEE_DIMS : 0u);
+        const v3 ed = env_sample(P->env, u_sampler(R, d0), u_sampler(R, d0 + 1), &pdf);
+        if (pdf > 0) {
+            const double ph = rise_oracle_hg_eval(dot(ed, wo), g);
+            double contrib = sky_radiance(&P->sky, ed, nm) * ph / pdf;
+            if (ph > 0) contrib *= power_heuristic(pdf, ph);
+            if (contrib > 0) {
+                ++*n_shadow;
+                sum += contrib * shadow_tr(P, p, ed, RISE_ORACLE_INFINITY, mobj);
+            }
+        }
+    }
+    for (int k = 0; k < P->n_dirs; ++k) {
+        const dir_light* L = &P->dirs[k];
+        const double c = dot(L->d, wo);
+        if (!(c > 0)) continue;
+        const double v = L->le * rise_oracle_hg_eval(dot(L->d, wo), g) * c;
+        if (!(v > 0)) continue;
+        ++*n_shadow;
+        if (!occluded(P, p, L->d, RISE_ORACLE_INFINITY)) sum += v;
+    }
+    return sum;
 }
 
 /* Emission at a hit, weighted against NEE by the power heuristic when the
@@ -1286,13 +1602,14 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
                     if (ld > 0) result += thr * ld;
                     const double wo[3] = {dir.x, dir.y, dir.z};
                     double wi[3];
-                    rise_oracle_hg_sample(wo, M->g, u_sampler(R, base + NEE_DIMS), u_sampler(R, base + NEE_DIMS + 1), wi);
+                    const uint32_t ls = light_dims(P);
+                    rise_oracle_hg_sample(wo, M->g, u_sampler(R, base + ls), u_sampler(R, base + ls + 1), wi);
                     const v3 nd = mk(wi[0], wi[1], wi[2]);
                     const double pp = rise_oracle_hg_eval(dot(dir, nd), M->g);
                     if (pp <= NZ) break;
                     thr *= rise_oracle_hg_eval(dot(dir, nd), M->g) / pp;
                     const rr_result rr = russian_roulette(depth + vb, s->rr_min_depth, s->rr_threshold, fabs(thr),
-                                                          importance, u_sampler(R, base + NEE_DIMS + 2));
+                                                          importance, u_sampler(R, base + ls + 2));
                     if (rr.terminate) break;
                     if (rr.survival < 1.0) thr /= rr.survival;
                     org = sp;
@@ -1309,9 +1626,9 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
                 }
             }
             if (tri < 0) {
-                /* Escape: the radiance map at this wavelength, full weight (no
-                 * environment light sample to weigh against). */
-                result += thr * sky_radiance(&P->sky, dir, nm);
+                /* Escape: the radiance map at this wavelength, weighed against
+                 * the environment sample. */
+                result += thr * (sky_radiance(&P->sky, dir, nm) * escape_mis(P, dir, bsdf_pdf));
                 break;
             }
             make_hit(P, org, dir, tri, t, &h);
@@ -1334,9 +1651,9 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
             dir = nd;
             continue;
         }
-        result += thr * direct_nm(P, &h, nm, amb, medium_object(P, &st), R, base, n_shadow);
+        result += thr * direct_nm(P, &h, nm, amb, medium_object(P, &st), R, base, 1, n_shadow);
         double u1, u2, u3;
-        const uint32_t rr_dim = bsdf_draws(R, h.mat->type, base + NEE_DIMS, &u1, &u2, &u3);
+        const uint32_t rr_dim = bsdf_draws(R, h.mat->type, base + light_dims(P), &u1, &u2, &u3);
         const rb_sample sc = bsdf_scatter(P, &h, nm, amb, u1, u2, u3);
         if (!sc.ok) break;
         double scat = sc.kray;
@@ -1407,8 +1724,10 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         if (tri < 0) {
             /* Escape (PathTracingIntegrator.cpp:4230-4290, 5070-5110): the sky at each
              * wavelength; a camera ray sees it only when the map is the background. */
-            if (depth > 0 || s->sky_background)
-                for (int w = 0; w < HWSS_N; ++w) result[w] += thr[w] * sky_radiance(&P->sky, dir, lam[w]);
+            if (depth > 0 || s->sky_background) {
+                const double wmis = escape_mis(P, dir, bsdf_pdf);
+                for (int w = 0; w < HWSS_N; ++w) result[w] += thr[w] * (sky_radiance(&P->sky, dir, lam[w]) * wmis);
+            }
             break;
         }
         hit h;
@@ -1428,11 +1747,12 @@ static void sample_xyz(prep* P, int32_t px, int32_t py, int32_t si, double xyz_o
         /* Next-event estimation, one light sample per wavelength
          * (PathTracingIntegrator.cpp:4521-4552). */
         for (int w = 0; w < HWSS_N; ++w)
-            result[w] += thr[w] * direct_nm(P, &h, lam[w], 1.0, -1, R, base + NEE_DIMS * (uint32_t)w, &n_shadow);
+            result[w] += thr[w] * direct_nm(P, &h, lam[w], 1.0, -1, R, base + light_dims(P) * (uint32_t)w, w == 0,
+                                            &n_shadow);
 
         /* Hero ScatterNM, then companions at the hero's direction (:4620-4718). */
         double u1, u2, u3;
-        const uint32_t rr_dim = bsdf_draws(R, h.mat->type, base + HWSS_N * NEE_DIMS, &u1, &u2, &u3);
+        const uint32_t rr_dim = bsdf_draws(R, h.mat->type, base + HWSS_N * light_dims(P), &u1, &u2, &u3);
         const rb_sample hs = bsdf_scatter(P, &h, lam[0], 1.0, u1, u2, u3);
         if (!hs.ok) break;
         const v3 nd = vr(hs.dir);

@@ -121,6 +121,13 @@ typedef struct ro_scene {
      * Sobol, both with the index shuffled per dimension pair; 2 the Z-order
      * sampler exactly as the reference draws it (no shuffle). */
     int32_t sampler;
+    /* The sky's matched sun (hosek_wilkie_skylight create_sun, power
+     * sun_intensity_scale) and the scene's directional lights (7 doubles
+     * each: direction toward the light, RGB colour, power). */
+    int32_t sun_enabled;
+    double sun_power;
+    int32_t n_dir_lights;
+    const double* dir_lights;
 } ro_scene;
 
 /* Counter-based uniform in [0,1) with 24 random bits (exactly representable

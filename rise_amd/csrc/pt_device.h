@@ -80,6 +80,17 @@ struct DevSky {
     float scale;
 };
 
+// A directional light (Lights/DirectionalLight.cpp): the direction toward the
+// light (unit) and Rec.709 luminance of its colour times its energy, the value
+// ComputeDirectLightingNM multiplies by f * cos at every wavelength.
+constexpr int kMaxDirLights = 4;
+struct DevDirLight {
+    float d[3];
+    float le;
+};
+
+struct EnvTable;
+
 struct PtSceneView {
     const NodeSlot* nodes;
     const Tri48* tris;      // leaf order; v0.w = global id, v1.w = object id
@@ -100,6 +111,11 @@ struct PtSceneView {
     const NodeSlot* mnodes;
     const Tri48* mtris;
     int has_media;
+    // Environment light sample over the sky (env_sampler.h; null: none) and
+    // the directional lights (the sky's matched sun, directional_light).
+    const EnvTable* env;
+    int n_dir;
+    DevDirLight dir[kMaxDirLights];
 };
 
 // ---- random numbers: counter-based hash ------------------------------------

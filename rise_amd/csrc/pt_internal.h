@@ -18,6 +18,9 @@ struct rise_hip_pt_scene {
     unsigned material_kinds = 0;    // bit k: a material of kind k (rise_hip_material_type) is present
     bool has_media = false;         // some object holds a participating medium
     rise_hip_batch* solo = nullptr;  // cached batch for rise_hip_render, rebuilt when the request changes
+    std::vector<rise_hip::DevDirLight> dir_lights;  // rise_hip_pt_scene_set_directional_lights
+    bool has_sun = false;           // the sky's matched sun light
+    rise_hip::DevDirLight sun{};
 };
 
 // Frees the scene's cached single-request batch (pt_render.hip).

@@ -31,6 +31,7 @@
 #include "api_common.h"
 #include "cie_table.h"
 #include "pt_ggx.h"
+#include "env_sampler.h"
 #include "pt_internal.h"
 #include "wave_fetch.h"
 
@@ -175,7 +176,13 @@ struct Paths {
     unsigned* mqueue[kBins]; // hits binned by material kind (and volume scatters), 4 * cap entries each
     // shadow records, 4 * cap: origin, unit direction, distance to the light point, value
     float *sox, *soy, *soz, *sdx, *sdy, *sdz, *stmax, *sval;
-    unsigned* sslot;     // slot * 4 + wavelength
+    unsigned* sslot;     // record code: slot * 4 + wavelength, kind, all-lanes flag (kRec*)
+    // Result arrays of the environment sample and of each directional light
+    // (4 * cap each; null when no scene of the pass has them), and a shared
+    // ray's four directional-light values awaiting their one shadow ray.
+    float* res_env;
+    float* res_dir[kMaxDirLights];
+    float* dir_val[kMaxDirLights];
     // Media only (null when no scene of the pass holds a medium):
     unsigned char* vb;   // 4 * cap: volume bounces of the lane
     unsigned* sinfo;     // per shadow record: kShadowInScatter | (object whose medium holds the origin) + 1
@@ -317,6 +324,9 @@ __global__ void k_generate(const ReqInfo* __restrict__ reqs, const PassEntry* __
         for (int w = 0; w < kHwss; ++w) {
             S.thr[w * S.cap + i] = 1.0f;
             S.res[w * S.cap + i] = 0.0f;
+            if (S.res_env) S.res_env[w * S.cap + i] = 0.0f;
+            for (int l = 0; l < kMaxDirLights; ++l)
+                if (S.res_dir[l]) S.res_dir[l][w * S.cap + i] = 0.0f;
         }
         S.lambda0[i] = R.lambda_begin + ul * (R.lambda_end - R.lambda_begin);
         S.pdf[i] = 0.0f;
@@ -630,6 +640,56 @@ constexpr unsigned kLaneDims = 1u << 20;  // key offset of lane w's independent 
 constexpr unsigned kModeNm = 1u;          // flags bit: the path continues as single-wavelength lanes
 
 constexpr unsigned kShadowInScatter = 0x80000000u;  // shadow record of an in-scattering NEE sample
+constexpr unsigned kShadowNoWalk = 0x40000000u;     // shadow record added without the transmittance walk
+
+// Shadow-record codes: slot * 4 + wavelength in the low 27 bits (slots < 2^25),
+// the kind of contribution above them -- each kind adds into its own result
+// array, so no two records of one bounce add into the same place -- and, for
+// a directional light seen by a shared ray, a flag for all four wavelengths.
+constexpr unsigned kRecSlotMask = (1u << 27) - 1u;
+constexpr unsigned kRecKindShift = 27;
+constexpr unsigned kRecLum = 0u, kRecEnv = 1u, kRecDir = 2u;  // kRecDir + l: directional light l
+constexpr unsigned kRecAllLanes = 1u << 30;
+
+// Sampler dimensions of one light sample: the luminary pick and point when the
+// scene has luminaries (an empty alias table draws nothing,
+// LightSampler.cpp:2053-2056), then the environment sample's two.
+__device__ inline unsigned light_sample_dims(const PtSceneView& sc)
+{
+    return (sc.n_lums > 0 ? kNeeDims : 0u) + (sc.env ? 2u : 0u);
+}
+
+__device__ inline void write_shadow(Paths& S, unsigned si, f3 o, f3 d, float tmax, float val, unsigned code,
+                                    unsigned info)
+{
+    S.sox[si] = o.x;
+    S.soy[si] = o.y;
+    S.soz[si] = o.z;
+    S.sdx[si] = d.x;
+    S.sdy[si] = d.y;
+    S.sdz[si] = d.z;
+    S.stmax[si] = tmax;
+    S.sval[si] = val;
+    S.sslot[si] = code;
+    if (S.sinfo) {
+        S.sinfo[si] = info;
+        S.svis[si] = 0u;
+    }
+}
+
+// Adds an unoccluded record's value, times the transmittance tr, to its result array.
+__device__ inline void add_shadow(Paths& S, unsigned code, float val, float tr)
+{
+    const unsigned cap = (unsigned)S.cap, base = code & kRecSlotMask, slot = base >> 2, w = base & 3u;
+    const unsigned kind = (code >> kRecKindShift) & 7u;
+    float* dst = kind == kRecLum ? S.res : (kind == kRecEnv ? S.res_env : S.res_dir[kind - kRecDir]);
+    if (code & kRecAllLanes) {
+        const float* v = S.dir_val[kind - kRecDir];
+        for (int k = 0; k < kHwss; ++k) dst[k * cap + slot] += v[k * cap + slot] * tr;
+    } else {
+        dst[w * cap + slot] += val * tr;
+    }
+}
 
 // The object on top of a lane's IOR stack when it holds a medium, else -1
 // (MediumTracking::GetCurrentMediumWithObject, Utilities/MediumTracking.h;
@@ -705,14 +765,23 @@ __global__ __launch_bounds__(kShadeBlock) void k_bin(const ReqInfo* __restrict__
             } else if (sc.sky.on && (depth > 0 || R.sky_background)) {
                 // Escape (PathTracingIntegrator.cpp:4230-4290, 5070-5110 and the
                 // single-wavelength twin): the sky at each wavelength the ray
-                // carries, full weight (no environment light sample to weigh
-                // against).
+                // carries.
                 float lam[kHwss];
                 wavelengths(S.lambda0[slot], R.lambda_begin, R.lambda_end, lam);
                 const f3 d = mk3(S.dx[li], S.dy[li], S.dz[li]);
+                // A BSDF-sampled ray is weighed against the environment light
+                // sample by the power heuristic (:4259-4282); camera rays and
+                // rays leaving a delta lobe (pdf 0) keep full weight.
+                float wmis = 1.0f;
+                const float pin = S.pdf[li];
+                if (sc.env && pin > 0.0f) {
+                    const double dd[3] = {d.x, d.y, d.z};
+                    const float pe = (float)env_pdf(*sc.env, dd);
+                    if (pe > 0.0f) wmis = power_heuristic(pin, pe);
+                }
                 for (int w = 0; w < kHwss; ++w)
-                    if (!nm) S.res[w * S.cap + slot] += S.thr[w * S.cap + slot] * sky_radiance(sc.sky, d, lam[w]);
-                if (nm) S.res[li] += S.thr0[li] * (S.thr[li] * sky_radiance(sc.sky, d, lam[lane]));
+                    if (!nm) S.res[w * S.cap + slot] += S.thr[w * S.cap + slot] * (sky_radiance(sc.sky, d, lam[w]) * wmis);
+                if (nm) S.res[li] += S.thr0[li] * (S.thr[li] * (sky_radiance(sc.sky, d, lam[lane]) * wmis));
             }
         }
         const unsigned at = push(cnt, kind);  // kind kMatVolume only when the pass has media
@@ -849,15 +918,23 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             if (nm) S.res[li] += S.thr0[li] * (S.thr[li] * (emitted(sc, mat, -h.dir, h.ng, lam[lane]) * mis));
         }
 
-        // Next-event estimation: one light sample per wavelength of a shared
-        // ray, one for a single-wavelength lane (:4521-4552 and the NM twin ->
-        // LightSampler::EvaluateDirectLightingNM, Lights/LightSampler.cpp:1987-2269).
+        // Next-event estimation (:4521-4552 and the NM twin ->
+        // LightSampler::EvaluateDirectLightingNM, Lights/LightSampler.cpp:1942-2362):
+        // per wavelength of a shared ray (one for a single-wavelength lane) one
+        // luminary sample, one environment sample when the sky is sampled, and
+        // every directional light deterministically.  Each kind of contribution
+        // has its own shadow record and result array, so no two records of a
+        // bounce add into the same place.
+        const unsigned ls_dims = light_sample_dims(sc);
+        const bool blk_env = __syncthreads_or(surface && sc.env != nullptr);
+        const bool blk_dir = __syncthreads_or(surface && sc.n_dir > 0);
         for (int w = 0; w < kHwss; ++w) {
+            const bool mine = surface && (!nm || (unsigned)w == lane);
+            const unsigned dim = nm ? base_dim : base_dim + ls_dims * (unsigned)w;
             bool has = false;
             float v = 0.0f, stm = 0.0f;
             f3 sorg = mk3(0, 0, 0), to = mk3(0, 0, 0);
-            if (surface && sc.n_lums > 0 && (!nm || (unsigned)w == lane)) {
-                const unsigned dim = nm ? base_dim : base_dim + kNeeDims * (unsigned)w;
+            if (mine && sc.n_lums > 0) {
                 const float usel = sample_u(key, dim);
                 int li_ = 0;
                 while (li_ < sc.n_lums - 1 && !(usel < sc.lums[li_].cdf_select)) ++li_;
@@ -899,20 +976,76 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 }
             }
             const unsigned si = push(cnt_sh, has);
-            if (has) {
-                S.sox[si] = sorg.x;
-                S.soy[si] = sorg.y;
-                S.soz[si] = sorg.z;
-                S.sdx[si] = to.x;
-                S.sdy[si] = to.y;
-                S.sdz[si] = to.z;
-                S.stmax[si] = stm;
-                S.sval[si] = v;
-                S.sslot[si] = slot * 4u + (unsigned)w;
-                if (S.sinfo) {
-                    S.sinfo[si] = (unsigned)(mobj + 1);
-                    S.svis[si] = 0u;
+            if (has) write_shadow(S, si, sorg, to, stm, v, slot * 4u + (unsigned)w, (unsigned)(mobj + 1));
+
+            // Environment sample (:2278-2358): a direction from the sky's
+            // importance table, the sky at this wavelength, power heuristic
+            // against the BSDF's pdf, a shadow ray to infinity.
+            if (blk_env) {
+                bool has_e = false;
+                float ve = 0.0f;
+                f3 ed = mk3(0, 0, 0), eo = mk3(0, 0, 0);
+                if (mine && sc.env) {
+                    const unsigned ed0 = dim + (sc.n_lums > 0 ? kNeeDims : 0u);
+                    double dd[3], epdf;
+                    env_sample(*sc.env, sample_u(key, ed0), sample_u(key, ed0 + 1), dd, epdf);
+                    ed = mk3((float)dd[0], (float)dd[1], (float)dd[2]);
+                    const float cos_e = dot3(ed, h.n);
+                    if (cos_e > 0.0f && epdf > 0.0) {
+                        const float pe = (float)epdf;
+                        float contrib = sky_radiance(sc.sky, ed, lam[w]) *
+                                        bsdf_value<MAT>(sc, mat, h, ed, lam[w], amb) * cos_e / pe;
+                        const float p_bsdf = bsdf_pdf<MAT>(sc, mat, h, ed, lam[w], amb);
+                        if (p_bsdf > 0.0f) contrib *= power_heuristic(pe, p_bsdf);
+                        ve = nm ? S.thr0[li] * (S.thr[li] * contrib) : S.thr[w * cap + slot] * contrib;
+                        if (ve > 0.0f) {
+                            has_e = true;
+                            eo = offset_ray(h.p, dot3(ed, h.ng) > 0.0f ? h.ng : -h.ng);
+                        }
+                    }
                 }
+                const unsigned se = push(cnt_sh, has_e);
+                if (has_e)
+                    write_shadow(S, se, eo, ed, INFINITY, ve, (slot * 4u + (unsigned)w) | (kRecEnv << kRecKindShift),
+                                 (unsigned)(mobj + 1));
+            }
+        }
+        // Directional lights (:1968-2000 -> DirectionalLight::ComputeDirectLightingNM,
+        // Lights/DirectionalLight.cpp:75-105): luminance x power x BSDF x cos along
+        // the light's direction, shadow ray to infinity, no transmittance walk.  A
+        // shared ray's four wavelengths share one shadow ray: their values wait in
+        // dir_val and the record adds all four.
+        if (blk_dir) {
+            for (int l = 0; l < kMaxDirLights; ++l) {
+                if (!__syncthreads_or(surface && l < sc.n_dir)) break;
+                bool has_d = false;
+                float vd = 0.0f;
+                f3 dd = mk3(0, 0, 0), dorg = mk3(0, 0, 0);
+                if (surface && l < sc.n_dir) {
+                    const DevDirLight L = sc.dir[l];
+                    dd = mk3(L.d[0], L.d[1], L.d[2]);
+                    const float cosd = dot3(dd, h.n);
+                    if (cosd > 0.0f) {
+                        if (nm) {
+                            vd = S.thr0[li] * (S.thr[li] * (L.le * bsdf_value<MAT>(sc, mat, h, dd, lam[lane], amb) * cosd));
+                            has_d = vd > 0.0f;
+                        } else {
+                            for (int w = 0; w < kHwss; ++w) {
+                                const float x = S.thr[w * cap + slot] *
+                                                (L.le * bsdf_value<MAT>(sc, mat, h, dd, lam[w], 1.0f) * cosd);
+                                S.dir_val[l][w * cap + slot] = x;
+                                has_d = has_d || x > 0.0f;
+                            }
+                        }
+                        if (has_d) dorg = offset_ray(h.p, dot3(dd, h.ng) > 0.0f ? h.ng : -h.ng);
+                    }
+                }
+                const unsigned sd = push(cnt_sh, has_d);
+                if (has_d)
+                    write_shadow(S, sd, dorg, dd, INFINITY, vd,
+                                 (slot * 4u + (nm ? lane : 0u)) | ((kRecDir + (unsigned)l) << kRecKindShift) |
+                                     (nm ? 0u : kRecAllLanes),
+                                 kShadowNoWalk | (unsigned)(mobj + 1));
             }
         }
 
@@ -927,7 +1060,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             if (surface && !nm) {
                 // Hero ScatterNM, companions at the hero's direction (:4620-4718).
                 float u1, u2, u3;
-                const unsigned rr_dim = bsdf_draws<MAT>(key, base_dim + kHwss * kNeeDims, u1, u2, u3);
+                const unsigned rr_dim = bsdf_draws<MAT>(key, base_dim + kHwss * ls_dims, u1, u2, u3);
                 const ScatterOut hs = bsdf_scatter<MAT>(sc, mat, h, lam[0], 1.0f, u1, u2, u3);
                 if (hs.ok) {
                     float thr[kHwss], sc4[kHwss];
@@ -978,7 +1111,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
             } else if (surface && nm) {
                 // IntegrateFromHitNM PART 3 at the lane's wavelength (:2905-3290).
                 float u1, u2, u3;
-                const unsigned rr_dim = bsdf_draws<MAT>(key, base_dim + kNeeDims, u1, u2, u3);
+                const unsigned rr_dim = bsdf_draws<MAT>(key, base_dim + ls_dims, u1, u2, u3);
                 const ScatterOut so = bsdf_scatter<MAT>(sc, mat, h, lam[lane], amb, u1, u2, u3);
                 float scat = so.kray;
                 bool go = so.ok && fabsf(scat) > kNearZeroF;
@@ -1184,17 +1317,60 @@ __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restric
         }
         const unsigned si = push(cnt_sh, has);
         if (has) {
-            S.sox[si] = p.x;
-            S.soy[si] = p.y;
-            S.soz[si] = p.z;
-            S.sdx[si] = to.x;
-            S.sdy[si] = to.y;
-            S.sdz[si] = to.z;
-            S.stmax[si] = dist;
-            S.sval[si] = v;
-            S.sslot[si] = slot * 4u + lane;
-            S.sinfo[si] = kShadowInScatter | (unsigned)(mobj + 1);
+            write_shadow(S, si, p, to, dist, v, slot * 4u + lane, kShadowInScatter | (unsigned)(mobj + 1));
             S.svis[si] = 1u;
+        }
+        // Environment sample from the scatter point (:2278-2358 with
+        // isVolumeScatter: cos 1, the phase function as BSDF and MIS pdf, no
+        // occlusion test without a shading object, the transmittance walk).
+        const unsigned ls_dims = light_sample_dims(sc);
+        if (__syncthreads_or(alive && sc.env != nullptr)) {
+            bool has_e = false;
+            float ve = 0.0f;
+            f3 ed = mk3(0, 0, 0);
+            if (alive && sc.env) {
+                const unsigned ed0 = base_dim + (sc.n_lums > 0 ? kNeeDims : 0u);
+                double dd[3], epdf;
+                env_sample(*sc.env, sample_u(key, ed0), sample_u(key, ed0 + 1), dd, epdf);
+                ed = mk3((float)dd[0], (float)dd[1], (float)dd[2]);
+                if (epdf > 0.0) {
+                    const float pe = (float)epdf;
+                    const float ph = hg_eval(dot3(ed, d), M.g);
+                    float contrib = sky_radiance(sc.sky, ed, nm) * ph / pe;
+                    if (ph > 0.0f) contrib *= power_heuristic(pe, ph);
+                    ve = S.thr0[li] * (thr * contrib);
+                    has_e = ve > 0.0f;
+                }
+            }
+            const unsigned se = push(cnt_sh, has_e);
+            if (has_e) {
+                write_shadow(S, se, p, ed, INFINITY, ve, (slot * 4u + lane) | (kRecEnv << kRecKindShift),
+                             kShadowInScatter | (unsigned)(mobj + 1));
+                S.svis[se] = 1u;
+            }
+        }
+        // Directional lights (ComputeDirectLightingNM with the scatter point's
+        // normal set to the ray direction, MediumTransport.cpp:160-175): cos
+        // against the ray direction, the phase function as BSDF, a shadow ray,
+        // no transmittance walk.
+        for (int l = 0; l < kMaxDirLights; ++l) {
+            if (!__syncthreads_or(alive && l < sc.n_dir)) break;
+            bool has_d = false;
+            float vd = 0.0f;
+            f3 dd = mk3(0, 0, 0);
+            if (alive && l < sc.n_dir) {
+                const DevDirLight L = sc.dir[l];
+                dd = mk3(L.d[0], L.d[1], L.d[2]);
+                const float cosd = dot3(dd, d);
+                if (cosd > 0.0f) {
+                    vd = S.thr0[li] * (thr * (L.le * hg_eval(dot3(dd, d), M.g) * cosd));
+                    has_d = vd > 0.0f;
+                }
+            }
+            const unsigned sd = push(cnt_sh, has_d);
+            if (has_d)
+                write_shadow(S, sd, p, dd, INFINITY, vd, (slot * 4u + lane) | ((kRecDir + (unsigned)l) << kRecKindShift),
+                             kShadowNoWalk | (unsigned)(mobj + 1));
         }
 
         // Phase sampling, Russian roulette, continuation.
@@ -1202,14 +1378,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_volume(const ReqInfo* __restric
         f3 nd = mk3(0, 0, 0);
         float pp = 0.0f;
         if (alive) {
-            nd = hg_sample(d, M.g, sample_u(key, base_dim + kNeeDims), sample_u(key, base_dim + kNeeDims + 1));
+            nd = hg_sample(d, M.g, sample_u(key, base_dim + ls_dims), sample_u(key, base_dim + ls_dims + 1));
             pp = hg_eval(dot3(d, nd), M.g);
             if (pp > kNearZeroF) {
                 float t = thr * (hg_eval(dot3(d, nd), M.g) / pp);
                 bool go = true;
                 if (depth + (int)vbn >= R.rr_min_depth) {
                     const float q = fminf(1.0f, fabsf(t) / fmaxf(imp, R.rr_threshold));
-                    if (sample_u(key, base_dim + kNeeDims + 2) >= q) go = false;
+                    if (sample_u(key, base_dim + ls_dims + 2) >= q) go = false;
                     else if (q > 0.0f && q < 1.0f) t /= q;
                 }
                 if (go) {
@@ -1297,11 +1473,14 @@ __global__ void k_walk(const ReqInfo* __restrict__ reqs, Paths S, int depth)
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (!S.svis[i]) continue;
         const unsigned code = S.sslot[i];
-        const ReqInfo& R = reqs[S.req[code >> 2]];
-        const int origin = (int)(S.sinfo[i] & ~kShadowInScatter) - 1;
-        const float tr = shadow_walk(R.sc, mk3(S.sox[i], S.soy[i], S.soz[i]), mk3(S.sdx[i], S.sdy[i], S.sdz[i]),
-                                     S.stmax[i], origin);
-        S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[i] * tr;
+        const ReqInfo& R = reqs[S.req[(code & kRecSlotMask) >> 2]];
+        const unsigned info = S.sinfo[i];
+        const int origin = (int)(info & ~(kShadowInScatter | kShadowNoWalk)) - 1;
+        const float tr = (info & kShadowNoWalk)
+                             ? 1.0f
+                             : shadow_walk(R.sc, mk3(S.sox[i], S.soy[i], S.soz[i]), mk3(S.sdx[i], S.sdy[i], S.sdz[i]),
+                                           S.stmax[i], origin);
+        add_shadow(S, code, S.sval[i], tr);
     }
 }
 
@@ -1323,7 +1502,7 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
         for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
             if (S.sinfo && (S.sinfo[i] & kShadowInScatter)) continue;
             const unsigned code = S.sslot[i];
-            const ReqInfo& R = reqs[S.req[code >> 2]];
+            const ReqInfo& R = reqs[S.req[(code & kRecSlotMask) >> 2]];
             RayState r;
             init_ray(r, S.sox[i], S.soy[i], S.soz[i], S.sdx[i], S.sdy[i], S.sdz[i]);
             unsigned nn = 0, nt = 0;
@@ -1334,7 +1513,7 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
                                        : trace_any<false>(r, R.sc.nodes, R.sc.tris, pst, how_far2, nn, nt);
             if (!occ) {
                 if (R.sc.has_media) S.svis[i] = 1u;
-                else S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[i];
+                else add_shadow(S, code, S.sval[i], 1.0f);
             }
         }
         return;
@@ -1356,7 +1535,7 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
             if (item >= 0 && S.sinfo && (S.sinfo[item] & kShadowInScatter)) {
                 item = -1;  // no occlusion test; the walk adds it
             } else if (item >= 0) {
-                const unsigned slot = S.sslot[item] >> 2;
+                const unsigned slot = (S.sslot[item] & kRecSlotMask) >> 2;
                 const ReqInfo& R = reqs[S.req[slot]];
                 nodes = R.sc.nodes;
                 tris = R.sc.tris;
@@ -1373,9 +1552,8 @@ __global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ r
                                         : any_step<false>(r, nodes, tris, pst, s, nn, nt);
             if (!more) {
                 if (!s.hit) {
-                    const unsigned code = S.sslot[item];
                     if (media) S.svis[item] = 1u;
-                    else S.res[(code & 3u) * S.cap + (code >> 2)] += S.sval[item];
+                    else add_shadow(S, S.sslot[item], S.sval[item], 1.0f);
                 }
                 item = -1;
             }
@@ -1448,7 +1626,12 @@ __global__ void k_film(const ReqInfo* __restrict__ reqs, const PassEntry* __rest
             for (int w = 0; w < kHwss; ++w) {
                 float x, y, z;
                 if (xyz_from_nm(lam[w], x, y, z)) {
-                    const float r = S.res[w * S.cap + slot];
+                    // Luminaries, emission and escapes; the environment sample;
+                    // each directional light: always in this order.
+                    float r = S.res[w * S.cap + slot];
+                    if (S.res_env) r += S.res_env[w * S.cap + slot];
+                    for (int l = 0; l < kMaxDirLights; ++l)
+                        if (S.res_dir[l]) r += S.res_dir[l][w * S.cap + slot];
                     X += x * r;
                     Y += y * r;
                     Z += z * r;
@@ -1484,8 +1667,10 @@ __global__ void k_resolve(const ReqInfo* __restrict__ reqs, int req)
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Carves the path pool for `cap` paths plus `passes` counter blocks from
-// `base` (NULL: sizes only); returns the size.
-size_t carve(char* base, int64_t cap, int passes, bool media, Paths* P, unsigned** counters)
+// `base` (NULL: sizes only); returns the size.  `env` and `n_dir`: the
+// environment sample's and n_dir directional lights' result arrays, and room
+// for their shadow records (4 per path and bounce for each kind).
+size_t carve(char* base, int64_t cap, int passes, bool media, bool env, int n_dir, Paths* P, unsigned** counters)
 {
     size_t off = 0;
     auto take = [&](size_t bytes) -> char* {
@@ -1494,6 +1679,7 @@ size_t carve(char* base, int64_t cap, int passes, bool media, Paths* P, unsigned
         return p;
     };
     const size_t c = (size_t)cap, c4 = 4 * c;
+    const size_t csh = c4 * (size_t)(1 + (env ? 1 : 0) + n_dir);  // shadow records
     Paths S{};
     S.cap = (int)cap;
     S.ox = (float*)take(c4 * 4);
@@ -1521,18 +1707,23 @@ size_t carve(char* base, int64_t cap, int passes, bool media, Paths* P, unsigned
     if (media) {
         S.mqueue[kMatVolume] = (unsigned*)take(c4 * 4);
         S.vb = (unsigned char*)take(c4);
-        S.sinfo = (unsigned*)take(c4 * 4);
-        S.svis = (unsigned*)take(c4 * 4);
+        S.sinfo = (unsigned*)take(csh * 4);
+        S.svis = (unsigned*)take(csh * 4);
     }
-    S.sox = (float*)take(c4 * 4);
-    S.soy = (float*)take(c4 * 4);
-    S.soz = (float*)take(c4 * 4);
-    S.sdx = (float*)take(c4 * 4);
-    S.sdy = (float*)take(c4 * 4);
-    S.sdz = (float*)take(c4 * 4);
-    S.stmax = (float*)take(c4 * 4);
-    S.sval = (float*)take(c4 * 4);
-    S.sslot = (unsigned*)take(c4 * 4);
+    S.sox = (float*)take(csh * 4);
+    S.soy = (float*)take(csh * 4);
+    S.soz = (float*)take(csh * 4);
+    S.sdx = (float*)take(csh * 4);
+    S.sdy = (float*)take(csh * 4);
+    S.sdz = (float*)take(csh * 4);
+    S.stmax = (float*)take(csh * 4);
+    S.sval = (float*)take(csh * 4);
+    S.sslot = (unsigned*)take(csh * 4);
+    if (env) S.res_env = (float*)take(c4 * 4);
+    for (int l = 0; l < n_dir; ++l) {
+        S.res_dir[l] = (float*)take(c4 * 4);
+        S.dir_val[l] = (float*)take(c4 * 4);
+    }
     unsigned* cnt = (unsigned*)take(sizeof(unsigned) * (size_t)kCounters * (size_t)std::max(1, passes));
     S.fetch = (unsigned long long*)take(sizeof(unsigned long long) * 2 * (kMaxPathDepth + 1));
     if (P) *P = S;
@@ -1837,7 +2028,8 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     int bounces = 1, depth = 1;
     int64_t max_tris = 0;
     unsigned mat_mask = 0;
-    bool has_media = false;
+    bool has_media = false, has_env = false;
+    int n_dir = 0;
     std::vector<int64_t> req_px(n, 0);
     size_t film_floats = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -1860,6 +2052,8 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
         depth = std::max(depth, reqs[k].scene->view.depth);
         mat_mask |= reqs[k].scene->material_kinds | (sc->has_media ? 1u << kMatVolume : 0u);
         has_media = has_media || sc->has_media;
+        has_env = has_env || sc->view.env != nullptr;
+        n_dir = std::max(n_dir, sc->view.n_dir);
         max_tris = std::max<int64_t>(max_tris, reqs[k].scene->n_tris);
         film_floats += 12 * (size_t)req_px[k];
     }
@@ -1890,7 +2084,7 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     // Device buffers.
     hipError_t e = hipGetDevice(&b->device);
     if (e != hipSuccess) return fail_hip(e, "hipGetDevice");
-    const size_t need = carve(nullptr, cap, passes, has_media, nullptr, nullptr);
+    const size_t need = carve(nullptr, cap, passes, has_media, has_env, n_dir, nullptr, nullptr);
     if (need > b->work_bytes || !b->work) {
         (void)hipFree(b->work);
         b->work = nullptr;
@@ -1918,7 +2112,7 @@ rise_hip_status plan(rise_hip_batch* b, const rise_hip_render_request* reqs, siz
     if (e != hipSuccess) return fail_hip(e, "batch tables");
 
     b->cap = cap;
-    carve((char*)b->work, cap, passes, has_media, &b->S, &b->counters);
+    carve((char*)b->work, cap, passes, has_media, has_env, n_dir, &b->S, &b->counters);
     b->film_floats = film_floats;
     b->entries.swap(entries);
     b->pass_first.swap(pass_first);
@@ -2144,6 +2338,7 @@ extern "C" rise_hip_status rise_hip_render(rise_hip_pt_scene* scene, const rise_
 namespace {
 
 constexpr int kRing = 4;          // passes in flight
+constexpr int kServerDirLights = 1;  // directional lights a served scene may have (a sky's sun)
 constexpr int kServerSlots = 256; // admitted requests at most
 constexpr int kPassEntries = 1024; // chunks per server pass at most
 
@@ -2395,7 +2590,7 @@ extern "C" rise_hip_status rise_hip_server_create(const rise_hip_server_config* 
     sv->cap = cfg.tuning.max_paths > 0 ? std::max<int64_t>(kChunkPaths, std::min<int64_t>(cfg.tuning.max_paths, kMaxPaths))
                                        : kMaxPaths;
     hipError_t e = hipGetDevice(&sv->device);
-    const size_t need = carve(nullptr, sv->cap, 1, true, nullptr, nullptr);
+    const size_t need = carve(nullptr, sv->cap, 1, true, true, kServerDirLights, nullptr, nullptr);
     if (e == hipSuccess) e = hipMalloc(&sv->work, need);
     if (e == hipSuccess) e = hipMalloc((void**)&sv->d_counters, sizeof(unsigned) * kCounters * kRing);
     if (e == hipSuccess) e = hipMalloc((void**)&sv->d_info, sizeof(ReqInfo) * kServerSlots);
@@ -2408,7 +2603,7 @@ extern "C" rise_hip_status rise_hip_server_create(const rise_hip_server_config* 
         server_free(sv);
         return fail_hip(e, "server allocation");
     }
-    carve((char*)sv->work, sv->cap, 1, true, &sv->S, nullptr);
+    carve((char*)sv->work, sv->cap, 1, true, true, kServerDirLights, &sv->S, nullptr);
     for (int k = sv->max_slots - 1; k >= 0; --k) sv->free_slots.push_back(k);
     *out_server = sv;
     return RISE_HIP_OK;
@@ -2421,6 +2616,9 @@ extern "C" rise_hip_status rise_hip_server_submit(rise_hip_server* server, const
     if (!server || !request) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
     rise_hip_status st = validate(*request, 0);
     if (st != RISE_HIP_OK) return st;
+    if (request->scene->view.n_dir > kServerDirLights)
+        return fail(RISE_HIP_ERR_NOT_SUPPORTED, "the server renders scenes with at most " +
+                                                    std::to_string(kServerDirLights) + " directional light");
     {
         ReqInfo probe;
         st = make_info(*request, probe);  // camera checks, so admission cannot fail later

@@ -4,6 +4,7 @@
 
 #include "api_common.h"
 #include "bvh_build.h"
+#include "env_sampler.h"
 #include "pt_internal.h"
 #include "rise_hip_rgb_uplift.h"
 
@@ -357,16 +358,73 @@ extern "C" rise_hip_status rise_hip_pt_scene_release(rise_hip_pt_scene* scene)
 // Job::AddHosekWilkieSkylight without the sun light: HosekWilkieSkyModel's
 // constructor (Utilities/HosekWilkieSkyModel.cpp:98-150) evaluated once in
 // FP64; the per-direction part runs on the device (sky_radiance).
+namespace {
+
+// The scene's directional lights as the kernels see them: the list set by
+// rise_hip_pt_scene_set_directional_lights, then the sky's matched sun.
+void refresh_dir_lights(rise_hip_pt_scene* s)
+{
+    PtSceneView& v = s->view;
+    v.n_dir = 0;
+    for (const DevDirLight& l : s->dir_lights) v.dir[v.n_dir++] = l;
+    if (s->has_sun) v.dir[v.n_dir++] = s->sun;
+}
+
+// DirectionalLight (Lights/DirectionalLight.cpp:25-30, 80-100): normalized
+// direction toward the light; Rec.709 luminance of the colour times the energy.
+bool make_dir_light(const float dir[3], const float color[3], float power, DevDirLight& out)
+{
+    const double x = dir[0], y = dir[1], z = dir[2];
+    const double m = std::sqrt(x * x + y * y + z * z);
+    if (!(m > 0.0) || !std::isfinite(m) || !std::isfinite(power)) return false;
+    out.d[0] = (float)(x / m);
+    out.d[1] = (float)(y / m);
+    out.d[2] = (float)(z / m);
+    const double lum = 0.2126 * color[0] + 0.7152 * color[1] + 0.0722 * color[2];
+    out.le = (float)(lum * (double)power);
+    return true;
+}
+
+}  // namespace
+
+extern "C" rise_hip_status rise_hip_pt_scene_set_directional_lights(rise_hip_pt_scene* scene,
+                                                                    const rise_hip_directional_light_desc* lights,
+                                                                    size_t n)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n && !lights) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "lights is NULL with a nonzero count");
+    if (n + (scene->has_sun ? 1 : 0) > (size_t)kMaxDirLights)
+        return fail(RISE_HIP_ERR_NOT_SUPPORTED, "more than " + std::to_string(kMaxDirLights) +
+                                                    " directional lights (a sky's sun counts as one)");
+    std::vector<DevDirLight> v(n);
+    for (size_t i = 0; i < n; ++i)
+        if (!make_dir_light(lights[i].direction, lights[i].color, lights[i].power, v[i]))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "directional light " + std::to_string(i) +
+                                                           ": zero or non-finite direction or power");
+    release_solo_batch(scene);  // a cached plan holds a copy of the scene tables
+    scene->dir_lights = v;
+    refresh_dir_lights(scene);
+    return RISE_HIP_OK;
+}
+
 extern "C" rise_hip_status rise_hip_pt_scene_set_sky(rise_hip_pt_scene* scene, const rise_hip_sky_desc* sky)
 {
     clear_error();
     if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (sky && (!(sky->turbidity >= 1.0f && sky->turbidity <= 64.0f) || !(sky->sky_scale >= 0.0f)))
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad sky turbidity or scale");
+    if (sky && sky->create_sun && scene->dir_lights.size() + 1 > (size_t)kMaxDirLights)
+        return fail(RISE_HIP_ERR_NOT_SUPPORTED, "no room for the sky's sun among the directional lights");
+    if (sky && sky->create_sun && !std::isfinite(sky->sun_intensity_scale))
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad sun_intensity_scale");
     release_solo_batch(scene);  // a cached plan holds a copy of the scene tables
     DevSky& d = scene->view.sky;
     d = DevSky{};
+    scene->view.env = nullptr;
+    scene->has_sun = false;
+    refresh_dir_lights(scene);
     if (!sky) return RISE_HIP_OK;
-    if (!(sky->turbidity >= 1.0f && sky->turbidity <= 64.0f) || !(sky->sky_scale >= 0.0f))
-        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad sky turbidity or scale");
     const double el = sky->solar_elevation_deg * (M_PI / 180.0), az = sky->solar_azimuth_deg * (M_PI / 180.0);
     const double T = sky->turbidity;
     // Preetham 1999 Table 2: each coefficient is t * T + c.
@@ -408,5 +466,31 @@ extern "C" rise_hip_status rise_hip_pt_scene_set_sky(rise_hip_pt_scene* scene, c
         d.zen[c] = (float)(zv[c] / perez(c, 1.0, ts));
     }
     d.scale = sky->sky_scale;
+
+    // The environment light sample's table (built in double on the host) and
+    // the matched sun (Job::AddHosekWilkieSkylight, Job.cpp:2734-2746).
+    SkyModelD md;
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 5; ++k) md.p[c][k] = p[c][k];
+    for (int c = 0; c < 3; ++c) md.zen[c] = zv[c];
+    md.theta_s = ts;
+    md.sun[0] = std::cos(el) * std::sin(az);
+    md.sun[1] = std::sin(el);
+    md.sun[2] = std::cos(el) * std::cos(az);
+    std::vector<EnvTable> tab(1);
+    build_env_table(md, tab[0]);
+    if (tab[0].total > 0.0) {  // a black sky disables the sample (RayCaster.cpp:352-360)
+        const EnvTable* dev = nullptr;
+        const rise_hip_status st = upload(scene, tab, &dev);  // kept until the scene is released
+        if (st != RISE_HIP_OK) return st;
+        scene->view.env = dev;
+    }
+    if (sky->create_sun) {
+        const float dir[3] = {(float)md.sun[0], (float)md.sun[1], (float)md.sun[2]};
+        const float white[3] = {1.0f, 1.0f, 1.0f};
+        make_dir_light(dir, white, sky->sun_intensity_scale, scene->sun);
+        scene->has_sun = true;
+    }
+    refresh_dir_lights(scene);
     return RISE_HIP_OK;
 }

@@ -43,6 +43,7 @@ struct rise_hip_scene_file {
     std::vector<rise_hip_medium_desc> media;
     bool has_sky = false;
     rise_hip_sky_desc sky{};
+    std::vector<rise_hip_directional_light_desc> dir_lights;
     rise_hip_render_desc desc{};
     int skipped = 0;
     bool any_media = false;
@@ -523,16 +524,34 @@ struct Builder {
         medium_of[unquote(c.get("name") ? *c.get("name") : "noname")] = (int)F.media.size() - 1;
     }
 
-    // :6931 (Job::AddHosekWilkieSkylight) without the matched sun light.
+    // :6931 (Job::AddHosekWilkieSkylight): the radiance map and, unless
+    // create_sun is FALSE, its matched directional sun light.  ground_albedo has
+    // no effect on the model (HosekWilkieSkyModel.cpp:135-155) and is ignored.
     void sky(const Chunk& c)
     {
-        if (get_bool(c, "create_sun", true))
-            error_at(c, "the sky's directional sun light is not supported: set create_sun FALSE");
         F.has_sky = true;
         F.sky.solar_elevation_deg = (float)get_double(c, "solar_elevation", 45.0);
         F.sky.solar_azimuth_deg = (float)get_double(c, "solar_azimuth", 0.0);
         F.sky.turbidity = (float)get_double(c, "turbidity", 3.0);
         F.sky.sky_scale = (float)get_double(c, "sky_intensity_scale", 1.0);
+        F.sky.create_sun = get_bool(c, "create_sun", true) ? 1 : 0;
+        F.sky.sun_intensity_scale = (float)get_double(c, "sun_intensity_scale", 3.14);
+    }
+
+    // :6976 (Job::AddDirectionalLight): power (1), direction and color (0 0 0).
+    void directional_light(const Chunk& c)
+    {
+        rise_hip_directional_light_desc d{};
+        double dir[3] = {0, 0, 0}, col[3] = {0, 0, 0};
+        get_vec3(c, "direction", dir);
+        get_vec3(c, "color", col);
+        if (!(dir[0] != 0 || dir[1] != 0 || dir[2] != 0)) error_at(c, "`direction` is zero");
+        for (int k = 0; k < 3; ++k) {
+            d.direction[k] = (float)dir[k];
+            d.color[k] = (float)col[k];
+        }
+        d.power = (float)get_double(c, "power", 1.0);
+        F.dir_lights.push_back(d);
     }
 
     // :5104 and Geometry/TriangleMeshLoaderRAW2.cpp: "nverts ntris", then
@@ -795,6 +814,7 @@ struct Builder {
             else if (k == "dielectric_material") dielectric(c);
             else if (k == "homogeneous_medium") medium(c);
             else if (k == "hosek_wilkie_skylight") sky(c);
+            else if (k == "directional_light") directional_light(c);
             else if (k == "rawmesh2_geometry") rawmesh2(c);
             else if (k == "clippedplane_geometry") clippedplane(c);
             else if (k == "risemesh_geometry") risemesh(c);
@@ -854,6 +874,16 @@ extern "C" rise_hip_status rise_hip_scene_file_get_info(const rise_hip_scene_fil
     out->has_sky = f->has_sky ? 1 : 0;
     out->n_skipped_chunks = f->skipped;
     out->n_outputs = (int32_t)f->outputs.size();
+    out->n_directional_lights = (int32_t)f->dir_lights.size();
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_scene_file_get_directional_lights(const rise_hip_scene_file* f,
+                                                                      const rise_hip_directional_light_desc** lights)
+{
+    clear_error();
+    if (!f || !lights) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    *lights = f->dir_lights.data();
     return RISE_HIP_OK;
 }
 
@@ -908,8 +938,10 @@ extern "C" rise_hip_status rise_hip_scene_file_upload(const rise_hip_scene_file*
                                          f->materials.size(), f->spds.data(), f->spds.size(), f->media.data(),
                                          f->media.size(), f->any_media ? f->object_media.data() : nullptr,
                                          trees.data(), out_scene);
-    if (st != RISE_HIP_OK || !f->has_sky) return st;
-    st = rise_hip_pt_scene_set_sky(*out_scene, &f->sky);
+    if (st != RISE_HIP_OK) return st;
+    if (!f->dir_lights.empty())
+        st = rise_hip_pt_scene_set_directional_lights(*out_scene, f->dir_lights.data(), f->dir_lights.size());
+    if (st == RISE_HIP_OK && f->has_sky) st = rise_hip_pt_scene_set_sky(*out_scene, &f->sky);
     if (st != RISE_HIP_OK) {
         rise_hip_pt_scene_release(*out_scene);
         *out_scene = nullptr;

@@ -155,11 +155,27 @@ public:
         return (int)materials_.size() - 1;
     }
 
-    // Job::AddHosekWilkieSkylight without the sun light; applied at PrepareForRendering.
+    // Job::AddHosekWilkieSkylight (Job.cpp:2717-2750): the sky map and, when
+    // sky.create_sun is set, its matched sun light of power
+    // sky.sun_intensity_scale; applied at PrepareForRendering.
     void SetSky(const rise_hip_sky_desc& sky)
     {
         sky_ = sky;
         has_sky_ = true;
+    }
+
+    // Job::AddDirectionalLight (Job.cpp:5424): light arriving from `direction`
+    // (toward the light) with RGB colour times power; at most 4 with the sky's
+    // sun.  Applied at PrepareForRendering.
+    void AddDirectionalLight(double power, const double color[3], const double direction[3])
+    {
+        rise_hip_directional_light_desc d{};
+        for (int c = 0; c < 3; ++c) {
+            d.color[c] = (float)color[c];
+            d.direction[c] = (float)direction[c];
+        }
+        d.power = (float)power;
+        dir_lights_.push_back(d);
     }
 
     // Job::AddHomogeneousMedium with `phase hg g`: RGB absorption and scattering per unit length.
@@ -195,6 +211,10 @@ public:
                                                        media_.size(), use_media_ ? object_media_.data() : nullptr,
                                                        &scene_),
                    "PrepareForRendering"))
+            return false;
+        if (!dir_lights_.empty() &&
+            !check(rise_hip_pt_scene_set_directional_lights(scene_, dir_lights_.data(), dir_lights_.size()),
+                   "PrepareForRendering (directional lights)"))
             return false;
         return !has_sky_ || check(rise_hip_pt_scene_set_sky(scene_, &sky_), "PrepareForRendering (sky)");
     }
@@ -252,6 +272,7 @@ private:
     bool use_media_ = false;
     rise_hip_sky_desc sky_{};
     bool has_sky_ = false;
+    std::vector<rise_hip_directional_light_desc> dir_lights_;
     rise_hip_pt_scene* scene_ = nullptr;
 };
 

@@ -110,13 +110,26 @@ class Camera:
 
 @dataclass
 class Sky:
-    """The spectral sky radiance map of ``hosek_wilkie_skylight``
-    (HosekWilkieSpectralRadianceMap; the reference's model is Preetham 1999):
-    sun elevation / azimuth in degrees, turbidity, radiance scale."""
+    """``hosek_wilkie_skylight``: the spectral sky radiance map
+    (HosekWilkieSpectralRadianceMap; the reference's model is Preetham 1999) --
+    sun elevation / azimuth in degrees, turbidity, radiance scale -- sampled by
+    next-event estimation, and its matched directional sun light (create_sun,
+    colour (1, 1, 1), power sun_intensity_scale), as the chunk's defaults have it."""
     elevation_deg: float = 45.0
     azimuth_deg: float = 0.0
     turbidity: float = 3.0
     scale: float = 1.0
+    create_sun: bool = True
+    sun_power: float = 3.14
+
+
+@dataclass
+class DirectionalLight:
+    """``directional_light``: light arriving from `direction` (toward the
+    light) with RGB `color` times `power`."""
+    direction: tuple
+    color: tuple = (1.0, 1.0, 1.0)
+    power: float = 1.0
 
 
 @dataclass
@@ -144,6 +157,7 @@ class Scene:
     camera: Camera | None = None
     sky: Sky | None = None
     media: list = field(default_factory=list)
+    lights: list = field(default_factory=list)   # DirectionalLight
 
     def flat(self):
         """All triangles (n, 9) float32 and their object index (n,) int32."""
@@ -494,6 +508,29 @@ def heightfield(nx: int, ny: int, extent=(20.0, 10.0), amplitude: float = 0.2, s
     t1 = np.concatenate([p00, p01, p10], -1).reshape(-1, 9)
     t2 = np.concatenate([p10, p01, p11], -1).reshape(-1, 9)
     return np.ascontiguousarray(np.concatenate([t1, t2], 0))
+
+
+def sky_plane(albedo=0.5, elevation_deg: float = 40.0, azimuth_deg: float = 30.0, turbidity: float = 3.0,
+              scale: float = 0.1, create_sun: bool = True, sun_power: float = 3.14, lights=()):
+    """A Lambertian ground quad (100 m square, `albedo` a number or an Spd)
+    under the spectral sky, seen straight down from 1 m through a narrow
+    lens, so every camera ray meets the ground and every light path leaves
+    it for the sky: the pixel radiance is albedo / pi times the ground's
+    irradiance from the sky, the sun and `lights` (DirectionalLight), a
+    closed form that checks the environment light sample, its MIS with the
+    BSDF sample that escapes, and the directional lights' deterministic
+    sample."""
+    s = Scene()
+    s.spds = [albedo if isinstance(albedo, Spd) else Spd.flat(albedo)]
+    s.materials = [Material(0)]
+    s.objects = [Object(quad((-50, 0, -50), (-50, 0, 50), (50, 0, 50), (50, 0, -50)), 0)]
+    s.camera = Camera(position=(0.0, 1.0, 0.0), look_at=(0.0, 0.0, 0.0), up=(0.0, 0.0, -1.0), fov_deg=20.0)
+    s.sky = Sky(elevation_deg=elevation_deg, azimuth_deg=azimuth_deg, turbidity=turbidity, scale=scale,
+                create_sun=create_sun, sun_power=sun_power)
+    s.lights = list(lights)
+    r = RenderSettings(width=16, height=16, spp=64, max_diffuse_bounce=4, max_glossy_bounce=0,
+                       max_transmission_bounce=0)
+    return s, r
 
 
 def terrain_spheres(nx: int = 2048, ny: int = 1024, subdiv: int = 4, seed: int = 7, n_side: int = 8):

@@ -67,7 +67,13 @@ class SkyDesc(ctypes.Structure):
         ("solar_azimuth_deg", ctypes.c_float),
         ("turbidity", ctypes.c_float),
         ("sky_scale", ctypes.c_float),
+        ("create_sun", ctypes.c_int32),
+        ("sun_intensity_scale", ctypes.c_float),
     ]
+
+
+class DirectionalLightDesc(ctypes.Structure):
+    _fields_ = [("direction", ctypes.c_float * 3), ("color", ctypes.c_float * 3), ("power", ctypes.c_float)]
 
 
 class CameraDesc(ctypes.Structure):
@@ -158,6 +164,8 @@ _SIG = {
                                              ctypes.c_size_t, _P, ctypes.POINTER(_P)], ctypes.c_int),
     "rise_hip_pt_scene_release": ([_P], ctypes.c_int),
     "rise_hip_pt_scene_set_sky": ([_P, ctypes.POINTER(SkyDesc)], ctypes.c_int),
+    "rise_hip_pt_scene_set_directional_lights": ([_P, ctypes.POINTER(DirectionalLightDesc), ctypes.c_size_t],
+                                                 ctypes.c_int),
     "rise_hip_render": ([_P, ctypes.POINTER(RenderDesc), _P, _P, ctypes.POINTER(RenderStats), _P], ctypes.c_int),
     "rise_hip_batch_create": ([_P, ctypes.c_size_t, ctypes.POINTER(_P)], ctypes.c_int),
     "rise_hip_batch_create_tuned": ([_P, ctypes.c_size_t, ctypes.POINTER(BatchTuning), ctypes.POINTER(_P)],
@@ -261,8 +269,16 @@ class PathTracingSpectralRasterizer:
             _lib.check(L.rise_hip_pt_scene_create(objs, len(scene.objects), mats, len(scene.materials), spds,
                                                   len(scene.spds), ctypes.byref(out)))
         self._h = out
+        if scene.lights:
+            dl = (DirectionalLightDesc * len(scene.lights))()
+            for k, l in enumerate(scene.lights):
+                dl[k].direction[:] = l.direction
+                dl[k].color[:] = l.color
+                dl[k].power = l.power
+            _lib.check(L.rise_hip_pt_scene_set_directional_lights(self._h, dl, len(scene.lights)))
         if scene.sky is not None:
-            sd = SkyDesc(scene.sky.elevation_deg, scene.sky.azimuth_deg, scene.sky.turbidity, scene.sky.scale)
+            k = scene.sky
+            sd = SkyDesc(k.elevation_deg, k.azimuth_deg, k.turbidity, k.scale, 1 if k.create_sun else 0, k.sun_power)
             _lib.check(L.rise_hip_pt_scene_set_sky(self._h, ctypes.byref(sd)))
         self._keep = []
 

@@ -17,10 +17,10 @@ import os
 import numpy as np
 
 from . import _lib
-from .pt_scene import Camera, Material, Medium, Object, RenderSettings, Scene, Sky, Spd
+from .pt_scene import Camera, DirectionalLight, Material, Medium, Object, RenderSettings, Scene, Sky, Spd
 from .output import OutputDesc, extension
-from .render import (MaterialDesc, MediumDesc, ObjectDesc, ObjectMedia, PathTracingSpectralRasterizer, RenderDesc,
-                     SkyDesc, SpdDesc)
+from .render import (DirectionalLightDesc, MaterialDesc, MediumDesc, ObjectDesc, ObjectMedia,
+                     PathTracingSpectralRasterizer, RenderDesc, SkyDesc, SpdDesc)
 
 
 class SceneFileInfo(ctypes.Structure):
@@ -33,11 +33,11 @@ class SceneFileInfo(ctypes.Structure):
         ("has_sky", ctypes.c_int32),
         ("n_skipped_chunks", ctypes.c_int32),
         ("n_outputs", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("n_directional_lights", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _P = ctypes.c_void_p
@@ -50,6 +50,7 @@ _SIG = {
     "rise_hip_scene_file_upload": ([_P, _PP], ctypes.c_int),
     "rise_hip_scene_file_get_output": ([_P, ctypes.c_int32, ctypes.POINTER(OutputDesc),
                                         ctypes.POINTER(ctypes.c_char_p)], ctypes.c_int),
+    "rise_hip_scene_file_get_directional_lights": ([_P, _PP], ctypes.c_int),
     "rise_hip_scene_file_release": ([_P], ctypes.c_int),
 }
 
@@ -123,7 +124,14 @@ class SceneFile:
             s.media.append(Medium(tuple(_f(x) for x in m.sigma_a), tuple(_f(x) for x in m.sigma_s), _f(m.hg_g)))
         if p[5].value:
             sd = ctypes.cast(p[5], ctypes.POINTER(SkyDesc)).contents
-            s.sky = Sky(_f(sd.solar_elevation_deg), _f(sd.solar_azimuth_deg), _f(sd.turbidity), _f(sd.sky_scale))
+            s.sky = Sky(_f(sd.solar_elevation_deg), _f(sd.solar_azimuth_deg), _f(sd.turbidity), _f(sd.sky_scale),
+                        create_sun=bool(sd.create_sun), sun_power=_f(sd.sun_intensity_scale))
+        if info["n_directional_lights"]:
+            lp = _P(None)
+            _lib.check(_lib_scene_file().rise_hip_scene_file_get_directional_lights(self._h, ctypes.byref(lp)))
+            for d in _array(lp, DirectionalLightDesc, info["n_directional_lights"]):
+                s.lights.append(DirectionalLight(tuple(_f(x) for x in d.direction), tuple(_f(x) for x in d.color),
+                                                 _f(d.power)))
         c = self.render_desc().camera
         s.camera = Camera(tuple(_f(x) for x in c.position), tuple(_f(x) for x in c.look_at), tuple(_f(x) for x in c.up),
                           _f(c.fov_deg))

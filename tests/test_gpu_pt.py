@@ -86,6 +86,30 @@ def test_gpu_spectral_furnace(cuda):
     assert abs(pt_closed_forms.furnace_y_hero_only(rho, le) - want) > 12 * se
 
 
+@pytest.mark.parametrize("case", range(4))
+def test_gpu_sky_plane(cuda, case):
+    """A Lambertian ground under the sky, its sun and directional lights: the
+    closed form (tests/pt_closed_forms.py) and the oracle on the same sample
+    streams."""
+    from tests.test_oracle_pt import SKY_PLANE_CASES
+
+    kw = SKY_PLANE_CASES[case]
+    log_case(f"sky plane {kw}")
+    s, r = pt_scene.sky_plane(**kw)
+    want = pt_closed_forms.sky_plane_y(s, n_mu=96, n_phi=192)
+    r.spp = 1024
+    g, gq, _ = _render(s, r)
+    n = r.width * r.height * r.spp
+    y = g[..., 1].mean()
+    se = math.sqrt(max(gq[..., 1].mean() - y * y, 0) / n)
+    assert abs(y - want) < 4 * se + 1e-3 * want, (y, want, se)
+    r.spp = 16
+    g, _, _ = _render(s, r)
+    sm, _, _ = oracle.render(s, r)
+    frac, mean_rel = _compare_same_streams(g, sm / r.spp)
+    assert frac >= 0.99 and mean_rel < 1e-3, (frac, mean_rel)
+
+
 def test_gpu_emitter_seen_directly(cuda):
     log_case("emitter seen directly")
     s, r = pt_scene.cornell_box()

@@ -89,6 +89,31 @@ def test_spectral_furnace_closed_form():
     assert abs(bug - want) > 12 * se
 
 
+SKY_PLANE_CASES = [
+    dict(create_sun=False),
+    dict(create_sun=True),
+    dict(create_sun=True, lights=[pt_scene.DirectionalLight((1.0, 2.0, -0.5), (0.2, 0.9, 0.4), 1.5),
+                                  pt_scene.DirectionalLight((0.3, -1.0, 0.2), (1.0, 1.0, 1.0), 9.0)]),
+    dict(create_sun=False, elevation_deg=8.0, azimuth_deg=250.0, turbidity=6.0),
+]
+
+
+@pytest.mark.parametrize("kw", SKY_PLANE_CASES)
+def test_sky_plane_closed_form(kw):
+    """A Lambertian ground under the sky: albedo / pi times the ground's
+    irradiance from the sky (hemisphere quadrature of the pinned sky model),
+    the sun (create_sun) and directional lights (one below the horizon, which
+    must add nothing).  Checks the environment light sample, its MIS pair with
+    an escaping BSDF ray (dropping the escape weight would double the sky's
+    share) and the directional lights' cosine and luminance."""
+    s, r = pt_scene.sky_plane(**kw)
+    want = pt_closed_forms.sky_plane_y(s, n_mu=96, n_phi=192)
+    r.spp = 256
+    sm, sq, _ = oracle.render(s, r)
+    y, se = _y_stats(sm, sq, r.width * r.height * r.spp)
+    assert abs(y - want) < 4 * se + 1e-3 * want, (y, want, se)
+
+
 def test_unshuffled_sobol_is_biased_on_the_spectral_furnace():
     """The reference's padded Sobol draws every dimension pair at the same
     index (SobolSequence.h:180-200), so dimensions 0, 2, 4, ... are hash

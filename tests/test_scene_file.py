@@ -73,7 +73,7 @@ def test_materials_media_sky_and_settings():
     f = SceneFile(SCENES / "features.RISEscene")
     info = f.info()
     assert info == {"n_objects": 4, "n_materials": 4, "n_spds": 5, "n_media": 2, "n_tris": 38, "has_sky": 1,
-                    "n_skipped_chunks": 1, "n_outputs": 1}
+                    "n_skipped_chunks": 1, "n_outputs": 1, "n_directional_lights": 0}
     (d, name), = f.outputs()
     # type EXR; bpp, colour space, display transform and alpha at the parser's defaults
     assert name == "rendered/features.exr"
@@ -88,7 +88,7 @@ def test_materials_media_sky_and_settings():
     fog, haze = s.media
     assert fog.sigma_a == pytest.approx((0.05,) * 3) and fog.sigma_s == pytest.approx((0.5, 0.4, 0.3))
     assert fog.g == pytest.approx(0.3) and haze.g == 0.0
-    assert s.sky == pt_scene.Sky(30.0, 120.0, 4.0, 0.5)
+    assert s.sky == pt_scene.Sky(30.0, 120.0, 4.0, 0.5, create_sun=False)
     assert [(o.medium, o.casts_shadows, o.double_sided) for o in s.objects] == [
         (-1, True, True), (-1, True, False), (-1, True, False), (0, False, False)]
     assert s.camera == pt_scene.Camera((0.0, 1.0, 4.0), (0.0, 0.5, 0.0), (0.0, 1.0, 0.0), 45.0)
@@ -165,7 +165,7 @@ pinhole_camera
 @pytest.mark.parametrize("extra, needle", [
     ("bogus_chunk\n{\n\tname x\n}\n", "bogus_chunk (line 24)"),
     ("uniformcolor_painter\n{\n\tname c\n\tcolor 1 0 0\n\tcolorspace ROMMRGB_Linear\n}\n", "colorspace"),
-    ("hosek_wilkie_skylight\n{\n\tturbidity 3\n}\n", "create_sun FALSE"),
+    ("directional_light\n{\n\tpower 2\n}\n", "`direction` is zero"),
     ("dielectric_material\n{\n\tname d\n\tior 1.5\n}\n", "smooth dielectrics"),
     ("ggx_material\n{\n\tname g\n\trs 1\n\talphax 0.1\n\talphay 0.3\n}\n", "anisotropic"),
     ("lambertian_material\n{\n\tname e\n\treflectance expr(0.5)\n}\n", "expr"),
@@ -195,6 +195,21 @@ def test_minimal_scene_defaults(tmp_path):
     assert (r.rr_min_depth, r.seed, r.sky_background) == (3, 1, True)
     assert s.camera == pt_scene.Camera((0.0, 0.0, 5.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 30.0)
     assert s.objects[0].double_sided  # clipped planes default to double-sided
+
+
+def test_sun_and_directional_lights(tmp_path):
+    # hosek_wilkie_skylight's sun at its defaults (create_sun TRUE, power
+    # sun_intensity_scale 3.14) and directional_light chunks in file order
+    p = tmp_path / "sun.RISEscene"
+    p.write_text(BASE + "hosek_wilkie_skylight\n{\n\tsolar_elevation 60\n}\n"
+                 "directional_light\n{\n\tdirection 0 1 1\n\tcolor 1 0.5 0.25\n\tpower 2\n}\n"
+                 "directional_light\n{\n\tdirection 1 0 0\n}\n")
+    f = SceneFile(p)
+    assert f.info()["n_directional_lights"] == 2
+    s = f.scene()
+    assert s.sky == pt_scene.Sky(60.0, 0.0, 3.0, 1.0, create_sun=True, sun_power=pytest.approx(3.14))
+    assert s.lights == [pt_scene.DirectionalLight((0.0, 1.0, 1.0), (1.0, 0.5, 0.25), 2.0),
+                        pt_scene.DirectionalLight((1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 1.0)]
 
 
 def test_missing_header_and_file(tmp_path):
