@@ -135,6 +135,19 @@ rise_hip_status rise_hip_intersect_closest(rise_hip_scene* scene, const float* r
 rise_hip_status rise_hip_intersect_any(rise_hip_scene* scene, const float* rays_soa, const float* tmax,
                                        int64_t n, uint8_t* occluded, hipStream_t stream);
 
+/* rise_hip_intersect_closest that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as rise_hip_intersect_closest; the
+ * persistent kernel's ray-fetch counter lives in a file-scope device array of
+ * 256 bytes (one per GPU, held until the process ends) instead of the scene's
+ * counter slots.  The counter is one 8-byte word whatever n is, so no ray
+ * count exceeds the scratch.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (two streams, or two host threads) race on it, and
+ * the caller must serialize them. */
+rise_hip_status rise_hip_intersect_closest_noalloc(rise_hip_scene* scene, const float* rays_soa, int64_t n,
+                                                   int32_t* prim_id, float* t, hipStream_t stream);
+
 /* As rise_hip_intersect_closest / _any, and also accumulates into
  * counters[0] the 8-wide nodes visited and into counters[1] the triangles
  * certified (counters: device, 2 uint64, not cleared by the call).  Used to

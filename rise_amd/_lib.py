@@ -73,6 +73,7 @@ _SIGNATURES = {
     "rise_hip_scene_set_arithmetic": ([_P, ctypes.c_int], ctypes.c_int),
     "rise_hip_intersect_closest": ([_P, _P, _I64, _P, _P, _P], ctypes.c_int),
     "rise_hip_intersect_any": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
+    "rise_hip_intersect_closest_noalloc": ([_P, _P, _I64, _P, _P, _P], ctypes.c_int),
     "rise_hip_intersect_closest_counted": ([_P, _P, _I64, _P, _P, _P, _P], ctypes.c_int),
     "rise_hip_intersect_any_counted": ([_P, _P, _P, _I64, _P, _P, _P], ctypes.c_int),
     "rise_hip_debug_trace_host": (

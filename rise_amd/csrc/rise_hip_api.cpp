@@ -36,7 +36,11 @@ struct rise_hip_scene {
     TraceScene view()
     {
         const unsigned k = next_slot.fetch_add(1u) % kFetchSlots;
-        return TraceScene{d_nodes, d_tris, stats.depth, d_fetch + k, arith.load()};
+        return view_on(d_fetch + k);
+    }
+    TraceScene view_on(unsigned long long* fetch_counter) const
+    {
+        return TraceScene{d_nodes, d_tris, stats.depth, fetch_counter, arith.load()};
     }
 };
 
@@ -236,6 +240,22 @@ rise_hip_status rise_hip_intersect_any(rise_hip_scene* scene, const float* rays_
                                        uint8_t* occluded, hipStream_t stream)
 {
     return any_impl(scene, rays_soa, tmax, n, occluded, nullptr, stream);
+}
+
+rise_hip_status rise_hip_intersect_closest_noalloc(rise_hip_scene* scene, const float* rays_soa, int64_t n,
+                                                   int32_t* prim_id, float* t, hipStream_t stream)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n == 0) return RISE_HIP_OK;
+    if (!rays_soa || !prim_id || !t) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    unsigned long long* counter = nullptr;
+    hipError_t e = noalloc_scratch(&counter);
+    if (e != hipSuccess) return fail_hip(e, "scratch address (hipGetSymbolAddress)");
+    e = launch_closest(scene->view_on(counter), rays_soa, n, prim_id, t, nullptr, stream);
+    if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
+    return RISE_HIP_OK;
 }
 
 rise_hip_status rise_hip_intersect_closest_counted(rise_hip_scene* scene, const float* rays_soa, int64_t n,

@@ -26,4 +26,8 @@ hipError_t launch_closest(const TraceScene& s, const float* rays, int64_t n, int
 hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax, int64_t n, uint8_t* occluded,
                       unsigned long long* counters, hipStream_t stream);
 
+// Device address (on the current device) of the allocation-free query's
+// scratch counter; looked up once per device with hipGetSymbolAddress.
+hipError_t noalloc_scratch(unsigned long long** out);
+
 }  // namespace rise_hip

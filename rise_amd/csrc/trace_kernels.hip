@@ -11,6 +11,7 @@
 #include "wave_fetch.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace rise_hip {
 
@@ -293,6 +294,39 @@ hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax,
 #undef RISE_LAUNCH_ANY_A
 #undef RISE_LAUNCH_ANY
     return hipGetLastError();
+}
+
+// Scratch of the allocation-free closest-hit query
+// (rise_hip_intersect_closest_noalloc): word 0 is the persistent kernel's
+// ray-fetch counter, cleared by a kernel at the start of every call.  A fixed
+// 256 bytes per device whatever the ray count, held until the process ends.
+// Host code never passes `scratch` itself to a kernel: it is a host-side
+// stand-in; noalloc_scratch() returns the device's own address.
+constexpr int kScratchWords = 32;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(unsigned long long** out)
+{
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<unsigned long long*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 }  // namespace rise_hip
