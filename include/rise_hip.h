@@ -135,6 +135,46 @@ rise_hip_status rise_hip_intersect_closest(rise_hip_scene* scene, const float* r
 rise_hip_status rise_hip_intersect_any(rise_hip_scene* scene, const float* rays_soa, const float* tmax,
                                        int64_t n, uint8_t* occluded, hipStream_t stream);
 
+/* Face selection of IObjectManager::IntersectRay / IntersectShadowRay
+ * (bHitFrontFaces, bHitBackFaces; Interfaces/IObjectManager.h:37-50): a
+ * triangle is front-facing when its face normal (v1 - v0) x (v2 - v0) points
+ * against the ray direction (TriangleMeshGeometryIndexedSpecializations.h:
+ * 141-159); edge-on triangles count as both.  The selection applies to every
+ * mesh of the scene (a RISE mesh marked double-sided ignores it). */
+typedef enum rise_hip_faces {
+    RISE_HIP_FACES_NONE = 0,
+    RISE_HIP_FACES_FRONT = 1,
+    RISE_HIP_FACES_BACK = 2,
+    RISE_HIP_FACES_BOTH = 3
+} rise_hip_faces;
+
+/* The intersection record RISE's RayIntersectionGeometric holds after
+ * Object::IntersectRay (Intersection/RayIntersectionGeometric.h:89-188;
+ * Objects/Object.cpp:528-649) for a mesh without vertex normals or texture
+ * coordinates under an identity transform. */
+typedef struct rise_hip_hit {
+    int32_t prim_id;        /* closest triangle id (as rise_hip_intersect_closest), -1 on a miss */
+    int32_t mesh;           /* index of the triangle's mesh in rise_hip_scene_upload's list, -1 on a miss */
+    double range;           /* world distance (ri.range after Object.cpp:641-643), +inf on a miss */
+    double alpha, beta;     /* barycentrics of the double-precision certifier (RayTriangleIntersection.cpp) */
+    float point[3];         /* world hit point o + dn (t - 1e-12) (ri.ptIntersection) */
+    float geom_normal[3];   /* unit face normal (ri.vGeomNormal; equal to ri.vNormal for these meshes) */
+} rise_hip_hit;
+
+/* rise_hip_intersect_closest with a face selection (faces: rise_hip_faces)
+ * and the full intersection record per ray.  hits: device, n records.  The
+ * hit decision is the same as rise_hip_intersect_closest's (bit for bit) when
+ * faces is RISE_HIP_FACES_BOTH.  Replaces IObjectManager::IntersectRay
+ * (Interfaces/IObjectManager.h:37-43) with its face flags. */
+rise_hip_status rise_hip_intersect_closest_ex(rise_hip_scene* scene, const float* rays_soa, int64_t n, int32_t faces,
+                                              rise_hip_hit* hits, hipStream_t stream);
+
+/* rise_hip_intersect_any with a face selection.  Replaces
+ * IObjectManager::IntersectShadowRay (Interfaces/IObjectManager.h:45-50) with
+ * its face flags. */
+rise_hip_status rise_hip_intersect_any_ex(rise_hip_scene* scene, const float* rays_soa, const float* tmax, int64_t n,
+                                          int32_t faces, uint8_t* occluded, hipStream_t stream);
+
 /* rise_hip_intersect_closest that allocates nothing.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -37,6 +37,22 @@ rise_hip_status rise_hip_debug_trace_host_risemesh(const rise_hip_risemesh* mesh
                                                    float* t, uint8_t* occluded, uint64_t* counters,
                                                    rise_hip_scene_stats* stats);
 
+/* rise_hip_debug_trace_host with the face selection and hit records of
+ * rise_hip_intersect_closest_ex / rise_hip_intersect_any_ex (mode bits as
+ * above; faces: rise_hip_faces).  hits (closest) or occluded (any): host, n
+ * entries.  The triangles form one mesh (mesh index 0). */
+rise_hip_status rise_hip_debug_trace_ex_host(const float* tris, int64_t n_tris, const float* rays_soa,
+                                             const float* tmax, int64_t n, int32_t mode, int32_t faces,
+                                             rise_hip_hit* hits, uint8_t* occluded);
+
+/* The kernels' FP32 slab test of one 8-wide node (80 bytes, the layout the
+ * builder writes) against one ray (ox, oy, oz, dx, dy, dz), on the host:
+ * bit s of *accept set when slot s is occupied and its box is taken for the
+ * parameter range up to `best` (a distance along the normalized direction;
+ * >= 1.7976931348623158e308 for no limit). */
+rise_hip_status rise_hip_debug_node_accept_host(const void* node, const float ray[6], double best,
+                                                uint32_t* accept);
+
 #ifdef __cplusplus
 }
 #endif

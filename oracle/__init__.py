@@ -51,6 +51,7 @@ def oracle_lib() -> ctypes.CDLL:
         L.rise_oracle_closest_brute.argtypes = [_P, _I64, _P, _I64, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int]
         L.rise_oracle_any_brute.argtypes = [_P, _I64, _P, _P, _I64, _P, ctypes.c_int, ctypes.c_int]
         L.rise_oracle_filter_float.argtypes = [_P] * 5 + [ctypes.c_float]
+        L.rise_oracle_closest_attr.argtypes = [_P, _I64, _P, _I64] + [_P] * 6 + [ctypes.c_int, ctypes.c_int]
         L.rise_oracle_filter_float.restype = ctypes.c_int
         _oracle = L
     return _oracle
@@ -70,6 +71,13 @@ def ray_triangle(o, d, v0, eA, eB):
 
 NO_FILTER = 1
 X86_FMA = 2
+NO_FRONT = 4
+NO_BACK = 8
+
+
+def _faces_opts(faces: int) -> int:
+    """Face selection (bit 0 front, bit 1 back) as the oracle's opts bits."""
+    return (0 if faces & 1 else NO_FRONT) | (0 if faces & 2 else NO_BACK)
 
 
 def _opts(no_filter: bool, arith: str) -> int:
@@ -79,7 +87,7 @@ def _opts(no_filter: bool, arith: str) -> int:
 
 
 def closest(tris: np.ndarray, rays: np.ndarray, threads: int = 0, flags: bool = False, no_filter: bool = False,
-            arith: str = "ieee"):
+            arith: str = "ieee", faces: int = 3):
     """Brute-force closest hit.  Returns (prim_id int32, t float32, t64 float64),
     plus, with flags=True, a uint8 array marking rays whose result depends on
     traversal order (the winner fails the float filter's distance test against
@@ -96,12 +104,12 @@ def closest(tris: np.ndarray, rays: np.ndarray, threads: int = 0, flags: bool = 
     fl = np.empty(n, dtype=np.uint8) if flags else None
     L.rise_oracle_closest_brute(tris.ctypes.data, tris.shape[0], rays.ctypes.data, n, pid.ctypes.data,
                                 t.ctypes.data, t64.ctypes.data, None if fl is None else fl.ctypes.data,
-                                _opts(no_filter, arith), threads)
+                                _opts(no_filter, arith) | _faces_opts(faces), threads)
     return (pid, t, t64, fl) if flags else (pid, t, t64)
 
 
 def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, threads: int = 0,
-            no_filter: bool = False, arith: str = "ieee") -> np.ndarray:
+            no_filter: bool = False, arith: str = "ieee", faces: int = 3) -> np.ndarray:
     L = oracle_lib()
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -109,9 +117,25 @@ def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, 
     occ = np.empty(n, dtype=np.uint8)
     tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
     L.rise_oracle_any_brute(tris.ctypes.data, tris.shape[0], rays.ctypes.data,
-                            None if tm is None else tm.ctypes.data, n, occ.ctypes.data, _opts(no_filter, arith),
-                            threads)
+                            None if tm is None else tm.ctypes.data, n, occ.ctypes.data,
+                            _opts(no_filter, arith) | _faces_opts(faces), threads)
     return occ
+
+
+def closest_attr(tris: np.ndarray, rays: np.ndarray, faces: int = 3, arith: str = "ieee", threads: int = 0) -> dict:
+    """Brute-force closest hit with the intersection record
+    (rise_oracle_closest_attr): prim_id, range, alpha, beta, normal (n, 3),
+    point (n, 3), all float64 except the ids."""
+    L = oracle_lib()
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    out = {"prim_id": np.empty(n, np.int32), "range": np.empty(n), "alpha": np.empty(n), "beta": np.empty(n),
+           "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
+    L.rise_oracle_closest_attr(tris.ctypes.data, tris.shape[0], rays.ctypes.data, n,
+                               *[out[k].ctypes.data for k in ("prim_id", "range", "alpha", "beta", "normal", "point")],
+                               _opts(False, arith) | _faces_opts(faces), threads)
+    return out
 
 
 def filter_float(o, d, p0, e1, e2, best: float) -> bool:
@@ -125,9 +149,12 @@ class RefMesh:
     """The reference's TriangleMeshGeometryIndexed (BVH4 + float filter +
     double certifier) built from its own sources (oracle/_ref)."""
 
-    def __init__(self, tris: np.ndarray | None, variant: str = "ieee", risemesh: str | None = None):
+    def __init__(self, tris: np.ndarray | None, variant: str = "ieee", risemesh: str | None = None,
+                 bary: bool = False):
         """Builds the mesh from `tris`, or (tris None) loads the .risemesh file
-        `risemesh` with the reference's Deserialize."""
+        `risemesh` with the reference's Deserialize.  bary: vertex normals
+        that make the interpolated shading normal carry the barycentrics
+        (rise_ref_mesh_create_bary)."""
         name = "libriseref_ieee.so" if variant == "ieee" else "libriseref.so"
         L = _load(REF_DIR / name)
         L.rise_ref_mesh_create.argtypes = [_P, _I64]
@@ -141,13 +168,18 @@ class RefMesh:
         L.rise_ref_risemesh_load.restype = _P
         L.rise_ref_bvh2_nodes.argtypes = [_P]
         L.rise_ref_bvh2_nodes.restype = ctypes.c_uint
+        L.rise_ref_mesh_create_bary.argtypes = [_P, _I64]
+        L.rise_ref_mesh_create_bary.restype = _P
+        L.rise_ref_closest_ex.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int] + [_P] * 6 + [ctypes.c_int]
+        L.rise_ref_any_ex.argtypes = [_P, _P, _P, _I64, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int]
         self._L = L
         if tris is None:
             self._tris = None
             self._h = L.rise_ref_risemesh_load(str(risemesh).encode())
         else:
             self._tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
-            self._h = L.rise_ref_mesh_create(self._tris.ctypes.data, self._tris.shape[0])
+            create = L.rise_ref_mesh_create_bary if bary else L.rise_ref_mesh_create
+            self._h = create(self._tris.ctypes.data, self._tris.shape[0])
 
     def loaded_bvh2_nodes(self) -> int:
         """BVH2 node count of a mesh loaded from a file (its cache's count when
@@ -176,6 +208,28 @@ class RefMesh:
         tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
         self._L.rise_ref_any(self._h, rays.ctypes.data, None if tm is None else tm.ctypes.data, n,
                              occ.ctypes.data, threads)
+        return occ
+
+    def closest_ex(self, rays: np.ndarray, faces: int = 3, threads: int = 0) -> dict:
+        """IntersectRay with the face flags: prim_id, range, alpha, beta
+        (meaningful on a bary mesh), normal (n, 3; the geometric normal, on a
+        plain mesh), point (n, 3)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[1]
+        out = {"prim_id": np.empty(n, np.int32), "range": np.empty(n), "alpha": np.empty(n), "beta": np.empty(n),
+               "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
+        self._L.rise_ref_closest_ex(self._h, rays.ctypes.data, n, faces & 1, (faces >> 1) & 1,
+                                    *[out[k].ctypes.data for k in ("prim_id", "range", "alpha", "beta", "normal",
+                                                                   "point")], threads)
+        return out
+
+    def any_ex(self, rays: np.ndarray, tmax: np.ndarray | None = None, faces: int = 3, threads: int = 0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[1]
+        occ = np.empty(n, dtype=np.uint8)
+        tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
+        self._L.rise_ref_any_ex(self._h, rays.ctypes.data, None if tm is None else tm.ctypes.data, n, faces & 1,
+                                (faces >> 1) & 1, occ.ctypes.data, threads)
         return occ
 
     def close(self) -> None:

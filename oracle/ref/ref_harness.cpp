@@ -208,6 +208,93 @@ void rise_ref_closest(void* handle, const float* rays, int64_t n, int32_t* prim_
     });
 }
 
+// rise_ref_mesh_create with the vertex normals (1,0,0), (0,1,0), (0,0,1) on
+// the three corners of every triangle: the shading normal the reference
+// interpolates at a hit, n0 + (n1 - n0) a + (n2 - n0) b
+// (...Specializations.h:187-191), is then (1 - a - b, a, b), whose y and z are
+// the certifier's barycentrics a and b exactly.  Hits and ranges are those of
+// the plain mesh.
+void* rise_ref_mesh_create_bary(const float* tris, int64_t n_tris)
+{
+    RefMesh* h = new RefMesh;
+    h->mesh = new Implementation::TriangleMeshGeometryIndexed(false, false);  // vertex normals kept
+    h->mesh->BeginIndexedTriangles();
+    h->mesh->AddNormal(Normal(1.0, 0.0, 0.0));
+    h->mesh->AddNormal(Normal(0.0, 1.0, 0.0));
+    h->mesh->AddNormal(Normal(0.0, 0.0, 1.0));
+    for (int64_t i = 0; i < n_tris; ++i) {
+        const float* f = tris + 9 * i;
+        for (int k = 0; k < 3; ++k)
+            h->mesh->AddVertex(Vertex((double)f[3 * k + 0], (double)f[3 * k + 1], (double)f[3 * k + 2]));
+        h->mesh->AddTexCoord(TexCoord((double)i, 0.0));
+        IndexedTriangle t;
+        for (int k = 0; k < 3; ++k) {
+            t.iVertices[k] = (unsigned int)(3 * i + k);
+            t.iNormals[k] = (unsigned int)k;
+            t.iCoords[k] = (unsigned int)i;
+        }
+        h->mesh->AddIndexedTriangle(t);
+    }
+    h->mesh->DoneIndexedTriangles();
+    return h;
+}
+
+// Closest hit with the face flags and the intersection record: id, world
+// range, ri.vNormal.y / .z (the barycentrics on a mesh from
+// rise_ref_mesh_create_bary), the geometric normal after Object::IntersectRay's
+// Normalize (Object.cpp:531-533; identity transform), and the world hit point
+// (Object.cpp:637-640).  normal and point: 3 doubles per ray.
+void rise_ref_closest_ex(void* handle, const float* rays, int64_t n, int front, int back, int32_t* prim_id,
+                         double* range, double* alpha, double* beta, double* normal, double* point, int n_threads)
+{
+    RefMesh* h = (RefMesh*)handle;
+    parallel_rays(n, n_threads, [&](int64_t i) {
+        const Point3 o(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        const Vector3 d(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        const Vector3 dn = Vector3Ops::Normalize(d);
+        const RasterizerState rs = {0, 0};
+        RayIntersectionGeometric ri(Ray(o, dn), rs);
+        h->mesh->IntersectRay(ri, front != 0, back != 0, false);
+        if (ri.bHit) {
+            const Point3 p = ri.ray.PointAtLength(ri.range - 1e-12);
+            prim_id[i] = (int32_t)ri.ptCoord.x;
+            range[i] = Vector3Ops::Magnitude(Vector3Ops::mkVector3(p, o));
+            alpha[i] = ri.vNormal.y;
+            beta[i] = ri.vNormal.z;
+            const Vector3 g = Vector3Ops::Normalize(ri.vGeomNormal);
+            normal[3 * i] = g.x;
+            normal[3 * i + 1] = g.y;
+            normal[3 * i + 2] = g.z;
+            point[3 * i] = p.x;
+            point[3 * i + 1] = p.y;
+            point[3 * i + 2] = p.z;
+        } else {
+            prim_id[i] = -1;
+            range[i] = INFINITY;
+            alpha[i] = beta[i] = 0.0;
+            for (int k = 0; k < 3; ++k) normal[3 * i + k] = point[3 * i + k] = 0.0;
+        }
+    });
+}
+
+// Any hit within tmax[i] with the face flags.
+void rise_ref_any_ex(void* handle, const float* rays, const float* tmax, int64_t n, int front, int back,
+                     uint8_t* occluded, int n_threads)
+{
+    RefMesh* h = (RefMesh*)handle;
+    parallel_rays(n, n_threads, [&](int64_t i) {
+        const Point3 o(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        const Vector3 d(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        const Scalar mag = Vector3Ops::Magnitude(d);
+        const Vector3 dn = Vector3Ops::Normalize(d);
+        const Scalar factor = (mag > NEARZERO) ? mag : Scalar(1.0);
+        const Scalar how_far = tmax ? (Scalar)tmax[i] : RISE_INFINITY;
+        Scalar how_far2 = how_far;
+        if ((how_far != RISE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
+        occluded[i] = h->mesh->IntersectRay_IntersectionOnly(Ray(o, dn), how_far2, front != 0, back != 0) ? 1 : 0;
+    });
+}
+
 // Any hit within tmax[i] (world distance; NULL = RISE_INFINITY).
 void rise_ref_any(void* handle, const float* rays, const float* tmax, int64_t n, uint8_t* occluded,
                   int n_threads)

@@ -180,6 +180,7 @@ typedef struct {
     double* t64;
     uint8_t* flags;
     uint8_t* occluded;
+    double *alpha, *beta, *normal, *point;
     int opts;
     int64_t next;
     pthread_mutex_t lock;
@@ -215,6 +216,23 @@ static void float_ray(const double o[3], const double dn[3], float fo[3], float 
     }
 }
 
+/* Face selection of TriangleMeshGeometryIndexed::RayElementIntersection
+ * (...Specializations.h:141-159): vFaceNormal = Cross(vEdgeA, vEdgeB) against
+ * the normalized object-space direction; Dot < 0 is a front face, skipped
+ * when bHitFrontFaces is false, Dot > 0 a back face, skipped when
+ * bHitBackFaces is false.  Returns 1 when the triangle is considered. */
+static int face_selected(int x, int opts, const tri64* t, const double dn[3])
+{
+    if (!(opts & (RISE_ORACLE_NO_FRONT | RISE_ORACLE_NO_BACK))) return 1;
+    const double n0 = sp64(x, t->eA[1], t->eB[2], t->eA[2], t->eB[1]);
+    const double n1 = sp64(x, t->eA[2], t->eB[0], t->eA[0], t->eB[2]);
+    const double n2 = sp64(x, t->eA[0], t->eB[1], t->eA[1], t->eB[0]);
+    const double dd = d364(x, n0, dn[0], n1, dn[1], n2, dn[2]);
+    if ((opts & RISE_ORACLE_NO_FRONT) && dd < 0.0) return 0;
+    if ((opts & RISE_ORACLE_NO_BACK) && dd > 0.0) return 0;
+    return 1;
+}
+
 static void* closest_worker(void* arg)
 {
     job* j = (job*)arg;
@@ -240,6 +258,7 @@ static void* closest_worker(void* arg)
                  * winner's own test is checked below. */
                 if (!nof && !filter_x(x, fo, fd, j->tris[k].p0, j->tris[k].e1, j->tris[k].e2, INFINITY))
                     continue;
+                if (!face_selected(x, j->opts, &j->tris[k], dn)) continue;
                 if (ray_triangle_x(x, o, dn, j->tris[k].v0, j->tris[k].eA, j->tris[k].eB, &tt, NULL, NULL)) {
                     /* Strict-closer commit (...Specializations.h:172); k
                      * ascends, so an exact tie keeps the lowest id. */
@@ -269,12 +288,34 @@ static void* closest_worker(void* arg)
                 const double vx = px - o[0], vy = py - o[1], vz = pz - o[2];
                 const double range = sqrt(d364(x, vx, vx, vy, vy, vz, vz));
                 j->prim_id[i] = (int32_t)best_id;
-                j->t[i] = (float)range;
+                if (j->t) j->t[i] = (float)range;
                 if (j->t64) j->t64[i] = range;
+                if (j->alpha) {
+                    /* RayElementIntersection (...Specializations.h:172-205) and
+                     * Object::IntersectRay (Object.cpp:531-533, 637-640). */
+                    const tri64* w = &j->tris[best_id];
+                    double tt, a, b;
+                    ray_triangle_x(x, o, dn, w->v0, w->eA, w->eB, &tt, &a, &b);
+                    j->alpha[i] = a;
+                    j->beta[i] = b;
+                    const double fn[3] = {sp64(x, w->eA[1], w->eB[2], w->eA[2], w->eB[1]),
+                                          sp64(x, w->eA[2], w->eB[0], w->eA[0], w->eB[2]),
+                                          sp64(x, w->eA[0], w->eB[1], w->eA[1], w->eB[0])};
+                    double n1[3];
+                    normalize_x(x, fn, n1);
+                    normalize_x(x, n1, j->normal + 3 * i);
+                    j->point[3 * i + 0] = px;
+                    j->point[3 * i + 1] = py;
+                    j->point[3 * i + 2] = pz;
+                }
             } else {
                 j->prim_id[i] = -1;
-                j->t[i] = INFINITY;
+                if (j->t) j->t[i] = INFINITY;
                 if (j->t64) j->t64[i] = INFINITY;
+                if (j->alpha) {
+                    j->alpha[i] = j->beta[i] = 0.0;
+                    for (int k = 0; k < 3; ++k) j->normal[3 * i + k] = j->point[3 * i + k] = 0.0;
+                }
             }
         }
     }
@@ -313,6 +354,7 @@ static void* any_worker(void* arg)
                 double tt;
                 if (!nof && !filter_x(x, fo, fd, j->tris[k].p0, j->tris[k].e1, j->tris[k].e2, fbest))
                     continue;
+                if (!face_selected(x, j->opts, &j->tris[k], dn)) continue;
                 /* ...Specializations.h:517-519 */
                 if (ray_triangle_x(x, o, dn, j->tris[k].v0, j->tris[k].eA, j->tris[k].eB, &tt, NULL, NULL) &&
                     tt > NZ && tt < how_far2)
@@ -375,5 +417,27 @@ void rise_oracle_any_brute(const float* tris, int64_t n_tris,
     j.occluded = occluded;
     j.opts = opts;
     run(&j, any_worker, n_threads);
+    free(tt);
+}
+
+void rise_oracle_closest_attr(const float* tris, int64_t n_tris, const float* rays, int64_t n_rays,
+                              int32_t* prim_id, double* range, double* alpha, double* beta, double* normal,
+                              double* point, int opts, int n_threads)
+{
+    job j;
+    memset(&j, 0, sizeof(j));
+    tri64* tt = expand_tris(tris, n_tris);
+    j.tris = tt;
+    j.n_tris = n_tris;
+    j.rays = rays;
+    j.n_rays = n_rays;
+    j.prim_id = prim_id;
+    j.t64 = range;
+    j.alpha = alpha;
+    j.beta = beta;
+    j.normal = normal;
+    j.point = point;
+    j.opts = opts;
+    run(&j, closest_worker, n_threads);
     free(tt);
 }

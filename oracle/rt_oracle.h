@@ -55,6 +55,8 @@ extern "C" {
 #define RISE_ORACLE_X86_FMA 2   /* the reference's Linux build's contraction (g++ -O3
                                    -march=x86-64-v3): a*b - c*d -> fma(a, b, -(c*d)),
                                    a*b + c*d + e*f -> fma(e, f, fma(a, b, c*d)) */
+#define RISE_ORACLE_NO_FRONT 4  /* skip front faces (bHitFrontFaces false) */
+#define RISE_ORACLE_NO_BACK 8   /* skip back faces (bHitBackFaces false) */
 
 /* RayTriangleIntersection (RayTriangleIntersection.cpp:21-68).  Returns 1 on
  * hit and fills t/alpha/beta.  edgeA = v1 - v0, edgeB = v2 - v0. */
@@ -90,6 +92,16 @@ void rise_oracle_closest_brute(const float* tris, int64_t n_tris,
 void rise_oracle_any_brute(const float* tris, int64_t n_tris,
                            const float* rays, const float* tmax, int64_t n_rays,
                            uint8_t* occluded, int opts, int n_threads);
+
+/* rise_oracle_closest_brute with the intersection record: alpha, beta (the
+ * certifier's barycentrics), the unit face normal as RayIntersectionGeometric
+ * ends up holding it for a mesh without vertex normals under an identity
+ * transform (Normalize of the face normal, normalized again by
+ * Object::IntersectRay) and the world hit point o + dn (t - 1e-12).
+ * normal and point: 3 doubles per ray (zero on a miss). */
+void rise_oracle_closest_attr(const float* tris, int64_t n_tris, const float* rays, int64_t n_rays,
+                              int32_t* prim_id, double* range, double* alpha, double* beta, double* normal,
+                              double* point, int opts, int n_threads);
 
 #ifdef __cplusplus
 }

@@ -74,12 +74,16 @@ _SIGNATURES = {
     "rise_hip_intersect_closest": ([_P, _P, _I64, _P, _P, _P], ctypes.c_int),
     "rise_hip_intersect_any": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "rise_hip_intersect_closest_noalloc": ([_P, _P, _I64, _P, _P, _P], ctypes.c_int),
+    "rise_hip_intersect_closest_ex": ([_P, _P, _I64, _I32, _P, _P], ctypes.c_int),
+    "rise_hip_intersect_any_ex": ([_P, _P, _P, _I64, _I32, _P, _P], ctypes.c_int),
+    "rise_hip_debug_trace_ex_host": ([_P, _I64, _P, _P, _I64, _I32, _I32, _P, _P], ctypes.c_int),
     "rise_hip_intersect_closest_counted": ([_P, _P, _I64, _P, _P, _P, _P], ctypes.c_int),
     "rise_hip_intersect_any_counted": ([_P, _P, _P, _I64, _P, _P, _P], ctypes.c_int),
     "rise_hip_debug_trace_host": (
         [_P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _P, ctypes.POINTER(SceneStats)],
         ctypes.c_int,
     ),
+    "rise_hip_debug_node_accept_host": ([_P, _P, ctypes.c_double, _P], ctypes.c_int),
 }
 
 _lib: ctypes.CDLL | None = None

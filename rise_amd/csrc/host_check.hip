@@ -8,6 +8,7 @@
 #include "trace_core.h"
 
 #include <atomic>
+#include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
@@ -48,6 +49,101 @@ rise_hip_status trace_host(const WideBVH& bvh, int64_t n_tris, int32_t bvh_sourc
                            uint8_t* occluded, uint64_t* counters, rise_hip_scene_stats* stats);
 
 }  // namespace
+
+extern "C" rise_hip_status rise_hip_debug_node_accept_host(const void* node, const float ray[6], double best,
+                                                           uint32_t* accept)
+{
+    clear_error();
+    if (!node || !ray || !accept) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    uint4 w[5];
+    std::memcpy(w, node, sizeof(w));
+    RayState r;
+    init_ray(r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
+    *accept = node_accept_mask(r, w[0], w[1], w[2], w[3], w[4], box_limit(r, best));
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_debug_trace_ex_host(const float* tris, int64_t n_tris, const float* rays,
+                                                        const float* tmax, int64_t n, int32_t mode, int32_t faces,
+                                                        rise_hip_hit* hits, uint8_t* occluded)
+{
+    clear_error();
+    if (n_tris < 0 || n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative count");
+    if ((n_tris > 0 && !tris) || (n > 0 && !rays)) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (mode < 0 || mode > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mode must be 0..3");
+    if (faces < 0 || faces > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "faces must be 0..3");
+    if (n > 0 && (mode & 1) == 0 && !hits) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if (n > 0 && (mode & 1) == 1 && !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    WideBVH bvh;
+    std::string err;
+    try {
+        if (!build_wide_bvh(tris, n_tris, nullptr, BuildParams(), bvh, err))
+            return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    const NodeSlot* nodes = bvh.nodes.data();
+    const Tri48* trs = bvh.tris.data();
+    std::atomic<int64_t> next(0);
+    std::atomic<bool> overflow(false);
+    auto one = [&](auto r, int64_t i) {
+        init_ray(r, rays[i], rays[n + i], rays[2 * n + i], rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        HostStack st;
+        unsigned nn = 0, nt = 0;
+        if ((mode & 1) == 0) {
+            ClosestState cs;
+            closest_begin(r, cs);
+            cs.faces = (unsigned)faces;
+            while (closest_step<false, true>(r, nodes, trs, st, cs, nn, nt)) {
+            }
+            rise_hip_hit h{};
+            h.prim_id = cs.best_id;
+            h.mesh = -1;
+            h.range = INFINITY;
+            if (cs.best_id >= 0) {
+                double ab[2], nrm[3], pt[3];
+                hit_attributes(r, trs + cs.best_leaf, cs.best_t, ab, nrm, pt);
+                h.mesh = float_to_bits(trs[cs.best_leaf].v1[3]);
+                h.range = world_range(r, cs.best_t);
+                h.alpha = ab[0];
+                h.beta = ab[1];
+                for (int k = 0; k < 3; ++k) {
+                    h.point[k] = (float)pt[k];
+                    h.geom_normal[k] = (float)nrm[k];
+                }
+            }
+            hits[i] = h;
+        } else {
+            AnyState as;
+            const double hf = tmax ? (double)tmax[i] : kRiseInfinity;
+            any_begin(r, scale_how_far(r, hf), as);
+            as.faces = (unsigned)faces;
+            while (any_step<false, true>(r, nodes, trs, st, as, nn, nt)) {
+            }
+            occluded[i] = as.hit ? 1 : 0;
+        }
+    };
+    auto worker = [&]() {
+        for (;;) {
+            const int64_t s0 = next.fetch_add(64);
+            if (s0 >= n) break;
+            for (int64_t i = s0; i < std::min<int64_t>(s0 + 64, n); ++i) {
+                try {
+                    if (mode & 2) one(RayStateT<kArithX86Fma>{}, i);
+                    else one(RayStateT<kArithIeee>{}, i);
+                } catch (const std::exception&) {
+                    overflow = true;
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < host_threads(); ++k) th.emplace_back(worker);
+    worker();
+    for (auto& x : th) x.join();
+    if (overflow) return fail(RISE_HIP_ERR_BUILD, "traversal stack overflow");
+    return RISE_HIP_OK;
+}
 
 extern "C" rise_hip_status rise_hip_debug_trace_host(const float* tris, int64_t n_tris, const float* rays,
                                                      const float* tmax, int64_t n, int32_t mode, int32_t* prim_id,

@@ -157,10 +157,17 @@ RISE_HD inline unsigned reverse_bits32(unsigned n) { return __builtin_bitreverse
 
 RISE_HD inline unsigned sobol_dim1(unsigned index)
 {
-    unsigned v = 0u, d = 1u << 31;
-    for (; index; index >>= 1, d ^= d >> 1)
-        if (index & 1u) v ^= d;
-    return v;
+    // Direction number i has bit 31 - j set iff C(i, j) is odd, i.e. iff j is
+    // a bit-subset of i (Lucas), so bit 31 - j of the result is the XOR of the
+    // index bits at every superset of j: a superset-sum over the five bits of
+    // the position, then a bit reversal.  Equal to the XOR of the direction
+    // numbers of the set bits for every 32-bit index.
+    index ^= (index >> 1) & 0x55555555u;
+    index ^= (index >> 2) & 0x33333333u;
+    index ^= (index >> 4) & 0x0f0f0f0fu;
+    index ^= (index >> 8) & 0x00ff00ffu;
+    index ^= (index >> 16) & 0x0000ffffu;
+    return reverse_bits32(index);
 }
 
 RISE_HD inline unsigned hash_combine(unsigned a, unsigned b)

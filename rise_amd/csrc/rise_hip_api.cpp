@@ -117,8 +117,20 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
     std::string err;
     BuildParams params;
     try {
-        if (!build_wide_bvh(verts.data(), total, ids.data(), params, bvh, err))
+        // Built over the running index, then each leaf triangle takes its
+        // caller id (v0.w) and its mesh index (v1.w, reported in hit records).
+        if (!build_wide_bvh(verts.data(), total, nullptr, params, bvh, err))
             return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+        std::vector<int32_t> mesh_of((size_t)total);
+        int64_t o = 0;
+        for (size_t m = 0; m < n_meshes; ++m)
+            for (int64_t i = 0; i < meshes[m].n_tris; ++i) mesh_of[(size_t)(o++)] = (int32_t)m;
+        for (Tri48& tr : bvh.tris) {
+            int32_t p;
+            std::memcpy(&p, &tr.v0[3], 4);
+            std::memcpy(&tr.v0[3], &ids[(size_t)p], 4);
+            std::memcpy(&tr.v1[3], &mesh_of[(size_t)p], 4);
+        }
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }
@@ -240,6 +252,34 @@ rise_hip_status rise_hip_intersect_any(rise_hip_scene* scene, const float* rays_
                                        uint8_t* occluded, hipStream_t stream)
 {
     return any_impl(scene, rays_soa, tmax, n, occluded, nullptr, stream);
+}
+
+rise_hip_status rise_hip_intersect_closest_ex(rise_hip_scene* scene, const float* rays_soa, int64_t n, int32_t faces,
+                                              rise_hip_hit* hits, hipStream_t stream)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (faces < 0 || faces > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "faces must be a combination of RISE_HIP_FACES_*");
+    if (n == 0) return RISE_HIP_OK;
+    if (!rays_soa || !hits) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    const hipError_t e = launch_closest_ex(scene->view(), rays_soa, n, (unsigned)faces, hits, stream);
+    if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
+    return RISE_HIP_OK;
+}
+
+rise_hip_status rise_hip_intersect_any_ex(rise_hip_scene* scene, const float* rays_soa, const float* tmax, int64_t n,
+                                          int32_t faces, uint8_t* occluded, hipStream_t stream)
+{
+    clear_error();
+    if (!scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (faces < 0 || faces > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "faces must be a combination of RISE_HIP_FACES_*");
+    if (n == 0) return RISE_HIP_OK;
+    if (!rays_soa || !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    const hipError_t e = launch_any_ex(scene->view(), rays_soa, tmax, n, (unsigned)faces, occluded, stream);
+    if (e != hipSuccess) return fail_hip(e, "any-hit launch");
+    return RISE_HIP_OK;
 }
 
 rise_hip_status rise_hip_intersect_closest_noalloc(rise_hip_scene* scene, const float* rays_soa, int64_t n,

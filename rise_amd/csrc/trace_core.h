@@ -43,11 +43,23 @@ constexpr double kNearZero = 1e-12;                      // Utilities/Math3D/Con
 constexpr double kRiseInfinity = 1.7976931348623158e+308;  // Constants.h:33
 constexpr double kSurfaceBackoff = 1e-12;                // Objects/Object.cpp:36
 
-// 1 + 2^-19: relative widening of the FP32 slab interval.  The computed entry
-// and exit distances (b - o) * (1/d) carry at most ~3 float roundings
-// (relative 2^-22 each), so scaling the exit distance by this factor (and the
-// running best by it again) never rejects a box the exact ray enters.
+// 1 + 2^-19: relative widening of the FP32 slab interval.  After the absolute
+// slack of each plane (slab_axis below) the computed entry and exit distances
+// are within 2.0001 * 2^-24 (relative) of the exact ones, so scaling the exit
+// distance by this factor (and the running best by it again) never rejects a
+// box the exact ray enters.
 #define RISE_SLAB_PAD 1.0000019073486328f
+
+// The slab test runs along d * 2^k, the direction scaled by the power of two
+// that puts its largest component in [1, 2).  A component whose scaled size is
+// below 2^-80 (zero included) takes the inverse +-2^80 instead of 1/d: that
+// shrinks its slab interval toward t = 0 only by a factor the box padding
+// covers.  Every leaf box is padded by at least 1e-9 of the scene extent, so
+// a ray that can hit a triangle of the box lies at least that far inside the
+// box along such an axis, and its interval still reaches 1e-9 * extent * 2^80
+// (> 1e15 scene extents) -- beyond any hit, since t along the scaled direction
+// is at most the distance travelled along its largest axis.
+constexpr float kInvClamp = 0x1p80f;
 
 // Arithmetic of the reference build whose results are reproduced.
 //   kArithIeee: every float and double operation rounded on its own, in the
@@ -98,10 +110,11 @@ struct RayStateT {
     float ox, oy, oz;      // float origin (exact input)
     float dx, dy, dz;      // direction as given
     float fx, fy, fz;      // (float)dn: the filter's direction (BVH.h:1212-1225)
-    float ix, iy, iz;      // 1/d with d == 0 -> FLT_MAX (BVH.h:1212-1225)
+    float ix, iy, iz;      // slab inverses 1/(d 2^k) (kInvClamp for tiny components)
     unsigned oinv;         // (~octant) & 7
     double ov;             // Normalize's 1/|d|
     double len_d;          // |d| in double
+    double len_s;          // |d| 2^k: t along the scaled direction = distance / len_s
 };
 using RayState = RayStateT<kArithIeee>;
 
@@ -112,27 +125,25 @@ RISE_HD inline double dn1(const RayStateT<A>& r) { return (double)r.dy * r.ov; }
 template <int A>
 RISE_HD inline double dn2(const RayStateT<A>& r) { return (double)r.dz * r.ov; }
 
-RISE_HD inline float safe_inv(float d)
+// Inverse of a scaled direction component (largest component in [1, 2)):
+// correctly rounded 1/d, or +-kInvClamp when |d| < 2^-80 (+ for a zero).
+RISE_HD inline float slab_inv(float d)
 {
-    if (d == 0.0f) return FLT_MAX;
-    float r = 1.0f / d;
-    if (!(fabsf(r) <= FLT_MAX)) r = copysignf(FLT_MAX, d);
-    return r;
+    if (fabsf(d) < 0x1p-80f) return d < 0.0f ? -kInvClamp : kInvClamp;
+    return 1.0f / d;
 }
 
-// Vector3Ops::Normalize (Utilities/Math3D/VectorsOps.h:229-238).
-RISE_HD inline void normalize_rise(const double v[3], double out[3])
+// Vector3Ops::Normalize (Utilities/Math3D/VectorsOps.h:229-238) under
+// arithmetic model A.
+template <int A>
+RISE_HD inline void normalize_model(const double v[3], double out[3])
 {
-    const double mag = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double mag = sqrt(dot3<A>(v[0], v[0], v[1], v[1], v[2], v[2]));
     if (mag > 0.0) {
         const double ov = 1.0 / mag;
-        out[0] = v[0] * ov;
-        out[1] = v[1] * ov;
-        out[2] = v[2] * ov;
+        for (int k = 0; k < 3; ++k) out[k] = v[k] * ov;
     } else {
-        out[0] = v[0];
-        out[1] = v[1];
-        out[2] = v[2];
+        for (int k = 0; k < 3; ++k) out[k] = v[k];
     }
 }
 
@@ -142,9 +153,13 @@ RISE_HD inline void init_ray(RayStateT<A>& r, float ox, float oy, float oz, floa
     r.ox = ox;
     r.oy = oy;
     r.oz = oz;
-    r.ix = safe_inv(dx);
-    r.iy = safe_inv(dy);
-    r.iz = safe_inv(dz);
+    const float m = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz));
+    int k = 0;
+    if (m > 0.0f) (void)frexpf(m, &k);  // m = f 2^k, f in [0.5, 1)
+    const int sh = 1 - k;                // d 2^sh: largest component in [1, 2)
+    r.ix = slab_inv(ldexpf(dx, sh));
+    r.iy = slab_inv(ldexpf(dy, sh));
+    r.iz = slab_inv(ldexpf(dz, sh));
     const unsigned oct = (dx < 0.0f ? 1u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 4u : 0u);
     r.oinv = (~oct) & 7u;
     r.dx = dx;
@@ -154,6 +169,7 @@ RISE_HD inline void init_ray(RayStateT<A>& r, float ox, float oy, float oz, floa
     const double mag = sqrt(dot3<A>(d[0], d[0], d[1], d[1], d[2], d[2]));
     r.ov = mag > 0.0 ? 1.0 / mag : 1.0;
     r.len_d = mag;
+    r.len_s = mag > 0.0 ? ldexp(mag, sh) : 1.0;
     r.fx = (float)dn0(r);
     r.fy = (float)dn1(r);
     r.fz = (float)dn2(r);
@@ -194,7 +210,8 @@ RISE_HD inline bool filter_float(const RayStateT<A>& r, const float4& a, const f
 // FP64 Moller-Trumbore, RayTriangleIntersection.cpp:21-68 with the edges of
 // ...Specializations.h:141-142 (computed in double from the vertices).
 template <int A>
-RISE_HD inline bool certify(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, double& t_out)
+RISE_HD inline bool certify(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, double& t_out,
+                            double* ab = nullptr)
 {
     const double v0x = a.x, v0y = a.y, v0z = a.z;
     const double eAx = (double)b.x - v0x, eAy = (double)b.y - v0y, eAz = (double)b.z - v0z;
@@ -217,9 +234,31 @@ RISE_HD inline bool certify(const RayStateT<A>& r, const float4& a, const float4
     const double tt = dot3<A>(eBx, q0, eBy, q1, eBz, q2) * oodet;
     if (tt >= kNearZero) {
         t_out = tt;
+        if (ab) {
+            ab[0] = al;
+            ab[1] = be;
+        }
         return true;
     }
     return false;
+}
+
+// Face selection of TriangleMeshGeometryIndexed::RayElementIntersection
+// (...Specializations.h:141-159): the face normal (v1 - v0) x (v2 - v0) in
+// double against the normalized direction; dot < 0 is a front face.
+// faces: bit 0 hit front faces, bit 1 hit back faces.
+template <int A>
+RISE_HD inline bool face_ok(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, unsigned faces)
+{
+    const double eAx = (double)b.x - a.x, eAy = (double)b.y - a.y, eAz = (double)b.z - a.z;
+    const double eBx = (double)c.x - a.x, eBy = (double)c.y - a.y, eBz = (double)c.z - a.z;
+    const double nx = sub_prod<A>(eAy, eBz, eAz, eBy);
+    const double ny = sub_prod<A>(eAz, eBx, eAx, eBz);
+    const double nz = sub_prod<A>(eAx, eBy, eAy, eBx);
+    const double dd = dot3<A>(nx, dn0(r), ny, dn1(r), nz, dn2(r));
+    if (!(faces & 1u) && dd < 0.0) return false;
+    if (!(faces & 2u) && dd > 0.0) return false;
+    return true;
 }
 
 // World range of a committed hit: Object.cpp:641-643 for an identity
@@ -245,6 +284,72 @@ RISE_HD inline double world_range(const RayStateT<A>& r, double t)
 
 RISE_HD inline unsigned byte_of(unsigned w, int k) { return (w >> (8 * k)) & 0xffu; }
 
+// One axis of a node's slab test in fused form: the plane at quantized
+// coordinate q lies at distance t(q) = q s + b along the scaled direction,
+// with s = 2^e / d (exact scaling of the inverse) and b = (p - o) / d.  The
+// form rounds b before adding q s, so t carries an absolute error of up to
+// 2.0001 * 2^-24 |b| besides its relative error (2.0001 * 2^-24 |t|): the
+// near plane subtracts, the far plane adds, 2^-21 |b| + 2^-100 (which also
+// covers flushed subnormals), leaving only the relative error for
+// RISE_SLAB_PAD.  Should b or s be too large for q s + b to stay finite
+// (|b| < 2^126 and |s| < 2^119 keep 255 |s| + |b| (1 + 2^-21) below 2^128),
+// the axis constrains nothing (s = 0, planes at -inf / +inf): still
+// conservative.  With the 2^80 inverse of a zero direction component that
+// happens only for nodes 2^46 away or 2^39 wide.
+struct SlabAxis {
+    float s, bn, bf;
+};
+
+RISE_HD inline SlabAxis slab_axis(float p, int e, float o, float inv)
+{
+    const float b = (p - o) * inv;
+    const float s = ldexpf(inv, e);
+    const float slack = fmaf(fabsf(b), 0x1p-21f, 0x1p-100f);
+    if (fabsf(b) < 0x1p126f && fabsf(s) < 0x1p119f) return SlabAxis{s, b - slack, b + slack};
+    return SlabAxis{0.0f, -INFINITY, INFINITY};
+}
+
+// The eight child boxes of one node against the ray: bit s of the result set
+// when slot s is occupied and its box meets [0, tmax] (conservatively).  The
+// quantized planes are split by the sign of the inverse into near and far
+// words per axis, so each slot costs six byte conversions, six FMAs and the
+// interval reductions.
+template <int A>
+RISE_HD inline unsigned node_accept_mask(const RayStateT<A>& r, const uint4& w0, const uint4& w1, const uint4& w2,
+                                         const uint4& w3, const uint4& w4, float tmax)
+{
+    const SlabAxis ax = slab_axis(bits_to_float(w0.x), (int)(signed char)(w0.w & 0xff), r.ox, r.ix);
+    const SlabAxis ay = slab_axis(bits_to_float(w0.y), (int)(signed char)((w0.w >> 8) & 0xff), r.oy, r.iy);
+    const SlabAxis az = slab_axis(bits_to_float(w0.z), (int)(signed char)((w0.w >> 16) & 0xff), r.oz, r.iz);
+    // Low planes: qlox = w2.x|y, qloy = w2.z|w, qloz = w3.x|y; high planes:
+    // qhix = w3.z|w, qhiy = w4.x|y, qhiz = w4.z|w (slots 0-3 | 4-7).  With a
+    // negative inverse the high plane is the near one.
+    const bool nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
+    const unsigned nx0 = nx ? w3.z : w2.x, nx1 = nx ? w3.w : w2.y;
+    const unsigned fx0 = nx ? w2.x : w3.z, fx1 = nx ? w2.y : w3.w;
+    const unsigned ny0 = ny ? w4.x : w2.z, ny1 = ny ? w4.y : w2.w;
+    const unsigned fy0 = ny ? w2.z : w4.x, fy1 = ny ? w2.w : w4.y;
+    const unsigned nz0 = nz ? w4.z : w3.x, nz1 = nz ? w4.w : w3.y;
+    const unsigned fz0 = nz ? w3.x : w4.z, fz1 = nz ? w3.y : w4.w;
+    unsigned acc = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int k = s & 3;
+        const bool hi4 = s >= 4;
+        const unsigned meta = byte_of(s < 4 ? w1.z : w1.w, s & 3);
+        const float tnx = fmaf((float)byte_of(hi4 ? nx1 : nx0, k), ax.s, ax.bn);
+        const float tny = fmaf((float)byte_of(hi4 ? ny1 : ny0, k), ay.s, ay.bn);
+        const float tnz = fmaf((float)byte_of(hi4 ? nz1 : nz0, k), az.s, az.bn);
+        const float tfx = fmaf((float)byte_of(hi4 ? fx1 : fx0, k), ax.s, ax.bf);
+        const float tfy = fmaf((float)byte_of(hi4 ? fy1 : fy0, k), ay.s, ay.bf);
+        const float tfz = fmaf((float)byte_of(hi4 ? fz1 : fz0, k), az.s, az.bf);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+        if ((meta != 0) && (tn <= tf * RISE_SLAB_PAD)) acc |= 1u << s;
+    }
+    return acc;
+}
+
 // Tests the eight children of one node.  Returns the permuted internal-hit
 // mask (bit k <-> slot k ^ oinv) and fills the node's leaf-triangle mask.
 template <int A>
@@ -253,42 +358,19 @@ RISE_HD inline unsigned test_node(const RayStateT<A>& r, const NodeSlot* __restr
 {
     const uint4* p = reinterpret_cast<const uint4*>(nodes + ni);
     const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
-    const float px = bits_to_float(w0.x), py = bits_to_float(w0.y), pz = bits_to_float(w0.z);
-    const int ex = (int)(signed char)(w0.w & 0xff);
-    const int ey = (int)(signed char)((w0.w >> 8) & 0xff);
-    const int ez = (int)(signed char)((w0.w >> 16) & 0xff);
     imask = (w0.w >> 24) & 0xffu;
     child_base = w1.x;
     tri_base = w1.y;
-    const float sx = bits_to_float((unsigned)(ex + 127) << 23);
-    const float sy = bits_to_float((unsigned)(ey + 127) << 23);
-    const float sz = bits_to_float((unsigned)(ez + 127) << 23);
-    // Child planes relative to the origin, pre-scaled: (p + q*s - o) * inv.
+    const unsigned acc = node_accept_mask(r, w0, w1, w2, w3, w4, tmax);
     unsigned hits = 0;
     tmask = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const unsigned meta = byte_of(s < 4 ? w1.z : w1.w, s & 3);
-        const int k = s & 3;
-        const bool hi4 = s >= 4;
-        const float qlx = (float)byte_of(hi4 ? w2.y : w2.x, k);
-        const float qly = (float)byte_of(hi4 ? w2.w : w2.z, k);
-        const float qlz = (float)byte_of(hi4 ? w3.y : w3.x, k);
-        const float qhx = (float)byte_of(hi4 ? w3.w : w3.z, k);
-        const float qhy = (float)byte_of(hi4 ? w4.y : w4.x, k);
-        const float qhz = (float)byte_of(hi4 ? w4.w : w4.z, k);
-        const float lx = fmaf(qlx, sx, px), ly = fmaf(qly, sy, py), lz = fmaf(qlz, sz, pz);
-        const float hx = fmaf(qhx, sx, px), hy = fmaf(qhy, sy, py), hz = fmaf(qhz, sz, pz);
-        const float t1x = (lx - r.ox) * r.ix, t2x = (hx - r.ox) * r.ix;
-        const float t1y = (ly - r.oy) * r.iy, t2y = (hy - r.oy) * r.iy;
-        const float t1z = (lz - r.oz) * r.iz, t2z = (hz - r.oz) * r.iz;
-        const float tn = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fmaxf(fminf(t1z, t2z), 0.0f));
-        const float tf = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fminf(fmaxf(t1z, t2z), tmax));
-        const bool hit = (meta != 0) && (tn <= tf * RISE_SLAB_PAD);
-        if (hit) {
+        if ((acc >> s) & 1u) {
             if ((imask >> s) & 1u) {
                 hits |= 1u << (s ^ r.oinv);
             } else {
+                const unsigned meta = byte_of(s < 4 ? w1.z : w1.w, s & 3);
                 const unsigned cnt = meta >> 5, off = meta & 31u;
                 tmask |= ((1u << cnt) - 1u) << off;
             }
@@ -298,12 +380,12 @@ RISE_HD inline unsigned test_node(const RayStateT<A>& r, const NodeSlot* __restr
 }
 
 // Converts a best distance along dn into the float traversal parameter along
-// d, widened (RISE_SLAB_PAD); kRiseInfinity maps to +inf.
+// the scaled direction, widened (RISE_SLAB_PAD); kRiseInfinity maps to +inf.
 template <int A>
 RISE_HD inline float box_limit(const RayStateT<A>& r, double best)
 {
     if (!(best < kRiseInfinity)) return INFINITY;
-    const double s = best / r.len_d;
+    const double s = best / r.len_s;
     return (float)s * RISE_SLAB_PAD;
 }
 
@@ -334,6 +416,7 @@ struct ClosestState {
     int best_id;
     float tlim;
     double best_t;
+    unsigned faces;  // face selection (face_ok), read by the culling variant only
 };
 
 template <int A>
@@ -346,6 +429,7 @@ RISE_HD inline void closest_begin(const RayStateT<A>& r, ClosestState& s)
     s.best_id = -1;
     s.tlim = INFINITY;
     s.best_t = kRiseInfinity;
+    s.faces = 3u;
 }
 
 // One traversal step: visit the next node of the current group (near-to-far
@@ -353,7 +437,8 @@ RISE_HD inline void closest_begin(const RayStateT<A>& r, ClosestState& s)
 // is used up, pop the next one.  Returns false once the traversal is over.
 // Closest hit: smallest t, exact ties to the lowest caller id (the
 // reference's strict-closer guard, ...Specializations.h:172, plus the tie rule).
-template <bool COUNT, class Stack, int A>
+// CULL: apply the face selection in s.faces (face_ok) before the filter.
+template <bool COUNT, bool CULL = false, class Stack, int A>
 RISE_HD inline bool closest_step(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
                                  const Tri48* __restrict__ tris, Stack& st, ClosestState& s, unsigned& n_nodes,
                                  unsigned& n_tris)
@@ -377,6 +462,7 @@ RISE_HD inline bool closest_step(const RayStateT<A>& r, const NodeSlot* __restri
         const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
+        if (CULL && s.faces != 3u && !face_ok(r, a, b, c, s.faces)) continue;
         double tt;
         if (filter_float(r, a, b, c, (float)s.best_t) && certify(r, a, b, c, tt)) {
             const int id = float_to_bits(a.w);
@@ -418,6 +504,7 @@ RISE_HD inline int trace_closest(const RayStateT<A>& r, const NodeSlot* __restri
 // Any-hit state: the current group, the box limit and the result.
 struct AnyState {
     unsigned gbase, ghits, gimask;
+    unsigned faces;  // face selection (face_ok), read by the culling variant only
     bool hit;
     float tlim;
     float fbest;  // (float)dHowFar2: the filter's range (BVH.h:1283)
@@ -430,6 +517,7 @@ RISE_HD inline void any_begin(const RayStateT<A>& r, double how_far2, AnyState& 
     s.gbase = 0;
     s.ghits = 1u << (0u ^ r.oinv);
     s.gimask = 1u;
+    s.faces = 3u;
     s.hit = false;
     s.how_far2 = how_far2;
     s.fbest = (float)how_far2;
@@ -440,7 +528,7 @@ RISE_HD inline void any_begin(const RayStateT<A>& r, double how_far2, AnyState& 
 // kNearZero < t < how_far2 (...Specializations.h:517-519) and casts shadows
 // (bit 0 of v2.w clear; ObjectManager.cpp:119-124).  Returns false once
 // the query is decided (a hit, or the tree exhausted).
-template <bool COUNT, class Stack, int A>
+template <bool COUNT, bool CULL = false, class Stack, int A>
 RISE_HD inline bool any_step(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
                              Stack& st, AnyState& s, unsigned& n_nodes, unsigned& n_tris)
 {
@@ -463,6 +551,7 @@ RISE_HD inline bool any_step(const RayStateT<A>& r, const NodeSlot* __restrict__
         const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
+        if (CULL && s.faces != 3u && !face_ok(r, a, b, c, s.faces)) continue;
         double tt;
         if (filter_float(r, a, b, c, s.fbest) && certify(r, a, b, c, tt) && tt > kNearZero && tt < s.how_far2 &&
             !(float_to_bits(c.w) & 1u)) {
@@ -489,6 +578,36 @@ RISE_HD inline bool trace_any(const RayStateT<A>& r, const NodeSlot* __restrict_
     while (any_step<COUNT>(r, nodes, tris, st, s, n_nodes, n_tris)) {
     }
     return s.hit;
+}
+
+// Attributes of a committed hit on triangle `tri` at distance t along dn,
+// as TriangleMeshGeometryIndexed::RayElementIntersection and
+// Object::IntersectRay leave them for a mesh without vertex normals or
+// texture coordinates under an identity transform: the certifier's
+// barycentrics, the face normal (v1 - v0) x (v2 - v0) normalized
+// (...Specializations.h:196-205) and normalized again by the object's normal
+// transform (Object.cpp:531-533), and the hit point o + dn (t - 1e-12)
+// (Object.cpp:637-640).
+template <int A>
+RISE_HD inline void hit_attributes(const RayStateT<A>& r, const Tri48* __restrict__ tri, double t, double ab[2],
+                                   double n_out[3], double p_out[3])
+{
+    const float4* tp = reinterpret_cast<const float4*>(tri);
+    const float4 a = tp[0], b = tp[1], c = tp[2];
+    double tt = 0.0;
+    ab[0] = ab[1] = 0.0;
+    (void)certify(r, a, b, c, tt, ab);
+    const double eAx = (double)b.x - a.x, eAy = (double)b.y - a.y, eAz = (double)b.z - a.z;
+    const double eBx = (double)c.x - a.x, eBy = (double)c.y - a.y, eBz = (double)c.z - a.z;
+    const double fn[3] = {sub_prod<A>(eAy, eBz, eAz, eBy), sub_prod<A>(eAz, eBx, eAx, eBz),
+                          sub_prod<A>(eAx, eBy, eAy, eBx)};
+    double n1[3];
+    normalize_model<A>(fn, n1);
+    normalize_model<A>(n1, n_out);
+    const double s = t - kSurfaceBackoff;
+    const double d[3] = {dn0(r), dn1(r), dn2(r)};
+    const double o[3] = {r.ox, r.oy, r.oz};
+    for (int k = 0; k < 3; ++k) p_out[k] = A == kArithX86Fma ? fma(d[k], s, o[k]) : o[k] + d[k] * s;
 }
 
 // Object.cpp:472-493: the world limit dHowFar becomes dHowFar2 = |d| * dHowFar

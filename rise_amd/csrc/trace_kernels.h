@@ -2,6 +2,7 @@
 #pragma once
 
 #include "bvh8.h"
+#include "rise_hip.h"
 
 #include <hip/hip_runtime_api.h>
 
@@ -25,6 +26,13 @@ hipError_t launch_closest(const TraceScene& s, const float* rays, int64_t n, int
 // Any hit within tmax (world distance per ray; NULL = unbounded).
 hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax, int64_t n, uint8_t* occluded,
                       unsigned long long* counters, hipStream_t stream);
+
+// Closest hit with a face selection (bit 0 front, bit 1 back) and hit records;
+// any hit with a face selection.
+hipError_t launch_closest_ex(const TraceScene& s, const float* rays, int64_t n, unsigned faces, rise_hip_hit* hits,
+                             hipStream_t stream);
+hipError_t launch_any_ex(const TraceScene& s, const float* rays, const float* tmax, int64_t n, unsigned faces,
+                         uint8_t* occluded, hipStream_t stream);
 
 // Device address (on the current device) of the allocation-free query's
 // scratch counter; looked up once per device with hipGetSymbolAddress.

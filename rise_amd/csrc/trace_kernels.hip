@@ -201,6 +201,115 @@ __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restr
     flush_counts<COUNT>(counters, nn, nt);
 }
 
+// Closest hit with the face selection and the hit record of
+// rise_hip_intersect_closest_ex.
+template <int STACK, int A>
+__global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* __restrict__ nodes,
+                                                                const Tri48* __restrict__ tris,
+                                                                const float* __restrict__ rays, int64_t n,
+                                                                unsigned faces, rise_hip_hit* __restrict__ hits,
+                                                                unsigned long long* next)
+{
+    __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
+    LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
+    PrivStack pst;
+    pst.sp = 0;
+    RayStateT<A> r;
+    ClosestState s;
+    long long ray = -1;
+    unsigned nn = 0, nt = 0;
+    const int64_t grid_threads = (int64_t)gridDim.x * blockDim.x;
+    for (bool first_pass = true;; first_pass = false) {
+        const long long got = first_pass ? first_item(n) : refill(ray == -1, ray >= 0, next, n, grid_threads);
+        if (ray == -1 && got != -1) {
+            ray = got;
+            if (ray >= 0) {
+                float ox, oy, oz, dx, dy, dz;
+                load_ray(rays, n, ray, ox, oy, oz, dx, dy, dz);
+                init_ray(r, ox, oy, oz, dx, dy, dz);
+                closest_begin(r, s);
+                s.faces = faces;
+                lst.sp = 0;
+                pst.sp = 0;
+            }
+        }
+        if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
+        if (ray >= 0) {
+            const bool more = STACK > 0 ? closest_step<false, true>(r, nodes, tris, lst, s, nn, nt)
+                                        : closest_step<false, true>(r, nodes, tris, pst, s, nn, nt);
+            if (!more) {
+                rise_hip_hit h;
+                h.prim_id = s.best_id;
+                if (s.best_id >= 0) {
+                    const Tri48* tri = tris + s.best_leaf;
+                    double ab[2], nrm[3], pt[3];
+                    hit_attributes(r, tri, s.best_t, ab, nrm, pt);
+                    h.mesh = float_to_bits(tri->v1[3]);
+                    h.range = world_range(r, s.best_t);
+                    h.alpha = ab[0];
+                    h.beta = ab[1];
+                    for (int k = 0; k < 3; ++k) {
+                        h.point[k] = (float)pt[k];
+                        h.geom_normal[k] = (float)nrm[k];
+                    }
+                } else {
+                    h.mesh = -1;
+                    h.range = INFINITY;
+                    h.alpha = h.beta = 0.0;
+                    for (int k = 0; k < 3; ++k) h.point[k] = h.geom_normal[k] = 0.0f;
+                }
+                hits[ray] = h;
+                ray = -1;
+            }
+        }
+    }
+}
+
+// Any hit with the face selection of rise_hip_intersect_any_ex.
+template <int STACK, int A>
+__global__ __launch_bounds__(kBlock) void any_ex_persistent(const NodeSlot* __restrict__ nodes,
+                                                            const Tri48* __restrict__ tris,
+                                                            const float* __restrict__ rays,
+                                                            const float* __restrict__ tmax, int64_t n,
+                                                            unsigned faces, uint8_t* __restrict__ occluded,
+                                                            unsigned long long* next)
+{
+    __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
+    LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
+    PrivStack pst;
+    pst.sp = 0;
+    RayStateT<A> r;
+    AnyState s;
+    long long ray = -1;
+    unsigned nn = 0, nt = 0;
+    const int64_t grid_threads = (int64_t)gridDim.x * blockDim.x;
+    for (bool first_pass = true;; first_pass = false) {
+        const long long got = first_pass ? first_item(n) : refill(ray == -1, ray >= 0, next, n, grid_threads);
+        if (ray == -1 && got != -1) {
+            ray = got;
+            if (ray >= 0) {
+                float ox, oy, oz, dx, dy, dz;
+                load_ray(rays, n, ray, ox, oy, oz, dx, dy, dz);
+                init_ray(r, ox, oy, oz, dx, dy, dz);
+                const double how_far = tmax ? (double)tmax[ray] : kRiseInfinity;
+                any_begin(r, scale_how_far(r, how_far), s);
+                s.faces = faces;
+                lst.sp = 0;
+                pst.sp = 0;
+            }
+        }
+        if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
+        if (ray >= 0) {
+            const bool more = STACK > 0 ? any_step<false, true>(r, nodes, tris, lst, s, nn, nt)
+                                        : any_step<false, true>(r, nodes, tris, pst, s, nn, nt);
+            if (!more) {
+                occluded[ray] = s.hit ? 1 : 0;
+                ray = -1;
+            }
+        }
+    }
+}
+
 __global__ void reset_counter(unsigned long long* c) { *c = 0ull; }
 
 // Waves resident at once for a persistent kernel: CUs x blocks per CU.
@@ -293,6 +402,52 @@ hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax,
     }
 #undef RISE_LAUNCH_ANY_A
 #undef RISE_LAUNCH_ANY
+    return hipGetLastError();
+}
+
+hipError_t launch_closest_ex(const TraceScene& s, const float* rays, int64_t n, unsigned faces, rise_hip_hit* hits,
+                             hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int need = s.depth - 1;
+    const dim3 block(kBlock);
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
+#define RISE_LAUNCH_EX(S, A)                                                                                      \
+    hipLaunchKernelGGL((closest_ex_persistent<S, A>), dim3(persistent_grid(closest_ex_persistent<S, A>, n)), block, \
+                       0, stream, s.nodes, s.tris, rays, n, faces, hits, s.fetch_counter)
+#define RISE_LAUNCH_EX_A(S)                                          \
+    do {                                                             \
+        if (s.arith == kArithX86Fma) RISE_LAUNCH_EX(S, kArithX86Fma); \
+        else RISE_LAUNCH_EX(S, kArithIeee);                          \
+    } while (0)
+    if (need <= 8) RISE_LAUNCH_EX_A(8);
+    else if (need <= 16) RISE_LAUNCH_EX_A(16);
+    else RISE_LAUNCH_EX_A(0);
+#undef RISE_LAUNCH_EX_A
+#undef RISE_LAUNCH_EX
+    return hipGetLastError();
+}
+
+hipError_t launch_any_ex(const TraceScene& s, const float* rays, const float* tmax, int64_t n, unsigned faces,
+                         uint8_t* occluded, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int need = s.depth - 1;
+    const dim3 block(kBlock);
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
+#define RISE_LAUNCH_EX(S, A)                                                                              \
+    hipLaunchKernelGGL((any_ex_persistent<S, A>), dim3(persistent_grid(any_ex_persistent<S, A>, n)), block, 0, \
+                       stream, s.nodes, s.tris, rays, tmax, n, faces, occluded, s.fetch_counter)
+#define RISE_LAUNCH_EX_A(S)                                          \
+    do {                                                             \
+        if (s.arith == kArithX86Fma) RISE_LAUNCH_EX(S, kArithX86Fma); \
+        else RISE_LAUNCH_EX(S, kArithIeee);                          \
+    } while (0)
+    if (need <= 8) RISE_LAUNCH_EX_A(8);
+    else if (need <= 16) RISE_LAUNCH_EX_A(16);
+    else RISE_LAUNCH_EX_A(0);
+#undef RISE_LAUNCH_EX_A
+#undef RISE_LAUNCH_EX
     return hipGetLastError();
 }
 

@@ -80,6 +80,27 @@ public:
         return check(rise_hip_intersect_any(scene_, rays_soa, tmax, n, occluded, stream), "IntersectShadowRay");
     }
 
+    // IObjectManager::IntersectRay(ri, bHitFrontFaces, bHitBackFaces, ...)
+    // (Interfaces/IObjectManager.h:37-43) for a batch: the face flags select
+    // the triangles, hits (device, n records) receive id, mesh, world range,
+    // barycentrics, hit point and geometric normal.
+    bool IntersectRay(const float* rays_soa, int64_t n, bool hit_front_faces, bool hit_back_faces,
+                      rise_hip_hit* hits, hipStream_t stream) const
+    {
+        const int32_t faces = (hit_front_faces ? RISE_HIP_FACES_FRONT : 0) | (hit_back_faces ? RISE_HIP_FACES_BACK : 0);
+        return check(rise_hip_intersect_closest_ex(scene_, rays_soa, n, faces, hits, stream), "IntersectRay");
+    }
+
+    // IObjectManager::IntersectShadowRay(ray, dHowFar, bHitFrontFaces,
+    // bHitBackFaces) (IObjectManager.h:45-50) for a batch.
+    bool IntersectShadowRay(const float* rays_soa, const float* tmax, int64_t n, bool hit_front_faces,
+                            bool hit_back_faces, uint8_t* occluded, hipStream_t stream) const
+    {
+        const int32_t faces = (hit_front_faces ? RISE_HIP_FACES_FRONT : 0) | (hit_back_faces ? RISE_HIP_FACES_BACK : 0);
+        return check(rise_hip_intersect_any_ex(scene_, rays_soa, tmax, n, faces, occluded, stream),
+                     "IntersectShadowRay");
+    }
+
     // The built 8-wide tree saved to / reloaded from a scene cache file
     // (rise_hip_scene_cache_*): a reload replaces PrepareForRendering with no build.
     bool SaveCache(const char* path) const { return check(rise_hip_scene_cache_write(scene_, path), "SaveCache"); }

@@ -32,6 +32,29 @@ ARITH_X86_FMA = 1
 """Hits as the reference's Linux build (g++ -O3 -march=x86-64-v3) computes them."""
 
 
+FACES_NONE, FACES_FRONT, FACES_BACK, FACES_BOTH = 0, 1, 2, 3
+
+
+class Hit(ctypes.Structure):
+    """rise_hip_hit."""
+    _fields_ = [("prim_id", ctypes.c_int32), ("mesh", ctypes.c_int32), ("range", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("point", ctypes.c_float * 3),
+                ("geom_normal", ctypes.c_float * 3)]
+
+
+HIT_DTYPE = np.dtype([("prim_id", "<i4"), ("mesh", "<i4"), ("range", "<f8"), ("alpha", "<f8"), ("beta", "<f8"),
+                      ("point", "<f4", (3,)), ("geom_normal", "<f4", (3,))])
+assert HIT_DTYPE.itemsize == ctypes.sizeof(Hit) == 56
+
+
+def hit_fields(records) -> np.ndarray:
+    """rise_hip_hit records (bytes: a numpy or torch uint8 buffer) as a
+    structured numpy array."""
+    if hasattr(records, "cpu"):
+        records = records.cpu().numpy()
+    return np.frombuffer(np.ascontiguousarray(records).tobytes(), dtype=HIT_DTYPE)
+
+
 class ObjectManager:
     """Resident triangle scene answering batched ray queries on the GPU.
 
@@ -128,6 +151,33 @@ class ObjectManager:
         _lib.check(st)
         return occluded
 
+    def intersect_ray_ex(self, rays, faces: int = FACES_BOTH, hits=None, stream=None):
+        """rise_hip_intersect_closest_ex: closest hit under a face selection
+        (FACES_*) with the intersection record.  Returns a (n,) uint8 CUDA
+        tensor view of rise_hip_hit records; hit_fields() splits it."""
+        import torch
+
+        n = rays.shape[1]
+        _check_rays(rays)
+        if hits is None:
+            hits = torch.empty(n * ctypes.sizeof(Hit), dtype=torch.uint8, device=rays.device)
+        _lib.check(_lib.lib().rise_hip_intersect_closest_ex(self.handle, rays.data_ptr(), n, int(faces),
+                                                            hits.data_ptr(), _stream_handle(stream)))
+        return hits
+
+    def intersect_shadow_ray_ex(self, rays, faces: int = FACES_BOTH, how_far=None, occluded=None, stream=None):
+        """rise_hip_intersect_any_ex: any hit within how_far under a face selection."""
+        import torch
+
+        n = rays.shape[1]
+        _check_rays(rays)
+        if occluded is None:
+            occluded = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        hf = 0 if how_far is None else how_far.data_ptr()
+        _lib.check(_lib.lib().rise_hip_intersect_any_ex(self.handle, rays.data_ptr(), hf, n, int(faces),
+                                                        occluded.data_ptr(), _stream_handle(stream)))
+        return occluded
+
     def save_cache(self, path) -> None:
         """Saves the built 8-wide tree (rise_hip_scene_cache_write)."""
         L = _lib.lib()
@@ -200,3 +250,26 @@ def host_trace(tris: np.ndarray, rays: np.ndarray, mode: int = 0, tmax: np.ndarr
         )
     )
     return occ, counters, stats.as_dict()
+
+
+def host_trace_ex(tris: np.ndarray, rays: np.ndarray, mode: int = 0, faces: int = FACES_BOTH,
+                  tmax: np.ndarray | None = None, arith: int = ARITH_IEEE):
+    """rise_hip_debug_trace_ex_host: the kernels' traversal with a face
+    selection on host threads (validation only).  Mode 0: structured hit
+    records (HIT_DTYPE); mode 1: occlusion bytes."""
+    mode = (mode & 1) | (2 if arith == ARITH_X86_FMA else 0)
+    L = _lib.lib()
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
+    if (mode & 1) == 0:
+        out = np.zeros(n, dtype=HIT_DTYPE)
+        _lib.check(L.rise_hip_debug_trace_ex_host(tris.ctypes.data, tris.shape[0], rays.ctypes.data, None, n, mode,
+                                                  int(faces), out.ctypes.data, None))
+        return out
+    occ = np.empty(n, dtype=np.uint8)
+    _lib.check(L.rise_hip_debug_trace_ex_host(tris.ctypes.data, tris.shape[0], rays.ctypes.data,
+                                              None if tm is None else tm.ctypes.data, n, mode, int(faces), None,
+                                              occ.ctypes.data))
+    return occ
