@@ -396,13 +396,64 @@ def widened_leg(args, tris):
     return out
 
 
+SIZE_CLASSES = (("small", 1 << 18), ("medium", 1 << 22), ("large", 1 << 62))
+
+
+def _size_class(r) -> str:
+    """small: <= 2^18 paths (e.g. 128^2 x 16 spp), medium: <= 2^22, large: above."""
+    p = r.width * r.height * r.spp
+    return next(name for name, lim in SIZE_CLASSES if p <= lim)
+
+
+def _serve(sv, rasts, timed):
+    """Submits (arrival s, scene, settings) requests on time and polls until
+    all are done.  Returns (makespan s, {ticket: latency s}, {ticket: index})."""
+    lat, idx = {}, {}
+    pending = list(enumerate(timed))
+    t_start = time.perf_counter()
+    while len(lat) < len(timed):
+        now = time.perf_counter() - t_start
+        while pending and pending[0][1][0] <= now:
+            i, (_, k, r) = pending.pop(0)
+            tk, _, _ = sv.submit(rasts[k], r)
+            idx[tk] = i
+        for tk, l in sv.poll():
+            lat[tk] = l
+        if pending and not sv.stats()["admitted"]:
+            time.sleep(max(0.0, min(0.001, pending[0][1][0] - (time.perf_counter() - t_start))))
+    return time.perf_counter() - t_start, lat, idx
+
+
+def _class_latencies(reqs, lat, idx, alone_s=None):
+    import numpy as np
+
+    out = {}
+    for name, _ in SIZE_CLASSES:
+        ls = [lat[t] * 1e3 for t, i in idx.items() if _size_class(reqs[i][2]) == name]
+        if not ls:
+            continue
+        e = {"requests": len(ls), "latency_ms_p50": round(float(np.percentile(ls, 50)), 1),
+             "latency_ms_p99": round(float(np.percentile(ls, 99)), 1)}
+        if alone_s is not None:
+            a = [alone_s[i] * 1e3 for i in range(len(reqs)) if _size_class(reqs[i][2]) == name]
+            e["alone_ms_mean"] = round(float(np.mean(a)), 2)
+        out[name] = e
+    return out
+
+
 def serving_leg(args):
     """Serving: 16 resident scenes of 50k-2M triangles (Lambertian, GGX and
-    glass), a Poisson stream (200 requests/s) of 64 render requests at
-    128^2-1024^2 and 1-64 spp.  The server admits requests as they arrive and
-    batches them continuously; latency is submit-to-finish per request.
-    Compared with the same requests rendered one at a time with
-    rise_hip_render, back to back."""
+    glass, half with fog), three streams:
+    * burst: 64 render requests at 128^2-1024^2 and 1-64 spp arriving as a
+      Poisson stream at 200 requests/s (faster than the server can render),
+    * sustained: the same 64 requests arriving at 60% of the burst's
+      measured throughput, so latency is service time rather than queueing,
+    * previews: 256 small requests (64^2-128^2, 1-4 spp) all at once.
+    The server admits requests as they arrive and batches them continuously;
+    latency is submit-to-finish per request, reported per size class
+    (small <= 2^18 paths < medium <= 2^22 < large).  Compared with the same
+    requests rendered one at a time with rise_hip_render, back to back (each
+    timed alone for the per-class means)."""
     import numpy as np
     import torch
 
@@ -422,31 +473,40 @@ def serving_leg(args):
     sv.drain()
     torch.cuda.synchronize()
     st0 = sv.stats()
-    lat = {}
-    pending = list(reqs)
-    t_start = time.perf_counter()
-    submitted = 0
-    while len(lat) < len(reqs):
-        now = time.perf_counter() - t_start
-        while pending and pending[0][0] <= now:
-            _, k, r = pending.pop(0)
-            tk, _, _ = sv.submit(rasts[k], r)
-            submitted += 1
-        for tk, l in sv.poll():
-            lat[tk] = l
-        if pending and not sv.stats()["admitted"]:
-            time.sleep(max(0.0, min(0.001, pending[0][0] - (time.perf_counter() - t_start))))
-    makespan = time.perf_counter() - t_start
+    makespan, lat, idx = _serve(sv, rasts, reqs)
     st = sv.stats()
     rays = (st["extension_rays"] - st0["extension_rays"]) + (st["shadow_rays"] - st0["shadow_rays"])
     lats = np.array(sorted(lat.values())) * 1e3
-    sv.release()
     # The same requests one at a time through the single-request path.
+    alone_s = []
     w0 = time.perf_counter()
     for _, k, r in reqs:
+        a0 = time.perf_counter()
+        rasts[k].rasterize_scene(r, stats=False)
+        torch.cuda.synchronize()
+        alone_s.append(time.perf_counter() - a0)
+    seq_s = time.perf_counter() - w0
+    burst_rps = len(reqs) / makespan
+    # Sustained stream at 60% of the burst throughput.
+    rate = 0.6 * burst_rps
+    arr = pt_scene.poisson_arrivals(len(reqs), rate)
+    sus = [(float(arr[i]), k, r) for i, (_, k, r) in enumerate(reqs)]
+    sus_makespan, sus_lat, sus_idx = _serve(sv, rasts, sus)
+    sus_l = np.array(sorted(sus_lat.values())) * 1e3
+    # Previews: many small requests at once, against one at a time.
+    prev = pt_scene.preview_requests(len(scenes))
+    prev_t = [(0.0, k, r) for k, r in prev]
+    _serve(sv, rasts, prev_t[:16])                      # warm-up of the small shapes
+    pv_makespan, pv_lat, pv_idx = _serve(sv, rasts, prev_t)
+    for k, r in prev[:16]:
         rasts[k].rasterize_scene(r, stats=False)
     torch.cuda.synchronize()
-    seq_s = time.perf_counter() - w0
+    p0 = time.perf_counter()
+    for k, r in prev:
+        rasts[k].rasterize_scene(r, stats=False)
+        torch.cuda.synchronize()
+    pv_seq = time.perf_counter() - p0
+    sv.release()
     out = {
         "workload": "Poisson 200 req/s, 64 requests over 16 resident scenes (50k-2M tris; Lambertian/GGX/glass; "
         "half with a ground-fog box, homogeneous sigma_s 0.5 sigma_a 0.05 HG g 0.3), "
@@ -454,7 +514,7 @@ def serving_leg(args):
         "requests": len(reqs),
         "paths": int(paths),
         "makespan_s": round(makespan, 3),
-        "requests_per_s": round(len(reqs) / makespan, 2),
+        "requests_per_s": round(burst_rps, 2),
         "mrays_per_s": round(rays / makespan / 1e6, 1),
         "paths_per_s": round(paths / makespan, 1),
         "latency_ms_p50": round(float(np.percentile(lats, 50)), 1),
@@ -462,9 +522,25 @@ def serving_leg(args):
         "passes": st["passes"] - st0["passes"],
         "sequential_single_request": {"s": round(seq_s, 3), "requests_per_s": round(len(reqs) / seq_s, 2)},
         "arrival_span_s": round(reqs[-1][0], 3),
+        "by_size": _class_latencies(reqs, lat, idx, alone_s),
+        "sustained": {"rate_per_s": round(rate, 2), "arrival_span_s": round(float(arr[-1]), 2),
+                      "makespan_s": round(sus_makespan, 3),
+                      "latency_ms_p50": round(float(np.percentile(sus_l, 50)), 1),
+                      "latency_ms_p99": round(float(np.percentile(sus_l, 99)), 1),
+                      "by_size": _class_latencies(sus, sus_lat, sus_idx)},
+        "previews": {"requests": len(prev), "sizes": "64^2-128^2, 1-4 spp",
+                     "served_s": round(pv_makespan, 3), "served_requests_per_s": round(len(prev) / pv_makespan, 1),
+                     "one_at_a_time_s": round(pv_seq, 3),
+                     "one_at_a_time_requests_per_s": round(len(prev) / pv_seq, 1),
+                     "speedup": round(pv_seq / pv_makespan, 2),
+                     "latency_ms_p50": round(float(np.percentile([v * 1e3 for v in pv_lat.values()], 50)), 1),
+                     "latency_ms_p99": round(float(np.percentile([v * 1e3 for v in pv_lat.values()], 99)), 1)},
     }
     log(f"serving: makespan {makespan:.2f} s, {out['requests_per_s']} req/s, {out['mrays_per_s']} Mrays/s, "
-        f"p50 {out['latency_ms_p50']} ms, p99 {out['latency_ms_p99']} ms; sequential {seq_s:.2f} s")
+        f"p50 {out['latency_ms_p50']} ms, p99 {out['latency_ms_p99']} ms; sequential {seq_s:.2f} s; "
+        f"by size {out['by_size']}")
+    log(f"serving: sustained {out['sustained']}")
+    log(f"serving: previews {out['previews']}")
     for r in rasts:
         r.release()
     return out

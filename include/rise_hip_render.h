@@ -288,7 +288,24 @@ typedef struct rise_hip_server_stats {
     int64_t submitted, completed, admitted, waiting;
     int64_t passes, paths, extension_rays, shadow_rays;  /* over completed passes */
     int64_t hbm_in_use;                                  /* film accumulators now allocated */
+    int64_t cancelled, timed_out, failed;                /* finished requests by their status */
+    int64_t alloc_retries;                               /* film allocations that failed and were retried */
 } rise_hip_server_stats;
+
+/* How a served request finished. */
+typedef enum rise_hip_request_status {
+    RISE_HIP_REQUEST_DONE = 0,       /* rendered; the film holds the result */
+    RISE_HIP_REQUEST_CANCELLED = 1,  /* rise_hip_server_cancel; the film is not written */
+    RISE_HIP_REQUEST_TIMED_OUT = 2,  /* its timeout passed before its last pass was launched */
+    RISE_HIP_REQUEST_FAILED = 3      /* its film accumulators could not be allocated */
+} rise_hip_request_status;
+
+typedef struct rise_hip_server_result {
+    uint64_t ticket;
+    int64_t latency_ns;  /* submit to finish */
+    int32_t status;      /* rise_hip_request_status */
+    int32_t reserved;
+} rise_hip_server_result;
 
 typedef struct rise_hip_server rise_hip_server;
 
@@ -309,6 +326,33 @@ rise_hip_status rise_hip_server_poll(rise_hip_server* server, uint64_t* tickets,
 /* As poll, but blocks until every submitted request has finished. */
 rise_hip_status rise_hip_server_drain(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
                                       size_t capacity, size_t* n_done);
+
+/* poll / drain reporting each finished request's status as well. */
+rise_hip_status rise_hip_server_poll_results(rise_hip_server* server, rise_hip_server_result* results,
+                                             size_t capacity, size_t* n_done);
+rise_hip_status rise_hip_server_drain_results(rise_hip_server* server, rise_hip_server_result* results,
+                                              size_t capacity, size_t* n_done);
+
+/* Cancels a request (the reference's IsCancelled check between passes,
+ * Rendering/PixelBasedRasterizerHelper.cpp:964-967).  A waiting request
+ * finishes at once; an admitted one gets no further chunks, its film is left
+ * unwritten and it finishes once the passes already holding its chunks have
+ * run.  A request whose last pass is already launched completes normally.
+ * Fails for an unknown or already finished ticket. */
+rise_hip_status rise_hip_server_cancel(rise_hip_server* server, uint64_t ticket);
+
+/* Ends the request as RISE_HIP_REQUEST_TIMED_OUT, as rise_hip_server_cancel
+ * does, if its last pass has not been launched `timeout_ns` after its
+ * submission (0: no timeout).  Checked at every poll and drain. */
+rise_hip_status rise_hip_server_set_timeout(rise_hip_server* server, uint64_t ticket, int64_t timeout_ns);
+
+/* Admission and its failures: a request whose film accumulators cannot be
+ * allocated stays first in the queue and is tried again at later calls, as
+ * memory frees; after 3 failures while nothing else is admitted it finishes
+ * as RISE_HIP_REQUEST_FAILED.  Test hook with no counterpart in the
+ * reference: the next n film allocations report out-of-memory without
+ * allocating, so tests can force that path from the host. */
+rise_hip_status rise_hip_server_fail_next_allocations(rise_hip_server* server, int32_t n);
 
 rise_hip_status rise_hip_server_get_stats(const rise_hip_server* server, rise_hip_server_stats* out_stats);
 

@@ -14,8 +14,15 @@ import itertools
 import time
 
 
-def candidates(persistent=(1, 2), trace_block=(64, 128, 256), max_paths=(1 << 24, 1 << 25), stack=(0,)):
+def candidates(persistent=(1, 2), trace_block=(64, 128, 256), max_paths=(1 << 24, 1 << 25), stack=(0, 16, -1)):
+    """The search space: every workgroup size, path-pool size and traversal
+    stack (0: the smallest LDS stack that fits the trees, 16 LDS entries per
+    lane, -1 private memory) for persistent traversal; one ray per thread
+    (persistent 2) with the automatic stack only, since its stack choice is
+    the same trade-off and it has never been the fastest."""
     for p, b, m, s in itertools.product(persistent, trace_block, max_paths, stack):
+        if p == 2 and s != 0:
+            continue
         yield {"persistent": p, "trace_block": b, "max_paths": m, "stack": s}
 
 

@@ -2341,6 +2341,7 @@ constexpr int kRing = 4;          // passes in flight
 constexpr int kServerDirLights = 1;  // directional lights a served scene may have (a sky's sun)
 constexpr int kServerSlots = 256; // admitted requests at most
 constexpr int kPassEntries = 1024; // chunks per server pass at most
+constexpr int kAllocRetries = 3;    // failed film allocations before a request that runs alone fails
 
 struct ServerReq {
     uint64_t ticket = 0;
@@ -2350,8 +2351,11 @@ struct ServerReq {
     int slot = -1;
     float* acc = nullptr;          // film accumulators, 12 floats per pixel
     size_t acc_bytes = 0;
-    int last_ring = -1;            // ring slot of the pass that holds the last chunk
-    uint64_t last_pass = 0;        // sequence number of that pass
+    bool any_launched = false;     // some chunk has been put in a pass
+    uint64_t last_pass = 0;        // sequence number of the last pass holding one of its chunks
+    int status = RISE_HIP_REQUEST_DONE;
+    int64_t deadline_ns = 0;       // 0: none
+    int alloc_failures = 0;        // failed film allocations so far
     int64_t submit_ns = 0;
     int64_t px = 0;
     int hits = 1;                  // bounce iterations this request needs
@@ -2395,7 +2399,8 @@ struct rise_hip_server {
     std::deque<ServerReq> waiting;
     std::vector<ServerReq> active;   // admission order
     std::vector<int> free_slots;
-    std::vector<std::pair<uint64_t, int64_t>> done;  // (ticket, latency ns) not yet reported
+    std::deque<rise_hip_server_result> done;  // finished, not yet reported
+    int inject_alloc_failures = 0;   // test hook: film allocations to report as failed
     rise_hip_server_stats stats{};
 };
 
@@ -2442,11 +2447,13 @@ void server_retire(rise_hip_server* sv, bool block)
     const int64_t t = now_ns();
     for (size_t i = 0; i < sv->active.size();) {
         ServerReq& r = sv->active[i];
-        if (r.next == r.chunks.size() && r.last_ring >= 0 && r.last_pass < sv->retired_upto) {
-            sv->done.emplace_back(r.ticket, t - r.submit_ns);
+        if (r.next == r.chunks.size() && (!r.any_launched || r.last_pass < sv->retired_upto)) {
+            sv->done.push_back(rise_hip_server_result{r.ticket, t - r.submit_ns, r.status, 0});
             sv->used -= (int64_t)r.acc_bytes;
             sv->free_slots.push_back(r.slot);
             sv->stats.completed += 1;
+            if (r.status == RISE_HIP_REQUEST_CANCELLED) sv->stats.cancelled += 1;
+            if (r.status == RISE_HIP_REQUEST_TIMED_OUT) sv->stats.timed_out += 1;
             sv->active.erase(sv->active.begin() + (long)i);
         } else {
             ++i;
@@ -2454,7 +2461,54 @@ void server_retire(rise_hip_server* sv, bool block)
     }
 }
 
-// Admits waiting requests in arrival order while memory and slots allow.
+// Ends a request early with `status` (cancelled or timed out).  A waiting
+// request finishes at once; an admitted one gets no further chunks, its
+// accumulators are freed in stream order after the passes already holding
+// its chunks, its film is left as it is (not resolved), and it finishes once
+// those passes have run.  A request whose last chunk is already in a pass
+// completes normally.  Returns false for an unknown or finished ticket.
+bool server_end(rise_hip_server* sv, uint64_t ticket, int status)
+{
+    for (size_t i = 0; i < sv->waiting.size(); ++i) {
+        if (sv->waiting[i].ticket != ticket) continue;
+        ServerReq& r = sv->waiting[i];
+        sv->done.push_back(rise_hip_server_result{r.ticket, now_ns() - r.submit_ns, status, 0});
+        sv->stats.completed += 1;
+        if (status == RISE_HIP_REQUEST_CANCELLED) sv->stats.cancelled += 1;
+        if (status == RISE_HIP_REQUEST_TIMED_OUT) sv->stats.timed_out += 1;
+        sv->waiting.erase(sv->waiting.begin() + (long)i);
+        return true;
+    }
+    for (ServerReq& r : sv->active) {
+        if (r.ticket != ticket) continue;
+        if (r.acc) {  // its last chunk is not in a pass yet
+            (void)hipFreeAsync(r.acc, sv->stream);
+            r.acc = nullptr;
+            r.next = r.chunks.size();
+            r.status = status;
+        }
+        return true;
+    }
+    return false;
+}
+
+// Ends every request whose deadline has passed.
+void server_expire(rise_hip_server* sv)
+{
+    const int64_t t = now_ns();
+    std::vector<uint64_t> late;
+    for (const ServerReq& r : sv->waiting)
+        if (r.deadline_ns > 0 && t > r.deadline_ns) late.push_back(r.ticket);
+    for (const ServerReq& r : sv->active)
+        if (r.deadline_ns > 0 && t > r.deadline_ns && r.acc) late.push_back(r.ticket);
+    for (uint64_t k : late) (void)server_end(sv, k, RISE_HIP_REQUEST_TIMED_OUT);
+}
+
+// Admits waiting requests in arrival order while memory and slots allow.  A
+// request whose film accumulators cannot be allocated stays at the head of
+// the queue and is tried again at a later call (memory frees as admitted
+// requests finish); one that fails kAllocRetries times while nothing else is
+// admitted is reported failed.
 rise_hip_status server_admit(rise_hip_server* sv)
 {
     while (!sv->waiting.empty() && !sv->free_slots.empty()) {
@@ -2464,7 +2518,26 @@ rise_hip_status server_admit(rise_hip_server* sv)
         ReqInfo info;
         rise_hip_status st = make_info(r.q, info);
         if (st != RISE_HIP_OK) return st;
-        hipError_t e = hipMallocAsync((void**)&r.acc, r.acc_bytes, sv->stream);
+        hipError_t e;
+        if (sv->inject_alloc_failures > 0) {
+            --sv->inject_alloc_failures;
+            e = hipErrorOutOfMemory;
+        } else {
+            e = hipMallocAsync((void**)&r.acc, r.acc_bytes, sv->stream);
+        }
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            r.acc = nullptr;
+            sv->stats.alloc_retries += 1;
+            if (++r.alloc_failures >= kAllocRetries && sv->active.empty()) {
+                sv->done.push_back(rise_hip_server_result{r.ticket, now_ns() - r.submit_ns, RISE_HIP_REQUEST_FAILED, 0});
+                sv->stats.completed += 1;
+                sv->stats.failed += 1;
+                sv->waiting.pop_front();
+                continue;
+            }
+            break;
+        }
         if (e != hipSuccess) return fail_hip(e, "film accumulators");
         const size_t f = 3 * (size_t)r.px;
         info.fsum = r.acc;
@@ -2508,6 +2581,8 @@ rise_hip_status server_launch(rise_hip_server* sv)
         for (size_t i = 0; i < sv->active.size(); ++i) {
             if (!taken[i]) continue;
             ServerReq& r = sv->active[i];
+            r.any_launched = true;
+            r.last_pass = sv->seq;
             bounces = std::max(bounces, r.hits);
             depth = std::max(depth, r.depth);
             tris = std::max(tris, r.tris);
@@ -2533,8 +2608,6 @@ rise_hip_status server_launch(rise_hip_server* sv)
             e = hipFreeAsync(r->acc, sv->stream);
             if (e != hipSuccess) return fail_hip(e, "film accumulators");
             r->acc = nullptr;
-            r->last_ring = k;
-            r->last_pass = sv->seq;
         }
         e = hipMemcpyAsync(sv->h_counters + (size_t)k * kCounters, cnt, kCounters * sizeof(unsigned),
                            hipMemcpyDeviceToHost, sv->stream);
@@ -2550,18 +2623,57 @@ rise_hip_status server_launch(rise_hip_server* sv)
     }
 }
 
-rise_hip_status server_report(rise_hip_server* sv, uint64_t* tickets, int64_t* latency_ns, size_t cap,
-                              size_t* n_done)
+rise_hip_status server_report(rise_hip_server* sv, uint64_t* tickets, int64_t* latency_ns,
+                              rise_hip_server_result* results, size_t cap, size_t* n_done)
 {
     size_t n = 0;
     while (n < cap && !sv->done.empty()) {
-        if (tickets) tickets[n] = sv->done.front().first;
-        if (latency_ns) latency_ns[n] = sv->done.front().second;
-        sv->done.erase(sv->done.begin());
+        const rise_hip_server_result& d = sv->done.front();
+        if (tickets) tickets[n] = d.ticket;
+        if (latency_ns) latency_ns[n] = d.latency_ns;
+        if (results) results[n] = d;
+        sv->done.pop_front();
         ++n;
     }
     if (n_done) *n_done = n;
     return RISE_HIP_OK;
+}
+
+rise_hip_status server_poll(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
+                            rise_hip_server_result* results, size_t capacity, size_t* n_done)
+{
+    clear_error();
+    if (!server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "server is NULL");
+    server_retire(server, false);
+    server_expire(server);
+    rise_hip_status st = server_admit(server);
+    if (st == RISE_HIP_OK) st = server_launch(server);
+    if (st != RISE_HIP_OK) return st;
+    return server_report(server, tickets, latency_ns, results, capacity, n_done);
+}
+
+rise_hip_status server_drain(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
+                             rise_hip_server_result* results, size_t capacity, size_t* n_done)
+{
+    clear_error();
+    if (!server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "server is NULL");
+    while (!server->waiting.empty() || !server->active.empty()) {
+        server_retire(server, false);
+        server_expire(server);
+        rise_hip_status st = server_admit(server);
+        if (st == RISE_HIP_OK) st = server_launch(server);
+        if (st != RISE_HIP_OK) return st;
+        if (!server->active.empty() || !server->waiting.empty()) {
+            // wait for the oldest pass in flight before going round again
+            int oldest = -1;
+            for (int k = 0; k < kRing; ++k)
+                if (server->busy[k] && (oldest < 0 || server->pass_seq[k] < server->pass_seq[oldest])) oldest = k;
+            if (oldest >= 0) (void)hipEventSynchronize(server->ev[oldest]);
+            else if (server->waiting.empty() && server->active.empty()) break;
+        }
+    }
+    server_retire(server, true);
+    return server_report(server, tickets, latency_ns, results, capacity, n_done);
 }
 
 }  // namespace
@@ -2650,36 +2762,60 @@ extern "C" rise_hip_status rise_hip_server_submit(rise_hip_server* server, const
 extern "C" rise_hip_status rise_hip_server_poll(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
                                                 size_t capacity, size_t* n_done)
 {
-    clear_error();
-    if (!server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "server is NULL");
-    server_retire(server, false);
-    rise_hip_status st = server_admit(server);
-    if (st == RISE_HIP_OK) st = server_launch(server);
-    if (st != RISE_HIP_OK) return st;
-    return server_report(server, tickets, latency_ns, capacity, n_done);
+    return server_poll(server, tickets, latency_ns, nullptr, capacity, n_done);
 }
 
 extern "C" rise_hip_status rise_hip_server_drain(rise_hip_server* server, uint64_t* tickets, int64_t* latency_ns,
                                                  size_t capacity, size_t* n_done)
 {
+    return server_drain(server, tickets, latency_ns, nullptr, capacity, n_done);
+}
+
+extern "C" rise_hip_status rise_hip_server_poll_results(rise_hip_server* server, rise_hip_server_result* results,
+                                                        size_t capacity, size_t* n_done)
+{
+    return server_poll(server, nullptr, nullptr, results, capacity, n_done);
+}
+
+extern "C" rise_hip_status rise_hip_server_drain_results(rise_hip_server* server, rise_hip_server_result* results,
+                                                         size_t capacity, size_t* n_done)
+{
+    return server_drain(server, nullptr, nullptr, results, capacity, n_done);
+}
+
+extern "C" rise_hip_status rise_hip_server_cancel(rise_hip_server* server, uint64_t ticket)
+{
     clear_error();
     if (!server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "server is NULL");
-    while (!server->waiting.empty() || !server->active.empty()) {
-        server_retire(server, false);
-        rise_hip_status st = server_admit(server);
-        if (st == RISE_HIP_OK) st = server_launch(server);
-        if (st != RISE_HIP_OK) return st;
-        if (!server->active.empty() || !server->waiting.empty()) {
-            // wait for the oldest pass in flight before going round again
-            int oldest = -1;
-            for (int k = 0; k < kRing; ++k)
-                if (server->busy[k] && (oldest < 0 || server->pass_seq[k] < server->pass_seq[oldest])) oldest = k;
-            if (oldest >= 0) (void)hipEventSynchronize(server->ev[oldest]);
-            else if (server->waiting.empty() && server->active.empty()) break;
+    if (!server_end(server, ticket, RISE_HIP_REQUEST_CANCELLED))
+        return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "unknown or finished ticket");
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_server_set_timeout(rise_hip_server* server, uint64_t ticket, int64_t timeout_ns)
+{
+    clear_error();
+    if (!server) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "server is NULL");
+    if (timeout_ns < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative timeout");
+    for (ServerReq& r : server->waiting)
+        if (r.ticket == ticket) {
+            r.deadline_ns = timeout_ns > 0 ? r.submit_ns + timeout_ns : 0;
+            return RISE_HIP_OK;
         }
-    }
-    server_retire(server, true);
-    return server_report(server, tickets, latency_ns, capacity, n_done);
+    for (ServerReq& r : server->active)
+        if (r.ticket == ticket) {
+            r.deadline_ns = timeout_ns > 0 ? r.submit_ns + timeout_ns : 0;
+            return RISE_HIP_OK;
+        }
+    return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "unknown or finished ticket");
+}
+
+extern "C" rise_hip_status rise_hip_server_fail_next_allocations(rise_hip_server* server, int32_t n)
+{
+    clear_error();
+    if (!server || n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "bad argument");
+    server->inject_alloc_failures = n;
+    return RISE_HIP_OK;
 }
 
 extern "C" rise_hip_status rise_hip_server_get_stats(const rise_hip_server* server, rise_hip_server_stats* out)

@@ -563,6 +563,26 @@ def terrain_spheres(nx: int = 2048, ny: int = 1024, subdiv: int = 4, seed: int =
     return s, r
 
 
+def preview_requests(n_scenes: int, n_requests: int = 256, seed: int = 6):
+    """Small render requests over `n_scenes` resident scenes (previews and
+    thumbnails: 64^2 or 128^2 pixels, 1, 2 or 4 spp, depth 6), as
+    [(scene index, RenderSettings)]: the request sizes where one-at-a-time
+    rendering is bound by per-request overhead rather than by the GPU."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_requests):
+        res = int(rng.choice([64, 128]))
+        r = RenderSettings(width=res, height=res, spp=int(rng.choice([1, 2, 4])), max_diffuse_bounce=6,
+                           max_glossy_bounce=6, max_transmission_bounce=6, seed=seed * 100000 + i)
+        out.append((int(rng.integers(n_scenes)), r))
+    return out
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 7) -> np.ndarray:
+    """Arrival times (s) of a Poisson stream of n requests at `rate` per second."""
+    return np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
+
+
 def serving_workload(seed: int = 5, n_requests: int = 64, n_scenes: int = 16, rate: float = 200.0):
     """The serving workload: `n_scenes` resident scenes of 50k-2M triangles
     (log-uniform; Cornell-style boxes with a seeded triangle cloud, every third

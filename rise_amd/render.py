@@ -150,7 +150,8 @@ class ServerConfig(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "admitted", "waiting", "passes", "paths",
-                                              "extension_rays", "shadow_rays", "hbm_in_use")]
+                                              "extension_rays", "shadow_rays", "hbm_in_use", "cancelled",
+                                              "timed_out", "failed", "alloc_retries")]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -179,8 +180,21 @@ _SIG = {
     "rise_hip_server_poll": ([_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "rise_hip_server_drain": ([_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "rise_hip_server_get_stats": ([_P, ctypes.POINTER(ServerStats)], ctypes.c_int),
+    "rise_hip_server_poll_results": ([_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "rise_hip_server_drain_results": ([_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "rise_hip_server_cancel": ([_P, ctypes.c_uint64], ctypes.c_int),
+    "rise_hip_server_set_timeout": ([_P, ctypes.c_uint64, ctypes.c_int64], ctypes.c_int),
+    "rise_hip_server_fail_next_allocations": ([_P, ctypes.c_int32], ctypes.c_int),
     "rise_hip_server_release": ([_P], ctypes.c_int),
 }
+
+
+REQUEST_DONE, REQUEST_CANCELLED, REQUEST_TIMED_OUT, REQUEST_FAILED = 0, 1, 2, 3
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("latency_ns", ctypes.c_int64), ("status", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 def _lib_render():
@@ -409,6 +423,7 @@ class RenderServer:
         self._keep = {}
         self._tk = (ctypes.c_uint64 * 512)()
         self._lat = (ctypes.c_int64 * 512)()
+        self._res = (ServerResult * 512)()
 
     def submit(self, rast, settings, spp=None, spp_offset: int = 0, spp_total=None, want_sq: bool = False):
         import torch
@@ -436,6 +451,38 @@ class RenderServer:
 
     def poll(self) -> list:
         return self._collect(_lib_render().rise_hip_server_poll)
+
+    def _collect_results(self, fn) -> list:
+        n = ctypes.c_size_t()
+        _lib.check(fn(self._h, self._res, 512, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = self._res[i]
+            out.append((int(r.ticket), r.latency_ns * 1e-9, int(r.status)))
+            self._keep.pop(int(r.ticket), None)
+        return out
+
+    def poll_results(self) -> list:
+        """Finished requests as (ticket, latency s, status: REQUEST_*)."""
+        return self._collect_results(_lib_render().rise_hip_server_poll_results)
+
+    def drain_results(self) -> list:
+        done = []
+        while True:
+            got = self._collect_results(_lib_render().rise_hip_server_drain_results)
+            done += got
+            if len(got) < 512:
+                return done
+
+    def cancel(self, ticket: int) -> None:
+        _lib.check(_lib_render().rise_hip_server_cancel(self._h, ticket))
+
+    def set_timeout(self, ticket: int, seconds: float) -> None:
+        _lib.check(_lib_render().rise_hip_server_set_timeout(self._h, ticket, int(seconds * 1e9)))
+
+    def fail_next_allocations(self, n: int) -> None:
+        """Test hook: the next n film allocations report out-of-memory."""
+        _lib.check(_lib_render().rise_hip_server_fail_next_allocations(self._h, n))
 
     def drain(self) -> list:
         done = []
