@@ -203,6 +203,9 @@ typedef struct rise_hip_render_stats {
     int64_t shadow_rays;     /* any-hit queries (next-event estimation) */
     int64_t paths;           /* camera samples */
     int64_t bounces;         /* extension passes launched */
+    int64_t ior_overflows;   /* refractions into a 4th nested dielectric: the index-of-refraction stack holds
+                                3 objects (the reference's is unbounded, Utilities/IORStack.h:95), so such a
+                                path goes on with the enclosing index; nonzero means results differ there */
 } rise_hip_render_stats;
 
 /* Renders samples [spp_offset, spp_offset + spp) of every pixel.
@@ -290,6 +293,7 @@ typedef struct rise_hip_server_stats {
     int64_t hbm_in_use;                                  /* film accumulators now allocated */
     int64_t cancelled, timed_out, failed;                /* finished requests by their status */
     int64_t alloc_retries;                               /* film allocations that failed and were retried */
+    int64_t ior_overflows;                               /* as rise_hip_render_stats::ior_overflows */
 } rise_hip_server_stats;
 
 /* How a served request finished. */

@@ -385,6 +385,14 @@ def render(scene, settings, s0: int = 0, s1: int | None = None, spp_total: int |
     return sm, sq, cnt
 
 
+def last_ior_deep_pushes() -> int:
+    """Refractions of the last render into a 4th nested dielectric (pushes a
+    3-object index-of-refraction stack would drop)."""
+    L = oracle_lib()
+    L.rise_oracle_last_ior_deep_pushes.restype = ctypes.c_uint64
+    return int(L.rise_oracle_last_ior_deep_pushes())
+
+
 def sobol_u32(index: int, dim: int, seed: int) -> int:
     """SobolSequence::Sample as the 32-bit scrambled value."""
     L = oracle_lib()

@@ -1013,7 +1013,13 @@ static rb_sample bsdf_scatter(const prep* P, const hit* h, double nm, double amb
 
 /* ---- index-of-refraction stack (IORStack.h) ------------------------------- */
 
-#define IOR_CAP 4 /* air + 3 nested objects; deeper pushes are dropped */
+/* IORStack (Utilities/IORStack.h:95) is a std::stack over a vector: unbounded.
+ * A path pushes at most one object per bounce, so air plus MAX_PATH_DEPTH + 1
+ * entries never fill.  The HIP renderer holds 3 objects; the pushes it would
+ * drop (a 4th nested object) are counted here so tests can compare. */
+#define IOR_CAP (MAX_PATH_DEPTH + 2)
+#define IOR_DEVICE_OBJECTS 3
+static uint64_t g_ior_deep_pushes; /* of the last render (atomic adds) */
 
 typedef struct {
     int n;
@@ -1643,6 +1649,8 @@ static double nm_path(prep* P, const hit* h0, double nm, int lane, int depth0, d
             if (!dielectric_scatter(P, &h, nm, &st, u_sampler(R, base + 2), &nd, &wgt, &type, &nst))
                 break;
             if (over_cap(s, type, &diffuse, &glossy, &transmission)) break;
+            if (nst.n > st.n && nst.n - 1 > IOR_DEVICE_OBJECTS)
+                __atomic_add_fetch(&g_ior_deep_pushes, 1, __ATOMIC_RELAXED);
             importance *= fabs(wgt);
             thr *= wgt;
             bsdf_pdf = 0.0;
@@ -1834,9 +1842,12 @@ static void* render_worker(void* arg)
     return NULL;
 }
 
+uint64_t rise_oracle_last_ior_deep_pushes(void) { return __atomic_load_n(&g_ior_deep_pushes, __ATOMIC_RELAXED); }
+
 int rise_oracle_render(const ro_scene* s, int32_t s0, int32_t s1, double* xyz_sum, double* xyz_sq,
                        uint64_t* counters, int n_threads)
 {
+    __atomic_store_n(&g_ior_deep_pushes, 0, __ATOMIC_RELAXED);
     if (!s || !xyz_sum || s->width <= 0 || s->height <= 0 || s0 < 0 || s1 < s0 || s1 > s->spp_total) return -1;
     for (int o = 0; o < s->n_objects; ++o) {
         const int32_t mi = s->object_material[o];

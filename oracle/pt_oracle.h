@@ -167,6 +167,10 @@ double rise_oracle_shadow_transmittance(const ro_scene* scene, const double o[3]
 /* CIE_Y_Integral (ColorUtils.cpp:327-350). */
 double rise_oracle_cie_y_integral(double a, double b);
 
+/* Refractions of the last rise_oracle_render into a 4th nested dielectric:
+ * the pushes a 3-object index-of-refraction stack would have to drop. */
+uint64_t rise_oracle_last_ior_deep_pushes(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,7 +67,10 @@ constexpr int64_t kChunkPaths = int64_t(1) << 20;   // paths per chunk (at most)
 #define RISE_MAX_PATHS_LOG2 25
 #endif
 constexpr int64_t kMaxPaths = int64_t(1) << RISE_MAX_PATHS_LOG2;  // paths in flight per pass (at most)
-constexpr int kCounters = 6 * (kMaxPathDepth + 1);  // extension, shadow, then 4 binned-hit counts, per pass
+// Per pass: extension rays, shadow rays and 4 binned-hit counts per depth,
+// then the index-of-refraction pushes dropped for lack of room.
+constexpr int kCounters = 6 * (kMaxPathDepth + 1) + 1;
+constexpr int kOverflowCounter = 6 * (kMaxPathDepth + 1);
 constexpr int kBins = 4;                            // hit bins: the 3 material kinds, then volume scatters
 constexpr int kMatVolume = 3;                       // bin of the lanes that scatter inside a medium
 constexpr int kMaxVolumeBounce = 64;                // StabilityConfig maxVolumeBounce (StabilityConfig.h:112)
@@ -190,6 +193,7 @@ struct Paths {
     unsigned* cnt_ext;   // [kMaxPathDepth + 1] of the current pass
     unsigned* cnt_sh;    // [kMaxPathDepth + 1]
     unsigned* cnt_mat;   // [kBins][kMaxPathDepth + 1]: binned hits per material kind, then volume scatters
+    unsigned* cnt_ovf;   // index-of-refraction pushes dropped (stack full)
     unsigned long long* fetch;  // [2 * (kMaxPathDepth + 1)]: work counters of the persistent traversal kernels
 };
 
@@ -510,8 +514,9 @@ __device__ inline unsigned bsdf_draws(const SampleKey& k, unsigned s0, float& u1
 
 // Index-of-refraction stack of one single-wavelength path (IORStack.h):
 // objects above the ambient air; the index of an entry is its dielectric's
-// index at the path's wavelength.  At most 3 nested objects; a deeper push is
-// dropped.
+// index at the path's wavelength.  At most 3 nested objects (the reference's
+// stack is unbounded, IORStack.h:95): a deeper push is dropped and counted,
+// and the count is reported as rise_hip_render_stats::ior_overflows.
 struct IorStack {
     int n;
     int obj[3];
@@ -534,9 +539,12 @@ __device__ inline bool ior_contains(const IorStack& s, int o)
     return c;
 }
 
-__device__ inline void ior_push(IorStack& s, int o)
+// Returns false when the stack is full and the push is dropped.
+__device__ inline bool ior_push(IorStack& s, int o)
 {
-    if (s.n < 3) s.obj[s.n++] = o;
+    if (s.n >= 3) return false;
+    s.obj[s.n++] = o;
+    return true;
 }
 
 __device__ inline void ior_pop(IorStack& s, int o)
@@ -556,8 +564,9 @@ __device__ inline void ior_pop(IorStack& s, int o)
 // origin of the ray that hit (for absorption over the segment inside).
 __device__ inline bool dielectric_scatter(const PtSceneView& sc, const DevMaterial& m, const Surf& h, f3 from,
                                           float nm, const IorStack& st, float u_select, f3& dir, float& weight,
-                                          int& type, IorStack& st_out)
+                                          int& type, IorStack& st_out, bool& dropped)
 {
+    bool push_dropped = false;
     const bool inside = ior_contains(st, h.object);
     const float rindex = spd_or(sc, m.ior_spd, nm, 1.33f);
     const float k_refr = inside ? powf(spd_or(sc, m.ext_spd, nm, 1.0f), len3(h.p - from)) : 1.0f;
@@ -582,7 +591,7 @@ __device__ inline bool dielectric_scatter(const PtSceneView& sc, const DevMateri
         const float top = ior_top(sc, st, nm);
         if (refract3(w, top, rindex, refr)) {
             ref = dielectric_reflectance(h.dir, refr, w, top, rindex);
-            ior_push(st_refr, h.object);
+            push_dropped = !ior_push(st_refr, h.object);
         } else {
             ref = 1.0f;
         }
@@ -609,11 +618,13 @@ __device__ inline bool dielectric_scatter(const PtSceneView& sc, const DevMateri
         if (kp > kNearZeroF) select = kp / total;
     }
     if (select < kNearZeroF) return false;
+    dropped = false;
     if (pick_a) {
         dir = refr;
         weight = ka / select;
         type = 2;
         st_out = st_refr;
+        dropped = push_dropped;
     } else {
         dir = refl;
         weight = kb / select;
@@ -1146,9 +1157,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                 int type = 0;
                 IorStack nst = ist;
                 unsigned nb = bst;
+                bool dropped = false;
                 if (valid && (!nm || (unsigned)w == lane)) {
                     if (dielectric_scatter(sc, mat, h, o, lam[w], ist, sample_u(key, base_dim + 2u), nd, wgt, type,
-                                           nst) &&
+                                           nst, dropped) &&
                         bump_bounces(nb, type, R)) {
                         // A shared ray's wavelength starts its own loop at throughput 1
                         // (PathTracingIntegrator.cpp:1593), its shared throughput kept aside.
@@ -1158,6 +1170,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(const ReqInfo* __restrict
                         cont = fabsf(t) <= 1e6f;
                     }
                 }
+                if (cont && dropped) atomicAdd(S.cnt_ovf, 1u);
                 const unsigned qi = push(cnt_next, cont);
                 if (cont)
                     write_continuation(S, surface_origin(h, nd), slot, (unsigned)w, qout + qi, nd, 0.0f, imp * fabsf(wgt),
@@ -2174,6 +2187,7 @@ Paths pass_counters(Paths S, unsigned* block)
     S.cnt_ext = block;
     S.cnt_sh = block + (kMaxPathDepth + 1);
     S.cnt_mat = block + 2 * (kMaxPathDepth + 1);
+    S.cnt_ovf = block + kOverflowCounter;
     return S;
 }
 
@@ -2212,6 +2226,7 @@ rise_hip_status read_stats(rise_hip_batch* b, hipStream_t stream, rise_hip_rende
             s.shadow_rays += c[kMaxPathDepth + 1 + d];
             if (c[d]) ++s.bounces;
         }
+        s.ior_overflows += c[kOverflowCounter];
         s.paths += b->pass_paths[p];
     }
     *out = s;
@@ -2437,6 +2452,7 @@ void server_retire(rise_hip_server* sv, bool block)
             sv->stats.extension_rays += c[d];
             sv->stats.shadow_rays += c[kMaxPathDepth + 1 + d];
         }
+        sv->stats.ior_overflows += c[kOverflowCounter];
         sv->stats.paths += sv->pass_paths[k];
     }
     // Passes complete in launch order on one stream: the oldest still busy bounds retired_upto.

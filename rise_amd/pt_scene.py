@@ -375,6 +375,24 @@ def furnace_with(kind: int, radiance: float = 1.0, alpha: float = 0.3, ior: floa
     return s, r
 
 
+def nested_glass(levels: int = 4, max_bounce: int = 64):
+    """The emitting box of furnace_with around `levels` concentric glass
+    spheres (radii 0.9 down to 0.3, indices 1.3, 1.45, 1.6, ...): a path that
+    refracts into the innermost of 4 nests 4 dielectrics, one more than the
+    HIP renderer's index-of-refraction stack holds."""
+    s, r = furnace_with(DIELECTRIC, max_bounce=max_bounce)
+    s.objects = s.objects[:6]
+    s.materials = s.materials[:1]
+    for k in range(levels):
+        s.spds.append(Spd.flat(1.3 + 0.15 * k))
+        s.materials.append(Material(kind=DIELECTRIC, ior=len(s.spds) - 1))
+        rad = 0.9 - 0.6 * k / max(1, levels - 1) if levels > 1 else 0.9
+        s.objects.append(Object(icosphere((0, 0, 0), rad, 3), len(s.materials) - 1))
+    s.camera = Camera(position=(0.0, 0.0, 1.8), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=30.0)
+    r.width = r.height = 16
+    return s, r
+
+
 def medium_shell(lo, hi, medium: int, material: int) -> Object:
     """A closed box shell (outward normals) holding medium `medium`, with a
     dielectric boundary `material` and no shadows of its own."""

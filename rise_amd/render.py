@@ -112,6 +112,7 @@ class RenderStats(ctypes.Structure):
         ("shadow_rays", ctypes.c_int64),
         ("paths", ctypes.c_int64),
         ("bounces", ctypes.c_int64),
+        ("ior_overflows", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -151,7 +152,7 @@ class ServerConfig(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "admitted", "waiting", "passes", "paths",
                                               "extension_rays", "shadow_rays", "hbm_in_use", "cancelled",
-                                              "timed_out", "failed", "alloc_retries")]
+                                              "timed_out", "failed", "alloc_retries", "ior_overflows")]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

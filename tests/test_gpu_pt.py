@@ -110,6 +110,29 @@ def test_gpu_sky_plane(cuda, case):
     assert frac >= 0.99 and mean_rel < 1e-3, (frac, mean_rel)
 
 
+def test_gpu_ior_stack_overflow_is_reported(cuda):
+    """The GPU's index-of-refraction stack holds 3 nested objects: with three
+    nested glass spheres nothing overflows and the film matches the oracle's
+    on the same streams; with four, the refractions into the innermost sphere
+    are reported (rise_hip_render_stats::ior_overflows) and their count is
+    the oracle's (whose stack is unbounded) up to the rare path that FP32
+    and FP64 send different ways."""
+    for levels in (3, 4):
+        log_case(f"nested glass, {levels} levels")
+        s, r = pt_scene.nested_glass(levels)
+        r.spp = 16
+        g, _, st = _render(s, r)
+        sm, _, _ = oracle.render(s, r)
+        deep = oracle.last_ior_deep_pushes()
+        if levels == 3:
+            assert st["ior_overflows"] == 0 and deep == 0
+            frac, mean_rel = _compare_same_streams(g, sm / r.spp, tol=1e-2)
+            assert frac >= 0.97 and mean_rel < 2e-3, (frac, mean_rel)
+        else:
+            assert deep > 1000
+            assert abs(st["ior_overflows"] / deep - 1.0) < 0.03, (st["ior_overflows"], deep)
+
+
 def test_gpu_emitter_seen_directly(cuda):
     log_case("emitter seen directly")
     s, r = pt_scene.cornell_box()

@@ -114,6 +114,24 @@ def test_sky_plane_closed_form(kw):
     assert abs(y - want) < 4 * se + 1e-3 * want, (y, want, se)
 
 
+def test_ior_stack_is_unbounded():
+    """The index-of-refraction stack follows the reference's unbounded
+    IORStack (Utilities/IORStack.h:95): four nested lossless glass spheres in
+    an emitting box still show radiance 1, and the refractions into the 4th
+    nested sphere (which the HIP renderer's 3-object stack reports as
+    overflows) are counted; with three nested spheres there are none."""
+    s, r = pt_scene.nested_glass(3)
+    r.spp = 8
+    oracle.render(s, r)
+    assert oracle.last_ior_deep_pushes() == 0
+    s, r = pt_scene.nested_glass(4)
+    r.spp = 16
+    sm, sq, _ = oracle.render(s, r)
+    assert oracle.last_ior_deep_pushes() > 1000
+    y, se = _y_stats(sm, sq, r.width * r.height * r.spp)
+    assert abs(y - 1.0) < 4 * se + 2e-3, (y, se)
+
+
 def test_unshuffled_sobol_is_biased_on_the_spectral_furnace():
     """The reference's padded Sobol draws every dimension pair at the same
     index (SobolSequence.h:180-200), so dimensions 0, 2, 4, ... are hash
