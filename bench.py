@@ -59,6 +59,76 @@ def copy_calibration(steps: int) -> None:
     log(f"copied {steps} x {n * 4} bytes")
 
 
+def load_kernel_counters(leg: str):
+    """Per-kernel HBM traffic and VALU busy of a render leg from the committed
+    rocprofv3 summary (profiles/path_tracing_kernel_counters.json), or None."""
+    p = ROOT / "profiles" / "path_tracing_kernel_counters.json"
+    if not p.exists():
+        return None
+    d = json.loads(p.read_text())
+    k = d.get("legs", {}).get(leg)
+    return {"from": "profiles/path_tracing_kernel_counters_batch_terrain.md", "kernels": k} if k else None
+
+
+def camera_rays(cam, width: int, height: int):
+    """Pinhole rays through the pixel centres of `cam` (pt_scene.Camera), SoA (6, n)."""
+    import numpy as np
+
+    pos, at, up = (np.asarray(v, np.float64) for v in (cam.position, cam.look_at, cam.up))
+    w = at - pos
+    w /= np.linalg.norm(w)
+    u = np.cross(w, up)
+    u /= np.linalg.norm(u)
+    v = np.cross(u, w)
+    half = np.tan(np.radians(cam.fov_deg) / 2.0)
+    xs = ((np.arange(width) + 0.5) / width * 2.0 - 1.0) * half * width / height
+    ys = (1.0 - (np.arange(height) + 0.5) / height * 2.0) * half
+    X, Y = np.meshgrid(xs, ys)
+    d = w[None, None] + X[..., None] * u + Y[..., None] * v
+    d = d.reshape(-1, 3)
+    o = np.broadcast_to(pos, d.shape)
+    return np.ascontiguousarray(np.concatenate([o.T, d.T], 0), np.float32)
+
+
+def terrain_cpu_baseline(scene, rs) -> dict | None:
+    """The reference's own BVH4 mesh intersection (oracle/_ref, its Linux-flags
+    build) on the terrain: closest hits of the 1920x1080 camera rays, then
+    shadow rays from those hits toward the sun, on the host threads.  Path
+    tracing adds sampling, shading and film work to these queries, so this
+    rate bounds the reference's path-tracing Mrays/s from above."""
+    import numpy as np
+
+    import oracle
+
+    if not oracle.ref_available("fast"):
+        return None
+    tris, _ = scene.flat()
+    t0 = time.perf_counter()
+    ref = oracle.RefMesh(tris, variant="fast")
+    build_s = time.perf_counter() - t0
+    th = cpu_threads()
+    prim = camera_rays(scene.camera, rs.width, rs.height)
+    t0 = time.perf_counter()
+    pid, _, t64 = ref.closest(prim, threads=th)
+    t_closest = time.perf_counter() - t0
+    hit = pid >= 0
+    dn = prim[3:][:, hit] / np.linalg.norm(prim[3:][:, hit], axis=0)
+    p = prim[:3][:, hit] + dn * (t64[hit] * (1 - 1e-6))
+    el, az = np.radians(scene.sky.elevation_deg), np.radians(scene.sky.azimuth_deg)
+    sun = np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
+    shadow = np.ascontiguousarray(np.concatenate([p, np.repeat(sun[:, None], p.shape[1], 1)], 0), np.float32)
+    t0 = time.perf_counter()
+    ref.any_hit(shadow, threads=th)
+    t_any = time.perf_counter() - t0
+    ref.close()
+    n = prim.shape[1] + shadow.shape[1]
+    return {"value": round(n / (t_closest + t_any) / 1e6, 3), "unit": "Mrays/s", "cores": th, "kind": "reference",
+            "sample": f"{prim.shape[1]} camera rays (closest hit) + {shadow.shape[1]} sun shadow rays (any hit) "
+                      f"on the terrain scene, RISE BVH4 mesh path built with its Linux flags; intersection only, "
+                      f"an upper bound on its path-tracing rate; {t_closest + t_any:.1f} s of work; BVH build "
+                      f"{build_s:.1f} s not counted"}
+
+
 def load_traffic():
     p = ROOT / "profiles" / "closest_hit_traffic.json"
     if not p.exists():
@@ -272,9 +342,15 @@ def terrain_leg(args):
         "rays_per_render": {"extension": stats["extension_rays"], "shadow": stats["shadow_rays"]},
         "mean_Y": round(y, 6),
         "setup_s": round(setup, 1),
+        "kernel_counters": load_kernel_counters("terrain"),
     }
     log(f"terrain: {ms:.0f} ms, {out['mrays_per_s']} Mrays/s, {out['paths_per_s']:.3g} paths/s "
         f"({scene.n_tris} tris, setup {setup:.1f} s)")
+    if not args.no_cpu_baseline:
+        cb = terrain_cpu_baseline(scene, rs)
+        if cb:
+            out["cpu_baseline"] = cb
+            log(f"terrain CPU reference (intersection only): {cb['value']} Mrays/s on {cb['cores']} threads")
     if not args.no_quality:
         import dataclasses
 
@@ -649,6 +725,7 @@ def batch_leg(args):
         "tuning": batch.tuning(),
         "autotune": tune_log,
         "films_identical_alone_vs_batched": bool(identical),
+        "kernel_counters": load_kernel_counters("batch"),
     }
     log(f"batch: sequential {seq_s * 1e3:.1f} ms, batched {batch_ms:.1f} ms, graph {graph_ms:.1f} ms, "
         f"{out['batched']['mrays_per_s']} Mrays/s, identical films {identical}")
