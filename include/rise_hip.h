@@ -87,6 +87,41 @@ typedef struct rise_hip_scene_stats {
 rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n_meshes,
                                       rise_hip_scene** out_scene);
 
+/* One object of an ObjectManager: a mesh placed by its transform
+ * (Objects/Object.h; Utilities/Transformable.cpp:438-449).  Matrices are
+ * RISE's Matrix4 in member order _00, _01, ..., _33 (row-vector convention:
+ * a point p maps to p.x row 0 + p.y row 1 + p.z row 2 + row 3,
+ * Utilities/Math3D/PointsOps.h:126-133); the fourth column is not read. */
+typedef struct rise_hip_object_instance {
+    int32_t mesh;          /* index into the meshes of rise_hip_scene_upload_objects */
+    int32_t reserved;      /* must be 0 */
+    double to_world[16];   /* m_mxFinalTrans */
+    double to_object[16];  /* m_mxInvFinalTrans (Matrix4Ops::Inverse of to_world) */
+} rise_hip_object_instance;
+
+/* Uploads a scene of objects: meshes in their own (object) space, each placed
+ * by one or more objects.  Ray queries on the scene follow
+ * ObjectManager::IntersectRay / IntersectShadowRay over the objects
+ * (Managers/ObjectManager.cpp:270-373) with Object::IntersectRay /
+ * IntersectRay_IntersectionOnly per object (Objects/Object.cpp:392-712):
+ * the world ray is taken into the object's frame in double (origin by
+ * to_object, direction by to_object's linear part, then normalized), the
+ * mesh is intersected there, and the hit point, backed off 1e-12, goes back
+ * to world space by to_world; the world distance from the ray origin to that
+ * point is the hit's range, and the strictly smallest range wins in object
+ * order.  A shadow ray's dHowFar is scaled by |to_object d| per object.
+ * Objects whose two matrices are exactly the identity share one world-space
+ * tree; every other object is intersected in its own frame through its
+ * mesh's object-space tree (built once per mesh).  Triangle ids: the mesh's
+ * tri_ids, or the triangle's index within its mesh.  On this scene,
+ * rise_hip_hit.mesh holds the index of the object hit, and t /
+ * rise_hip_hit.range the world range.  The _counted queries and the scene
+ * cache are not supported on a scene with transformed objects
+ * (RISE_HIP_ERR_NOT_SUPPORTED).  Synchronous. */
+rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, size_t n_meshes,
+                                              const rise_hip_object_instance* objects, size_t n_objects,
+                                              rise_hip_scene** out_scene);
+
 rise_hip_status rise_hip_scene_release(rise_hip_scene* scene);
 
 rise_hip_status rise_hip_scene_get_stats(const rise_hip_scene* scene, rise_hip_scene_stats* out_stats);

@@ -53,6 +53,23 @@ rise_hip_status rise_hip_debug_trace_ex_host(const float* tris, int64_t n_tris, 
 rise_hip_status rise_hip_debug_node_accept_host(const void* node, const float ray[6], double best,
                                                 uint32_t* accept);
 
+/* rise_hip_scene_upload_objects followed by rise_hip_intersect_closest_ex
+ * (mode bit 0 clear: fills hits) or rise_hip_intersect_any_ex (bit 0 set:
+ * fills occluded; tmax may be NULL), on the host: the same trees and the same
+ * per-ray code.  mode bit 1 selects RISE_HIP_ARITH_X86_FMA.  All buffers are
+ * host memory. */
+rise_hip_status rise_hip_debug_objects_host(const rise_hip_mesh_desc* meshes, size_t n_meshes,
+                                            const rise_hip_object_instance* objects, size_t n_objects,
+                                            const float* rays_soa, const float* tmax, int64_t n, int32_t mode,
+                                            int32_t faces, rise_hip_hit* hits, uint8_t* occluded);
+
+/* The kernels' double slab test of one 8-wide node against the object-space
+ * ray that the world ray (ox, oy, oz, dx, dy, dz) becomes under to_object
+ * (16 doubles, Matrix4 order), on the host; `best` is a distance along the
+ * object-space normalized direction. */
+rise_hip_status rise_hip_debug_node_accept_obj_host(const void* node, const float ray[6], const double to_object[16],
+                                                    double best, uint32_t* accept);
+
 #ifdef __cplusplus
 }
 #endif

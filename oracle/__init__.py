@@ -73,6 +73,7 @@ NO_FILTER = 1
 X86_FMA = 2
 NO_FRONT = 4
 NO_BACK = 8
+OBJ_PREHIT = 16
 
 
 def _faces_opts(faces: int) -> int:
@@ -136,6 +137,73 @@ def closest_attr(tris: np.ndarray, rays: np.ndarray, faces: int = 3, arith: str 
                                *[out[k].ctypes.data for k in ("prim_id", "range", "alpha", "beta", "normal", "point")],
                                _opts(False, arith) | _faces_opts(faces), threads)
     return out
+
+
+# rise_oracle_object: mesh index, reserved, to_world and to_object (Matrix4
+# member order, 16 doubles each).
+OBJECT_DTYPE = np.dtype([("mesh", "<i4"), ("reserved", "<i4"), ("to_world", "<f8", (16,)),
+                         ("to_object", "<f8", (16,))])
+
+
+def _objects(objects) -> np.ndarray:
+    o = np.ascontiguousarray(objects)
+    if o.dtype != OBJECT_DTYPE:
+        o = np.ascontiguousarray(o.astype(OBJECT_DTYPE))
+    return o
+
+
+def _meshes(meshes):
+    arrs = [np.ascontiguousarray(m, dtype=np.float32).reshape(-1, 9) for m in meshes]
+    off = np.zeros(len(arrs) + 1, np.int64)
+    off[1:] = np.cumsum([a.shape[0] for a in arrs])
+    tris = np.ascontiguousarray(np.concatenate(arrs) if arrs else np.zeros((0, 9), np.float32))
+    return tris, off
+
+
+def objects_closest(meshes, objects, rays: np.ndarray, faces: int = 3, arith: str = "ieee", threads: int = 0,
+                    flags: bool = False, prehit: bool = False) -> dict:
+    """Closest hit over a scene of objects (rise_oracle_objects_closest):
+    prim_id (index within the mesh), object, range, alpha, beta, normal (n, 3),
+    point (n, 3), and with flags=True "flags" (traversal-order dependent).
+    prehit: model Object::IntersectRay's box pre-test (RISE_ORACLE_OBJ_PREHIT)."""
+    L = oracle_lib()
+    L.rise_oracle_objects_closest.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P, _I64] + [_P] * 8 + [
+        ctypes.c_int, ctypes.c_int]
+    tris, off = _meshes(meshes)
+    obj = _objects(objects)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    out = {"prim_id": np.empty(n, np.int32), "object": np.empty(n, np.int32), "range": np.empty(n),
+           "alpha": np.empty(n), "beta": np.empty(n), "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
+    fl = np.empty(n, np.uint8) if flags else None
+    L.rise_oracle_objects_closest(tris.ctypes.data, off.ctypes.data, len(off) - 1, obj.ctypes.data, len(obj),
+                                  rays.ctypes.data, n,
+                                  *[out[k].ctypes.data for k in ("prim_id", "object", "range", "alpha", "beta",
+                                                                 "normal", "point")],
+                                  None if fl is None else fl.ctypes.data,
+                                  _opts(False, arith) | _faces_opts(faces) | (OBJ_PREHIT if prehit else 0), threads)
+    if flags:
+        out["flags"] = fl
+    return out
+
+
+def objects_any(meshes, objects, rays: np.ndarray, tmax: np.ndarray | None = None, faces: int = 3,
+                arith: str = "ieee", threads: int = 0, prehit: bool = False) -> np.ndarray:
+    """Any hit over a scene of objects (rise_oracle_objects_any); prehit as
+    for objects_closest."""
+    L = oracle_lib()
+    L.rise_oracle_objects_any.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P, _P, _I64, _P, ctypes.c_int,
+                                          ctypes.c_int]
+    tris, off = _meshes(meshes)
+    obj = _objects(objects)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    occ = np.empty(n, np.uint8)
+    tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
+    L.rise_oracle_objects_any(tris.ctypes.data, off.ctypes.data, len(off) - 1, obj.ctypes.data, len(obj),
+                              rays.ctypes.data, None if tm is None else tm.ctypes.data, n, occ.ctypes.data,
+                              _opts(False, arith) | _faces_opts(faces) | (OBJ_PREHIT if prehit else 0), threads)
+    return occ
 
 
 def filter_float(o, d, p0, e1, e2, best: float) -> bool:
@@ -242,6 +310,49 @@ class RefMesh:
             self.close()
         except Exception:
             pass
+
+
+def ref_objects_closest(ref_meshes, objects, rays: np.ndarray, faces: int = 3, bary: bool = False,
+                        threads: int = 0) -> dict:
+    """ObjectManager::IntersectRay over objects on reference meshes
+    (rise_ref_objects_closest); ref_meshes: RefMesh list of one variant."""
+    L = ref_meshes[0]._L
+    L.rise_ref_objects_closest.argtypes = [_P, _P, _P, _P, ctypes.c_int, _P, _I64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int] + [_P] * 7 + [ctypes.c_int]
+    obj = _objects(objects)
+    handles = (ctypes.c_void_p * len(ref_meshes))(*[m._h for m in ref_meshes])
+    om = np.ascontiguousarray(obj["mesh"], np.int32)
+    tw = np.ascontiguousarray(obj["to_world"], np.float64)
+    to = np.ascontiguousarray(obj["to_object"], np.float64)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    out = {"prim_id": np.empty(n, np.int32), "object": np.empty(n, np.int32), "range": np.empty(n),
+           "alpha": np.empty(n), "beta": np.empty(n), "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
+    L.rise_ref_objects_closest(handles, om.ctypes.data, tw.ctypes.data, to.ctypes.data, len(obj), rays.ctypes.data,
+                               n, faces & 1, (faces >> 1) & 1, int(bary),
+                               *[out[k].ctypes.data for k in ("prim_id", "object", "range", "alpha", "beta",
+                                                              "normal", "point")], threads)
+    return out
+
+
+def ref_objects_any(ref_meshes, objects, rays: np.ndarray, tmax: np.ndarray | None = None, faces: int = 3,
+                    threads: int = 0) -> np.ndarray:
+    """ObjectManager::IntersectShadowRay over objects (rise_ref_objects_any)."""
+    L = ref_meshes[0]._L
+    L.rise_ref_objects_any.argtypes = [_P, _P, _P, ctypes.c_int, _P, _P, _I64, ctypes.c_int, ctypes.c_int, _P,
+                                       ctypes.c_int]
+    obj = _objects(objects)
+    handles = (ctypes.c_void_p * len(ref_meshes))(*[m._h for m in ref_meshes])
+    om = np.ascontiguousarray(obj["mesh"], np.int32)
+    to = np.ascontiguousarray(obj["to_object"], np.float64)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    occ = np.empty(n, np.uint8)
+    tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
+    L.rise_ref_objects_any(handles, om.ctypes.data, to.ctypes.data, len(obj), rays.ctypes.data,
+                           None if tm is None else tm.ctypes.data, n, faces & 1, (faces >> 1) & 1, occ.ctypes.data,
+                           threads)
+    return occ
 
 
 def ref_available(variant: str = "ieee") -> bool:

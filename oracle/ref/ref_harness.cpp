@@ -10,18 +10,22 @@
 //     -> BVH::IntersectRay4 -> MollerTrumboreFloat -> RayElementIntersection -> RayTriangleIntersection
 //
 // The Object layer (Objects/Object.cpp) cannot be built from a few sources
-// (it links the whole material library), so for an identity transform this
-// harness performs Object::IntersectRay's two steps around the mesh call with
-// the reference's own Math3D operations: normalize the direction
+// (it links the whole material library), so this harness performs
+// Object::IntersectRay's steps around the mesh call with the reference's own
+// Math3D operations: for an identity transform, normalize the direction
 // (Object.cpp:414-425) and recompute the world range after the 1e-12
-// back-off (Object.cpp:641-643); for the shadow ray, the dHowFar scaling of
-// Object.cpp:472-493.
+// back-off (Object.cpp:641-643), and for the shadow ray scale dHowFar
+// (Object.cpp:472-493); for objects with transforms, every step of
+// Object::IntersectRay / _IntersectionOnly in an ObjectManager's linear loop
+// (rise_ref_objects_closest / _any).
 //
 // Triangle ids: triangle i gets its own texture coordinate (i, 0) on all three
 // corners, so the reference's interpolated ri.ptCoord.x returns i exactly.
 
 #include "Geometry/TriangleMeshGeometryIndexed.h"
 #include "Intersection/RayIntersectionGeometric.h"
+#include "Intersection/RayPrimitiveIntersections.h"
+#include "Utilities/GeometricUtilities.h"
 #include "Utilities/MemoryBuffer.h"
 #include "RasterImages/HDRWriter.h"
 #include "RasterImages/PPMWriter.h"
@@ -310,6 +314,120 @@ void rise_ref_any(void* handle, const float* rays, const float* tmax, int64_t n,
         Scalar how_far2 = how_far;
         if ((how_far != RISE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
         occluded[i] = h->mesh->IntersectRay_IntersectionOnly(Ray(o, dn), how_far2, true, true) ? 1 : 0;
+    });
+}
+
+// A scene of objects with transforms: ObjectManager::IntersectRay's linear
+// loop (Managers/ObjectManager.cpp:290-310) around Object::IntersectRay
+// (Objects/Object.cpp:392-649), performed here step by step with the
+// reference's Math3D operations around the genuine mesh call, since the
+// Object class links the whole material library: frame change
+// (Object.cpp:414-425), dHowFar scaling (:472-493), the box pre-test
+// (:513-527), the normal transform (:531-533) and the world point and range
+// (:637-643).  meshes: handles from rise_ref_mesh_create(_bary); per object
+// k: obj_mesh[k] and 16 + 16 doubles of to_world / to_object (Matrix4
+// order).  prim_id is the triangle's index within its mesh; alpha / beta are
+// read from ri.vNormal (meshes from rise_ref_mesh_create_bary), else 0.
+void rise_ref_objects_closest(void** meshes, const int32_t* obj_mesh, const double* to_world, const double* to_object,
+                              int n_objs, const float* rays, int64_t n, int front, int back, int bary,
+                              int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
+                              double* normal, double* point, int n_threads)
+{
+    std::vector<Matrix4> M(n_objs), Mi(n_objs), MiT(n_objs);
+    std::vector<BoundingBox> box(n_objs);
+    for (int k = 0; k < n_objs; ++k) {
+        const double* a = to_world + 16 * k;
+        const double* b = to_object + 16 * k;
+        M[k] = Matrix4(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11], a[12], a[13], a[14],
+                       a[15]);
+        Mi[k] = Matrix4(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8], b[9], b[10], b[11], b[12], b[13], b[14],
+                        b[15]);
+        MiT[k] = Matrix4Ops::Transpose(Mi[k]);
+        box[k] = ((RefMesh*)meshes[obj_mesh[k]])->mesh->GenerateBoundingBox();
+    }
+    parallel_rays(n, n_threads, [&](int64_t i) {
+        const Point3 o(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        const Vector3 d(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        Scalar best = RISE_INFINITY;
+        prim_id[i] = -1;
+        object[i] = -1;
+        range[i] = INFINITY;
+        alpha[i] = beta[i] = 0.0;
+        for (int c = 0; c < 3; ++c) normal[3 * i + c] = point[3 * i + c] = 0.0;
+        for (int k = 0; k < n_objs; ++k) {
+            const Point3 op = Point3Ops::Transform(Mi[k], o);
+            const Vector3 dl = Vector3Ops::Transform(Mi[k], d);
+            const Scalar mag = Vector3Ops::Magnitude(dl);
+            const Ray r(op, Vector3Ops::Normalize(dl));
+            const Scalar factor = (mag > NEARZERO) ? mag : Scalar(1.0);
+            Scalar how_far2 = best;
+            if ((best != RISE_INFINITY) || (factor < 1.0)) how_far2 = factor * best;
+            BOX_HIT bh;
+            RayBoxIntersection(r, bh, box[k].ll, box[k].ur);
+            if (!bh.bHit) continue;
+            if (!(bh.dRange2 < 0) && bh.dRange > how_far2) continue;
+            const RasterizerState rs = {0, 0};
+            RayIntersectionGeometric ri(r, rs);
+            ((RefMesh*)meshes[obj_mesh[k]])->mesh->IntersectRay(ri, front != 0, back != 0, false);
+            if (!ri.bHit) continue;
+            const Point3 pobj = ri.ray.PointAtLength(ri.range - 1e-12);
+            const Point3 pw = Point3Ops::Transform(M[k], pobj);
+            const Scalar w = Vector3Ops::Magnitude(Vector3Ops::mkVector3(pw, o));
+            if (!(w < best)) continue;
+            best = w;
+            prim_id[i] = (int32_t)ri.ptCoord.x;
+            object[i] = k;
+            range[i] = w;
+            alpha[i] = bary ? ri.vNormal.y : 0.0;
+            beta[i] = bary ? ri.vNormal.z : 0.0;
+            const Vector3 g = Vector3Ops::Normalize(Vector3Ops::Transform(MiT[k], ri.vGeomNormal));
+            normal[3 * i] = g.x;
+            normal[3 * i + 1] = g.y;
+            normal[3 * i + 2] = g.z;
+            point[3 * i] = pw.x;
+            point[3 * i + 1] = pw.y;
+            point[3 * i + 2] = pw.z;
+        }
+    });
+}
+
+// ObjectManager::IntersectShadowRay's loop (ObjectManager.cpp:320-373) around
+// Object::IntersectRay_IntersectionOnly (Object.cpp:652-712), as above.
+void rise_ref_objects_any(void** meshes, const int32_t* obj_mesh, const double* to_object, int n_objs,
+                          const float* rays, const float* tmax, int64_t n, int front, int back, uint8_t* occluded,
+                          int n_threads)
+{
+    std::vector<Matrix4> Mi(n_objs);
+    std::vector<BoundingBox> box(n_objs);
+    for (int k = 0; k < n_objs; ++k) {
+        const double* b = to_object + 16 * k;
+        Mi[k] = Matrix4(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8], b[9], b[10], b[11], b[12], b[13], b[14],
+                        b[15]);
+        box[k] = ((RefMesh*)meshes[obj_mesh[k]])->mesh->GenerateBoundingBox();
+    }
+    parallel_rays(n, n_threads, [&](int64_t i) {
+        const Point3 o(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        const Vector3 d(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        const Scalar how_far = tmax ? (Scalar)tmax[i] : RISE_INFINITY;
+        uint8_t occ = 0;
+        for (int k = 0; k < n_objs && !occ; ++k) {
+            const Point3 op = Point3Ops::Transform(Mi[k], o);
+            const Vector3 dl = Vector3Ops::Transform(Mi[k], d);
+            const Scalar mag = Vector3Ops::Magnitude(dl);
+            const Ray r(op, Vector3Ops::Normalize(dl));
+            const Scalar factor = (mag > NEARZERO) ? mag : Scalar(1.0);
+            Scalar how_far2 = how_far;
+            if ((how_far != RISE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
+            BOX_HIT bh;
+            RayBoxIntersection(r, bh, box[k].ll, box[k].ur);
+            if (!bh.bHit) continue;
+            if (bh.dRange > how_far2 && !GeometricUtilities::IsPointInsideBox(r.origin, box[k].ll, box[k].ur))
+                continue;
+            if (((RefMesh*)meshes[obj_mesh[k]])->mesh->IntersectRay_IntersectionOnly(r, how_far2, front != 0,
+                                                                                      back != 0))
+                occ = 1;
+        }
+        occluded[i] = occ;
     });
 }
 

@@ -441,3 +441,331 @@ void rise_oracle_closest_attr(const float* tris, int64_t n_tris, const float* ra
     run(&j, closest_worker, n_threads);
     free(tt);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Scenes of objects with transforms                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Point3Ops::Transform (Utilities/Math3D/PointsOps.h:126-133) and
+ * Vector3Ops::Transform (VectorsOps.h:241-248): Matrix4 members in order
+ * _00.._33, m[4 i + j] = m._ij, out_j = m._0j x + m._1j y + m._2j z (+ m._3j
+ * for a point), evaluated left to right. */
+static void xpoint(int x, const double* m, const double p[3], double out[3])
+{
+    for (int j = 0; j < 3; ++j) out[j] = d364(x, m[j], p[0], m[4 + j], p[1], m[8 + j], p[2]) + m[12 + j];
+}
+static void xvector(int x, const double* m, const double v[3], double out[3])
+{
+    for (int j = 0; j < 3; ++j) out[j] = d364(x, m[j], v[0], m[4 + j], v[1], m[8 + j], v[2]);
+}
+
+typedef struct {
+    const tri64* tris;          /* all meshes, concatenated */
+    const int64_t* mesh_off;    /* n_meshes + 1 offsets into tris */
+    const rise_oracle_object* objs;
+    int n_objs;
+    const float* rays;
+    const float* tmax;
+    int64_t n_rays;
+    int32_t *prim_id, *object;
+    double *range, *alpha, *beta, *normal, *point;
+    uint8_t* flags;
+    uint8_t* occluded;
+    int opts;
+    int64_t next;
+    pthread_mutex_t lock;
+} ojob;
+
+static int64_t ograb(ojob* j, int64_t chunk, int64_t* end)
+{
+    pthread_mutex_lock(&j->lock);
+    const int64_t s = j->next;
+    j->next += chunk;
+    pthread_mutex_unlock(&j->lock);
+    *end = s + chunk < j->n_rays ? s + chunk : j->n_rays;
+    return s;
+}
+
+/* Object::IntersectRay's frame change (Object.cpp:414-425, :472-485):
+ * o' = M^-1 o, dl = M^-1 d (d as given), factor = |dl| (1 when not above
+ * NEARZERO), dn' = Normalize(dl). */
+static void object_ray(int x, const rise_oracle_object* ob, const double o[3], const double d[3], double op[3],
+                       double dn[3], double* factor)
+{
+    double dl[3];
+    xpoint(x, ob->to_object, o, op);
+    xvector(x, ob->to_object, d, dl);
+    const double mag = sqrt(d364(x, dl[0], dl[0], dl[1], dl[1], dl[2], dl[2]));
+    *factor = mag > NZ ? mag : 1.0;
+    normalize_x(x, dl, dn);
+}
+
+/* RayBoxIntersection (Intersection/RayBoxIntersection.cpp) on the ray
+ * (o, d) with Ray::RecomputeInvDir's inverse (Utilities/Ray.h:36-45: 1/d,
+ * or +-1e30 for |d| <= 1e-30).  Returns bHit; range / range2 as dRange /
+ * dRange2 (entry and exit, or exit and the negative entry when the origin
+ * is inside). */
+static int ray_box(const double o[3], const double d[3], const double lo[3], const double hi[3], double* range,
+                   double* range2)
+{
+    double inv[3];
+    int sg[3];
+    for (int k = 0; k < 3; ++k) {
+        inv[k] = (d[k] > 1e-30 || d[k] < -1e-30) ? 1.0 / d[k] : (d[k] >= 0 ? 1e30 : -1e30);
+        sg[k] = inv[k] < 0 ? 1 : 0;
+    }
+    const double bx[2] = {lo[0], hi[0]}, by[2] = {lo[1], hi[1]}, bz[2] = {lo[2], hi[2]};
+    double tmin = (bx[sg[0]] - o[0]) * inv[0];
+    double tmax = (bx[1 - sg[0]] - o[0]) * inv[0];
+    const double tymin = (by[sg[1]] - o[1]) * inv[1];
+    const double tymax = (by[1 - sg[1]] - o[1]) * inv[1];
+    if (tmin > tymax || tymin > tmax) return 0;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    const double tzmin = (bz[sg[2]] - o[2]) * inv[2];
+    const double tzmax = (bz[1 - sg[2]] - o[2]) * inv[2];
+    if (tmin > tzmax || tzmin > tmax) return 0;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    if (tmin < 0 && tmax < 0) return 0;
+    if (tmin > 0) {
+        *range = tmin;
+        *range2 = tmax;
+    } else {
+        *range = tmax;
+        *range2 = tmin;
+    }
+    return 1;
+}
+
+/* The mesh's GenerateBoundingBox: its BVH's overall box, the bounds of all
+ * its vertices (TriangleMeshGeometryIndexed.cpp:503-517, 576-583). */
+static void mesh_box(const tri64* tris, int64_t b0, int64_t b1, double lo[3], double hi[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = RISE_ORACLE_INFINITY;
+        hi[k] = -RISE_ORACLE_INFINITY;
+    }
+    for (int64_t q = b0; q < b1; ++q)
+        for (int k = 0; k < 3; ++k) {
+            const double v[3] = {tris[q].v0[k], tris[q].v0[k] + tris[q].eA[k], tris[q].v0[k] + tris[q].eB[k]};
+            for (int c = 0; c < 3; ++c) {
+                if (v[c] < lo[k]) lo[k] = v[c];
+                if (v[c] > hi[k]) hi[k] = v[c];
+            }
+        }
+}
+
+static void* objects_closest_worker(void* arg)
+{
+    ojob* j = (ojob*)arg;
+    const int x = (j->opts & RISE_ORACLE_X86_FMA) != 0;
+    const int nof = (j->opts & RISE_ORACLE_NO_FILTER) != 0;
+    for (;;) {
+        int64_t e;
+        const int64_t s = ograb(j, 16, &e);
+        if (s >= j->n_rays) break;
+        for (int64_t i = s; i < e; ++i) {
+            double o[3], d[3];
+            load_ray(j->rays, j->n_rays, i, o, d);
+            /* ObjectManager::IntersectRay's loop (ObjectManager.cpp:290-310):
+             * each object's own closest hit, kept when its world range is
+             * strictly below the best so far. */
+            double best_w = RISE_ORACLE_INFINITY;
+            int32_t best_obj = -1, best_prim = -1;
+            double best_pw[3] = {0, 0, 0}, best_a = 0, best_b = 0, best_n[3] = {0, 0, 0};
+            uint8_t flag = 0;
+            for (int k = 0; k < j->n_objs; ++k) {
+                const rise_oracle_object* ob = &j->objs[k];
+                double op[3], dn[3], factor;
+                object_ray(x, ob, o, d, op, dn, &factor);
+                float fo[3], fd[3];
+                float_ray(op, dn, fo, fd);
+                const int64_t b0 = j->mesh_off[ob->mesh], b1 = j->mesh_off[ob->mesh + 1];
+                if (j->opts & RISE_ORACLE_OBJ_PREHIT) {
+                    /* Object.cpp:472-527: the running best scaled by factor,
+                     * then the box pre-test on the object-space ray. */
+                    double how_far2 = best_w, lo[3], hi[3], r1, r2;
+                    if ((best_w != RISE_ORACLE_INFINITY) || (factor < 1.0)) how_far2 = factor * best_w;
+                    mesh_box(j->tris, b0, b1, lo, hi);
+                    if (!ray_box(op, dn, lo, hi, &r1, &r2)) continue;
+                    if (!(r2 < 0) && r1 > how_far2) continue;
+                }
+                double bt = RISE_ORACLE_INFINITY;
+                int64_t bk = -1;
+                for (int64_t q = b0; q < b1; ++q) {
+                    double tt;
+                    const tri64* t = &j->tris[q];
+                    if (!nof && !filter_x(x, fo, fd, t->p0, t->e1, t->e2, INFINITY)) continue;
+                    if (!face_selected(x, j->opts, t, dn)) continue;
+                    if (ray_triangle_x(x, op, dn, t->v0, t->eA, t->eB, &tt, NULL, NULL) && tt < bt) {
+                        bt = tt;
+                        bk = q;
+                    }
+                }
+                if (bk < 0) continue;
+                /* Object.cpp:637-643: back off, to world space, world range. */
+                const double s2 = bt - 1e-12;
+                double po[3], pw[3];
+                for (int c = 0; c < 3; ++c) po[c] = x ? fma(dn[c], s2, op[c]) : op[c] + dn[c] * s2;
+                xpoint(x, ob->to_world, po, pw);
+                const double vx = pw[0] - o[0], vy = pw[1] - o[1], vz = pw[2] - o[2];
+                const double w = sqrt(d364(x, vx, vx, vy, vy, vz, vz));
+                if (!(w < best_w)) continue;
+                const tri64* t = &j->tris[bk];
+                best_w = w;
+                best_obj = k;
+                best_prim = (int32_t)(bk - b0);
+                for (int c = 0; c < 3; ++c) best_pw[c] = pw[c];
+                flag = (!nof && !filter_x(x, fo, fd, t->p0, t->e1, t->e2, (float)bt)) ? 1 : 0;
+                if (j->alpha) {
+                    /* Barycentrics of the certifier; the face normal
+                     * normalized by the mesh (...Specializations.h:196-205),
+                     * then by M^-T and Normalize (Object.cpp:531-533, with
+                     * m_mxInvTranspose = Transpose(M^-1), :827). */
+                    double tt, a, b;
+                    ray_triangle_x(x, op, dn, t->v0, t->eA, t->eB, &tt, &a, &b);
+                    best_a = a;
+                    best_b = b;
+                    const double fn[3] = {sp64(x, t->eA[1], t->eB[2], t->eA[2], t->eB[1]),
+                                          sp64(x, t->eA[2], t->eB[0], t->eA[0], t->eB[2]),
+                                          sp64(x, t->eA[0], t->eB[1], t->eA[1], t->eB[0])};
+                    double n1[3], nw[3];
+                    normalize_x(x, fn, n1);
+                    const double* mi = ob->to_object;
+                    for (int c = 0; c < 3; ++c)
+                        nw[c] = d364(x, mi[4 * c], n1[0], mi[4 * c + 1], n1[1], mi[4 * c + 2], n1[2]);
+                    normalize_x(x, nw, best_n);
+                }
+            }
+            j->prim_id[i] = best_prim;
+            j->object[i] = best_obj;
+            j->range[i] = best_obj >= 0 ? best_w : INFINITY;
+            if (j->flags) j->flags[i] = flag;
+            if (j->alpha) {
+                j->alpha[i] = best_a;
+                j->beta[i] = best_b;
+                for (int c = 0; c < 3; ++c) {
+                    j->normal[3 * i + c] = best_n[c];
+                    j->point[3 * i + c] = best_pw[c];
+                }
+            }
+        }
+    }
+    return NULL;
+}
+
+static void* objects_any_worker(void* arg)
+{
+    ojob* j = (ojob*)arg;
+    const int x = (j->opts & RISE_ORACLE_X86_FMA) != 0;
+    const int nof = (j->opts & RISE_ORACLE_NO_FILTER) != 0;
+    for (;;) {
+        int64_t e;
+        const int64_t s = ograb(j, 16, &e);
+        if (s >= j->n_rays) break;
+        for (int64_t i = s; i < e; ++i) {
+            double o[3], d[3];
+            load_ray(j->rays, j->n_rays, i, o, d);
+            const double how_far = j->tmax ? (double)j->tmax[i] : RISE_ORACLE_INFINITY;
+            uint8_t occ = 0;
+            /* ObjectManager::IntersectShadowRay (ObjectManager.cpp:320-373):
+             * any object occluding within its own scaled limit. */
+            for (int k = 0; k < j->n_objs && !occ; ++k) {
+                const rise_oracle_object* ob = &j->objs[k];
+                double op[3], dn[3], factor;
+                object_ray(x, ob, o, d, op, dn, &factor);
+                /* Object.cpp:688-693 */
+                double how_far2 = how_far;
+                if ((how_far != RISE_ORACLE_INFINITY) || (factor < 1.0)) how_far2 = factor * how_far;
+                float fo[3], fd[3];
+                float_ray(op, dn, fo, fd);
+                const float fbest = (float)how_far2;
+                const int64_t b0 = j->mesh_off[ob->mesh], b1 = j->mesh_off[ob->mesh + 1];
+                if (j->opts & RISE_ORACLE_OBJ_PREHIT) {
+                    /* Object.cpp:695-707: box pre-test; beyond the limit only
+                     * an origin strictly inside the box goes on. */
+                    double lo[3], hi[3], r1, r2;
+                    mesh_box(j->tris, b0, b1, lo, hi);
+                    if (!ray_box(op, dn, lo, hi, &r1, &r2)) continue;
+                    if (r1 > how_far2 && !(op[0] > lo[0] && op[0] < hi[0] && op[1] > lo[1] && op[1] < hi[1] &&
+                                           op[2] > lo[2] && op[2] < hi[2]))
+                        continue;
+                }
+                for (int64_t q = b0; q < b1 && !occ; ++q) {
+                    double tt;
+                    const tri64* t = &j->tris[q];
+                    if (!nof && !filter_x(x, fo, fd, t->p0, t->e1, t->e2, fbest)) continue;
+                    if (!face_selected(x, j->opts, t, dn)) continue;
+                    if (ray_triangle_x(x, op, dn, t->v0, t->eA, t->eB, &tt, NULL, NULL) && tt > NZ &&
+                        tt < how_far2)
+                        occ = 1;
+                }
+            }
+            j->occluded[i] = occ;
+        }
+    }
+    return NULL;
+}
+
+static void orun(ojob* j, void* (*fn)(void*), int n_threads)
+{
+    if (n_threads <= 0) {
+        long c = sysconf(_SC_NPROCESSORS_ONLN);
+        n_threads = c > 0 ? (int)c : 1;
+    }
+    if (n_threads > 256) n_threads = 256;
+    pthread_t th[256];
+    pthread_mutex_init(&j->lock, NULL);
+    j->next = 0;
+    for (int i = 0; i < n_threads; ++i) pthread_create(&th[i], NULL, fn, j);
+    for (int i = 0; i < n_threads; ++i) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&j->lock);
+}
+
+void rise_oracle_objects_closest(const float* tris, const int64_t* mesh_off, int n_meshes,
+                                 const rise_oracle_object* objs, int n_objs, const float* rays, int64_t n_rays,
+                                 int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
+                                 double* normal, double* point, uint8_t* flags, int opts, int n_threads)
+{
+    ojob j;
+    memset(&j, 0, sizeof(j));
+    tri64* tt = expand_tris(tris, mesh_off[n_meshes]);
+    j.tris = tt;
+    j.mesh_off = mesh_off;
+    j.objs = objs;
+    j.n_objs = n_objs;
+    j.rays = rays;
+    j.n_rays = n_rays;
+    j.prim_id = prim_id;
+    j.object = object;
+    j.range = range;
+    j.alpha = alpha;
+    j.beta = beta;
+    j.normal = normal;
+    j.point = point;
+    j.flags = flags;
+    j.opts = opts;
+    orun(&j, objects_closest_worker, n_threads);
+    free(tt);
+}
+
+void rise_oracle_objects_any(const float* tris, const int64_t* mesh_off, int n_meshes, const rise_oracle_object* objs,
+                             int n_objs, const float* rays, const float* tmax, int64_t n_rays, uint8_t* occluded,
+                             int opts, int n_threads)
+{
+    ojob j;
+    memset(&j, 0, sizeof(j));
+    tri64* tt = expand_tris(tris, mesh_off[n_meshes]);
+    j.tris = tt;
+    j.mesh_off = mesh_off;
+    j.objs = objs;
+    j.n_objs = n_objs;
+    j.rays = rays;
+    j.tmax = tmax;
+    j.n_rays = n_rays;
+    j.occluded = occluded;
+    j.opts = opts;
+    orun(&j, objects_any_worker, n_threads);
+    free(tt);
+}

@@ -57,6 +57,8 @@ extern "C" {
                                    a*b + c*d + e*f -> fma(e, f, fma(a, b, c*d)) */
 #define RISE_ORACLE_NO_FRONT 4  /* skip front faces (bHitFrontFaces false) */
 #define RISE_ORACLE_NO_BACK 8   /* skip back faces (bHitBackFaces false) */
+#define RISE_ORACLE_OBJ_PREHIT 16 /* object queries: model Object::IntersectRay's box pre-test
+                                     (Object.cpp:513-527, :695-707) on the mesh's vertex bounds */
 
 /* RayTriangleIntersection (RayTriangleIntersection.cpp:21-68).  Returns 1 on
  * hit and fills t/alpha/beta.  edgeA = v1 - v0, edgeB = v2 - v0. */
@@ -102,6 +104,43 @@ void rise_oracle_any_brute(const float* tris, int64_t n_tris,
 void rise_oracle_closest_attr(const float* tris, int64_t n_tris, const float* rays, int64_t n_rays,
                               int32_t* prim_id, double* range, double* alpha, double* beta, double* normal,
                               double* point, int opts, int n_threads);
+
+/* One object: a mesh placed by Matrix4 transforms (members _00.._33 in order;
+ * row-vector convention, translation in _30.._32).  Same layout as the
+ * library's rise_hip_object_instance. */
+typedef struct rise_oracle_object {
+    int32_t mesh;
+    int32_t reserved;
+    double to_world[16];  /* m_mxFinalTrans */
+    double to_object[16]; /* m_mxInvFinalTrans */
+} rise_oracle_object;
+
+/* ObjectManager::IntersectRay's linear loop over objects
+ * (Managers/ObjectManager.cpp:290-310) with Object::IntersectRay per object
+ * (Objects/Object.cpp:414-425 frame change, :531-533 normal, :637-643 world
+ * point and range), brute force over each object's mesh.
+ * tris: all meshes' triangles (9 floats each), mesh m being
+ * [mesh_off[m], mesh_off[m+1]).  Outputs per ray: prim_id (index within its
+ * mesh, -1 on a miss), object (index, -1), range (world distance, +inf);
+ * optional (NULL or all given): alpha, beta, normal (3), point (3, world);
+ * optional flags: 1 where the winner fails the filter against its own t
+ * (traversal-order dependent).  The box pre-test of Object.cpp:513-527 is
+ * modelled only with RISE_ORACLE_OBJ_PREHIT in opts.  For a unit world
+ * direction it drops only objects whose hits lie beyond the running best
+ * (hits within 1e-12 of their box's entry distance aside); for |d| < 1 the
+ * running best is scaled by |M^-1 d| = |d| |M^-1 d/|d||, short of its
+ * object-space image, and it can drop a closer object. */
+void rise_oracle_objects_closest(const float* tris, const int64_t* mesh_off, int n_meshes,
+                                 const rise_oracle_object* objs, int n_objs, const float* rays, int64_t n_rays,
+                                 int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
+                                 double* normal, double* point, uint8_t* flags, int opts, int n_threads);
+
+/* ObjectManager::IntersectShadowRay's loop (ObjectManager.cpp:320-373) with
+ * Object::IntersectRay_IntersectionOnly per object (Object.cpp:652-712):
+ * dHowFar scaled by |M^-1 d| per object.  tmax: NULL = RISE_INFINITY. */
+void rise_oracle_objects_any(const float* tris, const int64_t* mesh_off, int n_meshes, const rise_oracle_object* objs,
+                             int n_objs, const float* rays, const float* tmax, int64_t n_rays, uint8_t* occluded,
+                             int opts, int n_threads);
 
 #ifdef __cplusplus
 }

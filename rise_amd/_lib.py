@@ -84,6 +84,9 @@ _SIGNATURES = {
         ctypes.c_int,
     ),
     "rise_hip_debug_node_accept_host": ([_P, _P, ctypes.c_double, _P], ctypes.c_int),
+    "rise_hip_scene_upload_objects": ([_P, _SIZE, _P, _SIZE, ctypes.POINTER(_P)], ctypes.c_int),
+    "rise_hip_debug_objects_host": ([_P, _SIZE, _P, _SIZE, _P, _P, _I64, _I32, _I32, _P, _P], ctypes.c_int),
+    "rise_hip_debug_node_accept_obj_host": ([_P, _P, _P, ctypes.c_double, _P], ctypes.c_int),
 }
 
 _lib: ctypes.CDLL | None = None

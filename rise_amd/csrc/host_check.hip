@@ -4,6 +4,8 @@
 
 #include "api_common.h"
 #include "bvh_build.h"
+#include "object_scene.h"
+#include "object_trace.h"
 #include "risemesh_internal.h"
 #include "trace_core.h"
 
@@ -11,6 +13,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 using namespace rise_hip;
@@ -142,6 +145,82 @@ extern "C" rise_hip_status rise_hip_debug_trace_ex_host(const float* tris, int64
     worker();
     for (auto& x : th) x.join();
     if (overflow) return fail(RISE_HIP_ERR_BUILD, "traversal stack overflow");
+    return RISE_HIP_OK;
+}
+
+extern "C" rise_hip_status rise_hip_debug_objects_host(const rise_hip_mesh_desc* meshes, size_t n_meshes,
+                                                       const rise_hip_object_instance* objects, size_t n_objects,
+                                                       const float* rays, const float* tmax, int64_t n, int32_t mode,
+                                                       int32_t faces, rise_hip_hit* hits, uint8_t* occluded)
+{
+    clear_error();
+    if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative count");
+    if (n > 0 && !rays) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (mode < 0 || mode > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mode must be 0..3");
+    if (faces < 0 || faces > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "faces must be 0..3");
+    if (n > 0 && (mode & 1) == 0 && !hits) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    if (n > 0 && (mode & 1) == 1 && !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL output");
+    ObjectSceneHost host;
+    const rise_hip_status st = build_object_scene(meshes, n_meshes, objects, n_objects, host);
+    if (st != RISE_HIP_OK) return st;
+    std::vector<BlasView> views;
+    for (const WideBVH& b : host.blas) views.push_back(BlasView{b.nodes.data(), b.tris.data()});
+    const ObjectSceneView sc{host.world_tris > 0 ? host.world.nodes.data() : nullptr, host.world.tris.data(),
+                             host.objects.data(), (int32_t)host.objects.size(), views.data()};
+    std::atomic<int64_t> next(0);
+    std::atomic<bool> overflow(false);
+    auto one = [&](auto arith, int64_t i) {
+        constexpr int A = decltype(arith)::value;
+        float ray[6];
+        for (int k = 0; k < 6; ++k) ray[k] = rays[(int64_t)k * n + i];
+        HostStack hs;
+        if ((mode & 1) == 0) {
+            ObjectHit h;
+            objects_closest<A>(sc, ray, (unsigned)faces, hs, h);
+            rise_hip_hit rec{};
+            objects_record<A>(sc, ray, h, rec);
+            hits[i] = rec;
+        } else {
+            const double hf = tmax ? (double)tmax[i] : kRiseInfinity;
+            occluded[i] = objects_any<A>(sc, ray, hf, (unsigned)faces, hs) ? 1 : 0;
+        }
+    };
+    auto worker = [&]() {
+        for (;;) {
+            const int64_t s0 = next.fetch_add(64);
+            if (s0 >= n) break;
+            for (int64_t i = s0; i < std::min<int64_t>(s0 + 64, n); ++i) {
+                try {
+                    if (mode & 2) one(std::integral_constant<int, kArithX86Fma>{}, i);
+                    else one(std::integral_constant<int, kArithIeee>{}, i);
+                } catch (const std::exception&) {
+                    overflow = true;
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < host_threads(); ++k) th.emplace_back(worker);
+    worker();
+    for (auto& x : th) x.join();
+    if (overflow) return fail(RISE_HIP_ERR_BUILD, "traversal stack overflow");
+    return RISE_HIP_OK;
+}
+
+// The kernels' double slab test of one 8-wide node against an object-space
+// ray made from a world ray and an object's inverse matrix.
+extern "C" rise_hip_status rise_hip_debug_node_accept_obj_host(const void* node, const float ray[6],
+                                                               const double to_object[16], double best,
+                                                               uint32_t* accept)
+{
+    clear_error();
+    if (!node || !ray || !to_object || !accept) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    uint4 w[5];
+    std::memcpy(w, node, sizeof(w));
+    const double ow[3] = {ray[0], ray[1], ray[2]}, dw[3] = {ray[3], ray[4], ray[5]};
+    ObjRayT<kArithIeee> q;
+    init_obj_ray(q, ow, dw, to_object);
+    *accept = node_accept_mask(q, w[0], w[1], w[2], w[3], w[4], box_limit(q, best));
     return RISE_HIP_OK;
 }
 

@@ -1,0 +1,165 @@
+// object_scene.cpp -- builds the trees of a scene of objects with transforms.
+#include "object_scene.h"
+
+#include "api_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace rise_hip {
+
+namespace {
+
+bool is_identity(const double* m)
+{
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            if (m[4 * i + j] != (i == j ? 1.0 : 0.0)) return false;
+    return true;
+}
+
+bool all_finite(const double* m)
+{
+    for (int k = 0; k < 16; ++k)
+        if (!std::isfinite(m[k])) return false;
+    return true;
+}
+
+// Point3Ops::Transform in double (row-vector convention, translation in row 3).
+void xform(const double* m, const double p[3], double out[3])
+{
+    for (int j = 0; j < 3; ++j) out[j] = m[j] * p[0] + m[4 + j] * p[1] + m[8 + j] * p[2] + m[12 + j];
+}
+
+// World box of an object: the mesh's vertex bounds in object space, widened by
+// 1e-7 of their size and coordinates plus 1e-10 (the hit point the reference
+// transforms back lies within the certifier's 1e-12 tolerance and the 1e-12
+// back-off of the triangle), its eight corners taken to world space, and the
+// result widened again by 1e-7.  It contains every world point the reference
+// can report for the object, and every point of the exact world ray that
+// passes within rounding of one.
+void world_box(const float* verts, int64_t n_tris, const double* to_world, double lo[3], double hi[3])
+{
+    double olo[3] = {INFINITY, INFINITY, INFINITY}, ohi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n_tris * 3; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const double v = verts[3 * i + a];
+            olo[a] = std::min(olo[a], v);
+            ohi[a] = std::max(ohi[a], v);
+        }
+    double span = 0.0;
+    for (int a = 0; a < 3; ++a) span = std::max({span, ohi[a] - olo[a], std::fabs(olo[a]), std::fabs(ohi[a])});
+    const double pad_o = span * 1e-7 + 1e-10;
+    for (int a = 0; a < 3; ++a) {
+        olo[a] -= pad_o;
+        ohi[a] += pad_o;
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+    }
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(c & 1) ? ohi[0] : olo[0], (c & 2) ? ohi[1] : olo[1], (c & 4) ? ohi[2] : olo[2]};
+        double w[3];
+        xform(to_world, p, w);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], w[a]);
+            hi[a] = std::max(hi[a], w[a]);
+        }
+    }
+    double wspan = 0.0;
+    for (int a = 0; a < 3; ++a) wspan = std::max({wspan, hi[a] - lo[a], std::fabs(lo[a]), std::fabs(hi[a])});
+    const double pad_w = wspan * 1e-7 + 1e-10;
+    for (int a = 0; a < 3; ++a) {
+        lo[a] -= pad_w;
+        hi[a] += pad_w;
+    }
+}
+
+}  // namespace
+
+rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_meshes,
+                                   const rise_hip_object_instance* objects, size_t n_objects, ObjectSceneHost& out)
+{
+    if (n_meshes > 0 && !meshes) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "meshes is NULL");
+    if (n_objects > 0 && !objects) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "objects is NULL");
+    if (n_objects > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "more than 2^31-1 objects");
+    for (size_t m = 0; m < n_meshes; ++m) {
+        if (meshes[m].n_tris < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative triangle count");
+        if (meshes[m].n_tris > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh over 2^31-1 triangles");
+        if (meshes[m].n_tris > 0 && !meshes[m].vertices)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh vertices is NULL");
+    }
+    int64_t world_total = 0;
+    for (size_t k = 0; k < n_objects; ++k) {
+        const rise_hip_object_instance& ob = objects[k];
+        if (ob.mesh < 0 || (size_t)ob.mesh >= n_meshes)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "object " + std::to_string(k) + ": mesh index out of range");
+        if (ob.reserved != 0)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "object " + std::to_string(k) + ": reserved must be 0");
+        if (!all_finite(ob.to_world) || !all_finite(ob.to_object))
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "object " + std::to_string(k) + ": non-finite matrix entry");
+        if (is_identity(ob.to_world) && is_identity(ob.to_object)) world_total += meshes[ob.mesh].n_tris;
+    }
+    if (world_total > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "more than 2^31-1 world triangles");
+
+    std::string err;
+    try {
+        // Identity objects: one world-space tree built over the running index,
+        // then each leaf triangle takes its caller id (v0.w) and its object (v1.w).
+        std::vector<float> verts((size_t)world_total * 9);
+        std::vector<int32_t> ids((size_t)world_total), obj_of((size_t)world_total);
+        int64_t off = 0;
+        for (size_t k = 0; k < n_objects; ++k) {
+            const rise_hip_object_instance& ob = objects[k];
+            if (!(is_identity(ob.to_world) && is_identity(ob.to_object))) continue;
+            const rise_hip_mesh_desc& m = meshes[ob.mesh];
+            if (m.n_tris == 0) continue;
+            std::memcpy(verts.data() + off * 9, m.vertices, (size_t)m.n_tris * 9 * sizeof(float));
+            for (int64_t i = 0; i < m.n_tris; ++i) {
+                ids[(size_t)(off + i)] = m.tri_ids ? m.tri_ids[i] : (int32_t)i;
+                obj_of[(size_t)(off + i)] = (int32_t)k;
+            }
+            off += m.n_tris;
+        }
+        out.world_tris = world_total;
+        if (!build_wide_bvh(verts.data(), world_total, nullptr, BuildParams(), out.world, err))
+            return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+        for (Tri48& tr : out.world.tris) {
+            int32_t p;
+            std::memcpy(&p, &tr.v0[3], 4);
+            std::memcpy(&tr.v0[3], &ids[(size_t)p], 4);
+            std::memcpy(&tr.v1[3], &obj_of[(size_t)p], 4);
+        }
+        // Transformed objects: one object-space tree per mesh they use.
+        std::vector<int32_t> blas_of(n_meshes, -1);
+        out.total_tris = world_total;
+        for (size_t k = 0; k < n_objects; ++k) {
+            const rise_hip_object_instance& ob = objects[k];
+            if (is_identity(ob.to_world) && is_identity(ob.to_object)) continue;
+            const rise_hip_mesh_desc& m = meshes[ob.mesh];
+            if (m.n_tris == 0) continue;  // nothing to hit
+            if (blas_of[ob.mesh] < 0) {
+                WideBVH b;
+                if (!build_wide_bvh(m.vertices, m.n_tris, m.tri_ids, BuildParams(), b, err))
+                    return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+                for (Tri48& tr : b.tris) std::memcpy(&tr.v1[3], &ob.mesh, 4);
+                blas_of[ob.mesh] = (int32_t)out.blas.size();
+                out.total_tris += m.n_tris;
+                out.blas.push_back(std::move(b));
+            }
+            DevObject d{};
+            std::memcpy(d.to_world, ob.to_world, sizeof(d.to_world));
+            std::memcpy(d.to_object, ob.to_object, sizeof(d.to_object));
+            world_box(m.vertices, m.n_tris, ob.to_world, d.lo, d.hi);
+            d.blas = blas_of[ob.mesh];
+            d.object = (int32_t)k;
+            out.objects.push_back(d);
+        }
+    } catch (const std::bad_alloc&) {
+        return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+    }
+    return RISE_HIP_OK;
+}
+
+}  // namespace rise_hip

@@ -4,6 +4,8 @@
 
 #include "api_common.h"
 #include "bvh_build.h"
+#include "object_kernels.h"
+#include "object_scene.h"
 #include "trace_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -33,6 +35,16 @@ struct rise_hip_scene {
     std::atomic<int> arith{RISE_HIP_ARITH_IEEE};
     int device = 0;
     rise_hip_scene_stats stats;
+    // Transformed objects (rise_hip_scene_upload_objects); none otherwise.
+    DevObject* d_objects = nullptr;
+    BlasView* d_blas = nullptr;
+    std::vector<void*> blas_allocs;
+    int32_t n_objects = 0;
+    bool world_empty = false;
+    ObjectSceneView objects_view() const
+    {
+        return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_objects, n_objects, d_blas};
+    }
     TraceScene view()
     {
         const unsigned k = next_slot.fetch_add(1u) % kFetchSlots;
@@ -185,10 +197,61 @@ rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_sour
 
 extern "C" {
 
+rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, size_t n_meshes,
+                                              const rise_hip_object_instance* objects, size_t n_objects,
+                                              rise_hip_scene** out_scene)
+{
+    clear_error();
+    if (!out_scene) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "out_scene is NULL");
+    *out_scene = nullptr;
+    ObjectSceneHost host;
+    rise_hip_status st = build_object_scene(meshes, n_meshes, objects, n_objects, host);
+    if (st != RISE_HIP_OK) return st;
+    rise_hip_scene* s = nullptr;
+    st = upload_scene(host.world, host.world_tris, 0, &s);
+    if (st != RISE_HIP_OK) return st;
+    s->world_empty = host.world_tris == 0;
+    s->stats.n_tris = host.total_tris;
+    hipError_t e = hipSuccess;
+    std::vector<BlasView> views;
+    for (const WideBVH& b : host.blas) {
+        if (e != hipSuccess) break;
+        const size_t nb = b.nodes.size() * sizeof(NodeSlot), tb = std::max<size_t>(b.tris.size(), 1) * sizeof(Tri48);
+        void *dn = nullptr, *dt = nullptr;
+        e = hipMalloc(&dn, nb);
+        if (e == hipSuccess) s->blas_allocs.push_back(dn);
+        if (e == hipSuccess) e = hipMalloc(&dt, tb);
+        if (e == hipSuccess) s->blas_allocs.push_back(dt);
+        if (e == hipSuccess) e = hipMemcpy(dn, b.nodes.data(), nb, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !b.tris.empty())
+            e = hipMemcpy(dt, b.tris.data(), b.tris.size() * sizeof(Tri48), hipMemcpyHostToDevice);
+        views.push_back(BlasView{(const NodeSlot*)dn, (const Tri48*)dt});
+        s->stats.device_bytes += (int64_t)(nb + tb);
+    }
+    if (e == hipSuccess && !host.objects.empty()) {
+        const size_t ob = host.objects.size() * sizeof(DevObject), vb = views.size() * sizeof(BlasView);
+        e = hipMalloc((void**)&s->d_objects, ob);
+        if (e == hipSuccess) e = hipMalloc((void**)&s->d_blas, vb);
+        if (e == hipSuccess) e = hipMemcpy(s->d_objects, host.objects.data(), ob, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(s->d_blas, views.data(), vb, hipMemcpyHostToDevice);
+        s->stats.device_bytes += (int64_t)(ob + vb);
+    }
+    if (e != hipSuccess) {
+        (void)rise_hip_scene_release(s);
+        return fail_hip(e, "object scene upload");
+    }
+    s->n_objects = (int32_t)host.objects.size();
+    *out_scene = s;
+    return RISE_HIP_OK;
+}
+
 rise_hip_status rise_hip_scene_release(rise_hip_scene* scene)
 {
     clear_error();
     if (!scene) return RISE_HIP_OK;
+    for (void* p : scene->blas_allocs) (void)hipFree(p);
+    (void)hipFree(scene->d_objects);
+    (void)hipFree(scene->d_blas);
     (void)hipFree(scene->d_nodes);
     (void)hipFree(scene->d_tris);
     (void)hipFree(scene->d_fetch);
@@ -222,6 +285,13 @@ static rise_hip_status closest_impl(rise_hip_scene* scene, const float* rays, in
     if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
     if (n == 0) return RISE_HIP_OK;
     if (!rays || !prim_id || !t) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (scene->n_objects > 0) {
+        if (counters)
+            return fail(RISE_HIP_ERR_NOT_SUPPORTED, "counted queries on a scene with transformed objects");
+        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), rays, n, 3u, prim_id, t,
+                                                    nullptr, stream);
+        return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "closest-hit launch");
+    }
     const hipError_t e = launch_closest(scene->view(), rays, n, prim_id, t,
                                         reinterpret_cast<unsigned long long*>(counters), stream);
     if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
@@ -236,6 +306,13 @@ static rise_hip_status any_impl(rise_hip_scene* scene, const float* rays, const 
     if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
     if (n == 0) return RISE_HIP_OK;
     if (!rays || !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (scene->n_objects > 0) {
+        if (counters)
+            return fail(RISE_HIP_ERR_NOT_SUPPORTED, "counted queries on a scene with transformed objects");
+        const hipError_t e = launch_any_objects(scene->objects_view(), scene->arith.load(), rays, tmax, n, 3u, occluded,
+                                                stream);
+        return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "any-hit launch");
+    }
     const hipError_t e = launch_any(scene->view(), rays, tmax, n, occluded,
                                     reinterpret_cast<unsigned long long*>(counters), stream);
     if (e != hipSuccess) return fail_hip(e, "any-hit launch");
@@ -263,7 +340,10 @@ rise_hip_status rise_hip_intersect_closest_ex(rise_hip_scene* scene, const float
     if (faces < 0 || faces > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "faces must be a combination of RISE_HIP_FACES_*");
     if (n == 0) return RISE_HIP_OK;
     if (!rays_soa || !hits) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
-    const hipError_t e = launch_closest_ex(scene->view(), rays_soa, n, (unsigned)faces, hits, stream);
+    const hipError_t e = scene->n_objects > 0
+                             ? launch_closest_objects(scene->objects_view(), scene->arith.load(), rays_soa, n,
+                                                      (unsigned)faces, nullptr, nullptr, hits, stream)
+                             : launch_closest_ex(scene->view(), rays_soa, n, (unsigned)faces, hits, stream);
     if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
     return RISE_HIP_OK;
 }
@@ -277,7 +357,10 @@ rise_hip_status rise_hip_intersect_any_ex(rise_hip_scene* scene, const float* ra
     if (faces < 0 || faces > 3) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "faces must be a combination of RISE_HIP_FACES_*");
     if (n == 0) return RISE_HIP_OK;
     if (!rays_soa || !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
-    const hipError_t e = launch_any_ex(scene->view(), rays_soa, tmax, n, (unsigned)faces, occluded, stream);
+    const hipError_t e = scene->n_objects > 0
+                             ? launch_any_objects(scene->objects_view(), scene->arith.load(), rays_soa, tmax, n,
+                                                  (unsigned)faces, occluded, stream)
+                             : launch_any_ex(scene->view(), rays_soa, tmax, n, (unsigned)faces, occluded, stream);
     if (e != hipSuccess) return fail_hip(e, "any-hit launch");
     return RISE_HIP_OK;
 }
@@ -290,6 +373,12 @@ rise_hip_status rise_hip_intersect_closest_noalloc(rise_hip_scene* scene, const 
     if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
     if (n == 0) return RISE_HIP_OK;
     if (!rays_soa || !prim_id || !t) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (scene->n_objects > 0) {
+        // The object kernels keep no scratch: nothing to redirect.
+        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), rays_soa, n, 3u,
+                                                    prim_id, t, nullptr, stream);
+        return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "closest-hit launch");
+    }
     unsigned long long* counter = nullptr;
     hipError_t e = noalloc_scratch(&counter);
     if (e != hipSuccess) return fail_hip(e, "scratch address (hipGetSymbolAddress)");
@@ -373,6 +462,8 @@ extern "C" rise_hip_status rise_hip_scene_cache_write(const rise_hip_scene* scen
 {
     clear_error();
     if (!scene || !path) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (scene->n_objects > 0)
+        return fail(RISE_HIP_ERR_NOT_SUPPORTED, "scene cache of a scene with transformed objects");
     const uint64_t nn = (uint64_t)scene->stats.n_nodes, nt = (uint64_t)scene->stats.n_tris;
     std::vector<NodeSlot> nodes;
     std::vector<Tri48> tris;

@@ -107,6 +107,7 @@ RISE_HD inline double dot3(double a0, double b0, double a1, double b1, double a2
 // certified, bit-identically.  A selects the arithmetic (kArith*).
 template <int A>
 struct RayStateT {
+    static constexpr int kArith = A;
     float ox, oy, oz;      // float origin (exact input)
     float dx, dy, dz;      // direction as given
     float fx, fy, fz;      // (float)dn: the filter's direction (BVH.h:1212-1225)
@@ -124,6 +125,13 @@ template <int A>
 RISE_HD inline double dn1(const RayStateT<A>& r) { return (double)r.dy * r.ov; }
 template <int A>
 RISE_HD inline double dn2(const RayStateT<A>& r) { return (double)r.dz * r.ov; }
+// The certifier's origin: the float input, exact in double.
+template <int A>
+RISE_HD inline double org0(const RayStateT<A>& r) { return (double)r.ox; }
+template <int A>
+RISE_HD inline double org1(const RayStateT<A>& r) { return (double)r.oy; }
+template <int A>
+RISE_HD inline double org2(const RayStateT<A>& r) { return (double)r.oz; }
 
 // Inverse of a scaled direction component (largest component in [1, 2)):
 // correctly rounded 1/d, or +-kInvClamp when |d| < 2^-80 (+ for a zero).
@@ -175,6 +183,88 @@ RISE_HD inline void init_ray(RayStateT<A>& r, float ox, float oy, float oz, floa
     r.fz = (float)dn2(r);
 }
 
+// Point3Ops::Transform and Vector3Ops::Transform (Utilities/Math3D/
+// PointsOps.h:126-133, VectorsOps.h:241-248) with the row-vector convention of
+// Matrix4: m[4 i + j] = m._ij, the translation in row 3.
+template <int A>
+RISE_HD inline void xform_point(const double* m, const double p[3], double out[3])
+{
+    for (int j = 0; j < 3; ++j) out[j] = dot3<A>(m[j], p[0], m[4 + j], p[1], m[8 + j], p[2]) + m[12 + j];
+}
+template <int A>
+RISE_HD inline void xform_vector(const double* m, const double v[3], double out[3])
+{
+    for (int j = 0; j < 3; ++j) out[j] = dot3<A>(m[j], v[0], m[4 + j], v[1], m[8 + j], v[2]);
+}
+// Vector3Ops::Transform by the inverse transpose (Object.cpp:531-533, with
+// m_mxInvTranspose = Transpose(m_mxInvFinalTrans), Object.cpp:827), from the
+// inverse itself: out_j = minv._j0 n.x + minv._j1 n.y + minv._j2 n.z.
+template <int A>
+RISE_HD inline void xform_normal(const double* minv, const double n[3], double out[3])
+{
+    for (int j = 0; j < 3; ++j) out[j] = dot3<A>(minv[4 * j], n[0], minv[4 * j + 1], n[1], minv[4 * j + 2], n[2]);
+}
+
+// Object-space ray of a transformed object, as Object::IntersectRay and
+// _IntersectionOnly make it (Objects/Object.cpp:414-425, :666-674):
+// o' = Transform(M^-1, o), dl = Transform(M^-1, d) for the world direction d
+// as given, dn' = Normalize(dl), all in double, and factor = |dl| (1 when it
+// is not above NEARZERO, :472-485).  The float pre-filter sees (float)o' and
+// (float)dn' (PrepRayFloat, BVH.h:1212-1225); the certifier sees o' and dn'.
+// o' is not a float, so this ray's boxes are tested in double
+// (node_accept_mask below).
+template <int A>
+struct ObjRayT {
+    static constexpr int kArith = A;
+    float ox, oy, oz;  // (float)o'
+    float fx, fy, fz;  // (float)dn'
+    unsigned oinv;     // (~octant of dn') & 7
+    double o[3];       // o'
+    double dn[3];      // dn'
+    double inv[3];     // 1/dn' (+-2^200 for components below 2^-200, + for a zero)
+    double mag;        // |dl|
+    double factor;     // |dl|, or 1 when |dl| <= NEARZERO
+};
+
+template <int A>
+RISE_HD inline double dn0(const ObjRayT<A>& r) { return r.dn[0]; }
+template <int A>
+RISE_HD inline double dn1(const ObjRayT<A>& r) { return r.dn[1]; }
+template <int A>
+RISE_HD inline double dn2(const ObjRayT<A>& r) { return r.dn[2]; }
+template <int A>
+RISE_HD inline double org0(const ObjRayT<A>& r) { return r.o[0]; }
+template <int A>
+RISE_HD inline double org1(const ObjRayT<A>& r) { return r.o[1]; }
+template <int A>
+RISE_HD inline double org2(const ObjRayT<A>& r) { return r.o[2]; }
+
+// minv: the object's m_mxInvFinalTrans (16 doubles, Matrix4 order).
+template <int A>
+RISE_HD inline void init_obj_ray(ObjRayT<A>& r, const double ow[3], const double dw[3], const double* minv)
+{
+    xform_point<A>(minv, ow, r.o);
+    double dl[3];
+    xform_vector<A>(minv, dw, dl);
+    const double mag = sqrt(dot3<A>(dl[0], dl[0], dl[1], dl[1], dl[2], dl[2]));
+    r.mag = mag;
+    r.factor = mag > kNearZero ? mag : 1.0;
+    normalize_model<A>(dl, r.dn);
+    r.ox = (float)r.o[0];
+    r.oy = (float)r.o[1];
+    r.oz = (float)r.o[2];
+    r.fx = (float)r.dn[0];
+    r.fy = (float)r.dn[1];
+    r.fz = (float)r.dn[2];
+    unsigned oct = 0;
+    for (int k = 0; k < 3; ++k) {
+        const double v = r.dn[k];
+        if (v < 0.0) oct |= 1u << k;
+        r.inv[k] = fabs(v) < 0x1p-200 ? (v < 0.0 ? -0x1p200 : 0x1p200) : 1.0 / v;
+    }
+    r.oinv = (~oct) & 7u;
+}
+
 // BVH::MollerTrumboreFloat (Acceleration/BVH.h:1111-1167) on the leaf
 // triangle's float vertices: p0 = v0, e1 = v1 - v0, e2 = v2 - v0 in float
 // (BuildFastFilter, BVH.h:431-467), the float origin and the float
@@ -182,9 +272,10 @@ RISE_HD inline void init_ray(RayStateT<A>& r, float ox, float oy, float oz, floa
 // caller's (float) current range.  Every operation is one IEEE float
 // operation in the reference's order (no contraction; 1/det correctly
 // rounded).  A triangle it rejects is never certified.
-template <int A>
-RISE_HD inline bool filter_float(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, float best)
+template <class R>
+RISE_HD inline bool filter_float(const R& r, const float4& a, const float4& b, const float4& c, float best)
 {
+    constexpr int A = R::kArith;
     constexpr float kEps = 1e-5f;
     const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
     const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
@@ -209,10 +300,11 @@ RISE_HD inline bool filter_float(const RayStateT<A>& r, const float4& a, const f
 
 // FP64 Moller-Trumbore, RayTriangleIntersection.cpp:21-68 with the edges of
 // ...Specializations.h:141-142 (computed in double from the vertices).
-template <int A>
-RISE_HD inline bool certify(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, double& t_out,
+template <class R>
+RISE_HD inline bool certify(const R& r, const float4& a, const float4& b, const float4& c, double& t_out,
                             double* ab = nullptr)
 {
+    constexpr int A = R::kArith;
     const double v0x = a.x, v0y = a.y, v0z = a.z;
     const double eAx = (double)b.x - v0x, eAy = (double)b.y - v0y, eAz = (double)b.z - v0z;
     const double eBx = (double)c.x - v0x, eBy = (double)c.y - v0y, eBz = (double)c.z - v0z;
@@ -223,7 +315,7 @@ RISE_HD inline bool certify(const RayStateT<A>& r, const float4& a, const float4
     const double det = dot3<A>(eAx, p0, eAy, p1, eAz, p2);
     if (det > -kNearZero && det < kNearZero) return false;
     const double oodet = 1.0 / det;
-    const double t0 = (double)r.ox - v0x, t1 = (double)r.oy - v0y, t2 = (double)r.oz - v0z;
+    const double t0 = org0(r) - v0x, t1 = org1(r) - v0y, t2 = org2(r) - v0z;
     const double al = dot3<A>(t0, p0, t1, p1, t2, p2) * oodet;
     if ((al < 0.0 - kNearZero) || (al > 1.0 + kNearZero)) return false;
     const double q0 = sub_prod<A>(t1, eAz, t2, eAy);
@@ -247,9 +339,10 @@ RISE_HD inline bool certify(const RayStateT<A>& r, const float4& a, const float4
 // (...Specializations.h:141-159): the face normal (v1 - v0) x (v2 - v0) in
 // double against the normalized direction; dot < 0 is a front face.
 // faces: bit 0 hit front faces, bit 1 hit back faces.
-template <int A>
-RISE_HD inline bool face_ok(const RayStateT<A>& r, const float4& a, const float4& b, const float4& c, unsigned faces)
+template <class R>
+RISE_HD inline bool face_ok(const R& r, const float4& a, const float4& b, const float4& c, unsigned faces)
 {
+    constexpr int A = R::kArith;
     const double eAx = (double)b.x - a.x, eAy = (double)b.y - a.y, eAz = (double)b.z - a.z;
     const double eBx = (double)c.x - a.x, eBy = (double)c.y - a.y, eBz = (double)c.z - a.z;
     const double nx = sub_prod<A>(eAy, eBz, eAz, eBy);
@@ -350,10 +443,57 @@ RISE_HD inline unsigned node_accept_mask(const RayStateT<A>& r, const uint4& w0,
     return acc;
 }
 
+// The eight child boxes of one node against an object-space ray, in double.
+// The plane at quantized coordinate q lies at p + q 2^e; computed in double it
+// is within 2^-53 of its size, so each plane is first moved outward by
+// 2^-50 (|p| + 256 2^e), which leaves the box containing the exact one.  Its
+// distance (plane - o') / dn' then carries only the relative error of the
+// subtraction and the product (2^-52), covered by widening the exit distance
+// by 1 + 2^-40.  Tiny and zero direction components use the inverse +-2^200
+// instead of 1/dn': the builder's box padding keeps every ray that can hit a
+// triangle of the box inside it along such an axis by at least 1e-9 of the
+// scene extent, so its interval there still reaches past any hit.
+template <int A>
+RISE_HD inline unsigned node_accept_mask(const ObjRayT<A>& r, const uint4& w0, const uint4& w1, const uint4& w2,
+                                         const uint4& w3, const uint4& w4, float tmax)
+{
+    const unsigned pw[3] = {w0.x, w0.y, w0.z};
+    // Low planes: x in w2.x|y, y in w2.z|w, z in w3.x|y; high planes: x in
+    // w3.z|w, y in w4.x|y, z in w4.z|w (slots 0-3 | 4-7).
+    const unsigned lo[3][2] = {{w2.x, w2.y}, {w2.z, w2.w}, {w3.x, w3.y}};
+    const unsigned hi[3][2] = {{w3.z, w3.w}, {w4.x, w4.y}, {w4.z, w4.w}};
+    double P[3], S[3], G[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        P[a] = (double)bits_to_float(pw[a]);
+        S[a] = ldexp(1.0, (int)(signed char)((w0.w >> (8 * a)) & 0xffu));
+        G[a] = (fabs(P[a]) + 256.0 * S[a]) * 0x1p-50 + 0x1p-1000;
+    }
+    unsigned acc = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const unsigned meta = byte_of(s < 4 ? w1.z : w1.w, s & 3);
+        if (meta == 0) continue;
+        double tn = 0.0, tf = (double)tmax;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double l = (P[a] + (double)byte_of(lo[a][s >> 2], s & 3) * S[a]) - G[a];
+            const double h = (P[a] + (double)byte_of(hi[a][s >> 2], s & 3) * S[a]) + G[a];
+            const double t1 = (l - r.o[a]) * r.inv[a];
+            const double t2 = (h - r.o[a]) * r.inv[a];
+            const bool neg = r.inv[a] < 0.0;
+            tn = fmax(tn, neg ? t2 : t1);
+            tf = fmin(tf, neg ? t1 : t2);
+        }
+        if (tn <= tf * (1.0 + 0x1p-40)) acc |= 1u << s;
+    }
+    return acc;
+}
+
 // Tests the eight children of one node.  Returns the permuted internal-hit
 // mask (bit k <-> slot k ^ oinv) and fills the node's leaf-triangle mask.
-template <int A>
-RISE_HD inline unsigned test_node(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes, unsigned ni, float tmax,
+template <class R>
+RISE_HD inline unsigned test_node(const R& r, const NodeSlot* __restrict__ nodes, unsigned ni, float tmax,
                                      unsigned& child_base, unsigned& imask, unsigned& tri_base, unsigned& tmask)
 {
     const uint4* p = reinterpret_cast<const uint4*>(nodes + ni);
@@ -389,6 +529,15 @@ RISE_HD inline float box_limit(const RayStateT<A>& r, double best)
     return (float)s * RISE_SLAB_PAD;
 }
 
+// box_limit for an object-space ray: t is the distance along the unit dn'.
+template <int A>
+RISE_HD inline float box_limit(const ObjRayT<A>& r, double best)
+{
+    (void)r;
+    if (!(best < kRiseInfinity)) return INFINITY;
+    return (float)best * RISE_SLAB_PAD;
+}
+
 template <int STACK>
 struct LdsStack {
     uint2* base;   // STACK rows of blockDim.x entries
@@ -419,8 +568,8 @@ struct ClosestState {
     unsigned faces;  // face selection (face_ok), read by the culling variant only
 };
 
-template <int A>
-RISE_HD inline void closest_begin(const RayStateT<A>& r, ClosestState& s)
+template <class R>
+RISE_HD inline void closest_begin(const R& r, ClosestState& s)
 {
     s.gbase = 0;
     s.ghits = 1u << (0u ^ r.oinv);  // the root, in slot 0 of a virtual group
@@ -438,8 +587,8 @@ RISE_HD inline void closest_begin(const RayStateT<A>& r, ClosestState& s)
 // Closest hit: smallest t, exact ties to the lowest caller id (the
 // reference's strict-closer guard, ...Specializations.h:172, plus the tie rule).
 // CULL: apply the face selection in s.faces (face_ok) before the filter.
-template <bool COUNT, bool CULL = false, class Stack, int A>
-RISE_HD inline bool closest_step(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
+template <bool COUNT, bool CULL = false, class Stack, class R>
+RISE_HD inline bool closest_step(const R& r, const NodeSlot* __restrict__ nodes,
                                  const Tri48* __restrict__ tris, Stack& st, ClosestState& s, unsigned& n_nodes,
                                  unsigned& n_tris)
 {
@@ -487,8 +636,8 @@ RISE_HD inline bool closest_step(const RayStateT<A>& r, const NodeSlot* __restri
 // Closest-hit traversal.  Returns the caller id (-1 on miss), the best
 // parametric distance along dn in best_t and the leaf-order triangle index
 // of the hit in best_leaf.
-template <bool COUNT, class Stack, int A>
-RISE_HD inline int trace_closest(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
+template <bool COUNT, class Stack, class R>
+RISE_HD inline int trace_closest(const R& r, const NodeSlot* __restrict__ nodes,
                                  const Tri48* __restrict__ tris, Stack& st, double& best_t, unsigned& best_leaf,
                                  unsigned& n_nodes, unsigned& n_tris)
 {
@@ -511,8 +660,8 @@ struct AnyState {
     double how_far2;
 };
 
-template <int A>
-RISE_HD inline void any_begin(const RayStateT<A>& r, double how_far2, AnyState& s)
+template <class R>
+RISE_HD inline void any_begin(const R& r, double how_far2, AnyState& s)
 {
     s.gbase = 0;
     s.ghits = 1u << (0u ^ r.oinv);
@@ -528,8 +677,8 @@ RISE_HD inline void any_begin(const RayStateT<A>& r, double how_far2, AnyState& 
 // kNearZero < t < how_far2 (...Specializations.h:517-519) and casts shadows
 // (bit 0 of v2.w clear; ObjectManager.cpp:119-124).  Returns false once
 // the query is decided (a hit, or the tree exhausted).
-template <bool COUNT, bool CULL = false, class Stack, int A>
-RISE_HD inline bool any_step(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
+template <bool COUNT, bool CULL = false, class Stack, class R>
+RISE_HD inline bool any_step(const R& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
                              Stack& st, AnyState& s, unsigned& n_nodes, unsigned& n_tris)
 {
     unsigned tmask = 0, tbase = 0;
@@ -569,8 +718,8 @@ RISE_HD inline bool any_step(const RayStateT<A>& r, const NodeSlot* __restrict__
     return true;
 }
 
-template <bool COUNT, class Stack, int A>
-RISE_HD inline bool trace_any(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
+template <bool COUNT, class Stack, class R>
+RISE_HD inline bool trace_any(const R& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
                               Stack& st, double how_far2, unsigned& n_nodes, unsigned& n_tris)
 {
     AnyState s;

@@ -55,17 +55,68 @@ def hit_fields(records) -> np.ndarray:
     return np.frombuffer(np.ascontiguousarray(records).tobytes(), dtype=HIT_DTYPE)
 
 
+OBJECT_DTYPE = np.dtype([("mesh", "<i4"), ("reserved", "<i4"), ("to_world", "<f8", (16,)),
+                         ("to_object", "<f8", (16,))])
+"""rise_hip_object_instance: mesh index, 0, and the object's two Matrix4s
+(member order _00.._33, row-vector convention, translation in _30.._32)."""
+
+
+def object_matrices(linear=None, translation=(0.0, 0.0, 0.0), to_object=None):
+    """The Matrix4 pair (to_world, to_object) of an object, each 16 float64 in
+    member order.  linear: 3x3 acting on row vectors (p' = p @ linear +
+    translation), identity when None; to_object: its inverse when given (RISE
+    computes it with Matrix4Ops::Inverse), else numpy's inverse."""
+    m = np.eye(4)
+    if linear is not None:
+        m[:3, :3] = np.asarray(linear, dtype=np.float64)
+    m[3, :3] = np.asarray(translation, dtype=np.float64)
+    inv = np.linalg.inv(m) if to_object is None else np.asarray(to_object, dtype=np.float64).reshape(4, 4)
+    return m.reshape(16).copy(), inv.reshape(16).copy()
+
+
+def make_objects(entries) -> np.ndarray:
+    """OBJECT_DTYPE array from (mesh, to_world, to_object) triples."""
+    out = np.zeros(len(entries), dtype=OBJECT_DTYPE)
+    for k, (mesh, tw, to) in enumerate(entries):
+        out[k]["mesh"] = mesh
+        out[k]["to_world"] = np.asarray(tw, dtype=np.float64).reshape(16)
+        out[k]["to_object"] = np.asarray(to, dtype=np.float64).reshape(16)
+    return out
+
+
 class ObjectManager:
     """Resident triangle scene answering batched ray queries on the GPU.
+
+    Meshes added with add_mesh alone are world-space geometry (identity
+    objects).  Once add_object places a mesh, the scene is a list of objects
+    (rise_hip_scene_upload_objects): each object is one mesh with its
+    transform, intersected in its own frame as Object::IntersectRay does.
 
     arith selects the reference build whose hits are reproduced bit for bit
     (ARITH_IEEE or ARITH_X86_FMA; rise_hip_scene_set_arithmetic)."""
 
     def __init__(self, arith: int = ARITH_IEEE) -> None:
         self._meshes: list[tuple[np.ndarray, np.ndarray | None]] = []
+        self._objects: list[tuple[int, np.ndarray, np.ndarray]] = []
         self._scene = ctypes.c_void_p(None)
         self._arith = arith
         self._from_cache = False
+
+    def add_object(self, mesh: int, to_world=None, to_object=None) -> int:
+        """Places mesh `mesh` (an add_mesh index) by the Matrix4 to_world (16
+        float64 in member order, or 4x4; identity when None) with inverse
+        to_object (numpy's inverse when None).  Returns the object's index,
+        which hit records report in their `mesh` field."""
+        if not 0 <= mesh < len(self._meshes):
+            raise ValueError("mesh index out of range")
+        if to_world is None:
+            tw, to = object_matrices()
+        else:
+            tw = np.asarray(to_world, dtype=np.float64).reshape(16).copy()
+            to = (np.linalg.inv(tw.reshape(4, 4)).reshape(16) if to_object is None
+                  else np.asarray(to_object, dtype=np.float64).reshape(16).copy())
+        self._objects.append((int(mesh), tw, to))
+        return len(self._objects) - 1
 
     def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None) -> None:
         """tris: (n, 9) float32 world-space triangles; ids: optional (n,) int32."""
@@ -88,7 +139,12 @@ class ObjectManager:
             descs[k].n_tris = t.shape[0]
             descs[k].tri_ids = None if i is None else i.ctypes.data
         out = ctypes.c_void_p(None)
-        _lib.check(L.rise_hip_scene_upload(descs, len(self._meshes), ctypes.byref(out)))
+        if self._objects:
+            objs = make_objects(self._objects)
+            _lib.check(L.rise_hip_scene_upload_objects(descs, len(self._meshes), objs.ctypes.data, len(objs),
+                                                       ctypes.byref(out)))
+        else:
+            _lib.check(L.rise_hip_scene_upload(descs, len(self._meshes), ctypes.byref(out)))
         self._scene = out
         self.set_arithmetic(self._arith)
 
@@ -272,4 +328,36 @@ def host_trace_ex(tris: np.ndarray, rays: np.ndarray, mode: int = 0, faces: int 
     _lib.check(L.rise_hip_debug_trace_ex_host(tris.ctypes.data, tris.shape[0], rays.ctypes.data,
                                               None if tm is None else tm.ctypes.data, n, mode, int(faces), None,
                                               occ.ctypes.data))
+    return occ
+
+
+def host_trace_objects(meshes, objects, rays: np.ndarray, mode: int = 0, faces: int = FACES_BOTH,
+                       tmax: np.ndarray | None = None, arith: int = ARITH_IEEE, ids=None):
+    """rise_hip_debug_objects_host: a scene of objects (meshes: list of (n, 9)
+    float32 arrays; objects: OBJECT_DTYPE array) queried with the kernels'
+    per-ray code on host threads (validation only).  Mode 0: hit records
+    (HIT_DTYPE, `mesh` = object index); mode 1: occlusion bytes."""
+    mode = (mode & 1) | (2 if arith == ARITH_X86_FMA else 0)
+    L = _lib.lib()
+    arrs = [np.ascontiguousarray(m, dtype=np.float32).reshape(-1, 9) for m in meshes]
+    id_arrs = [None if ids is None or ids[k] is None else np.ascontiguousarray(ids[k], np.int32)
+               for k in range(len(arrs))]
+    descs = (_lib.MeshDesc * max(1, len(arrs)))()
+    for k, t in enumerate(arrs):
+        descs[k].vertices = t.ctypes.data
+        descs[k].n_tris = t.shape[0]
+        descs[k].tri_ids = None if id_arrs[k] is None else id_arrs[k].ctypes.data
+    objs = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
+    if (mode & 1) == 0:
+        out = np.zeros(n, dtype=HIT_DTYPE)
+        _lib.check(L.rise_hip_debug_objects_host(descs, len(arrs), objs.ctypes.data, len(objs), rays.ctypes.data,
+                                                 None, n, mode, int(faces), out.ctypes.data, None))
+        return out
+    occ = np.empty(n, dtype=np.uint8)
+    _lib.check(L.rise_hip_debug_objects_host(descs, len(arrs), objs.ctypes.data, len(objs), rays.ctypes.data,
+                                             None if tm is None else tm.ctypes.data, n, mode, int(faces), None,
+                                             occ.ctypes.data))
     return occ

@@ -119,3 +119,72 @@ def near_edge_rays(tris: np.ndarray, n: int, seed: int, band: float = 3e-5) -> n
     d = p - o
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return rays_from_rows(o, d)
+
+
+def random_object_matrix(g: np.random.Generator, mirror: bool = False, spread: float = 4.0):
+    """A random placement: rotation (a reflection too when mirror), non-uniform
+    scale in [0.3, 3] and a translation in [-spread, spread]^3, as the
+    Matrix4 pair (to_world, to_object) of 16 float64 each, member order
+    (row-vector convention)."""
+    from .raycast import object_matrices
+
+    q, r = np.linalg.qr(g.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if (np.linalg.det(q) < 0) != mirror:
+        q[:, 0] = -q[:, 0]
+    lin = np.diag(g.uniform(0.3, 3.0, 3)) @ q
+    return object_matrices(lin, g.uniform(-spread, spread, 3))
+
+
+def instanced_scene(n_meshes: int = 3, n_objects: int = 12, tris_per_mesh: int = 2000, seed: int = 31,
+                    n_identity: int = 2):
+    """A scene of objects: n_meshes seeded soups in their own frame (centroids
+    in [-1, 1]^3, edges N(0, 0.08^2)), placed by n_objects objects that share
+    them; the first n_identity objects keep identity matrices, one object is
+    mirrored.  Returns (meshes, objects) with objects an OBJECT_DTYPE array."""
+    from .raycast import make_objects, object_matrices
+
+    g = _rng(seed)
+    meshes = []
+    for m in range(n_meshes):
+        t = soup_triangles(tris_per_mesh, seed=seed * 100 + m, sigma=0.08).astype(np.float64)
+        meshes.append(np.ascontiguousarray((t * 2.0 - 1.0).astype(np.float32)))
+    entries = []
+    for k in range(n_objects):
+        mesh = k % n_meshes
+        if k < n_identity:
+            tw, to = object_matrices()
+        else:
+            tw, to = random_object_matrix(g, mirror=(k == n_identity))
+        entries.append((mesh, tw, to))
+    return meshes, make_objects(entries)
+
+
+def object_rays(meshes, objects, n: int, seed: int, spread: float = 6.0) -> np.ndarray:
+    """Rays for a scene of objects: three quarters aimed at world points of
+    random triangles of random objects (a quarter of those within 3e-5, in
+    barycentrics, of an edge), the rest from random origins in random
+    directions; origins uniform in [-spread, spread]^3, directions of random
+    length in [0.5, 2] (the reference does not normalize them)."""
+    g = _rng(seed)
+    o = g.uniform(-spread, spread, (n, 3))
+    d = g.normal(size=(n, 3))
+    aimed = g.random(n) < 0.75
+    for i in np.nonzero(aimed)[0]:
+        ob = objects[int(g.integers(0, len(objects)))]
+        t = np.asarray(meshes[int(ob["mesh"])], np.float64).reshape(-1, 9)
+        tri = t[int(g.integers(0, t.shape[0]))]
+        if g.random() < 0.25:
+            along, off = g.random(), (g.random() * 2.0 - 1.0) * 3e-5
+            u, v = [(off, along), (along, off), (along - off, 1.0 - along)][int(g.integers(0, 3))]
+        else:
+            u, v = g.random(), g.random()
+            if u + v > 1.0:
+                u, v = 1.0 - u, 1.0 - v
+        p = tri[0:3] + u * (tri[3:6] - tri[0:3]) + v * (tri[6:9] - tri[0:3])
+        m = np.asarray(ob["to_world"], np.float64).reshape(4, 4)
+        pw = p @ m[:3, :3] + m[3, :3]
+        d[i] = pw - o[i]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d *= g.uniform(0.5, 2.0, (n, 1))
+    return rays_from_rows(o, d)
