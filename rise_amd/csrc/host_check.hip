@@ -166,24 +166,26 @@ extern "C" rise_hip_status rise_hip_debug_objects_host(const rise_hip_mesh_desc*
     std::vector<BlasView> views;
     for (const WideBVH& b : host.blas) views.push_back(BlasView{b.nodes.data(), b.tris.data()});
     const ObjectSceneView sc{host.world_tris > 0 ? host.world.nodes.data() : nullptr, host.world.tris.data(),
-                             host.objects.data(), (int32_t)host.objects.size(), views.data()};
+                             host.objects.data(), (int32_t)host.objects.size(), views.data(),
+                             host.top.nodes.data(), host.top.tris.data()};
     std::atomic<int64_t> next(0);
     std::atomic<bool> overflow(false);
     auto one = [&](auto arith, int64_t i) {
         constexpr int A = decltype(arith)::value;
         float ray[6];
         for (int k = 0; k < 6; ++k) ray[k] = rays[(int64_t)k * n + i];
-        HostStack hs;
+        ObjStack st;
         if ((mode & 1) == 0) {
             ObjectHit h;
-            objects_closest<A>(sc, ray, (unsigned)faces, hs, h);
+            objects_closest<A>(sc, ray, (unsigned)faces, st, h);
             rise_hip_hit rec{};
             objects_record<A>(sc, ray, h, rec);
             hits[i] = rec;
         } else {
             const double hf = tmax ? (double)tmax[i] : kRiseInfinity;
-            occluded[i] = objects_any<A>(sc, ray, hf, (unsigned)faces, hs) ? 1 : 0;
+            occluded[i] = objects_any<A>(sc, ray, hf, (unsigned)faces, st) ? 1 : 0;
         }
+        if (st.overflow) overflow = true;
     };
     auto worker = [&]() {
         for (;;) {

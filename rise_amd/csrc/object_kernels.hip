@@ -1,7 +1,8 @@
 // object_kernels.hip -- batch closest-hit and any-hit kernels over a scene of
 // objects with transforms (object_trace.h).  One lane per ray, the node-group
-// stack in a private array; every lane runs the world tree and then the
-// transformed objects in the caller's order.
+// stack in a private array; every lane runs the world tree, then the
+// top-level tree over the transformed objects and the trees of the objects
+// it reaches.
 #include "object_kernels.h"
 
 #include "object_trace.h"
@@ -29,8 +30,7 @@ __global__ __launch_bounds__(kObjBlock) void closest_objects(ObjectSceneView sc,
     if (i >= n) return;
     float ray[6];
     load_ray6(rays, n, i, ray);
-    PrivStack st;
-    st.sp = 0;
+    ObjStack st;
     ObjectHit h;
     objects_closest<A>(sc, ray, faces, st, h);
     if (REC) {
@@ -52,8 +52,7 @@ __global__ __launch_bounds__(kObjBlock) void any_objects(ObjectSceneView sc, con
     if (i >= n) return;
     float ray[6];
     load_ray6(rays, n, i, ray);
-    PrivStack st;
-    st.sp = 0;
+    ObjStack st;
     const double how_far = tmax ? (double)tmax[i] : kRiseInfinity;
     occluded[i] = objects_any<A>(sc, ray, how_far, faces, st) ? 1 : 0;
 }

@@ -33,14 +33,28 @@ void xform(const double* m, const double p[3], double out[3])
     for (int j = 0; j < 3; ++j) out[j] = m[j] * p[0] + m[4 + j] * p[1] + m[8 + j] * p[2] + m[12 + j];
 }
 
+// Rounding bound of a world position taken to object space and back:
+// 2^-40 times 4 ||L||_inf ||L^-1||_inf of the linear parts (the two matrices'
+// upper 3x3), far above the few ulps of the two transforms.
+double transform_slack(const double* m, const double* mi)
+{
+    double n = 0.0, ni = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        n = std::max(n, std::fabs(m[4 * i]) + std::fabs(m[4 * i + 1]) + std::fabs(m[4 * i + 2]));
+        ni = std::max(ni, std::fabs(mi[4 * i]) + std::fabs(mi[4 * i + 1]) + std::fabs(mi[4 * i + 2]));
+    }
+    return 4.0 * n * ni * 0x1p-40;
+}
+
 // World box of an object: the mesh's vertex bounds in object space, widened by
 // 1e-7 of their size and coordinates plus 1e-10 (the hit point the reference
 // transforms back lies within the certifier's 1e-12 tolerance and the 1e-12
 // back-off of the triangle), its eight corners taken to world space, and the
-// result widened again by 1e-7.  It contains every world point the reference
-// can report for the object, and every point of the exact world ray that
-// passes within rounding of one.
-void world_box(const float* verts, int64_t n_tris, const double* to_world, double lo[3], double hi[3])
+// result widened again by the larger of 1e-7 and the transform's slack.  It
+// contains every world point the reference can report for the object, and
+// every point of the exact world ray that passes within rounding of one.
+void world_box(const float* verts, int64_t n_tris, const double* to_world, double slack, double lo[3],
+               double hi[3])
 {
     double olo[3] = {INFINITY, INFINITY, INFINITY}, ohi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t i = 0; i < n_tris * 3; ++i)
@@ -69,7 +83,7 @@ void world_box(const float* verts, int64_t n_tris, const double* to_world, doubl
     }
     double wspan = 0.0;
     for (int a = 0; a < 3; ++a) wspan = std::max({wspan, hi[a] - lo[a], std::fabs(lo[a]), std::fabs(hi[a])});
-    const double pad_w = wspan * 1e-7 + 1e-10;
+    const double pad_w = wspan * std::max(1e-7, slack) + 1e-10;
     for (int a = 0; a < 3; ++a) {
         lo[a] -= pad_w;
         hi[a] += pad_w;
@@ -151,11 +165,34 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             DevObject d{};
             std::memcpy(d.to_world, ob.to_world, sizeof(d.to_world));
             std::memcpy(d.to_object, ob.to_object, sizeof(d.to_object));
-            world_box(m.vertices, m.n_tris, ob.to_world, d.lo, d.hi);
+            d.slack = transform_slack(ob.to_world, ob.to_object);
+            world_box(m.vertices, m.n_tris, ob.to_world, d.slack, d.lo, d.hi);
             d.blas = blas_of[ob.mesh];
             d.object = (int32_t)k;
             out.objects.push_back(d);
         }
+        // Top-level tree: each object's world box as the triangle (lo, hi,
+        // (lo.x, lo.y, hi.z)), whose bounds are the box, rounded outward to
+        // float; the leaf record's id is the object's position in `objects`.
+        const size_t no = out.objects.size();
+        std::vector<float> boxes(no * 9);
+        for (size_t k = 0; k < no; ++k) {
+            float lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = (float)out.objects[k].lo[a];
+                if ((double)lo[a] > out.objects[k].lo[a]) lo[a] = std::nextafter(lo[a], -INFINITY);
+                hi[a] = (float)out.objects[k].hi[a];
+                if ((double)hi[a] < out.objects[k].hi[a]) hi[a] = std::nextafter(hi[a], INFINITY);
+            }
+            const float v[9] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], lo[0], lo[1], hi[2]};
+            std::memcpy(&boxes[k * 9], v, sizeof(v));
+        }
+        if (!build_wide_bvh(boxes.data(), (int64_t)no, nullptr, BuildParams(), out.top, err))
+            return fail(RISE_HIP_ERR_BUILD, "top-level BVH build failed: " + err);
+        int blas_depth = 0;
+        for (const WideBVH& b : out.blas) blas_depth = std::max(blas_depth, b.stats.depth);
+        if (no > 0 && out.top.stats.depth + blas_depth > kObjStack)
+            return fail(RISE_HIP_ERR_BUILD, "top-level and object trees too deep for the traversal stack");
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }

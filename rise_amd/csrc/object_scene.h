@@ -16,6 +16,7 @@ struct ObjectSceneHost {
     int64_t world_tris = 0;
     std::vector<WideBVH> blas;      // one per mesh used by a transformed object, object space
     std::vector<DevObject> objects; // the transformed objects, in the caller's order
+    WideBVH top;                    // over the transformed objects' world boxes (leaf v0.w = index into objects)
     int64_t total_tris = 0;         // triangles over all trees
 };
 

@@ -11,10 +11,14 @@
 // world-space tree (their object space is world space, and for them the
 // reference's transforms are exact), and every other object keeps an
 // object-space tree of its mesh, shared by all objects on that mesh.  A query
-// runs the world tree with the float-origin ray state (RayStateT), then each
-// transformed object whose conservative world box the ray reaches within the
-// current best, with the double object-space ray state (ObjRayT).  The per-ray
-// functions here are compiled for host and device.
+// runs the world tree with the float-origin ray state (RayStateT), then walks
+// a top-level tree over the transformed objects' padded world boxes near to
+// far, skipping boxes entered beyond the current best; each object reached
+// traverses its own tree with the double object-space ray state (ObjRayT),
+// its boxes pruned at the best's object-space image (widened for rounding).
+// Objects are visited out of the caller's order, so ties are settled as the
+// caller's order would: an equal world range goes to the lower object index.
+// The per-ray functions here are compiled for host and device.
 #pragma once
 
 #include "object_types.h"
@@ -74,6 +78,67 @@ RISE_HD inline double obj_how_far(double factor, double how_far)
     return how_far;
 }
 
+// Node-group stack of an object query with a movable floor: an object's tree
+// is walked on top of the top-level tree's entries.  The upload checks that
+// the depths fit; a push past the end is dropped and flagged.
+struct ObjStack {
+    uint2 e[kObjStack];
+    int sp;
+    int base;
+    bool overflow;
+    RISE_HD inline void push(uint2 v)
+    {
+        if (sp < kObjStack) e[sp++] = v;
+        else overflow = true;
+    }
+    RISE_HD inline uint2 pop() { return e[--sp]; }
+    RISE_HD inline bool empty() const { return sp == base; }
+};
+
+// Walks the top-level tree near to far with the world ray; visit(k) is called
+// for each object record k reached and returns the new box limit, or a
+// negative value to end the walk.
+template <int A, class Visit>
+RISE_HD inline void top_walk(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
+                             const Tri48* __restrict__ leaves, ObjStack& st, float tlim, Visit&& visit)
+{
+    unsigned gbase = 0, ghits = 1u << (0u ^ r.oinv), gimask = 1u;
+    const int floor = st.sp;
+    for (;;) {
+        unsigned tmask = 0, tbase = 0;
+        if (ghits) {
+            const unsigned k = __builtin_ctz(ghits);
+            ghits &= ghits - 1u;
+            const unsigned slot = k ^ r.oinv;
+            const unsigned ni = gbase + __builtin_popcount(gimask & ((1u << slot) - 1u));
+            if (ghits) st.push(make_uint2(gbase, ghits | (gimask << 8)));
+            unsigned cb, im;
+            ghits = test_node(r, nodes, ni, tlim, cb, im, tbase, tmask);
+            gbase = cb;
+            gimask = im;
+        }
+        while (tmask) {
+            const unsigned j = __builtin_ctz(tmask);
+            tmask &= tmask - 1u;
+            tlim = visit(float_to_bits(leaves[tbase + j].v0[3]));
+            if (tlim < 0.0f) {
+                st.sp = floor;
+                return;
+            }
+        }
+        if (!ghits) {
+            if (st.sp == floor) break;
+            const uint2 g = st.pop();
+            gbase = g.x;
+            ghits = g.y & 0xffu;
+            gimask = (g.y >> 8) & 0xffu;
+        }
+    }
+}
+
+// Largest absolute coordinate.
+RISE_HD inline double max_abs3(const double v[3]) { return fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2]))); }
+
 // Result of a closest-hit query over objects.
 struct ObjectHit {
     int32_t prim_id;  // caller id of the triangle, -1 on a miss
@@ -85,11 +150,11 @@ struct ObjectHit {
     double p_world[3];
 };
 
-// ObjectManager::IntersectRay over the scene (linear order of the objects;
-// strictly closer world range wins).  Stack: a node-group stack with
-// push/pop/empty and an `sp` member.
-template <int A, class Stack>
-RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6], unsigned faces, Stack& st,
+// ObjectManager::IntersectRay over the scene: each object's own closest hit
+// (Object::IntersectRay), the strictly closer world range winning, ties to the
+// lower object index (the caller's order).
+template <int A>
+RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6], unsigned faces, ObjStack& st,
                                     ObjectHit& h)
 {
     h.prim_id = -1;
@@ -98,6 +163,9 @@ RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6
     h.leaf = 0;
     h.t = kRiseInfinity;
     h.range = kRiseInfinity;
+    st.sp = 0;
+    st.base = 0;
+    st.overflow = false;
     RayStateT<A> r;
     init_ray(r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
     unsigned nn = 0, nt = 0;
@@ -105,7 +173,6 @@ RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6
         ClosestState s;
         closest_begin(r, s);
         s.faces = faces;
-        st.sp = 0;
         while (closest_step<false, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt)) {
         }
         if (s.best_id >= 0) {
@@ -117,24 +184,39 @@ RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6
             h.range = world_range(r, s.best_t);
         }
     }
+    if (sc.n_objects == 0) return;
     const double ow[3] = {ray[0], ray[1], ray[2]};
     const double dw[3] = {ray[3], ray[4], ray[5]};
-    for (int32_t k = 0; k < sc.n_objects; ++k) {
+    const double omax = max_abs3(ow);
+    // A hit closer than the best lies in its object's padded box within the
+    // best (plus rounding) along the world ray.
+    auto limit = [&]() { return box_limit(r, h.range < kRiseInfinity ? h.range * (1.0 + 1e-7) : kRiseInfinity); };
+    top_walk(r, sc.top_nodes, sc.top_leaves, st, limit(), [&](int k) -> float {
         const DevObject& ob = sc.objects[k];
-        if (!world_box_reach(ob.lo, ob.hi, ow, dw, r.len_d, h.range)) continue;
+        if (!world_box_reach(ob.lo, ob.hi, ow, dw, r.len_d, h.range)) return limit();
         ObjRayT<A> q;
         init_obj_ray(q, ow, dw, ob.to_object);
         const BlasView bl = sc.blas[ob.blas];
         ClosestState s;
         closest_begin(q, s);
         s.faces = faces;
-        st.sp = 0;
+        if (h.range < kRiseInfinity && r.len_d > 0.0) {
+            // A world range w is an object-space distance of about w |dl| / |d|;
+            // the rounding of o' and of the point taken back to world space is
+            // below slack (|o| + |translation| + w).
+            const double tw[3] = {ob.to_world[12], ob.to_world[13], ob.to_world[14]};
+            const double lim_w = h.range * (1.0 + ob.slack + 1e-9) + ob.slack * (omax + max_abs3(tw));
+            s.tlim = box_limit(q, lim_w * (q.mag / r.len_d) * (1.0 + 1e-9));
+        }
+        const int floor = st.base;
+        st.base = st.sp;
         while (closest_step<false, true>(q, bl.nodes, bl.tris, st, s, nn, nt)) {
         }
-        if (s.best_id < 0) continue;
+        st.base = floor;
+        if (s.best_id < 0) return limit();
         double pw[3];
         const double w = obj_world_range(q, ob.to_world, ow, s.best_t, pw);
-        if (w < h.range) {
+        if (w < h.range || (w == h.range && ob.object < h.object)) {
             h.prim_id = s.best_id;
             h.object = ob.object;
             h.src = k;
@@ -143,7 +225,8 @@ RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6
             h.range = w;
             for (int c = 0; c < 3; ++c) h.p_world[c] = pw[c];
         }
-    }
+        return limit();
+    });
 }
 
 // The intersection record of a committed hit (rise_hip_hit): barycentrics of
@@ -203,11 +286,15 @@ RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6]
 
 // ObjectManager::IntersectShadowRay over the scene (ObjectManager.cpp:320-373
 // with Object::IntersectRay_IntersectionOnly per object).  how_far: the
-// world limit dHowFar.
-template <int A, class Stack>
+// world limit dHowFar.  The answer does not depend on the order objects are
+// tried in.
+template <int A>
 RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], double how_far, unsigned faces,
-                                Stack& st)
+                                ObjStack& st)
 {
+    st.sp = 0;
+    st.base = 0;
+    st.overflow = false;
     RayStateT<A> r;
     init_ray(r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
     unsigned nn = 0, nt = 0;
@@ -215,14 +302,17 @@ RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], d
         AnyState s;
         any_begin(r, scale_how_far(r, how_far), s);
         s.faces = faces;
-        st.sp = 0;
         while (any_step<false, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt)) {
         }
         if (s.hit) return true;
     }
+    if (sc.n_objects == 0) return false;
     const double ow[3] = {ray[0], ray[1], ray[2]};
     const double dw[3] = {ray[3], ray[4], ray[5]};
-    for (int32_t k = 0; k < sc.n_objects; ++k) {
+    bool hit = false;
+    // The top-level walk takes no distance limit: an object whose transform
+    // nearly collapses the direction has no bounded world reach.
+    top_walk(r, sc.top_nodes, sc.top_leaves, st, INFINITY, [&](int k) -> float {
         const DevObject& ob = sc.objects[k];
         ObjRayT<A> q;
         init_obj_ray(q, ow, dw, ob.to_object);
@@ -232,17 +322,21 @@ RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], d
         // space (widened by 1e-6 for the rounding of the two transforms).
         const double lim = (how_far2 < kRiseInfinity && q.mag > 0.0) ? how_far2 * r.len_d / q.mag * (1.0 + 1e-6)
                                                                      : kRiseInfinity;
-        if (!world_box_reach(ob.lo, ob.hi, ow, dw, r.len_d, lim)) continue;
+        if (!world_box_reach(ob.lo, ob.hi, ow, dw, r.len_d, lim)) return INFINITY;
         const BlasView bl = sc.blas[ob.blas];
         AnyState s;
         any_begin(q, how_far2, s);
         s.faces = faces;
-        st.sp = 0;
+        const int floor = st.base;
+        st.base = st.sp;
         while (any_step<false, true>(q, bl.nodes, bl.tris, st, s, nn, nt)) {
         }
-        if (s.hit) return true;
-    }
-    return false;
+        st.sp = st.base;  // an early exit leaves the object's entries
+        st.base = floor;
+        hit = s.hit;
+        return hit ? -1.0f : INFINITY;
+    });
+    return hit;
 }
 
 }  // namespace rise_hip

@@ -38,12 +38,15 @@ struct rise_hip_scene {
     // Transformed objects (rise_hip_scene_upload_objects); none otherwise.
     DevObject* d_objects = nullptr;
     BlasView* d_blas = nullptr;
+    NodeSlot* d_top_nodes = nullptr;
+    Tri48* d_top_leaves = nullptr;
     std::vector<void*> blas_allocs;
     int32_t n_objects = 0;
     bool world_empty = false;
     ObjectSceneView objects_view() const
     {
-        return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_objects, n_objects, d_blas};
+        return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_objects, n_objects, d_blas,
+                               d_top_nodes, d_top_leaves};
     }
     TraceScene view()
     {
@@ -230,11 +233,16 @@ rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, 
     }
     if (e == hipSuccess && !host.objects.empty()) {
         const size_t ob = host.objects.size() * sizeof(DevObject), vb = views.size() * sizeof(BlasView);
+        const size_t tn = host.top.nodes.size() * sizeof(NodeSlot), tl = host.top.tris.size() * sizeof(Tri48);
         e = hipMalloc((void**)&s->d_objects, ob);
         if (e == hipSuccess) e = hipMalloc((void**)&s->d_blas, vb);
+        if (e == hipSuccess) e = hipMalloc((void**)&s->d_top_nodes, tn);
+        if (e == hipSuccess) e = hipMalloc((void**)&s->d_top_leaves, tl);
         if (e == hipSuccess) e = hipMemcpy(s->d_objects, host.objects.data(), ob, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(s->d_blas, views.data(), vb, hipMemcpyHostToDevice);
-        s->stats.device_bytes += (int64_t)(ob + vb);
+        if (e == hipSuccess) e = hipMemcpy(s->d_top_nodes, host.top.nodes.data(), tn, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(s->d_top_leaves, host.top.tris.data(), tl, hipMemcpyHostToDevice);
+        s->stats.device_bytes += (int64_t)(ob + vb + tn + tl);
     }
     if (e != hipSuccess) {
         (void)rise_hip_scene_release(s);
@@ -252,6 +260,8 @@ rise_hip_status rise_hip_scene_release(rise_hip_scene* scene)
     for (void* p : scene->blas_allocs) (void)hipFree(p);
     (void)hipFree(scene->d_objects);
     (void)hipFree(scene->d_blas);
+    (void)hipFree(scene->d_top_nodes);
+    (void)hipFree(scene->d_top_leaves);
     (void)hipFree(scene->d_nodes);
     (void)hipFree(scene->d_tris);
     (void)hipFree(scene->d_fetch);

@@ -211,17 +211,21 @@ RISE_HD inline void xform_normal(const double* minv, const double n[3], double o
 // as given, dn' = Normalize(dl), all in double, and factor = |dl| (1 when it
 // is not above NEARZERO, :472-485).  The float pre-filter sees (float)o' and
 // (float)dn' (PrepRayFloat, BVH.h:1212-1225); the certifier sees o' and dn'.
-// o' is not a float, so this ray's boxes are tested in double
-// (node_accept_mask below).
+// The boxes are tested with the FP32 slab test of the float ray ((float)o',
+// (float)dn' scaled as in init_ray); o' is not a float, so each axis's plane
+// distances take the origin's rounding |o' - (float)o'| / d as extra slack
+// (the direction's rounding is relative, covered by RISE_SLAB_PAD).
 template <int A>
 struct ObjRayT {
     static constexpr int kArith = A;
     float ox, oy, oz;  // (float)o'
     float fx, fy, fz;  // (float)dn'
-    unsigned oinv;     // (~octant of dn') & 7
+    float ix, iy, iz;  // slab inverses of (float)dn' scaled as in init_ray
+    float ex, ey, ez;  // extra slack per axis: |o' - (float)o'| |inverse| (1 + 2^-20)
+    unsigned oinv;     // (~octant of (float)dn') & 7
     double o[3];       // o'
     double dn[3];      // dn'
-    double inv[3];     // 1/dn' (+-2^200 for components below 2^-200, + for a zero)
+    double len_s;      // |(float)dn'| 2^k: t along the scaled direction = distance / len_s
     double mag;        // |dl|
     double factor;     // |dl|, or 1 when |dl| <= NEARZERO
 };
@@ -256,13 +260,21 @@ RISE_HD inline void init_obj_ray(ObjRayT<A>& r, const double ow[3], const double
     r.fx = (float)r.dn[0];
     r.fy = (float)r.dn[1];
     r.fz = (float)r.dn[2];
-    unsigned oct = 0;
-    for (int k = 0; k < 3; ++k) {
-        const double v = r.dn[k];
-        if (v < 0.0) oct |= 1u << k;
-        r.inv[k] = fabs(v) < 0x1p-200 ? (v < 0.0 ? -0x1p200 : 0x1p200) : 1.0 / v;
-    }
+    const float m = fmaxf(fmaxf(fabsf(r.fx), fabsf(r.fy)), fabsf(r.fz));
+    int k = 0;
+    if (m > 0.0f) (void)frexpf(m, &k);
+    const int sh = 1 - k;
+    r.ix = slab_inv(ldexpf(r.fx, sh));
+    r.iy = slab_inv(ldexpf(r.fy, sh));
+    r.iz = slab_inv(ldexpf(r.fz, sh));
+    const unsigned oct = (r.fx < 0.0f ? 1u : 0u) | (r.fy < 0.0f ? 2u : 0u) | (r.fz < 0.0f ? 4u : 0u);
     r.oinv = (~oct) & 7u;
+    const double f[3] = {r.fx, r.fy, r.fz};
+    const double lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    r.len_s = lf > 0.0 ? ldexp(lf, sh) : 1.0;
+    r.ex = (float)(fabs(r.o[0] - (double)r.ox) * fabs((double)r.ix) * (1.0 + 0x1p-20));
+    r.ey = (float)(fabs(r.o[1] - (double)r.oy) * fabs((double)r.iy) * (1.0 + 0x1p-20));
+    r.ez = (float)(fabs(r.o[2] - (double)r.oz) * fabs((double)r.iz) * (1.0 + 0x1p-20));
 }
 
 // BVH::MollerTrumboreFloat (Acceleration/BVH.h:1111-1167) on the leaf
@@ -402,22 +414,30 @@ RISE_HD inline SlabAxis slab_axis(float p, int e, float o, float inv)
     return SlabAxis{0.0f, -INFINITY, INFINITY};
 }
 
-// The eight child boxes of one node against the ray: bit s of the result set
-// when slot s is occupied and its box meets [0, tmax] (conservatively).  The
-// quantized planes are split by the sign of the inverse into near and far
-// words per axis, so each slot costs six byte conversions, six FMAs and the
-// interval reductions.
-template <int A>
-RISE_HD inline unsigned node_accept_mask(const RayStateT<A>& r, const uint4& w0, const uint4& w1, const uint4& w2,
-                                         const uint4& w3, const uint4& w4, float tmax)
+// slab_axis for an origin known only to within `extra / |inv|` of o: the
+// plane distances widen by `extra` on top of their own slack (the sum, rounded
+// to nearest, is below 2^127 while |b| < 2^126 and extra < 2^125).
+RISE_HD inline SlabAxis slab_axis_widened(float p, int e, float o, float inv, float extra)
 {
-    const SlabAxis ax = slab_axis(bits_to_float(w0.x), (int)(signed char)(w0.w & 0xff), r.ox, r.ix);
-    const SlabAxis ay = slab_axis(bits_to_float(w0.y), (int)(signed char)((w0.w >> 8) & 0xff), r.oy, r.iy);
-    const SlabAxis az = slab_axis(bits_to_float(w0.z), (int)(signed char)((w0.w >> 16) & 0xff), r.oz, r.iz);
+    const float b = (p - o) * inv;
+    const float s = ldexpf(inv, e);
+    const float slack = fmaf(fabsf(b), 0x1p-21f, 0x1p-100f) + extra * (1.0f + 0x1p-22f);
+    if (fabsf(b) < 0x1p126f && fabsf(s) < 0x1p119f && extra < 0x1p125f) return SlabAxis{s, b - slack, b + slack};
+    return SlabAxis{0.0f, -INFINITY, INFINITY};
+}
+
+// The eight child boxes of one node against the axes' plane distances: bit s
+// of the result set when slot s is occupied and its box meets [0, tmax]
+// (conservatively).  The quantized planes are split by the sign of the
+// inverse into near and far words per axis, so each slot costs six byte
+// conversions, six FMAs and the interval reductions.
+RISE_HD inline unsigned accept_slots(const SlabAxis& ax, const SlabAxis& ay, const SlabAxis& az, bool nx, bool ny,
+                                     bool nz, const uint4& w1, const uint4& w2, const uint4& w3, const uint4& w4,
+                                     float tmax)
+{
     // Low planes: qlox = w2.x|y, qloy = w2.z|w, qloz = w3.x|y; high planes:
     // qhix = w3.z|w, qhiy = w4.x|y, qhiz = w4.z|w (slots 0-3 | 4-7).  With a
     // negative inverse the high plane is the near one.
-    const bool nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
     const unsigned nx0 = nx ? w3.z : w2.x, nx1 = nx ? w3.w : w2.y;
     const unsigned fx0 = nx ? w2.x : w3.z, fx1 = nx ? w2.y : w3.w;
     const unsigned ny0 = ny ? w4.x : w2.z, ny1 = ny ? w4.y : w2.w;
@@ -443,51 +463,28 @@ RISE_HD inline unsigned node_accept_mask(const RayStateT<A>& r, const uint4& w0,
     return acc;
 }
 
-// The eight child boxes of one node against an object-space ray, in double.
-// The plane at quantized coordinate q lies at p + q 2^e; computed in double it
-// is within 2^-53 of its size, so each plane is first moved outward by
-// 2^-50 (|p| + 256 2^e), which leaves the box containing the exact one.  Its
-// distance (plane - o') / dn' then carries only the relative error of the
-// subtraction and the product (2^-52), covered by widening the exit distance
-// by 1 + 2^-40.  Tiny and zero direction components use the inverse +-2^200
-// instead of 1/dn': the builder's box padding keeps every ray that can hit a
-// triangle of the box inside it along such an axis by at least 1e-9 of the
-// scene extent, so its interval there still reaches past any hit.
+template <int A>
+RISE_HD inline unsigned node_accept_mask(const RayStateT<A>& r, const uint4& w0, const uint4& w1, const uint4& w2,
+                                         const uint4& w3, const uint4& w4, float tmax)
+{
+    const SlabAxis ax = slab_axis(bits_to_float(w0.x), (int)(signed char)(w0.w & 0xff), r.ox, r.ix);
+    const SlabAxis ay = slab_axis(bits_to_float(w0.y), (int)(signed char)((w0.w >> 8) & 0xff), r.oy, r.iy);
+    const SlabAxis az = slab_axis(bits_to_float(w0.z), (int)(signed char)((w0.w >> 16) & 0xff), r.oz, r.iz);
+    return accept_slots(ax, ay, az, r.ix < 0.0f, r.iy < 0.0f, r.iz < 0.0f, w1, w2, w3, w4, tmax);
+}
+
+// The same for an object-space ray: the float ray's planes widened by the
+// origin's rounding (ObjRayT).
 template <int A>
 RISE_HD inline unsigned node_accept_mask(const ObjRayT<A>& r, const uint4& w0, const uint4& w1, const uint4& w2,
                                          const uint4& w3, const uint4& w4, float tmax)
 {
-    const unsigned pw[3] = {w0.x, w0.y, w0.z};
-    // Low planes: x in w2.x|y, y in w2.z|w, z in w3.x|y; high planes: x in
-    // w3.z|w, y in w4.x|y, z in w4.z|w (slots 0-3 | 4-7).
-    const unsigned lo[3][2] = {{w2.x, w2.y}, {w2.z, w2.w}, {w3.x, w3.y}};
-    const unsigned hi[3][2] = {{w3.z, w3.w}, {w4.x, w4.y}, {w4.z, w4.w}};
-    double P[3], S[3], G[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        P[a] = (double)bits_to_float(pw[a]);
-        S[a] = ldexp(1.0, (int)(signed char)((w0.w >> (8 * a)) & 0xffu));
-        G[a] = (fabs(P[a]) + 256.0 * S[a]) * 0x1p-50 + 0x1p-1000;
-    }
-    unsigned acc = 0;
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const unsigned meta = byte_of(s < 4 ? w1.z : w1.w, s & 3);
-        if (meta == 0) continue;
-        double tn = 0.0, tf = (double)tmax;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const double l = (P[a] + (double)byte_of(lo[a][s >> 2], s & 3) * S[a]) - G[a];
-            const double h = (P[a] + (double)byte_of(hi[a][s >> 2], s & 3) * S[a]) + G[a];
-            const double t1 = (l - r.o[a]) * r.inv[a];
-            const double t2 = (h - r.o[a]) * r.inv[a];
-            const bool neg = r.inv[a] < 0.0;
-            tn = fmax(tn, neg ? t2 : t1);
-            tf = fmin(tf, neg ? t1 : t2);
-        }
-        if (tn <= tf * (1.0 + 0x1p-40)) acc |= 1u << s;
-    }
-    return acc;
+    const SlabAxis ax = slab_axis_widened(bits_to_float(w0.x), (int)(signed char)(w0.w & 0xff), r.ox, r.ix, r.ex);
+    const SlabAxis ay =
+        slab_axis_widened(bits_to_float(w0.y), (int)(signed char)((w0.w >> 8) & 0xff), r.oy, r.iy, r.ey);
+    const SlabAxis az =
+        slab_axis_widened(bits_to_float(w0.z), (int)(signed char)((w0.w >> 16) & 0xff), r.oz, r.iz, r.ez);
+    return accept_slots(ax, ay, az, r.ix < 0.0f, r.iy < 0.0f, r.iz < 0.0f, w1, w2, w3, w4, tmax);
 }
 
 // Tests the eight children of one node.  Returns the permuted internal-hit
@@ -529,13 +526,15 @@ RISE_HD inline float box_limit(const RayStateT<A>& r, double best)
     return (float)s * RISE_SLAB_PAD;
 }
 
-// box_limit for an object-space ray: t is the distance along the unit dn'.
+// box_limit for an object-space ray: its slab test runs along the scaled
+// float direction, whose length differs from |dn'| = 1 by a relative 2^-24
+// (covered by RISE_SLAB_PAD, applied as for the world ray).
 template <int A>
 RISE_HD inline float box_limit(const ObjRayT<A>& r, double best)
 {
-    (void)r;
     if (!(best < kRiseInfinity)) return INFINITY;
-    return (float)best * RISE_SLAB_PAD;
+    const double s = best / r.len_s;
+    return (float)s * RISE_SLAB_PAD;
 }
 
 template <int STACK>

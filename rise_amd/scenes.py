@@ -137,17 +137,17 @@ def random_object_matrix(g: np.random.Generator, mirror: bool = False, spread: f
 
 
 def instanced_scene(n_meshes: int = 3, n_objects: int = 12, tris_per_mesh: int = 2000, seed: int = 31,
-                    n_identity: int = 2):
+                    n_identity: int = 2, sigma: float = 0.04):
     """A scene of objects: n_meshes seeded soups in their own frame (centroids
-    in [-1, 1]^3, edges N(0, 0.08^2)), placed by n_objects objects that share
-    them; the first n_identity objects keep identity matrices, one object is
-    mirrored.  Returns (meshes, objects) with objects an OBJECT_DTYPE array."""
+    in [-1, 1]^3, edges N(0, (2 sigma)^2)), placed by n_objects objects that
+    share them; the first n_identity objects keep identity matrices, one object
+    is mirrored.  Returns (meshes, objects) with objects an OBJECT_DTYPE array."""
     from .raycast import make_objects, object_matrices
 
     g = _rng(seed)
     meshes = []
     for m in range(n_meshes):
-        t = soup_triangles(tris_per_mesh, seed=seed * 100 + m, sigma=0.08).astype(np.float64)
+        t = soup_triangles(tris_per_mesh, seed=seed * 100 + m, sigma=sigma).astype(np.float64)
         meshes.append(np.ascontiguousarray((t * 2.0 - 1.0).astype(np.float32)))
     entries = []
     for k in range(n_objects):
@@ -169,22 +169,29 @@ def object_rays(meshes, objects, n: int, seed: int, spread: float = 6.0) -> np.n
     g = _rng(seed)
     o = g.uniform(-spread, spread, (n, 3))
     d = g.normal(size=(n, 3))
-    aimed = g.random(n) < 0.75
-    for i in np.nonzero(aimed)[0]:
-        ob = objects[int(g.integers(0, len(objects)))]
+    aimed = np.nonzero(g.random(n) < 0.75)[0]
+    k = aimed.size
+    which = g.integers(0, len(objects), k)
+    pick = g.random(k)
+    u, v = g.random(k), g.random(k)
+    flip = u + v > 1.0
+    u, v = np.where(flip, 1.0 - u, u), np.where(flip, 1.0 - v, v)
+    along, off, edge = g.random(k), (g.random(k) * 2.0 - 1.0) * 3e-5, g.integers(0, 3, k)
+    eu = np.where(edge == 0, off, np.where(edge == 1, along, along - off))
+    ev = np.where(edge == 0, along, np.where(edge == 1, off, 1.0 - along))
+    near = pick < 0.25
+    u, v = np.where(near, eu, u), np.where(near, ev, v)
+    tri_pick = g.random(k)
+    for ob_i in range(len(objects)):
+        sel = which == ob_i
+        if not sel.any():
+            continue
+        ob = objects[ob_i]
         t = np.asarray(meshes[int(ob["mesh"])], np.float64).reshape(-1, 9)
-        tri = t[int(g.integers(0, t.shape[0]))]
-        if g.random() < 0.25:
-            along, off = g.random(), (g.random() * 2.0 - 1.0) * 3e-5
-            u, v = [(off, along), (along, off), (along - off, 1.0 - along)][int(g.integers(0, 3))]
-        else:
-            u, v = g.random(), g.random()
-            if u + v > 1.0:
-                u, v = 1.0 - u, 1.0 - v
-        p = tri[0:3] + u * (tri[3:6] - tri[0:3]) + v * (tri[6:9] - tri[0:3])
+        tri = t[np.minimum((tri_pick[sel] * t.shape[0]).astype(np.int64), t.shape[0] - 1)]
+        p = tri[:, 0:3] + u[sel, None] * (tri[:, 3:6] - tri[:, 0:3]) + v[sel, None] * (tri[:, 6:9] - tri[:, 0:3])
         m = np.asarray(ob["to_world"], np.float64).reshape(4, 4)
-        pw = p @ m[:3, :3] + m[3, :3]
-        d[i] = pw - o[i]
+        d[aimed[sel]] = p @ m[:3, :3] + m[3, :3] - o[aimed[sel]]
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     d *= g.uniform(0.5, 2.0, (n, 1))
     return rays_from_rows(o, d)

@@ -19,7 +19,7 @@ _IntersectionOnly (Objects/Object.cpp:392-712) inside ObjectManager's loop
 * The library's per-ray code (object_trace.h, run on the host) against the
   oracle; identity objects against the plain world-space path.
 * Closed forms: a quad placed by a known transform.
-* The double slab test of object-space rays against exact rational slabs.
+* The slab test of object-space rays against exact rational slabs.
 Bar: bit-exact (doubles compared as doubles; the library's float fields as
 the oracle's doubles rounded to float)."""
 import ctypes
@@ -123,6 +123,20 @@ def test_host_objects_match_oracle(arith, unit):
             assert not hit.any() and not occ.any()
 
 
+def test_coincident_objects_keep_the_first():
+    """Two objects with the same mesh and the same matrices give equal world
+    ranges; the reference keeps the first in its order (strictly closer
+    wins), whatever order the top-level tree visits them in."""
+    meshes, objs, rays = _scene(seed=13, n_rays=3000)
+    dup = np.concatenate([objs[[5]], objs[2:5], objs[[5]], objs[[9]], objs[[9]]])
+    for arith in ("ieee", "fast"):
+        o = oracle.objects_closest(meshes, dup, rays, arith=arith)
+        h = raycast.host_trace_objects(meshes, dup, rays, arith=ARITH[arith])
+        assert _same(h["prim_id"], o["prim_id"]) and _same(h["mesh"], o["object"])
+        assert (o["object"] == 0).sum() > 100 and (o["object"] == 5).sum() > 100
+        assert not ((o["object"] == 4) | (o["object"] == 6)).any()
+
+
 def test_identity_objects_equal_world_space_path():
     meshes, _, rays = _scene(seed=9)
     objs = raycast.make_objects([(k, *raycast.object_matrices()) for k in range(len(meshes))])
@@ -190,7 +204,7 @@ def test_object_arguments_are_validated():
 
 
 # ---------------------------------------------------------------------------
-# The double slab test of object-space rays.
+# The slab test of object-space rays.
 
 def _obj_ray(ray, to_object):
     """o' and dn' exactly as the library and the reference compute them
@@ -234,10 +248,10 @@ def _exact_hit(op, dn, lo, hi, best):
 
 @pytest.mark.parametrize("seed", range(3))
 def test_object_slab_test_is_conservative(seed):
-    """Boxes the exact object-space ray (o', dn') meets within `best` are
-    taken; boxes missed by more than 1e-9 of the coordinates, or entered
-    beyond best (1 + 1e-5) (the limit is widened by 2^-19 as a float), are
-    not."""
+    """The FP32 slab test of the float object-space ray, widened by the
+    origin's rounding: boxes the exact object-space ray (o', dn') meets within
+    `best` are taken; boxes missed by more than 1e-5 of the coordinates and
+    the origin's size, or entered beyond best (1 + 1e-5), are not."""
     g = np.random.default_rng(500 + seed)
     must = taken = far = wrongly = 0
     for _ in range(150):
@@ -285,7 +299,7 @@ def test_object_slab_test_is_conservative(seed):
                     must += 1
                     taken += (got >> s) & 1
                 else:
-                    gpad = F(1, 10 ** 9) * (max(abs(x) for x in bl + bh) + max(abs(F(x)) for x in op)) + F(1, 10 ** 30)
+                    gpad = F(1, 10 ** 5) * (max(abs(x) for x in bl + bh) + max(abs(F(x)) for x in op)) + F(1, 10 ** 30)
                     if not _exact_hit(op, dn, [x - gpad for x in bl], [x + gpad for x in bh],
                                       best * (1 + 1e-5) if best < 1e308 else 1.8e308):
                         far += 1
