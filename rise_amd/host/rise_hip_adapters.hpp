@@ -52,18 +52,40 @@ public:
     HipObjectManager& operator=(const HipObjectManager&) = delete;
     ~HipObjectManager() { rise_hip_scene_release(scene_); }
 
-    // Adds a world-space mesh (n_tris * 9 floats); copied at PrepareForRendering.
-    void AddMesh(const float* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr)
+    // Adds a mesh (n_tris * 9 floats); copied at PrepareForRendering.  Without
+    // objects it is world-space geometry; with AddObject it is placed by the
+    // objects that refer to it.  Returns its index.
+    int AddMesh(const float* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr)
     {
         meshes_.push_back({vertices, n_tris, tri_ids});
+        return (int)meshes_.size() - 1;
+    }
+
+    // Adds an object: mesh `mesh` placed by the object's Matrix4s
+    // (m_mxFinalTrans, m_mxInvFinalTrans; 16 doubles each in member order).
+    // Returns its index, which hit records report in rise_hip_hit::mesh.
+    int AddObject(int mesh, const double to_world[16], const double to_object[16])
+    {
+        rise_hip_object_instance o{};
+        o.mesh = mesh;
+        for (int k = 0; k < 16; ++k) {
+            o.to_world[k] = to_world[k];
+            o.to_object[k] = to_object[k];
+        }
+        objects_.push_back(o);
+        return (int)objects_.size() - 1;
     }
 
     // ObjectManager::PrepareForRendering (Managers/ObjectManager.cpp:404-423):
-    // builds the acceleration structure and uploads it.
+    // builds the acceleration structures and uploads them.
     bool PrepareForRendering()
     {
         rise_hip_scene_release(scene_);
         scene_ = nullptr;
+        if (!objects_.empty())
+            return check(rise_hip_scene_upload_objects(meshes_.data(), meshes_.size(), objects_.data(),
+                                                       objects_.size(), &scene_),
+                         "PrepareForRendering");
         return check(rise_hip_scene_upload(meshes_.data(), meshes_.size(), &scene_), "PrepareForRendering");
     }
 
@@ -118,6 +140,7 @@ public:
 
 private:
     std::vector<rise_hip_mesh_desc> meshes_;
+    std::vector<rise_hip_object_instance> objects_;
     rise_hip_scene* scene_ = nullptr;
 };
 
