@@ -57,13 +57,21 @@ const char* rise_hip_version(void);
 typedef struct rise_hip_scene rise_hip_scene;
 
 /* One triangle mesh in world space (identity object transform).
- * vertices: host, n_tris * 9 floats (v0.xyz, v1.xyz, v2.xyz per triangle).
- * tri_ids:  host, optional n_tris ids reported for hits (NULL: the running
- *           index over all meshes of the scene, starting at 0). */
+ * vertices:     host, n_tris * 9 floats (v0.xyz, v1.xyz, v2.xyz per triangle).
+ * n_tris:       triangle count.
+ * tri_ids:      host, optional n_tris ids reported for hits (NULL: the running
+ *               index over all meshes of the scene, starting at 0).
+ * vertices_f64: host, optional n_tris * 9 doubles, the same layout: the
+ *               mesh's vertices as RISE holds them (Point3, double).  When
+ *               given, `vertices` may be NULL: the double certifier and the
+ *               face test use these, the float pre-filter and the tree their
+ *               float rounding, as the reference's mesh does
+ *               (...Specializations.h:32-47, :141-142). */
 typedef struct rise_hip_mesh_desc {
     const float* vertices;
     int64_t n_tris;
     const int32_t* tri_ids;
+    const double* vertices_f64;
 } rise_hip_mesh_desc;
 
 typedef struct rise_hip_scene_stats {

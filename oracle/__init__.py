@@ -111,7 +111,20 @@ def closest(tris: np.ndarray, rays: np.ndarray, threads: int = 0, flags: bool = 
 
 def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, threads: int = 0,
             no_filter: bool = False, arith: str = "ieee", faces: int = 3) -> np.ndarray:
+    """Brute-force any hit; float64 triangles are double vertices
+    (rise_oracle_any_brute_f64)."""
     L = oracle_lib()
+    if np.asarray(tris).dtype == np.float64:
+        L.rise_oracle_any_brute_f64.argtypes = [_P, _I64, _P, _P, _I64, _P, ctypes.c_int, ctypes.c_int]
+        t64 = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[1]
+        occ = np.empty(n, dtype=np.uint8)
+        tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
+        L.rise_oracle_any_brute_f64(t64.ctypes.data, t64.shape[0], rays.ctypes.data,
+                                    None if tm is None else tm.ctypes.data, n, occ.ctypes.data,
+                                    _opts(no_filter, arith) | _faces_opts(faces), threads)
+        return occ
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
@@ -126,8 +139,21 @@ def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, 
 def closest_attr(tris: np.ndarray, rays: np.ndarray, faces: int = 3, arith: str = "ieee", threads: int = 0) -> dict:
     """Brute-force closest hit with the intersection record
     (rise_oracle_closest_attr): prim_id, range, alpha, beta, normal (n, 3),
-    point (n, 3), all float64 except the ids."""
+    point (n, 3), all float64 except the ids.  float64 triangles are double
+    vertices (rise_oracle_closest_attr_f64)."""
     L = oracle_lib()
+    if np.asarray(tris).dtype == np.float64:
+        L.rise_oracle_closest_attr_f64.argtypes = [_P, _I64, _P, _I64] + [_P] * 6 + [ctypes.c_int, ctypes.c_int]
+        t64 = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[1]
+        out = {"prim_id": np.empty(n, np.int32), "range": np.empty(n), "alpha": np.empty(n), "beta": np.empty(n),
+               "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
+        L.rise_oracle_closest_attr_f64(t64.ctypes.data, t64.shape[0], rays.ctypes.data, n,
+                                       *[out[k].ctypes.data for k in ("prim_id", "range", "alpha", "beta", "normal",
+                                                                      "point")],
+                                       _opts(False, arith) | _faces_opts(faces), threads)
+        return out
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
@@ -153,10 +179,12 @@ def _objects(objects) -> np.ndarray:
 
 
 def _meshes(meshes):
-    arrs = [np.ascontiguousarray(m, dtype=np.float32).reshape(-1, 9) for m in meshes]
+    """All meshes' vertices as doubles (float meshes widened exactly) and the
+    mesh offsets."""
+    arrs = [np.ascontiguousarray(np.asarray(m).reshape(-1, 9).astype(np.float64)) for m in meshes]
     off = np.zeros(len(arrs) + 1, np.int64)
     off[1:] = np.cumsum([a.shape[0] for a in arrs])
-    tris = np.ascontiguousarray(np.concatenate(arrs) if arrs else np.zeros((0, 9), np.float32))
+    tris = np.ascontiguousarray(np.concatenate(arrs) if arrs else np.zeros((0, 9), np.float64))
     return tris, off
 
 
@@ -219,7 +247,8 @@ class RefMesh:
 
     def __init__(self, tris: np.ndarray | None, variant: str = "ieee", risemesh: str | None = None,
                  bary: bool = False):
-        """Builds the mesh from `tris`, or (tris None) loads the .risemesh file
+        """Builds the mesh from `tris` (float32, or float64 kept as double
+        vertices), or (tris None) loads the .risemesh file
         `risemesh` with the reference's Deserialize.  bary: vertex normals
         that make the interpolated shading normal carry the barycentrics
         (rise_ref_mesh_create_bary)."""
@@ -241,9 +270,14 @@ class RefMesh:
         L.rise_ref_closest_ex.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int] + [_P] * 6 + [ctypes.c_int]
         L.rise_ref_any_ex.argtypes = [_P, _P, _P, _I64, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int]
         self._L = L
+        L.rise_ref_mesh_create_f64.argtypes = [_P, _I64, ctypes.c_int]
+        L.rise_ref_mesh_create_f64.restype = _P
         if tris is None:
             self._tris = None
             self._h = L.rise_ref_risemesh_load(str(risemesh).encode())
+        elif np.asarray(tris).dtype == np.float64:
+            self._tris = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+            self._h = L.rise_ref_mesh_create_f64(self._tris.ctypes.data, self._tris.shape[0], int(bary))
         else:
             self._tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
             create = L.rise_ref_mesh_create_bary if bary else L.rise_ref_mesh_create

@@ -178,6 +178,34 @@ void* rise_ref_mesh_create(const float* tris, int64_t n_tris)
     return h;
 }
 
+// rise_ref_mesh_create / _create_bary on double vertices (9 doubles per
+// triangle), kept as the mesh's Point3 vertices.
+void* rise_ref_mesh_create_f64(const double* tris, int64_t n_tris, int bary)
+{
+    RefMesh* h = new RefMesh;
+    h->mesh = new Implementation::TriangleMeshGeometryIndexed(false, !bary);
+    h->mesh->BeginIndexedTriangles();
+    if (bary) {
+        h->mesh->AddNormal(Normal(1.0, 0.0, 0.0));
+        h->mesh->AddNormal(Normal(0.0, 1.0, 0.0));
+        h->mesh->AddNormal(Normal(0.0, 0.0, 1.0));
+    }
+    for (int64_t i = 0; i < n_tris; ++i) {
+        const double* f = tris + 9 * i;
+        for (int k = 0; k < 3; ++k) h->mesh->AddVertex(Vertex(f[3 * k + 0], f[3 * k + 1], f[3 * k + 2]));
+        h->mesh->AddTexCoord(TexCoord((double)i, 0.0));
+        IndexedTriangle t;
+        for (int k = 0; k < 3; ++k) {
+            t.iVertices[k] = (unsigned int)(3 * i + k);
+            t.iNormals[k] = bary ? (unsigned int)k : 0u;
+            t.iCoords[k] = (unsigned int)i;
+        }
+        h->mesh->AddIndexedTriangle(t);
+    }
+    h->mesh->DoneIndexedTriangles();
+    return h;
+}
+
 void rise_ref_mesh_release(void* handle)
 {
     RefMesh* h = (RefMesh*)handle;

@@ -39,6 +39,26 @@ static inline float d332(int x, float a0, float b0, float a1, float b1, float a2
     return x ? fmaf(a2, b2, fmaf(a0, b0, a1 * b1)) : a0 * b0 + a1 * b1 + a2 * b2;
 }
 
+/* The face normal eA x eB of RayElementIntersection
+ * (TriangleMeshGeometryIndexedSpecializations.h:141-159, :196-205).  The
+ * Linux build contracts its components differently from the certifier's
+ * cross product (read from the compiled code): x and y as
+ * fma(-(a.z), b.y, a.y*b.z) and fma(-(a.x), b.z, a.z*b.x), z as
+ * fma(a.x, b.y, -(a.y*b.x)).  Float vertices make every product exact, so
+ * only double vertices tell the forms apart. */
+static void face_cross(int x, const double a[3], const double b[3], double n[3])
+{
+    if (x) {
+        n[0] = fma(-a[2], b[1], a[1] * b[2]);
+        n[1] = fma(-a[0], b[2], a[2] * b[0]);
+        n[2] = fma(a[0], b[1], -(a[1] * b[0]));
+    } else {
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+    }
+}
+
 /* Vector3Ops::Normalize (Utilities/Math3D/VectorsOps.h:229-238):
  * mag = sqrt(x*x + y*y + z*z); if mag > 0, v * (1/mag). */
 static void normalize_x(int x, const double v[3], double out[3]);
@@ -169,6 +189,29 @@ static tri64* expand_tris(const float* tris, int64_t n)
     return out;
 }
 
+/* The same from double vertices, as a mesh that holds them does: the
+ * certifier's v0 and edges in double from the doubles, the filter's record
+ * from their float casts (GetFloatTriangleVertices,
+ * TriangleMeshGeometryIndexedSpecializations.h:32-47). */
+static tri64* expand_tris_f64(const double* tris, int64_t n)
+{
+    tri64* out = (tri64*)malloc(sizeof(tri64) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        const double* f = tris + 9 * i;
+        for (int k = 0; k < 3; ++k) {
+            const double a = f[k], b = f[3 + k], c = f[6 + k];
+            const float af = (float)a, bf = (float)b, cf = (float)c;
+            out[i].v0[k] = a;
+            out[i].eA[k] = b - a;
+            out[i].eB[k] = c - a;
+            out[i].p0[k] = af;
+            out[i].e1[k] = bf - af;
+            out[i].e2[k] = cf - af;
+        }
+    }
+    return out;
+}
+
 typedef struct {
     const tri64* tris;
     int64_t n_tris;
@@ -224,10 +267,9 @@ static void float_ray(const double o[3], const double dn[3], float fo[3], float 
 static int face_selected(int x, int opts, const tri64* t, const double dn[3])
 {
     if (!(opts & (RISE_ORACLE_NO_FRONT | RISE_ORACLE_NO_BACK))) return 1;
-    const double n0 = sp64(x, t->eA[1], t->eB[2], t->eA[2], t->eB[1]);
-    const double n1 = sp64(x, t->eA[2], t->eB[0], t->eA[0], t->eB[2]);
-    const double n2 = sp64(x, t->eA[0], t->eB[1], t->eA[1], t->eB[0]);
-    const double dd = d364(x, n0, dn[0], n1, dn[1], n2, dn[2]);
+    double n[3];
+    face_cross(x, t->eA, t->eB, n);
+    const double dd = d364(x, n[0], dn[0], n[1], dn[1], n[2], dn[2]);
     if ((opts & RISE_ORACLE_NO_FRONT) && dd < 0.0) return 0;
     if ((opts & RISE_ORACLE_NO_BACK) && dd > 0.0) return 0;
     return 1;
@@ -298,9 +340,8 @@ static void* closest_worker(void* arg)
                     ray_triangle_x(x, o, dn, w->v0, w->eA, w->eB, &tt, &a, &b);
                     j->alpha[i] = a;
                     j->beta[i] = b;
-                    const double fn[3] = {sp64(x, w->eA[1], w->eB[2], w->eA[2], w->eB[1]),
-                                          sp64(x, w->eA[2], w->eB[0], w->eA[0], w->eB[2]),
-                                          sp64(x, w->eA[0], w->eB[1], w->eA[1], w->eB[0])};
+                    double fn[3];
+                    face_cross(x, w->eA, w->eB, fn);
                     double n1[3];
                     normalize_x(x, fn, n1);
                     normalize_x(x, n1, j->normal + 3 * i);
@@ -409,6 +450,45 @@ void rise_oracle_any_brute(const float* tris, int64_t n_tris,
     job j;
     memset(&j, 0, sizeof(j));
     tri64* tt = expand_tris(tris, n_tris);
+    j.tris = tt;
+    j.n_tris = n_tris;
+    j.rays = rays;
+    j.tmax = tmax;
+    j.n_rays = n_rays;
+    j.occluded = occluded;
+    j.opts = opts;
+    run(&j, any_worker, n_threads);
+    free(tt);
+}
+
+void rise_oracle_closest_attr_f64(const double* tris, int64_t n_tris, const float* rays, int64_t n_rays,
+                                  int32_t* prim_id, double* range, double* alpha, double* beta, double* normal,
+                                  double* point, int opts, int n_threads)
+{
+    job j;
+    memset(&j, 0, sizeof(j));
+    tri64* tt = expand_tris_f64(tris, n_tris);
+    j.tris = tt;
+    j.n_tris = n_tris;
+    j.rays = rays;
+    j.n_rays = n_rays;
+    j.prim_id = prim_id;
+    j.t64 = range;
+    j.alpha = alpha;
+    j.beta = beta;
+    j.normal = normal;
+    j.point = point;
+    j.opts = opts;
+    run(&j, closest_worker, n_threads);
+    free(tt);
+}
+
+void rise_oracle_any_brute_f64(const double* tris, int64_t n_tris, const float* rays, const float* tmax,
+                               int64_t n_rays, uint8_t* occluded, int opts, int n_threads)
+{
+    job j;
+    memset(&j, 0, sizeof(j));
+    tri64* tt = expand_tris_f64(tris, n_tris);
     j.tris = tt;
     j.n_tris = n_tris;
     j.rays = rays;
@@ -627,9 +707,8 @@ static void* objects_closest_worker(void* arg)
                     ray_triangle_x(x, op, dn, t->v0, t->eA, t->eB, &tt, &a, &b);
                     best_a = a;
                     best_b = b;
-                    const double fn[3] = {sp64(x, t->eA[1], t->eB[2], t->eA[2], t->eB[1]),
-                                          sp64(x, t->eA[2], t->eB[0], t->eA[0], t->eB[2]),
-                                          sp64(x, t->eA[0], t->eB[1], t->eA[1], t->eB[0])};
+                    double fn[3];
+                    face_cross(x, t->eA, t->eB, fn);
                     double n1[3], nw[3];
                     normalize_x(x, fn, n1);
                     const double* mi = ob->to_object;
@@ -723,14 +802,14 @@ static void orun(ojob* j, void* (*fn)(void*), int n_threads)
     pthread_mutex_destroy(&j->lock);
 }
 
-void rise_oracle_objects_closest(const float* tris, const int64_t* mesh_off, int n_meshes,
+void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, int n_meshes,
                                  const rise_oracle_object* objs, int n_objs, const float* rays, int64_t n_rays,
                                  int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
                                  double* normal, double* point, uint8_t* flags, int opts, int n_threads)
 {
     ojob j;
     memset(&j, 0, sizeof(j));
-    tri64* tt = expand_tris(tris, mesh_off[n_meshes]);
+    tri64* tt = expand_tris_f64(tris, mesh_off[n_meshes]);
     j.tris = tt;
     j.mesh_off = mesh_off;
     j.objs = objs;
@@ -750,13 +829,13 @@ void rise_oracle_objects_closest(const float* tris, const int64_t* mesh_off, int
     free(tt);
 }
 
-void rise_oracle_objects_any(const float* tris, const int64_t* mesh_off, int n_meshes, const rise_oracle_object* objs,
+void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const rise_oracle_object* objs,
                              int n_objs, const float* rays, const float* tmax, int64_t n_rays, uint8_t* occluded,
                              int opts, int n_threads)
 {
     ojob j;
     memset(&j, 0, sizeof(j));
-    tri64* tt = expand_tris(tris, mesh_off[n_meshes]);
+    tri64* tt = expand_tris_f64(tris, mesh_off[n_meshes]);
     j.tris = tt;
     j.mesh_off = mesh_off;
     j.objs = objs;

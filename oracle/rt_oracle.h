@@ -105,6 +105,16 @@ void rise_oracle_closest_attr(const float* tris, int64_t n_tris, const float* ra
                               int32_t* prim_id, double* range, double* alpha, double* beta, double* normal,
                               double* point, int opts, int n_threads);
 
+/* rise_oracle_closest_attr and rise_oracle_any_brute on double vertices
+ * (n_tris * 9 doubles), as a mesh holding Point3 vertices intersects them:
+ * the certifier and the face test on the doubles, the float filter on their
+ * float casts (...Specializations.h:32-47, :141-142). */
+void rise_oracle_closest_attr_f64(const double* tris, int64_t n_tris, const float* rays, int64_t n_rays,
+                                  int32_t* prim_id, double* range, double* alpha, double* beta, double* normal,
+                                  double* point, int opts, int n_threads);
+void rise_oracle_any_brute_f64(const double* tris, int64_t n_tris, const float* rays, const float* tmax,
+                               int64_t n_rays, uint8_t* occluded, int opts, int n_threads);
+
 /* One object: a mesh placed by Matrix4 transforms (members _00.._33 in order;
  * row-vector convention, translation in _30.._32).  Same layout as the
  * library's rise_hip_object_instance. */
@@ -119,7 +129,8 @@ typedef struct rise_oracle_object {
  * (Managers/ObjectManager.cpp:290-310) with Object::IntersectRay per object
  * (Objects/Object.cpp:414-425 frame change, :531-533 normal, :637-643 world
  * point and range), brute force over each object's mesh.
- * tris: all meshes' triangles (9 floats each), mesh m being
+ * tris: all meshes' triangles (9 doubles each, the vertices as the meshes
+ * hold them; float meshes widened exactly), mesh m being
  * [mesh_off[m], mesh_off[m+1]).  Outputs per ray: prim_id (index within its
  * mesh, -1 on a miss), object (index, -1), range (world distance, +inf);
  * optional (NULL or all given): alpha, beta, normal (3), point (3, world);
@@ -130,7 +141,7 @@ typedef struct rise_oracle_object {
  * (hits within 1e-12 of their box's entry distance aside); for |d| < 1 the
  * running best is scaled by |M^-1 d| = |d| |M^-1 d/|d||, short of its
  * object-space image, and it can drop a closer object. */
-void rise_oracle_objects_closest(const float* tris, const int64_t* mesh_off, int n_meshes,
+void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, int n_meshes,
                                  const rise_oracle_object* objs, int n_objs, const float* rays, int64_t n_rays,
                                  int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
                                  double* normal, double* point, uint8_t* flags, int opts, int n_threads);
@@ -138,7 +149,7 @@ void rise_oracle_objects_closest(const float* tris, const int64_t* mesh_off, int
 /* ObjectManager::IntersectShadowRay's loop (ObjectManager.cpp:320-373) with
  * Object::IntersectRay_IntersectionOnly per object (Object.cpp:652-712):
  * dHowFar scaled by |M^-1 d| per object.  tmax: NULL = RISE_INFINITY. */
-void rise_oracle_objects_any(const float* tris, const int64_t* mesh_off, int n_meshes, const rise_oracle_object* objs,
+void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const rise_oracle_object* objs,
                              int n_objs, const float* rays, const float* tmax, int64_t n_rays, uint8_t* occluded,
                              int opts, int n_threads);
 

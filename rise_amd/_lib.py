@@ -38,6 +38,7 @@ class MeshDesc(ctypes.Structure):
         ("vertices", ctypes.c_void_p),
         ("n_tris", ctypes.c_int64),
         ("tri_ids", ctypes.c_void_p),
+        ("vertices_f64", ctypes.c_void_p),
     ]
 
 

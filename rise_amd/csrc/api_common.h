@@ -6,6 +6,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <string>
+#include <vector>
 
 namespace rise_hip {
 
@@ -21,7 +22,10 @@ int host_threads();
 
 struct WideBVH;
 // Uploads a built wide BVH as a ray-cast scene (rise_hip_scene_upload's
-// device part); bvh_source goes to the scene's statistics.
-rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene);
+// device part); bvh_source goes to the scene's statistics.  tri64: NULL, or
+// the leaf records' double vertices (9 per record, leaf order), which the
+// certifier then uses.
+rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene,
+                             const std::vector<double>* tri64 = nullptr);
 
 }  // namespace rise_hip

@@ -164,9 +164,10 @@ extern "C" rise_hip_status rise_hip_debug_objects_host(const rise_hip_mesh_desc*
     const rise_hip_status st = build_object_scene(meshes, n_meshes, objects, n_objects, host);
     if (st != RISE_HIP_OK) return st;
     std::vector<BlasView> views;
-    for (const WideBVH& b : host.blas) views.push_back(BlasView{b.nodes.data(), b.tris.data()});
+    for (size_t k = 0; k < host.blas.size(); ++k)
+        views.push_back(BlasView{host.blas[k].nodes.data(), host.blas[k].tris.data(), host.blas_tri64[k].data()});
     const ObjectSceneView sc{host.world_tris > 0 ? host.world.nodes.data() : nullptr, host.world.tris.data(),
-                             host.objects.data(), (int32_t)host.objects.size(), views.data(),
+                             host.world_tri64.data(), host.objects.data(), (int32_t)host.objects.size(), views.data(),
                              host.top.nodes.data(), host.top.tris.data()};
     std::atomic<int64_t> next(0);
     std::atomic<bool> overflow(false);

@@ -53,13 +53,12 @@ double transform_slack(const double* m, const double* mi)
 // result widened again by the larger of 1e-7 and the transform's slack.  It
 // contains every world point the reference can report for the object, and
 // every point of the exact world ray that passes within rounding of one.
-void world_box(const float* verts, int64_t n_tris, const double* to_world, double slack, double lo[3],
-               double hi[3])
+void world_box(const std::vector<double>& v64, const double* to_world, double slack, double lo[3], double hi[3])
 {
     double olo[3] = {INFINITY, INFINITY, INFINITY}, ohi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t i = 0; i < n_tris * 3; ++i)
+    for (size_t i = 0; i < v64.size() / 3; ++i)
         for (int a = 0; a < 3; ++a) {
-            const double v = verts[3 * i + a];
+            const double v = v64[3 * i + a];
             olo[a] = std::min(olo[a], v);
             ohi[a] = std::max(ohi[a], v);
         }
@@ -92,6 +91,34 @@ void world_box(const float* verts, int64_t n_tris, const double* to_world, doubl
 
 }  // namespace
 
+void mesh_vertices(const rise_hip_mesh_desc& m, std::vector<float>& f, std::vector<double>& d)
+{
+    const size_t n = (size_t)m.n_tris * 9;
+    const size_t f0 = f.size(), d0 = d.size();
+    f.resize(f0 + n);
+    d.resize(d0 + n);
+    for (size_t i = 0; i < n; ++i) {
+        if (m.vertices_f64) {
+            d[d0 + i] = m.vertices_f64[i];
+            f[f0 + i] = (float)m.vertices_f64[i];
+        } else {
+            f[f0 + i] = m.vertices[i];
+            d[d0 + i] = m.vertices[i];
+        }
+    }
+}
+
+std::vector<double> leaf_vertices(const WideBVH& bvh, const std::vector<double>& v64)
+{
+    std::vector<double> out(bvh.tris.size() * 9);
+    for (size_t i = 0; i < bvh.tris.size(); ++i) {
+        int32_t p;
+        std::memcpy(&p, &bvh.tris[i].v0[3], 4);
+        for (int k = 0; k < 9; ++k) out[9 * i + k] = v64[9 * (size_t)p + k];
+    }
+    return out;
+}
+
 rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_meshes,
                                    const rise_hip_object_instance* objects, size_t n_objects, ObjectSceneHost& out)
 {
@@ -101,7 +128,7 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
     for (size_t m = 0; m < n_meshes; ++m) {
         if (meshes[m].n_tris < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative triangle count");
         if (meshes[m].n_tris > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh over 2^31-1 triangles");
-        if (meshes[m].n_tris > 0 && !meshes[m].vertices)
+        if (meshes[m].n_tris > 0 && !meshes[m].vertices && !meshes[m].vertices_f64)
             return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh vertices is NULL");
     }
     int64_t world_total = 0;
@@ -121,7 +148,8 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
     try {
         // Identity objects: one world-space tree built over the running index,
         // then each leaf triangle takes its caller id (v0.w) and its object (v1.w).
-        std::vector<float> verts((size_t)world_total * 9);
+        std::vector<float> verts;
+        std::vector<double> verts64;
         std::vector<int32_t> ids((size_t)world_total), obj_of((size_t)world_total);
         int64_t off = 0;
         for (size_t k = 0; k < n_objects; ++k) {
@@ -129,7 +157,7 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             if (!(is_identity(ob.to_world) && is_identity(ob.to_object))) continue;
             const rise_hip_mesh_desc& m = meshes[ob.mesh];
             if (m.n_tris == 0) continue;
-            std::memcpy(verts.data() + off * 9, m.vertices, (size_t)m.n_tris * 9 * sizeof(float));
+            mesh_vertices(m, verts, verts64);
             for (int64_t i = 0; i < m.n_tris; ++i) {
                 ids[(size_t)(off + i)] = m.tri_ids ? m.tri_ids[i] : (int32_t)i;
                 obj_of[(size_t)(off + i)] = (int32_t)k;
@@ -139,6 +167,7 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
         out.world_tris = world_total;
         if (!build_wide_bvh(verts.data(), world_total, nullptr, BuildParams(), out.world, err))
             return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+        out.world_tri64 = leaf_vertices(out.world, verts64);
         for (Tri48& tr : out.world.tris) {
             int32_t p;
             std::memcpy(&p, &tr.v0[3], 4);
@@ -155,9 +184,19 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             if (m.n_tris == 0) continue;  // nothing to hit
             if (blas_of[ob.mesh] < 0) {
                 WideBVH b;
-                if (!build_wide_bvh(m.vertices, m.n_tris, m.tri_ids, BuildParams(), b, err))
+                std::vector<float> mv;
+                std::vector<double> mv64;
+                mesh_vertices(m, mv, mv64);
+                if (!build_wide_bvh(mv.data(), m.n_tris, nullptr, BuildParams(), b, err))
                     return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
-                for (Tri48& tr : b.tris) std::memcpy(&tr.v1[3], &ob.mesh, 4);
+                out.blas_tri64.push_back(leaf_vertices(b, mv64));
+                for (Tri48& tr : b.tris) {
+                    int32_t p;
+                    std::memcpy(&p, &tr.v0[3], 4);
+                    const int32_t id = m.tri_ids ? m.tri_ids[p] : p;
+                    std::memcpy(&tr.v0[3], &id, 4);
+                    std::memcpy(&tr.v1[3], &ob.mesh, 4);
+                }
                 blas_of[ob.mesh] = (int32_t)out.blas.size();
                 out.total_tris += m.n_tris;
                 out.blas.push_back(std::move(b));
@@ -166,7 +205,7 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             std::memcpy(d.to_world, ob.to_world, sizeof(d.to_world));
             std::memcpy(d.to_object, ob.to_object, sizeof(d.to_object));
             d.slack = transform_slack(ob.to_world, ob.to_object);
-            world_box(m.vertices, m.n_tris, ob.to_world, d.slack, d.lo, d.hi);
+            world_box(out.blas_tri64[(size_t)blas_of[ob.mesh]], ob.to_world, d.slack, d.lo, d.hi);
             d.blas = blas_of[ob.mesh];
             d.object = (int32_t)k;
             out.objects.push_back(d);

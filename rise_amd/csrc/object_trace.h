@@ -7,7 +7,10 @@
 // distance from the world origin decides, strictly smaller winning, which
 // object's hit is kept (Managers/ObjectManager.cpp:68-99, :290-310).
 //
-// Here objects whose two matrices are both the identity are merged into one
+// Every triangle of an object scene keeps its double vertices beside its leaf
+// record (the certifier and the face test read them; the float filter and the
+// boxes use their float rounding, as the reference's BVH filter does).
+// Objects whose two matrices are both the identity are merged into one
 // world-space tree (their object space is world space, and for them the
 // reference's transforms are exact), and every other object keeps an
 // object-space tree of its mesh, shared by all objects on that mesh.  A query
@@ -173,7 +176,7 @@ RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6
         ClosestState s;
         closest_begin(r, s);
         s.faces = faces;
-        while (closest_step<false, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt)) {
+        while (closest_step<false, true, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt, sc.world_tri64)) {
         }
         if (s.best_id >= 0) {
             const float4* tp = reinterpret_cast<const float4*>(sc.world_tris + s.best_leaf);
@@ -210,7 +213,7 @@ RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6
         }
         const int floor = st.base;
         st.base = st.sp;
-        while (closest_step<false, true>(q, bl.nodes, bl.tris, st, s, nn, nt)) {
+        while (closest_step<false, true, true>(q, bl.nodes, bl.tris, st, s, nn, nt, bl.tri64)) {
         }
         st.base = floor;
         if (s.best_id < 0) return limit();
@@ -250,7 +253,7 @@ RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6]
         RayStateT<A> r;
         init_ray(r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
         double ab[2], n[3], p[3];
-        hit_attributes(r, sc.world_tris + h.leaf, h.t, ab, n, p);
+        hit_attributes(r, sc.world_tris + h.leaf, h.t, ab, n, p, sc.world_tri64 + 9 * (size_t)h.leaf);
         out.alpha = ab[0];
         out.beta = ab[1];
         for (int k = 0; k < 3; ++k) {
@@ -264,16 +267,10 @@ RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6]
     const double dw[3] = {ray[3], ray[4], ray[5]};
     ObjRayT<A> q;
     init_obj_ray(q, ow, dw, ob.to_object);
-    const float4* tp = reinterpret_cast<const float4*>(sc.blas[ob.blas].tris + h.leaf);
-    const float4 a = tp[0], b = tp[1], c = tp[2];
-    double tt = 0.0, ab[2] = {0.0, 0.0};
-    (void)certify(q, a, b, c, tt, ab);
-    const double eAx = (double)b.x - a.x, eAy = (double)b.y - a.y, eAz = (double)b.z - a.z;
-    const double eBx = (double)c.x - a.x, eBy = (double)c.y - a.y, eBz = (double)c.z - a.z;
-    const double fn[3] = {sub_prod<A>(eAy, eBz, eAz, eBy), sub_prod<A>(eAz, eBx, eAx, eBz),
-                          sub_prod<A>(eAx, eBy, eAy, eBx)};
-    double n1[3], nw[3], n2[3];
-    normalize_model<A>(fn, n1);
+    const BlasView bl = sc.blas[ob.blas];
+    double v[9], ab[2], n1[3], nw[3], n2[3];
+    triangle_vertices<ObjRayT<A>>(bl.tris + h.leaf, bl.tri64 + 9 * (size_t)h.leaf, v);
+    hit_face(q, v, ab, n1);
     xform_normal<A>(ob.to_object, n1, nw);
     normalize_model<A>(nw, n2);
     out.alpha = ab[0];
@@ -302,7 +299,7 @@ RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], d
         AnyState s;
         any_begin(r, scale_how_far(r, how_far), s);
         s.faces = faces;
-        while (any_step<false, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt)) {
+        while (any_step<false, true, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt, sc.world_tri64)) {
         }
         if (s.hit) return true;
     }
@@ -329,7 +326,7 @@ RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], d
         s.faces = faces;
         const int floor = st.base;
         st.base = st.sp;
-        while (any_step<false, true>(q, bl.nodes, bl.tris, st, s, nn, nt)) {
+        while (any_step<false, true, true>(q, bl.nodes, bl.tris, st, s, nn, nt, bl.tri64)) {
         }
         st.sp = st.base;  // an early exit leaves the object's entries
         st.base = floor;

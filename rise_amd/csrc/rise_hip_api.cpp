@@ -27,6 +27,7 @@ using namespace rise_hip;
 struct rise_hip_scene {
     NodeSlot* d_nodes = nullptr;
     Tri48* d_tris = nullptr;
+    double* d_tri64 = nullptr;  // leaf records' double vertices (9 each), or NULL: the float vertices are exact
     // Dynamic-fetch counters: each query takes the next slot, so queries on
     // different streams do not share a counter (up to kFetchSlots in flight).
     static constexpr int kFetchSlots = 64;
@@ -45,7 +46,7 @@ struct rise_hip_scene {
     bool world_empty = false;
     ObjectSceneView objects_view() const
     {
-        return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_objects, n_objects, d_blas,
+        return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_tri64, d_objects, n_objects, d_blas,
                                d_top_nodes, d_top_leaves};
     }
     TraceScene view()
@@ -55,7 +56,7 @@ struct rise_hip_scene {
     }
     TraceScene view_on(unsigned long long* fetch_counter) const
     {
-        return TraceScene{d_nodes, d_tris, stats.depth, fetch_counter, arith.load()};
+        return TraceScene{d_nodes, d_tris, stats.depth, fetch_counter, arith.load(), d_tri64};
     }
 };
 
@@ -102,19 +103,23 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
     *out_scene = nullptr;
     if (n_meshes > 0 && !meshes) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "meshes is NULL");
     int64_t total = 0;
+    bool any64 = false;
     for (size_t m = 0; m < n_meshes; ++m) {
         if (meshes[m].n_tris < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative triangle count");
-        if (meshes[m].n_tris > 0 && !meshes[m].vertices)
+        if (meshes[m].n_tris > 0 && !meshes[m].vertices && !meshes[m].vertices_f64)
             return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh vertices is NULL");
         total += meshes[m].n_tris;
+        any64 = any64 || (meshes[m].n_tris > 0 && meshes[m].vertices_f64);
     }
     if (total > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "more than 2^31-1 triangles");
 
     // Concatenate meshes (identity transforms: world space == object space).
     std::vector<float> verts;
+    std::vector<double> verts64;
     std::vector<int32_t> ids;
     try {
-        verts.resize((size_t)total * 9);
+        verts.reserve((size_t)total * 9);
+        verts64.reserve((size_t)total * 9);
         ids.resize((size_t)total);
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -123,12 +128,13 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
     for (size_t m = 0; m < n_meshes; ++m) {
         const int64_t k = meshes[m].n_tris;
         if (k == 0) continue;
-        std::memcpy(verts.data() + off * 9, meshes[m].vertices, (size_t)k * 9 * sizeof(float));
+        mesh_vertices(meshes[m], verts, verts64);
         for (int64_t i = 0; i < k; ++i) ids[off + i] = meshes[m].tri_ids ? meshes[m].tri_ids[i] : (int32_t)(off + i);
         off += k;
     }
 
     WideBVH bvh;
+    std::vector<double> tri64;
     std::string err;
     BuildParams params;
     try {
@@ -136,6 +142,7 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
         // caller id (v0.w) and its mesh index (v1.w, reported in hit records).
         if (!build_wide_bvh(verts.data(), total, nullptr, params, bvh, err))
             return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
+        if (any64) tri64 = leaf_vertices(bvh, verts64);
         std::vector<int32_t> mesh_of((size_t)total);
         int64_t o = 0;
         for (size_t m = 0; m < n_meshes; ++m)
@@ -150,14 +157,15 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }
 
-    return upload_scene(bvh, total, 0, out_scene);
+    return upload_scene(bvh, total, 0, out_scene, any64 ? &tri64 : nullptr);
 }
 
 }  // extern "C"
 
 namespace rise_hip {
 
-rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene)
+rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene,
+                             const std::vector<double>* tri64)
 {
     rise_hip_scene* s = new (std::nothrow) rise_hip_scene;
     if (!s) return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -174,9 +182,14 @@ rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_sour
     if (e == hipSuccess) e = hipMemcpy(s->d_nodes, bvh.nodes.data(), nb, hipMemcpyHostToDevice);
     if (e == hipSuccess && !bvh.tris.empty())
         e = hipMemcpy(s->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(Tri48), hipMemcpyHostToDevice);
+    const size_t vb = tri64 ? std::max<size_t>(tri64->size(), 1) * sizeof(double) : 0;
+    if (e == hipSuccess && tri64) e = hipMalloc((void**)&s->d_tri64, vb);
+    if (e == hipSuccess && tri64 && !tri64->empty())
+        e = hipMemcpy(s->d_tri64, tri64->data(), tri64->size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(s->d_nodes);
         (void)hipFree(s->d_tris);
+        (void)hipFree(s->d_tri64);
         (void)hipFree(s->d_fetch);
         delete s;
         return fail_hip(e, "scene upload");
@@ -190,7 +203,7 @@ rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_sour
     s->stats.tri_bytes = (int32_t)sizeof(Tri48);
     s->stats.build_ms = bvh.stats.build_ms;
     s->stats.sah_cost = bvh.stats.sah_cost;
-    s->stats.device_bytes = (int64_t)(nb + tb);
+    s->stats.device_bytes = (int64_t)(nb + tb + vb);
     s->stats.bvh_source = bvh_source;
     *out_scene = s;
     return RISE_HIP_OK;
@@ -211,25 +224,32 @@ rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, 
     rise_hip_status st = build_object_scene(meshes, n_meshes, objects, n_objects, host);
     if (st != RISE_HIP_OK) return st;
     rise_hip_scene* s = nullptr;
-    st = upload_scene(host.world, host.world_tris, 0, &s);
+    st = upload_scene(host.world, host.world_tris, 0, &s, &host.world_tri64);
     if (st != RISE_HIP_OK) return st;
     s->world_empty = host.world_tris == 0;
     s->stats.n_tris = host.total_tris;
     hipError_t e = hipSuccess;
     std::vector<BlasView> views;
-    for (const WideBVH& b : host.blas) {
+    for (size_t k = 0; k < host.blas.size(); ++k) {
         if (e != hipSuccess) break;
+        const WideBVH& b = host.blas[k];
+        const std::vector<double>& v64 = host.blas_tri64[k];
         const size_t nb = b.nodes.size() * sizeof(NodeSlot), tb = std::max<size_t>(b.tris.size(), 1) * sizeof(Tri48);
-        void *dn = nullptr, *dt = nullptr;
+        const size_t vb = std::max<size_t>(v64.size(), 1) * sizeof(double);
+        void *dn = nullptr, *dt = nullptr, *dv = nullptr;
         e = hipMalloc(&dn, nb);
         if (e == hipSuccess) s->blas_allocs.push_back(dn);
         if (e == hipSuccess) e = hipMalloc(&dt, tb);
         if (e == hipSuccess) s->blas_allocs.push_back(dt);
+        if (e == hipSuccess) e = hipMalloc(&dv, vb);
+        if (e == hipSuccess) s->blas_allocs.push_back(dv);
         if (e == hipSuccess) e = hipMemcpy(dn, b.nodes.data(), nb, hipMemcpyHostToDevice);
         if (e == hipSuccess && !b.tris.empty())
             e = hipMemcpy(dt, b.tris.data(), b.tris.size() * sizeof(Tri48), hipMemcpyHostToDevice);
-        views.push_back(BlasView{(const NodeSlot*)dn, (const Tri48*)dt});
-        s->stats.device_bytes += (int64_t)(nb + tb);
+        if (e == hipSuccess && !v64.empty())
+            e = hipMemcpy(dv, v64.data(), v64.size() * sizeof(double), hipMemcpyHostToDevice);
+        views.push_back(BlasView{(const NodeSlot*)dn, (const Tri48*)dt, (const double*)dv});
+        s->stats.device_bytes += (int64_t)(nb + tb + vb);
     }
     if (e == hipSuccess && !host.objects.empty()) {
         const size_t ob = host.objects.size() * sizeof(DevObject), vb = views.size() * sizeof(BlasView);
@@ -264,6 +284,7 @@ rise_hip_status rise_hip_scene_release(rise_hip_scene* scene)
     (void)hipFree(scene->d_top_leaves);
     (void)hipFree(scene->d_nodes);
     (void)hipFree(scene->d_tris);
+    (void)hipFree(scene->d_tri64);
     (void)hipFree(scene->d_fetch);
     delete scene;
     return RISE_HIP_OK;
@@ -472,8 +493,9 @@ extern "C" rise_hip_status rise_hip_scene_cache_write(const rise_hip_scene* scen
 {
     clear_error();
     if (!scene || !path) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (scene->n_objects > 0)
-        return fail(RISE_HIP_ERR_NOT_SUPPORTED, "scene cache of a scene with transformed objects");
+    if (scene->n_objects > 0 || scene->d_tri64)
+        return fail(RISE_HIP_ERR_NOT_SUPPORTED,
+                    "scene cache of a scene with transformed objects or double-precision vertices");
     const uint64_t nn = (uint64_t)scene->stats.n_nodes, nt = (uint64_t)scene->stats.n_tris;
     std::vector<NodeSlot> nodes;
     std::vector<Tri48> tris;

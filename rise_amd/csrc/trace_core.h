@@ -310,16 +310,14 @@ RISE_HD inline bool filter_float(const R& r, const float4& a, const float4& b, c
     return true;
 }
 
-// FP64 Moller-Trumbore, RayTriangleIntersection.cpp:21-68 with the edges of
-// ...Specializations.h:141-142 (computed in double from the vertices).
+// FP64 Moller-Trumbore, RayTriangleIntersection.cpp:21-68, on v0 and the
+// edges eA = v1 - v0, eB = v2 - v0 of ...Specializations.h:141-142 (computed
+// in double from the vertices).
 template <class R>
-RISE_HD inline bool certify(const R& r, const float4& a, const float4& b, const float4& c, double& t_out,
-                            double* ab = nullptr)
+RISE_HD inline bool certify_edges(const R& r, double v0x, double v0y, double v0z, double eAx, double eAy, double eAz,
+                                  double eBx, double eBy, double eBz, double& t_out, double* ab)
 {
     constexpr int A = R::kArith;
-    const double v0x = a.x, v0y = a.y, v0z = a.z;
-    const double eAx = (double)b.x - v0x, eAy = (double)b.y - v0y, eAz = (double)b.z - v0z;
-    const double eBx = (double)c.x - v0x, eBy = (double)c.y - v0y, eBz = (double)c.z - v0z;
     const double d[3] = {dn0(r), dn1(r), dn2(r)};
     const double p0 = sub_prod<A>(d[1], eBz, d[2], eBy);
     const double p1 = sub_prod<A>(d[2], eBx, d[0], eBz);
@@ -347,23 +345,72 @@ RISE_HD inline bool certify(const R& r, const float4& a, const float4& b, const 
     return false;
 }
 
-// Face selection of TriangleMeshGeometryIndexed::RayElementIntersection
-// (...Specializations.h:141-159): the face normal (v1 - v0) x (v2 - v0) in
-// double against the normalized direction; dot < 0 is a front face.
-// faces: bit 0 hit front faces, bit 1 hit back faces.
+// The certifier on the leaf record's float vertices (exact in double).
 template <class R>
-RISE_HD inline bool face_ok(const R& r, const float4& a, const float4& b, const float4& c, unsigned faces)
+RISE_HD inline bool certify(const R& r, const float4& a, const float4& b, const float4& c, double& t_out,
+                            double* ab = nullptr)
+{
+    return certify_edges(r, a.x, a.y, a.z, (double)b.x - a.x, (double)b.y - a.y, (double)b.z - a.z,
+                         (double)c.x - a.x, (double)c.y - a.y, (double)c.z - a.z, t_out, ab);
+}
+
+// The certifier on a triangle's double vertices v[0..8] (v0, v1, v2), as a
+// mesh with double vertices holds them (PointerTriangle, Polygon.h:142).
+template <class R>
+RISE_HD inline bool certify64(const R& r, const double* __restrict__ v, double& t_out, double* ab = nullptr)
+{
+    return certify_edges(r, v[0], v[1], v[2], v[3] - v[0], v[4] - v[1], v[5] - v[2], v[6] - v[0], v[7] - v[1],
+                         v[8] - v[2], t_out, ab);
+}
+
+// The face normal eA x eB as RayElementIntersection computes it
+// (...Specializations.h:141-159, :196-205).  The Linux build contracts its x
+// and y components as fma(-(a.z), b.y, a.y b.z) and fma(-(a.x), b.z, a.z b.x)
+// and its z component as fma(a.x, b.y, -(a.y b.x)) (read from the compiled
+// code; with float vertices every product of edges is exact and the forms
+// agree, double vertices tell them apart).
+template <int A>
+RISE_HD inline void face_cross(double ax, double ay, double az, double bx, double by, double bz, double n[3])
+{
+    if (A == kArithX86Fma) {
+        n[0] = fma(-az, by, ay * bz);
+        n[1] = fma(-ax, bz, az * bx);
+        n[2] = fma(ax, by, -(ay * bx));
+    } else {
+        n[0] = ay * bz - az * by;
+        n[1] = az * bx - ax * bz;
+        n[2] = ax * by - ay * bx;
+    }
+}
+
+// Face selection of TriangleMeshGeometryIndexed::RayElementIntersection
+// (...Specializations.h:141-159): the face normal against the normalized
+// direction; dot < 0 is a front face.  faces: bit 0 hit front faces, bit 1
+// hit back faces.
+template <class R>
+RISE_HD inline bool face_ok_edges(const R& r, double eAx, double eAy, double eAz, double eBx, double eBy, double eBz,
+                                  unsigned faces)
 {
     constexpr int A = R::kArith;
-    const double eAx = (double)b.x - a.x, eAy = (double)b.y - a.y, eAz = (double)b.z - a.z;
-    const double eBx = (double)c.x - a.x, eBy = (double)c.y - a.y, eBz = (double)c.z - a.z;
-    const double nx = sub_prod<A>(eAy, eBz, eAz, eBy);
-    const double ny = sub_prod<A>(eAz, eBx, eAx, eBz);
-    const double nz = sub_prod<A>(eAx, eBy, eAy, eBx);
-    const double dd = dot3<A>(nx, dn0(r), ny, dn1(r), nz, dn2(r));
+    double n[3];
+    face_cross<A>(eAx, eAy, eAz, eBx, eBy, eBz, n);
+    const double dd = dot3<A>(n[0], dn0(r), n[1], dn1(r), n[2], dn2(r));
     if (!(faces & 1u) && dd < 0.0) return false;
     if (!(faces & 2u) && dd > 0.0) return false;
     return true;
+}
+
+template <class R>
+RISE_HD inline bool face_ok(const R& r, const float4& a, const float4& b, const float4& c, unsigned faces)
+{
+    return face_ok_edges(r, (double)b.x - a.x, (double)b.y - a.y, (double)b.z - a.z, (double)c.x - a.x,
+                         (double)c.y - a.y, (double)c.z - a.z, faces);
+}
+
+template <class R>
+RISE_HD inline bool face_ok64(const R& r, const double* __restrict__ v, unsigned faces)
+{
+    return face_ok_edges(r, v[3] - v[0], v[4] - v[1], v[5] - v[2], v[6] - v[0], v[7] - v[1], v[8] - v[2], faces);
 }
 
 // World range of a committed hit: Object.cpp:641-643 for an identity
@@ -586,10 +633,12 @@ RISE_HD inline void closest_begin(const R& r, ClosestState& s)
 // Closest hit: smallest t, exact ties to the lowest caller id (the
 // reference's strict-closer guard, ...Specializations.h:172, plus the tie rule).
 // CULL: apply the face selection in s.faces (face_ok) before the filter.
-template <bool COUNT, bool CULL = false, class Stack, class R>
+// F64: the triangles' double vertices are in tri64 (9 doubles per leaf
+// record, leaf order) and the certifier and face test use them.
+template <bool COUNT, bool CULL = false, bool F64 = false, class Stack, class R>
 RISE_HD inline bool closest_step(const R& r, const NodeSlot* __restrict__ nodes,
                                  const Tri48* __restrict__ tris, Stack& st, ClosestState& s, unsigned& n_nodes,
-                                 unsigned& n_tris)
+                                 unsigned& n_tris, const double* __restrict__ tri64 = nullptr)
 {
     unsigned tmask = 0, tbase = 0;
     if (s.ghits) {
@@ -610,9 +659,10 @@ RISE_HD inline bool closest_step(const R& r, const NodeSlot* __restrict__ nodes,
         const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
-        if (CULL && s.faces != 3u && !face_ok(r, a, b, c, s.faces)) continue;
+        const double* v64 = F64 ? tri64 + 9 * (size_t)(tbase + j) : nullptr;
+        if (CULL && s.faces != 3u && !(F64 ? face_ok64(r, v64, s.faces) : face_ok(r, a, b, c, s.faces))) continue;
         double tt;
-        if (filter_float(r, a, b, c, (float)s.best_t) && certify(r, a, b, c, tt)) {
+        if (filter_float(r, a, b, c, (float)s.best_t) && (F64 ? certify64(r, v64, tt) : certify(r, a, b, c, tt))) {
             const int id = float_to_bits(a.w);
             if (tt < s.best_t || (tt == s.best_t && id < s.best_id)) {
                 s.best_t = tt;
@@ -676,9 +726,10 @@ RISE_HD inline void any_begin(const R& r, double how_far2, AnyState& s)
 // kNearZero < t < how_far2 (...Specializations.h:517-519) and casts shadows
 // (bit 0 of v2.w clear; ObjectManager.cpp:119-124).  Returns false once
 // the query is decided (a hit, or the tree exhausted).
-template <bool COUNT, bool CULL = false, class Stack, class R>
+template <bool COUNT, bool CULL = false, bool F64 = false, class Stack, class R>
 RISE_HD inline bool any_step(const R& r, const NodeSlot* __restrict__ nodes, const Tri48* __restrict__ tris,
-                             Stack& st, AnyState& s, unsigned& n_nodes, unsigned& n_tris)
+                             Stack& st, AnyState& s, unsigned& n_nodes, unsigned& n_tris,
+                             const double* __restrict__ tri64 = nullptr)
 {
     unsigned tmask = 0, tbase = 0;
     if (s.ghits) {
@@ -699,9 +750,11 @@ RISE_HD inline bool any_step(const R& r, const NodeSlot* __restrict__ nodes, con
         const float4* tp = reinterpret_cast<const float4*>(tris + tbase + j);
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
-        if (CULL && s.faces != 3u && !face_ok(r, a, b, c, s.faces)) continue;
+        const double* v64 = F64 ? tri64 + 9 * (size_t)(tbase + j) : nullptr;
+        if (CULL && s.faces != 3u && !(F64 ? face_ok64(r, v64, s.faces) : face_ok(r, a, b, c, s.faces))) continue;
         double tt;
-        if (filter_float(r, a, b, c, s.fbest) && certify(r, a, b, c, tt) && tt > kNearZero && tt < s.how_far2 &&
+        if (filter_float(r, a, b, c, s.fbest) && (F64 ? certify64(r, v64, tt) : certify(r, a, b, c, tt)) &&
+            tt > kNearZero && tt < s.how_far2 &&
             !(float_to_bits(c.w) & 1u)) {
             s.hit = true;
             return false;
@@ -735,22 +788,42 @@ RISE_HD inline bool trace_any(const R& r, const NodeSlot* __restrict__ nodes, co
 // barycentrics, the face normal (v1 - v0) x (v2 - v0) normalized
 // (...Specializations.h:196-205) and normalized again by the object's normal
 // transform (Object.cpp:531-533), and the hit point o + dn (t - 1e-12)
-// (Object.cpp:637-640).
-template <int A>
-RISE_HD inline void hit_attributes(const RayStateT<A>& r, const Tri48* __restrict__ tri, double t, double ab[2],
-                                   double n_out[3], double p_out[3])
+// (Object.cpp:637-640).  v64: the triangle's double vertices, or NULL for
+// the leaf record's floats.
+template <class R>
+RISE_HD inline void triangle_vertices(const Tri48* __restrict__ tri, const double* __restrict__ v64, double v[9])
 {
+    if (v64) {
+        for (int k = 0; k < 9; ++k) v[k] = v64[k];
+        return;
+    }
     const float4* tp = reinterpret_cast<const float4*>(tri);
     const float4 a = tp[0], b = tp[1], c = tp[2];
+    const double w[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+    for (int k = 0; k < 9; ++k) v[k] = w[k];
+}
+
+// The certifier's barycentrics and the face normal (v1 - v0) x (v2 - v0)
+// normalized once by the mesh (...Specializations.h:196-205).
+template <class R>
+RISE_HD inline void hit_face(const R& r, const double v[9], double ab[2], double n1[3])
+{
+    constexpr int A = R::kArith;
     double tt = 0.0;
     ab[0] = ab[1] = 0.0;
-    (void)certify(r, a, b, c, tt, ab);
-    const double eAx = (double)b.x - a.x, eAy = (double)b.y - a.y, eAz = (double)b.z - a.z;
-    const double eBx = (double)c.x - a.x, eBy = (double)c.y - a.y, eBz = (double)c.z - a.z;
-    const double fn[3] = {sub_prod<A>(eAy, eBz, eAz, eBy), sub_prod<A>(eAz, eBx, eAx, eBz),
-                          sub_prod<A>(eAx, eBy, eAy, eBx)};
-    double n1[3];
+    (void)certify64(r, v, tt, ab);
+    double fn[3];
+    face_cross<A>(v[3] - v[0], v[4] - v[1], v[5] - v[2], v[6] - v[0], v[7] - v[1], v[8] - v[2], fn);
     normalize_model<A>(fn, n1);
+}
+
+template <int A>
+RISE_HD inline void hit_attributes(const RayStateT<A>& r, const Tri48* __restrict__ tri, double t, double ab[2],
+                                   double n_out[3], double p_out[3], const double* __restrict__ v64 = nullptr)
+{
+    double v[9], n1[3];
+    triangle_vertices<RayStateT<A>>(tri, v64, v);
+    hit_face(r, v, ab, n1);
     normalize_model<A>(n1, n_out);
     const double s = t - kSurfaceBackoff;
     const double d[3] = {dn0(r), dn1(r), dn2(r)};

@@ -16,6 +16,7 @@ struct TraceScene {
     int depth;  // wide-node levels; selects the stack variant
     unsigned long long* fetch_counter;  // device counter for dynamic ray fetch (one per in-flight call)
     int arith;  // rise_hip_arith: kArithIeee or kArithX86Fma (trace_core.h), same values
+    const double* tri64;  // leaf records' double vertices (9 each) for the certifier, or NULL
 };
 
 // Closest hit: rays SoA (ox, oy, oz, dx, dy, dz), n each.  counters: NULL, or

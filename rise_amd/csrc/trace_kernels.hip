@@ -111,13 +111,15 @@ __global__ __launch_bounds__(kBlock) void any_kernel(const NodeSlot* __restrict_
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-template <bool COUNT, int STACK, int A>
+// F64: certify on the leaf records' double vertices (tri64).
+template <bool COUNT, int STACK, int A, bool F64 = false>
 __global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __restrict__ nodes,
                                                              const Tri48* __restrict__ tris,
                                                              const float* __restrict__ rays, int64_t n,
                                                              int32_t* __restrict__ prim_id,
                                                              float* __restrict__ t_out, unsigned long long* counters,
-                                                             unsigned long long* next)
+                                                             unsigned long long* next,
+                                                             const double* __restrict__ tri64)
 {
     __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
     LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
@@ -144,8 +146,8 @@ __global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __r
         }
         if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
         if (ray >= 0) {
-            const bool more = STACK > 0 ? closest_step<COUNT>(r, nodes, tris, lst, s, nn, nt)
-                                        : closest_step<COUNT>(r, nodes, tris, pst, s, nn, nt);
+            const bool more = STACK > 0 ? closest_step<COUNT, false, F64>(r, nodes, tris, lst, s, nn, nt, tri64)
+                                        : closest_step<COUNT, false, F64>(r, nodes, tris, pst, s, nn, nt, tri64);
             if (!more) {
                 prim_id[ray] = s.best_id;
                 t_out[ray] = s.best_id >= 0 ? (float)world_range(r, s.best_t) : INFINITY;
@@ -156,13 +158,14 @@ __global__ __launch_bounds__(kBlock) void closest_persistent(const NodeSlot* __r
     flush_counts<COUNT>(counters, nn, nt);
 }
 
-template <bool COUNT, int STACK, int A>
+template <bool COUNT, int STACK, int A, bool F64 = false>
 __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restrict__ nodes,
                                                          const Tri48* __restrict__ tris,
                                                          const float* __restrict__ rays,
                                                          const float* __restrict__ tmax, int64_t n,
                                                          uint8_t* __restrict__ occluded, unsigned long long* counters,
-                                                         unsigned long long* next)
+                                                         unsigned long long* next,
+                                                         const double* __restrict__ tri64)
 {
     __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
     LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
@@ -190,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restr
         }
         if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
         if (ray >= 0) {
-            const bool more = STACK > 0 ? any_step<COUNT>(r, nodes, tris, lst, s, nn, nt)
-                                        : any_step<COUNT>(r, nodes, tris, pst, s, nn, nt);
+            const bool more = STACK > 0 ? any_step<COUNT, false, F64>(r, nodes, tris, lst, s, nn, nt, tri64)
+                                        : any_step<COUNT, false, F64>(r, nodes, tris, pst, s, nn, nt, tri64);
             if (!more) {
                 occluded[ray] = s.hit ? 1 : 0;
                 ray = -1;
@@ -203,12 +206,13 @@ __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restr
 
 // Closest hit with the face selection and the hit record of
 // rise_hip_intersect_closest_ex.
-template <int STACK, int A>
+template <int STACK, int A, bool F64 = false>
 __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* __restrict__ nodes,
                                                                 const Tri48* __restrict__ tris,
                                                                 const float* __restrict__ rays, int64_t n,
                                                                 unsigned faces, rise_hip_hit* __restrict__ hits,
-                                                                unsigned long long* next)
+                                                                unsigned long long* next,
+                                                                const double* __restrict__ tri64)
 {
     __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
     LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
@@ -235,15 +239,15 @@ __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* 
         }
         if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
         if (ray >= 0) {
-            const bool more = STACK > 0 ? closest_step<false, true>(r, nodes, tris, lst, s, nn, nt)
-                                        : closest_step<false, true>(r, nodes, tris, pst, s, nn, nt);
+            const bool more = STACK > 0 ? closest_step<false, true, F64>(r, nodes, tris, lst, s, nn, nt, tri64)
+                                        : closest_step<false, true, F64>(r, nodes, tris, pst, s, nn, nt, tri64);
             if (!more) {
                 rise_hip_hit h;
                 h.prim_id = s.best_id;
                 if (s.best_id >= 0) {
                     const Tri48* tri = tris + s.best_leaf;
                     double ab[2], nrm[3], pt[3];
-                    hit_attributes(r, tri, s.best_t, ab, nrm, pt);
+                    hit_attributes(r, tri, s.best_t, ab, nrm, pt, F64 ? tri64 + 9 * (size_t)s.best_leaf : nullptr);
                     h.mesh = float_to_bits(tri->v1[3]);
                     h.range = world_range(r, s.best_t);
                     h.alpha = ab[0];
@@ -266,13 +270,14 @@ __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* 
 }
 
 // Any hit with the face selection of rise_hip_intersect_any_ex.
-template <int STACK, int A>
+template <int STACK, int A, bool F64 = false>
 __global__ __launch_bounds__(kBlock) void any_ex_persistent(const NodeSlot* __restrict__ nodes,
                                                             const Tri48* __restrict__ tris,
                                                             const float* __restrict__ rays,
                                                             const float* __restrict__ tmax, int64_t n,
                                                             unsigned faces, uint8_t* __restrict__ occluded,
-                                                            unsigned long long* next)
+                                                            unsigned long long* next,
+                                                            const double* __restrict__ tri64)
 {
     __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
     LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
@@ -300,8 +305,8 @@ __global__ __launch_bounds__(kBlock) void any_ex_persistent(const NodeSlot* __re
         }
         if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
         if (ray >= 0) {
-            const bool more = STACK > 0 ? any_step<false, true>(r, nodes, tris, lst, s, nn, nt)
-                                        : any_step<false, true>(r, nodes, tris, pst, s, nn, nt);
+            const bool more = STACK > 0 ? any_step<false, true, F64>(r, nodes, tris, lst, s, nn, nt, tri64)
+                                        : any_step<false, true, F64>(r, nodes, tris, pst, s, nn, nt, tri64);
             if (!more) {
                 occluded[ray] = s.hit ? 1 : 0;
                 ray = -1;
@@ -343,10 +348,16 @@ hipError_t launch_closest(const TraceScene& s, const float* rays, int64_t n, int
     const dim3 grid(grid_for(n)), block(kBlock);
 #if RISE_PERSISTENT
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
-#define RISE_LAUNCH_CLOSEST(C, S, A)                                                                      \
-    hipLaunchKernelGGL((closest_persistent<C, S, A>), dim3(persistent_grid(closest_persistent<C, S, A>, n)), block, \
-                       0, stream, s.nodes, s.tris, rays, n, prim_id, t, counters, s.fetch_counter)
+#define RISE_LAUNCH_CLOSEST_F(C, S, A, F)                                                                   \
+    hipLaunchKernelGGL((closest_persistent<C, S, A, F>), dim3(persistent_grid(closest_persistent<C, S, A, F>, n)), \
+                       block, 0, stream, s.nodes, s.tris, rays, n, prim_id, t, counters, s.fetch_counter, s.tri64)
+#define RISE_LAUNCH_CLOSEST(C, S, A)                   \
+    do {                                               \
+        if (s.tri64) RISE_LAUNCH_CLOSEST_F(C, S, A, true); \
+        else RISE_LAUNCH_CLOSEST_F(C, S, A, false);     \
+    } while (0)
 #else
+    if (s.tri64) return hipErrorNotSupported;  // the one-ray-per-thread kernels certify on floats only
 #define RISE_LAUNCH_CLOSEST(C, S, A)                                                                          \
     hipLaunchKernelGGL((closest_kernel<C, S, A>), grid, block, 0, stream, s.nodes, s.tris, rays, n, prim_id, t, \
                        counters)
@@ -367,6 +378,7 @@ hipError_t launch_closest(const TraceScene& s, const float* rays, int64_t n, int
     }
 #undef RISE_LAUNCH_CLOSEST_A
 #undef RISE_LAUNCH_CLOSEST
+#undef RISE_LAUNCH_CLOSEST_F
     return hipGetLastError();
 }
 
@@ -378,10 +390,16 @@ hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax,
     const dim3 grid(grid_for(n)), block(kBlock);
 #if RISE_PERSISTENT
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
-#define RISE_LAUNCH_ANY(C, S, A)                                                                           \
-    hipLaunchKernelGGL((any_persistent<C, S, A>), dim3(persistent_grid(any_persistent<C, S, A>, n)), block, 0, \
-                       stream, s.nodes, s.tris, rays, tmax, n, occluded, counters, s.fetch_counter)
+#define RISE_LAUNCH_ANY_F(C, S, A, F)                                                                        \
+    hipLaunchKernelGGL((any_persistent<C, S, A, F>), dim3(persistent_grid(any_persistent<C, S, A, F>, n)), block, \
+                       0, stream, s.nodes, s.tris, rays, tmax, n, occluded, counters, s.fetch_counter, s.tri64)
+#define RISE_LAUNCH_ANY(C, S, A)                   \
+    do {                                           \
+        if (s.tri64) RISE_LAUNCH_ANY_F(C, S, A, true); \
+        else RISE_LAUNCH_ANY_F(C, S, A, false);     \
+    } while (0)
 #else
+    if (s.tri64) return hipErrorNotSupported;  // the one-ray-per-thread kernels certify on floats only
 #define RISE_LAUNCH_ANY(C, S, A)                                                                             \
     hipLaunchKernelGGL((any_kernel<C, S, A>), grid, block, 0, stream, s.nodes, s.tris, rays, tmax, n, occluded, \
                        counters)
@@ -402,6 +420,7 @@ hipError_t launch_any(const TraceScene& s, const float* rays, const float* tmax,
     }
 #undef RISE_LAUNCH_ANY_A
 #undef RISE_LAUNCH_ANY
+#undef RISE_LAUNCH_ANY_F
     return hipGetLastError();
 }
 
@@ -412,9 +431,14 @@ hipError_t launch_closest_ex(const TraceScene& s, const float* rays, int64_t n, 
     const int need = s.depth - 1;
     const dim3 block(kBlock);
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
-#define RISE_LAUNCH_EX(S, A)                                                                                      \
-    hipLaunchKernelGGL((closest_ex_persistent<S, A>), dim3(persistent_grid(closest_ex_persistent<S, A>, n)), block, \
-                       0, stream, s.nodes, s.tris, rays, n, faces, hits, s.fetch_counter)
+#define RISE_LAUNCH_EX_F(S, A, F)                                                                           \
+    hipLaunchKernelGGL((closest_ex_persistent<S, A, F>), dim3(persistent_grid(closest_ex_persistent<S, A, F>, n)), \
+                       block, 0, stream, s.nodes, s.tris, rays, n, faces, hits, s.fetch_counter, s.tri64)
+#define RISE_LAUNCH_EX(S, A)                   \
+    do {                                       \
+        if (s.tri64) RISE_LAUNCH_EX_F(S, A, true); \
+        else RISE_LAUNCH_EX_F(S, A, false);     \
+    } while (0)
 #define RISE_LAUNCH_EX_A(S)                                          \
     do {                                                             \
         if (s.arith == kArithX86Fma) RISE_LAUNCH_EX(S, kArithX86Fma); \
@@ -425,6 +449,7 @@ hipError_t launch_closest_ex(const TraceScene& s, const float* rays, int64_t n, 
     else RISE_LAUNCH_EX_A(0);
 #undef RISE_LAUNCH_EX_A
 #undef RISE_LAUNCH_EX
+#undef RISE_LAUNCH_EX_F
     return hipGetLastError();
 }
 
@@ -435,9 +460,14 @@ hipError_t launch_any_ex(const TraceScene& s, const float* rays, const float* tm
     const int need = s.depth - 1;
     const dim3 block(kBlock);
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
-#define RISE_LAUNCH_EX(S, A)                                                                              \
-    hipLaunchKernelGGL((any_ex_persistent<S, A>), dim3(persistent_grid(any_ex_persistent<S, A>, n)), block, 0, \
-                       stream, s.nodes, s.tris, rays, tmax, n, faces, occluded, s.fetch_counter)
+#define RISE_LAUNCH_EX_F(S, A, F)                                                                          \
+    hipLaunchKernelGGL((any_ex_persistent<S, A, F>), dim3(persistent_grid(any_ex_persistent<S, A, F>, n)), block, \
+                       0, stream, s.nodes, s.tris, rays, tmax, n, faces, occluded, s.fetch_counter, s.tri64)
+#define RISE_LAUNCH_EX(S, A)                   \
+    do {                                       \
+        if (s.tri64) RISE_LAUNCH_EX_F(S, A, true); \
+        else RISE_LAUNCH_EX_F(S, A, false);     \
+    } while (0)
 #define RISE_LAUNCH_EX_A(S)                                          \
     do {                                                             \
         if (s.arith == kArithX86Fma) RISE_LAUNCH_EX(S, kArithX86Fma); \
@@ -448,6 +478,7 @@ hipError_t launch_any_ex(const TraceScene& s, const float* rays, const float* tm
     else RISE_LAUNCH_EX_A(0);
 #undef RISE_LAUNCH_EX_A
 #undef RISE_LAUNCH_EX
+#undef RISE_LAUNCH_EX_F
     return hipGetLastError();
 }
 

@@ -119,10 +119,13 @@ class ObjectManager:
         return len(self._objects) - 1
 
     def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None) -> None:
-        """tris: (n, 9) float32 world-space triangles; ids: optional (n,) int32."""
+        """tris: (n, 9) float32 world-space triangles, or float64 for a mesh
+        whose double vertices the certifier uses (rise_hip_mesh_desc.
+        vertices_f64); ids: optional (n,) int32."""
         if self._from_cache:
             raise RuntimeError("a scene loaded from a cache is complete; it takes no more meshes")
-        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        f64 = np.asarray(tris).dtype == np.float64
+        t = np.ascontiguousarray(tris, dtype=np.float64 if f64 else np.float32).reshape(-1, 9)
         i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
         if i is not None and i.shape[0] != t.shape[0]:
             raise ValueError("ids must have one entry per triangle")
@@ -135,9 +138,7 @@ class ObjectManager:
         self.release()
         descs = (_lib.MeshDesc * max(1, len(self._meshes)))()
         for k, (t, i) in enumerate(self._meshes):
-            descs[k].vertices = t.ctypes.data
-            descs[k].n_tris = t.shape[0]
-            descs[k].tri_ids = None if i is None else i.ctypes.data
+            _set_desc(descs[k], t, i)
         out = ctypes.c_void_p(None)
         if self._objects:
             objs = make_objects(self._objects)
@@ -266,6 +267,18 @@ class ObjectManager:
             pass
 
 
+def _set_desc(desc, t: np.ndarray, ids) -> None:
+    """Fills a rise_hip_mesh_desc from float32 or float64 (double vertices) triangles."""
+    if t.dtype == np.float64:
+        desc.vertices = None
+        desc.vertices_f64 = t.ctypes.data
+    else:
+        desc.vertices = t.ctypes.data
+        desc.vertices_f64 = None
+    desc.n_tris = t.shape[0]
+    desc.tri_ids = None if ids is None else ids.ctypes.data
+
+
 def _check_rays(rays) -> None:
     import torch
 
@@ -334,19 +347,18 @@ def host_trace_ex(tris: np.ndarray, rays: np.ndarray, mode: int = 0, faces: int 
 def host_trace_objects(meshes, objects, rays: np.ndarray, mode: int = 0, faces: int = FACES_BOTH,
                        tmax: np.ndarray | None = None, arith: int = ARITH_IEEE, ids=None):
     """rise_hip_debug_objects_host: a scene of objects (meshes: list of (n, 9)
-    float32 arrays; objects: OBJECT_DTYPE array) queried with the kernels'
+    float32 arrays, or float64 for double vertices; objects: OBJECT_DTYPE array) queried with the kernels'
     per-ray code on host threads (validation only).  Mode 0: hit records
     (HIT_DTYPE, `mesh` = object index); mode 1: occlusion bytes."""
     mode = (mode & 1) | (2 if arith == ARITH_X86_FMA else 0)
     L = _lib.lib()
-    arrs = [np.ascontiguousarray(m, dtype=np.float32).reshape(-1, 9) for m in meshes]
+    arrs = [np.ascontiguousarray(m, dtype=np.float64 if np.asarray(m).dtype == np.float64 else np.float32)
+            .reshape(-1, 9) for m in meshes]
     id_arrs = [None if ids is None or ids[k] is None else np.ascontiguousarray(ids[k], np.int32)
                for k in range(len(arrs))]
     descs = (_lib.MeshDesc * max(1, len(arrs)))()
     for k, t in enumerate(arrs):
-        descs[k].vertices = t.ctypes.data
-        descs[k].n_tris = t.shape[0]
-        descs[k].tri_ids = None if id_arrs[k] is None else id_arrs[k].ctypes.data
+        _set_desc(descs[k], t, id_arrs[k])
     objs = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
