@@ -118,16 +118,17 @@ private:
                 GlobalLog()->PrintEasyError("HipAcceleratedObjectManager: only triangle meshes go to the GPU");
                 return false;
             }
-            // The mesh's triangles in its own frame, float as its BVH's
-            // filter stores them (BVH.h:431-467).
-            std::vector<float>& v = vertices_[k];
+            // The mesh's triangles in its own frame, with the double
+            // vertices the mesh holds (the library takes their float casts
+            // for the filter, as the mesh's BVH does, BVH.h:431-467).
+            std::vector<double>& v = vertices_[k];
             for (const PointerTriangle& t : mesh->getFaces())
                 for (int c3 = 0; c3 < 3; ++c3) {
-                    v.push_back((float)t.pVertices[c3]->x);
-                    v.push_back((float)t.pVertices[c3]->y);
-                    v.push_back((float)t.pVertices[c3]->z);
+                    v.push_back(t.pVertices[c3]->x);
+                    v.push_back(t.pVertices[c3]->y);
+                    v.push_back(t.pVertices[c3]->z);
                 }
-            const int m = gpu_->AddMesh(v.data(), (int64_t)(v.size() / 9));
+            const int m = gpu_->AddMeshF64(v.data(), (int64_t)(v.size() / 9));
             double tw[16], to[16];
             ToArray(objects_[k]->GetFinalTransformMatrix(), tw);
             ToArray(objects_[k]->GetFinalInverseTransformMatrix(), to);
@@ -145,7 +146,7 @@ private:
 
     mutable std::unique_ptr<rise_hip_host::HipObjectManager> gpu_;
     mutable std::vector<const IObjectPriv*> objects_;
-    mutable std::vector<std::vector<float>> vertices_;  // kept until the next upload
+    mutable std::vector<std::vector<double>> vertices_;  // kept until the next upload
     mutable bool gpu_ok_ = false;
 };
 

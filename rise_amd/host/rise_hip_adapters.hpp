@@ -57,7 +57,15 @@ public:
     // objects that refer to it.  Returns its index.
     int AddMesh(const float* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr)
     {
-        meshes_.push_back({vertices, n_tris, tri_ids});
+        meshes_.push_back({vertices, n_tris, tri_ids, nullptr});
+        return (int)meshes_.size() - 1;
+    }
+
+    // The same with the mesh's double vertices (n_tris * 9), as RISE holds
+    // them: the certifier uses them, the float filter their float casts.
+    int AddMeshF64(const double* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr)
+    {
+        meshes_.push_back({nullptr, n_tris, tri_ids, vertices});
         return (int)meshes_.size() - 1;
     }
 
