@@ -66,12 +66,19 @@ typedef struct rise_hip_scene rise_hip_scene;
  *               given, `vertices` may be NULL: the double certifier and the
  *               face test use these, the float pre-filter and the tree their
  *               float rounding, as the reference's mesh does
- *               (...Specializations.h:32-47, :141-142). */
+ *               (...Specializations.h:32-47, :141-142).
+ * flags:        RISE_HIP_MESH_DOUBLE_SIDED: the mesh is double-sided
+ *               (TriangleMeshGeometryIndexed's bDoubleSided,
+ *               TriangleMeshGeometryIndexed.cpp:151-205): the face selection
+ *               of the queries does not apply to it, and a hit record's
+ *               normal is turned toward the ray's origin. */
+#define RISE_HIP_MESH_DOUBLE_SIDED 1u
 typedef struct rise_hip_mesh_desc {
     const float* vertices;
     int64_t n_tris;
     const int32_t* tri_ids;
     const double* vertices_f64;
+    uint32_t flags;
 } rise_hip_mesh_desc;
 
 typedef struct rise_hip_scene_stats {

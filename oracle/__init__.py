@@ -74,6 +74,7 @@ X86_FMA = 2
 NO_FRONT = 4
 NO_BACK = 8
 OBJ_PREHIT = 16
+DOUBLE_SIDED = 32
 
 
 def _faces_opts(faces: int) -> int:
@@ -110,7 +111,7 @@ def closest(tris: np.ndarray, rays: np.ndarray, threads: int = 0, flags: bool = 
 
 
 def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, threads: int = 0,
-            no_filter: bool = False, arith: str = "ieee", faces: int = 3) -> np.ndarray:
+            no_filter: bool = False, arith: str = "ieee", faces: int = 3, double_sided: bool = False) -> np.ndarray:
     """Brute-force any hit; float64 triangles are double vertices
     (rise_oracle_any_brute_f64)."""
     L = oracle_lib()
@@ -123,7 +124,8 @@ def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, 
         tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
         L.rise_oracle_any_brute_f64(t64.ctypes.data, t64.shape[0], rays.ctypes.data,
                                     None if tm is None else tm.ctypes.data, n, occ.ctypes.data,
-                                    _opts(no_filter, arith) | _faces_opts(faces), threads)
+                                    _opts(no_filter, arith) | _faces_opts(faces) | (DOUBLE_SIDED if double_sided else 0),
+                                    threads)
         return occ
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -132,11 +134,13 @@ def any_hit(tris: np.ndarray, rays: np.ndarray, tmax: np.ndarray | None = None, 
     tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
     L.rise_oracle_any_brute(tris.ctypes.data, tris.shape[0], rays.ctypes.data,
                             None if tm is None else tm.ctypes.data, n, occ.ctypes.data,
-                            _opts(no_filter, arith) | _faces_opts(faces), threads)
+                            _opts(no_filter, arith) | _faces_opts(faces) | (DOUBLE_SIDED if double_sided else 0),
+                            threads)
     return occ
 
 
-def closest_attr(tris: np.ndarray, rays: np.ndarray, faces: int = 3, arith: str = "ieee", threads: int = 0) -> dict:
+def closest_attr(tris: np.ndarray, rays: np.ndarray, faces: int = 3, arith: str = "ieee", threads: int = 0,
+                 double_sided: bool = False) -> dict:
     """Brute-force closest hit with the intersection record
     (rise_oracle_closest_attr): prim_id, range, alpha, beta, normal (n, 3),
     point (n, 3), all float64 except the ids.  float64 triangles are double
@@ -152,7 +156,8 @@ def closest_attr(tris: np.ndarray, rays: np.ndarray, faces: int = 3, arith: str 
         L.rise_oracle_closest_attr_f64(t64.ctypes.data, t64.shape[0], rays.ctypes.data, n,
                                        *[out[k].ctypes.data for k in ("prim_id", "range", "alpha", "beta", "normal",
                                                                       "point")],
-                                       _opts(False, arith) | _faces_opts(faces), threads)
+                                       _opts(False, arith) | _faces_opts(faces) | (DOUBLE_SIDED if double_sided else 0),
+                                       threads)
         return out
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -161,7 +166,8 @@ def closest_attr(tris: np.ndarray, rays: np.ndarray, faces: int = 3, arith: str 
            "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
     L.rise_oracle_closest_attr(tris.ctypes.data, tris.shape[0], rays.ctypes.data, n,
                                *[out[k].ctypes.data for k in ("prim_id", "range", "alpha", "beta", "normal", "point")],
-                               _opts(False, arith) | _faces_opts(faces), threads)
+                               _opts(False, arith) | _faces_opts(faces) | (DOUBLE_SIDED if double_sided else 0),
+                               threads)
     return out
 
 
@@ -188,23 +194,32 @@ def _meshes(meshes):
     return tris, off
 
 
+def _sided(double_sided, n):
+    if double_sided is None:
+        return None
+    return np.ascontiguousarray(np.asarray(double_sided, dtype=np.uint8).reshape(n))
+
+
 def objects_closest(meshes, objects, rays: np.ndarray, faces: int = 3, arith: str = "ieee", threads: int = 0,
-                    flags: bool = False, prehit: bool = False) -> dict:
+                    flags: bool = False, prehit: bool = False, double_sided=None) -> dict:
     """Closest hit over a scene of objects (rise_oracle_objects_closest):
     prim_id (index within the mesh), object, range, alpha, beta, normal (n, 3),
     point (n, 3), and with flags=True "flags" (traversal-order dependent).
-    prehit: model Object::IntersectRay's box pre-test (RISE_ORACLE_OBJ_PREHIT)."""
+    prehit: model Object::IntersectRay's box pre-test (RISE_ORACLE_OBJ_PREHIT);
+    double_sided: None or one bool per mesh."""
     L = oracle_lib()
-    L.rise_oracle_objects_closest.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P, _I64] + [_P] * 8 + [
+    L.rise_oracle_objects_closest.argtypes = [_P, _P, ctypes.c_int, _P, _P, ctypes.c_int, _P, _I64] + [_P] * 8 + [
         ctypes.c_int, ctypes.c_int]
     tris, off = _meshes(meshes)
+    ds = _sided(double_sided, len(off) - 1)
     obj = _objects(objects)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
     out = {"prim_id": np.empty(n, np.int32), "object": np.empty(n, np.int32), "range": np.empty(n),
            "alpha": np.empty(n), "beta": np.empty(n), "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
     fl = np.empty(n, np.uint8) if flags else None
-    L.rise_oracle_objects_closest(tris.ctypes.data, off.ctypes.data, len(off) - 1, obj.ctypes.data, len(obj),
+    L.rise_oracle_objects_closest(tris.ctypes.data, off.ctypes.data, len(off) - 1,
+                                  None if ds is None else ds.ctypes.data, obj.ctypes.data, len(obj),
                                   rays.ctypes.data, n,
                                   *[out[k].ctypes.data for k in ("prim_id", "object", "range", "alpha", "beta",
                                                                  "normal", "point")],
@@ -216,19 +231,21 @@ def objects_closest(meshes, objects, rays: np.ndarray, faces: int = 3, arith: st
 
 
 def objects_any(meshes, objects, rays: np.ndarray, tmax: np.ndarray | None = None, faces: int = 3,
-                arith: str = "ieee", threads: int = 0, prehit: bool = False) -> np.ndarray:
+                arith: str = "ieee", threads: int = 0, prehit: bool = False, double_sided=None) -> np.ndarray:
     """Any hit over a scene of objects (rise_oracle_objects_any); prehit as
     for objects_closest."""
     L = oracle_lib()
-    L.rise_oracle_objects_any.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P, _P, _I64, _P, ctypes.c_int,
-                                          ctypes.c_int]
+    L.rise_oracle_objects_any.argtypes = [_P, _P, ctypes.c_int, _P, _P, ctypes.c_int, _P, _P, _I64, _P,
+                                          ctypes.c_int, ctypes.c_int]
     tris, off = _meshes(meshes)
+    ds = _sided(double_sided, len(off) - 1)
     obj = _objects(objects)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
     occ = np.empty(n, np.uint8)
     tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float32)
-    L.rise_oracle_objects_any(tris.ctypes.data, off.ctypes.data, len(off) - 1, obj.ctypes.data, len(obj),
+    L.rise_oracle_objects_any(tris.ctypes.data, off.ctypes.data, len(off) - 1,
+                              None if ds is None else ds.ctypes.data, obj.ctypes.data, len(obj),
                               rays.ctypes.data, None if tm is None else tm.ctypes.data, n, occ.ctypes.data,
                               _opts(False, arith) | _faces_opts(faces) | (OBJ_PREHIT if prehit else 0), threads)
     return occ
@@ -246,12 +263,12 @@ class RefMesh:
     double certifier) built from its own sources (oracle/_ref)."""
 
     def __init__(self, tris: np.ndarray | None, variant: str = "ieee", risemesh: str | None = None,
-                 bary: bool = False):
+                 bary: bool = False, double_sided: bool = False):
         """Builds the mesh from `tris` (float32, or float64 kept as double
         vertices), or (tris None) loads the .risemesh file
         `risemesh` with the reference's Deserialize.  bary: vertex normals
         that make the interpolated shading normal carry the barycentrics
-        (rise_ref_mesh_create_bary)."""
+        (rise_ref_mesh_create_bary).  double_sided: the mesh's bDoubleSided."""
         name = "libriseref_ieee.so" if variant == "ieee" else "libriseref.so"
         L = _load(REF_DIR / name)
         L.rise_ref_mesh_create.argtypes = [_P, _I64]
@@ -270,14 +287,15 @@ class RefMesh:
         L.rise_ref_closest_ex.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int] + [_P] * 6 + [ctypes.c_int]
         L.rise_ref_any_ex.argtypes = [_P, _P, _P, _I64, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int]
         self._L = L
-        L.rise_ref_mesh_create_f64.argtypes = [_P, _I64, ctypes.c_int]
+        L.rise_ref_mesh_create_f64.argtypes = [_P, _I64, ctypes.c_int, ctypes.c_int]
         L.rise_ref_mesh_create_f64.restype = _P
         if tris is None:
             self._tris = None
             self._h = L.rise_ref_risemesh_load(str(risemesh).encode())
-        elif np.asarray(tris).dtype == np.float64:
-            self._tris = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
-            self._h = L.rise_ref_mesh_create_f64(self._tris.ctypes.data, self._tris.shape[0], int(bary))
+        elif np.asarray(tris).dtype == np.float64 or double_sided:
+            self._tris = np.ascontiguousarray(np.asarray(tris).astype(np.float64)).reshape(-1, 9)
+            self._h = L.rise_ref_mesh_create_f64(self._tris.ctypes.data, self._tris.shape[0], int(bary),
+                                                 int(double_sided))
         else:
             self._tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
             create = L.rise_ref_mesh_create_bary if bary else L.rise_ref_mesh_create

@@ -179,11 +179,12 @@ void* rise_ref_mesh_create(const float* tris, int64_t n_tris)
 }
 
 // rise_ref_mesh_create / _create_bary on double vertices (9 doubles per
-// triangle), kept as the mesh's Point3 vertices.
-void* rise_ref_mesh_create_f64(const double* tris, int64_t n_tris, int bary)
+// triangle), kept as the mesh's Point3 vertices; double_sided: the mesh's
+// bDoubleSided.
+void* rise_ref_mesh_create_f64(const double* tris, int64_t n_tris, int bary, int double_sided)
 {
     RefMesh* h = new RefMesh;
-    h->mesh = new Implementation::TriangleMeshGeometryIndexed(false, !bary);
+    h->mesh = new Implementation::TriangleMeshGeometryIndexed(double_sided != 0, !bary);
     h->mesh->BeginIndexedTriangles();
     if (bary) {
         h->mesh->AddNormal(Normal(1.0, 0.0, 0.0));

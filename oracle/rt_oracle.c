@@ -267,6 +267,7 @@ static void float_ray(const double o[3], const double dn[3], float fo[3], float 
 static int face_selected(int x, int opts, const tri64* t, const double dn[3])
 {
     if (!(opts & (RISE_ORACLE_NO_FRONT | RISE_ORACLE_NO_BACK))) return 1;
+    if (opts & RISE_ORACLE_DOUBLE_SIDED) return 1; /* TriangleMeshGeometryIndexed.cpp:169 */
     double n[3];
     face_cross(x, t->eA, t->eB, n);
     const double dd = d364(x, n[0], dn[0], n[1], dn[1], n[2], dn[2]);
@@ -344,6 +345,9 @@ static void* closest_worker(void* arg)
                     face_cross(x, w->eA, w->eB, fn);
                     double n1[3];
                     normalize_x(x, fn, n1);
+                    /* TriangleMeshGeometryIndexed.cpp:170-188 */
+                    if ((j->opts & RISE_ORACLE_DOUBLE_SIDED) && d364(x, n1[0], dn[0], n1[1], dn[1], n1[2], dn[2]) > 0.0)
+                        for (int c = 0; c < 3; ++c) n1[c] = -n1[c];
                     normalize_x(x, n1, j->normal + 3 * i);
                     j->point[3 * i + 0] = px;
                     j->point[3 * i + 1] = py;
@@ -542,6 +546,7 @@ static void xvector(int x, const double* m, const double v[3], double out[3])
 typedef struct {
     const tri64* tris;          /* all meshes, concatenated */
     const int64_t* mesh_off;    /* n_meshes + 1 offsets into tris */
+    const uint8_t* double_sided; /* per mesh, or NULL */
     const rise_oracle_object* objs;
     int n_objs;
     const float* rays;
@@ -657,6 +662,7 @@ static void* objects_closest_worker(void* arg)
             uint8_t flag = 0;
             for (int k = 0; k < j->n_objs; ++k) {
                 const rise_oracle_object* ob = &j->objs[k];
+                const int mopts = j->opts | ((j->double_sided && j->double_sided[ob->mesh]) ? RISE_ORACLE_DOUBLE_SIDED : 0);
                 double op[3], dn[3], factor;
                 object_ray(x, ob, o, d, op, dn, &factor);
                 float fo[3], fd[3];
@@ -677,7 +683,7 @@ static void* objects_closest_worker(void* arg)
                     double tt;
                     const tri64* t = &j->tris[q];
                     if (!nof && !filter_x(x, fo, fd, t->p0, t->e1, t->e2, INFINITY)) continue;
-                    if (!face_selected(x, j->opts, t, dn)) continue;
+                    if (!face_selected(x, mopts, t, dn)) continue;
                     if (ray_triangle_x(x, op, dn, t->v0, t->eA, t->eB, &tt, NULL, NULL) && tt < bt) {
                         bt = tt;
                         bk = q;
@@ -711,6 +717,9 @@ static void* objects_closest_worker(void* arg)
                     face_cross(x, t->eA, t->eB, fn);
                     double n1[3], nw[3];
                     normalize_x(x, fn, n1);
+                    /* TriangleMeshGeometryIndexed.cpp:170-188, in the object's frame */
+                    if ((mopts & RISE_ORACLE_DOUBLE_SIDED) && d364(x, n1[0], dn[0], n1[1], dn[1], n1[2], dn[2]) > 0.0)
+                        for (int c = 0; c < 3; ++c) n1[c] = -n1[c];
                     const double* mi = ob->to_object;
                     for (int c = 0; c < 3; ++c)
                         nw[c] = d364(x, mi[4 * c], n1[0], mi[4 * c + 1], n1[1], mi[4 * c + 2], n1[2]);
@@ -752,6 +761,7 @@ static void* objects_any_worker(void* arg)
              * any object occluding within its own scaled limit. */
             for (int k = 0; k < j->n_objs && !occ; ++k) {
                 const rise_oracle_object* ob = &j->objs[k];
+                const int mopts = j->opts | ((j->double_sided && j->double_sided[ob->mesh]) ? RISE_ORACLE_DOUBLE_SIDED : 0);
                 double op[3], dn[3], factor;
                 object_ray(x, ob, o, d, op, dn, &factor);
                 /* Object.cpp:688-693 */
@@ -775,7 +785,7 @@ static void* objects_any_worker(void* arg)
                     double tt;
                     const tri64* t = &j->tris[q];
                     if (!nof && !filter_x(x, fo, fd, t->p0, t->e1, t->e2, fbest)) continue;
-                    if (!face_selected(x, j->opts, t, dn)) continue;
+                    if (!face_selected(x, mopts, t, dn)) continue;
                     if (ray_triangle_x(x, op, dn, t->v0, t->eA, t->eB, &tt, NULL, NULL) && tt > NZ &&
                         tt < how_far2)
                         occ = 1;
@@ -803,7 +813,8 @@ static void orun(ojob* j, void* (*fn)(void*), int n_threads)
 }
 
 void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, int n_meshes,
-                                 const rise_oracle_object* objs, int n_objs, const float* rays, int64_t n_rays,
+                                 const uint8_t* double_sided, const rise_oracle_object* objs, int n_objs,
+                                 const float* rays, int64_t n_rays,
                                  int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
                                  double* normal, double* point, uint8_t* flags, int opts, int n_threads)
 {
@@ -812,6 +823,7 @@ void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, in
     tri64* tt = expand_tris_f64(tris, mesh_off[n_meshes]);
     j.tris = tt;
     j.mesh_off = mesh_off;
+    j.double_sided = double_sided;
     j.objs = objs;
     j.n_objs = n_objs;
     j.rays = rays;
@@ -829,15 +841,16 @@ void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, in
     free(tt);
 }
 
-void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const rise_oracle_object* objs,
-                             int n_objs, const float* rays, const float* tmax, int64_t n_rays, uint8_t* occluded,
-                             int opts, int n_threads)
+void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const uint8_t* double_sided,
+                             const rise_oracle_object* objs, int n_objs, const float* rays, const float* tmax,
+                             int64_t n_rays, uint8_t* occluded, int opts, int n_threads)
 {
     ojob j;
     memset(&j, 0, sizeof(j));
     tri64* tt = expand_tris_f64(tris, mesh_off[n_meshes]);
     j.tris = tt;
     j.mesh_off = mesh_off;
+    j.double_sided = double_sided;
     j.objs = objs;
     j.n_objs = n_objs;
     j.rays = rays;

@@ -39,6 +39,7 @@ class MeshDesc(ctypes.Structure):
         ("n_tris", ctypes.c_int64),
         ("tri_ids", ctypes.c_void_p),
         ("vertices_f64", ctypes.c_void_p),
+        ("flags", ctypes.c_uint32),
     ]
 
 

@@ -42,8 +42,10 @@ struct alignas(RISE_NODE_STRIDE) NodeSlot : Node8 {
 static_assert(sizeof(NodeSlot) == RISE_NODE_STRIDE, "node stride");
 
 // 48 bytes: three float4 loads.  w of v0 carries the caller's triangle id
-// (bit pattern of an int32), w of v1 / v2 is 0 (path-tracing scenes put the
-// object id in v1.w and flags in v2.w).
+// (bit pattern of an int32), w of v1 the mesh or object index, w of v2 flags
+// (kTriNoShadow: the object casts no shadow; kTriDoubleSided: the mesh is
+// double-sided).
+constexpr unsigned kTriDoubleSided = 2u;
 struct alignas(16) Tri48 {
     float v0[4];
     float v1[4];

@@ -130,6 +130,8 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
         if (meshes[m].n_tris > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh over 2^31-1 triangles");
         if (meshes[m].n_tris > 0 && !meshes[m].vertices && !meshes[m].vertices_f64)
             return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh vertices is NULL");
+        if (meshes[m].flags & ~RISE_HIP_MESH_DOUBLE_SIDED)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "unknown mesh flags");
     }
     int64_t world_total = 0;
     for (size_t k = 0; k < n_objects; ++k) {
@@ -172,7 +174,10 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             int32_t p;
             std::memcpy(&p, &tr.v0[3], 4);
             std::memcpy(&tr.v0[3], &ids[(size_t)p], 4);
-            std::memcpy(&tr.v1[3], &obj_of[(size_t)p], 4);
+            const int32_t k = obj_of[(size_t)p];
+            std::memcpy(&tr.v1[3], &k, 4);
+            const uint32_t fl = (meshes[objects[k].mesh].flags & RISE_HIP_MESH_DOUBLE_SIDED) ? kTriDoubleSided : 0u;
+            std::memcpy(&tr.v2[3], &fl, 4);
         }
         // Transformed objects: one object-space tree per mesh they use.
         std::vector<int32_t> blas_of(n_meshes, -1);
@@ -190,12 +195,14 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
                 if (!build_wide_bvh(mv.data(), m.n_tris, nullptr, BuildParams(), b, err))
                     return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
                 out.blas_tri64.push_back(leaf_vertices(b, mv64));
+                const uint32_t fl = (m.flags & RISE_HIP_MESH_DOUBLE_SIDED) ? kTriDoubleSided : 0u;
                 for (Tri48& tr : b.tris) {
                     int32_t p;
                     std::memcpy(&p, &tr.v0[3], 4);
                     const int32_t id = m.tri_ids ? m.tri_ids[p] : p;
                     std::memcpy(&tr.v0[3], &id, 4);
                     std::memcpy(&tr.v1[3], &ob.mesh, 4);
+                    std::memcpy(&tr.v2[3], &fl, 4);
                 }
                 blas_of[ob.mesh] = (int32_t)out.blas.size();
                 out.total_tris += m.n_tris;

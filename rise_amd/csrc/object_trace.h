@@ -270,7 +270,7 @@ RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6]
     const BlasView bl = sc.blas[ob.blas];
     double v[9], ab[2], n1[3], nw[3], n2[3];
     triangle_vertices<ObjRayT<A>>(bl.tris + h.leaf, bl.tri64 + 9 * (size_t)h.leaf, v);
-    hit_face(q, v, ab, n1);
+    hit_face(q, v, tri_double_sided(bl.tris + h.leaf), ab, n1);
     xform_normal<A>(ob.to_object, n1, nw);
     normalize_model<A>(nw, n2);
     out.alpha = ab[0];

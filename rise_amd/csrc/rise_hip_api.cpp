@@ -108,6 +108,8 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
         if (meshes[m].n_tris < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative triangle count");
         if (meshes[m].n_tris > 0 && !meshes[m].vertices && !meshes[m].vertices_f64)
             return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "mesh vertices is NULL");
+        if (meshes[m].flags & ~RISE_HIP_MESH_DOUBLE_SIDED)
+            return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "unknown mesh flags");
         total += meshes[m].n_tris;
         any64 = any64 || (meshes[m].n_tris > 0 && meshes[m].vertices_f64);
     }
@@ -151,7 +153,10 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
             int32_t p;
             std::memcpy(&p, &tr.v0[3], 4);
             std::memcpy(&tr.v0[3], &ids[(size_t)p], 4);
-            std::memcpy(&tr.v1[3], &mesh_of[(size_t)p], 4);
+            const int32_t m = mesh_of[(size_t)p];
+            std::memcpy(&tr.v1[3], &m, 4);
+            const uint32_t fl = (meshes[m].flags & RISE_HIP_MESH_DOUBLE_SIDED) ? kTriDoubleSided : 0u;
+            std::memcpy(&tr.v2[3], &fl, 4);
         }
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");

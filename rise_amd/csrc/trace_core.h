@@ -660,7 +660,9 @@ RISE_HD inline bool closest_step(const R& r, const NodeSlot* __restrict__ nodes,
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
         const double* v64 = F64 ? tri64 + 9 * (size_t)(tbase + j) : nullptr;
-        if (CULL && s.faces != 3u && !(F64 ? face_ok64(r, v64, s.faces) : face_ok(r, a, b, c, s.faces))) continue;
+        if (CULL && s.faces != 3u && !(float_to_bits(c.w) & kTriDoubleSided) &&
+            !(F64 ? face_ok64(r, v64, s.faces) : face_ok(r, a, b, c, s.faces)))
+            continue;
         double tt;
         if (filter_float(r, a, b, c, (float)s.best_t) && (F64 ? certify64(r, v64, tt) : certify(r, a, b, c, tt))) {
             const int id = float_to_bits(a.w);
@@ -751,7 +753,9 @@ RISE_HD inline bool any_step(const R& r, const NodeSlot* __restrict__ nodes, con
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++n_tris;
         const double* v64 = F64 ? tri64 + 9 * (size_t)(tbase + j) : nullptr;
-        if (CULL && s.faces != 3u && !(F64 ? face_ok64(r, v64, s.faces) : face_ok(r, a, b, c, s.faces))) continue;
+        if (CULL && s.faces != 3u && !(float_to_bits(c.w) & kTriDoubleSided) &&
+            !(F64 ? face_ok64(r, v64, s.faces) : face_ok(r, a, b, c, s.faces)))
+            continue;
         double tt;
         if (filter_float(r, a, b, c, s.fbest) && (F64 ? certify64(r, v64, tt) : certify(r, a, b, c, tt)) &&
             tt > kNearZero && tt < s.how_far2 &&
@@ -804,9 +808,11 @@ RISE_HD inline void triangle_vertices(const Tri48* __restrict__ tri, const doubl
 }
 
 // The certifier's barycentrics and the face normal (v1 - v0) x (v2 - v0)
-// normalized once by the mesh (...Specializations.h:196-205).
+// normalized once by the mesh (...Specializations.h:196-205), turned toward
+// the ray's origin on a double-sided mesh (TriangleMeshGeometryIndexed.cpp:
+// 170-188: negated when its dot product with the direction is positive).
 template <class R>
-RISE_HD inline void hit_face(const R& r, const double v[9], double ab[2], double n1[3])
+RISE_HD inline void hit_face(const R& r, const double v[9], bool double_sided, double ab[2], double n1[3])
 {
     constexpr int A = R::kArith;
     double tt = 0.0;
@@ -815,6 +821,13 @@ RISE_HD inline void hit_face(const R& r, const double v[9], double ab[2], double
     double fn[3];
     face_cross<A>(v[3] - v[0], v[4] - v[1], v[5] - v[2], v[6] - v[0], v[7] - v[1], v[8] - v[2], fn);
     normalize_model<A>(fn, n1);
+    if (double_sided && dot3<A>(n1[0], dn0(r), n1[1], dn1(r), n1[2], dn2(r)) > 0.0)
+        for (int k = 0; k < 3; ++k) n1[k] = -n1[k];
+}
+
+RISE_HD inline bool tri_double_sided(const Tri48* __restrict__ tri)
+{
+    return (float_to_bits(reinterpret_cast<const float4*>(tri)[2].w) & kTriDoubleSided) != 0;
 }
 
 template <int A>
@@ -823,7 +836,7 @@ RISE_HD inline void hit_attributes(const RayStateT<A>& r, const Tri48* __restric
 {
     double v[9], n1[3];
     triangle_vertices<RayStateT<A>>(tri, v64, v);
-    hit_face(r, v, ab, n1);
+    hit_face(r, v, tri_double_sided(tri), ab, n1);
     normalize_model<A>(n1, n_out);
     const double s = t - kSurfaceBackoff;
     const double d[3] = {dn0(r), dn1(r), dn2(r)};

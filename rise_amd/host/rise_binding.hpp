@@ -92,6 +92,12 @@ protected:
     ~HipAcceleratedObjectManager() override = default;
 
 private:
+    // Reads a mesh's protected bDoubleSided (TriangleMeshGeometryIndexed.h:72)
+    // through a pointer to member taken in a derived class's scope.
+    struct Sidedness : public TriangleMeshGeometryIndexed {
+        static bool double_sided(const TriangleMeshGeometryIndexed& m) { return m.*(&Sidedness::bDoubleSided); }
+    };
+
     // Collects the visible objects in the manager's order.
     struct Collect : public IEnumCallback<IObjectPriv> {
         std::vector<const IObjectPriv*>* out;
@@ -128,7 +134,8 @@ private:
                     v.push_back(t.pVertices[c3]->y);
                     v.push_back(t.pVertices[c3]->z);
                 }
-            const int m = gpu_->AddMeshF64(v.data(), (int64_t)(v.size() / 9));
+            const int m = gpu_->AddMeshF64(v.data(), (int64_t)(v.size() / 9), nullptr,
+                                           Sidedness::double_sided(*mesh) ? RISE_HIP_MESH_DOUBLE_SIDED : 0u);
             double tw[16], to[16];
             ToArray(objects_[k]->GetFinalTransformMatrix(), tw);
             ToArray(objects_[k]->GetFinalInverseTransformMatrix(), to);

@@ -55,17 +55,17 @@ public:
     // Adds a mesh (n_tris * 9 floats); copied at PrepareForRendering.  Without
     // objects it is world-space geometry; with AddObject it is placed by the
     // objects that refer to it.  Returns its index.
-    int AddMesh(const float* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr)
+    int AddMesh(const float* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr, uint32_t flags = 0)
     {
-        meshes_.push_back({vertices, n_tris, tri_ids, nullptr});
+        meshes_.push_back({vertices, n_tris, tri_ids, nullptr, flags});
         return (int)meshes_.size() - 1;
     }
 
     // The same with the mesh's double vertices (n_tris * 9), as RISE holds
     // them: the certifier uses them, the float filter their float casts.
-    int AddMeshF64(const double* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr)
+    int AddMeshF64(const double* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr, uint32_t flags = 0)
     {
-        meshes_.push_back({nullptr, n_tris, tri_ids, vertices});
+        meshes_.push_back({nullptr, n_tris, tri_ids, vertices, flags});
         return (int)meshes_.size() - 1;
     }
 

@@ -96,7 +96,7 @@ class ObjectManager:
     (ARITH_IEEE or ARITH_X86_FMA; rise_hip_scene_set_arithmetic)."""
 
     def __init__(self, arith: int = ARITH_IEEE) -> None:
-        self._meshes: list[tuple[np.ndarray, np.ndarray | None]] = []
+        self._meshes: list[tuple[np.ndarray, np.ndarray | None, bool]] = []
         self._objects: list[tuple[int, np.ndarray, np.ndarray]] = []
         self._scene = ctypes.c_void_p(None)
         self._arith = arith
@@ -118,10 +118,11 @@ class ObjectManager:
         self._objects.append((int(mesh), tw, to))
         return len(self._objects) - 1
 
-    def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None) -> None:
+    def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None, double_sided: bool = False) -> None:
         """tris: (n, 9) float32 world-space triangles, or float64 for a mesh
         whose double vertices the certifier uses (rise_hip_mesh_desc.
-        vertices_f64); ids: optional (n,) int32."""
+        vertices_f64); ids: optional (n,) int32; double_sided: the mesh is
+        double-sided (RISE_HIP_MESH_DOUBLE_SIDED)."""
         if self._from_cache:
             raise RuntimeError("a scene loaded from a cache is complete; it takes no more meshes")
         f64 = np.asarray(tris).dtype == np.float64
@@ -129,7 +130,7 @@ class ObjectManager:
         i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
         if i is not None and i.shape[0] != t.shape[0]:
             raise ValueError("ids must have one entry per triangle")
-        self._meshes.append((t, i))
+        self._meshes.append((t, i, bool(double_sided)))
 
     def prepare_for_rendering(self) -> None:
         if self._from_cache:
@@ -137,8 +138,8 @@ class ObjectManager:
         L = _lib.lib()
         self.release()
         descs = (_lib.MeshDesc * max(1, len(self._meshes)))()
-        for k, (t, i) in enumerate(self._meshes):
-            _set_desc(descs[k], t, i)
+        for k, (t, i, ds) in enumerate(self._meshes):
+            _set_desc(descs[k], t, i, ds)
         out = ctypes.c_void_p(None)
         if self._objects:
             objs = make_objects(self._objects)
@@ -267,8 +268,9 @@ class ObjectManager:
             pass
 
 
-def _set_desc(desc, t: np.ndarray, ids) -> None:
+def _set_desc(desc, t: np.ndarray, ids, double_sided: bool = False) -> None:
     """Fills a rise_hip_mesh_desc from float32 or float64 (double vertices) triangles."""
+    desc.flags = 1 if double_sided else 0
     if t.dtype == np.float64:
         desc.vertices = None
         desc.vertices_f64 = t.ctypes.data
@@ -345,7 +347,7 @@ def host_trace_ex(tris: np.ndarray, rays: np.ndarray, mode: int = 0, faces: int 
 
 
 def host_trace_objects(meshes, objects, rays: np.ndarray, mode: int = 0, faces: int = FACES_BOTH,
-                       tmax: np.ndarray | None = None, arith: int = ARITH_IEEE, ids=None):
+                       tmax: np.ndarray | None = None, arith: int = ARITH_IEEE, ids=None, double_sided=None):
     """rise_hip_debug_objects_host: a scene of objects (meshes: list of (n, 9)
     float32 arrays, or float64 for double vertices; objects: OBJECT_DTYPE array) queried with the kernels'
     per-ray code on host threads (validation only).  Mode 0: hit records
@@ -358,7 +360,7 @@ def host_trace_objects(meshes, objects, rays: np.ndarray, mode: int = 0, faces: 
                for k in range(len(arrs))]
     descs = (_lib.MeshDesc * max(1, len(arrs)))()
     for k, t in enumerate(arrs):
-        _set_desc(descs[k], t, id_arrs[k])
+        _set_desc(descs[k], t, id_arrs[k], bool(double_sided[k]) if double_sided is not None else False)
     objs = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]

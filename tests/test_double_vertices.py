@@ -104,3 +104,57 @@ def test_transformed_double_meshes(arith):
     assert np.array_equal(h["prim_id"], o["prim_id"]) and np.array_equal(h["mesh"], o["object"])
     assert np.array_equal(h["range"][hit], o["range"][hit])
     assert np.array_equal(h["point"][hit], o["point"][hit].astype(np.float32))
+
+
+@needs_ref
+@pytest.mark.parametrize("arith", ["ieee", "fast"])
+def test_double_sided_mesh_matches_reference(arith):
+    """A double-sided mesh (TriangleMeshGeometryIndexed.cpp:151-205): face
+    flags do not apply, and the record's normal faces the ray."""
+    tris = _double_soup(3000, 17)
+    rays = _rays(tris, 18)
+    plain = oracle.RefMesh(tris, variant=arith, double_sided=True)
+    bary = oracle.RefMesh(tris, variant=arith, bary=True, double_sided=True)
+    one = oracle.RefMesh(tris, variant=arith)
+    ident = raycast.make_objects([(0, *raycast.object_matrices())])
+    for faces in FACES:
+        o = oracle.closest_attr(tris, rays, faces=faces, arith=arith, double_sided=True)
+        a = plain.closest_ex(rays, faces)
+        b = bary.closest_ex(rays, faces)
+        hit = o["prim_id"] >= 0
+        assert np.array_equal(o["prim_id"], a["prim_id"]) and np.array_equal(a["prim_id"], b["prim_id"])
+        for k in ("range", "point", "normal"):
+            assert np.array_equal(o[k][hit], a[k][hit]), k
+        # the bary mesh's interpolated shading normal (1 - a - b, a, b) is
+        # turned toward the ray as well, so it holds +-a
+        assert np.array_equal(np.abs(o["alpha"][hit]), np.abs(b["alpha"][hit]))
+        d = rays[3:6].T.astype(np.float64)
+        assert (np.sum(o["normal"][hit] * d[hit], axis=1) <= 0).all()
+        assert np.array_equal(oracle.any_hit(tris, rays, faces=faces, arith=arith, double_sided=True),
+                              plain.any_ex(rays, faces=faces))
+        h = raycast.host_trace_objects([tris], ident, rays, faces=faces, arith=ARITH[arith], double_sided=[True])
+        assert np.array_equal(h["prim_id"], o["prim_id"]) and np.array_equal(h["range"][hit], o["range"][hit])
+        assert np.array_equal(h["geom_normal"][hit], o["normal"][hit].astype(np.float32))
+    # single-sided, the face flags cut hits
+    assert (one.closest_ex(rays, raycast.FACES_FRONT)["prim_id"] >= 0).sum() < hit.sum()
+
+
+@needs_ref
+def test_mixed_sidedness_objects():
+    meshes, objs = scenes.instanced_scene(tris_per_mesh=400, seed=19)
+    rays = scenes.object_rays(meshes, objs, 5000, seed=20)
+    d = rays[3:6].astype(np.float64)
+    rays[3:6] = (d / np.linalg.norm(d, axis=0)).astype(np.float32)
+    sides = [True, False, True]
+    ref = [oracle.RefMesh(m, variant="ieee", double_sided=sd) for m, sd in zip(meshes, sides)]
+    for faces in FACES:
+        o = oracle.objects_closest(meshes, objs, rays, faces=faces, double_sided=sides)
+        r = oracle.ref_objects_closest(ref, objs, rays, faces=faces)
+        h = raycast.host_trace_objects(meshes, objs, rays, faces=faces, double_sided=sides)
+        hit = o["prim_id"] >= 0
+        assert np.array_equal(o["prim_id"], r["prim_id"]) and np.array_equal(o["object"], r["object"])
+        assert np.array_equal(o["normal"][hit], r["normal"][hit]) and np.array_equal(o["range"][hit], r["range"][hit])
+        assert np.array_equal(h["prim_id"], o["prim_id"]) and np.array_equal(h["mesh"], o["object"])
+        assert np.array_equal(h["geom_normal"][hit], o["normal"][hit].astype(np.float32))
+        assert np.array_equal(oracle.objects_any(meshes, objs, rays, faces=faces, double_sided=sides),
+                              oracle.ref_objects_any(ref, objs, rays, faces=faces))

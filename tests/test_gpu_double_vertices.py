@@ -84,3 +84,48 @@ def test_gpu_transformed_double_meshes(cuda):
     assert np.array_equal(h["geom_normal"][hit], o["normal"][hit].astype(np.float32))
     assert np.array_equal(om.intersect_shadow_ray(r).cpu().numpy(), oracle.objects_any(meshes, objs, rays))
     om.release()
+
+
+def test_gpu_double_sided(cuda):
+    """Double-sided meshes (RISE_HIP_MESH_DOUBLE_SIDED): face selection does
+    not apply to them and the record's normal faces the ray; a world-space
+    scene of one such mesh, and objects mixing both kinds."""
+    import torch
+
+    tris = _double_soup(20000, 41)
+    rays = np.ascontiguousarray(np.concatenate([scenes.pinhole_rays(80, 60), scenes.incoherent_rays(8000, 42)], 1),
+                                np.float32)
+    om = raycast.ObjectManager()
+    om.add_mesh(tris, double_sided=True)
+    om.prepare_for_rendering()
+    r = torch.from_numpy(rays).to(cuda)
+    for faces in (raycast.FACES_BOTH, raycast.FACES_FRONT, raycast.FACES_BACK):
+        log_case(f"double-sided mesh: 20000 tris, faces {faces}")
+        g = raycast.hit_fields(om.intersect_ray_ex(r, faces))
+        o = oracle.closest_attr(tris, rays, faces=faces, double_sided=True)
+        hit = o["prim_id"] >= 0
+        assert np.array_equal(g["prim_id"], o["prim_id"]) and np.array_equal(g["range"][hit], o["range"][hit])
+        assert np.array_equal(g["geom_normal"][hit], o["normal"][hit].astype(np.float32))
+        assert np.array_equal(om.intersect_shadow_ray_ex(r, faces).cpu().numpy(),
+                              oracle.any_hit(tris, rays, faces=faces, double_sided=True))
+    om.release()
+    meshes, objs = scenes.instanced_scene(tris_per_mesh=1500, seed=43)
+    sides = [True, False, True]
+    orays = scenes.object_rays(meshes, objs, 8000, seed=44)
+    om = raycast.ObjectManager()
+    for m, sd in zip(meshes, sides):
+        om.add_mesh(m, double_sided=sd)
+    for ob in objs:
+        om.add_object(int(ob["mesh"]), ob["to_world"], ob["to_object"])
+    om.prepare_for_rendering()
+    r = torch.from_numpy(orays).to(cuda)
+    for faces in (raycast.FACES_FRONT, raycast.FACES_BACK):
+        log_case(f"double-sided objects: faces {faces}")
+        g = raycast.hit_fields(om.intersect_ray_ex(r, faces))
+        o = oracle.objects_closest(meshes, objs, orays, faces=faces, double_sided=sides)
+        hit = o["prim_id"] >= 0
+        assert np.array_equal(g["prim_id"], o["prim_id"]) and np.array_equal(g["mesh"], o["object"])
+        assert np.array_equal(g["geom_normal"][hit], o["normal"][hit].astype(np.float32))
+        assert np.array_equal(om.intersect_shadow_ray_ex(r, faces).cpu().numpy(),
+                              oracle.objects_any(meshes, objs, orays, faces=faces, double_sided=sides))
+    om.release()
