@@ -71,7 +71,14 @@ typedef struct rise_hip_scene rise_hip_scene;
  *               (TriangleMeshGeometryIndexed's bDoubleSided,
  *               TriangleMeshGeometryIndexed.cpp:151-205): the face selection
  *               of the queries does not apply to it, and a hit record's
- *               normal is turned toward the ray's origin. */
+ *               normals are turned toward the ray's origin.
+ * normals:      host, optional n_tris * 9 doubles: each triangle's three
+ *               vertex normals (the mesh's pNormals of its corners); a hit
+ *               record's shading normal interpolates them
+ *               (...Specializations.h:194-196), else it is the face normal.
+ * coords:       host, optional n_tris * 6 doubles: each triangle's three
+ *               texture coordinates (pCoords); a hit record's uv
+ *               interpolates them (:214-216), else it is 0. */
 #define RISE_HIP_MESH_DOUBLE_SIDED 1u
 typedef struct rise_hip_mesh_desc {
     const float* vertices;
@@ -79,6 +86,8 @@ typedef struct rise_hip_mesh_desc {
     const int32_t* tri_ids;
     const double* vertices_f64;
     uint32_t flags;
+    const double* normals;
+    const double* coords;
 } rise_hip_mesh_desc;
 
 typedef struct rise_hip_scene_stats {
@@ -200,15 +209,17 @@ typedef enum rise_hip_faces {
 
 /* The intersection record RISE's RayIntersectionGeometric holds after
  * Object::IntersectRay (Intersection/RayIntersectionGeometric.h:89-188;
- * Objects/Object.cpp:528-649) for a mesh without vertex normals or texture
- * coordinates under an identity transform. */
+ * Objects/Object.cpp:528-649): 80 bytes. */
 typedef struct rise_hip_hit {
     int32_t prim_id;        /* closest triangle id (as rise_hip_intersect_closest), -1 on a miss */
     int32_t mesh;           /* index of the triangle's mesh in rise_hip_scene_upload's list, -1 on a miss */
     double range;           /* world distance (ri.range after Object.cpp:641-643), +inf on a miss */
     double alpha, beta;     /* barycentrics of the double-precision certifier (RayTriangleIntersection.cpp) */
     float point[3];         /* world hit point o + dn (t - 1e-12) (ri.ptIntersection) */
-    float geom_normal[3];   /* unit face normal (ri.vGeomNormal; equal to ri.vNormal for these meshes) */
+    float geom_normal[3];   /* unit face normal on the shading normal's side (ri.vGeomNormal) */
+    float shading_normal[3]; /* unit shading normal (ri.vNormal): the vertex normals interpolated, or the
+                                face normal */
+    float uv[2];            /* texture coordinates interpolated (ri.ptCoord), 0 without them */
 } rise_hip_hit;
 
 /* rise_hip_intersect_closest with a face selection (faces: rise_hip_faces)

@@ -200,29 +200,55 @@ def _sided(double_sided, n):
     return np.ascontiguousarray(np.asarray(double_sided, dtype=np.uint8).reshape(n))
 
 
+def _mesh_attrs(meshes, double_sided, normals, coords):
+    """Per-mesh flags and the normals / coordinates aligned with _meshes."""
+    n = len(meshes)
+    fl = np.zeros(n, np.uint8)
+    nrm, crd = [], []
+    for k, m in enumerate(meshes):
+        nt = np.asarray(m).reshape(-1, 9).shape[0]
+        if double_sided is not None and double_sided[k]:
+            fl[k] |= 1
+        if normals is not None and normals[k] is not None:
+            fl[k] |= 2
+            nrm.append(np.asarray(normals[k], np.float64).reshape(nt, 9))
+        else:
+            nrm.append(np.zeros((nt, 9)))
+        if coords is not None and coords[k] is not None:
+            fl[k] |= 4
+            crd.append(np.asarray(coords[k], np.float64).reshape(nt, 6))
+        else:
+            crd.append(np.zeros((nt, 6)))
+    cat = lambda xs, w: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros((0, w)))
+    return fl, cat(nrm, 9), cat(crd, 6)
+
+
 def objects_closest(meshes, objects, rays: np.ndarray, faces: int = 3, arith: str = "ieee", threads: int = 0,
-                    flags: bool = False, prehit: bool = False, double_sided=None) -> dict:
+                    flags: bool = False, prehit: bool = False, double_sided=None, normals=None, coords=None) -> dict:
     """Closest hit over a scene of objects (rise_oracle_objects_closest):
     prim_id (index within the mesh), object, range, alpha, beta, normal (n, 3),
     point (n, 3), and with flags=True "flags" (traversal-order dependent).
     prehit: model Object::IntersectRay's box pre-test (RISE_ORACLE_OBJ_PREHIT);
-    double_sided: None or one bool per mesh."""
+    double_sided: None or one bool per mesh; normals / coords: None or one
+    (n, 9) / (n, 6) array (or None) per mesh of per-corner vertex normals and
+    texture coordinates.  Also returns "shading" (n, 3) and "uv" (n, 2)."""
     L = oracle_lib()
-    L.rise_oracle_objects_closest.argtypes = [_P, _P, ctypes.c_int, _P, _P, ctypes.c_int, _P, _I64] + [_P] * 8 + [
-        ctypes.c_int, ctypes.c_int]
+    L.rise_oracle_objects_closest.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, _P, _I64] + \
+        [_P] * 10 + [ctypes.c_int, ctypes.c_int]
     tris, off = _meshes(meshes)
-    ds = _sided(double_sided, len(off) - 1)
+    mfl, nrm, crd = _mesh_attrs(meshes, double_sided, normals, coords)
     obj = _objects(objects)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
     out = {"prim_id": np.empty(n, np.int32), "object": np.empty(n, np.int32), "range": np.empty(n),
-           "alpha": np.empty(n), "beta": np.empty(n), "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
+           "alpha": np.empty(n), "beta": np.empty(n), "normal": np.empty((n, 3)), "point": np.empty((n, 3)),
+           "shading": np.empty((n, 3)), "uv": np.empty((n, 2))}
     fl = np.empty(n, np.uint8) if flags else None
-    L.rise_oracle_objects_closest(tris.ctypes.data, off.ctypes.data, len(off) - 1,
-                                  None if ds is None else ds.ctypes.data, obj.ctypes.data, len(obj),
+    L.rise_oracle_objects_closest(tris.ctypes.data, off.ctypes.data, len(off) - 1, mfl.ctypes.data,
+                                  nrm.ctypes.data, crd.ctypes.data, obj.ctypes.data, len(obj),
                                   rays.ctypes.data, n,
                                   *[out[k].ctypes.data for k in ("prim_id", "object", "range", "alpha", "beta",
-                                                                 "normal", "point")],
+                                                                 "normal", "point", "shading", "uv")],
                                   None if fl is None else fl.ctypes.data,
                                   _opts(False, arith) | _faces_opts(faces) | (OBJ_PREHIT if prehit else 0), threads)
     if flags:
@@ -263,12 +289,14 @@ class RefMesh:
     double certifier) built from its own sources (oracle/_ref)."""
 
     def __init__(self, tris: np.ndarray | None, variant: str = "ieee", risemesh: str | None = None,
-                 bary: bool = False, double_sided: bool = False):
+                 bary: bool = False, double_sided: bool = False, normals=None, vcoords=None):
         """Builds the mesh from `tris` (float32, or float64 kept as double
         vertices), or (tris None) loads the .risemesh file
         `risemesh` with the reference's Deserialize.  bary: vertex normals
         that make the interpolated shading normal carry the barycentrics
-        (rise_ref_mesh_create_bary).  double_sided: the mesh's bDoubleSided."""
+        (rise_ref_mesh_create_bary).  double_sided: the mesh's bDoubleSided.
+        normals (n, 9) / vcoords (n, 3): per-corner vertex normals and v
+        texture coordinates (rise_ref_mesh_create_attr; u carries the id)."""
         name = "libriseref_ieee.so" if variant == "ieee" else "libriseref.so"
         L = _load(REF_DIR / name)
         L.rise_ref_mesh_create.argtypes = [_P, _I64]
@@ -289,9 +317,20 @@ class RefMesh:
         self._L = L
         L.rise_ref_mesh_create_f64.argtypes = [_P, _I64, ctypes.c_int, ctypes.c_int]
         L.rise_ref_mesh_create_f64.restype = _P
+        L.rise_ref_mesh_create_attr.argtypes = [_P, _P, _P, _I64, ctypes.c_int]
+        L.rise_ref_mesh_create_attr.restype = _P
         if tris is None:
             self._tris = None
             self._h = L.rise_ref_risemesh_load(str(risemesh).encode())
+        elif normals is not None or vcoords is not None:
+            self._tris = np.ascontiguousarray(np.asarray(tris).astype(np.float64)).reshape(-1, 9)
+            nt = self._tris.shape[0]
+            self._nrm = None if normals is None else np.ascontiguousarray(normals, np.float64).reshape(nt, 9)
+            self._vc = None if vcoords is None else np.ascontiguousarray(vcoords, np.float64).reshape(nt, 3)
+            self._h = L.rise_ref_mesh_create_attr(self._tris.ctypes.data,
+                                                  None if self._nrm is None else self._nrm.ctypes.data,
+                                                  None if self._vc is None else self._vc.ctypes.data, nt,
+                                                  int(double_sided))
         elif np.asarray(tris).dtype == np.float64 or double_sided:
             self._tris = np.ascontiguousarray(np.asarray(tris).astype(np.float64)).reshape(-1, 9)
             self._h = L.rise_ref_mesh_create_f64(self._tris.ctypes.data, self._tris.shape[0], int(bary),
@@ -370,7 +409,7 @@ def ref_objects_closest(ref_meshes, objects, rays: np.ndarray, faces: int = 3, b
     (rise_ref_objects_closest); ref_meshes: RefMesh list of one variant."""
     L = ref_meshes[0]._L
     L.rise_ref_objects_closest.argtypes = [_P, _P, _P, _P, ctypes.c_int, _P, _I64, ctypes.c_int, ctypes.c_int,
-                                           ctypes.c_int] + [_P] * 7 + [ctypes.c_int]
+                                           ctypes.c_int] + [_P] * 9 + [ctypes.c_int]
     obj = _objects(objects)
     handles = (ctypes.c_void_p * len(ref_meshes))(*[m._h for m in ref_meshes])
     om = np.ascontiguousarray(obj["mesh"], np.int32)
@@ -379,11 +418,12 @@ def ref_objects_closest(ref_meshes, objects, rays: np.ndarray, faces: int = 3, b
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
     out = {"prim_id": np.empty(n, np.int32), "object": np.empty(n, np.int32), "range": np.empty(n),
-           "alpha": np.empty(n), "beta": np.empty(n), "normal": np.empty((n, 3)), "point": np.empty((n, 3))}
+           "alpha": np.empty(n), "beta": np.empty(n), "normal": np.empty((n, 3)), "point": np.empty((n, 3)),
+           "shading": np.empty((n, 3)), "vcoord": np.empty(n)}
     L.rise_ref_objects_closest(handles, om.ctypes.data, tw.ctypes.data, to.ctypes.data, len(obj), rays.ctypes.data,
                                n, faces & 1, (faces >> 1) & 1, int(bary),
                                *[out[k].ctypes.data for k in ("prim_id", "object", "range", "alpha", "beta",
-                                                              "normal", "point")], threads)
+                                                              "normal", "point", "shading", "vcoord")], threads)
     return out
 
 

@@ -207,6 +207,38 @@ void* rise_ref_mesh_create_f64(const double* tris, int64_t n_tris, int bary, int
     return h;
 }
 
+// A mesh of double vertices with per-corner vertex normals (NULL: none, the
+// face normal shades) and texture coordinates: corner k of triangle i gets
+// the normal normals[9 i + 3 k ..] and the coordinate (i, vcoords[3 i + k]),
+// so the interpolated ri.ptCoord.x is the id and .y the interpolated v.
+void* rise_ref_mesh_create_attr(const double* tris, const double* normals, const double* vcoords, int64_t n_tris,
+                                int double_sided)
+{
+    RefMesh* h = new RefMesh;
+    h->mesh = new Implementation::TriangleMeshGeometryIndexed(double_sided != 0, normals == nullptr);
+    h->mesh->BeginIndexedTriangles();
+    for (int64_t i = 0; i < n_tris; ++i) {
+        const double* f = tris + 9 * i;
+        for (int k = 0; k < 3; ++k) {
+            h->mesh->AddVertex(Vertex(f[3 * k + 0], f[3 * k + 1], f[3 * k + 2]));
+            if (normals) {
+                const double* nn = normals + 9 * i + 3 * k;
+                h->mesh->AddNormal(Normal(nn[0], nn[1], nn[2]));
+            }
+            h->mesh->AddTexCoord(TexCoord((double)i, vcoords ? vcoords[3 * i + k] : 0.0));
+        }
+        IndexedTriangle t;
+        for (int k = 0; k < 3; ++k) {
+            t.iVertices[k] = (unsigned int)(3 * i + k);
+            t.iNormals[k] = normals ? (unsigned int)(3 * i + k) : 0u;
+            t.iCoords[k] = (unsigned int)(3 * i + k);
+        }
+        h->mesh->AddIndexedTriangle(t);
+    }
+    h->mesh->DoneIndexedTriangles();
+    return h;
+}
+
 void rise_ref_mesh_release(void* handle)
 {
     RefMesh* h = (RefMesh*)handle;
@@ -360,7 +392,7 @@ void rise_ref_any(void* handle, const float* rays, const float* tmax, int64_t n,
 void rise_ref_objects_closest(void** meshes, const int32_t* obj_mesh, const double* to_world, const double* to_object,
                               int n_objs, const float* rays, int64_t n, int front, int back, int bary,
                               int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
-                              double* normal, double* point, int n_threads)
+                              double* normal, double* point, double* shading, double* vcoord, int n_threads)
 {
     std::vector<Matrix4> M(n_objs), Mi(n_objs), MiT(n_objs);
     std::vector<BoundingBox> box(n_objs);
@@ -383,6 +415,9 @@ void rise_ref_objects_closest(void** meshes, const int32_t* obj_mesh, const doub
         range[i] = INFINITY;
         alpha[i] = beta[i] = 0.0;
         for (int c = 0; c < 3; ++c) normal[3 * i + c] = point[3 * i + c] = 0.0;
+        if (shading)
+            for (int c = 0; c < 3; ++c) shading[3 * i + c] = 0.0;
+        if (vcoord) vcoord[i] = 0.0;
         for (int k = 0; k < n_objs; ++k) {
             const Point3 op = Point3Ops::Transform(Mi[k], o);
             const Vector3 dl = Vector3Ops::Transform(Mi[k], d);
@@ -413,6 +448,13 @@ void rise_ref_objects_closest(void** meshes, const int32_t* obj_mesh, const doub
             normal[3 * i] = g.x;
             normal[3 * i + 1] = g.y;
             normal[3 * i + 2] = g.z;
+            if (shading) {
+                const Vector3 sn = Vector3Ops::Normalize(Vector3Ops::Transform(MiT[k], ri.vNormal));
+                shading[3 * i] = sn.x;
+                shading[3 * i + 1] = sn.y;
+                shading[3 * i + 2] = sn.z;
+            }
+            if (vcoord) vcoord[i] = ri.ptCoord.y;
             point[3 * i] = pw.x;
             point[3 * i + 1] = pw.y;
             point[3 * i + 2] = pw.z;

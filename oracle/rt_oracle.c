@@ -59,6 +59,25 @@ static void face_cross(int x, const double a[3], const double b[3], double n[3])
     }
 }
 
+/* p0 + (p1 - p0) a + (p2 - p0) b, one component of the vertex-normal and
+ * texture-coordinate interpolation (...Specializations.h:194-216), left to
+ * right; the Linux build contracts it as fma(p2 - p0, b, fma(p1 - p0, a, p0)). */
+static inline double interp3(int x, double p0, double p1, double p2, double a, double b)
+{
+    const double d1 = p1 - p0, d2 = p2 - p0;
+    return x ? fma(d2, b, fma(d1, a, p0)) : (p0 + d1 * a) + d2 * b;
+}
+
+/* The texture coordinate's interpolation (...Specializations.h:214-216):
+ * p0 + ((p1 - p0) a + (p2 - p0) b), the offset summed first
+ * (Point2Ops::mkPoint2 of a Vector2 sum); the Linux build contracts the sum
+ * as fma(p2 - p0, b, (p1 - p0) a). */
+static inline double interp_uv(int x, double p0, double p1, double p2, double a, double b)
+{
+    const double d1 = p1 - p0, d2 = p2 - p0;
+    return x ? p0 + fma(d2, b, d1 * a) : p0 + (d1 * a + d2 * b);
+}
+
 /* Vector3Ops::Normalize (Utilities/Math3D/VectorsOps.h:229-238):
  * mag = sqrt(x*x + y*y + z*z); if mag > 0, v * (1/mag). */
 static void normalize_x(int x, const double v[3], double out[3]);
@@ -546,7 +565,10 @@ static void xvector(int x, const double* m, const double v[3], double out[3])
 typedef struct {
     const tri64* tris;          /* all meshes, concatenated */
     const int64_t* mesh_off;    /* n_meshes + 1 offsets into tris */
-    const uint8_t* double_sided; /* per mesh, or NULL */
+    const uint8_t* mesh_flags;  /* per mesh (RISE_ORACLE_MESH_*), or NULL */
+    const double* normals;      /* per-corner vertex normals, 9 per triangle, or NULL */
+    const double* coords;       /* per-corner texture coordinates, 6 per triangle, or NULL */
+    double *shading, *uv;
     const rise_oracle_object* objs;
     int n_objs;
     const float* rays;
@@ -659,10 +681,12 @@ static void* objects_closest_worker(void* arg)
             double best_w = RISE_ORACLE_INFINITY;
             int32_t best_obj = -1, best_prim = -1;
             double best_pw[3] = {0, 0, 0}, best_a = 0, best_b = 0, best_n[3] = {0, 0, 0};
+            double best_s[3] = {0, 0, 0}, best_uv[2] = {0, 0};
             uint8_t flag = 0;
             for (int k = 0; k < j->n_objs; ++k) {
                 const rise_oracle_object* ob = &j->objs[k];
-                const int mopts = j->opts | ((j->double_sided && j->double_sided[ob->mesh]) ? RISE_ORACLE_DOUBLE_SIDED : 0);
+                const int mflags = j->mesh_flags ? j->mesh_flags[ob->mesh] : 0;
+                const int mopts = j->opts | ((mflags & RISE_ORACLE_MESH_DOUBLE_SIDED) ? RISE_ORACLE_DOUBLE_SIDED : 0);
                 double op[3], dn[3], factor;
                 object_ray(x, ob, o, d, op, dn, &factor);
                 float fo[3], fd[3];
@@ -705,25 +729,50 @@ static void* objects_closest_worker(void* arg)
                 for (int c = 0; c < 3; ++c) best_pw[c] = pw[c];
                 flag = (!nof && !filter_x(x, fo, fd, t->p0, t->e1, t->e2, (float)bt)) ? 1 : 0;
                 if (j->alpha) {
-                    /* Barycentrics of the certifier; the face normal
-                     * normalized by the mesh (...Specializations.h:196-205),
-                     * then by M^-T and Normalize (Object.cpp:531-533, with
+                    /* Barycentrics of the certifier; the shading normal (the
+                     * vertex normals interpolated, or the face normal) and the
+                     * face normal normalized and turned to the shading one's
+                     * side, the texture coordinates interpolated
+                     * (...Specializations.h:191-216); both normals turned
+                     * toward the ray on a double-sided mesh
+                     * (TriangleMeshGeometryIndexed.cpp:170-188), then taken
+                     * by M^-T and normalized (Object.cpp:531-533, with
                      * m_mxInvTranspose = Transpose(M^-1), :827). */
                     double tt, a, b;
                     ray_triangle_x(x, op, dn, t->v0, t->eA, t->eB, &tt, &a, &b);
                     best_a = a;
                     best_b = b;
-                    double fn[3];
+                    double fn[3], sn[3], gn[3];
                     face_cross(x, t->eA, t->eB, fn);
-                    double n1[3], nw[3];
-                    normalize_x(x, fn, n1);
-                    /* TriangleMeshGeometryIndexed.cpp:170-188, in the object's frame */
-                    if ((mopts & RISE_ORACLE_DOUBLE_SIDED) && d364(x, n1[0], dn[0], n1[1], dn[1], n1[2], dn[2]) > 0.0)
-                        for (int c = 0; c < 3; ++c) n1[c] = -n1[c];
+                    if ((mflags & RISE_ORACLE_MESH_NORMALS) && j->normals) {
+                        const double* nn = j->normals + 9 * bk;
+                        for (int c = 0; c < 3; ++c) sn[c] = interp3(x, nn[c], nn[3 + c], nn[6 + c], a, b);
+                    } else {
+                        for (int c = 0; c < 3; ++c) sn[c] = fn[c];
+                    }
+                    normalize_x(x, fn, gn);
+                    if (d364(x, gn[0], sn[0], gn[1], sn[1], gn[2], sn[2]) < 0.0)
+                        for (int c = 0; c < 3; ++c) gn[c] = -gn[c];
+                    if ((mflags & RISE_ORACLE_MESH_COORDS) && j->coords) {
+                        const double* cc = j->coords + 6 * bk;
+                        for (int c = 0; c < 2; ++c) best_uv[c] = interp_uv(x, cc[c], cc[2 + c], cc[4 + c], a, b);
+                    } else {
+                        best_uv[0] = best_uv[1] = 0.0;
+                    }
+                    if (mopts & RISE_ORACLE_DOUBLE_SIDED) {
+                        if (d364(x, sn[0], dn[0], sn[1], dn[1], sn[2], dn[2]) > 0.0)
+                            for (int c = 0; c < 3; ++c) sn[c] = -sn[c];
+                        if (d364(x, gn[0], dn[0], gn[1], dn[1], gn[2], dn[2]) > 0.0)
+                            for (int c = 0; c < 3; ++c) gn[c] = -gn[c];
+                    }
                     const double* mi = ob->to_object;
-                    for (int c = 0; c < 3; ++c)
-                        nw[c] = d364(x, mi[4 * c], n1[0], mi[4 * c + 1], n1[1], mi[4 * c + 2], n1[2]);
+                    double nw[3], sw[3];
+                    for (int c = 0; c < 3; ++c) {
+                        nw[c] = d364(x, mi[4 * c], gn[0], mi[4 * c + 1], gn[1], mi[4 * c + 2], gn[2]);
+                        sw[c] = d364(x, mi[4 * c], sn[0], mi[4 * c + 1], sn[1], mi[4 * c + 2], sn[2]);
+                    }
                     normalize_x(x, nw, best_n);
+                    normalize_x(x, sw, best_s);
                 }
             }
             j->prim_id[i] = best_prim;
@@ -738,6 +787,10 @@ static void* objects_closest_worker(void* arg)
                     j->point[3 * i + c] = best_pw[c];
                 }
             }
+            if (j->shading)
+                for (int c = 0; c < 3; ++c) j->shading[3 * i + c] = best_s[c];
+            if (j->uv)
+                for (int c = 0; c < 2; ++c) j->uv[2 * i + c] = best_uv[c];
         }
     }
     return NULL;
@@ -761,7 +814,8 @@ static void* objects_any_worker(void* arg)
              * any object occluding within its own scaled limit. */
             for (int k = 0; k < j->n_objs && !occ; ++k) {
                 const rise_oracle_object* ob = &j->objs[k];
-                const int mopts = j->opts | ((j->double_sided && j->double_sided[ob->mesh]) ? RISE_ORACLE_DOUBLE_SIDED : 0);
+                const int mflags = j->mesh_flags ? j->mesh_flags[ob->mesh] : 0;
+                const int mopts = j->opts | ((mflags & RISE_ORACLE_MESH_DOUBLE_SIDED) ? RISE_ORACLE_DOUBLE_SIDED : 0);
                 double op[3], dn[3], factor;
                 object_ray(x, ob, o, d, op, dn, &factor);
                 /* Object.cpp:688-693 */
@@ -813,17 +867,22 @@ static void orun(ojob* j, void* (*fn)(void*), int n_threads)
 }
 
 void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, int n_meshes,
-                                 const uint8_t* double_sided, const rise_oracle_object* objs, int n_objs,
-                                 const float* rays, int64_t n_rays,
+                                 const uint8_t* mesh_flags, const double* normals, const double* coords,
+                                 const rise_oracle_object* objs, int n_objs, const float* rays, int64_t n_rays,
                                  int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
-                                 double* normal, double* point, uint8_t* flags, int opts, int n_threads)
+                                 double* normal, double* point, double* shading, double* uv, uint8_t* flags,
+                                 int opts, int n_threads)
 {
     ojob j;
     memset(&j, 0, sizeof(j));
     tri64* tt = expand_tris_f64(tris, mesh_off[n_meshes]);
     j.tris = tt;
     j.mesh_off = mesh_off;
-    j.double_sided = double_sided;
+    j.mesh_flags = mesh_flags;
+    j.normals = normals;
+    j.coords = coords;
+    j.shading = shading;
+    j.uv = uv;
     j.objs = objs;
     j.n_objs = n_objs;
     j.rays = rays;
@@ -841,7 +900,7 @@ void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, in
     free(tt);
 }
 
-void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const uint8_t* double_sided,
+void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const uint8_t* mesh_flags,
                              const rise_oracle_object* objs, int n_objs, const float* rays, const float* tmax,
                              int64_t n_rays, uint8_t* occluded, int opts, int n_threads)
 {
@@ -850,7 +909,7 @@ void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_
     tri64* tt = expand_tris_f64(tris, mesh_off[n_meshes]);
     j.tris = tt;
     j.mesh_off = mesh_off;
-    j.double_sided = double_sided;
+    j.mesh_flags = mesh_flags;
     j.objs = objs;
     j.n_objs = n_objs;
     j.rays = rays;

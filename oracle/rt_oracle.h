@@ -59,6 +59,10 @@ extern "C" {
 #define RISE_ORACLE_NO_BACK 8   /* skip back faces (bHitBackFaces false) */
 #define RISE_ORACLE_OBJ_PREHIT 16 /* object queries: model Object::IntersectRay's box pre-test
                                      (Object.cpp:513-527, :695-707) on the mesh's vertex bounds */
+/* Per-mesh flags of the object queries. */
+#define RISE_ORACLE_MESH_DOUBLE_SIDED 1 /* the mesh is double-sided */
+#define RISE_ORACLE_MESH_NORMALS 2      /* the mesh has per-corner vertex normals */
+#define RISE_ORACLE_MESH_COORDS 4       /* the mesh has per-corner texture coordinates */
 #define RISE_ORACLE_DOUBLE_SIDED 32 /* every triangle belongs to a double-sided mesh
                                        (TriangleMeshGeometryIndexed.cpp:151-205): face flags do not
                                        apply and the record's normal is turned toward the ray */
@@ -134,8 +138,12 @@ typedef struct rise_oracle_object {
  * point and range), brute force over each object's mesh.
  * tris: all meshes' triangles (9 doubles each, the vertices as the meshes
  * hold them; float meshes widened exactly), mesh m being
- * [mesh_off[m], mesh_off[m+1]); double_sided: NULL, or one byte per mesh
- * (nonzero: double-sided, as RISE_ORACLE_DOUBLE_SIDED for that mesh).  Outputs per ray: prim_id (index within its
+ * [mesh_off[m], mesh_off[m+1]); mesh_flags: NULL, or one byte per mesh of
+ * RISE_ORACLE_MESH_* bits; normals (9 per triangle) and coords (6 per
+ * triangle): per-corner vertex normals and texture coordinates aligned with
+ * tris, read for meshes flagged so (NULL when none is).  Extra outputs
+ * (NULL or given): shading (3 per ray, the world shading normal, ri.vNormal)
+ * and uv (2 per ray, ri.ptCoord; 0 without coordinates).  Outputs per ray: prim_id (index within its
  * mesh, -1 on a miss), object (index, -1), range (world distance, +inf);
  * optional (NULL or all given): alpha, beta, normal (3), point (3, world);
  * optional flags: 1 where the winner fails the filter against its own t
@@ -146,15 +154,16 @@ typedef struct rise_oracle_object {
  * running best is scaled by |M^-1 d| = |d| |M^-1 d/|d||, short of its
  * object-space image, and it can drop a closer object. */
 void rise_oracle_objects_closest(const double* tris, const int64_t* mesh_off, int n_meshes,
-                                 const uint8_t* double_sided, const rise_oracle_object* objs, int n_objs,
-                                 const float* rays, int64_t n_rays,
+                                 const uint8_t* mesh_flags, const double* normals, const double* coords,
+                                 const rise_oracle_object* objs, int n_objs, const float* rays, int64_t n_rays,
                                  int32_t* prim_id, int32_t* object, double* range, double* alpha, double* beta,
-                                 double* normal, double* point, uint8_t* flags, int opts, int n_threads);
+                                 double* normal, double* point, double* shading, double* uv, uint8_t* flags,
+                                 int opts, int n_threads);
 
 /* ObjectManager::IntersectShadowRay's loop (ObjectManager.cpp:320-373) with
  * Object::IntersectRay_IntersectionOnly per object (Object.cpp:652-712):
  * dHowFar scaled by |M^-1 d| per object.  tmax: NULL = RISE_INFINITY. */
-void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const uint8_t* double_sided,
+void rise_oracle_objects_any(const double* tris, const int64_t* mesh_off, int n_meshes, const uint8_t* mesh_flags,
                              const rise_oracle_object* objs, int n_objs, const float* rays, const float* tmax,
                              int64_t n_rays, uint8_t* occluded, int opts, int n_threads);
 

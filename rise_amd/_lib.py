@@ -40,6 +40,8 @@ class MeshDesc(ctypes.Structure):
         ("tri_ids", ctypes.c_void_p),
         ("vertices_f64", ctypes.c_void_p),
         ("flags", ctypes.c_uint32),
+        ("normals", ctypes.c_void_p),
+        ("coords", ctypes.c_void_p),
     ]
 
 

@@ -24,8 +24,9 @@ struct WideBVH;
 // Uploads a built wide BVH as a ray-cast scene (rise_hip_scene_upload's
 // device part); bvh_source goes to the scene's statistics.  tri64: NULL, or
 // the leaf records' double vertices (9 per record, leaf order), which the
-// certifier then uses.
+// certifier then uses; attr: NULL, or the records' vertex normals and
+// texture coordinates (kAttrDoubles per record) for hit records.
 rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene,
-                             const std::vector<double>* tri64 = nullptr);
+                             const std::vector<double>* tri64 = nullptr, const std::vector<double>* attr = nullptr);
 
 }  // namespace rise_hip

@@ -44,8 +44,12 @@ static_assert(sizeof(NodeSlot) == RISE_NODE_STRIDE, "node stride");
 // 48 bytes: three float4 loads.  w of v0 carries the caller's triangle id
 // (bit pattern of an int32), w of v1 the mesh or object index, w of v2 flags
 // (kTriNoShadow: the object casts no shadow; kTriDoubleSided: the mesh is
-// double-sided).
+// double-sided; kTriNormals / kTriCoords: the triangle's per-corner vertex
+// normals / texture coordinates are in the scene's attribute array).
 constexpr unsigned kTriDoubleSided = 2u;
+constexpr unsigned kTriNormals = 4u;
+constexpr unsigned kTriCoords = 8u;
+constexpr int kAttrDoubles = 15;  // per leaf record: 3 normals, 3 texture coordinates
 struct alignas(16) Tri48 {
     float v0[4];
     float v1[4];

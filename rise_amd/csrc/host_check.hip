@@ -104,8 +104,8 @@ extern "C" rise_hip_status rise_hip_debug_trace_ex_host(const float* tris, int64
             h.mesh = -1;
             h.range = INFINITY;
             if (cs.best_id >= 0) {
-                double ab[2], nrm[3], pt[3];
-                hit_attributes(r, trs + cs.best_leaf, cs.best_t, ab, nrm, pt);
+                double ab[2], nrm[3], pt[3], sn[3], uv[2];
+                hit_attributes(r, trs + cs.best_leaf, cs.best_t, ab, nrm, pt, nullptr, nullptr, sn, uv);
                 h.mesh = float_to_bits(trs[cs.best_leaf].v1[3]);
                 h.range = world_range(r, cs.best_t);
                 h.alpha = ab[0];
@@ -113,7 +113,10 @@ extern "C" rise_hip_status rise_hip_debug_trace_ex_host(const float* tris, int64
                 for (int k = 0; k < 3; ++k) {
                     h.point[k] = (float)pt[k];
                     h.geom_normal[k] = (float)nrm[k];
+                    h.shading_normal[k] = (float)sn[k];
                 }
+                h.uv[0] = (float)uv[0];
+                h.uv[1] = (float)uv[1];
             }
             hits[i] = h;
         } else {
@@ -165,9 +168,11 @@ extern "C" rise_hip_status rise_hip_debug_objects_host(const rise_hip_mesh_desc*
     if (st != RISE_HIP_OK) return st;
     std::vector<BlasView> views;
     for (size_t k = 0; k < host.blas.size(); ++k)
-        views.push_back(BlasView{host.blas[k].nodes.data(), host.blas[k].tris.data(), host.blas_tri64[k].data()});
+        views.push_back(BlasView{host.blas[k].nodes.data(), host.blas[k].tris.data(), host.blas_tri64[k].data(),
+                                 host.blas_attr[k].empty() ? nullptr : host.blas_attr[k].data()});
     const ObjectSceneView sc{host.world_tris > 0 ? host.world.nodes.data() : nullptr, host.world.tris.data(),
-                             host.world_tri64.data(), host.objects.data(), (int32_t)host.objects.size(), views.data(),
+                             host.world_tri64.data(), host.world_attr.empty() ? nullptr : host.world_attr.data(),
+                             host.objects.data(), (int32_t)host.objects.size(), views.data(),
                              host.top.nodes.data(), host.top.tris.data()};
     std::atomic<int64_t> next(0);
     std::atomic<bool> overflow(false);

@@ -108,15 +108,39 @@ void mesh_vertices(const rise_hip_mesh_desc& m, std::vector<float>& f, std::vect
     }
 }
 
-std::vector<double> leaf_vertices(const WideBVH& bvh, const std::vector<double>& v64)
+std::vector<double> leaf_rows(const WideBVH& bvh, const std::vector<double>& rows, int stride)
 {
-    std::vector<double> out(bvh.tris.size() * 9);
+    std::vector<double> out(bvh.tris.size() * (size_t)stride);
     for (size_t i = 0; i < bvh.tris.size(); ++i) {
         int32_t p;
         std::memcpy(&p, &bvh.tris[i].v0[3], 4);
-        for (int k = 0; k < 9; ++k) out[9 * i + k] = v64[9 * (size_t)p + k];
+        for (int k = 0; k < stride; ++k) out[(size_t)stride * i + k] = rows[(size_t)stride * (size_t)p + k];
     }
     return out;
+}
+
+std::vector<double> leaf_vertices(const WideBVH& bvh, const std::vector<double>& v64)
+{
+    return leaf_rows(bvh, v64, 9);
+}
+
+void mesh_attrs(const rise_hip_mesh_desc& m, std::vector<double>& a)
+{
+    const size_t a0 = a.size();
+    a.resize(a0 + (size_t)m.n_tris * kAttrDoubles, 0.0);
+    for (int64_t i = 0; i < m.n_tris; ++i) {
+        double* row = a.data() + a0 + (size_t)i * kAttrDoubles;
+        if (m.normals)
+            for (int k = 0; k < 9; ++k) row[k] = m.normals[9 * i + k];
+        if (m.coords)
+            for (int k = 0; k < 6; ++k) row[9 + k] = m.coords[6 * i + k];
+    }
+}
+
+uint32_t mesh_tri_flags(const rise_hip_mesh_desc& m)
+{
+    return ((m.flags & RISE_HIP_MESH_DOUBLE_SIDED) ? kTriDoubleSided : 0u) | (m.normals ? kTriNormals : 0u) |
+           (m.coords ? kTriCoords : 0u);
 }
 
 rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_meshes,
@@ -151,7 +175,8 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
         // Identity objects: one world-space tree built over the running index,
         // then each leaf triangle takes its caller id (v0.w) and its object (v1.w).
         std::vector<float> verts;
-        std::vector<double> verts64;
+        std::vector<double> verts64, attrs;
+        bool world_has_attr = false;
         std::vector<int32_t> ids((size_t)world_total), obj_of((size_t)world_total);
         int64_t off = 0;
         for (size_t k = 0; k < n_objects; ++k) {
@@ -160,6 +185,8 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             const rise_hip_mesh_desc& m = meshes[ob.mesh];
             if (m.n_tris == 0) continue;
             mesh_vertices(m, verts, verts64);
+            mesh_attrs(m, attrs);
+            world_has_attr = world_has_attr || m.normals || m.coords;
             for (int64_t i = 0; i < m.n_tris; ++i) {
                 ids[(size_t)(off + i)] = m.tri_ids ? m.tri_ids[i] : (int32_t)i;
                 obj_of[(size_t)(off + i)] = (int32_t)k;
@@ -170,13 +197,14 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
         if (!build_wide_bvh(verts.data(), world_total, nullptr, BuildParams(), out.world, err))
             return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
         out.world_tri64 = leaf_vertices(out.world, verts64);
+        if (world_has_attr) out.world_attr = leaf_rows(out.world, attrs, kAttrDoubles);
         for (Tri48& tr : out.world.tris) {
             int32_t p;
             std::memcpy(&p, &tr.v0[3], 4);
             std::memcpy(&tr.v0[3], &ids[(size_t)p], 4);
             const int32_t k = obj_of[(size_t)p];
             std::memcpy(&tr.v1[3], &k, 4);
-            const uint32_t fl = (meshes[objects[k].mesh].flags & RISE_HIP_MESH_DOUBLE_SIDED) ? kTriDoubleSided : 0u;
+            const uint32_t fl = mesh_tri_flags(meshes[objects[k].mesh]);
             std::memcpy(&tr.v2[3], &fl, 4);
         }
         // Transformed objects: one object-space tree per mesh they use.
@@ -190,12 +218,18 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             if (blas_of[ob.mesh] < 0) {
                 WideBVH b;
                 std::vector<float> mv;
-                std::vector<double> mv64;
+                std::vector<double> mv64, ma;
                 mesh_vertices(m, mv, mv64);
                 if (!build_wide_bvh(mv.data(), m.n_tris, nullptr, BuildParams(), b, err))
                     return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
                 out.blas_tri64.push_back(leaf_vertices(b, mv64));
-                const uint32_t fl = (m.flags & RISE_HIP_MESH_DOUBLE_SIDED) ? kTriDoubleSided : 0u;
+                if (m.normals || m.coords) {
+                    mesh_attrs(m, ma);
+                    out.blas_attr.push_back(leaf_rows(b, ma, kAttrDoubles));
+                } else {
+                    out.blas_attr.emplace_back();
+                }
+                const uint32_t fl = mesh_tri_flags(m);
                 for (Tri48& tr : b.tris) {
                     int32_t p;
                     std::memcpy(&p, &tr.v0[3], 4);

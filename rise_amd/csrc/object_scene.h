@@ -14,9 +14,11 @@ namespace rise_hip {
 struct ObjectSceneHost {
     WideBVH world;                  // identity objects, world space; v1.w = object index
     std::vector<double> world_tri64;  // its leaf records' double vertices (9 each, leaf order)
+    std::vector<double> world_attr;   // their normals and texture coordinates (kAttrDoubles each), or empty
     int64_t world_tris = 0;
     std::vector<WideBVH> blas;      // one per mesh used by a transformed object, object space
     std::vector<std::vector<double>> blas_tri64;
+    std::vector<std::vector<double>> blas_attr;  // empty for a mesh without normals or coordinates
     std::vector<DevObject> objects; // the transformed objects, in the caller's order
     WideBVH top;                    // over the transformed objects' world boxes (leaf v0.w = index into objects)
     int64_t total_tris = 0;         // triangles over all trees
@@ -30,6 +32,19 @@ struct ObjectSceneHost {
 // as the reference's filter takes them) and double vertices (as given, or the
 // floats widened), n_tris * 9 each, appended.
 void mesh_vertices(const rise_hip_mesh_desc& m, std::vector<float>& f, std::vector<double>& d);
+
+// A mesh's per-corner vertex normals and texture coordinates (kAttrDoubles
+// per triangle, zeros where the mesh has none), appended.
+void mesh_attrs(const rise_hip_mesh_desc& m, std::vector<double>& a);
+
+// The leaf-record flags a mesh contributes (kTriDoubleSided, kTriNormals,
+// kTriCoords).
+uint32_t mesh_tri_flags(const rise_hip_mesh_desc& m);
+
+// Per-leaf-record rows of `stride` doubles gathered from rows indexed by the
+// running index the tree was built over (the record's v0.w before ids are
+// assigned).
+std::vector<double> leaf_rows(const WideBVH& bvh, const std::vector<double>& rows, int stride);
 
 // The leaf records' double vertices: record i takes v64[9 p .. 9 p + 8] where
 // p is the running index the tree was built over (the record's v0.w before

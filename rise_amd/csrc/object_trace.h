@@ -232,34 +232,73 @@ RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6
     });
 }
 
+// Fills the record's attributes from a hit in the mesh's frame: normals taken
+// by the object's inverse transpose (minv; NULL: identity) and normalized
+// (Object.cpp:531-533).
+template <int A>
+RISE_HD inline void record_normals(const HitLocal& l, const double* minv, rise_hip_hit& out)
+{
+    double g[3], sh[3], gw[3], sw[3];
+    if (minv) {
+        xform_normal<A>(minv, l.gn, g);
+        xform_normal<A>(minv, l.sn, sh);
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            g[k] = l.gn[k];
+            sh[k] = l.sn[k];
+        }
+    }
+    normalize_model<A>(g, gw);
+    normalize_model<A>(sh, sw);
+    out.alpha = l.ab[0];
+    out.beta = l.ab[1];
+    for (int k = 0; k < 3; ++k) {
+        out.geom_normal[k] = (float)gw[k];
+        out.shading_normal[k] = (float)sw[k];
+    }
+    out.uv[0] = (float)l.uv[0];
+    out.uv[1] = (float)l.uv[1];
+}
+
+RISE_HD inline void record_miss(rise_hip_hit& out)
+{
+    out.prim_id = -1;
+    out.mesh = -1;
+    out.range = INFINITY;
+    out.alpha = out.beta = 0.0;
+    for (int k = 0; k < 3; ++k) out.point[k] = out.geom_normal[k] = out.shading_normal[k] = 0.0f;
+    out.uv[0] = out.uv[1] = 0.0f;
+}
+
 // The intersection record of a committed hit (rise_hip_hit): barycentrics of
-// the certifier, the face normal normalized by the mesh
-// (...Specializations.h:196-205) and, in world space, by the object's
-// inverse transpose and Normalize (Object.cpp:531-533), the world point.
+// the certifier, the geometric and shading normals and the texture
+// coordinates as the mesh leaves them (hit_local), the normals taken to world
+// space by the object's inverse transpose and Normalize (Object.cpp:531-533),
+// the world point.
 template <int A>
 RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6], const ObjectHit& h,
                                    rise_hip_hit& out)
 {
-    out.prim_id = h.prim_id;
-    out.mesh = h.object;
     if (h.prim_id < 0) {
-        out.range = INFINITY;
-        out.alpha = out.beta = 0.0;
-        for (int k = 0; k < 3; ++k) out.point[k] = out.geom_normal[k] = 0.0f;
+        record_miss(out);
         return;
     }
+    out.prim_id = h.prim_id;
+    out.mesh = h.object;
     out.range = h.range;
+    double v[9];
+    HitLocal l;
     if (h.src < 0) {
         RayStateT<A> r;
         init_ray(r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
-        double ab[2], n[3], p[3];
-        hit_attributes(r, sc.world_tris + h.leaf, h.t, ab, n, p, sc.world_tri64 + 9 * (size_t)h.leaf);
-        out.alpha = ab[0];
-        out.beta = ab[1];
-        for (int k = 0; k < 3; ++k) {
-            out.point[k] = (float)p[k];
-            out.geom_normal[k] = (float)n[k];
-        }
+        const Tri48* tri = sc.world_tris + h.leaf;
+        triangle_vertices<RayStateT<A>>(tri, sc.world_tri64 + 9 * (size_t)h.leaf, v);
+        hit_local(r, v, tri_flags(tri), sc.world_attr ? sc.world_attr + kAttrDoubles * (size_t)h.leaf : nullptr, l);
+        record_normals<A>(l, nullptr, out);
+        const double s = h.t - kSurfaceBackoff;
+        const double d[3] = {dn0(r), dn1(r), dn2(r)};
+        const double o[3] = {r.ox, r.oy, r.oz};
+        for (int k = 0; k < 3; ++k) out.point[k] = (float)(A == kArithX86Fma ? fma(d[k], s, o[k]) : o[k] + d[k] * s);
         return;
     }
     const DevObject& ob = sc.objects[h.src];
@@ -268,17 +307,11 @@ RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6]
     ObjRayT<A> q;
     init_obj_ray(q, ow, dw, ob.to_object);
     const BlasView bl = sc.blas[ob.blas];
-    double v[9], ab[2], n1[3], nw[3], n2[3];
-    triangle_vertices<ObjRayT<A>>(bl.tris + h.leaf, bl.tri64 + 9 * (size_t)h.leaf, v);
-    hit_face(q, v, tri_double_sided(bl.tris + h.leaf), ab, n1);
-    xform_normal<A>(ob.to_object, n1, nw);
-    normalize_model<A>(nw, n2);
-    out.alpha = ab[0];
-    out.beta = ab[1];
-    for (int k = 0; k < 3; ++k) {
-        out.point[k] = (float)h.p_world[k];
-        out.geom_normal[k] = (float)n2[k];
-    }
+    const Tri48* tri = bl.tris + h.leaf;
+    triangle_vertices<ObjRayT<A>>(tri, bl.tri64 + 9 * (size_t)h.leaf, v);
+    hit_local(q, v, tri_flags(tri), bl.attr ? bl.attr + kAttrDoubles * (size_t)h.leaf : nullptr, l);
+    record_normals<A>(l, ob.to_object, out);
+    for (int k = 0; k < 3; ++k) out.point[k] = (float)h.p_world[k];
 }
 
 // ObjectManager::IntersectShadowRay over the scene (ObjectManager.cpp:320-373

@@ -29,12 +29,14 @@ struct BlasView {
     const NodeSlot* nodes;
     const Tri48* tris;
     const double* tri64;  // the triangles' double vertices, 9 per leaf record (leaf order)
+    const double* attr;   // per-corner normals and texture coordinates, kAttrDoubles per leaf record, or NULL
 };
 
 struct ObjectSceneView {
     const NodeSlot* world_nodes;  // NULL when no object has identity matrices
     const Tri48* world_tris;
     const double* world_tri64;    // the world triangles' double vertices, 9 per leaf record
+    const double* world_attr;     // their per-corner normals and texture coordinates, or NULL
     const DevObject* objects;
     int32_t n_objects;
     const BlasView* blas;

@@ -28,6 +28,7 @@ struct rise_hip_scene {
     NodeSlot* d_nodes = nullptr;
     Tri48* d_tris = nullptr;
     double* d_tri64 = nullptr;  // leaf records' double vertices (9 each), or NULL: the float vertices are exact
+    double* d_attr = nullptr;   // leaf records' normals and texture coordinates (kAttrDoubles each), or NULL
     // Dynamic-fetch counters: each query takes the next slot, so queries on
     // different streams do not share a counter (up to kFetchSlots in flight).
     static constexpr int kFetchSlots = 64;
@@ -46,8 +47,8 @@ struct rise_hip_scene {
     bool world_empty = false;
     ObjectSceneView objects_view() const
     {
-        return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_tri64, d_objects, n_objects, d_blas,
-                               d_top_nodes, d_top_leaves};
+        return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_tri64, d_attr, d_objects, n_objects,
+                               d_blas, d_top_nodes, d_top_leaves};
     }
     TraceScene view()
     {
@@ -56,7 +57,7 @@ struct rise_hip_scene {
     }
     TraceScene view_on(unsigned long long* fetch_counter) const
     {
-        return TraceScene{d_nodes, d_tris, stats.depth, fetch_counter, arith.load(), d_tri64};
+        return TraceScene{d_nodes, d_tris, stats.depth, fetch_counter, arith.load(), d_tri64, d_attr};
     }
 };
 
@@ -103,7 +104,7 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
     *out_scene = nullptr;
     if (n_meshes > 0 && !meshes) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "meshes is NULL");
     int64_t total = 0;
-    bool any64 = false;
+    bool any64 = false, any_attr = false;
     for (size_t m = 0; m < n_meshes; ++m) {
         if (meshes[m].n_tris < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative triangle count");
         if (meshes[m].n_tris > 0 && !meshes[m].vertices && !meshes[m].vertices_f64)
@@ -112,12 +113,13 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
             return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "unknown mesh flags");
         total += meshes[m].n_tris;
         any64 = any64 || (meshes[m].n_tris > 0 && meshes[m].vertices_f64);
+        any_attr = any_attr || (meshes[m].n_tris > 0 && (meshes[m].normals || meshes[m].coords));
     }
     if (total > 0x7fffffff) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "more than 2^31-1 triangles");
 
     // Concatenate meshes (identity transforms: world space == object space).
     std::vector<float> verts;
-    std::vector<double> verts64;
+    std::vector<double> verts64, attrs;
     std::vector<int32_t> ids;
     try {
         verts.reserve((size_t)total * 9);
@@ -131,12 +133,13 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
         const int64_t k = meshes[m].n_tris;
         if (k == 0) continue;
         mesh_vertices(meshes[m], verts, verts64);
+        if (any_attr) mesh_attrs(meshes[m], attrs);
         for (int64_t i = 0; i < k; ++i) ids[off + i] = meshes[m].tri_ids ? meshes[m].tri_ids[i] : (int32_t)(off + i);
         off += k;
     }
 
     WideBVH bvh;
-    std::vector<double> tri64;
+    std::vector<double> tri64, leaf_attr;
     std::string err;
     BuildParams params;
     try {
@@ -145,6 +148,7 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
         if (!build_wide_bvh(verts.data(), total, nullptr, params, bvh, err))
             return fail(RISE_HIP_ERR_BUILD, "BVH build failed: " + err);
         if (any64) tri64 = leaf_vertices(bvh, verts64);
+        if (any_attr) leaf_attr = leaf_rows(bvh, attrs, kAttrDoubles);
         std::vector<int32_t> mesh_of((size_t)total);
         int64_t o = 0;
         for (size_t m = 0; m < n_meshes; ++m)
@@ -155,14 +159,14 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
             std::memcpy(&tr.v0[3], &ids[(size_t)p], 4);
             const int32_t m = mesh_of[(size_t)p];
             std::memcpy(&tr.v1[3], &m, 4);
-            const uint32_t fl = (meshes[m].flags & RISE_HIP_MESH_DOUBLE_SIDED) ? kTriDoubleSided : 0u;
+            const uint32_t fl = mesh_tri_flags(meshes[m]);
             std::memcpy(&tr.v2[3], &fl, 4);
         }
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }
 
-    return upload_scene(bvh, total, 0, out_scene, any64 ? &tri64 : nullptr);
+    return upload_scene(bvh, total, 0, out_scene, any64 ? &tri64 : nullptr, any_attr ? &leaf_attr : nullptr);
 }
 
 }  // extern "C"
@@ -170,7 +174,7 @@ rise_hip_status rise_hip_scene_upload(const rise_hip_mesh_desc* meshes, size_t n
 namespace rise_hip {
 
 rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_source, rise_hip_scene** out_scene,
-                             const std::vector<double>* tri64)
+                             const std::vector<double>* tri64, const std::vector<double>* attr)
 {
     rise_hip_scene* s = new (std::nothrow) rise_hip_scene;
     if (!s) return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -191,10 +195,14 @@ rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_sour
     if (e == hipSuccess && tri64) e = hipMalloc((void**)&s->d_tri64, vb);
     if (e == hipSuccess && tri64 && !tri64->empty())
         e = hipMemcpy(s->d_tri64, tri64->data(), tri64->size() * sizeof(double), hipMemcpyHostToDevice);
+    const size_t ab = (attr && !attr->empty()) ? attr->size() * sizeof(double) : 0;
+    if (e == hipSuccess && ab) e = hipMalloc((void**)&s->d_attr, ab);
+    if (e == hipSuccess && ab) e = hipMemcpy(s->d_attr, attr->data(), ab, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(s->d_nodes);
         (void)hipFree(s->d_tris);
         (void)hipFree(s->d_tri64);
+        (void)hipFree(s->d_attr);
         (void)hipFree(s->d_fetch);
         delete s;
         return fail_hip(e, "scene upload");
@@ -208,7 +216,7 @@ rise_hip_status upload_scene(const WideBVH& bvh, int64_t total, int32_t bvh_sour
     s->stats.tri_bytes = (int32_t)sizeof(Tri48);
     s->stats.build_ms = bvh.stats.build_ms;
     s->stats.sah_cost = bvh.stats.sah_cost;
-    s->stats.device_bytes = (int64_t)(nb + tb + vb);
+    s->stats.device_bytes = (int64_t)(nb + tb + vb + ab);
     s->stats.bvh_source = bvh_source;
     *out_scene = s;
     return RISE_HIP_OK;
@@ -229,7 +237,8 @@ rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, 
     rise_hip_status st = build_object_scene(meshes, n_meshes, objects, n_objects, host);
     if (st != RISE_HIP_OK) return st;
     rise_hip_scene* s = nullptr;
-    st = upload_scene(host.world, host.world_tris, 0, &s, &host.world_tri64);
+    st = upload_scene(host.world, host.world_tris, 0, &s, &host.world_tri64,
+                      host.world_attr.empty() ? nullptr : &host.world_attr);
     if (st != RISE_HIP_OK) return st;
     s->world_empty = host.world_tris == 0;
     s->stats.n_tris = host.total_tris;
@@ -239,9 +248,10 @@ rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, 
         if (e != hipSuccess) break;
         const WideBVH& b = host.blas[k];
         const std::vector<double>& v64 = host.blas_tri64[k];
+        const std::vector<double>& at = host.blas_attr[k];
         const size_t nb = b.nodes.size() * sizeof(NodeSlot), tb = std::max<size_t>(b.tris.size(), 1) * sizeof(Tri48);
         const size_t vb = std::max<size_t>(v64.size(), 1) * sizeof(double);
-        void *dn = nullptr, *dt = nullptr, *dv = nullptr;
+        void *dn = nullptr, *dt = nullptr, *dv = nullptr, *da = nullptr;
         e = hipMalloc(&dn, nb);
         if (e == hipSuccess) s->blas_allocs.push_back(dn);
         if (e == hipSuccess) e = hipMalloc(&dt, tb);
@@ -253,8 +263,11 @@ rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, 
             e = hipMemcpy(dt, b.tris.data(), b.tris.size() * sizeof(Tri48), hipMemcpyHostToDevice);
         if (e == hipSuccess && !v64.empty())
             e = hipMemcpy(dv, v64.data(), v64.size() * sizeof(double), hipMemcpyHostToDevice);
-        views.push_back(BlasView{(const NodeSlot*)dn, (const Tri48*)dt, (const double*)dv});
-        s->stats.device_bytes += (int64_t)(nb + tb + vb);
+        if (e == hipSuccess && !at.empty()) e = hipMalloc(&da, at.size() * sizeof(double));
+        if (e == hipSuccess && da) s->blas_allocs.push_back(da);
+        if (e == hipSuccess && da) e = hipMemcpy(da, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice);
+        views.push_back(BlasView{(const NodeSlot*)dn, (const Tri48*)dt, (const double*)dv, (const double*)da});
+        s->stats.device_bytes += (int64_t)(nb + tb + vb + at.size() * sizeof(double));
     }
     if (e == hipSuccess && !host.objects.empty()) {
         const size_t ob = host.objects.size() * sizeof(DevObject), vb = views.size() * sizeof(BlasView);
@@ -290,6 +303,7 @@ rise_hip_status rise_hip_scene_release(rise_hip_scene* scene)
     (void)hipFree(scene->d_nodes);
     (void)hipFree(scene->d_tris);
     (void)hipFree(scene->d_tri64);
+    (void)hipFree(scene->d_attr);
     (void)hipFree(scene->d_fetch);
     delete scene;
     return RISE_HIP_OK;
@@ -498,9 +512,10 @@ extern "C" rise_hip_status rise_hip_scene_cache_write(const rise_hip_scene* scen
 {
     clear_error();
     if (!scene || !path) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (scene->n_objects > 0 || scene->d_tri64)
+    if (scene->n_objects > 0 || scene->d_tri64 || scene->d_attr)
         return fail(RISE_HIP_ERR_NOT_SUPPORTED,
-                    "scene cache of a scene with transformed objects or double-precision vertices");
+                    "scene cache of a scene with transformed objects, double-precision vertices or vertex "
+                    "attributes");
     const uint64_t nn = (uint64_t)scene->stats.n_nodes, nt = (uint64_t)scene->stats.n_tris;
     std::vector<NodeSlot> nodes;
     std::vector<Tri48> tris;

@@ -807,37 +807,97 @@ RISE_HD inline void triangle_vertices(const Tri48* __restrict__ tri, const doubl
     for (int k = 0; k < 9; ++k) v[k] = w[k];
 }
 
-// The certifier's barycentrics and the face normal (v1 - v0) x (v2 - v0)
-// normalized once by the mesh (...Specializations.h:196-205), turned toward
-// the ray's origin on a double-sided mesh (TriangleMeshGeometryIndexed.cpp:
-// 170-188: negated when its dot product with the direction is positive).
+// p0 + (p1 - p0) a + (p2 - p0) b for a vertex-normal component
+// (...Specializations.h:194-196), left to right; the Linux build contracts it
+// as fma(p2 - p0, b, fma(p1 - p0, a, p0)).
+template <int A>
+RISE_HD inline double interp_normal(double p0, double p1, double p2, double a, double b)
+{
+    const double d1 = p1 - p0, d2 = p2 - p0;
+    if (A == kArithX86Fma) return fma(d2, b, fma(d1, a, p0));
+    return (p0 + d1 * a) + d2 * b;
+}
+
+// The same for a texture coordinate (...Specializations.h:214-216): the
+// offset is summed first, p0 + ((p1 - p0) a + (p2 - p0) b); the Linux build
+// contracts the sum as fma(p2 - p0, b, (p1 - p0) a).
+template <int A>
+RISE_HD inline double interp_uv(double p0, double p1, double p2, double a, double b)
+{
+    const double d1 = p1 - p0, d2 = p2 - p0;
+    if (A == kArithX86Fma) return p0 + fma(d2, b, d1 * a);
+    return p0 + (d1 * a + d2 * b);
+}
+
+// A committed hit in the mesh's frame, as RayElementIntersection and the
+// mesh's IntersectRay leave it (...Specializations.h:191-216,
+// TriangleMeshGeometryIndexed.cpp:170-188): the certifier's barycentrics; the
+// shading normal, the vertex normals interpolated or else the face normal
+// (v1 - v0) x (v2 - v0); the face normal normalized and turned to the shading
+// normal's side; the texture coordinates interpolated (0 without them); on a
+// double-sided mesh both normals turned toward the ray.
+struct HitLocal {
+    double ab[2];
+    double gn[3];  // geometric normal (unit)
+    double sn[3];  // shading normal (not normalized)
+    double uv[2];
+};
+
 template <class R>
-RISE_HD inline void hit_face(const R& r, const double v[9], bool double_sided, double ab[2], double n1[3])
+RISE_HD inline void hit_local(const R& r, const double v[9], unsigned tflags, const double* __restrict__ attr,
+                              HitLocal& h)
 {
     constexpr int A = R::kArith;
     double tt = 0.0;
-    ab[0] = ab[1] = 0.0;
-    (void)certify64(r, v, tt, ab);
+    h.ab[0] = h.ab[1] = 0.0;
+    (void)certify64(r, v, tt, h.ab);
+    const double a = h.ab[0], b = h.ab[1];
     double fn[3];
     face_cross<A>(v[3] - v[0], v[4] - v[1], v[5] - v[2], v[6] - v[0], v[7] - v[1], v[8] - v[2], fn);
-    normalize_model<A>(fn, n1);
-    if (double_sided && dot3<A>(n1[0], dn0(r), n1[1], dn1(r), n1[2], dn2(r)) > 0.0)
-        for (int k = 0; k < 3; ++k) n1[k] = -n1[k];
+    if ((tflags & kTriNormals) && attr) {
+        for (int c = 0; c < 3; ++c) h.sn[c] = interp_normal<A>(attr[c], attr[3 + c], attr[6 + c], a, b);
+    } else {
+        for (int c = 0; c < 3; ++c) h.sn[c] = fn[c];
+    }
+    normalize_model<A>(fn, h.gn);
+    if (dot3<A>(h.gn[0], h.sn[0], h.gn[1], h.sn[1], h.gn[2], h.sn[2]) < 0.0)
+        for (int c = 0; c < 3; ++c) h.gn[c] = -h.gn[c];
+    if ((tflags & kTriCoords) && attr) {
+        for (int c = 0; c < 2; ++c) h.uv[c] = interp_uv<A>(attr[9 + c], attr[11 + c], attr[13 + c], a, b);
+    } else {
+        h.uv[0] = h.uv[1] = 0.0;
+    }
+    if (tflags & kTriDoubleSided) {
+        if (dot3<A>(h.sn[0], dn0(r), h.sn[1], dn1(r), h.sn[2], dn2(r)) > 0.0)
+            for (int c = 0; c < 3; ++c) h.sn[c] = -h.sn[c];
+        if (dot3<A>(h.gn[0], dn0(r), h.gn[1], dn1(r), h.gn[2], dn2(r)) > 0.0)
+            for (int c = 0; c < 3; ++c) h.gn[c] = -h.gn[c];
+    }
 }
 
-RISE_HD inline bool tri_double_sided(const Tri48* __restrict__ tri)
+RISE_HD inline unsigned tri_flags(const Tri48* __restrict__ tri)
 {
-    return (float_to_bits(reinterpret_cast<const float4*>(tri)[2].w) & kTriDoubleSided) != 0;
+    return (unsigned)float_to_bits(reinterpret_cast<const float4*>(tri)[2].w);
 }
 
 template <int A>
 RISE_HD inline void hit_attributes(const RayStateT<A>& r, const Tri48* __restrict__ tri, double t, double ab[2],
-                                   double n_out[3], double p_out[3], const double* __restrict__ v64 = nullptr)
+                                   double n_out[3], double p_out[3], const double* __restrict__ v64 = nullptr,
+                                   const double* __restrict__ attr = nullptr, double* s_out = nullptr,
+                                   double* uv_out = nullptr)
 {
-    double v[9], n1[3];
+    double v[9];
     triangle_vertices<RayStateT<A>>(tri, v64, v);
-    hit_face(r, v, tri_double_sided(tri), ab, n1);
-    normalize_model<A>(n1, n_out);
+    HitLocal h;
+    hit_local(r, v, tri_flags(tri), attr, h);
+    ab[0] = h.ab[0];
+    ab[1] = h.ab[1];
+    normalize_model<A>(h.gn, n_out);
+    if (s_out) normalize_model<A>(h.sn, s_out);
+    if (uv_out) {
+        uv_out[0] = h.uv[0];
+        uv_out[1] = h.uv[1];
+    }
     const double s = t - kSurfaceBackoff;
     const double d[3] = {dn0(r), dn1(r), dn2(r)};
     const double o[3] = {r.ox, r.oy, r.oz};

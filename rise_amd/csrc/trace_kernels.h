@@ -17,6 +17,7 @@ struct TraceScene {
     unsigned long long* fetch_counter;  // device counter for dynamic ray fetch (one per in-flight call)
     int arith;  // rise_hip_arith: kArithIeee or kArithX86Fma (trace_core.h), same values
     const double* tri64;  // leaf records' double vertices (9 each) for the certifier, or NULL
+    const double* attr;   // leaf records' normals and texture coordinates (kAttrDoubles each), or NULL
 };
 
 // Closest hit: rays SoA (ox, oy, oz, dx, dy, dz), n each.  counters: NULL, or

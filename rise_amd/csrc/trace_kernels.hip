@@ -212,7 +212,8 @@ __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* 
                                                                 const float* __restrict__ rays, int64_t n,
                                                                 unsigned faces, rise_hip_hit* __restrict__ hits,
                                                                 unsigned long long* next,
-                                                                const double* __restrict__ tri64)
+                                                                const double* __restrict__ tri64,
+                                                                const double* __restrict__ attr)
 {
     __shared__ uint2 lds[STACK > 0 ? STACK * kBlock : 1];
     LdsStack<STACK> lst{lds + threadIdx.x, kBlock, 0};
@@ -246,8 +247,9 @@ __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* 
                 h.prim_id = s.best_id;
                 if (s.best_id >= 0) {
                     const Tri48* tri = tris + s.best_leaf;
-                    double ab[2], nrm[3], pt[3];
-                    hit_attributes(r, tri, s.best_t, ab, nrm, pt, F64 ? tri64 + 9 * (size_t)s.best_leaf : nullptr);
+                    double ab[2], nrm[3], pt[3], sn[3], uv[2];
+                    hit_attributes(r, tri, s.best_t, ab, nrm, pt, F64 ? tri64 + 9 * (size_t)s.best_leaf : nullptr,
+                                   attr ? attr + kAttrDoubles * (size_t)s.best_leaf : nullptr, sn, uv);
                     h.mesh = float_to_bits(tri->v1[3]);
                     h.range = world_range(r, s.best_t);
                     h.alpha = ab[0];
@@ -255,12 +257,16 @@ __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* 
                     for (int k = 0; k < 3; ++k) {
                         h.point[k] = (float)pt[k];
                         h.geom_normal[k] = (float)nrm[k];
+                        h.shading_normal[k] = (float)sn[k];
                     }
+                    h.uv[0] = (float)uv[0];
+                    h.uv[1] = (float)uv[1];
                 } else {
                     h.mesh = -1;
                     h.range = INFINITY;
                     h.alpha = h.beta = 0.0;
-                    for (int k = 0; k < 3; ++k) h.point[k] = h.geom_normal[k] = 0.0f;
+                    for (int k = 0; k < 3; ++k) h.point[k] = h.geom_normal[k] = h.shading_normal[k] = 0.0f;
+                    h.uv[0] = h.uv[1] = 0.0f;
                 }
                 hits[ray] = h;
                 ray = -1;
@@ -433,7 +439,7 @@ hipError_t launch_closest_ex(const TraceScene& s, const float* rays, int64_t n, 
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
 #define RISE_LAUNCH_EX_F(S, A, F)                                                                           \
     hipLaunchKernelGGL((closest_ex_persistent<S, A, F>), dim3(persistent_grid(closest_ex_persistent<S, A, F>, n)), \
-                       block, 0, stream, s.nodes, s.tris, rays, n, faces, hits, s.fetch_counter, s.tri64)
+                       block, 0, stream, s.nodes, s.tris, rays, n, faces, hits, s.fetch_counter, s.tri64, s.attr)
 #define RISE_LAUNCH_EX(S, A)                   \
     do {                                       \
         if (s.tri64) RISE_LAUNCH_EX_F(S, A, true); \

@@ -39,12 +39,14 @@ class Hit(ctypes.Structure):
     """rise_hip_hit."""
     _fields_ = [("prim_id", ctypes.c_int32), ("mesh", ctypes.c_int32), ("range", ctypes.c_double),
                 ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("point", ctypes.c_float * 3),
-                ("geom_normal", ctypes.c_float * 3)]
+                ("geom_normal", ctypes.c_float * 3), ("shading_normal", ctypes.c_float * 3),
+                ("uv", ctypes.c_float * 2)]
 
 
 HIT_DTYPE = np.dtype([("prim_id", "<i4"), ("mesh", "<i4"), ("range", "<f8"), ("alpha", "<f8"), ("beta", "<f8"),
-                      ("point", "<f4", (3,)), ("geom_normal", "<f4", (3,))])
-assert HIT_DTYPE.itemsize == ctypes.sizeof(Hit) == 56
+                      ("point", "<f4", (3,)), ("geom_normal", "<f4", (3,)), ("shading_normal", "<f4", (3,)),
+                      ("uv", "<f4", (2,)), ("_pad", "<i4")])
+assert HIT_DTYPE.itemsize == ctypes.sizeof(Hit) == 80
 
 
 def hit_fields(records) -> np.ndarray:
@@ -96,7 +98,7 @@ class ObjectManager:
     (ARITH_IEEE or ARITH_X86_FMA; rise_hip_scene_set_arithmetic)."""
 
     def __init__(self, arith: int = ARITH_IEEE) -> None:
-        self._meshes: list[tuple[np.ndarray, np.ndarray | None, bool]] = []
+        self._meshes: list[tuple] = []  # (triangles, ids, double-sided, normals, coordinates)
         self._objects: list[tuple[int, np.ndarray, np.ndarray]] = []
         self._scene = ctypes.c_void_p(None)
         self._arith = arith
@@ -118,11 +120,14 @@ class ObjectManager:
         self._objects.append((int(mesh), tw, to))
         return len(self._objects) - 1
 
-    def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None, double_sided: bool = False) -> None:
+    def add_mesh(self, tris: np.ndarray, ids: np.ndarray | None = None, double_sided: bool = False,
+                 normals: np.ndarray | None = None, coords: np.ndarray | None = None) -> None:
         """tris: (n, 9) float32 world-space triangles, or float64 for a mesh
         whose double vertices the certifier uses (rise_hip_mesh_desc.
         vertices_f64); ids: optional (n,) int32; double_sided: the mesh is
-        double-sided (RISE_HIP_MESH_DOUBLE_SIDED)."""
+        double-sided (RISE_HIP_MESH_DOUBLE_SIDED); normals (n, 9) / coords
+        (n, 6): per-corner vertex normals and texture coordinates (float64)
+        for the hit records."""
         if self._from_cache:
             raise RuntimeError("a scene loaded from a cache is complete; it takes no more meshes")
         f64 = np.asarray(tris).dtype == np.float64
@@ -130,7 +135,9 @@ class ObjectManager:
         i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
         if i is not None and i.shape[0] != t.shape[0]:
             raise ValueError("ids must have one entry per triangle")
-        self._meshes.append((t, i, bool(double_sided)))
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float64).reshape(t.shape[0], 9)
+        crd = None if coords is None else np.ascontiguousarray(coords, np.float64).reshape(t.shape[0], 6)
+        self._meshes.append((t, i, bool(double_sided), nrm, crd))
 
     def prepare_for_rendering(self) -> None:
         if self._from_cache:
@@ -138,8 +145,8 @@ class ObjectManager:
         L = _lib.lib()
         self.release()
         descs = (_lib.MeshDesc * max(1, len(self._meshes)))()
-        for k, (t, i, ds) in enumerate(self._meshes):
-            _set_desc(descs[k], t, i, ds)
+        for k, (t, i, ds, nrm, crd) in enumerate(self._meshes):
+            _set_desc(descs[k], t, i, ds, nrm, crd)
         out = ctypes.c_void_p(None)
         if self._objects:
             objs = make_objects(self._objects)
@@ -268,9 +275,11 @@ class ObjectManager:
             pass
 
 
-def _set_desc(desc, t: np.ndarray, ids, double_sided: bool = False) -> None:
+def _set_desc(desc, t: np.ndarray, ids, double_sided: bool = False, normals=None, coords=None) -> None:
     """Fills a rise_hip_mesh_desc from float32 or float64 (double vertices) triangles."""
     desc.flags = 1 if double_sided else 0
+    desc.normals = None if normals is None else normals.ctypes.data
+    desc.coords = None if coords is None else coords.ctypes.data
     if t.dtype == np.float64:
         desc.vertices = None
         desc.vertices_f64 = t.ctypes.data
@@ -347,7 +356,8 @@ def host_trace_ex(tris: np.ndarray, rays: np.ndarray, mode: int = 0, faces: int 
 
 
 def host_trace_objects(meshes, objects, rays: np.ndarray, mode: int = 0, faces: int = FACES_BOTH,
-                       tmax: np.ndarray | None = None, arith: int = ARITH_IEEE, ids=None, double_sided=None):
+                       tmax: np.ndarray | None = None, arith: int = ARITH_IEEE, ids=None, double_sided=None,
+                       normals=None, coords=None):
     """rise_hip_debug_objects_host: a scene of objects (meshes: list of (n, 9)
     float32 arrays, or float64 for double vertices; objects: OBJECT_DTYPE array) queried with the kernels'
     per-ray code on host threads (validation only).  Mode 0: hit records
@@ -359,8 +369,13 @@ def host_trace_objects(meshes, objects, rays: np.ndarray, mode: int = 0, faces: 
     id_arrs = [None if ids is None or ids[k] is None else np.ascontiguousarray(ids[k], np.int32)
                for k in range(len(arrs))]
     descs = (_lib.MeshDesc * max(1, len(arrs)))()
+    nrms = [None if normals is None or normals[k] is None else
+            np.ascontiguousarray(normals[k], np.float64).reshape(-1, 9) for k in range(len(arrs))]
+    crds = [None if coords is None or coords[k] is None else
+            np.ascontiguousarray(coords[k], np.float64).reshape(-1, 6) for k in range(len(arrs))]
     for k, t in enumerate(arrs):
-        _set_desc(descs[k], t, id_arrs[k], bool(double_sided[k]) if double_sided is not None else False)
+        _set_desc(descs[k], t, id_arrs[k], bool(double_sided[k]) if double_sided is not None else False,
+                  nrms[k], crds[k])
     objs = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
     n = rays.shape[1]
