@@ -117,6 +117,8 @@ private:
         EnumerateObjects(c);
         gpu_.reset(new rise_hip_host::HipObjectManager());
         vertices_.resize(objects_.size());
+        normals_.assign(objects_.size(), {});
+        coords_.assign(objects_.size(), {});
         for (size_t k = 0; k < objects_.size(); ++k) {
             const TriangleMeshGeometryIndexed* mesh =
                 dynamic_cast<const TriangleMeshGeometryIndexed*>(objects_[k]->GetGeometry());
@@ -127,15 +129,35 @@ private:
             // The mesh's triangles in its own frame, with the double
             // vertices the mesh holds (the library takes their float casts
             // for the filter, as the mesh's BVH does, BVH.h:431-467).
+            // Its corners' vertex normals and texture coordinates go with
+            // them when the mesh has them, for the hit records.
             std::vector<double>& v = vertices_[k];
+            std::vector<double>& nrm = normals_[k];
+            std::vector<double>& crd = coords_[k];
+            bool has_n = true, has_c = true;
+            for (const PointerTriangle& t : mesh->getFaces())
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    has_n = has_n && t.pNormals[c3];
+                    has_c = has_c && t.pCoords[c3];
+                }
             for (const PointerTriangle& t : mesh->getFaces())
                 for (int c3 = 0; c3 < 3; ++c3) {
                     v.push_back(t.pVertices[c3]->x);
                     v.push_back(t.pVertices[c3]->y);
                     v.push_back(t.pVertices[c3]->z);
+                    if (has_n) {
+                        nrm.push_back(t.pNormals[c3]->x);
+                        nrm.push_back(t.pNormals[c3]->y);
+                        nrm.push_back(t.pNormals[c3]->z);
+                    }
+                    if (has_c) {
+                        crd.push_back(t.pCoords[c3]->x);
+                        crd.push_back(t.pCoords[c3]->y);
+                    }
                 }
             const int m = gpu_->AddMeshF64(v.data(), (int64_t)(v.size() / 9), nullptr,
-                                           Sidedness::double_sided(*mesh) ? RISE_HIP_MESH_DOUBLE_SIDED : 0u);
+                                           Sidedness::double_sided(*mesh) ? RISE_HIP_MESH_DOUBLE_SIDED : 0u,
+                                           has_n ? nrm.data() : nullptr, has_c ? crd.data() : nullptr);
             double tw[16], to[16];
             ToArray(objects_[k]->GetFinalTransformMatrix(), tw);
             ToArray(objects_[k]->GetFinalInverseTransformMatrix(), to);
@@ -154,6 +176,7 @@ private:
     mutable std::unique_ptr<rise_hip_host::HipObjectManager> gpu_;
     mutable std::vector<const IObjectPriv*> objects_;
     mutable std::vector<std::vector<double>> vertices_;  // kept until the next upload
+    mutable std::vector<std::vector<double>> normals_, coords_;
     mutable bool gpu_ok_ = false;
 };
 

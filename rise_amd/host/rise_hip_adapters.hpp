@@ -57,15 +57,18 @@ public:
     // objects that refer to it.  Returns its index.
     int AddMesh(const float* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr, uint32_t flags = 0)
     {
-        meshes_.push_back({vertices, n_tris, tri_ids, nullptr, flags});
+        meshes_.push_back({vertices, n_tris, tri_ids, nullptr, flags, nullptr, nullptr});
         return (int)meshes_.size() - 1;
     }
 
     // The same with the mesh's double vertices (n_tris * 9), as RISE holds
     // them: the certifier uses them, the float filter their float casts.
-    int AddMeshF64(const double* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr, uint32_t flags = 0)
+    // normals (n_tris * 9) and coords (n_tris * 6), optional: per-corner
+    // vertex normals and texture coordinates for the hit records.
+    int AddMeshF64(const double* vertices, int64_t n_tris, const int32_t* tri_ids = nullptr, uint32_t flags = 0,
+                   const double* normals = nullptr, const double* coords = nullptr)
     {
-        meshes_.push_back({nullptr, n_tris, tri_ids, vertices, flags});
+        meshes_.push_back({nullptr, n_tris, tri_ids, vertices, flags, normals, coords});
         return (int)meshes_.size() - 1;
     }
 
