@@ -13,11 +13,12 @@ struct ObjectSceneView;
 
 // Closest hit over objects: hits (records) when non-NULL, else prim_id and t.
 // arith: kArithIeee or kArithX86Fma.
-hipError_t launch_closest_objects(const ObjectSceneView& sc, int arith, const float* rays, int64_t n, unsigned faces,
-                                  int32_t* prim_id, float* t, rise_hip_hit* hits, hipStream_t stream);
+// stack_need: node-group stack entries a query needs (the scene's depths).
+hipError_t launch_closest_objects(const ObjectSceneView& sc, int arith, int stack_need, const float* rays, int64_t n,
+                                  unsigned faces, int32_t* prim_id, float* t, rise_hip_hit* hits, hipStream_t stream);
 
 // Any hit over objects within tmax (world dHowFar per ray; NULL = unbounded).
-hipError_t launch_any_objects(const ObjectSceneView& sc, int arith, const float* rays, const float* tmax, int64_t n,
-                              unsigned faces, uint8_t* occluded, hipStream_t stream);
+hipError_t launch_any_objects(const ObjectSceneView& sc, int arith, int stack_need, const float* rays,
+                              const float* tmax, int64_t n, unsigned faces, uint8_t* occluded, hipStream_t stream);
 
 }  // namespace rise_hip

@@ -271,7 +271,8 @@ rise_hip_status build_object_scene(const rise_hip_mesh_desc* meshes, size_t n_me
             return fail(RISE_HIP_ERR_BUILD, "top-level BVH build failed: " + err);
         int blas_depth = 0;
         for (const WideBVH& b : out.blas) blas_depth = std::max(blas_depth, b.stats.depth);
-        if (no > 0 && out.top.stats.depth + blas_depth > kObjStack)
+        out.stack_need = std::max(out.world.stats.depth, no > 0 ? out.top.stats.depth + blas_depth : 0);
+        if (out.stack_need > kObjStack)
             return fail(RISE_HIP_ERR_BUILD, "top-level and object trees too deep for the traversal stack");
     } catch (const std::bad_alloc&) {
         return fail(RISE_HIP_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");

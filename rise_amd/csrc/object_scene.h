@@ -21,6 +21,7 @@ struct ObjectSceneHost {
     std::vector<std::vector<double>> blas_attr;  // empty for a mesh without normals or coordinates
     std::vector<DevObject> objects; // the transformed objects, in the caller's order
     WideBVH top;                    // over the transformed objects' world boxes (leaf v0.w = index into objects)
+    int stack_need = 0;             // node-group stack entries a query needs at most
     int64_t total_tris = 0;         // triangles over all trees
 };
 

@@ -83,7 +83,9 @@ RISE_HD inline double obj_how_far(double factor, double how_far)
 
 // Node-group stack of an object query with a movable floor: an object's tree
 // is walked on top of the top-level tree's entries.  The upload checks that
-// the depths fit; a push past the end is dropped and flagged.
+// the depths fit (ObjectSceneHost::stack_need); a push past the end is
+// dropped and flagged.  In a private array of kObjStack entries, or strided
+// in LDS (CAP entries per lane, `stride` lanes apart).
 struct ObjStack {
     uint2 e[kObjStack];
     int sp;
@@ -98,12 +100,28 @@ struct ObjStack {
     RISE_HD inline bool empty() const { return sp == base; }
 };
 
+template <int CAP>
+struct ObjLdsStack {
+    uint2* e;
+    int stride;
+    int sp;
+    int base;
+    bool overflow;
+    RISE_HD inline void push(uint2 v)
+    {
+        if (sp < CAP) e[sp++ * stride] = v;
+        else overflow = true;
+    }
+    RISE_HD inline uint2 pop() { return e[--sp * stride]; }
+    RISE_HD inline bool empty() const { return sp == base; }
+};
+
 // Walks the top-level tree near to far with the world ray; visit(k) is called
 // for each object record k reached and returns the new box limit, or a
 // negative value to end the walk.
-template <int A, class Visit>
+template <int A, class Stack, class Visit>
 RISE_HD inline void top_walk(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
-                             const Tri48* __restrict__ leaves, ObjStack& st, float tlim, Visit&& visit)
+                             const Tri48* __restrict__ leaves, Stack& st, float tlim, Visit&& visit)
 {
     unsigned gbase = 0, ghits = 1u << (0u ^ r.oinv), gimask = 1u;
     const int floor = st.sp;
@@ -156,8 +174,8 @@ struct ObjectHit {
 // ObjectManager::IntersectRay over the scene: each object's own closest hit
 // (Object::IntersectRay), the strictly closer world range winning, ties to the
 // lower object index (the caller's order).
-template <int A>
-RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6], unsigned faces, ObjStack& st,
+template <int A, class Stack>
+RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6], unsigned faces, Stack& st,
                                     ObjectHit& h)
 {
     h.prim_id = -1;
@@ -318,9 +336,9 @@ RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6]
 // with Object::IntersectRay_IntersectionOnly per object).  how_far: the
 // world limit dHowFar.  The answer does not depend on the order objects are
 // tried in.
-template <int A>
+template <int A, class Stack>
 RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], double how_far, unsigned faces,
-                                ObjStack& st)
+                                Stack& st)
 {
     st.sp = 0;
     st.base = 0;

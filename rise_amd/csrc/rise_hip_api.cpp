@@ -44,6 +44,7 @@ struct rise_hip_scene {
     Tri48* d_top_leaves = nullptr;
     std::vector<void*> blas_allocs;
     int32_t n_objects = 0;
+    int32_t stack_need = 0;  // node-group stack entries an object query needs
     bool world_empty = false;
     ObjectSceneView objects_view() const
     {
@@ -287,6 +288,7 @@ rise_hip_status rise_hip_scene_upload_objects(const rise_hip_mesh_desc* meshes, 
         return fail_hip(e, "object scene upload");
     }
     s->n_objects = (int32_t)host.objects.size();
+    s->stack_need = host.stack_need;
     *out_scene = s;
     return RISE_HIP_OK;
 }
@@ -338,7 +340,7 @@ static rise_hip_status closest_impl(rise_hip_scene* scene, const float* rays, in
     if (scene->n_objects > 0) {
         if (counters)
             return fail(RISE_HIP_ERR_NOT_SUPPORTED, "counted queries on a scene with transformed objects");
-        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), rays, n, 3u, prim_id, t,
+        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays, n, 3u, prim_id, t,
                                                     nullptr, stream);
         return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "closest-hit launch");
     }
@@ -359,7 +361,7 @@ static rise_hip_status any_impl(rise_hip_scene* scene, const float* rays, const 
     if (scene->n_objects > 0) {
         if (counters)
             return fail(RISE_HIP_ERR_NOT_SUPPORTED, "counted queries on a scene with transformed objects");
-        const hipError_t e = launch_any_objects(scene->objects_view(), scene->arith.load(), rays, tmax, n, 3u, occluded,
+        const hipError_t e = launch_any_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays, tmax, n, 3u, occluded,
                                                 stream);
         return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "any-hit launch");
     }
@@ -391,7 +393,7 @@ rise_hip_status rise_hip_intersect_closest_ex(rise_hip_scene* scene, const float
     if (n == 0) return RISE_HIP_OK;
     if (!rays_soa || !hits) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
     const hipError_t e = scene->n_objects > 0
-                             ? launch_closest_objects(scene->objects_view(), scene->arith.load(), rays_soa, n,
+                             ? launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays_soa, n,
                                                       (unsigned)faces, nullptr, nullptr, hits, stream)
                              : launch_closest_ex(scene->view(), rays_soa, n, (unsigned)faces, hits, stream);
     if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
@@ -408,7 +410,7 @@ rise_hip_status rise_hip_intersect_any_ex(rise_hip_scene* scene, const float* ra
     if (n == 0) return RISE_HIP_OK;
     if (!rays_soa || !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
     const hipError_t e = scene->n_objects > 0
-                             ? launch_any_objects(scene->objects_view(), scene->arith.load(), rays_soa, tmax, n,
+                             ? launch_any_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays_soa, tmax, n,
                                                   (unsigned)faces, occluded, stream)
                              : launch_any_ex(scene->view(), rays_soa, tmax, n, (unsigned)faces, occluded, stream);
     if (e != hipSuccess) return fail_hip(e, "any-hit launch");
@@ -425,7 +427,7 @@ rise_hip_status rise_hip_intersect_closest_noalloc(rise_hip_scene* scene, const 
     if (!rays_soa || !prim_id || !t) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
     if (scene->n_objects > 0) {
         // The object kernels keep no scratch: nothing to redirect.
-        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), rays_soa, n, 3u,
+        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays_soa, n, 3u,
                                                     prim_id, t, nullptr, stream);
         return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "closest-hit launch");
     }
