@@ -116,45 +116,57 @@ struct ObjLdsStack {
     RISE_HD inline bool empty() const { return sp == base; }
 };
 
-// Walks the top-level tree near to far with the world ray; visit(k) is called
-// for each object record k reached and returns the new box limit, or a
-// negative value to end the walk.
-template <int A, class Stack, class Visit>
-RISE_HD inline void top_walk(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
-                             const Tri48* __restrict__ leaves, Stack& st, float tlim, Visit&& visit)
+// The top-level walk's state: the node group being tested and the leaves of
+// the last node tested.
+struct TopWalk {
+    unsigned gbase, ghits, gimask;
+    unsigned tbase, tmask;
+    float tlim;  // box limit along the scaled world direction
+};
+
+template <int A>
+RISE_HD inline void top_begin(const RayStateT<A>& r, float tlim, TopWalk& w)
 {
-    unsigned gbase = 0, ghits = 1u << (0u ^ r.oinv), gimask = 1u;
-    const int floor = st.sp;
-    for (;;) {
-        unsigned tmask = 0, tbase = 0;
-        if (ghits) {
-            const unsigned k = __builtin_ctz(ghits);
-            ghits &= ghits - 1u;
-            const unsigned slot = k ^ r.oinv;
-            const unsigned ni = gbase + __builtin_popcount(gimask & ((1u << slot) - 1u));
-            if (ghits) st.push(make_uint2(gbase, ghits | (gimask << 8)));
-            unsigned cb, im;
-            ghits = test_node(r, nodes, ni, tlim, cb, im, tbase, tmask);
-            gbase = cb;
-            gimask = im;
-        }
-        while (tmask) {
-            const unsigned j = __builtin_ctz(tmask);
-            tmask &= tmask - 1u;
-            tlim = visit(float_to_bits(leaves[tbase + j].v0[3]));
-            if (tlim < 0.0f) {
-                st.sp = floor;
-                return;
-            }
-        }
-        if (!ghits) {
-            if (st.sp == floor) break;
-            const uint2 g = st.pop();
-            gbase = g.x;
-            ghits = g.y & 0xffu;
-            gimask = (g.y >> 8) & 0xffu;
-        }
+    w.gbase = 0;
+    w.ghits = 1u << (0u ^ r.oinv);  // the root, in slot 0 of a virtual group
+    w.gimask = 1u;
+    w.tbase = 0;
+    w.tmask = 0;
+    w.tlim = tlim;
+}
+
+// One step of the top-level walk, near to far with the world ray: returns the
+// object record index of the next leaf reached, -1 after a node test or a
+// pop, -2 once the walk has ended (the stack down to its floor).  A node's
+// leaves are handed out before the next node is tested, so a limit lowered
+// by one object applies to the nodes after it.
+template <int A, class Stack>
+RISE_HD inline int top_step(const RayStateT<A>& r, const NodeSlot* __restrict__ nodes,
+                            const Tri48* __restrict__ leaves, Stack& st, TopWalk& w)
+{
+    if (w.tmask) {
+        const unsigned j = __builtin_ctz(w.tmask);
+        w.tmask &= w.tmask - 1u;
+        return (int)float_to_bits(leaves[w.tbase + j].v0[3]);
     }
+    if (w.ghits) {
+        const unsigned k = __builtin_ctz(w.ghits);
+        w.ghits &= w.ghits - 1u;
+        const unsigned slot = k ^ r.oinv;
+        const unsigned ni = w.gbase + __builtin_popcount(w.gimask & ((1u << slot) - 1u));
+        if (w.ghits) st.push(make_uint2(w.gbase, w.ghits | (w.gimask << 8)));
+        unsigned cb, im;
+        w.ghits = test_node(r, nodes, ni, w.tlim, cb, im, w.tbase, w.tmask);
+        w.gbase = cb;
+        w.gimask = im;
+        return -1;
+    }
+    if (st.empty()) return -2;
+    const uint2 g = st.pop();
+    w.gbase = g.x;
+    w.ghits = g.y & 0xffu;
+    w.gimask = (g.y >> 8) & 0xffu;
+    return -1;
 }
 
 // Largest absolute coordinate.
@@ -171,83 +183,156 @@ struct ObjectHit {
     double p_world[3];
 };
 
-// ObjectManager::IntersectRay over the scene: each object's own closest hit
-// (Object::IntersectRay), the strictly closer world range winning, ties to the
-// lower object index (the caller's order).
+// Phases of an object query.
+enum : int { kObjPhaseWorld = 0, kObjPhaseTop = 1, kObjPhaseObject = 2, kObjPhaseDone = 3 };
+
+// ObjectManager::IntersectRay over the scene (each object's own closest hit,
+// Object::IntersectRay; the strictly closer world range wins, ties to the
+// lower object index, the caller's order) as a state machine advanced one
+// traversal step at a time, so that a persistent kernel can give a lane
+// whose query has ended a new ray while the other lanes of its wave go on.
+// Phases: the world tree, then the top-level walk, which enters the tree of
+// each object it reaches and resumes when that tree is done.
+template <int A>
+struct ObjClosestQuery {
+    RayStateT<A> r;  // the world ray (world tree and top-level tree)
+    ObjRayT<A> q;    // the ray of the object being traversed
+    ClosestState s;  // the walk of the tree being traversed
+    TopWalk w;
+    ObjectHit h;     // the best so far
+    const NodeSlot* nodes;  // the tree of the object being traversed
+    const Tri48* tris;
+    const double* tri64;
+    unsigned faces;
+    int k;  // its object record
+    int phase;
+};
+
+// The top-level box limit: a hit closer than the best lies in its object's
+// padded box within the best (plus rounding) along the world ray.
+template <int A>
+RISE_HD inline float obj_closest_limit(const ObjClosestQuery<A>& Q)
+{
+    return box_limit(Q.r, Q.h.range < kRiseInfinity ? Q.h.range * (1.0 + 1e-7) : kRiseInfinity);
+}
+
+template <int A>
+RISE_HD inline void obj_closest_to_top(const ObjectSceneView& sc, ObjClosestQuery<A>& Q)
+{
+    if (sc.n_objects == 0) {
+        Q.phase = kObjPhaseDone;
+        return;
+    }
+    top_begin(Q.r, obj_closest_limit(Q), Q.w);
+    Q.phase = kObjPhaseTop;
+}
+
+template <int A, class Stack>
+RISE_HD inline void objects_closest_begin(const ObjectSceneView& sc, const float ray[6], unsigned faces, Stack& st,
+                                          ObjClosestQuery<A>& Q)
+{
+    Q.h.prim_id = -1;
+    Q.h.object = -1;
+    Q.h.src = -1;
+    Q.h.leaf = 0;
+    Q.h.t = kRiseInfinity;
+    Q.h.range = kRiseInfinity;
+    st.sp = 0;
+    st.base = 0;
+    st.overflow = false;
+    init_ray(Q.r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
+    Q.faces = faces;
+    if (sc.world_nodes) {
+        closest_begin(Q.r, Q.s);
+        Q.s.faces = faces;
+        Q.phase = kObjPhaseWorld;
+    } else {
+        obj_closest_to_top(sc, Q);
+    }
+}
+
+// One step; false once the query has ended (the hit is in Q.h).
+template <int A, class Stack>
+RISE_HD inline bool objects_closest_step(const ObjectSceneView& sc, Stack& st, ObjClosestQuery<A>& Q)
+{
+    unsigned nn = 0, nt = 0;
+    const double ow[3] = {Q.r.ox, Q.r.oy, Q.r.oz};
+    if (Q.phase == kObjPhaseWorld) {
+        if (closest_step<false, true, true>(Q.r, sc.world_nodes, sc.world_tris, st, Q.s, nn, nt, sc.world_tri64))
+            return true;
+        if (Q.s.best_id >= 0) {
+            const float4* tp = reinterpret_cast<const float4*>(sc.world_tris + Q.s.best_leaf);
+            Q.h.prim_id = Q.s.best_id;
+            Q.h.object = float_to_bits(tp[1].w);
+            Q.h.leaf = Q.s.best_leaf;
+            Q.h.t = Q.s.best_t;
+            Q.h.range = world_range(Q.r, Q.s.best_t);
+        }
+        obj_closest_to_top(sc, Q);
+        return Q.phase != kObjPhaseDone;
+    }
+    if (Q.phase == kObjPhaseObject) {
+        if (closest_step<false, true, true>(Q.q, Q.nodes, Q.tris, st, Q.s, nn, nt, Q.tri64)) return true;
+        st.base = 0;  // back to the top-level walk's entries
+        Q.phase = kObjPhaseTop;
+        if (Q.s.best_id >= 0) {
+            const DevObject& ob = sc.objects[Q.k];
+            double pw[3];
+            const double w = obj_world_range(Q.q, ob.to_world, ow, Q.s.best_t, pw);
+            if (w < Q.h.range || (w == Q.h.range && ob.object < Q.h.object)) {
+                Q.h.prim_id = Q.s.best_id;
+                Q.h.object = ob.object;
+                Q.h.src = Q.k;
+                Q.h.leaf = Q.s.best_leaf;
+                Q.h.t = Q.s.best_t;
+                Q.h.range = w;
+                for (int c = 0; c < 3; ++c) Q.h.p_world[c] = pw[c];
+                Q.w.tlim = obj_closest_limit(Q);
+            }
+        }
+        return true;
+    }
+    if (Q.phase != kObjPhaseTop) return false;
+    const int k = top_step(Q.r, sc.top_nodes, sc.top_leaves, st, Q.w);
+    if (k == -2) {
+        Q.phase = kObjPhaseDone;
+        return false;
+    }
+    if (k < 0) return true;
+    const DevObject& ob = sc.objects[k];
+    const double dw[3] = {Q.r.dx, Q.r.dy, Q.r.dz};
+    if (!world_box_reach(ob.lo, ob.hi, ow, dw, Q.r.len_d, Q.h.range)) return true;
+    init_obj_ray(Q.q, ow, dw, ob.to_object);
+    const BlasView bl = sc.blas[ob.blas];
+    Q.nodes = bl.nodes;
+    Q.tris = bl.tris;
+    Q.tri64 = bl.tri64;
+    closest_begin(Q.q, Q.s);
+    Q.s.faces = Q.faces;
+    if (Q.h.range < kRiseInfinity && Q.r.len_d > 0.0) {
+        // A world range w is an object-space distance of about w |dl| / |d|;
+        // the rounding of o' and of the point taken back to world space is
+        // below slack (|o| + |translation| + w).
+        const double tw[3] = {ob.to_world[12], ob.to_world[13], ob.to_world[14]};
+        const double lim_w = Q.h.range * (1.0 + ob.slack + 1e-9) + ob.slack * (max_abs3(ow) + max_abs3(tw));
+        Q.s.tlim = box_limit(Q.q, lim_w * (Q.q.mag / Q.r.len_d) * (1.0 + 1e-9));
+    }
+    st.base = st.sp;  // the object's entries go above the top-level walk's
+    Q.k = k;
+    Q.phase = kObjPhaseObject;
+    return true;
+}
+
+// The whole closest-hit query for one ray.
 template <int A, class Stack>
 RISE_HD inline void objects_closest(const ObjectSceneView& sc, const float ray[6], unsigned faces, Stack& st,
                                     ObjectHit& h)
 {
-    h.prim_id = -1;
-    h.object = -1;
-    h.src = -1;
-    h.leaf = 0;
-    h.t = kRiseInfinity;
-    h.range = kRiseInfinity;
-    st.sp = 0;
-    st.base = 0;
-    st.overflow = false;
-    RayStateT<A> r;
-    init_ray(r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
-    unsigned nn = 0, nt = 0;
-    if (sc.world_nodes) {
-        ClosestState s;
-        closest_begin(r, s);
-        s.faces = faces;
-        while (closest_step<false, true, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt, sc.world_tri64)) {
-        }
-        if (s.best_id >= 0) {
-            const float4* tp = reinterpret_cast<const float4*>(sc.world_tris + s.best_leaf);
-            h.prim_id = s.best_id;
-            h.object = float_to_bits(tp[1].w);
-            h.leaf = s.best_leaf;
-            h.t = s.best_t;
-            h.range = world_range(r, s.best_t);
-        }
+    ObjClosestQuery<A> Q;
+    objects_closest_begin(sc, ray, faces, st, Q);
+    while (objects_closest_step(sc, st, Q)) {
     }
-    if (sc.n_objects == 0) return;
-    const double ow[3] = {ray[0], ray[1], ray[2]};
-    const double dw[3] = {ray[3], ray[4], ray[5]};
-    const double omax = max_abs3(ow);
-    // A hit closer than the best lies in its object's padded box within the
-    // best (plus rounding) along the world ray.
-    auto limit = [&]() { return box_limit(r, h.range < kRiseInfinity ? h.range * (1.0 + 1e-7) : kRiseInfinity); };
-    top_walk(r, sc.top_nodes, sc.top_leaves, st, limit(), [&](int k) -> float {
-        const DevObject& ob = sc.objects[k];
-        if (!world_box_reach(ob.lo, ob.hi, ow, dw, r.len_d, h.range)) return limit();
-        ObjRayT<A> q;
-        init_obj_ray(q, ow, dw, ob.to_object);
-        const BlasView bl = sc.blas[ob.blas];
-        ClosestState s;
-        closest_begin(q, s);
-        s.faces = faces;
-        if (h.range < kRiseInfinity && r.len_d > 0.0) {
-            // A world range w is an object-space distance of about w |dl| / |d|;
-            // the rounding of o' and of the point taken back to world space is
-            // below slack (|o| + |translation| + w).
-            const double tw[3] = {ob.to_world[12], ob.to_world[13], ob.to_world[14]};
-            const double lim_w = h.range * (1.0 + ob.slack + 1e-9) + ob.slack * (omax + max_abs3(tw));
-            s.tlim = box_limit(q, lim_w * (q.mag / r.len_d) * (1.0 + 1e-9));
-        }
-        const int floor = st.base;
-        st.base = st.sp;
-        while (closest_step<false, true, true>(q, bl.nodes, bl.tris, st, s, nn, nt, bl.tri64)) {
-        }
-        st.base = floor;
-        if (s.best_id < 0) return limit();
-        double pw[3];
-        const double w = obj_world_range(q, ob.to_world, ow, s.best_t, pw);
-        if (w < h.range || (w == h.range && ob.object < h.object)) {
-            h.prim_id = s.best_id;
-            h.object = ob.object;
-            h.src = k;
-            h.leaf = s.best_leaf;
-            h.t = s.best_t;
-            h.range = w;
-            for (int c = 0; c < 3; ++c) h.p_world[c] = pw[c];
-        }
-        return limit();
-    });
+    h = Q.h;
 }
 
 // Fills the record's attributes from a hit in the mesh's frame: normals taken
@@ -333,58 +418,124 @@ RISE_HD inline void objects_record(const ObjectSceneView& sc, const float ray[6]
 }
 
 // ObjectManager::IntersectShadowRay over the scene (ObjectManager.cpp:320-373
-// with Object::IntersectRay_IntersectionOnly per object).  how_far: the
-// world limit dHowFar.  The answer does not depend on the order objects are
-// tried in.
+// with Object::IntersectRay_IntersectionOnly per object) as a state machine
+// like ObjClosestQuery.  how_far: the world limit dHowFar.  The answer does
+// not depend on the order objects are tried in.
+template <int A>
+struct ObjAnyQuery {
+    RayStateT<A> r;
+    ObjRayT<A> q;
+    AnyState s;
+    TopWalk w;
+    const NodeSlot* nodes;
+    const Tri48* tris;
+    const double* tri64;
+    double how_far;
+    unsigned faces;
+    int phase;
+    bool hit;
+};
+
+template <int A>
+RISE_HD inline void obj_any_to_top(const ObjectSceneView& sc, ObjAnyQuery<A>& Q)
+{
+    if (sc.n_objects == 0) {
+        Q.phase = kObjPhaseDone;
+        return;
+    }
+    // No distance limit on the top-level walk: an object whose transform
+    // nearly collapses the direction has no bounded world reach.
+    top_begin(Q.r, INFINITY, Q.w);
+    Q.phase = kObjPhaseTop;
+}
+
 template <int A, class Stack>
-RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], double how_far, unsigned faces,
-                                Stack& st)
+RISE_HD inline void objects_any_begin(const ObjectSceneView& sc, const float ray[6], double how_far, unsigned faces,
+                                      Stack& st, ObjAnyQuery<A>& Q)
 {
     st.sp = 0;
     st.base = 0;
     st.overflow = false;
-    RayStateT<A> r;
-    init_ray(r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
-    unsigned nn = 0, nt = 0;
+    init_ray(Q.r, ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
+    Q.how_far = how_far;
+    Q.faces = faces;
+    Q.hit = false;
     if (sc.world_nodes) {
-        AnyState s;
-        any_begin(r, scale_how_far(r, how_far), s);
-        s.faces = faces;
-        while (any_step<false, true, true>(r, sc.world_nodes, sc.world_tris, st, s, nn, nt, sc.world_tri64)) {
-        }
-        if (s.hit) return true;
+        any_begin(Q.r, scale_how_far(Q.r, how_far), Q.s);
+        Q.s.faces = faces;
+        Q.phase = kObjPhaseWorld;
+    } else {
+        obj_any_to_top(sc, Q);
     }
-    if (sc.n_objects == 0) return false;
-    const double ow[3] = {ray[0], ray[1], ray[2]};
-    const double dw[3] = {ray[3], ray[4], ray[5]};
-    bool hit = false;
-    // The top-level walk takes no distance limit: an object whose transform
-    // nearly collapses the direction has no bounded world reach.
-    top_walk(r, sc.top_nodes, sc.top_leaves, st, INFINITY, [&](int k) -> float {
-        const DevObject& ob = sc.objects[k];
-        ObjRayT<A> q;
-        init_obj_ray(q, ow, dw, ob.to_object);
-        const double how_far2 = obj_how_far(q.factor, how_far);
-        // An object-space distance t along dn' is t |d| / |dl| along the
-        // world ray, so the limit reaches |d| how_far2 / |dl| in world
-        // space (widened by 1e-6 for the rounding of the two transforms).
-        const double lim = (how_far2 < kRiseInfinity && q.mag > 0.0) ? how_far2 * r.len_d / q.mag * (1.0 + 1e-6)
-                                                                     : kRiseInfinity;
-        if (!world_box_reach(ob.lo, ob.hi, ow, dw, r.len_d, lim)) return INFINITY;
-        const BlasView bl = sc.blas[ob.blas];
-        AnyState s;
-        any_begin(q, how_far2, s);
-        s.faces = faces;
-        const int floor = st.base;
-        st.base = st.sp;
-        while (any_step<false, true, true>(q, bl.nodes, bl.tris, st, s, nn, nt, bl.tri64)) {
+}
+
+// One step; false once the query has ended (the answer is in Q.hit).
+template <int A, class Stack>
+RISE_HD inline bool objects_any_step(const ObjectSceneView& sc, Stack& st, ObjAnyQuery<A>& Q)
+{
+    unsigned nn = 0, nt = 0;
+    if (Q.phase == kObjPhaseWorld) {
+        if (any_step<false, true, true>(Q.r, sc.world_nodes, sc.world_tris, st, Q.s, nn, nt, sc.world_tri64))
+            return true;
+        if (Q.s.hit) {
+            Q.hit = true;
+            Q.phase = kObjPhaseDone;
+            return false;
         }
-        st.sp = st.base;  // an early exit leaves the object's entries
-        st.base = floor;
-        hit = s.hit;
-        return hit ? -1.0f : INFINITY;
-    });
-    return hit;
+        obj_any_to_top(sc, Q);
+        return Q.phase != kObjPhaseDone;
+    }
+    if (Q.phase == kObjPhaseObject) {
+        if (any_step<false, true, true>(Q.q, Q.nodes, Q.tris, st, Q.s, nn, nt, Q.tri64)) return true;
+        if (Q.s.hit) {
+            Q.hit = true;
+            Q.phase = kObjPhaseDone;
+            return false;
+        }
+        st.sp = st.base;  // back to the top-level walk's entries
+        st.base = 0;
+        Q.phase = kObjPhaseTop;
+        return true;
+    }
+    if (Q.phase != kObjPhaseTop) return false;
+    const int k = top_step(Q.r, sc.top_nodes, sc.top_leaves, st, Q.w);
+    if (k == -2) {
+        Q.phase = kObjPhaseDone;
+        return false;
+    }
+    if (k < 0) return true;
+    const DevObject& ob = sc.objects[k];
+    const double ow[3] = {Q.r.ox, Q.r.oy, Q.r.oz};
+    const double dw[3] = {Q.r.dx, Q.r.dy, Q.r.dz};
+    init_obj_ray(Q.q, ow, dw, ob.to_object);
+    const double how_far2 = obj_how_far(Q.q.factor, Q.how_far);
+    // An object-space distance t along dn' is t |d| / |dl| along the world
+    // ray, so the limit reaches |d| how_far2 / |dl| in world space (widened
+    // by 1e-6 for the rounding of the two transforms).
+    const double lim = (how_far2 < kRiseInfinity && Q.q.mag > 0.0) ? how_far2 * Q.r.len_d / Q.q.mag * (1.0 + 1e-6)
+                                                                   : kRiseInfinity;
+    if (!world_box_reach(ob.lo, ob.hi, ow, dw, Q.r.len_d, lim)) return true;
+    const BlasView bl = sc.blas[ob.blas];
+    Q.nodes = bl.nodes;
+    Q.tris = bl.tris;
+    Q.tri64 = bl.tri64;
+    any_begin(Q.q, how_far2, Q.s);
+    Q.s.faces = Q.faces;
+    st.base = st.sp;
+    Q.phase = kObjPhaseObject;
+    return true;
+}
+
+// The whole any-hit query for one ray.
+template <int A, class Stack>
+RISE_HD inline bool objects_any(const ObjectSceneView& sc, const float ray[6], double how_far, unsigned faces,
+                                Stack& st)
+{
+    ObjAnyQuery<A> Q;
+    objects_any_begin(sc, ray, how_far, faces, st, Q);
+    while (objects_any_step(sc, st, Q)) {
+    }
+    return Q.hit;
 }
 
 }  // namespace rise_hip

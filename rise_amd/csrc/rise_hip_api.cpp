@@ -51,11 +51,11 @@ struct rise_hip_scene {
         return ObjectSceneView{world_empty ? nullptr : d_nodes, d_tris, d_tri64, d_attr, d_objects, n_objects,
                                d_blas, d_top_nodes, d_top_leaves};
     }
-    TraceScene view()
+    unsigned long long* fetch_slot()
     {
-        const unsigned k = next_slot.fetch_add(1u) % kFetchSlots;
-        return view_on(d_fetch + k);
+        return d_fetch + next_slot.fetch_add(1u) % kFetchSlots;
     }
+    TraceScene view() { return view_on(fetch_slot()); }
     TraceScene view_on(unsigned long long* fetch_counter) const
     {
         return TraceScene{d_nodes, d_tris, stats.depth, fetch_counter, arith.load(), d_tri64, d_attr};
@@ -340,8 +340,8 @@ static rise_hip_status closest_impl(rise_hip_scene* scene, const float* rays, in
     if (scene->n_objects > 0) {
         if (counters)
             return fail(RISE_HIP_ERR_NOT_SUPPORTED, "counted queries on a scene with transformed objects");
-        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays, n, 3u, prim_id, t,
-                                                    nullptr, stream);
+        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays, n,
+                                                    3u, prim_id, t, nullptr, scene->fetch_slot(), stream);
         return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "closest-hit launch");
     }
     const hipError_t e = launch_closest(scene->view(), rays, n, prim_id, t,
@@ -361,8 +361,8 @@ static rise_hip_status any_impl(rise_hip_scene* scene, const float* rays, const 
     if (scene->n_objects > 0) {
         if (counters)
             return fail(RISE_HIP_ERR_NOT_SUPPORTED, "counted queries on a scene with transformed objects");
-        const hipError_t e = launch_any_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays, tmax, n, 3u, occluded,
-                                                stream);
+        const hipError_t e = launch_any_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays, tmax,
+                                                n, 3u, occluded, scene->fetch_slot(), stream);
         return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "any-hit launch");
     }
     const hipError_t e = launch_any(scene->view(), rays, tmax, n, occluded,
@@ -394,7 +394,7 @@ rise_hip_status rise_hip_intersect_closest_ex(rise_hip_scene* scene, const float
     if (!rays_soa || !hits) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
     const hipError_t e = scene->n_objects > 0
                              ? launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays_soa, n,
-                                                      (unsigned)faces, nullptr, nullptr, hits, stream)
+                                                      (unsigned)faces, nullptr, nullptr, hits, scene->fetch_slot(), stream)
                              : launch_closest_ex(scene->view(), rays_soa, n, (unsigned)faces, hits, stream);
     if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
     return RISE_HIP_OK;
@@ -411,7 +411,7 @@ rise_hip_status rise_hip_intersect_any_ex(rise_hip_scene* scene, const float* ra
     if (!rays_soa || !occluded) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
     const hipError_t e = scene->n_objects > 0
                              ? launch_any_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays_soa, tmax, n,
-                                                  (unsigned)faces, occluded, stream)
+                                                  (unsigned)faces, occluded, scene->fetch_slot(), stream)
                              : launch_any_ex(scene->view(), rays_soa, tmax, n, (unsigned)faces, occluded, stream);
     if (e != hipSuccess) return fail_hip(e, "any-hit launch");
     return RISE_HIP_OK;
@@ -425,16 +425,14 @@ rise_hip_status rise_hip_intersect_closest_noalloc(rise_hip_scene* scene, const 
     if (n < 0) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "negative ray count");
     if (n == 0) return RISE_HIP_OK;
     if (!rays_soa || !prim_id || !t) return fail(RISE_HIP_ERR_INVALID_ARGUMENT, "NULL buffer");
-    if (scene->n_objects > 0) {
-        // The object kernels keep no scratch: nothing to redirect.
-        const hipError_t e = launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays_soa, n, 3u,
-                                                    prim_id, t, nullptr, stream);
-        return e == hipSuccess ? RISE_HIP_OK : fail_hip(e, "closest-hit launch");
-    }
     unsigned long long* counter = nullptr;
     hipError_t e = noalloc_scratch(&counter);
     if (e != hipSuccess) return fail_hip(e, "scratch address (hipGetSymbolAddress)");
-    e = launch_closest(scene->view_on(counter), rays_soa, n, prim_id, t, nullptr, stream);
+    if (scene->n_objects > 0)
+        e = launch_closest_objects(scene->objects_view(), scene->arith.load(), scene->stack_need, rays_soa, n, 3u,
+                                   prim_id, t, nullptr, counter, stream);
+    else
+        e = launch_closest(scene->view_on(counter), rays_soa, n, prim_id, t, nullptr, stream);
     if (e != hipSuccess) return fail_hip(e, "closest-hit launch");
     return RISE_HIP_OK;
 }

@@ -2,7 +2,8 @@
 through the C ABI: its ids and distances equal rise_hip_intersect_closest's,
 bit for bit, on two calls in a row (the second with fewer rays, so a stale
 fetch counter or leftover results from the first call would show), on a
-60,000-triangle and a 500-triangle soup, in both arithmetic models.  Bar: ids and distances bit-exact (integer and index work).
+60,000-triangle and a 500-triangle soup and on a scene of transformed objects,
+in both arithmetic models.  Bar: ids and distances bit-exact (integer and index work).
 """
 import numpy as np
 import pytest
@@ -38,6 +39,32 @@ def test_noalloc_equals_closest_on_two_calls(cuda, n_tris, sigma, arith):
     small = np.ascontiguousarray(scenes.incoherent_rays(7001, 23))
     for k, rays in enumerate((big, small)):
         log_case(f"no-alloc closest hit: {n_tris} tris, arith {arith}, call {k + 1}, {rays.shape[1]} rays")
+        r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).to(cuda)
+        want_p, want_t = _closest(L.rise_hip_intersect_closest, om, r, cuda)
+        got_p, got_t = _closest(L.rise_hip_intersect_closest_noalloc, om, r, cuda)
+        assert np.array_equal(got_p, want_p)
+        assert np.array_equal(got_t.view(np.uint32), want_t.view(np.uint32))
+        assert (want_p >= 0).any()
+    om.release()
+
+
+@pytest.mark.parametrize("arith", [raycast.ARITH_IEEE, raycast.ARITH_X86_FMA])
+def test_noalloc_equals_closest_on_objects(cuda, arith):
+    """A scene of transformed objects: the object kernels take their fetch
+    counter from the same scratch array."""
+    import torch
+
+    L = _lib.lib()
+    meshes, objs = scenes.instanced_scene(tris_per_mesh=3000, seed=25)
+    om = raycast.ObjectManager(arith)
+    for m in meshes:
+        om.add_mesh(m)
+    for ob in objs:
+        om.add_object(int(ob["mesh"]), ob["to_world"], ob["to_object"])
+    om.prepare_for_rendering()
+    for k, n in enumerate((30000, 4001)):
+        log_case(f"no-alloc closest hit on objects: arith {arith}, call {k + 1}, {n} rays")
+        rays = scenes.object_rays(meshes, objs, n, seed=26 + k)
         r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).to(cuda)
         want_p, want_t = _closest(L.rise_hip_intersect_closest, om, r, cuda)
         got_p, got_t = _closest(L.rise_hip_intersect_closest_noalloc, om, r, cuda)
