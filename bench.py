@@ -477,7 +477,8 @@ def objects_leg():
     """Objects with transforms: 16 objects on 4 meshes of 250,000 triangles
     (soups in [-1, 1]^3 with edges N(0, 0.005^2); 4 identity objects, 12
     rotated, scaled and translated ones: 4M world triangles), 2,000,000 rays aimed at them (three quarters) or random,
-    closest hit with records off.  Timed with HIP events on the launch
+    closest hit with records off, then shadow queries on the same rays within
+    half of each closest hit's range.  Timed with HIP events on the launch
     stream, against the same world triangles baked into one world-space
     scene (float vertices, so the hits agree only to rounding)."""
     import numpy as np
@@ -522,10 +523,24 @@ def objects_leg():
         ms = a.elapsed_time(b) / reps
         res[name] = {"ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1),
                      "hit_fraction": round(float((pid >= 0).float().mean().item()), 4)}
+        # Shadow queries: any hit within half the closest hit's world range
+        # (misses: within 3e38).
+        how_far = torch.where(pid >= 0, t * 0.5, torch.full_like(t, 3e38))
+        occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+        mgr.intersect_shadow_ray(r, how_far, occ, stream=s)
+        a.record(s)
+        for _ in range(reps):
+            mgr.intersect_shadow_ray(r, how_far, occ, stream=s)
+        b.record(s)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        res[name]["shadow"] = {"ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1),
+                               "occluded_fraction": round(float(occ.float().mean().item()), 4)}
     om.release()
     flat.release()
     log(f"objects: 16 objects (4M world tris) {res['objects']['mrays_s']} Mrays/s vs the same triangles baked into "
-        f"one world tree {res['baked_world']['mrays_s']} Mrays/s")
+        f"one world tree {res['baked_world']['mrays_s']} Mrays/s; shadow queries {res['objects']['shadow']['mrays_s']} "
+        f"vs {res['baked_world']['shadow']['mrays_s']} Mrays/s")
     return {"workload": "16 objects on 4 meshes x 250,000 tris (4 identity, 12 transformed), 2,000,000 rays",
             **res}
 
