@@ -477,8 +477,8 @@ def objects_leg():
     """Objects with transforms: 16 objects on 4 meshes of 250,000 triangles
     (soups in [-1, 1]^3 with edges N(0, 0.005^2); 4 identity objects, 12
     rotated, scaled and translated ones: 4M world triangles), 2,000,000 rays aimed at them (three quarters) or random,
-    closest hit with records off, then shadow queries on the same rays within
-    half of each closest hit's range.  Timed with HIP events on the launch
+    closest hit with records off, then with full intersection records, then
+    shadow queries on the same rays within half of each closest hit's range.  Timed with HIP events on the launch
     stream, against the same world triangles baked into one world-space
     scene (float vertices, so the hits agree only to rounding)."""
     import numpy as np
@@ -523,6 +523,15 @@ def objects_leg():
         ms = a.elapsed_time(b) / reps
         res[name] = {"ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1),
                      "hit_fraction": round(float((pid >= 0).float().mean().item()), 4)}
+        # Closest hit with full intersection records (rise_hip_intersect_closest_ex).
+        hits = mgr.intersect_ray_ex(r, stream=s)
+        a.record(s)
+        for _ in range(reps):
+            mgr.intersect_ray_ex(r, hits=hits, stream=s)
+        b.record(s)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        res[name]["records"] = {"ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1)}
         # Shadow queries: any hit within half the closest hit's world range
         # (misses: within 3e38).
         how_far = torch.where(pid >= 0, t * 0.5, torch.full_like(t, 3e38))
@@ -539,7 +548,8 @@ def objects_leg():
     om.release()
     flat.release()
     log(f"objects: 16 objects (4M world tris) {res['objects']['mrays_s']} Mrays/s vs the same triangles baked into "
-        f"one world tree {res['baked_world']['mrays_s']} Mrays/s; shadow queries {res['objects']['shadow']['mrays_s']} "
+        f"one world tree {res['baked_world']['mrays_s']} Mrays/s; with records {res['objects']['records']['mrays_s']} vs "
+        f"{res['baked_world']['records']['mrays_s']}; shadow queries {res['objects']['shadow']['mrays_s']} "
         f"vs {res['baked_world']['shadow']['mrays_s']} Mrays/s")
     return {"workload": "16 objects on 4 meshes x 250,000 tris (4 identity, 12 transformed), 2,000,000 rays",
             **res}

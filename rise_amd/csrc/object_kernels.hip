@@ -51,10 +51,16 @@ __device__ inline typename StackOf<STACK>::type make_stack(uint2* lds)
 
 __global__ void reset_fetch(unsigned long long* c) { *c = 0ull; }
 
+// A query's result staged in its rise_hip_hit slot until object_records
+// turns it into the record (keeps the record's double arithmetic out of the
+// traversal loop's registers).
+static_assert(sizeof(ObjectHit) <= sizeof(rise_hip_hit) && alignof(ObjectHit) <= alignof(rise_hip_hit),
+              "an ObjectHit must fit in a hit record's slot");
+
 // Persistent: the grid is sized to the resident waves, each lane's first ray
 // is assigned statically and later ones are fetched wave-wide (refill) as
-// lanes finish.  REC: write rise_hip_hit records; else prim_id and t (float
-// world range).
+// lanes finish.  REC: stage the ObjectHit in hits[ray] for object_records;
+// else write prim_id and t (float world range).
 template <int A, bool REC, int STACK>
 __global__ __launch_bounds__(kObjBlock) void closest_objects(ObjectSceneView sc, const float* __restrict__ rays,
                                                              int64_t n, unsigned faces, int32_t* __restrict__ prim_id,
@@ -80,10 +86,7 @@ __global__ __launch_bounds__(kObjBlock) void closest_objects(ObjectSceneView sc,
         if (__ballot(ray >= 0) == 0ull && __ballot(ray == -1) == 0ull) break;
         if (ray >= 0 && !objects_closest_step(sc, st, Q)) {
             if (REC) {
-                const float r6[6] = {Q.r.ox, Q.r.oy, Q.r.oz, Q.r.dx, Q.r.dy, Q.r.dz};
-                rise_hip_hit rec;
-                objects_record<A>(sc, r6, Q.h, rec);
-                hits[ray] = rec;
+                *reinterpret_cast<ObjectHit*>(hits + ray) = Q.h;
             } else {
                 prim_id[ray] = Q.h.prim_id;
                 t_out[ray] = Q.h.prim_id >= 0 ? (float)Q.h.range : INFINITY;
@@ -91,6 +94,21 @@ __global__ __launch_bounds__(kObjBlock) void closest_objects(ObjectSceneView sc,
             ray = -1;
         }
     }
+}
+
+// The intersection record of each ray from its staged ObjectHit, in place.
+template <int A>
+__global__ __launch_bounds__(kObjBlock) void object_records(ObjectSceneView sc, const float* __restrict__ rays,
+                                                            int64_t n, rise_hip_hit* __restrict__ hits)
+{
+    const int64_t i = (int64_t)blockIdx.x * kObjBlock + threadIdx.x;
+    if (i >= n) return;
+    const ObjectHit h = *reinterpret_cast<const ObjectHit*>(hits + i);
+    float r6[6];
+    load_ray6(rays, n, i, r6);
+    rise_hip_hit rec;
+    objects_record<A>(sc, r6, h, rec);
+    hits[i] = rec;
 }
 
 template <int A, int STACK>
@@ -149,10 +167,14 @@ hipError_t launch_closest_objects(const ObjectSceneView& sc, int arith, int stac
     closest_objects<A, R, S><<<resident_grid(closest_objects<A, R, S>, n), kObjBlock, 0, stream>>>(sc, rays, n, \
                                                                                                   faces, prim_id, \
                                                                                                   t, hits, next)
-#define RISE_OBJ_CLOSEST(A, S)                      \
-    do {                                            \
-        if (hits) RISE_OBJ_CLOSEST_K(A, true, S);   \
-        else RISE_OBJ_CLOSEST_K(A, false, S);       \
+#define RISE_OBJ_CLOSEST(A, S)                                                                   \
+    do {                                                                                         \
+        if (hits) {                                                                              \
+            RISE_OBJ_CLOSEST_K(A, true, S);                                                      \
+            object_records<A><<<obj_grid(n), kObjBlock, 0, stream>>>(sc, rays, n, hits);          \
+        } else {                                                                                 \
+            RISE_OBJ_CLOSEST_K(A, false, S);                                                     \
+        }                                                                                        \
     } while (0)
     const bool lds = stack_need <= kObjLds;
     if (arith == kArithX86Fma) {
