@@ -204,8 +204,61 @@ __global__ __launch_bounds__(kBlock) void any_persistent(const NodeSlot* __restr
     flush_counts<COUNT>(counters, nn, nt);
 }
 
-// Closest hit with the face selection and the hit record of
-// rise_hip_intersect_closest_ex.
+// The closest_ex query's result staged in its rise_hip_hit slot until
+// ex_records turns it into the record (keeps the record's double arithmetic
+// out of the traversal loop's registers).
+struct StagedHit {
+    int32_t id;
+    unsigned leaf;
+    double t;
+};
+static_assert(sizeof(StagedHit) <= sizeof(rise_hip_hit), "a staged hit must fit in a hit record's slot");
+
+// The intersection record of each ray from its staged hit, in place: the
+// ray state is rebuilt from the ray as the traversal built it.
+template <int A, bool F64>
+__global__ __launch_bounds__(kBlock) void ex_records(const Tri48* __restrict__ tris, const float* __restrict__ rays,
+                                                     int64_t n, rise_hip_hit* __restrict__ hits,
+                                                     const double* __restrict__ tri64,
+                                                     const double* __restrict__ attr)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const StagedHit sh = *reinterpret_cast<const StagedHit*>(hits + i);
+    RayStateT<A> r;
+    float ox, oy, oz, dx, dy, dz;
+    load_ray(rays, n, i, ox, oy, oz, dx, dy, dz);
+    init_ray(r, ox, oy, oz, dx, dy, dz);
+    rise_hip_hit h;
+    h.prim_id = sh.id;
+    if (sh.id >= 0) {
+        const Tri48* tri = tris + sh.leaf;
+        double ab[2], nrm[3], pt[3], sn[3], uv[2];
+        hit_attributes(r, tri, sh.t, ab, nrm, pt, F64 ? tri64 + 9 * (size_t)sh.leaf : nullptr,
+                       attr ? attr + kAttrDoubles * (size_t)sh.leaf : nullptr, sn, uv);
+        h.mesh = float_to_bits(tri->v1[3]);
+        h.range = world_range(r, sh.t);
+        h.alpha = ab[0];
+        h.beta = ab[1];
+        for (int k = 0; k < 3; ++k) {
+            h.point[k] = (float)pt[k];
+            h.geom_normal[k] = (float)nrm[k];
+            h.shading_normal[k] = (float)sn[k];
+        }
+        h.uv[0] = (float)uv[0];
+        h.uv[1] = (float)uv[1];
+    } else {
+        h.mesh = -1;
+        h.range = INFINITY;
+        h.alpha = h.beta = 0.0;
+        for (int k = 0; k < 3; ++k) h.point[k] = h.geom_normal[k] = h.shading_normal[k] = 0.0f;
+        h.uv[0] = h.uv[1] = 0.0f;
+    }
+    hits[i] = h;
+}
+
+// Closest hit with the face selection of rise_hip_intersect_closest_ex; the
+// result is staged in hits for ex_records.
 template <int STACK, int A, bool F64 = false>
 __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* __restrict__ nodes,
                                                                 const Tri48* __restrict__ tris,
@@ -243,32 +296,7 @@ __global__ __launch_bounds__(kBlock) void closest_ex_persistent(const NodeSlot* 
             const bool more = STACK > 0 ? closest_step<false, true, F64>(r, nodes, tris, lst, s, nn, nt, tri64)
                                         : closest_step<false, true, F64>(r, nodes, tris, pst, s, nn, nt, tri64);
             if (!more) {
-                rise_hip_hit h;
-                h.prim_id = s.best_id;
-                if (s.best_id >= 0) {
-                    const Tri48* tri = tris + s.best_leaf;
-                    double ab[2], nrm[3], pt[3], sn[3], uv[2];
-                    hit_attributes(r, tri, s.best_t, ab, nrm, pt, F64 ? tri64 + 9 * (size_t)s.best_leaf : nullptr,
-                                   attr ? attr + kAttrDoubles * (size_t)s.best_leaf : nullptr, sn, uv);
-                    h.mesh = float_to_bits(tri->v1[3]);
-                    h.range = world_range(r, s.best_t);
-                    h.alpha = ab[0];
-                    h.beta = ab[1];
-                    for (int k = 0; k < 3; ++k) {
-                        h.point[k] = (float)pt[k];
-                        h.geom_normal[k] = (float)nrm[k];
-                        h.shading_normal[k] = (float)sn[k];
-                    }
-                    h.uv[0] = (float)uv[0];
-                    h.uv[1] = (float)uv[1];
-                } else {
-                    h.mesh = -1;
-                    h.range = INFINITY;
-                    h.alpha = h.beta = 0.0;
-                    for (int k = 0; k < 3; ++k) h.point[k] = h.geom_normal[k] = h.shading_normal[k] = 0.0f;
-                    h.uv[0] = h.uv[1] = 0.0f;
-                }
-                hits[ray] = h;
+                *reinterpret_cast<StagedHit*>(hits + ray) = StagedHit{s.best_id, s.best_leaf, s.best_t};
                 ray = -1;
             }
         }
@@ -438,8 +466,13 @@ hipError_t launch_closest_ex(const TraceScene& s, const float* rays, int64_t n, 
     const dim3 block(kBlock);
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, s.fetch_counter);
 #define RISE_LAUNCH_EX_F(S, A, F)                                                                           \
-    hipLaunchKernelGGL((closest_ex_persistent<S, A, F>), dim3(persistent_grid(closest_ex_persistent<S, A, F>, n)), \
-                       block, 0, stream, s.nodes, s.tris, rays, n, faces, hits, s.fetch_counter, s.tri64, s.attr)
+    do {                                                                                                    \
+        hipLaunchKernelGGL((closest_ex_persistent<S, A, F>),                                                \
+                           dim3(persistent_grid(closest_ex_persistent<S, A, F>, n)), block, 0, stream, s.nodes, \
+                           s.tris, rays, n, faces, hits, s.fetch_counter, s.tri64, s.attr);                 \
+        hipLaunchKernelGGL((ex_records<A, F>), dim3(grid_for(n)), block, 0, stream, s.tris, rays, n, hits,  \
+                           s.tri64, s.attr);                                                                \
+    } while (0)
 #define RISE_LAUNCH_EX(S, A)                   \
     do {                                       \
         if (s.tri64) RISE_LAUNCH_EX_F(S, A, true); \
