@@ -1497,6 +1497,10 @@ __global__ void k_walk(const ReqInfo* __restrict__ reqs, Paths S, int depth)
     }
 }
 
+// Waves per SIMD k_connect's registers are budgeted for: 5, i.e. at most 96
+// VGPRs (it needs 98 unbudgeted, which leaves 4).
+constexpr int kConnectWaves = 5;
+
 // Shadow rays (LightSampler::ShadowOccludedNM -> ObjectManager::IntersectShadowRay)
 // to 0.001 short of the light point (LightSampler.cpp:2206); adds each
 // unoccluded contribution to its path and wavelength, or, in a scene with
@@ -1504,7 +1508,7 @@ __global__ void k_walk(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 // occlusion test.  There is one record per (path, wavelength) and bounce, so
 // the additions never collide.  The same two traversal schedules as k_extend.
 template <int STACK, int BLOCK, bool PERSIST>
-__global__ __launch_bounds__(BLOCK) void k_connect(const ReqInfo* __restrict__ reqs, Paths S, int depth)
+__global__ __launch_bounds__(BLOCK, kConnectWaves) void k_connect(const ReqInfo* __restrict__ reqs, Paths S, int depth)
 {
     __shared__ uint2 lds[STACK > 0 ? STACK * BLOCK : 1];
     LdsStack<STACK> lst{lds + threadIdx.x, BLOCK, 0};
